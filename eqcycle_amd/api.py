@@ -1,0 +1,386 @@
+"""Host-side mirror of the reference routines on the hot path, backed by the HIP library.
+
+Each function keeps the reference's name (``!`` dropped), argument order and
+meaning, and its error behaviour: where the reference logs ``@error`` and
+returns ``nothing`` (src/NRsearch_serial.jl:19-32, src/PCG.jl:55-63) the
+mirror returns ``None``; anything else that fails raises :class:`EqError`.
+
+    ctx = EqContext(P)                       # P from eqcycle_amd.params.set_parameters
+    psi1, Vf1, tau1, psi2, Vf2, tau2 = FBC(ctx, IDstate, P, NFBC, ...)
+    dnew = cg(ctx, dnew, kni_rhs, tol=1e-6)  # cg!(dnew, kni, rhs; Pl=p, tol)
+"""
+from __future__ import annotations
+
+import ctypes as C
+import logging
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import lib as L
+from .lib import f64, i64, ptr
+
+log = logging.getLogger("eqcycle_amd")
+
+PREC = {"fp64": L.EQ_PREC_FP64, "fp32": L.EQ_PREC_FP32, "mixed": L.EQ_PREC_MIXED}
+PC = {"none": L.EQ_PC_NONE, "jacobi": L.EQ_PC_JACOBI, "amg": L.EQ_PC_AMG}
+MASK = {"all": L.EQ_MASK_ALL, "interior": L.EQ_MASK_INTERIOR, "neg": L.EQ_MASK_NEG}
+
+
+class EqError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{L.STATUS_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+class EqContext:
+    """One mesh on one GPU: owns the device buffers (eq_create / eq_destroy)."""
+
+    def __init__(self, P, amg: bool | dict = False):
+        self.lib = L.load()
+        self.P = P
+        self._keep = []
+        iglob = i64(P.iglob)
+        xg, wg = f64(P.xgll), f64(P.wgll)
+        Hc = f64(np.asarray(P.H).T)        # column-major H[i,j]
+        self._keep += [iglob, xg, wg, Hc]
+        desc = L.MeshDesc(5, P.NelX, P.NelY, P.dxe, P.dye, ptr(iglob, C.c_int64), ptr(xg), ptr(wg), ptr(Hc))
+        h = C.c_void_p()
+        rc = self.lib.eq_create(C.byref(desc), C.byref(h))
+        self.h = h
+        if rc != 0:
+            msg = self.lib.eq_last_error(h).decode() if h.value else "eq_create failed"
+            if h.value:
+                self.lib.eq_destroy(h)
+                self.h = C.c_void_p()
+            raise EqError(rc, msg)
+        self._check(self.lib.eq_set_material(self.h, ptr(f64(P.W))))
+        self._set_params(P)
+        n = [C.c_int64(), C.c_int64(), C.c_int64()]
+        self.lib.eq_sizes(self.h, *[C.byref(x) for x in n])
+        self.nglob, self.nI, self.nfault = (x.value for x in n)
+        if amg:
+            self.amg_setup(**(amg if isinstance(amg, dict) else {}))
+
+    def _set_params(self, P):
+        arrs = {k: f64(getattr(P, k)) for k in ["fo", "Vo", "xLf", "M", "BcLC", "BcTC", "FltB", "FltZ", "FltX",
+                                                "cca", "ccb", "Seff", "tauo", "XiLf"]}
+        iarrs = {k: i64(getattr(P, k)) for k in ["iFlt", "iBcL", "iBcT", "FltIglobBC", "out_seis"]}
+        self._keep += list(arrs.values()) + list(iarrs.values())
+        d = L.ParamsDesc()
+        d.FltNglob, d.nglob, d.IDstate = P.FltNglob, P.nglob, P.IDstate
+        d.Vpl, d.Vthres, d.Vevne, d.dt0 = P.Vpl, P.Vthres, P.Vevne, P.dt0
+        for k, v in arrs.items():
+            setattr(d, k, ptr(v))
+        d.iFlt = ptr(iarrs["iFlt"], C.c_int64)
+        d.iBcL, d.n_iBcL = ptr(iarrs["iBcL"], C.c_int64), len(iarrs["iBcL"])
+        d.iBcT, d.n_iBcT = ptr(iarrs["iBcT"], C.c_int64), len(iarrs["iBcT"])
+        d.FltIglobBC, d.n_FltIglobBC = ptr(iarrs["FltIglobBC"], C.c_int64), len(iarrs["FltIglobBC"])
+        d.out_seis, d.n_out_seis = ptr(iarrs["out_seis"], C.c_int64), len(iarrs["out_seis"])
+        self._check(self.lib.eq_set_params(self.h, C.byref(d)))
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.eq_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _err(self):
+        return self.lib.eq_last_error(self.h).decode()
+
+    def _check(self, rc, allow=()):
+        if rc != 0 and rc not in allow:
+            raise EqError(rc, self._err())
+        return rc
+
+    # ---- operators -------------------------------------------------------
+    def ku(self, u, mask="all", prec="fp64"):
+        """y = K u (mul!(y, Ksparse', u)), reference node numbering."""
+        u = f64(u)
+        y = np.empty_like(u)
+        self._check(self.lib.eq_ku_apply(self.h, ptr(u), ptr(y), MASK[mask], PREC[prec]))
+        return y
+
+    def kni(self, x, prec="fp64"):
+        """y = kni x with kni = -K[FltNI, FltNI]."""
+        x = f64(x)
+        y = np.empty_like(x)
+        self._check(self.lib.eq_kni_apply(self.h, ptr(x), ptr(y), PREC[prec]))
+        return y
+
+    def element_computation(self, u):
+        """a_elem[Nel, 5(j), 5(i)] of element_computation! (src/PCG.jl:71-86)."""
+        u = f64(u)
+        out = np.empty((self.P.Nel, 5, 5))
+        self._check(self.lib.eq_element_computation(self.h, ptr(u), ptr(out)))
+        return out
+
+    # ---- AMG / CG ----------------------------------------------------------
+    def amg_setup(self, theta=0.25, max_levels=10, max_coarse=10, strength_abs=1, smoother="jacobi", degree=1):
+        sm = {"jacobi": L.EQ_SMOOTH_JACOBI, "chebyshev": L.EQ_SMOOTH_CHEBYSHEV}[smoother]
+        self._check(self.lib.eq_amg_setup(self.h, theta, max_levels, max_coarse, strength_abs, sm, degree))
+        return self.amg_info()
+
+    def amg_info(self):
+        nl = C.c_int()
+        n = np.zeros(32, np.int64)
+        nnz = np.zeros(32, np.int64)
+        self._check(self.lib.eq_amg_info(self.h, C.byref(nl), ptr(n, C.c_int64), ptr(nnz, C.c_int64), 32))
+        return list(zip(n[:nl.value].tolist(), nnz[:nl.value].tolist()))
+
+    def amg_ldiv(self, b, prec="fp64"):
+        b = f64(b)
+        x = np.empty_like(b)
+        self._check(self.lib.eq_amg_ldiv(self.h, ptr(b), ptr(x), PREC[prec]))
+        return x
+
+    def cg(self, x, b, tol, maxiter=0, pc="amg", prec="fp64"):
+        """cg!(x, kni, b; Pl, tol, maxiter) on the GPU.  Returns (x, iters, relres, converged)."""
+        x = f64(x).copy()
+        b = f64(b)
+        it = C.c_int64()
+        rr = C.c_double()
+        rc = self._check(self.lib.eq_cg(self.h, ptr(x), ptr(b), tol, maxiter, PC[pc], PREC[prec],
+                                        C.byref(it), C.byref(rr)), allow=(L.EQ_PCG_NOCONV,))
+        return x, it.value, rr.value, rc == 0
+
+    def pcg_jacobi(self, F, dnew, tol=1e-5, maxit=8000):
+        F = f64(F)
+        d = f64(dnew).copy()
+        it = C.c_int64()
+        rc = self._check(self.lib.eq_pcg_jacobi(self.h, ptr(F), ptr(d), tol, maxit, C.byref(it)),
+                         allow=(L.EQ_PCG_NOCONV,))
+        return d, it.value, rc == 0
+
+    # ---- fault routines -------------------------------------------------------
+    def ids(self, xLf, Vo, psi, dt, Vf, cnd, IDstate=2):
+        a = [f64(np.atleast_1d(z)) for z in (xLf, Vo, psi, Vf)]
+        out = np.empty_like(a[0])
+        self._check(self.lib.eq_ids(self.h, len(out), ptr(a[0]), ptr(a[1]), ptr(a[2]), dt, ptr(a[3]), cnd,
+                                    IDstate, ptr(out)))
+        return out
+
+    def ids2(self, xLf, Vo, psi, psi1, dt, Vf, Vf1, IDstate=2):
+        a = [f64(np.atleast_1d(z)) for z in (xLf, Vo, psi, psi1, Vf, Vf1)]
+        out = np.empty_like(a[0])
+        self._check(self.lib.eq_ids2(self.h, len(out), ptr(a[0]), ptr(a[1]), ptr(a[2]), ptr(a[3]), dt, ptr(a[4]),
+                                     ptr(a[5]), IDstate, ptr(out)))
+        return out
+
+    def nrsearch(self, fo, Vo, cca, ccb, Seff, tau, tauo, psi, FltZ, FltVfree):
+        a = [f64(np.atleast_1d(z)) for z in (fo, Vo, cca, ccb, Seff, tau, tauo, psi, FltZ, FltVfree)]
+        n = len(a[0])
+        Vf = np.empty(n)
+        t = np.empty(n)
+        k = np.empty(n, np.int32)
+        self._check(self.lib.eq_nrsearch(self.h, n, *[ptr(z) for z in a], ptr(Vf), ptr(t), ptr(k, C.c_int32)),
+                    allow=(L.EQ_NR_NOCONV,))
+        return Vf, t, k
+
+    def fbc(self, IDstate, NFBC, FltNglob, psi1, Vf1, tau1, psi2, Vf2, tau2, psi, Vf, FltVfree, dt):
+        outs = [f64(z).copy() for z in (psi1, Vf1, tau1, psi2, Vf2, tau2)]
+        ins = [f64(z) for z in (psi, Vf, FltVfree)]
+        fail = C.c_int64()
+        rc = self._check(self.lib.eq_fbc(self.h, IDstate, NFBC, FltNglob, *[ptr(z) for z in outs],
+                                         *[ptr(z) for z in ins], dt, C.byref(fail)),
+                         allow=(L.EQ_NR_NAN, L.EQ_NR_NOCONV))
+        return rc, fail.value, outs
+
+    def slr(self, NFBC, FltNglob, psi, psi1, Vf, Vf1, IDstate, tau1, dt):
+        p1 = f64(psi1).copy()
+        v1 = f64(Vf1).copy()
+        self._check(self.lib.eq_slr(self.h, NFBC, FltNglob, ptr(f64(psi)), ptr(p1), ptr(f64(Vf)), ptr(v1),
+                                    IDstate, ptr(f64(tau1)), dt))
+        return p1, v1
+
+    def dtevol(self, dt, dtmin, XiLf, FaultNglob, NFBC, Vf, isolver):
+        out = C.c_double()
+        self._check(self.lib.eq_dtevol(self.h, dt, dtmin, ptr(f64(XiLf)), FaultNglob, NFBC, ptr(f64(Vf)),
+                                       isolver, C.byref(out)))
+        return out.value
+
+    # ---- time loop ------------------------------------------------------------
+    def set_solver(self, tol=1e-6, maxiter=0, pc="amg", prec="fp64", guess="previous"):
+        o = L.SolverOpts(tol, maxiter, PC[pc], PREC[prec],
+                         {"previous": L.EQ_GUESS_PREVIOUS, "learned": L.EQ_GUESS_LEARNED}[guess], 1)
+        self._check(self.lib.eq_set_solver(self.h, C.byref(o)))
+
+    def state_init(self, v_init=None, psi_init=None):
+        P = self.P
+        v0 = P.v_init if v_init is None else v_init
+        ps = f64(P.psi_init) if psi_init is None else (None if psi_init is False else f64(psi_init))
+        self._check(self.lib.eq_state_init(self.h, v0, ptr(ps)))
+
+    def state_get(self):
+        P = self.P
+        d, v, a = (np.empty(P.nglob) for _ in range(3))
+        psi, tau, Vf = (np.empty(P.FltNglob) for _ in range(3))
+        t, dt = C.c_double(), C.c_double()
+        it, iso = C.c_int64(), C.c_int64()
+        self._check(self.lib.eq_state_get(self.h, ptr(d), ptr(v), ptr(a), ptr(psi), ptr(tau), ptr(Vf),
+                                          C.byref(t), C.byref(dt), C.byref(it), C.byref(iso)))
+        return dict(d=d, v=v, a=a, psi=psi, tau=tau, Vf=Vf, t=t.value, dt=dt.value, it=it.value,
+                    isolver=iso.value)
+
+    def state_set(self, d=None, v=None, a=None, psi=None, tau=None, t=0.0, dt=None, isolver=1):
+        arrs = [None if z is None else f64(z) for z in (d, v, a, psi, tau)]
+        self._check(self.lib.eq_state_set(self.h, *[ptr(z) for z in arrs], t,
+                                          self.P.dt0 if dt is None else dt, isolver))
+
+    def step(self):
+        info = L.StepInfo()
+        self._check(self.lib.eq_step(self.h, C.byref(info)))
+        return StepInfo(info.t, info.dt_used, info.dt_next, info.Vfmax, info.it, info.isolver_used,
+                        info.isolver_next, (info.pcg_iters[0], info.pcg_iters[1]), info.nr_noconv)
+
+    def run(self, total_time=None, max_steps=0, cap_steps=200000, cap_events=400, cap_is=4000, cap_seis=30000,
+            cap_rec=190000, snapshots=True):
+        """main(P) on the GPU; returns a dict with the fields of struct results."""
+        P = self.P
+        F = P.FltNglob
+        ns = len(P.out_seis)
+        cap_steps = min(cap_steps, max_steps) if max_steps > 0 else cap_steps
+        bufs = dict(time_=np.zeros(cap_steps), Vfmax=np.zeros(cap_steps), isolver=np.zeros(cap_steps, np.int64),
+                    pcg_iters=np.zeros(2 * cap_steps, np.int64), tStart=np.zeros(cap_events),
+                    tEnd=np.zeros(cap_events), hypo=np.zeros(cap_events))
+        if snapshots:
+            for k in ("taubefore", "tauafter", "delfafter"):
+                bufs[k] = np.zeros((cap_events, F))
+            for k in ("is_slip", "is_slipvel", "is_stress"):
+                bufs[k] = np.zeros((cap_is, F))
+            for k in ("seismic_slip", "seismic_slipvel", "seismic_stress"):
+                bufs[k] = np.zeros((cap_seis, F))
+            for k in ("dSeis", "vSeis", "aSeis"):
+                bufs[k] = np.zeros((ns, cap_rec))
+        R = L.Results()
+        R.cap_steps, R.cap_events, R.cap_is, R.cap_seis, R.cap_rec = cap_steps, cap_events, cap_is, cap_seis, cap_rec
+        for k, v in bufs.items():
+            setattr(R, k, ptr(v, C.c_int64 if v.dtype == np.int64 else C.c_double))
+        T = P.Total_time if total_time is None else total_time
+        self._check(self.lib.eq_run(self.h, T, max_steps, C.byref(R)))
+        n = R.n_steps
+        out = dict(n_steps=n, n_qs=R.n_qs, n_dyn=R.n_dyn, total_pcg_iters=R.total_pcg_iters,
+                   nr_noconv=R.nr_noconv, wall_qs=R.wall_qs, wall_dyn=R.wall_dyn,
+                   time_=bufs["time_"][:min(n, cap_steps)], Vfmax=bufs["Vfmax"][:min(n, cap_steps)],
+                   isolver=bufs["isolver"][:min(n, cap_steps)],
+                   pcg_iters=bufs["pcg_iters"][:2 * min(n, cap_steps)].reshape(-1, 2),
+                   tStart=bufs["tStart"][:min(R.n_start, cap_events)], tEnd=bufs["tEnd"][:min(R.n_end, cap_events)],
+                   hypo=bufs["hypo"][:min(R.n_start, cap_events)])
+        if snapshots:
+            out.update(taubefore=bufs["taubefore"][:min(R.n_start, cap_events)],
+                       tauafter=bufs["tauafter"][:min(R.n_end, cap_events)],
+                       delfafter=bufs["delfafter"][:min(R.n_end, cap_events)],
+                       is_slip=bufs["is_slip"][:min(R.n_is, cap_is)],
+                       is_slipvel=bufs["is_slipvel"][:min(R.n_is, cap_is)],
+                       is_stress=bufs["is_stress"][:min(R.n_is, cap_is)],
+                       seismic_slip=bufs["seismic_slip"][:min(R.n_seis, cap_seis)],
+                       seismic_slipvel=bufs["seismic_slipvel"][:min(R.n_seis, cap_seis)],
+                       seismic_stress=bufs["seismic_stress"][:min(R.n_seis, cap_seis)],
+                       dSeis=bufs["dSeis"][:, :min(R.n_rec, cap_rec)].T, vSeis=bufs["vSeis"][:, :min(R.n_rec, cap_rec)].T,
+                       aSeis=bufs["aSeis"][:, :min(R.n_rec, cap_rec)].T)
+        return out
+
+
+    def time_kernel(self, which="ku", prec="fp64", reps=50):
+        """Average device ms per launch, HIP events on the library's stream."""
+        w = {"ku": L.EQ_TIME_KU, "cg_dir": L.EQ_TIME_CG_DIR, "vcycle": L.EQ_TIME_VCYCLE,
+             "cg_iter": L.EQ_TIME_CG_ITER, "newmark": L.EQ_TIME_NEWMARK}[which]
+        out = C.c_double()
+        self._check(self.lib.eq_time_kernel(self.h, w, PREC[prec], reps, C.byref(out)))
+        return out.value
+
+
+@dataclass
+class StepInfo:
+    t: float
+    dt_used: float
+    dt_next: float
+    Vfmax: float
+    it: int
+    isolver_used: int
+    isolver_next: int
+    pcg_iters: tuple
+    nr_noconv: int
+
+
+# ---------------------------------------------------------------------------
+# reference-named functions (positional signatures of the Julia routines)
+# ---------------------------------------------------------------------------
+def FBC(ctx: EqContext, IDstate, P, NFBC, FltNglob, psi1, Vf1, tau1, psi2, Vf2, tau2, psi, Vf, FltVfree, dt):
+    """FBC!(IDstate, P, NFBC, FltNglob, psi1, Vf1, tau1, psi2, Vf2, tau2, psi, Vf, FltVfree, dt)
+    (src/NRsearch_serial.jl:6-45).  Returns the 6-tuple, or None when the NaN / Vf guard fires."""
+    rc, fail, outs = ctx.fbc(IDstate, NFBC, FltNglob, psi1, Vf1, tau1, psi2, Vf2, tau2, psi, Vf, FltVfree, dt)
+    if rc == L.EQ_NR_NAN:
+        log.error("NR SEARCH FAILED! (fault location %d)", fail)
+        return None
+    for dst, src in zip((psi1, Vf1, tau1, psi2, Vf2, tau2), outs):
+        if isinstance(dst, np.ndarray) and dst.dtype == np.float64:
+            dst[:] = src
+    return tuple(outs)
+
+
+def NRsearch(ctx: EqContext, fo, Vo, cca, ccb, Seff, tau, tauo, psi, FltZ, FltVfree):
+    """NRsearch!(fo, Vo, cca, ccb, Seff, tau, tauo, psi, FltZ, FltVfree) -> (Vf, tau) (src/NRsearch_serial.jl:47-98)."""
+    Vf, t, k = ctx.nrsearch(fo, Vo, cca, ccb, Seff, tau, tauo, psi, FltZ, FltVfree)
+    for kk in np.atleast_1d(k)[np.atleast_1d(k) < 0]:
+        log.info("k = %d", -kk)
+    if np.ndim(fo) == 0:
+        return float(Vf[0]), float(t[0])
+    return Vf, t
+
+
+def IDS(ctx: EqContext, xLf, Vo, psi, dt, Vf, cnd, IDstate=2):
+    """IDS!(xLf, Vo, psi, dt, Vf, cnd, IDstate) (src/otherFunctions_serial.jl:5-32)."""
+    r = ctx.ids(xLf, Vo, psi, dt, Vf, cnd, IDstate)
+    return float(r[0]) if np.ndim(psi) == 0 else r
+
+
+def IDS2(ctx: EqContext, xLf, Vo, psi, psi1, dt, Vf, Vf1, IDstate=2):
+    """IDS2!(xLf, Vo, psi, psi1, dt, Vf, Vf1, IDstate) (src/otherFunctions_serial.jl:35-59)."""
+    r = ctx.ids2(xLf, Vo, psi, psi1, dt, Vf, Vf1, IDstate)
+    return float(r[0]) if np.ndim(psi) == 0 else r
+
+
+def slrFunc(ctx: EqContext, P, NFBC, FltNglob, psi, psi1, Vf, Vf1, IDstate, tau1, dt):
+    """slrFunc!(P, NFBC, FltNglob, psi, psi1, Vf, Vf1, IDstate, tau1, dt) -> (psi1, Vf1)."""
+    p1, v1 = ctx.slr(NFBC, FltNglob, psi, psi1, Vf, Vf1, IDstate, tau1, dt)
+    if isinstance(psi1, np.ndarray):
+        psi1[:] = p1
+    if isinstance(Vf1, np.ndarray):
+        Vf1[:] = v1
+    return p1, v1
+
+
+def dtevol(ctx: EqContext, dt, dtmin, XiLf, FaultNglob, NFBC, Vf, isolver):
+    """dtevol!(dt, dtmin, XiLf, FaultNglob, NFBC, Vf, isolver) -> dt (src/dtevol.jl:5-44)."""
+    return ctx.dtevol(dt, dtmin, XiLf, FaultNglob, NFBC, Vf, isolver)
+
+
+def cg(ctx: EqContext, x, b, tol, maxiter=0, pc="amg", prec="fp64"):
+    """cg!(x, kni, b; Pl, tol) (src/main.jl:256): solves kni x = b in place, returns x."""
+    xn, it, rr, ok = ctx.cg(x, b, tol, maxiter, pc, prec)
+    if isinstance(x, np.ndarray) and x.dtype == np.float64:
+        x[:] = xn
+    return xn
+
+
+def PCG(ctx: EqContext, F, dnew, tol=1e-5, maxit=8000):
+    """PCG! (src/PCG.jl:8-67): Jacobi CG; returns dnew, or None (with @error) when it does not converge."""
+    d, it, ok = ctx.pcg_jacobi(F, dnew, tol, maxit)
+    if not ok:
+        log.error("PCG did not converge")
+        return None
+    return d
+
+
+def main(P, tol=1e-6, pc="amg", prec="fp64", total_time=None, max_steps=0, amg=None):
+    """main(P) (src/main.jl:67-476) on the GPU; returns the results dict."""
+    ctx = EqContext(P, amg=(amg or {}) if pc == "amg" else False)
+    ctx.set_solver(tol=tol, pc=pc, prec=prec)
+    ctx.state_init()
+    return ctx.run(total_time=total_time, max_steps=max_steps)

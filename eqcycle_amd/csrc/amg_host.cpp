@@ -1,0 +1,371 @@
+// Host-side set-up for the GPU preconditioner and operators.
+//
+// * assemble_kii: K_II (stiffness rows/columns of the non-fault nodes) as CSR
+//   in the structured interior order, from nodal rigidity.  Same operator as
+//   stiffness_assembly / K_element (src/Kassemble.jl:3-63) restricted to
+//   FltNI (src/main.jl:195).
+// * amg_build: Ruge-Stueben hierarchy of the kind ruge_stuben(kni) builds
+//   (src/main.jl:200): classical strength with threshold theta, RS C/F
+//   splitting (Ruge & Stueben 1987, bucketed max-lambda selection), direct
+//   interpolation with separate positive / negative weight sums, R = P^T,
+//   Galerkin coarse operators, dense inverse on the coarsest level.
+//   AlgebraicMultigrid.jl is not part of the reference repository and its
+//   version is unpinned; this follows the published algorithm.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "eq_internal.h"
+
+namespace eq {
+
+static inline double wl(const Geom& g, int idx) { return g.w[idx & 3]; }
+
+// Sum of GLL weights of the elements that contain grid line L (of nl lines).
+static inline double wline(const Geom& g, int L, int nl) {
+  int r = L & 3;
+  if (r != 0) return g.w[r];
+  return (L == 0 || L == nl - 1) ? g.w[0] : g.w[0] + g.w[4];
+}
+
+void assemble_kii(const Geom& g, const double* mu, HostCsr* K) {
+  const int NX = g.NX, NY = g.NY;
+  const int64_t nI = (int64_t)NX * (NY - 1);
+  K->n = K->m = nI;
+  K->rowptr.assign(nI + 1, 0);
+  K->col.clear();
+  K->val.clear();
+  K->col.reserve(nI * 13);
+  K->val.reserve(nI * 13);
+  auto gx = [&](int I, int J) { return g.c1 * wline(g, J, NY) * wl(g, I) * mu[(int64_t)J * NX + I]; };
+  auto gy = [&](int I, int J) { return g.c2 * wline(g, I, NX) * wl(g, J) * mu[(int64_t)J * NX + I]; };
+  int64_t cols[32];
+  double vals[32];
+  for (int J = 1; J < NY; J++) {
+    for (int I = 0; I < NX; I++) {
+      int nc = 0;
+      auto add = [&](int64_t c, double v) {
+        for (int q = 0; q < nc; q++)
+          if (cols[q] == c) { vals[q] += v; return; }
+        cols[nc] = c; vals[nc] = v; nc++;
+      };
+      // x segments containing I
+      int s_lo = (I & 3) == 0 ? (I >> 2) - 1 : (I >> 2);
+      int s_hi = (I >> 2);
+      for (int s = s_lo; s <= s_hi; s++) {
+        if (s < 0 || s >= g.nelx) continue;
+        int i = I - 4 * s;
+        for (int k = 0; k < 5; k++) {
+          double v = 0;
+          for (int p = 0; p < 5; p++) v += g.D[p * 5 + i] * g.D[p * 5 + k] * gx(4 * s + p, J);
+          add((int64_t)J * NX + 4 * s + k, v);
+        }
+      }
+      int t_lo = (J & 3) == 0 ? (J >> 2) - 1 : (J >> 2);
+      int t_hi = (J >> 2);
+      for (int t = t_lo; t <= t_hi; t++) {
+        if (t < 0 || t >= g.nely) continue;
+        int j = J - 4 * t;
+        for (int l = 0; l < 5; l++) {
+          double v = 0;
+          for (int q = 0; q < 5; q++) v += g.D[q * 5 + j] * g.D[q * 5 + l] * gy(I, 4 * t + q);
+          int Jc = 4 * t + l;
+          if (Jc == 0) continue;  // fault column: not an unknown
+          add((int64_t)Jc * NX + I, v);
+        }
+      }
+      // sort by column, drop exact zeros, shift to the reduced index
+      int ord[32];
+      for (int q = 0; q < nc; q++) ord[q] = q;
+      std::sort(ord, ord + nc, [&](int a, int b) { return cols[a] < cols[b]; });
+      for (int q = 0; q < nc; q++) {
+        double v = vals[ord[q]];
+        if (v == 0.0) continue;
+        K->col.push_back((int32_t)(cols[ord[q]] - NX));
+        K->val.push_back(v);
+      }
+      int64_t r = (int64_t)(J - 1) * NX + I;
+      K->rowptr[r + 1] = (int64_t)K->col.size();
+    }
+  }
+}
+
+void csr_transpose(const HostCsr& A, HostCsr* T) {
+  T->n = A.m;
+  T->m = A.n;
+  T->rowptr.assign(A.m + 1, 0);
+  int64_t nnz = A.nnz();
+  T->col.resize(nnz);
+  T->val.resize(nnz);
+  for (int64_t q = 0; q < nnz; q++) T->rowptr[A.col[q] + 1]++;
+  for (int64_t i = 0; i < A.m; i++) T->rowptr[i + 1] += T->rowptr[i];
+  std::vector<int64_t> pos(T->rowptr.begin(), T->rowptr.end() - 1);
+  for (int64_t i = 0; i < A.n; i++)
+    for (int64_t q = A.rowptr[i]; q < A.rowptr[i + 1]; q++) {
+      int64_t c = A.col[q];
+      T->col[pos[c]] = (int32_t)i;
+      T->val[pos[c]] = A.val[q];
+      pos[c]++;
+    }
+}
+
+static void csr_matmul(const HostCsr& A, const HostCsr& B, HostCsr* C) {
+  const int64_t n = A.n, m = B.m;
+  C->n = n;
+  C->m = m;
+  C->rowptr.assign(n + 1, 0);
+  C->col.clear();
+  C->val.clear();
+  std::vector<int64_t> mark(m, -1);
+  std::vector<double> acc(m, 0.0);
+  std::vector<int32_t> cur;
+  for (int64_t i = 0; i < n; i++) {
+    cur.clear();
+    for (int64_t q = A.rowptr[i]; q < A.rowptr[i + 1]; q++) {
+      int64_t k = A.col[q];
+      double a = A.val[q];
+      for (int64_t s = B.rowptr[k]; s < B.rowptr[k + 1]; s++) {
+        int32_t j = B.col[s];
+        if (mark[j] != i) { mark[j] = i; cur.push_back(j); acc[j] = a * B.val[s]; }
+        else acc[j] += a * B.val[s];
+      }
+    }
+    std::sort(cur.begin(), cur.end());
+    for (int32_t j : cur) { C->col.push_back(j); C->val.push_back(acc[j]); }
+    C->rowptr[i + 1] = (int64_t)C->col.size();
+  }
+}
+
+// Strong connections of row i: |a_ij| >= theta max_{k!=i} |a_ik| (abs) or
+// -a_ij >= theta max_{k!=i} (-a_ik) (signed).  The diagonal is kept, with A's
+// values stored at the strong positions.
+static void strength(const HostCsr& A, double theta, bool use_abs, HostCsr* S) {
+  S->n = S->m = A.n;
+  S->rowptr.assign(A.n + 1, 0);
+  S->col.clear();
+  S->val.clear();
+  for (int64_t i = 0; i < A.n; i++) {
+    double mx = 0.0;
+    for (int64_t q = A.rowptr[i]; q < A.rowptr[i + 1]; q++)
+      if (A.col[q] != i) mx = std::max(mx, use_abs ? std::fabs(A.val[q]) : -A.val[q]);
+    double thr = theta * mx;
+    for (int64_t q = A.rowptr[i]; q < A.rowptr[i + 1]; q++) {
+      double a = A.val[q];
+      bool keep = (A.col[q] == i) ||
+                  (a != 0.0 && (use_abs ? std::fabs(a) >= thr : (mx > 0 && -a >= thr)));
+      if (keep) { S->col.push_back(A.col[q]); S->val.push_back(a); }
+    }
+    S->rowptr[i + 1] = (int64_t)S->col.size();
+  }
+}
+
+enum { U_PT = 0, C_PT = 1, F_PT = 2 };
+
+// Bucketed RS first pass: repeatedly take the undecided point with the most
+// strong dependents as C, make its dependents F, and raise the weight of the
+// points those new F points depend on.
+static void rs_split(const HostCsr& S, const HostCsr& T, std::vector<int>& split) {
+  const int64_t n = S.n;
+  std::vector<int64_t> lam(n), bucket_start(n + 1, 0), bucket_cnt(n + 1, 0), at(n), pos(n);
+  for (int64_t i = 0; i < n; i++) { lam[i] = T.rowptr[i + 1] - T.rowptr[i]; bucket_cnt[lam[i]]++; }
+  int64_t run = 0;
+  for (int64_t b = 0; b < n; b++) { bucket_start[b] = run; run += bucket_cnt[b]; bucket_cnt[b] = 0; }
+  for (int64_t i = 0; i < n; i++) {
+    int64_t b = lam[i], p = bucket_start[b] + bucket_cnt[b]++;
+    at[p] = i;
+    pos[i] = p;
+  }
+  split.assign(n, U_PT);
+  for (int64_t i = 0; i < n; i++)
+    if (lam[i] == 0 || (lam[i] == 1 && T.col[T.rowptr[i]] == i)) split[i] = F_PT;
+  auto swap_slots = [&](int64_t node, int64_t newp) {
+    int64_t oldp = pos[node], other = at[newp];
+    at[oldp] = other;
+    pos[other] = oldp;
+    at[newp] = node;
+    pos[node] = newp;
+  };
+  for (int64_t top = n - 1; top >= 0; top--) {
+    int64_t i = at[top];
+    bucket_cnt[lam[i]]--;
+    if (split[i] == F_PT) continue;
+    split[i] = C_PT;
+    for (int64_t q = T.rowptr[i]; q < T.rowptr[i + 1]; q++) {
+      int64_t j = T.col[q];
+      if (split[j] != U_PT) continue;
+      split[j] = F_PT;
+      for (int64_t r = S.rowptr[j]; r < S.rowptr[j + 1]; r++) {
+        int64_t k = S.col[r];
+        if (split[k] != U_PT || lam[k] >= n - 1) continue;
+        int64_t b = lam[k], newp = bucket_start[b] + bucket_cnt[b] - 1;
+        swap_slots(k, newp);
+        bucket_cnt[b]--;
+        bucket_cnt[b + 1]++;
+        bucket_start[b + 1] = newp;
+        lam[k]++;
+      }
+    }
+    for (int64_t q = S.rowptr[i]; q < S.rowptr[i + 1]; q++) {
+      int64_t j = S.col[q];
+      if (split[j] != U_PT || lam[j] == 0) continue;
+      int64_t b = lam[j], newp = bucket_start[b];
+      swap_slots(j, newp);
+      bucket_cnt[b]--;
+      bucket_cnt[b - 1]++;
+      bucket_start[b]++;
+      lam[j]--;
+    }
+  }
+}
+
+static void direct_interpolation(const HostCsr& A, const HostCsr& S, const std::vector<int>& split, HostCsr* P) {
+  const int64_t n = A.n;
+  std::vector<int64_t> cidx(n, -1);
+  int64_t nc = 0;
+  for (int64_t i = 0; i < n; i++)
+    if (split[i] == C_PT) cidx[i] = nc++;
+  P->n = n;
+  P->m = nc;
+  P->rowptr.assign(n + 1, 0);
+  P->col.clear();
+  P->val.clear();
+  for (int64_t i = 0; i < n; i++) {
+    if (split[i] == C_PT) {
+      P->col.push_back((int32_t)cidx[i]);
+      P->val.push_back(1.0);
+    } else {
+      double spos = 0, sneg = 0, apos = 0, aneg = 0, diag = 0;
+      for (int64_t q = S.rowptr[i]; q < S.rowptr[i + 1]; q++) {
+        int64_t j = S.col[q];
+        if (j != i && split[j] == C_PT) (S.val[q] < 0 ? sneg : spos) += S.val[q];
+      }
+      for (int64_t q = A.rowptr[i]; q < A.rowptr[i + 1]; q++) {
+        if (A.col[q] == i) diag += A.val[q];
+        else (A.val[q] < 0 ? aneg : apos) += A.val[q];
+      }
+      double alpha = sneg != 0 ? aneg / sneg : 0.0;
+      double beta = spos != 0 ? apos / spos : 0.0;
+      if (spos == 0) { diag += apos; beta = 0; }
+      double cneg = -alpha / diag, cpos = -beta / diag;
+      for (int64_t q = S.rowptr[i]; q < S.rowptr[i + 1]; q++) {
+        int64_t j = S.col[q];
+        if (j != i && split[j] == C_PT) {
+          P->col.push_back((int32_t)cidx[j]);
+          P->val.push_back(S.val[q] < 0 ? cneg * S.val[q] : cpos * S.val[q]);
+        }
+      }
+    }
+    P->rowptr[i + 1] = (int64_t)P->col.size();
+  }
+}
+
+static bool dense_inverse(int64_t n, std::vector<double> a, std::vector<double>* inv) {
+  inv->assign(n * n, 0.0);
+  for (int64_t i = 0; i < n; i++) (*inv)[i * n + i] = 1.0;
+  for (int64_t c = 0; c < n; c++) {
+    int64_t p = c;
+    for (int64_t r = c + 1; r < n; r++)
+      if (std::fabs(a[r * n + c]) > std::fabs(a[p * n + c])) p = r;
+    if (a[p * n + c] == 0.0) return false;
+    if (p != c)
+      for (int64_t j = 0; j < n; j++) {
+        std::swap(a[c * n + j], a[p * n + j]);
+        std::swap((*inv)[c * n + j], (*inv)[p * n + j]);
+      }
+    double d = a[c * n + c];
+    for (int64_t j = 0; j < n; j++) { a[c * n + j] /= d; (*inv)[c * n + j] /= d; }
+    for (int64_t r = 0; r < n; r++) {
+      if (r == c) continue;
+      double f = a[r * n + c];
+      if (f == 0.0) continue;
+      for (int64_t j = 0; j < n; j++) {
+        a[r * n + j] -= f * a[c * n + j];
+        (*inv)[r * n + j] -= f * (*inv)[c * n + j];
+      }
+    }
+  }
+  return true;
+}
+
+double estimate_lambda_max_dinvA(const HostCsr& A, const std::vector<double>& dinv, int iters) {
+  const int64_t n = A.n;
+  if (n == 0) return 1.0;
+  std::vector<double> x(n), y(n);
+  // deterministic start vector
+  uint64_t s = 0x2545F4914F6CDD1DULL;
+  for (int64_t i = 0; i < n; i++) {
+    s ^= s << 13; s ^= s >> 7; s ^= s << 17;
+    x[i] = 0.5 + (double)(s >> 11) * (1.0 / 9007199254740992.0);
+  }
+  double lam = 1.0;
+  for (int it = 0; it < iters; it++) {
+    double nx = 0;
+    for (int64_t i = 0; i < n; i++) nx += x[i] * x[i];
+    nx = std::sqrt(nx);
+    for (int64_t i = 0; i < n; i++) x[i] /= nx;
+    for (int64_t i = 0; i < n; i++) {
+      double acc = 0;
+      for (int64_t q = A.rowptr[i]; q < A.rowptr[i + 1]; q++) acc += A.val[q] * x[A.col[q]];
+      y[i] = dinv[i] * acc;
+    }
+    double num = 0;
+    for (int64_t i = 0; i < n; i++) num += x[i] * y[i];
+    lam = num;
+    x.swap(y);
+  }
+  return lam;
+}
+
+int amg_build(const HostCsr& A0, const AmgOptions& opt, HostHierarchy* H, std::string* err) {
+  H->lev.clear();
+  HostLevel L0;
+  L0.A = A0;
+  const double sgn = opt.build_on_negated ? -1.0 : 1.0;
+  if (sgn < 0)
+    for (auto& v : L0.A.val) v = -v;
+  H->lev.push_back(std::move(L0));
+  while ((int)H->lev.size() < opt.max_levels && H->lev.back().A.n > opt.max_coarse) {
+    HostLevel& cur = H->lev.back();
+    HostCsr S, T;
+    strength(cur.A, opt.theta, opt.strength_abs != 0, &S);
+    csr_transpose(S, &T);
+    std::vector<int> split;
+    rs_split(S, T, split);
+    direct_interpolation(cur.A, S, split, &cur.P);
+    if (cur.P.m == 0 || cur.P.m == cur.A.n) {
+      cur.P = HostCsr();
+      break;
+    }
+    csr_transpose(cur.P, &cur.R);
+    HostCsr RA, Ac;
+    csr_matmul(cur.R, cur.A, &RA);
+    csr_matmul(RA, cur.P, &Ac);
+    HostLevel next;
+    next.A = std::move(Ac);
+    H->lev.push_back(std::move(next));
+  }
+  // back to the sign of the operator the GPU applies (K_II, SPD)
+  if (sgn < 0)
+    for (auto& L : H->lev)
+      for (auto& v : L.A.val) v = -v;
+  for (auto& L : H->lev) {
+    L.dinv.assign(L.A.n, 0.0);
+    for (int64_t i = 0; i < L.A.n; i++)
+      for (int64_t q = L.A.rowptr[i]; q < L.A.rowptr[i + 1]; q++)
+        if (L.A.col[q] == i) L.dinv[i] = 1.0 / L.A.val[q];
+    L.lambda_max = estimate_lambda_max_dinvA(L.A, L.dinv, 30);
+  }
+  const HostCsr& Ac = H->lev.back().A;
+  H->ncoarse = Ac.n;
+  std::vector<double> dense(Ac.n * Ac.n, 0.0);
+  for (int64_t i = 0; i < Ac.n; i++)
+    for (int64_t q = Ac.rowptr[i]; q < Ac.rowptr[i + 1]; q++) dense[i * Ac.n + Ac.col[q]] += Ac.val[q];
+  if (!dense_inverse(Ac.n, dense, &H->coarse_inv)) {
+    if (err) *err = "AMG: singular coarsest operator";
+    return EQ_EINVAL;
+  }
+  return EQ_OK;
+}
+
+}  // namespace eq

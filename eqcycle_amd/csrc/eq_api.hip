@@ -1,0 +1,1364 @@
+// C ABI of libeqcycle_hip: context, boundary conversion (reference MeshBox
+// numbering <-> structured device layout), solver orchestration and the
+// device-resident time loop.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "eq_internal.h"
+#include "eq_kernels.h"
+
+using namespace eq;
+
+namespace {
+
+template <typename T>
+struct CgWork {
+  T *r = nullptr, *c = nullptr, *u[2] = {nullptr, nullptr}, *q = nullptr, *x0 = nullptr, *res0 = nullptr;
+  T *b = nullptr, *x = nullptr;   // solve-level vectors for the mixed-precision inner solve
+  int cur = 0;
+};
+
+template <typename T>
+struct DevLevelVecs {
+  T *b = nullptr, *x = nullptr, *xp = nullptr, *res = nullptr;
+};
+
+struct DevLevel {
+  int64_t n = 0;
+  DevCsr A, P, R;   // P, R link this level to the next
+  double* dinv = nullptr;
+  float* dinv32 = nullptr;
+  double omega = 0.0;
+  DevLevelVecs<double> v64;
+  DevLevelVecs<float> v32;
+};
+
+}  // namespace
+
+struct eq_ctx {
+  std::string err;
+  hipStream_t s = nullptr;
+  Geom g{};
+  int NX = 0, NY = 0;
+  int64_t N = 0, nI = 0, Nel = 0;
+  std::vector<int64_t> perm;          // reference index (0-based) -> structured node
+  std::vector<int64_t> fltni;         // FltNI position -> structured node
+  std::vector<void*> allocs;
+  // material
+  bool have_material = false;
+  std::vector<double> mu_host;        // structured
+  double* mu = nullptr;
+  float* mu32 = nullptr;
+  double* dinv = nullptr;             // 1 / diag(K_II) (fault row 0)
+  float* dinv32 = nullptr;
+  double* ones = nullptr;
+  float* ones32 = nullptr;
+  // params
+  bool have_params = false;
+  int64_t FltNglob = 0, NFBC = 1, IDstate = 2;
+  double Vpl = 0, Vthres = 1e-3, Vevne = 1e-3, dt0 = 0;
+  FaultDev fp{};
+  double* M = nullptr;
+  std::vector<double> FltX, tauo;
+  std::vector<int64_t> out_seis;      // structured
+  // state
+  bool have_state = false;
+  double *d = nullptr, *v = nullptr, *a = nullptr, *d2 = nullptr, *v2 = nullptr, *a2 = nullptr;
+  double *dPre = nullptr, *bq = nullptr;
+  FaultState fs{}, fs_api{};
+  double t = 0, dt = 0;
+  int64_t it = 0;
+  int isolver = 1;
+  // solver
+  eq_solver_opts opts{1e-6, 0, EQ_PC_AMG, EQ_PREC_FP64, EQ_GUESS_PREVIOUS, 1};
+  CgWork<double> w64;
+  CgWork<float> w32;
+  double *part_rr = nullptr, *part_bb = nullptr, *part_rho = nullptr, *part_uq = nullptr;
+  int npart = 0;
+  CgScal* sc = nullptr;
+  CgScal* sc_host = nullptr;
+  int64_t last_iters = 8;
+  // AMG
+  bool have_amg = false;
+  std::vector<DevLevel> lev;          // lev[0]: matrix-free fine level (A unused)
+  double* coarse_inv = nullptr;
+  int64_t ncoarse = 0;
+  std::vector<int64_t> amg_n, amg_nnz;
+  // step status
+  StepStatus* st = nullptr;
+  StepStatus* st_host = nullptr;
+  int* flags = nullptr;
+  int64_t nr_noconv_total = 0;
+  int64_t pcg_noconv = 0;
+
+  ~eq_ctx() {
+    for (void* p : allocs) (void)hipFree(p);
+    if (sc_host) (void)hipHostFree(sc_host);
+    if (st_host) (void)hipHostFree(st_host);
+    if (s) (void)hipStreamDestroy(s);
+  }
+  template <typename T>
+  T* alloc(int64_t n) {
+    void* p = nullptr;
+    if (hipMalloc(&p, sizeof(T) * (size_t)std::max<int64_t>(n, 1)) != hipSuccess) return nullptr;
+    allocs.push_back(p);
+    return (T*)p;
+  }
+};
+
+#define HIPCHK(ctx, call)                                                            \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess) {                                                          \
+      (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);                \
+      return EQ_HIP_ERR;                                                             \
+    }                                                                                \
+  } while (0)
+
+#define NEED(ptr)                                                 \
+  do {                                                            \
+    if (!(ptr)) {                                                 \
+      ctx->err = "device allocation failed";                      \
+      return EQ_NOMEM;                                            \
+    }                                                             \
+  } while (0)
+
+static int fail(eq_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+static int sync_check(eq_ctx* ctx) {
+  HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, hipStreamSynchronize(ctx->s));
+  return EQ_OK;
+}
+
+template <typename T>
+static int h2d(eq_ctx* ctx, T* dst, const T* src, int64_t n) {
+  HIPCHK(ctx, hipMemcpyAsync(dst, src, sizeof(T) * (size_t)n, hipMemcpyHostToDevice, ctx->s));
+  return EQ_OK;
+}
+template <typename T>
+static int d2h(eq_ctx* ctx, T* dst, const T* src, int64_t n) {
+  HIPCHK(ctx, hipMemcpyAsync(dst, src, sizeof(T) * (size_t)n, hipMemcpyDeviceToHost, ctx->s));
+  return EQ_OK;
+}
+
+// reference-ordered host vector -> structured device vector
+static int upload_ref(eq_ctx* ctx, const double* ref, double* dev) {
+  std::vector<double> tmp(ctx->N);
+  for (int64_t k = 0; k < ctx->N; k++) tmp[ctx->perm[k]] = ref[k];
+  if (int rc = h2d(ctx, dev, tmp.data(), ctx->N)) return rc;
+  return sync_check(ctx);
+}
+static int download_ref(eq_ctx* ctx, const double* dev, double* ref) {
+  std::vector<double> tmp(ctx->N);
+  if (int rc = d2h(ctx, tmp.data(), dev, ctx->N)) return rc;
+  if (int rc = sync_check(ctx)) return rc;
+  for (int64_t k = 0; k < ctx->N; k++) ref[k] = tmp[ctx->perm[k]];
+  return EQ_OK;
+}
+// FltNI-ordered host vector (length nI) <-> structured device vector (fault row 0)
+static int upload_ni(eq_ctx* ctx, const double* x, double* dev) {
+  std::vector<double> tmp(ctx->N, 0.0);
+  for (int64_t k = 0; k < ctx->nI; k++) tmp[ctx->fltni[k]] = x[k];
+  if (int rc = h2d(ctx, dev, tmp.data(), ctx->N)) return rc;
+  return sync_check(ctx);
+}
+static int download_ni(eq_ctx* ctx, const double* dev, double* x) {
+  std::vector<double> tmp(ctx->N);
+  if (int rc = d2h(ctx, tmp.data(), dev, ctx->N)) return rc;
+  if (int rc = sync_check(ctx)) return rc;
+  for (int64_t k = 0; k < ctx->nI; k++) x[k] = tmp[ctx->fltni[k]];
+  return EQ_OK;
+}
+
+// ===========================================================================
+// context
+// ===========================================================================
+extern "C" int eq_create(const eq_mesh_desc* mesh, eq_ctx** out) {
+  if (!out) return EQ_EINVAL;
+  *out = nullptr;
+  if (!mesh || !mesh->iglob || !mesh->xgll || !mesh->wgll || !mesh->H) return EQ_EINVAL;
+  std::unique_ptr<eq_ctx> ctx(new eq_ctx());
+  if (mesh->ngll != 5) { *out = ctx.release(); return fail(*out, EQ_EINVAL, "only NGLL = 5 is supported"); }
+  if (mesh->nelx < 1 || mesh->nely < 1 || !(mesh->dxe > 0) || !(mesh->dye > 0)) {
+    *out = ctx.release();
+    return fail(*out, EQ_EINVAL, "bad mesh sizes");
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    *out = ctx.release();
+    return fail(*out, EQ_HIP_ERR, "no HIP device visible");
+  }
+  ctx->NX = (int)(4 * mesh->nelx + 1);
+  ctx->NY = (int)(4 * mesh->nely + 1);
+  ctx->N = (int64_t)ctx->NX * ctx->NY;
+  ctx->nI = ctx->N - ctx->NX;
+  ctx->Nel = mesh->nelx * mesh->nely;
+  Geom& g = ctx->g;
+  g.NX = ctx->NX;
+  g.NY = ctx->NY;
+  g.nelx = (int)mesh->nelx;
+  g.nely = (int)mesh->nely;
+  const double dx_dxi = 0.5 * mesh->dxe, dy_deta = 0.5 * mesh->dye, jac = dx_dxi * dy_deta;
+  g.c1 = jac / (dx_dxi * dx_dxi);
+  g.c2 = jac / (dy_deta * dy_deta);
+  for (int p = 0; p < 5; p++)
+    for (int i = 0; i < 5; i++) g.D[p * 5 + i] = mesh->H[p * 5 + i];   // H[i + 5 p] = h'_i(x_p)
+  for (int p = 0; p < 5; p++) g.w[p] = mesh->wgll[p];
+  // numbering: iglob[i,j,e] -> node (4 ex + i, 4 ey + j)
+  ctx->perm.assign(ctx->N, -1);
+  std::vector<char> seen(ctx->N, 0);
+  for (int64_t e = 0; e < ctx->Nel; e++) {
+    const int64_t ex = e % mesh->nelx, ey = e / mesh->nelx;
+    for (int j = 0; j < 5; j++)
+      for (int i = 0; i < 5; i++) {
+        const int64_t ref = mesh->iglob[e * 25 + j * 5 + i] - 1;
+        const int64_t n = (4 * ey + j) * ctx->NX + 4 * ex + i;
+        if (ref < 0 || ref >= ctx->N) { *out = ctx.release(); return fail(*out, EQ_EINVAL, "iglob out of range"); }
+        if (ctx->perm[ref] == -1) {
+          ctx->perm[ref] = n;
+          if (seen[n]) { *out = ctx.release(); return fail(*out, EQ_EINVAL, "iglob is not a box mesh"); }
+          seen[n] = 1;
+        } else if (ctx->perm[ref] != n) {
+          *out = ctx.release();
+          return fail(*out, EQ_EINVAL, "iglob is not a structured box mesh (MeshBox!)");
+        }
+      }
+  }
+  for (int64_t k = 0; k < ctx->N; k++)
+    if (ctx->perm[k] < 0) { *out = ctx.release(); return fail(*out, EQ_EINVAL, "iglob does not cover all nodes"); }
+  // FltNI: non-fault nodes in increasing reference index (par.jl:209)
+  ctx->fltni.reserve(ctx->nI);
+  for (int64_t k = 0; k < ctx->N; k++)
+    if (ctx->perm[k] >= ctx->NX) ctx->fltni.push_back(ctx->perm[k]);
+  if (hipStreamCreate(&ctx->s) != hipSuccess) { *out = ctx.release(); return fail(*out, EQ_HIP_ERR, "stream"); }
+  eq_ctx* c = ctx.get();
+  const int64_t N = c->N;
+  c->npart = std::max(tile_count(g), vec_blocks(N)) + 1;
+  c->part_rr = c->alloc<double>(c->npart);
+  c->part_bb = c->alloc<double>(c->npart);
+  c->part_rho = c->alloc<double>(c->npart);
+  c->part_uq = c->alloc<double>(c->npart);
+  c->sc = c->alloc<CgScal>(1);
+  c->st = c->alloc<StepStatus>(1);
+  c->flags = c->alloc<int>(4);
+  if (!c->part_rr || !c->part_bb || !c->part_rho || !c->part_uq || !c->sc || !c->st || !c->flags) {
+    *out = ctx.release();
+    return fail(*out, EQ_NOMEM, "device allocation failed");
+  }
+  if (hipHostMalloc((void**)&c->sc_host, sizeof(CgScal), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&c->st_host, sizeof(StepStatus), hipHostMallocDefault) != hipSuccess) {
+    *out = ctx.release();
+    return fail(*out, EQ_NOMEM, "pinned allocation failed");
+  }
+  *out = ctx.release();
+  return EQ_OK;
+}
+
+extern "C" void eq_destroy(eq_ctx* ctx) { delete ctx; }
+
+extern "C" const char* eq_last_error(const eq_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+extern "C" int eq_sizes(const eq_ctx* ctx, int64_t* nglob, int64_t* nI, int64_t* nfault) {
+  if (!ctx) return EQ_EINVAL;
+  if (nglob) *nglob = ctx->N;
+  if (nI) *nI = ctx->nI;
+  if (nfault) *nfault = ctx->NX;
+  return EQ_OK;
+}
+
+template <typename T>
+static int alloc_work(eq_ctx* ctx, CgWork<T>& w) {
+  if (w.r) return EQ_OK;
+  const int64_t N = ctx->N;
+  w.r = ctx->alloc<T>(N); w.c = ctx->alloc<T>(N); w.u[0] = ctx->alloc<T>(N); w.u[1] = ctx->alloc<T>(N);
+  w.q = ctx->alloc<T>(N); w.x0 = ctx->alloc<T>(N); w.res0 = ctx->alloc<T>(N); w.b = ctx->alloc<T>(N);
+  w.x = ctx->alloc<T>(N);
+  if (!w.r || !w.c || !w.u[0] || !w.u[1] || !w.q || !w.x0 || !w.res0 || !w.b || !w.x)
+    return fail(ctx, EQ_NOMEM, "device allocation failed");
+  launch_fill<T>(N, w.x0, T(0), ctx->s);
+  launch_fill<T>(N, w.res0, T(0), ctx->s);
+  return EQ_OK;
+}
+
+extern "C" int eq_set_material(eq_ctx* ctx, const double* W) {
+  if (!ctx || !W) return EQ_EINVAL;
+  const int NX = ctx->NX;
+  std::vector<double> mu(ctx->N, NAN);
+  const double* w = ctx->g.w;
+  for (int64_t e = 0; e < ctx->Nel; e++) {
+    const int64_t ex = e % ctx->g.nelx, ey = e / ctx->g.nelx;
+    for (int j = 0; j < 5; j++)
+      for (int i = 0; i < 5; i++) {
+        const int64_t n = (4 * ey + j) * NX + 4 * ex + i;
+        const double m = W[e * 25 + j * 5 + i] / (w[i] * w[j]);
+        if (std::isnan(mu[n])) mu[n] = m;
+        else if (std::fabs(mu[n] - m) > 1e-12 * std::fabs(m))
+          return fail(ctx, EQ_EINVAL, "W is not wgll2 .* nodal mu (rigidity jumps across an element edge)");
+      }
+  }
+  ctx->mu_host = mu;
+  if (!ctx->mu) {
+    ctx->mu = ctx->alloc<double>(ctx->N); ctx->mu32 = ctx->alloc<float>(ctx->N);
+    ctx->dinv = ctx->alloc<double>(ctx->N); ctx->dinv32 = ctx->alloc<float>(ctx->N);
+    ctx->ones = ctx->alloc<double>(ctx->N); ctx->ones32 = ctx->alloc<float>(ctx->N);
+    NEED(ctx->mu); NEED(ctx->mu32); NEED(ctx->dinv); NEED(ctx->dinv32); NEED(ctx->ones); NEED(ctx->ones32);
+  }
+  if (int rc = h2d(ctx, ctx->mu, mu.data(), ctx->N)) return rc;
+  launch_convert<double, float>(ctx->N, ctx->mu, ctx->mu32, ctx->s);
+  // Jacobi diagonal of K_II, fault row zeroed
+  launch_diag(ctx->g, ctx->mu, ctx->dinv, ctx->s);
+  std::vector<double> dg(ctx->N);
+  if (int rc = d2h(ctx, dg.data(), ctx->dinv, ctx->N)) return rc;
+  if (int rc = sync_check(ctx)) return rc;
+  for (int64_t n = 0; n < ctx->N; n++) dg[n] = (n < NX) ? 0.0 : 1.0 / dg[n];
+  if (int rc = h2d(ctx, ctx->dinv, dg.data(), ctx->N)) return rc;
+  launch_convert<double, float>(ctx->N, ctx->dinv, ctx->dinv32, ctx->s);
+  for (int64_t n = 0; n < ctx->N; n++) dg[n] = (n < NX) ? 0.0 : 1.0;
+  if (int rc = h2d(ctx, ctx->ones, dg.data(), ctx->N)) return rc;
+  launch_convert<double, float>(ctx->N, ctx->ones, ctx->ones32, ctx->s);
+  if (int rc = alloc_work(ctx, ctx->w64)) return rc;
+  if (int rc = alloc_work(ctx, ctx->w32)) return rc;
+  ctx->have_material = true;
+  ctx->have_amg = false;
+  return sync_check(ctx);
+}
+
+static FaultState alloc_fault_state(eq_ctx* ctx, int64_t n) {
+  FaultState f{};
+  double* blk = ctx->alloc<double>(11 * n);
+  if (!blk) return f;
+  f.psi = blk; f.psi1 = blk + n; f.psi2 = blk + 2 * n; f.Vf = blk + 3 * n; f.Vf0 = blk + 4 * n;
+  f.Vf1 = blk + 5 * n; f.Vf2 = blk + 6 * n; f.tau = blk + 7 * n; f.tau1 = blk + 8 * n; f.tau2 = blk + 9 * n;
+  f.FltVfree = blk + 10 * n;
+  launch_fill<double>(11 * n, blk, 0.0, ctx->s);
+  return f;
+}
+
+extern "C" int eq_set_params(eq_ctx* ctx, const eq_params_desc* p) {
+  if (!ctx || !p) return EQ_EINVAL;
+  const int NX = ctx->NX, NY = ctx->NY;
+  if (p->FltNglob != NX || p->nglob != ctx->N) return fail(ctx, EQ_EINVAL, "params do not match the mesh");
+  const double* need[] = {p->fo, p->Vo, p->xLf, p->M, p->FltB, p->FltZ, p->FltX, p->cca, p->ccb, p->Seff, p->tauo,
+                          p->XiLf};
+  for (const double* q : need)
+    if (!q) return fail(ctx, EQ_EINVAL, "missing params array");
+  if (!p->iFlt) return fail(ctx, EQ_EINVAL, "missing iFlt");
+  for (int64_t j = 0; j < NX; j++)
+    if (p->iFlt[j] < 1 || p->iFlt[j] > ctx->N || ctx->perm[p->iFlt[j] - 1] != j)
+      return fail(ctx, EQ_EINVAL, "iFlt is not the y = 0 edge in depth order (BoundaryMatrix! side 'B')");
+  if (p->n_iBcL != NY || (p->n_iBcL && !p->iBcL) || p->n_iBcT != NX || (p->n_iBcT && !p->iBcT) || !p->BcLC ||
+      !p->BcTC)
+    return fail(ctx, EQ_EINVAL, "absorbing edges must be the full 'L' and 'T' sides");
+  for (int64_t k = 0; k < NY; k++)
+    if (p->iBcL[k] < 1 || p->iBcL[k] > ctx->N || ctx->perm[p->iBcL[k] - 1] != k * NX)
+      return fail(ctx, EQ_EINVAL, "iBcL is not side 'L'");
+  for (int64_t k = 0; k < NX; k++)
+    if (p->iBcT[k] < 1 || p->iBcT[k] > ctx->N || ctx->perm[p->iBcT[k] - 1] != (int64_t)(NY - 1) * NX + k)
+      return fail(ctx, EQ_EINVAL, "iBcT is not side 'T'");
+  // creeping nodes: FltIglobBC must be the leading (deepest) fault nodes
+  std::vector<char> creep(NX, 0);
+  for (int64_t k = 0; k < p->n_FltIglobBC; k++) {
+    const int64_t r = p->FltIglobBC[k] - 1;
+    if (r < 0 || r >= ctx->N || ctx->perm[r] >= NX) return fail(ctx, EQ_EINVAL, "FltIglobBC must be fault nodes");
+    creep[ctx->perm[r]] = 1;
+  }
+  int64_t ncreep = 0;
+  while (ncreep < NX && creep[ncreep]) ncreep++;
+  for (int64_t j = ncreep; j < NX; j++)
+    if (creep[j]) return fail(ctx, EQ_EINVAL, "FltIglobBC must be the deepest fault nodes");
+  ctx->FltNglob = NX;
+  ctx->NFBC = ncreep + 1;
+  ctx->IDstate = p->IDstate;
+  ctx->Vpl = p->Vpl; ctx->Vthres = p->Vthres; ctx->Vevne = p->Vevne; ctx->dt0 = p->dt0;
+  ctx->FltX.assign(p->FltX, p->FltX + NX);
+  ctx->tauo.assign(p->tauo, p->tauo + NX);
+  ctx->out_seis.clear();
+  for (int64_t k = 0; k < p->n_out_seis; k++) {
+    const int64_t r = p->out_seis[k] - 1;
+    if (r < 0 || r >= ctx->N) return fail(ctx, EQ_EINVAL, "out_seis out of range");
+    ctx->out_seis.push_back(ctx->perm[r]);
+  }
+  // device copies
+  double* blk = ctx->alloc<double>(11 * (int64_t)NX + NY + NX);
+  NEED(blk);
+  const double* src[11] = {p->fo, p->Vo, p->xLf, p->cca, p->ccb, p->Seff, p->tauo, p->XiLf, p->FltB, p->FltZ, nullptr};
+  std::vector<double> Mf(NX);
+  std::vector<double> Ms(ctx->N);
+  for (int64_t k = 0; k < ctx->N; k++) Ms[ctx->perm[k]] = p->M[k];
+  for (int j = 0; j < NX; j++) Mf[j] = Ms[j];
+  src[10] = Mf.data();
+  for (int k = 0; k < 11; k++)
+    if (int rc = h2d(ctx, blk + (int64_t)k * NX, src[k], NX)) return rc;
+  if (int rc = h2d(ctx, blk + 11 * (int64_t)NX, p->BcLC, NY)) return rc;
+  if (int rc = h2d(ctx, blk + 11 * (int64_t)NX + NY, p->BcTC, NX)) return rc;
+  FaultDev& f = ctx->fp;
+  f.fo = blk; f.Vo = blk + NX; f.xLf = blk + 2 * NX; f.cca = blk + 3 * NX; f.ccb = blk + 4 * NX;
+  f.Seff = blk + 5 * NX; f.tauo = blk + 6 * NX; f.XiLf = blk + 7 * NX; f.FltB = blk + 8 * NX;
+  f.FltZ = blk + 9 * NX; f.Mf = blk + 10 * NX; f.BcLC = blk + 11 * NX; f.BcTC = blk + 11 * NX + NY;
+  f.j0 = (int)(ctx->NFBC - 1);
+  f.nflt = NX;
+  f.IDstate = (int)p->IDstate;
+  f.Vpl = p->Vpl;
+  if (!ctx->M) {
+    ctx->M = ctx->alloc<double>(ctx->N);
+    NEED(ctx->M);
+  }
+  if (int rc = h2d(ctx, ctx->M, Ms.data(), ctx->N)) return rc;
+  if (!ctx->fs.psi) {
+    ctx->fs = alloc_fault_state(ctx, NX);
+    ctx->fs_api = alloc_fault_state(ctx, NX);
+    if (!ctx->fs.psi || !ctx->fs_api.psi) return fail(ctx, EQ_NOMEM, "device allocation failed");
+  }
+  ctx->have_params = true;
+  return sync_check(ctx);
+}
+
+// ===========================================================================
+// operators
+// ===========================================================================
+extern "C" int eq_ku_apply(eq_ctx* ctx, const double* u, double* y, int mask, int prec) {
+  if (!ctx || !u || !y) return EQ_EINVAL;
+  if (!ctx->have_material) return fail(ctx, EQ_ENOTREADY, "eq_set_material first");
+  if (mask < EQ_MASK_ALL || mask > EQ_MASK_NEG) return fail(ctx, EQ_EINVAL, "bad mask");
+  const int interior = mask == EQ_MASK_INTERIOR, neg = mask == EQ_MASK_NEG;
+  double* du = ctx->w64.x;
+  double* dy = ctx->w64.q;
+  if (int rc = upload_ref(ctx, u, du)) return rc;
+  if (prec == EQ_PREC_FP32) {
+    launch_convert<double, float>(ctx->N, du, ctx->w32.x, ctx->s);
+    launch_ku<float>(ctx->g, ctx->w32.x, ctx->mu32, ctx->w32.q, interior, neg, ctx->s);
+    launch_convert<float, double>(ctx->N, ctx->w32.q, dy, ctx->s);
+  } else {
+    launch_ku<double>(ctx->g, du, ctx->mu, dy, interior, neg, ctx->s);
+  }
+  if (int rc = sync_check(ctx)) return rc;
+  return download_ref(ctx, dy, y);
+}
+
+extern "C" int eq_kni_apply(eq_ctx* ctx, const double* x, double* y, int prec) {
+  if (!ctx || !x || !y) return EQ_EINVAL;
+  if (!ctx->have_material) return fail(ctx, EQ_ENOTREADY, "eq_set_material first");
+  double* dx = ctx->w64.x;
+  double* dy = ctx->w64.q;
+  if (int rc = upload_ni(ctx, x, dx)) return rc;
+  if (prec == EQ_PREC_FP32) {
+    launch_convert<double, float>(ctx->N, dx, ctx->w32.x, ctx->s);
+    launch_ku<float>(ctx->g, ctx->w32.x, ctx->mu32, ctx->w32.q, 1, 1, ctx->s);
+    launch_convert<float, double>(ctx->N, ctx->w32.q, dy, ctx->s);
+  } else {
+    launch_ku<double>(ctx->g, dx, ctx->mu, dy, 1, 1, ctx->s);
+  }
+  if (int rc = sync_check(ctx)) return rc;
+  return download_ni(ctx, dy, y);
+}
+
+extern "C" int eq_element_computation(eq_ctx* ctx, const double* u, double* a_elem) {
+  if (!ctx || !u || !a_elem) return EQ_EINVAL;
+  if (!ctx->have_material) return fail(ctx, EQ_ENOTREADY, "eq_set_material first");
+  double* du = ctx->w64.x;
+  if (int rc = upload_ref(ctx, u, du)) return rc;
+  double* ae = ctx->alloc<double>(25 * ctx->Nel);
+  NEED(ae);
+  launch_element_ku(ctx->g, du, ctx->mu, ae, ctx->s);
+  int rc = d2h(ctx, a_elem, ae, 25 * ctx->Nel);
+  if (!rc) rc = sync_check(ctx);
+  ctx->allocs.pop_back();
+  (void)hipFree(ae);
+  return rc;
+}
+
+// ===========================================================================
+// AMG
+// ===========================================================================
+static int upload_csr(eq_ctx* ctx, const HostCsr& H, DevCsr* D) {
+  D->n = H.n;
+  D->m = H.m;
+  D->nnz = H.nnz();
+  if (D->nnz > INT_MAX) return fail(ctx, EQ_EINVAL, "level too large for 32-bit CSR offsets");
+  std::vector<int32_t> rp(H.n + 1);
+  for (int64_t i = 0; i <= H.n; i++) rp[i] = (int32_t)H.rowptr[i];
+  D->rowptr = ctx->alloc<int32_t>(H.n + 1);
+  D->col = ctx->alloc<int32_t>(D->nnz);
+  D->val = ctx->alloc<double>(D->nnz);
+  D->val32 = ctx->alloc<float>(D->nnz);
+  NEED(D->rowptr); NEED(D->col); NEED(D->val); NEED(D->val32);
+  if (int rc = h2d(ctx, D->rowptr, rp.data(), H.n + 1)) return rc;
+  if (D->nnz) {
+    if (int rc = h2d(ctx, D->col, H.col.data(), D->nnz)) return rc;
+    if (int rc = h2d(ctx, D->val, H.val.data(), D->nnz)) return rc;
+    launch_convert<double, float>(D->nnz, D->val, D->val32, ctx->s);
+  }
+  const double avg = H.n ? (double)D->nnz / (double)H.n : 1.0;
+  D->lpr = avg <= 2.5 ? 1 : avg <= 5 ? 2 : avg <= 10 ? 4 : avg <= 24 ? 8 : 16;
+  return EQ_OK;
+}
+
+template <typename T>
+static int alloc_level_vecs(eq_ctx* ctx, DevLevelVecs<T>& v, int64_t n) {
+  v.b = ctx->alloc<T>(n); v.x = ctx->alloc<T>(n); v.xp = ctx->alloc<T>(n); v.res = ctx->alloc<T>(n);
+  NEED(v.b); NEED(v.x); NEED(v.xp); NEED(v.res);
+  return EQ_OK;
+}
+
+extern "C" int eq_amg_setup(eq_ctx* ctx, double theta, int max_levels, int64_t max_coarse, int strength_abs,
+                            int smoother, int smooth_degree) {
+  if (!ctx) return EQ_EINVAL;
+  if (!ctx->have_material) return fail(ctx, EQ_ENOTREADY, "eq_set_material first");
+  if (smoother != EQ_SMOOTH_JACOBI) return fail(ctx, EQ_EINVAL, "only the Jacobi smoother is available");
+  (void)smooth_degree;
+  HostCsr Kii;
+  assemble_kii(ctx->g, ctx->mu_host.data(), &Kii);
+  AmgOptions opt;
+  opt.theta = theta;
+  opt.max_levels = max_levels;
+  opt.max_coarse = max_coarse;
+  opt.strength_abs = strength_abs;
+  HostHierarchy H;
+  std::string err;
+  if (int rc = amg_build(Kii, opt, &H, &err)) return fail(ctx, rc, err);
+  if (H.lev.size() < 2) return fail(ctx, EQ_EINVAL, "AMG hierarchy has a single level; use the Jacobi preconditioner");
+  ctx->lev.clear();
+  ctx->lev.resize(H.lev.size());
+  ctx->amg_n.clear();
+  ctx->amg_nnz.clear();
+  for (size_t l = 0; l < H.lev.size(); l++) {
+    DevLevel& D = ctx->lev[l];
+    const HostLevel& L = H.lev[l];
+    D.n = L.A.n;
+    ctx->amg_n.push_back(L.A.n);
+    ctx->amg_nnz.push_back(L.A.nnz());
+    D.omega = 4.0 / (3.0 * L.lambda_max);
+    if (l > 0 && l + 1 < H.lev.size()) {
+      if (int rc = upload_csr(ctx, L.A, &D.A)) return rc;
+      D.dinv = ctx->alloc<double>(D.n);
+      D.dinv32 = ctx->alloc<float>(D.n);
+      NEED(D.dinv); NEED(D.dinv32);
+      if (int rc = h2d(ctx, D.dinv, L.dinv.data(), D.n)) return rc;
+      launch_convert<double, float>(D.n, D.dinv, D.dinv32, ctx->s);
+    }
+    if (l + 1 < H.lev.size()) {
+      if (int rc = upload_csr(ctx, L.P, &D.P)) return rc;
+      if (int rc = upload_csr(ctx, L.R, &D.R)) return rc;
+    }
+    if (l > 0) {
+      if (int rc = alloc_level_vecs(ctx, D.v64, D.n)) return rc;
+      if (int rc = alloc_level_vecs(ctx, D.v32, D.n)) return rc;
+    }
+  }
+  // level 0: matrix-free operator, Jacobi weight from the assembled K_II
+  ctx->ncoarse = H.ncoarse;
+  ctx->coarse_inv = ctx->alloc<double>(H.ncoarse * H.ncoarse);
+  NEED(ctx->coarse_inv);
+  if (int rc = h2d(ctx, ctx->coarse_inv, H.coarse_inv.data(), H.ncoarse * H.ncoarse)) return rc;
+  ctx->have_amg = true;
+  return sync_check(ctx);
+}
+
+extern "C" int eq_amg_info(const eq_ctx* ctx, int* nlev, int64_t* n, int64_t* nnz, int cap) {
+  if (!ctx || !nlev) return EQ_EINVAL;
+  *nlev = (int)ctx->amg_n.size();
+  for (int l = 0; l < *nlev && l < cap; l++) {
+    if (n) n[l] = ctx->amg_n[l];
+    if (nnz) nnz[l] = ctx->amg_nnz[l];
+  }
+  return EQ_OK;
+}
+
+template <typename T>
+static DevLevelVecs<T>& lvecs(DevLevel& L);
+template <>
+DevLevelVecs<double>& lvecs<double>(DevLevel& L) { return L.v64; }
+template <>
+DevLevelVecs<float>& lvecs<float>(DevLevel& L) { return L.v32; }
+template <typename T>
+static const T* ldinv(const DevLevel& L);
+template <>
+const double* ldinv<double>(const DevLevel& L) { return L.dinv; }
+template <>
+const float* ldinv<float>(const DevLevel& L) { return L.dinv32; }
+template <typename T>
+static CgWork<T>& work(eq_ctx* c);
+template <>
+CgWork<double>& work<double>(eq_ctx* c) { return c->w64; }
+template <>
+CgWork<float>& work<float>(eq_ctx* c) { return c->w32; }
+template <typename T>
+static const T* mu_of(eq_ctx* c);
+template <>
+const double* mu_of<double>(eq_ctx* c) { return c->mu; }
+template <>
+const float* mu_of<float>(eq_ctx* c) { return c->mu32; }
+template <typename T>
+static const T* dinv_of(eq_ctx* c);
+template <>
+const double* dinv_of<double>(eq_ctx* c) { return c->dinv; }
+template <>
+const float* dinv_of<float>(eq_ctx* c) { return c->dinv32; }
+template <typename T>
+static const T* ones_of(eq_ctx* c);
+template <>
+const double* ones_of<double>(eq_ctx* c) { return c->ones; }
+template <>
+const float* ones_of<float>(eq_ctx* c) { return c->ones32; }
+
+// One V-cycle: c = M^-1 r with partial c.r (and the cg! convergence test in
+// its first kernel).
+template <typename T>
+static void vcycle(eq_ctx* ctx, const T* r, T* c) {
+  CgWork<T>& w = work<T>(ctx);
+  const Geom& g = ctx->g;
+  hipStream_t s = ctx->s;
+  const int L = (int)ctx->lev.size();
+  DevLevel& L0 = ctx->lev[0];
+  const T om0 = (T)L0.omega;
+  launch_l0_down<T>(g, r, mu_of<T>(ctx), dinv_of<T>(ctx), om0, w.x0, w.res0, ctx->part_rr, ctx->npart, ctx->sc, s);
+  // restrict to level 1
+  launch_csr<T>(CSR_SPMV, L0.R, w.res0 + g.NX, nullptr, nullptr, T(0), lvecs<T>(ctx->lev[1]).b, nullptr, ctx->sc, s);
+  for (int l = 1; l <= L - 2; l++) {
+    DevLevel& D = ctx->lev[l];
+    DevLevelVecs<T>& v = lvecs<T>(D);
+    launch_csr<T>(CSR_DOWN, D.A, nullptr, v.b, ldinv<T>(D), (T)D.omega, v.res, v.x, ctx->sc, s);
+    launch_csr<T>(CSR_SPMV, D.R, v.res, nullptr, nullptr, T(0), lvecs<T>(ctx->lev[l + 1]).b, nullptr, ctx->sc, s);
+  }
+  DevLevelVecs<T>& vc = lvecs<T>(ctx->lev[L - 1]);
+  launch_dense<T>(ctx->ncoarse, ctx->coarse_inv, vc.b, vc.xp, ctx->sc, s);
+  for (int l = L - 2; l >= 1; l--) {
+    DevLevel& D = ctx->lev[l];
+    DevLevelVecs<T>& v = lvecs<T>(D);
+    launch_csr<T>(CSR_PROLONG_ADD, D.P, lvecs<T>(ctx->lev[l + 1]).xp, nullptr, nullptr, T(0), v.x, nullptr,
+                  ctx->sc, s);
+    launch_csr<T>(CSR_POST, D.A, v.x, v.b, ldinv<T>(D), (T)D.omega, v.xp, nullptr, ctx->sc, s);
+  }
+  launch_csr<T>(CSR_PROLONG_ADD, L0.P, lvecs<T>(ctx->lev[1]).xp, nullptr, nullptr, T(0), w.x0 + g.NX, nullptr,
+                ctx->sc, s);
+  launch_l0_post<T>(g, w.x0, r, mu_of<T>(ctx), dinv_of<T>(ctx), om0, c, ctx->part_rho, ctx->sc, s);
+}
+
+// ===========================================================================
+// CG in K_II form on device vectors (fault row ignored); x in/out, b input.
+// ===========================================================================
+template <typename T>
+static void cg_iteration(eq_ctx* ctx, T* x, int pc) {
+  CgWork<T>& w = work<T>(ctx);
+  const Geom& g = ctx->g;
+  hipStream_t s = ctx->s;
+  const int nt = tile_count(g);
+  int nrho;
+  if (pc == EQ_PC_AMG) {
+    vcycle<T>(ctx, w.r, w.c);
+    nrho = nt;
+  } else {
+    const T* dv = (pc == EQ_PC_JACOBI) ? dinv_of<T>(ctx) : ones_of<T>(ctx);
+    launch_jacobi_pc<T>(ctx->N, dv, w.r, w.c, ctx->part_rr, ctx->npart, ctx->sc, ctx->part_rho, s);
+    nrho = vec_blocks(ctx->N);
+  }
+  launch_cg_dir<T>(g, w.c, w.u[w.cur], mu_of<T>(ctx), w.u[1 - w.cur], w.q, ctx->part_rho, nrho, ctx->sc,
+                   ctx->part_uq, s);
+  launch_cg_update<T>(ctx->N, x, w.r, w.u[1 - w.cur], w.q, ctx->part_rho, nrho, ctx->part_uq, nt, ctx->sc,
+                      ctx->part_rr, ctx->npart, s);
+  w.cur = 1 - w.cur;
+}
+
+template <typename T>
+static int cg_device(eq_ctx* ctx, T* x, const T* b, double tol, int64_t maxiter, int pc, int64_t* iters,
+                     double* relres) {
+  CgWork<T>& w = work<T>(ctx);
+  const Geom& g = ctx->g;
+  hipStream_t s = ctx->s;
+  if (pc == EQ_PC_AMG && !ctx->have_amg) return fail(ctx, EQ_ENOTREADY, "eq_amg_setup first");
+  if (maxiter <= 0) maxiter = ctx->nI;
+  if (maxiter > INT_MAX) maxiter = INT_MAX;
+  launch_fill<double>(ctx->npart, ctx->part_rr, 0.0, s);
+  launch_fill<double>(ctx->npart, ctx->part_bb, 0.0, s);
+  launch_cg_residual<T>(g, x, mu_of<T>(ctx), b, w.r, ctx->part_rr, ctx->part_bb, s);
+  launch_cg_init_scal(ctx->sc, ctx->part_rr, ctx->part_bb, ctx->npart, tol, (int)maxiter, s);
+  launch_fill<T>(ctx->N, w.u[w.cur], T(0), s);
+  int64_t batch = std::max<int64_t>(1, ctx->last_iters - 1);
+  for (;;) {
+    for (int64_t k = 0; k < batch; k++) cg_iteration<T>(ctx, x, pc);
+    launch_cg_check(ctx->sc, ctx->part_rr, ctx->npart, s);
+    HIPCHK(ctx, hipMemcpyAsync(ctx->sc_host, ctx->sc, sizeof(CgScal), hipMemcpyDeviceToHost, s));
+    if (int rc = sync_check(ctx)) return rc;
+    if (ctx->sc_host->done) break;
+    batch = std::min<int64_t>(std::max<int64_t>(2, batch / 4), 16);
+  }
+  const CgScal& h = *ctx->sc_host;
+  if (iters) *iters = h.iter;
+  if (relres) *relres = h.bnorm > 0 ? h.resid / h.bnorm : h.resid;
+  ctx->last_iters = std::max<int64_t>(1, h.iter);
+  return (h.resid <= h.reltol) ? EQ_OK : EQ_PCG_NOCONV;
+}
+
+// Mixed precision: FP64 residual / update around FP32 inner PCG.
+static int cg_mixed(eq_ctx* ctx, double* x, const double* b, double tol, int64_t maxiter, int pc, int64_t* iters,
+                    double* relres) {
+  CgWork<double>& w = ctx->w64;
+  CgWork<float>& f = ctx->w32;
+  const Geom& g = ctx->g;
+  hipStream_t s = ctx->s;
+  if (maxiter <= 0) maxiter = ctx->nI;
+  int64_t total = 0;
+  double bnorm = 0, resid = 0;
+  const double inner_tol = 1e-3;
+  for (int outer = 0; outer < 50; outer++) {
+    launch_fill<double>(ctx->npart, ctx->part_rr, 0.0, s);
+    launch_fill<double>(ctx->npart, ctx->part_bb, 0.0, s);
+    launch_cg_residual<double>(g, x, ctx->mu, b, w.r, ctx->part_rr, ctx->part_bb, s);
+    launch_cg_init_scal(ctx->sc, ctx->part_rr, ctx->part_bb, ctx->npart, tol, 1, s);
+    HIPCHK(ctx, hipMemcpyAsync(ctx->sc_host, ctx->sc, sizeof(CgScal), hipMemcpyDeviceToHost, s));
+    if (int rc = sync_check(ctx)) return rc;
+    bnorm = ctx->sc_host->bnorm;
+    resid = ctx->sc_host->resid;
+    if (resid <= tol * bnorm || total >= maxiter) break;
+    launch_convert<double, float>(ctx->N, w.r, f.b, s);
+    launch_fill<float>(ctx->N, f.x, 0.0f, s);
+    int64_t it = 0;
+    double rr = 0;
+    int rc = cg_device<float>(ctx, f.x, f.b, inner_tol, maxiter - total, pc, &it, &rr);
+    if (rc != EQ_OK && rc != EQ_PCG_NOCONV) return rc;
+    total += it;
+    // x += e (FP32 correction, fault row of e is zero)
+    launch_convert<float, double>(ctx->N, f.x, w.q, s);
+    launch_axpy(ctx->N, 1.0, w.q, x, s);
+  }
+  if (iters) *iters = total;
+  if (relres) *relres = bnorm > 0 ? resid / bnorm : resid;
+  ctx->last_iters = std::max<int64_t>(1, total);
+  return resid <= tol * bnorm ? EQ_OK : EQ_PCG_NOCONV;
+}
+
+static int cg_any(eq_ctx* ctx, double* x, const double* b, double tol, int64_t maxiter, int pc, int prec,
+                  int64_t* iters, double* relres) {
+  if (prec == EQ_PREC_MIXED) return cg_mixed(ctx, x, b, tol, maxiter, pc, iters, relres);
+  if (prec == EQ_PREC_FP32) {
+    launch_convert<double, float>(ctx->N, x, ctx->w32.x, ctx->s);
+    launch_convert<double, float>(ctx->N, b, ctx->w32.b, ctx->s);
+    int rc = cg_device<float>(ctx, ctx->w32.x, ctx->w32.b, tol, maxiter, pc, iters, relres);
+    launch_convert<float, double>(ctx->N, ctx->w32.x, x, ctx->s);
+    int rc2 = sync_check(ctx);
+    return rc ? rc : rc2;
+  }
+  return cg_device<double>(ctx, x, b, tol, maxiter, pc, iters, relres);
+}
+
+extern "C" int eq_cg(eq_ctx* ctx, double* x, const double* b, double tol, int64_t maxiter, int pc, int prec,
+                     int64_t* iters, double* relres) {
+  if (!ctx || !x || !b) return EQ_EINVAL;
+  if (!ctx->have_material) return fail(ctx, EQ_ENOTREADY, "eq_set_material first");
+  if (pc < EQ_PC_NONE || pc > EQ_PC_AMG || prec < EQ_PREC_FP64 || prec > EQ_PREC_MIXED)
+    return fail(ctx, EQ_EINVAL, "bad pc / prec");
+  // cg!(x, kni, b): kni = -K_II, so solve K_II x = -b (identical CG iterates)
+  double* dx = ctx->w64.x;
+  double* db = ctx->w64.b;
+  if (int rc = upload_ni(ctx, x, dx)) return rc;
+  if (int rc = upload_ni(ctx, b, db)) return rc;
+  launch_scale_mask(ctx->N, ctx->NX, db, -1.0, db, ctx->s);
+  int rc = cg_any(ctx, dx, db, tol, maxiter, pc, prec, iters, relres);
+  if (rc != EQ_OK && rc != EQ_PCG_NOCONV) return rc;
+  if (int rc2 = download_ni(ctx, dx, x)) return rc2;
+  if (rc == EQ_PCG_NOCONV) ctx->err = "cg did not reach the tolerance within maxiter";
+  return rc;
+}
+
+extern "C" int eq_amg_ldiv(eq_ctx* ctx, const double* b, double* x, int prec) {
+  if (!ctx || !b || !x) return EQ_EINVAL;
+  if (!ctx->have_amg) return fail(ctx, EQ_ENOTREADY, "eq_amg_setup first");
+  // x = (V-cycle of kni) b = -(V-cycle of K_II) b
+  double* db = ctx->w64.b;
+  if (int rc = upload_ni(ctx, b, db)) return rc;
+  launch_scale_mask(ctx->N, ctx->NX, db, -1.0, db, ctx->s);
+  // make the convergence test pass: reltol = -1, iter 0 < maxiter
+  CgScal h{};
+  h.rho_prev = 1.0; h.reltol = -1.0; h.maxiter = INT_MAX; h.done = 0; h.iter = 0;
+  HIPCHK(ctx, hipMemcpyAsync(ctx->sc, &h, sizeof(CgScal), hipMemcpyHostToDevice, ctx->s));
+  launch_fill<double>(ctx->npart, ctx->part_rr, 1.0, ctx->s);
+  if (prec == EQ_PREC_FP32) {
+    launch_convert<double, float>(ctx->N, db, ctx->w32.r, ctx->s);
+    vcycle<float>(ctx, ctx->w32.r, ctx->w32.c);
+    launch_convert<float, double>(ctx->N, ctx->w32.c, ctx->w64.c, ctx->s);
+  } else {
+    vcycle<double>(ctx, db, ctx->w64.c);
+  }
+  if (int rc = sync_check(ctx)) return rc;
+  return download_ni(ctx, ctx->w64.c, x);
+}
+
+extern "C" int eq_pcg_jacobi(eq_ctx* ctx, const double* F, double* dnew, double tol, int64_t maxit, int64_t* iters) {
+  if (!ctx || !F || !dnew) return EQ_EINVAL;
+  if (!ctx->have_material) return fail(ctx, EQ_ENOTREADY, "eq_set_material first");
+  // Fnew = -(K F)[FltNI]
+  double* dF = ctx->w64.x;
+  if (int rc = upload_ref(ctx, F, dF)) return rc;
+  launch_ku<double>(ctx->g, dF, ctx->mu, ctx->w64.b, 0, 1, ctx->s);
+  launch_scale_mask(ctx->N, ctx->NX, ctx->w64.b, 1.0, ctx->w64.b, ctx->s);
+  if (int rc = upload_ni(ctx, dnew, ctx->w64.x)) return rc;
+  double rr = 0;
+  int rc = cg_device<double>(ctx, ctx->w64.x, ctx->w64.b, tol, maxit, EQ_PC_JACOBI, iters, &rr);
+  if (rc != EQ_OK && rc != EQ_PCG_NOCONV) return rc;
+  if (int rc2 = download_ni(ctx, ctx->w64.x, dnew)) return rc2;
+  if (rc == EQ_PCG_NOCONV) ctx->err = "PCG did not converge";
+  return rc;
+}
+
+// ===========================================================================
+// fault physics entry points
+// ===========================================================================
+namespace {
+struct DevTmp {
+  std::vector<void*> p;
+  ~DevTmp() {
+    for (void* q : p) (void)hipFree(q);
+  }
+  template <typename T>
+  T* get(int64_t n) {
+    void* q = nullptr;
+    if (hipMalloc(&q, sizeof(T) * (size_t)std::max<int64_t>(n, 1)) != hipSuccess) return nullptr;
+    p.push_back(q);
+    return (T*)q;
+  }
+};
+}  // namespace
+
+#define UP(dst, src, n)                                         \
+  do {                                                          \
+    if (!(dst)) return fail(ctx, EQ_NOMEM, "device alloc");     \
+    if (int rc_ = h2d(ctx, dst, src, n)) return rc_;            \
+  } while (0)
+
+extern "C" int eq_ids(eq_ctx* ctx, int64_t n, const double* xLf, const double* Vo, const double* psi, double dt,
+                      const double* Vf, double cnd, int64_t IDstate, double* psi1) {
+  if (!ctx || n < 0 || !xLf || !Vo || !psi || !Vf || !psi1) return EQ_EINVAL;
+  if (IDstate != 1 && IDstate != 2) return fail(ctx, EQ_EINVAL, "IDstate must be 1 or 2");
+  DevTmp t;
+  double *a = t.get<double>(n), *b = t.get<double>(n), *c = t.get<double>(n), *d = t.get<double>(n),
+         *o = t.get<double>(n);
+  UP(a, xLf, n); UP(b, Vo, n); UP(c, psi, n); UP(d, Vf, n);
+  if (!o) return fail(ctx, EQ_NOMEM, "device alloc");
+  launch_ids(n, a, b, c, dt, d, cnd, (int)IDstate, o, ctx->s);
+  if (int rc = d2h(ctx, psi1, o, n)) return rc;
+  return sync_check(ctx);
+}
+
+extern "C" int eq_ids2(eq_ctx* ctx, int64_t n, const double* xLf, const double* Vo, const double* psi,
+                       const double* psi1, double dt, const double* Vf, const double* Vf1, int64_t IDstate,
+                       double* psi2) {
+  if (!ctx || n < 0 || !xLf || !Vo || !psi || !psi1 || !Vf || !Vf1 || !psi2) return EQ_EINVAL;
+  if (IDstate != 1 && IDstate != 2) return fail(ctx, EQ_EINVAL, "IDstate must be 1 or 2");
+  DevTmp t;
+  double *a = t.get<double>(n), *b = t.get<double>(n), *c = t.get<double>(n), *d = t.get<double>(n),
+         *e = t.get<double>(n), *f = t.get<double>(n), *o = t.get<double>(n);
+  UP(a, xLf, n); UP(b, Vo, n); UP(c, psi, n); UP(d, psi1, n); UP(e, Vf, n); UP(f, Vf1, n);
+  if (!o) return fail(ctx, EQ_NOMEM, "device alloc");
+  launch_ids2(n, a, b, c, d, dt, e, f, (int)IDstate, o, ctx->s);
+  if (int rc = d2h(ctx, psi2, o, n)) return rc;
+  return sync_check(ctx);
+}
+
+extern "C" int eq_nrsearch(eq_ctx* ctx, int64_t n, const double* fo, const double* Vo, const double* cca,
+                           const double* ccb, const double* Seff, const double* tau, const double* tauo,
+                           const double* psi, const double* FltZ, const double* FltVfree, double* Vf_out,
+                           double* tau_out, int32_t* k_out) {
+  if (!ctx || n < 0 || !fo || !Vo || !cca || !ccb || !Seff || !tau || !tauo || !psi || !FltZ || !FltVfree ||
+      !Vf_out || !tau_out)
+    return EQ_EINVAL;
+  DevTmp t;
+  const double* in[10] = {fo, Vo, cca, ccb, Seff, tau, tauo, psi, FltZ, FltVfree};
+  double* din[10];
+  for (int k = 0; k < 10; k++) {
+    din[k] = t.get<double>(n);
+    UP(din[k], in[k], n);
+  }
+  double *ov = t.get<double>(n), *ot = t.get<double>(n);
+  int* ok = t.get<int>(n);
+  if (!ov || !ot || !ok) return fail(ctx, EQ_NOMEM, "device alloc");
+  launch_nrsearch(n, din[0], din[1], din[2], din[3], din[4], din[5], din[6], din[7], din[8], din[9], ov, ot, ok,
+                  ctx->s);
+  if (int rc = d2h(ctx, Vf_out, ov, n)) return rc;
+  if (int rc = d2h(ctx, tau_out, ot, n)) return rc;
+  std::vector<int> kk(n);
+  if (int rc = d2h(ctx, kk.data(), ok, n)) return rc;
+  if (int rc = sync_check(ctx)) return rc;
+  bool noconv = false;
+  for (int64_t i = 0; i < n; i++) {
+    if (k_out) k_out[i] = kk[i];
+    if (kk[i] < 0) noconv = true;
+  }
+  if (noconv) {
+    ctx->err = "NRsearch! hit |delta| > 1e10 or k == 1000 (last iterate returned)";
+    return EQ_NR_NOCONV;
+  }
+  return EQ_OK;
+}
+
+static int check_fault_call(eq_ctx* ctx, int64_t NFBC, int64_t FltNglob) {
+  if (!ctx->have_params) return fail(ctx, EQ_ENOTREADY, "eq_set_params first");
+  if (FltNglob != ctx->FltNglob || NFBC < 1 || NFBC > FltNglob + 1)
+    return fail(ctx, EQ_EINVAL, "NFBC / FltNglob do not match the parameters");
+  return EQ_OK;
+}
+
+extern "C" int eq_fbc(eq_ctx* ctx, int64_t IDstate, int64_t NFBC, int64_t FltNglob, double* psi1, double* Vf1,
+                      double* tau1, double* psi2, double* Vf2, double* tau2, const double* psi, const double* Vf,
+                      const double* FltVfree, double dt, int64_t* fail_node) {
+  if (!ctx || !psi1 || !Vf1 || !tau1 || !psi2 || !Vf2 || !tau2 || !psi || !Vf || !FltVfree) return EQ_EINVAL;
+  if (int rc = check_fault_call(ctx, NFBC, FltNglob)) return rc;
+  const int64_t n = FltNglob;
+  FaultState& f = ctx->fs_api;
+  if (int rc = h2d(ctx, f.psi1, psi1, n)) return rc;
+  if (int rc = h2d(ctx, f.Vf1, Vf1, n)) return rc;
+  if (int rc = h2d(ctx, f.tau1, tau1, n)) return rc;
+  if (int rc = h2d(ctx, f.psi2, psi2, n)) return rc;
+  if (int rc = h2d(ctx, f.Vf2, Vf2, n)) return rc;
+  if (int rc = h2d(ctx, f.tau2, tau2, n)) return rc;
+  if (int rc = h2d(ctx, f.psi, psi, n)) return rc;
+  if (int rc = h2d(ctx, f.Vf, Vf, n)) return rc;
+  if (int rc = h2d(ctx, f.FltVfree, FltVfree, n)) return rc;
+  FaultDev fp = ctx->fp;
+  fp.j0 = (int)(NFBC - 1);
+  fp.IDstate = (int)IDstate;
+  int hf[2] = {INT_MAX, 0};
+  if (int rc = h2d(ctx, ctx->flags, hf, 2)) return rc;
+  launch_fbc(fp, f, f.FltVfree, dt, ctx->flags, ctx->s);
+  if (int rc = d2h(ctx, psi1, f.psi1, n)) return rc;
+  if (int rc = d2h(ctx, Vf1, f.Vf1, n)) return rc;
+  if (int rc = d2h(ctx, tau1, f.tau1, n)) return rc;
+  if (int rc = d2h(ctx, psi2, f.psi2, n)) return rc;
+  if (int rc = d2h(ctx, Vf2, f.Vf2, n)) return rc;
+  if (int rc = d2h(ctx, tau2, f.tau2, n)) return rc;
+  if (int rc = d2h(ctx, hf, ctx->flags, 2)) return rc;
+  if (int rc = sync_check(ctx)) return rc;
+  if (fail_node) *fail_node = (hf[0] == INT_MAX) ? 0 : hf[0];
+  if (hf[0] != INT_MAX) {
+    char buf[128];
+    snprintf(buf, sizeof buf, "NR SEARCH FAILED! (fault location %d)", hf[0]);
+    ctx->err = buf;
+    return EQ_NR_NAN;
+  }
+  if (hf[1] > 0) {
+    ctx->err = "NRsearch! hit |delta| > 1e10 or k == 1000 (last iterate returned)";
+    return EQ_NR_NOCONV;
+  }
+  return EQ_OK;
+}
+
+extern "C" int eq_slr(eq_ctx* ctx, int64_t NFBC, int64_t FltNglob, const double* psi, double* psi1, const double* Vf,
+                      double* Vf1, int64_t IDstate, const double* tau1, double dt) {
+  if (!ctx || !psi || !psi1 || !Vf || !Vf1 || !tau1) return EQ_EINVAL;
+  if (int rc = check_fault_call(ctx, NFBC, FltNglob)) return rc;
+  const int64_t n = FltNglob;
+  FaultState& f = ctx->fs_api;
+  if (int rc = h2d(ctx, f.psi, psi, n)) return rc;
+  if (int rc = h2d(ctx, f.psi1, psi1, n)) return rc;
+  if (int rc = h2d(ctx, f.Vf, Vf, n)) return rc;
+  if (int rc = h2d(ctx, f.Vf1, Vf1, n)) return rc;
+  if (int rc = h2d(ctx, f.tau1, tau1, n)) return rc;
+  FaultDev fp = ctx->fp;
+  fp.j0 = (int)(NFBC - 1);
+  fp.IDstate = (int)IDstate;
+  launch_slr(fp, f, dt, ctx->s);
+  if (int rc = d2h(ctx, psi1, f.psi1, n)) return rc;
+  if (int rc = d2h(ctx, Vf1, f.Vf1, n)) return rc;
+  return sync_check(ctx);
+}
+
+extern "C" int eq_dtevol(eq_ctx* ctx, double dt, double dtmin, const double* XiLf, int64_t FaultNglob, int64_t NFBC,
+                         const double* Vf, int64_t isolver, double* dt_out) {
+  if (!ctx || !XiLf || !Vf || !dt_out || FaultNglob < 0 || NFBC < 1) return EQ_EINVAL;
+  DevTmp t;
+  double *x = t.get<double>(FaultNglob), *v = t.get<double>(FaultNglob), *o = t.get<double>(1);
+  UP(x, XiLf, FaultNglob);
+  UP(v, Vf, FaultNglob);
+  if (!o) return fail(ctx, EQ_NOMEM, "device alloc");
+  launch_dtevol(x, v, (int)(NFBC - 1), (int)FaultNglob, dt, dtmin, (int)isolver, o, ctx->s);
+  if (int rc = d2h(ctx, dt_out, o, 1)) return rc;
+  return sync_check(ctx);
+}
+
+// ===========================================================================
+// time loop
+// ===========================================================================
+extern "C" int eq_set_solver(eq_ctx* ctx, const eq_solver_opts* o) {
+  if (!ctx || !o) return EQ_EINVAL;
+  if (o->pc < EQ_PC_NONE || o->pc > EQ_PC_AMG) return fail(ctx, EQ_EINVAL, "bad pc");
+  if (o->prec != EQ_PREC_FP64 && o->prec != EQ_PREC_MIXED && o->prec != EQ_PREC_FP32)
+    return fail(ctx, EQ_EINVAL, "bad prec");
+  if (o->guess != EQ_GUESS_PREVIOUS) return fail(ctx, EQ_EINVAL, "learned guess is not configured");
+  ctx->opts = *o;
+  return EQ_OK;
+}
+
+static int alloc_state(eq_ctx* ctx) {
+  if (ctx->d) return EQ_OK;
+  const int64_t N = ctx->N;
+  double* blk = ctx->alloc<double>(8 * N);
+  NEED(blk);
+  ctx->d = blk; ctx->v = blk + N; ctx->a = blk + 2 * N; ctx->d2 = blk + 3 * N; ctx->v2 = blk + 4 * N;
+  ctx->a2 = blk + 5 * N; ctx->dPre = blk + 6 * N; ctx->bq = blk + 7 * N;
+  launch_fill<double>(8 * N, blk, 0.0, ctx->s);
+  return EQ_OK;
+}
+
+extern "C" int eq_state_init(eq_ctx* ctx, double v_init, const double* psi_init) {
+  if (!ctx) return EQ_EINVAL;
+  if (!ctx->have_material || !ctx->have_params) return fail(ctx, EQ_ENOTREADY, "material and params first");
+  if (int rc = alloc_state(ctx)) return rc;
+  const int64_t N = ctx->N, NX = ctx->NX;
+  std::vector<double> v(N, v_init - 0.5 * ctx->Vpl);
+  const int j0 = ctx->fp.j0;
+  for (int j = 0; j < j0; j++) v[j] = 0.0;
+  std::vector<double> hpsi(NX), hVf(NX);
+  if (psi_init) {
+    hpsi.assign(psi_init, psi_init + NX);
+  } else {
+    std::vector<double> fo(NX), Vo(NX), cca(NX), ccb(NX), Seff(NX);
+    if (int rc = d2h(ctx, fo.data(), ctx->fp.fo, NX)) return rc;
+    if (int rc = d2h(ctx, Vo.data(), ctx->fp.Vo, NX)) return rc;
+    if (int rc = d2h(ctx, cca.data(), ctx->fp.cca, NX)) return rc;
+    if (int rc = d2h(ctx, ccb.data(), ctx->fp.ccb, NX)) return rc;
+    if (int rc = d2h(ctx, Seff.data(), ctx->fp.Seff, NX)) return rc;
+    if (int rc = sync_check(ctx)) return rc;
+    for (int j = 0; j < NX; j++)
+      hpsi[j] = ctx->tauo[j] / (Seff[j] * ccb[j]) - fo[j] / ccb[j] - (cca[j] / ccb[j]) * std::log(2 * v_init / Vo[j]);
+  }
+  for (int j = 0; j < NX; j++) hVf[j] = (j < j0) ? 0.0 : 2 * v[j];
+  launch_fill<double>(6 * N, ctx->d, 0.0, ctx->s);   // d, v, a and their second buffers
+  launch_fill<double>(2 * N, ctx->dPre, 0.0, ctx->s);
+  launch_fill<double>(11 * NX, ctx->fs.psi, 0.0, ctx->s);
+  if (int rc = h2d(ctx, ctx->v, v.data(), N)) return rc;
+  if (int rc = h2d(ctx, ctx->fs.psi, hpsi.data(), NX)) return rc;
+  if (int rc = h2d(ctx, ctx->fs.Vf, hVf.data(), NX)) return rc;
+  ctx->t = 0;
+  ctx->dt = ctx->dt0;
+  ctx->it = 0;
+  ctx->isolver = 1;
+  ctx->nr_noconv_total = 0;
+  ctx->pcg_noconv = 0;
+  ctx->have_state = true;
+  return sync_check(ctx);
+}
+
+extern "C" int eq_state_get(eq_ctx* ctx, double* d, double* v, double* a, double* psi, double* tau, double* Vf,
+                            double* t, double* dt, int64_t* it, int64_t* isolver) {
+  if (!ctx) return EQ_EINVAL;
+  if (!ctx->have_state) return fail(ctx, EQ_ENOTREADY, "eq_state_init first");
+  if (d) if (int rc = download_ref(ctx, ctx->d, d)) return rc;
+  if (v) if (int rc = download_ref(ctx, ctx->v, v)) return rc;
+  if (a) if (int rc = download_ref(ctx, ctx->a, a)) return rc;
+  if (psi) if (int rc = d2h(ctx, psi, ctx->fs.psi, ctx->NX)) return rc;
+  if (tau) if (int rc = d2h(ctx, tau, ctx->fs.tau, ctx->NX)) return rc;
+  if (Vf) if (int rc = d2h(ctx, Vf, ctx->fs.Vf, ctx->NX)) return rc;
+  if (t) *t = ctx->t;
+  if (dt) *dt = ctx->dt;
+  if (it) *it = ctx->it;
+  if (isolver) *isolver = ctx->isolver;
+  return sync_check(ctx);
+}
+
+extern "C" int eq_state_set(eq_ctx* ctx, const double* d, const double* v, const double* a, const double* psi,
+                            const double* tau, double t, double dt, int64_t isolver) {
+  if (!ctx) return EQ_EINVAL;
+  if (!ctx->have_state) return fail(ctx, EQ_ENOTREADY, "eq_state_init first");
+  if (d) if (int rc = upload_ref(ctx, d, ctx->d)) return rc;
+  if (v) if (int rc = upload_ref(ctx, v, ctx->v)) return rc;
+  if (a) if (int rc = upload_ref(ctx, a, ctx->a)) return rc;
+  if (psi) if (int rc = h2d(ctx, ctx->fs.psi, psi, ctx->NX)) return rc;
+  if (tau) if (int rc = h2d(ctx, ctx->fs.tau, tau, ctx->NX)) return rc;
+  ctx->t = t;
+  ctx->dt = dt;
+  ctx->isolver = (int)isolver;
+  return sync_check(ctx);
+}
+
+static int read_status(eq_ctx* ctx) {
+  HIPCHK(ctx, hipMemcpyAsync(ctx->st_host, ctx->st, sizeof(StepStatus), hipMemcpyDeviceToHost, ctx->s));
+  return sync_check(ctx);
+}
+
+static int qs_step(eq_ctx* ctx, eq_step_info* info) {
+  const Geom& g = ctx->g;
+  hipStream_t s = ctx->s;
+  const double dt = ctx->dt;
+  for (int pass = 0; pass < 2; pass++) {
+    launch_qs_begin(g, ctx->fp, ctx->fs, ctx->mu, ctx->d, ctx->dPre, ctx->v, ctx->bq, dt, pass == 0, s);
+    int64_t it = 0;
+    double rr = 0;
+    int rc = cg_any(ctx, ctx->d, ctx->bq, ctx->opts.tol, ctx->opts.maxiter, ctx->opts.pc, ctx->opts.prec, &it, &rr);
+    if (rc == EQ_PCG_NOCONV) ctx->pcg_noconv++;
+    else if (rc) return rc;
+    info->pcg_iters[pass] = it;
+    launch_qs_fault(g, ctx->fp, ctx->fs, ctx->mu, ctx->d, ctx->v, dt, s);
+  }
+  launch_qs_finish(g, ctx->fp, ctx->fs, ctx->d, ctx->dPre, ctx->v, ctx->a, dt, s);
+  return EQ_OK;
+}
+
+static int dyn_step(eq_ctx* ctx) {
+  const double dtmin = ctx->dt0;
+  launch_newmark(ctx->g, ctx->fp, ctx->fs, ctx->mu, ctx->M, ctx->d, ctx->v, ctx->a, ctx->d2, ctx->v2, ctx->a2,
+                 ctx->dt, 0.5 * dtmin, 0.5 * dtmin * dtmin, ctx->flags, ctx->s);
+  std::swap(ctx->d, ctx->d2);
+  std::swap(ctx->v, ctx->v2);
+  std::swap(ctx->a, ctx->a2);
+  return EQ_OK;
+}
+
+extern "C" int eq_step(eq_ctx* ctx, eq_step_info* info) {
+  if (!ctx || !info) return EQ_EINVAL;
+  if (!ctx->have_state) return fail(ctx, EQ_ENOTREADY, "eq_state_init first");
+  if (ctx->opts.pc == EQ_PC_AMG && !ctx->have_amg) return fail(ctx, EQ_ENOTREADY, "eq_amg_setup first");
+  std::memset(info, 0, sizeof(*info));
+  ctx->it += 1;
+  ctx->t += ctx->dt;
+  info->dt_used = ctx->dt;
+  info->isolver_used = ctx->isolver;
+  launch_fill<int>(1, ctx->flags, INT_MAX, ctx->s);
+  launch_fill<int>(1, ctx->flags + 1, 0, ctx->s);
+  int rc = (ctx->isolver == 1) ? qs_step(ctx, info) : dyn_step(ctx);
+  if (rc) return rc;
+  launch_step_tail(ctx->fp, ctx->v, ctx->dt, ctx->dt0, ctx->isolver, ctx->st, ctx->flags, ctx->s);
+  if (int rc2 = read_status(ctx)) return rc2;
+  const StepStatus& st = *ctx->st_host;
+  info->t = ctx->t;
+  info->it = ctx->it;
+  info->Vfmax = st.Vfmax;
+  info->dt_next = st.dt_next;
+  info->isolver_next = st.isolver_next;
+  info->nr_noconv = st.nr_noconv;
+  ctx->nr_noconv_total += st.nr_noconv;
+  ctx->isolver = st.isolver_next;
+  ctx->dt = st.dt_next;
+  if (st.nr_flag) {
+    char buf[128];
+    snprintf(buf, sizeof buf, "NR SEARCH FAILED! (fault location %d)", st.nr_flag);
+    ctx->err = buf;
+    return EQ_NR_NAN;
+  }
+  if (!std::isfinite(st.Vfmax)) return fail(ctx, EQ_NR_NAN, "non-finite slip rate");
+  return EQ_OK;
+}
+
+// results bookkeeping (src/main.jl:361-436)
+extern "C" int eq_run(eq_ctx* ctx, double total_time, int64_t max_steps, eq_results* R) {
+  if (!ctx || !R) return EQ_EINVAL;
+  if (!ctx->have_state) return fail(ctx, EQ_ENOTREADY, "eq_state_init first");
+  const int64_t NX = ctx->NX;
+  const double Vpl = ctx->Vpl;
+  const double yr2sec = 365.0 * 24 * 60 * 60;
+  double tvsx = 5 * yr2sec;
+  const double tvsxinc = tvsx;
+  const double tevneinc = 0.5;
+  double tevneb = 0, tevne = 0;
+  int idelevne = 0, slipstart = 0;
+  std::vector<double> dF(NX), vF(NX), tau(NX), delfref(NX, 0.0);
+  const size_t nseis = ctx->out_seis.size();
+  R->n_steps = R->n_start = R->n_end = R->n_is = R->n_seis = R->n_rec = 0;
+  R->n_qs = R->n_dyn = R->total_pcg_iters = R->nr_noconv = 0;
+  R->wall_qs = R->wall_dyn = 0;
+  auto fault_fields = [&]() -> int {
+    if (int rc = d2h(ctx, dF.data(), ctx->d, NX)) return rc;
+    if (int rc = d2h(ctx, vF.data(), ctx->v, NX)) return rc;
+    if (int rc = d2h(ctx, tau.data(), ctx->fs.tau, NX)) return rc;
+    return sync_check(ctx);
+  };
+  using clk = std::chrono::steady_clock;
+  while (ctx->t < total_time && (max_steps <= 0 || R->n_steps < max_steps)) {
+    eq_step_info info;
+    const auto t0 = clk::now();
+    int rc = eq_step(ctx, &info);
+    const double el = std::chrono::duration<double>(clk::now() - t0).count();
+    if (info.isolver_used == 1) { R->n_qs++; R->wall_qs += el; } else { R->n_dyn++; R->wall_dyn += el; }
+    if (rc) return rc;
+    const int64_t k = R->n_steps;
+    if (k < R->cap_steps) {
+      if (R->time_) R->time_[k] = info.t;
+      if (R->Vfmax) R->Vfmax[k] = info.Vfmax;
+      if (R->isolver) R->isolver[k] = info.isolver_used;
+      if (R->pcg_iters) { R->pcg_iters[2 * k] = info.pcg_iters[0]; R->pcg_iters[2 * k + 1] = info.pcg_iters[1]; }
+    }
+    R->n_steps++;
+    R->total_pcg_iters += info.pcg_iters[0] + info.pcg_iters[1];
+    R->nr_noconv += info.nr_noconv;
+    const double t = info.t, Vfmax = info.Vfmax;
+    const bool need_start = Vfmax > 1.01 * ctx->Vthres && slipstart == 0;
+    const bool need_end = Vfmax < 0.99 * ctx->Vthres && slipstart == 1;
+    const bool need_is = t > tvsx;
+    bool need_seis = false;
+    if (Vfmax > ctx->Vevne) {
+      if (idelevne == 0) need_seis = true;
+      else if ((t - tevneb) > tevne) need_seis = true;
+    }
+    if (need_start || need_end || need_is || need_seis)
+      if (int rc2 = fault_fields()) return rc2;
+    auto put = [&](double* M, int64_t cap, int64_t col, const std::vector<double>& src) {
+      if (M && col < cap) std::memcpy(M + col * NX, src.data(), sizeof(double) * NX);
+    };
+    std::vector<double> tmp(NX);
+    if (need_start) {
+      for (int64_t j = 0; j < NX; j++) delfref[j] = 2 * dF[j] + Vpl * t;
+      slipstart = 1;
+      const int64_t e = R->n_start;
+      if (e < R->cap_events) {
+        if (R->tStart) R->tStart[e] = t;
+        for (int64_t j = 0; j < NX; j++) tmp[j] = (tau[j] + ctx->tauo[j]) / 1e6;
+        put(R->taubefore, R->cap_events, e, tmp);
+        if (R->hypo) R->hypo[e] = ctx->FltX[ctx->st_host->hypo];
+      }
+      R->n_start++;
+    }
+    if (need_end) {
+      const int64_t e = R->n_end;
+      if (e < R->cap_events) {
+        for (int64_t j = 0; j < NX; j++) tmp[j] = 2 * dF[j] + Vpl * t - delfref[j];
+        put(R->delfafter, R->cap_events, e, tmp);
+        for (int64_t j = 0; j < NX; j++) tmp[j] = (tau[j] + ctx->tauo[j]) / 1e6;
+        put(R->tauafter, R->cap_events, e, tmp);
+        if (R->tEnd) R->tEnd[e] = t;
+      }
+      R->n_end++;
+      slipstart = 0;
+    }
+    if (need_is) {
+      const int64_t e = R->n_is;
+      if (e < R->cap_is) {
+        for (int64_t j = 0; j < NX; j++) tmp[j] = 2 * dF[j] + Vpl * t;
+        put(R->is_slip, R->cap_is, e, tmp);
+        for (int64_t j = 0; j < NX; j++) tmp[j] = 2 * vF[j] + Vpl;
+        put(R->is_slipvel, R->cap_is, e, tmp);
+        for (int64_t j = 0; j < NX; j++) tmp[j] = (tau[j] + ctx->tauo[j]) / 1e6;
+        put(R->is_stress, R->cap_is, e, tmp);
+      }
+      R->n_is++;
+      tvsx += tvsxinc;
+    }
+    if (Vfmax > ctx->Vevne) {
+      if (idelevne == 0) {
+        idelevne = 1;
+        tevneb = t;
+        tevne = tevneinc;
+      } else if (need_seis) {
+        tevne += tevneinc;
+      }
+      if (need_seis) {
+        const int64_t e = R->n_seis;
+        if (e < R->cap_seis) {
+          for (int64_t j = 0; j < NX; j++) tmp[j] = 2 * dF[j] + Vpl * t;
+          put(R->seismic_slip, R->cap_seis, e, tmp);
+          for (int64_t j = 0; j < NX; j++) tmp[j] = 2 * vF[j] + Vpl;
+          put(R->seismic_slipvel, R->cap_seis, e, tmp);
+          for (int64_t j = 0; j < NX; j++) tmp[j] = (tau[j] + ctx->tauo[j]) / 1e6;
+          put(R->seismic_stress, R->cap_seis, e, tmp);
+        }
+        R->n_seis++;
+      }
+    } else {
+      idelevne = 0;
+    }
+    if (ctx->it % 10 == 0 && nseis > 0) {
+      const int64_t e = R->n_rec;
+      if (e < R->cap_rec) {
+        for (size_t q = 0; q < nseis; q++) {
+          double vals[3];
+          const int64_t n = ctx->out_seis[q];
+          if (int rc2 = d2h(ctx, &vals[0], ctx->d + n, 1)) return rc2;
+          if (int rc2 = d2h(ctx, &vals[1], ctx->v + n, 1)) return rc2;
+          if (int rc2 = d2h(ctx, &vals[2], ctx->a + n, 1)) return rc2;
+          if (int rc2 = sync_check(ctx)) return rc2;
+          if (R->dSeis) R->dSeis[q * R->cap_rec + e] = vals[0];
+          if (R->vSeis) R->vSeis[q * R->cap_rec + e] = vals[1];
+          if (R->aSeis) R->aSeis[q * R->cap_rec + e] = vals[2];
+        }
+      }
+      R->n_rec++;
+    }
+  }
+  return EQ_OK;
+}
+
+// ===========================================================================
+// measurement hook
+// ===========================================================================
+template <typename T>
+static void time_body(eq_ctx* ctx, int which) {
+  CgWork<T>& w = work<T>(ctx);
+  const Geom& g = ctx->g;
+  switch (which) {
+    case EQ_TIME_KU:
+      launch_ku<T>(g, w.x, mu_of<T>(ctx), w.q, 1, 0, ctx->s);
+      break;
+    case EQ_TIME_CG_DIR:
+      launch_cg_dir<T>(g, w.c, w.u[0], mu_of<T>(ctx), w.u[1], w.q, ctx->part_rho, tile_count(g), ctx->sc,
+                       ctx->part_uq, ctx->s);
+      break;
+    case EQ_TIME_VCYCLE:
+      vcycle<T>(ctx, w.r, w.c);
+      break;
+    default:
+      cg_iteration<T>(ctx, w.x, ctx->have_amg ? EQ_PC_AMG : EQ_PC_JACOBI);
+      break;
+  }
+}
+
+extern "C" int eq_time_kernel(eq_ctx* ctx, int which, int prec, int reps, double* avg_ms) {
+  if (!ctx || !avg_ms || reps < 1) return EQ_EINVAL;
+  if (!ctx->have_material) return fail(ctx, EQ_ENOTREADY, "eq_set_material first");
+  if (which == EQ_TIME_VCYCLE && !ctx->have_amg) return fail(ctx, EQ_ENOTREADY, "eq_amg_setup first");
+  if (which == EQ_TIME_NEWMARK && (!ctx->have_state || !ctx->have_params))
+    return fail(ctx, EQ_ENOTREADY, "eq_state_init first");
+  hipStream_t s = ctx->s;
+  // well-defined inputs: unit vectors, a CG state that never reports convergence
+  if (prec == EQ_PREC_FP32) {
+    launch_fill<float>(ctx->N, ctx->w32.x, 1.0f, s); launch_fill<float>(ctx->N, ctx->w32.r, 1.0f, s);
+    launch_fill<float>(ctx->N, ctx->w32.c, 1.0f, s); launch_fill<float>(ctx->N, ctx->w32.u[0], 0.0f, s);
+  } else {
+    launch_fill<double>(ctx->N, ctx->w64.x, 1.0, s); launch_fill<double>(ctx->N, ctx->w64.r, 1.0, s);
+    launch_fill<double>(ctx->N, ctx->w64.c, 1.0, s); launch_fill<double>(ctx->N, ctx->w64.u[0], 0.0, s);
+  }
+  CgScal h{};
+  h.rho_prev = 1.0; h.reltol = -1.0; h.maxiter = INT_MAX;
+  HIPCHK(ctx, hipMemcpyAsync(ctx->sc, &h, sizeof(CgScal), hipMemcpyHostToDevice, s));
+  launch_fill<double>(ctx->npart, ctx->part_rr, 1.0, s);
+  launch_fill<double>(ctx->npart, ctx->part_rho, 1.0, s);
+  launch_fill<double>(ctx->npart, ctx->part_uq, 1.0, s);
+  auto body = [&]() {
+    if (which == EQ_TIME_NEWMARK) {
+      launch_newmark(ctx->g, ctx->fp, ctx->fs_api, ctx->mu, ctx->M, ctx->d, ctx->v, ctx->a, ctx->d2, ctx->v2,
+                     ctx->a2, ctx->dt0, 0.5 * ctx->dt0, 0.5 * ctx->dt0 * ctx->dt0, ctx->flags, s);
+    } else if (prec == EQ_PREC_FP32) {
+      time_body<float>(ctx, which);
+    } else {
+      time_body<double>(ctx, which);
+    }
+  };
+  if (which == EQ_TIME_NEWMARK) {
+    // the fault routine works on the API scratch copy of the fault state
+    HIPCHK(ctx, hipMemcpyAsync(ctx->fs_api.psi, ctx->fs.psi, sizeof(double) * 11 * ctx->NX,
+                               hipMemcpyDeviceToDevice, s));
+  }
+  body();  // warm-up
+  hipEvent_t e0, e1;
+  HIPCHK(ctx, hipEventCreate(&e0));
+  HIPCHK(ctx, hipEventCreate(&e1));
+  HIPCHK(ctx, hipEventRecord(e0, s));
+  for (int k = 0; k < reps; k++) body();
+  HIPCHK(ctx, hipEventRecord(e1, s));
+  HIPCHK(ctx, hipEventSynchronize(e1));
+  float ms = 0;
+  HIPCHK(ctx, hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *avg_ms = (double)ms / reps;
+  return sync_check(ctx);
+}

@@ -1,0 +1,64 @@
+// Internal declarations shared by the eqcycle HIP library sources.
+//
+// Device layout: nodal fields live on the structured GLL grid, node (I, J) at
+// n = J * NX + I, with I = 0..4*NelX along dip (x) and J = 0..4*NelY fault
+// normal (y).  The fault is row J = 0 and fault node j (reference order, deep
+// to shallow) is I = j.  The reference's element-by-element numbering
+// (src/MeshBox.jl) is converted once, at the C-ABI boundary.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/eqcycle_hip.h"
+
+namespace eq {
+
+// Geometry and operator coefficients passed by value to every grid kernel.
+struct Geom {
+  int NX, NY;         // nodes per direction
+  int nelx, nely;
+  double c1, c2;      // jac/dx_dxi^2 (x term), jac/dy_deta^2 (y term)
+  double D[25];       // D[p*5+i] = h'_i(xi_p)  (= H[i,p])
+  double w[5];        // GLL weights
+};
+
+// Host-side RS-AMG hierarchy (CSR, int32 columns), built by amg_build().
+struct HostCsr {
+  int64_t n = 0, m = 0;
+  std::vector<int64_t> rowptr;
+  std::vector<int32_t> col;
+  std::vector<double> val;
+  int64_t nnz() const { return rowptr.empty() ? 0 : rowptr[n]; }
+};
+
+struct HostLevel {
+  HostCsr A, P, R;    // R = P^T
+  std::vector<double> dinv;
+  double lambda_max = 0;  // of D^-1 A
+};
+
+struct AmgOptions {
+  double theta = 0.25;
+  int max_levels = 10;
+  int64_t max_coarse = 10;
+  int strength_abs = 1;   // 1: |a_ij| >= theta max|a_ik| (AlgebraicMultigrid.jl Classical)
+  int build_on_negated = 1;  // build from kni = -K_II as the reference does
+};
+
+struct HostHierarchy {
+  std::vector<HostLevel> lev;   // lev[0].A = fine operator
+  std::vector<double> coarse_inv;  // dense inverse of the coarsest A (row-major)
+  int64_t ncoarse = 0;
+};
+
+// amg_host.cpp
+int amg_build(const HostCsr& A, const AmgOptions& opt, HostHierarchy* out, std::string* err);
+void csr_transpose(const HostCsr& A, HostCsr* T);
+double estimate_lambda_max_dinvA(const HostCsr& A, const std::vector<double>& dinv, int iters);
+// K_II in structured interior order (reduced index r = n - NX), from nodal mu
+void assemble_kii(const Geom& g, const double* mu_struct, HostCsr* K);
+
+}  // namespace eq

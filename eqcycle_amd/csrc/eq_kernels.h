@@ -1,0 +1,140 @@
+// Kernel launchers of libeqcycle_hip (device code lives in eq_kernels.hip).
+#pragma once
+#include "eq_internal.h"
+
+namespace eq {
+
+constexpr int BLOCK = 256;
+constexpr int TILE_I = 64;   // owned nodes per tile along x
+constexpr int TILE_J = 16;   // owned nodes per tile along y
+
+// Device scalars of one CG solve (all FP64).
+struct CgScal {
+  double rho_prev;   // rho of the previous iteration (1 before the first)
+  double resid;      // ||r||_2 at the last convergence check
+  double bnorm;      // ||b||_2
+  double reltol;     // tol * ||b||_2
+  int iter;          // completed iterations
+  int done;          // 1 once ||r|| <= reltol or iter >= maxiter
+  int maxiter;
+  int pad;
+};
+
+// Device fault parameters (fault order j = I), FP64.
+struct FaultDev {
+  const double *fo, *Vo, *xLf, *cca, *ccb, *Seff, *tauo, *XiLf, *FltB, *FltZ, *Mf;
+  const double *BcLC, *BcTC;  // [NY], [NX]
+  int j0;       // NFBC - 1: first rate-state node (0-based)
+  int nflt;     // FltNglob = NX
+  int IDstate;
+  double Vpl;
+};
+
+// Persistent fault state arrays (reference main(P) variables).
+struct FaultState {
+  double *psi, *psi1, *psi2, *Vf, *Vf0, *Vf1, *Vf2, *tau, *tau1, *tau2, *FltVfree;
+};
+
+struct StepStatus {   // written by the step tail kernel
+  double Vfmax;
+  double dt_next;
+  int isolver_next;
+  int hypo;          // argmax of the slip rate (0-based fault index)
+  int nr_flag;       // 1 + first failing node of the FBC! guard, or 0
+  int nr_noconv;     // Newton searches that took the failure exit
+};
+
+inline dim3 tile_grid(const Geom& g) {
+  return dim3((g.NX + TILE_I - 1) / TILE_I, (g.NY + TILE_J - 1) / TILE_J);
+}
+inline int tile_count(const Geom& g) {
+  dim3 d = tile_grid(g);
+  return (int)(d.x * d.y);
+}
+inline int vec_blocks(int64_t n) { return (int)((n + BLOCK * 4 - 1) / (BLOCK * 4)); }
+
+// ---- matrix-free operator -----------------------------------------------
+// y = sign * K u on the structured grid; interior != 0 masks the fault row.
+template <typename T>
+void launch_ku(const Geom& g, const T* u, const T* mu, T* y, int interior, int negate, hipStream_t s);
+void launch_diag(const Geom& g, const double* mu, double* diag, hipStream_t s);
+void launch_element_ku(const Geom& g, const double* u, const double* mu, double* a_elem, hipStream_t s);
+
+// ---- CG pieces (K_II form, vectors of length N with zero fault row) ------
+template <typename T>
+void launch_cg_residual(const Geom& g, const T* x, const T* mu, const T* b, T* r, double* part_rr,
+                        double* part_bb, hipStream_t s);
+void launch_cg_init_scal(CgScal* sc, const double* part_rr, const double* part_bb, int nparts, double tol,
+                         int maxiter, hipStream_t s);
+void launch_cg_check(CgScal* sc, const double* part_rr, int nparts, hipStream_t s);
+template <typename T>
+void launch_cg_dir(const Geom& g, const T* c, const T* u_old, const T* mu, T* u_new, T* q, const double* part_rho,
+                   int nrho, const CgScal* sc, double* part_uq, hipStream_t s);
+template <typename T>
+void launch_cg_update(int64_t n, T* x, T* r, const T* u, const T* q, const double* part_rho, int nrho,
+                      const double* part_uq, int nuq, CgScal* sc, double* part_rr, int npart, hipStream_t s);
+template <typename T>
+void launch_jacobi_pc(int64_t n, const T* dinv, const T* r, T* c, const double* part_rr, int nrr, CgScal* sc,
+                      double* part_rho, hipStream_t s);
+
+// ---- AMG V-cycle ----------------------------------------------------------
+template <typename T>
+void launch_l0_down(const Geom& g, const T* r, const T* mu, const T* dinv, T omega, T* x0, T* res,
+                    const double* part_rr, int nrr, CgScal* sc, hipStream_t s);
+template <typename T>
+void launch_l0_post(const Geom& g, const T* x0, const T* r, const T* mu, const T* dinv, T omega, T* c,
+                    double* part_rho, const CgScal* sc, hipStream_t s);
+
+struct DevCsr {
+  int64_t n = 0, m = 0, nnz = 0;
+  int32_t* rowptr = nullptr;
+  int32_t* col = nullptr;
+  double* val = nullptr;
+  float* val32 = nullptr;
+  int lpr = 1;   // lanes per row
+};
+enum CsrMode { CSR_SPMV = 0, CSR_DOWN = 1, CSR_PROLONG_ADD = 2, CSR_POST = 3 };
+// SPMV: y = M x.  DOWN: x = w dinv b, y(res) = b - M x.  PROLONG_ADD: y += M x.
+// POST: y = x + w dinv (b - M x).
+template <typename T>
+void launch_csr(CsrMode mode, const DevCsr& M, const T* x, const T* b, const T* dinv, T omega, T* y, T* xout,
+                const CgScal* sc, hipStream_t s);
+template <typename T>
+void launch_dense(int64_t n, const double* inv, const T* b, T* x, const CgScal* sc, hipStream_t s);
+
+// ---- time stepping --------------------------------------------------------
+void launch_newmark(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, const double* M,
+                    const double* d, const double* v, const double* a, double* d2, double* v2, double* a2,
+                    double dt, double half_dt, double half_dt_sq, int* flags, hipStream_t s);
+void launch_qs_begin(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, double* d, double* dPre,
+                     const double* v, double* b, double dt, int first_pass, hipStream_t s);
+void launch_qs_fault(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, const double* d,
+                     double* v, double dt, hipStream_t s);
+void launch_qs_finish(const Geom& g, const FaultDev& fp, FaultState fs, double* d, const double* dPre, double* v,
+                      double* a, double dt, hipStream_t s);
+void launch_step_tail(const FaultDev& fp, const double* v, double dt, double dtmin, int isolver, StepStatus* st,
+                      const int* flags, hipStream_t s);
+
+// ---- fault routines on their own (API parity entry points) ---------------
+void launch_ids(int64_t n, const double* xLf, const double* Vo, const double* psi, double dt, const double* Vf,
+                double cnd, int IDstate, double* psi1, hipStream_t s);
+void launch_ids2(int64_t n, const double* xLf, const double* Vo, const double* psi, const double* psi1, double dt,
+                 const double* Vf, const double* Vf1, int IDstate, double* psi2, hipStream_t s);
+void launch_nrsearch(int64_t n, const double* fo, const double* Vo, const double* cca, const double* ccb,
+                     const double* Seff, const double* tau, const double* tauo, const double* psi,
+                     const double* FltZ, const double* FltVfree, double* Vf, double* tau_out, int* k,
+                     hipStream_t s);
+void launch_fbc(const FaultDev& fp, FaultState fs, const double* FltVfree, double dt, int* flags, hipStream_t s);
+void launch_slr(const FaultDev& fp, FaultState fs, double dt, hipStream_t s);
+void launch_dtevol(const double* XiLf, const double* Vf, int j0, int n, double dt, double dtmin, int isolver,
+                   double* out, hipStream_t s);
+
+template <typename T, typename U>
+void launch_convert(int64_t n, const T* x, U* y, hipStream_t s);
+template <typename T>
+void launch_fill(int64_t n, T* y, T val, hipStream_t s);
+void launch_axpy(int64_t n, double alpha, const double* x, double* y, hipStream_t s);
+// y = sign * x on N-vectors, zeroing the fault row (first NX entries)
+void launch_scale_mask(int64_t n, int nx, const double* x, double sign, double* y, hipStream_t s);
+
+}  // namespace eq

@@ -1,0 +1,1108 @@
+// gfx950 kernels of libeqcycle_hip.
+//
+// Matrix-free stiffness.  For nodal rigidity mu (W_e[i,j] = w_i w_j mu, the
+// form mat_trap / mat_gauss build, src/MaterialProperties.jl:77-128) the
+// element operator of K_element (src/Kassemble.jl:31-63) summed over the mesh
+// splits into 1-D operators along grid lines:
+//   (K u)(I,J) = sum_{segments s on row J containing I} sum_p D[p][i] gx_p (D u)_p
+//              + sum_{segments t on column I containing J} sum_q D[q][j] gy_q (D u)_q
+// with gx = c1 * wline(J) * w_p * mu, gy = c2 * wline(I) * w_q * mu, where
+// wline(L) sums the GLL weight of line L over the elements containing it.
+// A workgroup owns a 64 x 16 node tile, stages a (64+5) x (16+5) window of
+// the input and of mu in LDS (4-node halo below / left, 1 above / right) and
+// evaluates every owned node by gather, so no atomics are needed and results
+// are deterministic.  The same tile kernel carries fused epilogues: CG
+// direction update + A u + <u, A u>, AMG level-0 Jacobi smoothing + residual,
+// and the whole Newmark step with the fault Newton search.
+//
+// Reductions: every reducing kernel writes one FP64 partial per workgroup; the
+// consumer kernel re-reduces the partials in a fixed order in each workgroup,
+// so scalars never round-trip through the host and results are reproducible.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <climits>
+#include <cmath>
+
+#include "eq_kernels.h"
+
+namespace eq {
+
+// --------------------------------------------------------------------------
+// reductions (blockDim == BLOCK == 256, wave64)
+// --------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+__device__ double block_sum(double v) {
+  __shared__ double sh[BLOCK / 64];
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double r = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  return r;
+}
+
+__device__ double reduce_parts(const double* p, int n) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += BLOCK) s += p[i];
+  return block_sum(s);
+}
+
+// --------------------------------------------------------------------------
+// geometry helpers
+// --------------------------------------------------------------------------
+__device__ __forceinline__ double wline(const Geom& g, int L, int nl) {
+  const int r = L & 3;
+  if (r != 0) return g.w[r];
+  return (L == 0 || L == nl - 1) ? g.w[0] : g.w[0] + g.w[4];
+}
+
+// One 5-node segment: all five outputs r[i] = sum_p D[p][i] w_p m_p sum_k D[p][k] u_k,
+// then select output li.  u and m are strided views (stride ST).
+template <typename T, int ST>
+__device__ __forceinline__ T seg_eval(const Geom& g, const T* u, const T* m, int li) {
+  T t[5];
+#pragma unroll
+  for (int p = 0; p < 5; p++) {
+    T s = 0;
+#pragma unroll
+    for (int k = 0; k < 5; k++) s += (T)g.D[p * 5 + k] * u[k * ST];
+    t[p] = s * ((T)g.w[p] * m[p * ST]);
+  }
+  T r0 = 0, r1 = 0, r2 = 0, r3 = 0, r4 = 0;
+#pragma unroll
+  for (int p = 0; p < 5; p++) {
+    r0 += (T)g.D[p * 5 + 0] * t[p];
+    r1 += (T)g.D[p * 5 + 1] * t[p];
+    r2 += (T)g.D[p * 5 + 2] * t[p];
+    r3 += (T)g.D[p * 5 + 3] * t[p];
+    r4 += (T)g.D[p * 5 + 4] * t[p];
+  }
+  T r = r0;
+  r = (li == 1) ? r1 : r;
+  r = (li == 2) ? r2 : r;
+  r = (li == 3) ? r3 : r;
+  r = (li == 4) ? r4 : r;
+  return r;
+}
+
+// (K u) at node (I, J) from a window with row stride WS whose (lc, lr)
+// entry is node (I, J).
+template <typename T, int WS>
+__device__ __forceinline__ T ku_window(const Geom& g, const T* su, const T* sm, int lc, int lr, int I, int J) {
+  T ax = 0, ay = 0;
+  {
+    const int i = I & 3, s = I >> 2;
+    const int row = lr * WS;
+    if (i != 0 || s < g.nelx)   // segment s, local index i
+      if (s < g.nelx) ax += seg_eval<T, 1>(g, su + row + lc - i, sm + row + lc - i, i);
+    if (i == 0 && s > 0)        // segment s-1, local index 4
+      ax += seg_eval<T, 1>(g, su + row + lc - 4, sm + row + lc - 4, 4);
+  }
+  {
+    const int j = J & 3, t = J >> 2;
+    if (t < g.nely) ay += seg_eval<T, WS>(g, su + (lr - j) * WS + lc, sm + (lr - j) * WS + lc, j);
+    if (j == 0 && t > 0) ay += seg_eval<T, WS>(g, su + (lr - 4) * WS + lc, sm + (lr - 4) * WS + lc, 4);
+  }
+  return (T)(g.c1 * wline(g, J, g.NY)) * ax + (T)(g.c2 * wline(g, I, g.NX)) * ay;
+}
+
+// Same evaluation straight from global memory (used for single fault rows).
+template <typename T>
+__device__ T ku_global(const Geom& g, const T* u, const T* mu, int I, int J) {
+  const int64_t n = (int64_t)J * g.NX + I;
+  T ax = 0, ay = 0;
+  {
+    const int i = I & 3, s = I >> 2;
+    if (s < g.nelx) ax += seg_eval<T, 1>(g, u + n - i, mu + n - i, i);
+    if (i == 0 && s > 0) ax += seg_eval<T, 1>(g, u + n - 4, mu + n - 4, 4);
+  }
+  {
+    const int j = J & 3, t = J >> 2;
+    const int NX = g.NX;
+    if (t < g.nely) {
+      const T* up = u + n - (int64_t)j * NX;
+      const T* mp = mu + n - (int64_t)j * NX;
+      T tt[5];
+#pragma unroll
+      for (int q = 0; q < 5; q++) {
+        T s = 0;
+#pragma unroll
+        for (int l = 0; l < 5; l++) s += (T)g.D[q * 5 + l] * up[(int64_t)l * NX];
+        tt[q] = s * ((T)g.w[q] * mp[(int64_t)q * NX]);
+      }
+      T r = 0;
+      for (int q = 0; q < 5; q++) r += (T)g.D[q * 5 + j] * tt[q];
+      ay += r;
+    }
+    if (j == 0 && t > 0) {
+      const T* up = u + n - 4 * (int64_t)NX;
+      const T* mp = mu + n - 4 * (int64_t)NX;
+      T tt[5];
+#pragma unroll
+      for (int q = 0; q < 5; q++) {
+        T s = 0;
+#pragma unroll
+        for (int l = 0; l < 5; l++) s += (T)g.D[q * 5 + l] * up[(int64_t)l * NX];
+        tt[q] = s * ((T)g.w[q] * mp[(int64_t)q * NX]);
+      }
+      T r = 0;
+      for (int q = 0; q < 5; q++) r += (T)g.D[q * 5 + 4] * tt[q];
+      ay += r;
+    }
+  }
+  return (T)(g.c1 * wline(g, J, g.NY)) * ax + (T)(g.c2 * wline(g, I, g.NX)) * ay;
+}
+
+// --------------------------------------------------------------------------
+// generic tile kernel
+// --------------------------------------------------------------------------
+constexpr int WIN_I = TILE_I + 5;
+constexpr int WIN_J = TILE_J + 5;
+
+struct PreNone {};
+
+template <typename T, class Op>
+__global__ __launch_bounds__(BLOCK) void tile_kernel(Geom g, Op op) {
+  __shared__ T su[WIN_J * WIN_I];
+  __shared__ T sm[WIN_J * WIN_I];
+  typename Op::Pre pre;
+  if (!op.prepare(pre)) return;   // block-uniform
+  const int I0 = blockIdx.x * TILE_I, J0 = blockIdx.y * TILE_J;
+  for (int k = threadIdx.x; k < WIN_J * WIN_I; k += BLOCK) {
+    const int r = k / WIN_I, c = k - r * WIN_I;
+    const int I = I0 - 4 + c, J = J0 - 4 + r;
+    T u = 0, m = 0;
+    if (I >= 0 && I < g.NX && J >= 0 && J < g.NY) {
+      const int64_t n = (int64_t)J * g.NX + I;
+      u = op.load(pre, n, I, J);
+      m = op.mu[n];
+    }
+    su[k] = u;
+    sm[k] = m;
+  }
+  __syncthreads();
+  double acc0 = 0.0, acc1 = 0.0;
+  for (int k = threadIdx.x; k < TILE_I * TILE_J; k += BLOCK) {
+    const int r = k / TILE_I, c = k - r * TILE_I;
+    const int I = I0 + c, J = J0 + r;
+    if (I >= g.NX || J >= g.NY) continue;
+    const int lc = c + 4, lr = r + 4;
+    const T ku = ku_window<T, WIN_I>(g, su, sm, lc, lr, I, J);
+    op.epilogue(pre, (int64_t)J * g.NX + I, I, J, ku, su[lr * WIN_I + lc], acc0, acc1);
+  }
+  if (Op::kParts >= 1) {
+    const double s = block_sum(acc0);
+    if (threadIdx.x == 0) op.part0[blockIdx.y * gridDim.x + blockIdx.x] = s;
+  }
+  if (Op::kParts >= 2) {
+    const double s = block_sum(acc1);
+    if (threadIdx.x == 0) op.part1[blockIdx.y * gridDim.x + blockIdx.x] = s;
+  }
+}
+
+// y = sign K u  (interior: fault row of u read as 0, of y written as 0)
+template <typename T>
+struct OpApply {
+  using Pre = PreNone;
+  static constexpr int kParts = 0;
+  const T* u;
+  const T* mu;
+  T* y;
+  int interior;
+  T sign;
+  double* part0;
+  double* part1;
+  __device__ bool prepare(Pre&) const { return true; }
+  __device__ T load(const Pre&, int64_t n, int, int J) const { return (interior && J == 0) ? T(0) : u[n]; }
+  __device__ void epilogue(const Pre&, int64_t n, int, int J, T ku, T, double&, double&) const {
+    y[n] = (interior && J == 0) ? T(0) : sign * ku;
+  }
+};
+
+// r = b - K_II x, partials of r.r and b.b
+template <typename T>
+struct OpResidual {
+  using Pre = PreNone;
+  static constexpr int kParts = 2;
+  const T* x;
+  const T* mu;
+  const T* b;
+  T* r;
+  double* part0;
+  double* part1;
+  __device__ bool prepare(Pre&) const { return true; }
+  __device__ T load(const Pre&, int64_t n, int, int J) const { return J == 0 ? T(0) : x[n]; }
+  __device__ void epilogue(const Pre&, int64_t n, int, int J, T ku, T, double& a0, double& a1) const {
+    const T bv = J == 0 ? T(0) : b[n];
+    const T rv = J == 0 ? T(0) : bv - ku;
+    r[n] = rv;
+    a0 += (double)rv * (double)rv;
+    a1 += (double)bv * (double)bv;
+  }
+};
+
+// beta = rho / rho_prev; u_new = c + beta u_old; q = K_II u_new; partial u_new.q
+template <typename T>
+struct OpCgDir {
+  struct Pre { T beta; };
+  static constexpr int kParts = 1;
+  const T* c;
+  const T* u_old;
+  const T* mu;
+  T* u_new;
+  T* q;
+  const double* part_rho;
+  int nrho;
+  const CgScal* sc;
+  double* part0;
+  double* part1;
+  __device__ bool prepare(Pre& p) const {
+    if (sc->done) return false;
+    const double rho = reduce_parts(part_rho, nrho);
+    p.beta = (T)(rho / sc->rho_prev);
+    return true;
+  }
+  __device__ T load(const Pre& p, int64_t n, int, int J) const { return J == 0 ? T(0) : c[n] + p.beta * u_old[n]; }
+  __device__ void epilogue(const Pre&, int64_t n, int, int J, T ku, T uc, double& a0, double&) const {
+    const T qv = J == 0 ? T(0) : ku;
+    u_new[n] = uc;
+    q[n] = qv;
+    a0 += (double)uc * (double)qv;
+  }
+};
+
+// Convergence test (cg! checks ||r|| <= tol ||b|| before each iteration),
+// then x0 = omega D^-1 r and res = r - K_II x0.
+template <typename T>
+struct OpL0Down {
+  using Pre = PreNone;
+  static constexpr int kParts = 0;
+  const T* r;
+  const T* mu;
+  const T* dinv;
+  T omega;
+  T* x0;
+  T* res;
+  const double* part_rr;
+  int nrr;
+  CgScal* sc;
+  double* part0;
+  double* part1;
+  __device__ bool prepare(Pre&) const {
+    if (sc->done) return false;
+    const double rr = reduce_parts(part_rr, nrr);
+    const double resid = sqrt(rr);
+    const bool fin = (resid <= sc->reltol) || (sc->iter >= sc->maxiter);
+    if (threadIdx.x == 0 && blockIdx.x == 0 && blockIdx.y == 0) {
+      sc->resid = resid;
+      if (fin) sc->done = 1;
+    }
+    return !fin;
+  }
+  __device__ T load(const Pre&, int64_t n, int, int J) const { return J == 0 ? T(0) : omega * dinv[n] * r[n]; }
+  __device__ void epilogue(const Pre&, int64_t n, int, int J, T ku, T uc, double&, double&) const {
+    x0[n] = uc;
+    res[n] = J == 0 ? T(0) : r[n] - ku;
+  }
+};
+
+// c = x0 + omega D^-1 (r - K_II x0), partial c.r
+template <typename T>
+struct OpL0Post {
+  using Pre = PreNone;
+  static constexpr int kParts = 1;
+  const T* x0;
+  const T* r;
+  const T* mu;
+  const T* dinv;
+  T omega;
+  T* c;
+  const CgScal* sc;
+  double* part0;
+  double* part1;
+  __device__ bool prepare(Pre&) const { return !sc->done; }
+  __device__ T load(const Pre&, int64_t n, int, int J) const { return J == 0 ? T(0) : x0[n]; }
+  __device__ void epilogue(const Pre&, int64_t n, int, int J, T ku, T uc, double& a0, double&) const {
+    const T cv = J == 0 ? T(0) : uc + omega * dinv[n] * (r[n] - ku);
+    c[n] = cv;
+    a0 += (double)cv * (double)r[n];
+  }
+};
+
+// --------------------------------------------------------------------------
+// fault physics (FP64, libm-equivalent device exp/log)
+// --------------------------------------------------------------------------
+// IDS! (src/otherFunctions_serial.jl:5-32)
+__device__ double d_ids(double xLf, double Vo, double psi, double dt, double Vf, double cnd, int IDstate) {
+  if (IDstate == 1) return psi + dt * ((Vo / xLf) * exp(-psi) - fabs(Vf) / xLf);
+  const double VdtL = fabs(Vf) * dt / xLf;
+  if (VdtL < cnd) return log(exp(psi - VdtL) + Vo * dt / xLf - 0.5 * Vo * fabs(Vf) * dt * dt / (xLf * xLf));
+  return log(exp(psi - VdtL) + (Vo / fabs(Vf)) * (1 - exp(-VdtL)));
+}
+
+// IDS2! (src/otherFunctions_serial.jl:35-59)
+__device__ double d_ids2(double xLf, double Vo, double psi, double psi1, double dt, double Vf, double Vf1,
+                         int IDstate) {
+  if (IDstate == 1)
+    return psi + 0.5 * dt * ((Vo / xLf) * exp(-psi) - fabs(Vf) / xLf + (Vo / xLf) * exp(-psi1) - fabs(Vf1) / xLf);
+  const double VdtL = 0.5 * fabs(Vf1 + Vf) * dt / xLf;
+  if (VdtL < 1e-6)
+    return log(exp(psi - VdtL) + Vo * dt / xLf - 0.5 * Vo * 0.5 * fabs(Vf1 + Vf) * dt * dt / (xLf * xLf));
+  return log(exp(psi - VdtL) + (Vo / (0.5 * fabs(Vf + Vf1))) * (1 - exp(-VdtL)));
+}
+
+// NRsearch! (src/NRsearch_serial.jl:47-98): Newton on
+//   g(tau) = Z Vfree - Z Vf(tau) + tauo - tau,  Vf = Vo (e^{h+fa} - e^{h-fa}),
+// stopping when |delta| <= 0.001 a sigma; returns k (negative on the
+// |delta| > 1e10 / k == 1000 exit, which returns the current iterate).
+__device__ int d_nrsearch(double fo, double Vo, double cca, double ccb, double Seff, double tau, double tauo,
+                          double psi, double FltZ, double FltVfree, double& Vf_out, double& tau_out) {
+  const double Vw = 1e10;
+  const double fact = 1. + (Vo / Vw) * exp(-psi);
+  const double eps = 0.001 * cca * Seff;
+  const double help = -(fo + ccb * psi) / cca;
+  int k = 0;
+  double delta = INFINITY, Vf = 0.0;
+  while (fabs(delta) > eps) {
+    const double fa = fact * tau / (Seff * cca);
+    const double help1 = exp(help + fa), help2 = exp(help - fa);
+    Vf = Vo * (help1 - help2);
+    const double Vfprime = fact * (Vo / (cca * Seff)) * (help1 + help2);
+    delta = (FltZ * FltVfree - FltZ * Vf + tauo - tau) / (1 + FltZ * Vfprime);
+    tau = tau + delta;
+    k = k + 1;
+    if (fabs(delta) > 1e10 || k == 1000) {
+      Vf_out = Vf;
+      tau_out = tau;
+      return -k;
+    }
+  }
+  const double fa = fact * tau / (Seff * cca);
+  Vf_out = Vo * (exp(help + fa) - exp(help - fa));
+  tau_out = tau;
+  return k;
+}
+
+// FBC! body for one node j (src/NRsearch_serial.jl:17-43)
+__device__ void d_fbc_node(const FaultDev& fp, const FaultState& fs, int j, double psi, double Vfj, double Vfree,
+                           double dt, int* flags, double& psi2_out, double& tau2_out) {
+  double psi1 = d_ids(fp.xLf[j], fp.Vo[j], psi, dt, Vfj, 1e-5, fp.IDstate);
+  double Vf1, tau1, Vf2, tau2;
+  const int k1 = d_nrsearch(fp.fo[j], fp.Vo[j], fp.cca[j], fp.ccb[j], fp.Seff[j], 0.0, fp.tauo[j], psi1, fp.FltZ[j],
+                            Vfree, Vf1, tau1);
+  if (Vfj > 1e10 || isnan(Vfj) || isnan(tau1)) atomicMin(&flags[0], j + 1);
+  const double psi2 = d_ids2(fp.xLf[j], fp.Vo[j], psi, psi1, dt, Vfj, Vf1, fp.IDstate);
+  const int k2 = d_nrsearch(fp.fo[j], fp.Vo[j], fp.cca[j], fp.ccb[j], fp.Seff[j], tau1, fp.tauo[j], psi2, fp.FltZ[j],
+                            Vfree, Vf2, tau2);
+  if (k1 < 0 || k2 < 0) atomicAdd(&flags[1], (k1 < 0) + (k2 < 0));
+  fs.psi1[j] = psi1;
+  fs.Vf1[j] = Vf1;
+  fs.tau1[j] = tau1;
+  fs.psi2[j] = psi2;
+  fs.Vf2[j] = Vf2;
+  fs.tau2[j] = tau2;
+  psi2_out = psi2;
+  tau2_out = tau2;
+}
+
+// Newmark step (src/main.jl:297-344) as a tile epilogue: predictor in the
+// window load, -K d, absorbing edges, FBC!, corrector.  Reads d, v, a and
+// writes d2, v2, a2 (double-buffered state).
+struct OpNewmark {
+  using Pre = PreNone;
+  static constexpr int kParts = 0;
+  FaultDev fp;
+  FaultState fs;
+  const double* mu;
+  const double* M;
+  const double *d, *v, *a;
+  double *d2, *v2, *a2;
+  double dt, half_dt, half_dt_sq;
+  int* flags;
+  int NY;
+  double* part0;
+  double* part1;
+  __device__ bool prepare(Pre&) const { return true; }
+  __device__ double load(const Pre&, int64_t n, int, int) const { return d[n] + dt * v[n] + half_dt_sq * a[n]; }
+  __device__ void epilogue(const Pre&, int64_t n, int I, int J, double ku, double dn, double&, double&) const {
+    const double vo = v[n];
+    const double vp = vo + half_dt * a[n];
+    double at = -ku;
+    const bool fault = (J == 0);
+    const bool deep = fault && (I < fp.j0);
+    if (deep) at = 0.0;
+    if (I == 0) at = at - fp.BcLC[J] * vp;
+    if (J == NY - 1) at = at - fp.BcTC[I] * vp;
+    if (fault) {
+      const int j = I;
+      const double Mj = M[n];
+      const double Vfree = 2 * vp + 2 * half_dt * at / Mj;
+      const double Vfj = 2 * vo + fp.Vpl;
+      fs.FltVfree[j] = Vfree;
+      fs.Vf[j] = Vfj;
+      double psi_new, tau2;
+      if (!deep) {
+        d_fbc_node(fp, fs, j, fs.psi[j], Vfj, Vfree, dt, flags, psi_new, tau2);
+      } else {
+        psi_new = fs.psi2[j];
+        tau2 = fs.tau2[j];
+      }
+      const double tau = deep ? 0.0 : tau2 - fp.tauo[j];
+      fs.tau[j] = tau;
+      fs.psi[j] = psi_new;
+      at = at - fp.FltB[j] * tau;
+    }
+    double an = at / M[n];
+    double vn = vp + half_dt * an;
+    if (deep) {
+      vn = 0.0;
+      an = 0.0;
+    }
+    d2[n] = dn;
+    v2[n] = vn;
+    a2[n] = an;
+  }
+};
+
+// --------------------------------------------------------------------------
+// element-wise kernels
+// --------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void cg_update_kernel(int64_t n, T* x, T* r, const T* u, const T* q,
+                                                          const double* part_rho, int nrho, const double* part_uq,
+                                                          int nuq, CgScal* sc, double* part_rr, int npart) {
+  if (sc->done) return;
+  // the residual partials are consumed over npart entries: clear the tail
+  if (blockIdx.x == 0)
+    for (int i = gridDim.x + threadIdx.x; i < npart; i += BLOCK) part_rr[i] = 0.0;
+  const double rho = reduce_parts(part_rho, nrho);
+  const double uq = reduce_parts(part_uq, nuq);
+  const T alpha = (T)(rho / uq);
+  double acc = 0.0;
+  const int64_t base = (int64_t)blockIdx.x * BLOCK * 4;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int64_t i = base + k * BLOCK + threadIdx.x;
+    if (i < n) {
+      x[i] += alpha * u[i];
+      const T rv = r[i] - alpha * q[i];
+      r[i] = rv;
+      acc += (double)rv * (double)rv;
+    }
+  }
+  const double s = block_sum(acc);
+  if (threadIdx.x == 0) part_rr[blockIdx.x] = s;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->rho_prev = rho;
+    sc->iter = sc->iter + 1;
+  }
+}
+
+// Jacobi preconditioner with the cg! convergence test: c = D^-1 r, partial c.r
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void jacobi_pc_kernel(int64_t n, const T* dinv, const T* r, T* c,
+                                                          const double* part_rr, int nrr, CgScal* sc,
+                                                          double* part_rho) {
+  if (sc->done) return;
+  const double rr = reduce_parts(part_rr, nrr);
+  const double resid = sqrt(rr);
+  const bool fin = (resid <= sc->reltol) || (sc->iter >= sc->maxiter);
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    sc->resid = resid;
+    if (fin) sc->done = 1;
+  }
+  if (fin) return;
+  double acc = 0.0;
+  const int64_t base = (int64_t)blockIdx.x * BLOCK * 4;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int64_t i = base + k * BLOCK + threadIdx.x;
+    if (i < n) {
+      const T cv = dinv[i] * r[i];
+      c[i] = cv;
+      acc += (double)cv * (double)r[i];
+    }
+  }
+  const double s = block_sum(acc);
+  if (threadIdx.x == 0) part_rho[blockIdx.x] = s;
+}
+
+__global__ void cg_init_scal_kernel(CgScal* sc, const double* part_rr, const double* part_bb, int nparts, double tol,
+                                    int maxiter) {
+  const double rr = reduce_parts(part_rr, nparts);
+  const double bb = reduce_parts(part_bb, nparts);
+  if (threadIdx.x == 0) {
+    sc->rho_prev = 1.0;
+    sc->resid = sqrt(rr);
+    sc->bnorm = sqrt(bb);
+    sc->reltol = tol * sqrt(bb);
+    sc->iter = 0;
+    sc->maxiter = maxiter;
+    sc->done = (sqrt(rr) <= tol * sqrt(bb) || maxiter <= 0) ? 1 : 0;
+  }
+}
+
+__global__ void cg_check_kernel(CgScal* sc, const double* part_rr, int nparts) {
+  if (sc->done) return;
+  const double rr = reduce_parts(part_rr, nparts);
+  if (threadIdx.x == 0) {
+    const double resid = sqrt(rr);
+    sc->resid = resid;
+    if (resid <= sc->reltol || sc->iter >= sc->maxiter) sc->done = 1;
+  }
+}
+
+// --------------------------------------------------------------------------
+// CSR kernels for AMG levels >= 1 (LPR lanes per row)
+// --------------------------------------------------------------------------
+template <typename T, int LPR, int MODE>
+__global__ __launch_bounds__(BLOCK) void csr_kernel(int64_t n, const int32_t* __restrict__ rp,
+                                                    const int32_t* __restrict__ col, const T* __restrict__ val,
+                                                    const T* x, const T* b, const T* dinv, T omega, T* y, T* xout,
+                                                    const CgScal* sc) {
+  if (sc && sc->done) return;
+  const int64_t gid = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  const int64_t row = gid / LPR;
+  const int lane = (int)(gid % LPR);
+  double acc = 0.0;
+  if (row < n) {
+    const int q1 = rp[row + 1];
+    for (int q = rp[row] + lane; q < q1; q += LPR) {
+      const int c = col[q];
+      const T xv = (MODE == CSR_DOWN) ? omega * dinv[c] * b[c] : x[c];
+      acc += (double)val[q] * (double)xv;
+    }
+  }
+#pragma unroll
+  for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_down(acc, off, LPR);
+  if (row < n && lane == 0) {
+    if (MODE == CSR_SPMV) {
+      y[row] = (T)acc;
+    } else if (MODE == CSR_DOWN) {
+      xout[row] = omega * dinv[row] * b[row];
+      y[row] = b[row] - (T)acc;
+    } else if (MODE == CSR_PROLONG_ADD) {
+      y[row] = y[row] + (T)acc;
+    } else {  // CSR_POST: y = x + w dinv (b - A x)
+      y[row] = x[row] + omega * dinv[row] * (b[row] - (T)acc);
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void dense_kernel(int64_t n, const double* inv, const T* b, T* x,
+                                                      const CgScal* sc) {
+  if (sc && sc->done) return;
+  for (int64_t i = threadIdx.x; i < n; i += BLOCK) {
+    double s = 0.0;
+    for (int64_t j = 0; j < n; j++) s += inv[i * n + j] * (double)b[j];
+    x[i] = (T)s;
+  }
+}
+
+// --------------------------------------------------------------------------
+// quasi-static step pieces (src/main.jl:226-295)
+// --------------------------------------------------------------------------
+// pass 1: dPre = d everywhere; fault: Vf0 = Vf = 2v + Vpl.  Both passes:
+// F = dPre + v dt on the fault, d[fault] = F, b rows 1..4 = -(K F) (only the
+// bottom element row couples to the fault).
+__global__ __launch_bounds__(BLOCK) void qs_begin_kernel(Geom g, FaultDev fp, FaultState fs, const double* mu,
+                                                         double* d, double* dPre, const double* v, double* b,
+                                                         double dt, int first_pass) {
+  const int64_t N = (int64_t)g.NX * g.NY;
+  const int64_t n = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (first_pass) {
+    if (n >= N) return;
+    dPre[n] = d[n];
+  } else if (n >= g.NX) {
+    return;
+  }
+  if (n >= g.NX) return;
+  const int I = (int)n;
+  const double vf = v[I];
+  if (first_pass) {
+    const double V0 = 2 * vf + fp.Vpl;
+    fs.Vf0[I] = V0;
+    fs.Vf[I] = V0;
+  }
+  const double F = (first_pass ? d[I] : dPre[I]) + vf * dt;
+  d[I] = F;
+  const double cy = g.c2 * wline(g, I, g.NX);
+  for (int J = 1; J <= 4; J++) {
+    double s = 0.0;
+    for (int q = 0; q < 5; q++) s += g.D[q * 5 + J] * (g.w[q] * mu[(int64_t)q * g.NX + I]) * g.D[q * 5 + 0];
+    b[(int64_t)J * g.NX + I] = -(cy * s) * F;
+  }
+}
+
+// tau1 = -(K d)[fault]/FltB (0 on creeping nodes), slrFunc!, Vf and fault v update
+__global__ __launch_bounds__(BLOCK) void qs_fault_kernel(Geom g, FaultDev fp, FaultState fs, const double* mu,
+                                                         const double* d, double* v, double dt) {
+  const int j = blockIdx.x * BLOCK + threadIdx.x;
+  if (j >= fp.nflt) return;
+  const bool deep = j < fp.j0;
+  const double a = deep ? 0.0 : ku_global<double>(g, d, mu, j, 0);
+  const double tau1 = -a / fp.FltB[j];
+  fs.tau1[j] = tau1;
+  double Vf1;
+  if (!deep) {
+    const double psi1 = d_ids(fp.xLf[j], fp.Vo[j], fs.psi[j], dt, fs.Vf[j], 1e-6, fp.IDstate);
+    fs.psi1[j] = psi1;
+    const double tauAB = tau1 + fp.tauo[j];
+    const double fa = tauAB / (fp.Seff[j] * fp.cca[j]);
+    const double help = -(fp.fo[j] + fp.ccb[j] * psi1) / fp.cca[j];
+    Vf1 = fp.Vo[j] * (exp(help + fa) - exp(help - fa));
+  } else {
+    Vf1 = fp.Vpl;
+  }
+  fs.Vf1[j] = Vf1;
+  const double Vf = (fs.Vf0[j] + Vf1) / 2;
+  fs.Vf[j] = Vf;
+  v[j] = 0.5 * (Vf - fp.Vpl);
+}
+
+__global__ __launch_bounds__(BLOCK) void qs_finish_kernel(Geom g, FaultDev fp, FaultState fs, double* d,
+                                                          const double* dPre, double* v, double* a, double dt) {
+  const int64_t N = (int64_t)g.NX * g.NY;
+  const int64_t n = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (n >= N) return;
+  if (n < g.NX) {
+    const int j = (int)n;
+    const bool deep = j < fp.j0;
+    fs.psi[j] = fs.psi1[j];
+    fs.tau[j] = deep ? 0.0 : fs.tau1[j];
+    if (deep) fs.Vf1[j] = fp.Vpl;
+    double vv = 0.5 * (fs.Vf1[j] - fp.Vpl);
+    if (deep) {
+      vv = 0.0;
+      d[n] = 0.0;
+    }
+    v[n] = vv;
+  } else {
+    v[n] = (d[n] - dPre[n]) / dt;
+  }
+  a[n] = 0.0;
+}
+
+// Vfmax (src/main.jl:348), regime switch (src/main.jl:439-443), dtevol! (src/dtevol.jl:5-44)
+__device__ double dtevol_dev(const double* XiLf, const double* v, bool slip_rate_from_v, double Vpl, int j0, int n,
+                             double dt, double dtmin, int isolver) {
+  if (isolver != 1) return dtmin;
+  const double dtmax = 50 * 24 * 60 * 60;
+  double dtnx = dtmax;
+  for (int i = j0 + threadIdx.x; i < n; i += BLOCK) {
+    const double Vc = slip_rate_from_v ? 2 * v[i] + Vpl : v[i];
+    if (fabs(Vc) * dtmax > XiLf[i]) {
+      const double dtcell = XiLf[i] / fabs(Vc);
+      if (dtcell < dtnx) dtnx = dtcell;
+    }
+  }
+  __shared__ double sh[BLOCK];
+  sh[threadIdx.x] = dtnx;
+  __syncthreads();
+  for (int s = BLOCK / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sh[threadIdx.x] = fmin(sh[threadIdx.x], sh[threadIdx.x + s]);
+    __syncthreads();
+  }
+  dtnx = sh[0];
+  __syncthreads();
+  if (dtmin > dtnx) dtnx = dtmin;
+  if (dtnx > 1.2 * dt) dtnx = 1.2 * dt;
+  return dtnx;
+}
+
+__global__ __launch_bounds__(BLOCK) void step_tail_kernel(FaultDev fp, const double* v, double dt, double dtmin,
+                                                          int isolver, StepStatus* st, const int* flags) {
+  __shared__ double smax[BLOCK];
+  __shared__ int sidx[BLOCK];
+  double m = -INFINITY;
+  int mi = INT_MAX;
+  for (int j = threadIdx.x; j < fp.nflt; j += BLOCK) {
+    const double vj = v[j];
+    if (vj > m) { m = vj; mi = j; }
+  }
+  smax[threadIdx.x] = m;
+  sidx[threadIdx.x] = mi;
+  __syncthreads();
+  for (int s = BLOCK / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      const double o = smax[threadIdx.x + s];
+      const int oi = sidx[threadIdx.x + s];
+      if (o > smax[threadIdx.x] || (o == smax[threadIdx.x] && oi < sidx[threadIdx.x])) {
+        smax[threadIdx.x] = o;
+        sidx[threadIdx.x] = oi;
+      }
+    }
+    __syncthreads();
+  }
+  const double Vfmax = 2 * smax[0] + fp.Vpl;
+  const int next = ((isolver == 1 && Vfmax < 5e-3) || (isolver == 2 && Vfmax < 2e-3)) ? 1 : 2;
+  const double dtn = dtevol_dev(fp.XiLf, v, true, fp.Vpl, fp.j0, fp.nflt, dt, dtmin, next);
+  if (threadIdx.x == 0) {
+    st->Vfmax = Vfmax;
+    st->dt_next = dtn;
+    st->isolver_next = next;
+    st->hypo = sidx[0];
+    st->nr_flag = (flags[0] == INT_MAX) ? 0 : flags[0];
+    st->nr_noconv = flags[1];
+  }
+}
+
+// --------------------------------------------------------------------------
+// stand-alone fault kernels
+// --------------------------------------------------------------------------
+__global__ void ids_kernel(int64_t n, const double* xLf, const double* Vo, const double* psi, double dt,
+                           const double* Vf, double cnd, int IDstate, double* psi1) {
+  const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (i < n) psi1[i] = d_ids(xLf[i], Vo[i], psi[i], dt, Vf[i], cnd, IDstate);
+}
+__global__ void ids2_kernel(int64_t n, const double* xLf, const double* Vo, const double* psi, const double* psi1,
+                            double dt, const double* Vf, const double* Vf1, int IDstate, double* psi2) {
+  const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (i < n) psi2[i] = d_ids2(xLf[i], Vo[i], psi[i], psi1[i], dt, Vf[i], Vf1[i], IDstate);
+}
+__global__ void nrsearch_kernel(int64_t n, const double* fo, const double* Vo, const double* cca, const double* ccb,
+                                const double* Seff, const double* tau, const double* tauo, const double* psi,
+                                const double* FltZ, const double* FltVfree, double* Vf, double* tau_out, int* k) {
+  const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= n) return;
+  double vf, t;
+  k[i] = d_nrsearch(fo[i], Vo[i], cca[i], ccb[i], Seff[i], tau[i], tauo[i], psi[i], FltZ[i], FltVfree[i], vf, t);
+  Vf[i] = vf;
+  tau_out[i] = t;
+}
+__global__ void fbc_kernel(FaultDev fp, FaultState fs, const double* FltVfree, double dt, int* flags) {
+  const int j = fp.j0 + blockIdx.x * BLOCK + threadIdx.x;
+  if (j >= fp.nflt) return;
+  double p2, t2;
+  d_fbc_node(fp, fs, j, fs.psi[j], fs.Vf[j], FltVfree[j], dt, flags, p2, t2);
+}
+__global__ void slr_kernel(FaultDev fp, FaultState fs, double dt) {
+  const int j = fp.j0 + blockIdx.x * BLOCK + threadIdx.x;
+  if (j >= fp.nflt) return;
+  const double psi1 = d_ids(fp.xLf[j], fp.Vo[j], fs.psi[j], dt, fs.Vf[j], 1e-6, fp.IDstate);
+  fs.psi1[j] = psi1;
+  const double tauAB = fs.tau1[j] + fp.tauo[j];
+  const double fa = tauAB / (fp.Seff[j] * fp.cca[j]);
+  const double help = -(fp.fo[j] + fp.ccb[j] * psi1) / fp.cca[j];
+  fs.Vf1[j] = fp.Vo[j] * (exp(help + fa) - exp(help - fa));
+}
+__global__ __launch_bounds__(BLOCK) void dtevol_kernel(const double* XiLf, const double* Vf, int j0, int n,
+                                                       double dt, double dtmin, int isolver, double* out) {
+  const double r = dtevol_dev(XiLf, Vf, false, 0.0, j0, n, dt, dtmin, isolver);
+  if (threadIdx.x == 0) *out = r;
+}
+
+// --------------------------------------------------------------------------
+// misc
+// --------------------------------------------------------------------------
+__global__ void diag_kernel(Geom g, const double* mu, double* diag) {
+  const int64_t N = (int64_t)g.NX * g.NY;
+  const int64_t n = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (n >= N) return;
+  const int I = (int)(n % g.NX), J = (int)(n / g.NX);
+  double ax = 0, ay = 0;
+  const int i = I & 3, s = I >> 2;
+  for (int ss = (i == 0 ? s - 1 : s); ss <= s; ss++) {
+    if (ss < 0 || ss >= g.nelx) continue;
+    const int li = I - 4 * ss;
+    for (int p = 0; p < 5; p++)
+      ax += g.D[p * 5 + li] * g.D[p * 5 + li] * g.w[p] * mu[(int64_t)J * g.NX + 4 * ss + p];
+  }
+  const int j = J & 3, t = J >> 2;
+  for (int tt = (j == 0 ? t - 1 : t); tt <= t; tt++) {
+    if (tt < 0 || tt >= g.nely) continue;
+    const int lj = J - 4 * tt;
+    for (int q = 0; q < 5; q++)
+      ay += g.D[q * 5 + lj] * g.D[q * 5 + lj] * g.w[q] * mu[(int64_t)(4 * tt + q) * g.NX + I];
+  }
+  diag[n] = g.c1 * wline(g, J, g.NY) * ax + g.c2 * wline(g, I, g.NX) * ay;
+}
+
+// element_computation! (src/PCG.jl:71-86): a_e = c1 H (W.(H' u_e)) + c2 (W.(u_e H)) H'
+__global__ void element_ku_kernel(Geom g, const double* u, const double* mu, double* a_elem) {
+  const int64_t e = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  const int64_t Nel = (int64_t)g.nelx * g.nely;
+  if (e >= Nel) return;
+  const int ex = (int)(e % g.nelx), ey = (int)(e / g.nelx);
+  double ue[5][5], W[5][5];
+  for (int j = 0; j < 5; j++)
+    for (int i = 0; i < 5; i++) {
+      const int64_t n = (int64_t)(4 * ey + j) * g.NX + 4 * ex + i;
+      ue[i][j] = u[n];
+      W[i][j] = g.w[i] * g.w[j] * mu[n];
+    }
+  for (int j = 0; j < 5; j++)
+    for (int i = 0; i < 5; i++) {
+      double sx = 0, sy = 0;
+      for (int p = 0; p < 5; p++) {
+        double tx = 0, ty = 0;
+        for (int k = 0; k < 5; k++) {
+          tx += g.D[p * 5 + k] * ue[k][j];
+          ty += g.D[p * 5 + k] * ue[i][k];
+        }
+        sx += g.D[p * 5 + i] * W[p][j] * tx;
+        sy += g.D[p * 5 + j] * W[i][p] * ty;
+      }
+      a_elem[e * 25 + j * 5 + i] = g.c1 * sx + g.c2 * sy;
+    }
+}
+
+template <typename T, typename U>
+__global__ void convert_kernel(int64_t n, const T* x, U* y) {
+  const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (i < n) y[i] = (U)x[i];
+}
+template <typename T>
+__global__ void fill_kernel(int64_t n, T* y, T v) {
+  const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (i < n) y[i] = v;
+}
+__global__ void axpy_kernel(int64_t n, double alpha, const double* x, double* y) {
+  const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (i < n) y[i] += alpha * x[i];
+}
+__global__ void scale_mask_kernel(int64_t n, int nx, const double* x, double sign, double* y) {
+  const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (i < n) y[i] = (i < nx) ? 0.0 : sign * x[i];
+}
+
+// --------------------------------------------------------------------------
+// launchers
+// --------------------------------------------------------------------------
+static inline int nblk(int64_t n) { return (int)((n + BLOCK - 1) / BLOCK); }
+
+template <typename T>
+void launch_ku(const Geom& g, const T* u, const T* mu, T* y, int interior, int negate, hipStream_t s) {
+  OpApply<T> op{u, mu, y, interior, negate ? T(-1) : T(1), nullptr, nullptr};
+  hipLaunchKernelGGL((tile_kernel<T, OpApply<T>>), tile_grid(g), dim3(BLOCK), 0, s, g, op);
+}
+
+void launch_diag(const Geom& g, const double* mu, double* diag, hipStream_t s) {
+  const int64_t N = (int64_t)g.NX * g.NY;
+  hipLaunchKernelGGL(diag_kernel, dim3(nblk(N)), dim3(BLOCK), 0, s, g, mu, diag);
+}
+
+void launch_element_ku(const Geom& g, const double* u, const double* mu, double* a_elem, hipStream_t s) {
+  const int64_t Nel = (int64_t)g.nelx * g.nely;
+  hipLaunchKernelGGL(element_ku_kernel, dim3(nblk(Nel)), dim3(BLOCK), 0, s, g, u, mu, a_elem);
+}
+
+template <typename T>
+void launch_cg_residual(const Geom& g, const T* x, const T* mu, const T* b, T* r, double* part_rr, double* part_bb,
+                        hipStream_t s) {
+  OpResidual<T> op{x, mu, b, r, part_rr, part_bb};
+  hipLaunchKernelGGL((tile_kernel<T, OpResidual<T>>), tile_grid(g), dim3(BLOCK), 0, s, g, op);
+}
+
+void launch_cg_init_scal(CgScal* sc, const double* part_rr, const double* part_bb, int nparts, double tol,
+                         int maxiter, hipStream_t s) {
+  hipLaunchKernelGGL(cg_init_scal_kernel, dim3(1), dim3(BLOCK), 0, s, sc, part_rr, part_bb, nparts, tol, maxiter);
+}
+
+void launch_cg_check(CgScal* sc, const double* part_rr, int nparts, hipStream_t s) {
+  hipLaunchKernelGGL(cg_check_kernel, dim3(1), dim3(BLOCK), 0, s, sc, part_rr, nparts);
+}
+
+template <typename T>
+void launch_cg_dir(const Geom& g, const T* c, const T* u_old, const T* mu, T* u_new, T* q, const double* part_rho,
+                   int nrho, const CgScal* sc, double* part_uq, hipStream_t s) {
+  OpCgDir<T> op{c, u_old, mu, u_new, q, part_rho, nrho, sc, part_uq, nullptr};
+  hipLaunchKernelGGL((tile_kernel<T, OpCgDir<T>>), tile_grid(g), dim3(BLOCK), 0, s, g, op);
+}
+
+template <typename T>
+void launch_cg_update(int64_t n, T* x, T* r, const T* u, const T* q, const double* part_rho, int nrho,
+                      const double* part_uq, int nuq, CgScal* sc, double* part_rr, int npart, hipStream_t s) {
+  hipLaunchKernelGGL((cg_update_kernel<T>), dim3(vec_blocks(n)), dim3(BLOCK), 0, s, n, x, r, u, q, part_rho, nrho,
+                     part_uq, nuq, sc, part_rr, npart);
+}
+
+template <typename T>
+void launch_jacobi_pc(int64_t n, const T* dinv, const T* r, T* c, const double* part_rr, int nrr, CgScal* sc,
+                      double* part_rho, hipStream_t s) {
+  hipLaunchKernelGGL((jacobi_pc_kernel<T>), dim3(vec_blocks(n)), dim3(BLOCK), 0, s, n, dinv, r, c, part_rr, nrr, sc,
+                     part_rho);
+}
+
+template <typename T>
+void launch_l0_down(const Geom& g, const T* r, const T* mu, const T* dinv, T omega, T* x0, T* res,
+                    const double* part_rr, int nrr, CgScal* sc, hipStream_t s) {
+  OpL0Down<T> op{r, mu, dinv, omega, x0, res, part_rr, nrr, sc, nullptr, nullptr};
+  hipLaunchKernelGGL((tile_kernel<T, OpL0Down<T>>), tile_grid(g), dim3(BLOCK), 0, s, g, op);
+}
+
+template <typename T>
+void launch_l0_post(const Geom& g, const T* x0, const T* r, const T* mu, const T* dinv, T omega, T* c,
+                    double* part_rho, const CgScal* sc, hipStream_t s) {
+  OpL0Post<T> op{x0, r, mu, dinv, omega, c, sc, part_rho, nullptr};
+  hipLaunchKernelGGL((tile_kernel<T, OpL0Post<T>>), tile_grid(g), dim3(BLOCK), 0, s, g, op);
+}
+
+template <typename T, int MODE>
+static void csr_dispatch(const DevCsr& M, const T* val, const T* x, const T* b, const T* dinv, T omega, T* y, T* xout,
+                         const CgScal* sc, hipStream_t s) {
+  const int64_t threads = M.n * (int64_t)M.lpr;
+  const dim3 grid((unsigned)((threads + BLOCK - 1) / BLOCK));
+  switch (M.lpr) {
+    case 1:
+      hipLaunchKernelGGL((csr_kernel<T, 1, MODE>), grid, dim3(BLOCK), 0, s, M.n, M.rowptr, M.col, val, x, b, dinv,
+                         omega, y, xout, sc);
+      break;
+    case 2:
+      hipLaunchKernelGGL((csr_kernel<T, 2, MODE>), grid, dim3(BLOCK), 0, s, M.n, M.rowptr, M.col, val, x, b, dinv,
+                         omega, y, xout, sc);
+      break;
+    case 4:
+      hipLaunchKernelGGL((csr_kernel<T, 4, MODE>), grid, dim3(BLOCK), 0, s, M.n, M.rowptr, M.col, val, x, b, dinv,
+                         omega, y, xout, sc);
+      break;
+    case 8:
+      hipLaunchKernelGGL((csr_kernel<T, 8, MODE>), grid, dim3(BLOCK), 0, s, M.n, M.rowptr, M.col, val, x, b, dinv,
+                         omega, y, xout, sc);
+      break;
+    default:
+      hipLaunchKernelGGL((csr_kernel<T, 16, MODE>), grid, dim3(BLOCK), 0, s, M.n, M.rowptr, M.col, val, x, b, dinv,
+                         omega, y, xout, sc);
+      break;
+  }
+}
+
+template <typename T>
+static const T* csr_vals(const DevCsr& M);
+template <>
+const double* csr_vals<double>(const DevCsr& M) { return M.val; }
+template <>
+const float* csr_vals<float>(const DevCsr& M) { return M.val32; }
+
+template <typename T>
+void launch_csr(CsrMode mode, const DevCsr& M, const T* x, const T* b, const T* dinv, T omega, T* y, T* xout,
+                const CgScal* sc, hipStream_t s) {
+  if (M.n == 0) return;
+  const T* val = csr_vals<T>(M);
+  switch (mode) {
+    case CSR_SPMV: csr_dispatch<T, CSR_SPMV>(M, val, x, b, dinv, omega, y, xout, sc, s); break;
+    case CSR_DOWN: csr_dispatch<T, CSR_DOWN>(M, val, x, b, dinv, omega, y, xout, sc, s); break;
+    case CSR_PROLONG_ADD: csr_dispatch<T, CSR_PROLONG_ADD>(M, val, x, b, dinv, omega, y, xout, sc, s); break;
+    default: csr_dispatch<T, CSR_POST>(M, val, x, b, dinv, omega, y, xout, sc, s); break;
+  }
+}
+
+template <typename T>
+void launch_dense(int64_t n, const double* inv, const T* b, T* x, const CgScal* sc, hipStream_t s) {
+  hipLaunchKernelGGL((dense_kernel<T>), dim3(1), dim3(BLOCK), 0, s, n, inv, b, x, sc);
+}
+
+void launch_newmark(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, const double* M,
+                    const double* d, const double* v, const double* a, double* d2, double* v2, double* a2,
+                    double dt, double half_dt, double half_dt_sq, int* flags, hipStream_t s) {
+  OpNewmark op{fp, fs, mu, M, d, v, a, d2, v2, a2, dt, half_dt, half_dt_sq, flags, g.NY, nullptr, nullptr};
+  hipLaunchKernelGGL((tile_kernel<double, OpNewmark>), tile_grid(g), dim3(BLOCK), 0, s, g, op);
+}
+
+void launch_qs_begin(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, double* d, double* dPre,
+                     const double* v, double* b, double dt, int first_pass, hipStream_t s) {
+  const int64_t N = first_pass ? (int64_t)g.NX * g.NY : g.NX;
+  hipLaunchKernelGGL(qs_begin_kernel, dim3(nblk(N)), dim3(BLOCK), 0, s, g, fp, fs, mu, d, dPre, v, b, dt,
+                     first_pass);
+}
+
+void launch_qs_fault(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, const double* d, double* v,
+                     double dt, hipStream_t s) {
+  hipLaunchKernelGGL(qs_fault_kernel, dim3(nblk(fp.nflt)), dim3(BLOCK), 0, s, g, fp, fs, mu, d, v, dt);
+}
+
+void launch_qs_finish(const Geom& g, const FaultDev& fp, FaultState fs, double* d, const double* dPre, double* v,
+                      double* a, double dt, hipStream_t s) {
+  const int64_t N = (int64_t)g.NX * g.NY;
+  hipLaunchKernelGGL(qs_finish_kernel, dim3(nblk(N)), dim3(BLOCK), 0, s, g, fp, fs, d, dPre, v, a, dt);
+}
+
+void launch_step_tail(const FaultDev& fp, const double* v, double dt, double dtmin, int isolver, StepStatus* st,
+                      const int* flags, hipStream_t s) {
+  hipLaunchKernelGGL(step_tail_kernel, dim3(1), dim3(BLOCK), 0, s, fp, v, dt, dtmin, isolver, st, flags);
+}
+
+void launch_ids(int64_t n, const double* xLf, const double* Vo, const double* psi, double dt, const double* Vf,
+                double cnd, int IDstate, double* psi1, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(ids_kernel, dim3(nblk(n)), dim3(BLOCK), 0, s, n, xLf, Vo, psi, dt, Vf, cnd, IDstate, psi1);
+}
+void launch_ids2(int64_t n, const double* xLf, const double* Vo, const double* psi, const double* psi1, double dt,
+                 const double* Vf, const double* Vf1, int IDstate, double* psi2, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(ids2_kernel, dim3(nblk(n)), dim3(BLOCK), 0, s, n, xLf, Vo, psi, psi1, dt, Vf, Vf1, IDstate,
+                     psi2);
+}
+void launch_nrsearch(int64_t n, const double* fo, const double* Vo, const double* cca, const double* ccb,
+                     const double* Seff, const double* tau, const double* tauo, const double* psi,
+                     const double* FltZ, const double* FltVfree, double* Vf, double* tau_out, int* k,
+                     hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(nrsearch_kernel, dim3(nblk(n)), dim3(BLOCK), 0, s, n, fo, Vo, cca, ccb, Seff, tau, tauo, psi,
+                     FltZ, FltVfree, Vf, tau_out, k);
+}
+void launch_fbc(const FaultDev& fp, FaultState fs, const double* FltVfree, double dt, int* flags, hipStream_t s) {
+  const int n = fp.nflt - fp.j0;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(fbc_kernel, dim3(nblk(n)), dim3(BLOCK), 0, s, fp, fs, FltVfree, dt, flags);
+}
+void launch_slr(const FaultDev& fp, FaultState fs, double dt, hipStream_t s) {
+  const int n = fp.nflt - fp.j0;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(slr_kernel, dim3(nblk(n)), dim3(BLOCK), 0, s, fp, fs, dt);
+}
+void launch_dtevol(const double* XiLf, const double* Vf, int j0, int n, double dt, double dtmin, int isolver,
+                   double* out, hipStream_t s) {
+  hipLaunchKernelGGL(dtevol_kernel, dim3(1), dim3(BLOCK), 0, s, XiLf, Vf, j0, n, dt, dtmin, isolver, out);
+}
+
+template <typename T, typename U>
+void launch_convert(int64_t n, const T* x, U* y, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL((convert_kernel<T, U>), dim3(nblk(n)), dim3(BLOCK), 0, s, n, x, y);
+}
+template <typename T>
+void launch_fill(int64_t n, T* y, T val, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL((fill_kernel<T>), dim3(nblk(n)), dim3(BLOCK), 0, s, n, y, val);
+}
+void launch_axpy(int64_t n, double alpha, const double* x, double* y, hipStream_t s) {
+  hipLaunchKernelGGL(axpy_kernel, dim3(nblk(n)), dim3(BLOCK), 0, s, n, alpha, x, y);
+}
+void launch_scale_mask(int64_t n, int nx, const double* x, double sign, double* y, hipStream_t s) {
+  hipLaunchKernelGGL(scale_mask_kernel, dim3(nblk(n)), dim3(BLOCK), 0, s, n, nx, x, sign, y);
+}
+
+// explicit instantiations
+#define EQ_INST(T)                                                                                                  \
+  template void launch_ku<T>(const Geom&, const T*, const T*, T*, int, int, hipStream_t);                          \
+  template void launch_cg_residual<T>(const Geom&, const T*, const T*, const T*, T*, double*, double*, hipStream_t); \
+  template void launch_cg_dir<T>(const Geom&, const T*, const T*, const T*, T*, T*, const double*, int,            \
+                                 const CgScal*, double*, hipStream_t);                                              \
+  template void launch_cg_update<T>(int64_t, T*, T*, const T*, const T*, const double*, int, const double*, int,    \
+                                    CgScal*, double*, int, hipStream_t);                                                 \
+  template void launch_jacobi_pc<T>(int64_t, const T*, const T*, T*, const double*, int, CgScal*, double*,          \
+                                    hipStream_t);                                                                   \
+  template void launch_l0_down<T>(const Geom&, const T*, const T*, const T*, T, T*, T*, const double*, int,         \
+                                  CgScal*, hipStream_t);                                                            \
+  template void launch_l0_post<T>(const Geom&, const T*, const T*, const T*, const T*, T, T*, double*,             \
+                                  const CgScal*, hipStream_t);                                                      \
+  template void launch_csr<T>(CsrMode, const DevCsr&, const T*, const T*, const T*, T, T*, T*, const CgScal*,       \
+                              hipStream_t);                                                                         \
+  template void launch_dense<T>(int64_t, const double*, const T*, T*, const CgScal*, hipStream_t);                 \
+  template void launch_fill<T>(int64_t, T*, T, hipStream_t);
+EQ_INST(double)
+EQ_INST(float)
+template void launch_convert<double, float>(int64_t, const double*, float*, hipStream_t);
+template void launch_convert<float, double>(int64_t, const float*, double*, hipStream_t);
+template void launch_fill<int>(int64_t, int*, int, hipStream_t);
+
+}  // namespace eq

@@ -1,0 +1,212 @@
+/*
+ * eqcycle_hip.h -- C ABI of libeqcycle_hip.so, the gfx950 implementation of
+ * eqcycle's time-stepping hot path.
+ *
+ * Every entry point replaces one reference routine or operator call; the
+ * reference file:line it stands in for is given beside it (paths relative to
+ * the eqcycle repository).  Conventions follow the reference:
+ *   - arrays are Float64 / Int64, index arrays 1-based, 3-D arrays column-major
+ *     (iglob[i,j,e] is element i + 5 j + 25 e);
+ *   - nodal vectors use the reference's MeshBox global numbering;
+ *   - the caller owns every host array; calls are synchronous on return;
+ *   - a context is not thread-safe: one host thread drives it.
+ * Every call returns an eq status; eq_last_error() gives the message.  The
+ * Julia binding maps a non-zero status to `@error` + `return nothing`, as the
+ * reference does; EQ_NR_NOCONV is informational (NRsearch! returns its last
+ * iterate, src/NRsearch_serial.jl:77-85).
+ */
+#ifndef EQCYCLE_HIP_H
+#define EQCYCLE_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  EQ_OK = 0,
+  EQ_EINVAL = 1,      /* bad argument, unsupported mesh or material */
+  EQ_HIP_ERR = 2,     /* HIP runtime error */
+  EQ_NR_NOCONV = 3,   /* a Newton search ended at |delta| > 1e10 or k == 1000 */
+  EQ_NR_NAN = 4,      /* FBC! guard: Vf > 1e10 or NaN (src/NRsearch_serial.jl:19-32) */
+  EQ_PCG_NOCONV = 5,  /* CG stopped at maxiter above tolerance */
+  EQ_ENOTREADY = 6,   /* a required set-up call is missing */
+  EQ_NOMEM = 7
+};
+
+enum { EQ_MASK_ALL = 0, EQ_MASK_INTERIOR = 1, EQ_MASK_NEG = 2 };
+enum { EQ_PREC_FP64 = 0, EQ_PREC_FP32 = 1, EQ_PREC_MIXED = 2 };
+enum { EQ_PC_NONE = 0, EQ_PC_JACOBI = 1, EQ_PC_AMG = 2 };
+enum { EQ_SMOOTH_JACOBI = 0, EQ_SMOOTH_CHEBYSHEV = 1 };
+enum { EQ_GUESS_PREVIOUS = 0, EQ_GUESS_LEARNED = 1 };
+
+typedef struct eq_ctx eq_ctx;
+
+/* Mesh of setParameters (par.jl:20-36, 99-106). */
+typedef struct {
+  int64_t ngll;            /* NGLL; must be 5 */
+  int64_t nelx, nely;      /* NelX, NelY */
+  double dxe, dye;         /* element sizes */
+  const int64_t *iglob;    /* [ngll*ngll*Nel] MeshBox! numbering (src/MeshBox.jl:27-88) */
+  const double *xgll;      /* GetGLL(5) nodes   (src/GetGLL.jl:14-24) */
+  const double *wgll;      /* GetGLL(5) weights */
+  const double *H;         /* GetGLL(5) H, column-major, H[i,j] = h'_i(xgll[j]) */
+} eq_mesh_desc;
+
+/* params_int / params_float / params_farray / params_iarray (par.jl:236-301). */
+typedef struct {
+  int64_t FltNglob, nglob, IDstate;
+  double Vpl, Vthres, Vevne, dt0;
+  const double *fo, *Vo, *xLf;     /* [FltNglob] */
+  const double *M;                 /* [nglob] lumped mass incl. absorbing terms */
+  const double *BcLC;              /* [n_iBcL] */
+  const double *BcTC;              /* [n_iBcT] */
+  const double *FltB, *FltZ, *FltX, *cca, *ccb, *Seff, *tauo, *XiLf;  /* [FltNglob] */
+  const int64_t *iFlt;             /* [FltNglob] */
+  const int64_t *iBcL; int64_t n_iBcL;
+  const int64_t *iBcT; int64_t n_iBcT;
+  const int64_t *FltIglobBC; int64_t n_FltIglobBC;
+  const int64_t *out_seis; int64_t n_out_seis;
+} eq_params_desc;
+
+/* ---- context ------------------------------------------------------------ */
+/* Builds the device mesh from setParameters' output (par.jl:99-106). */
+int eq_create(const eq_mesh_desc *mesh, eq_ctx **out);
+void eq_destroy(eq_ctx *ctx);
+const char *eq_last_error(const eq_ctx *ctx);
+/* Number of global nodes, interior (non-fault) nodes. */
+int eq_sizes(const eq_ctx *ctx, int64_t *nglob, int64_t *nI, int64_t *nfault);
+
+/* W[5,5,Nel] = wgll2 .* mu (src/MaterialProperties.jl:3-21,77-128).  mu must
+ * be nodal (equal on nodes shared by elements), as in mat_trap / mat_gauss and
+ * any homogeneous medium; otherwise EQ_EINVAL. */
+int eq_set_material(eq_ctx *ctx, const double *W);
+/* The params_* arrays used by the fault and time-stepping routines. */
+int eq_set_params(eq_ctx *ctx, const eq_params_desc *p);
+
+/* ---- stiffness operator ------------------------------------------------- */
+/* y = K u with K = stiffness_assembly (src/Kassemble.jl:3-63), matrix free;
+ * replaces Ksparse'*u (src/main.jl:246,267).  mask: EQ_MASK_ALL (K u),
+ * EQ_MASK_NEG (-K u = nKsparse'*u, src/main.jl:311), EQ_MASK_INTERIOR (fault
+ * entries of u read as 0 and of y set to 0).  prec: EQ_PREC_FP64 / _FP32. */
+int eq_ku_apply(eq_ctx *ctx, const double *u, double *y, int mask, int prec);
+/* y = kni * x, kni = -K[FltNI,FltNI] (src/main.jl:195,211); x, y in FltNI order. */
+int eq_kni_apply(eq_ctx *ctx, const double *x, double *y, int prec);
+/* a_elem[5,5,Nel] = element_computation!(u) (src/PCG.jl:71-86). */
+int eq_element_computation(eq_ctx *ctx, const double *u, double *a_elem);
+
+/* ---- AMG preconditioner -------------------------------------------------- */
+/* ruge_stuben(kni) + aspreconditioner(ml) (src/main.jl:200-201): Ruge-Stueben
+ * hierarchy built on the host, uploaded, applied as one V-cycle on the GPU
+ * with Jacobi or Chebyshev smoothing.  strength_abs = 1 is the classical
+ * |a_ij| test of AlgebraicMultigrid.jl; 0 uses -a_ij. */
+int eq_amg_setup(eq_ctx *ctx, double theta, int max_levels, int64_t max_coarse, int strength_abs,
+                 int smoother, int smooth_degree);
+/* Per-level sizes: n[l], nnz[l] for l < *nlev (arrays of capacity cap). */
+int eq_amg_info(const eq_ctx *ctx, int *nlev, int64_t *n, int64_t *nnz, int cap);
+/* x = Pl \ b for kni (ldiv!(x, p, b)); x, b in FltNI order. */
+int eq_amg_ldiv(eq_ctx *ctx, const double *b, double *x, int prec);
+
+/* ---- conjugate gradients ------------------------------------------------- */
+/* cg!(x, kni, b; Pl, tol, maxiter) (src/main.jl:256): stops when
+ * ||b - kni x||_2 <= tol ||b||_2 or after maxiter iterations.  x, b in FltNI
+ * order.  pc: EQ_PC_NONE / _JACOBI / _AMG; prec: _FP64 / _FP32 / _MIXED
+ * (FP32 inner CG inside FP64 iterative refinement). */
+int eq_cg(eq_ctx *ctx, double *x, const double *b, double tol, int64_t maxiter, int pc, int prec,
+          int64_t *iters, double *relres);
+/* PCG!(...) (src/PCG.jl:8-67): Jacobi CG on K_II with rhs -(K F)[FltNI];
+ * F is [nglob] (fault entries used), dnew [nI] in FltNI order, in/out. */
+int eq_pcg_jacobi(eq_ctx *ctx, const double *F, double *dnew, double tol, int64_t maxit, int64_t *iters);
+
+/* ---- fault physics (arrays in fault order, length FltNglob unless n) ---- */
+/* IDS! (src/otherFunctions_serial.jl:5-32), element-wise over n nodes. */
+int eq_ids(eq_ctx *ctx, int64_t n, const double *xLf, const double *Vo, const double *psi, double dt,
+           const double *Vf, double cnd, int64_t IDstate, double *psi1);
+/* IDS2! (src/otherFunctions_serial.jl:35-59). */
+int eq_ids2(eq_ctx *ctx, int64_t n, const double *xLf, const double *Vo, const double *psi,
+            const double *psi1, double dt, const double *Vf, const double *Vf1, int64_t IDstate,
+            double *psi2);
+/* NRsearch! (src/NRsearch_serial.jl:47-98), element-wise; k_out[i] = Newton
+ * iterations (negative when the non-convergence exit was taken). */
+int eq_nrsearch(eq_ctx *ctx, int64_t n, const double *fo, const double *Vo, const double *cca,
+                const double *ccb, const double *Seff, const double *tau, const double *tauo,
+                const double *psi, const double *FltZ, const double *FltVfree, double *Vf_out,
+                double *tau_out, int32_t *k_out);
+/* FBC! (src/NRsearch_serial.jl:6-45) over j = NFBC..FltNglob using the
+ * context's params; on the guard returns EQ_NR_NAN and *fail_node = j. */
+int eq_fbc(eq_ctx *ctx, int64_t IDstate, int64_t NFBC, int64_t FltNglob, double *psi1, double *Vf1,
+           double *tau1, double *psi2, double *Vf2, double *tau2, const double *psi, const double *Vf,
+           const double *FltVfree, double dt, int64_t *fail_node);
+/* slrFunc! (src/otherFunctions_serial.jl:62-83). */
+int eq_slr(eq_ctx *ctx, int64_t NFBC, int64_t FltNglob, const double *psi, double *psi1,
+           const double *Vf, double *Vf1, int64_t IDstate, const double *tau1, double dt);
+/* dtevol! (src/dtevol.jl:5-44). */
+int eq_dtevol(eq_ctx *ctx, double dt, double dtmin, const double *XiLf, int64_t FaultNglob, int64_t NFBC,
+              const double *Vf, int64_t isolver, double *dt_out);
+
+/* ---- device-resident time loop (main(P), src/main.jl:67-476) ----------- */
+typedef struct {
+  double tol;              /* cg! tol (src/main.jl:256 uses 1e-6) */
+  int64_t maxiter;         /* 0: nI, as cg! */
+  int pc;                  /* EQ_PC_* */
+  int prec;                /* EQ_PREC_FP64 or EQ_PREC_MIXED */
+  int guess;               /* EQ_GUESS_PREVIOUS (src/main.jl:241) or EQ_GUESS_LEARNED */
+  int record_snapshots;    /* copy fault fields at events (src/main.jl:361-422) */
+} eq_solver_opts;
+int eq_set_solver(eq_ctx *ctx, const eq_solver_opts *opts);
+
+/* Initial state (src/main.jl:99-190): v = v_init everywhere then -Vpl/2,
+ * d = a = 0, psi = psi_init (NULL: the formula of src/main.jl:125). */
+int eq_state_init(eq_ctx *ctx, double v_init, const double *psi_init);
+/* Copy state out (reference numbering); any pointer may be NULL. */
+int eq_state_get(eq_ctx *ctx, double *d, double *v, double *a, double *psi, double *tau, double *Vf,
+                 double *t, double *dt, int64_t *it, int64_t *isolver);
+/* Replace the nodal state (reference numbering); any pointer may be NULL. */
+int eq_state_set(eq_ctx *ctx, const double *d, const double *v, const double *a, const double *psi,
+                 const double *tau, double t, double dt, int64_t isolver);
+
+typedef struct {
+  double t;                /* time after the step */
+  double dt_used;          /* step length */
+  double dt_next;          /* dtevol! result */
+  double Vfmax;            /* 2 max(v[iFlt]) + Vpl (src/main.jl:348) */
+  int64_t it;              /* step counter */
+  int64_t isolver_used;    /* 1 quasi-static, 2 dynamic */
+  int64_t isolver_next;
+  int64_t pcg_iters[2];    /* CG iterations of the two QS passes */
+  int64_t nr_noconv;       /* Newton searches that hit the failure exit */
+} eq_step_info;
+/* One pass of the while-loop body (src/main.jl:221-451) without output
+ * bookkeeping; host reads back only the step scalars. */
+int eq_step(eq_ctx *ctx, eq_step_info *info);
+
+/* Output arrays of main(P) (struct results, src/main.jl:479-498): the caller
+ * allocates each with the given capacity; counts are returned.  Fault
+ * snapshot matrices are [FltNglob x cap] column-major and may be NULL. */
+typedef struct {
+  int64_t cap_steps, cap_events, cap_is, cap_seis, cap_rec;
+  double *time_, *Vfmax; int64_t *isolver; int64_t *pcg_iters;  /* [cap_steps] ([2*cap_steps] iters) */
+  double *tStart, *tEnd, *hypo;                       /* [cap_events] */
+  double *taubefore, *tauafter, *delfafter;           /* [FltNglob x cap_events] */
+  double *is_slip, *is_slipvel, *is_stress;           /* [FltNglob x cap_is] */
+  double *seismic_slip, *seismic_slipvel, *seismic_stress;  /* [FltNglob x cap_seis] */
+  double *dSeis, *vSeis, *aSeis;                      /* [cap_rec x n_out_seis] column-major */
+  int64_t n_steps, n_start, n_end, n_is, n_seis, n_rec;
+  int64_t n_qs, n_dyn, total_pcg_iters, nr_noconv;
+  double wall_qs, wall_dyn;                           /* seconds spent per step type */
+} eq_results;
+/* while t < total_time (and it < max_steps if > 0): eq_step + bookkeeping. */
+int eq_run(eq_ctx *ctx, double total_time, int64_t max_steps, eq_results *res);
+
+/* ---- measurement hook (no reference counterpart) ------------------------ */
+enum { EQ_TIME_KU = 0, EQ_TIME_CG_DIR = 1, EQ_TIME_VCYCLE = 2, EQ_TIME_CG_ITER = 3, EQ_TIME_NEWMARK = 4 };
+/* Average device time (ms) of one launch of the given kernel group over reps
+ * back-to-back launches, bracketed by HIP events on the context's stream.
+ * Runs on scratch vectors; EQ_TIME_NEWMARK advances a copy of the state and
+ * must not be called in the middle of a run. */
+int eq_time_kernel(eq_ctx *ctx, int which, int prec, int reps, double *avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,259 @@
+"""Pin the CPU oracle: reference fixtures, closed forms, brute force, direct solves.
+
+Each test is chosen so that a plausible restatement mistake (a swapped
+coefficient, a dropped term, a wrong index or sign) fails it.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from eqcycle_amd.configs import CONFIGS, small
+from eqcycle_amd.gll import gll
+from eqcycle_amd.params import meshbox_numbering, normal, set_parameters, splitmix64
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+# ---------------------------------------------------------------------------
+# GLL (src/GetGLL.jl + src/gll_xwh/gll_05.tab)
+# ---------------------------------------------------------------------------
+def test_gll_matches_reference_table():
+    g = json.load(open(os.path.join(GOLD, "gll_05.json")))
+    x, w, H = gll(5)
+    assert np.max(np.abs(x - g["xgll"])) < 2e-16
+    assert np.max(np.abs(w - g["wgll"])) < 2e-16
+    assert np.max(np.abs(H - np.array(g["H"]))) < 5e-15
+
+
+def test_gll_closed_forms():
+    x, w, H = gll(5)
+    assert np.allclose(x, [-1, -np.sqrt(3 / 7), 0, np.sqrt(3 / 7), 1], atol=2.5e-16, rtol=0)
+    assert np.allclose(w, [1 / 10, 49 / 90, 32 / 45, 49 / 90, 1 / 10], atol=2e-16, rtol=0)
+    assert H[0, 0] == -5.0 and H[4, 4] == 5.0
+    # H[i, j] = h'_i(x_j): differentiates polynomials of degree <= 4 exactly
+    for k in range(5):
+        d = (x ** k) @ H                       # sum_i x_i^k h'_i(x_j)
+        assert np.allclose(d, k * x ** (k - 1) if k else 0 * x, atol=1e-13)
+    # quadrature exact to degree 7
+    for k in range(8):
+        assert abs(w @ x ** k - (0 if k % 2 else 2 / (k + 1))) < 1e-15
+
+
+# ---------------------------------------------------------------------------
+# mesh numbering (src/MeshBox.jl)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("nx,ny", [(1, 1), (2, 2), (3, 5), (8, 4), (40, 40)])
+def test_meshbox_numbering(oracle, nx, ny):
+    P = set_parameters(small(nx, ny) if (nx, ny) != (40, 40) else CONFIGS["cfg1"])
+    ig, x, y = oracle.meshbox(5, P.Nel, P.NelX, P.NelY, P.FltNglob, P.dxe, P.dye, P.xgll)
+    assert np.array_equal(np.transpose(ig, (2, 1, 0)), P.iglob)
+    assert np.array_equal(x - P.LX, P.x)
+    assert np.array_equal(y, P.y)
+    ref = meshbox_numbering(nx, ny)
+    assert np.array_equal(np.sort(ref.ravel()), np.arange(1, P.nglob + 1))
+
+
+def test_splitmix64_c_and_numpy_agree(oracle):
+    assert np.array_equal(oracle.splitmix64(12345, 100), splitmix64(12345, 100))
+
+
+# ---------------------------------------------------------------------------
+# stiffness operator (src/Kassemble.jl, src/PCG.jl:71-86)
+# ---------------------------------------------------------------------------
+def _K(oracle, P):
+    return oracle.assemble_K(P.NelX, P.NelY, P.nglob, P.iglob, P.W, P.H, P.dxe, P.dye)
+
+
+def test_K_symmetric_and_constants_in_nullspace(oracle):
+    P = set_parameters(small(6, 4, material="trapezoid", lognormal_sigma=0.2))
+    Ks = _K(oracle, P).to_scipy()
+    scale = abs(Ks).max()
+    assert abs(Ks - Ks.T).max() < 1e-15 * scale
+    assert np.max(np.abs(Ks @ np.ones(P.nglob))) < 1e-14 * scale
+    # positive semi-definite, rank nglob-1
+    ev = np.linalg.eigvalsh(Ks.toarray())
+    assert ev[0] > -1e-9 * ev[-1] and ev[1] > 1e-9 * ev[-1]
+
+
+def test_K_energy_exact_for_cubic_fields(oracle):
+    """u^T K u = integral of mu |grad u|^2 for u of degree <= 3 per variable
+    (GLL-5 quadrature is exact); the mesh is not square so a swapped c1/c2 fails."""
+    P = set_parameters(small(3, 5))
+    mu = P.rho[0] * P.cfg.vs ** 2
+    x, y = P.x + P.LX, P.y
+    u = x ** 3 - 2 * x * y ** 2 + 3 * y ** 3 + x * y
+    E = u @ _K(oracle, P).matvec(u)
+    from numpy.polynomial import polynomial as Pl
+    LX, LY = P.LX, P.LY
+    # u_x = 3x^2 - 2y^2 + y ; u_y = -4xy + 9y^2 + x
+    def integ(f, n=12):
+        gx, gw = np.polynomial.legendre.leggauss(n)
+        X = 0.5 * LX * (gx + 1)
+        Y = 0.5 * LY * (gx + 1)
+        XX, YY = np.meshgrid(X, Y, indexing="ij")
+        return 0.25 * LX * LY * np.einsum("i,j,ij->", gw, gw, f(XX, YY))
+    exact = mu * integ(lambda X, Y: (3 * X ** 2 - 2 * Y ** 2 + Y) ** 2 + (-4 * X * Y + 9 * Y ** 2 + X) ** 2)
+    assert abs(E - exact) < 1e-12 * abs(exact)
+
+
+def test_single_element_matrix_free_equals_Ke(oracle):
+    """The src/dump.jl check: element_computation! on one element equals Ke * u."""
+    P = set_parameters(small(1, 1, lognormal_sigma=0.3))
+    K = _K(oracle, P)
+    u = normal(3, P.nglob)
+    c1 = (0.5 * P.dxe * 0.5 * P.dye) / (0.5 * P.dxe) ** 2
+    c2 = (0.5 * P.dxe * 0.5 * P.dye) / (0.5 * P.dye) ** 2
+    ig = P.iglob[0] - 1
+    a = oracle.element_ku(u[ig].T, P.W[0].T, P.H, c1, c2)
+    Ku = K.matvec(u)
+    assert np.max(np.abs(a.T - Ku[ig])) < 1e-13 * np.max(np.abs(Ku))
+
+
+def test_mass_and_fault_weights():
+    P = set_parameters(small(4, 6))
+    half_dt = 0.5 * P.dt0
+    M0 = P.M.copy()
+    M0[P.iBcL - 1] -= half_dt * P.BcLC
+    M0[P.iBcT - 1] -= half_dt * P.BcTC
+    assert abs(M0.sum() - P.cfg.rho * P.LX * P.LY) < 1e-12 * M0.sum()
+    assert abs(P.FltB.sum() - P.LX) < 1e-9
+    assert abs(P.BcLC.sum() - P.cfg.rho * P.cfg.vs * P.LY) < 1e-6 * P.BcLC.sum()
+
+
+# ---------------------------------------------------------------------------
+# solvers
+# ---------------------------------------------------------------------------
+def test_amg_pcg_matches_direct_solve(oracle):
+    import scipy.sparse.linalg as spla
+    P = set_parameters(small(16, 12, material="trapezoid"))
+    kni = _K(oracle, P).submatrix(P.FltNI - 1, -1.0)
+    A = kni.to_scipy().tocsc()
+    b = normal(9, A.shape[0])
+    xs = spla.spsolve(A, b)
+    ml = oracle.Amg(kni)
+    assert ml.nlev >= 2
+    x, it, rr, ok = oracle.cg(kni, np.zeros_like(b), b, ml, 1e-10)
+    assert ok and rr <= 1e-10
+    assert np.linalg.norm(x - xs) / np.linalg.norm(xs) < 1e-6
+    # Galerkin coarse operators: A_{l+1} = P^T A_l P
+    for l in range(ml.nlev - 1):
+        Al, Pl, Al1 = ml.level_csr(l, 0), ml.level_csr(l, 1), ml.level_csr(l + 1, 0)
+        assert abs(Pl.T @ Al @ Pl - Al1).max() < 1e-10 * abs(Al1).max()
+
+
+def test_amg_preconditioner_symmetric(oracle):
+    P = set_parameters(small(8, 8))
+    kni = _K(oracle, P).submatrix(P.FltNI - 1, -1.0)
+    ml = oracle.Amg(kni)
+    x, y = normal(1, kni.n), normal(2, kni.n)
+    assert abs(x @ ml.apply(y) - y @ ml.apply(x)) < 1e-10 * abs(x @ ml.apply(y))
+
+
+def test_pcg_jacobi_converges(oracle):
+    import scipy.sparse.linalg as spla
+    P = set_parameters(small(8, 6))
+    K = _K(oracle, P)
+    Kii = K.submatrix(P.FltNI - 1, 1.0)
+    F = np.zeros(P.nglob)
+    F[P.iFlt - 1] = normal(4, P.FltNglob)
+    Fnew = -K.matvec(F)[P.FltNI - 1]
+    x, it, ok = oracle.pcg_jacobi(Kii, Kii.to_scipy().diagonal().copy(), np.zeros(Kii.n), Fnew, 1e-10)
+    xs = spla.spsolve(Kii.to_scipy().tocsc(), Fnew)
+    assert ok and np.linalg.norm(x - xs) / np.linalg.norm(xs) < 1e-7
+
+
+# ---------------------------------------------------------------------------
+# fault physics
+# ---------------------------------------------------------------------------
+def test_nrsearch_fp64_matches_bigfloat_sequence(oracle):
+    g = json.load(open(os.path.join(GOLD, "nr_ids_cases.json")))
+    for c in g["nrsearch"]:
+        Vf, tau, k = oracle.nrsearch(c["fo"], c["Vo"], c["cca"], c["ccb"], c["Seff"], c["tau0"], c["tauo"],
+                                     c["psi"], c["FltZ"], c["FltVfree"])
+        assert k == c["k"]
+        assert abs(tau - c["tau"]) <= 1e-9 * max(abs(c["tau"]), 1.0)
+        assert abs(Vf - c["Vf"]) <= 1e-8 * abs(c["Vf"])
+
+
+def test_nrsearch_solution_properties(oracle):
+    """At the returned tau: Vf follows the regularised law exactly, whose inverse is
+    tau = (a sigma / fact) asinh(Vf e^{-h} / (2 V0)); the Newton residual is below
+    the step tolerance's scale."""
+    g = json.load(open(os.path.join(GOLD, "nr_ids_cases.json")))
+    for c in g["nrsearch"]:
+        Vf, tau, k = oracle.nrsearch(c["fo"], c["Vo"], c["cca"], c["ccb"], c["Seff"], c["tau0"], c["tauo"],
+                                     c["psi"], c["FltZ"], c["FltVfree"])
+        fact = 1 + (c["Vo"] / 1e10) * np.exp(-c["psi"])
+        h = -(c["fo"] + c["ccb"] * c["psi"]) / c["cca"]
+        tau_inv = (c["cca"] * c["Seff"] / fact) * np.arcsinh(Vf * np.exp(-h) / (2 * c["Vo"]))
+        assert abs(tau_inv - tau) < 1e-6 * max(abs(tau), 1.0)
+        resid = c["FltZ"] * c["FltVfree"] - c["FltZ"] * Vf + c["tauo"] - tau
+        assert abs(resid) < 0.001 * c["cca"] * c["Seff"] * (1 + c["FltZ"] * Vf / (c["cca"] * c["Seff"])) * 10
+
+
+def test_ids_matches_exact_aging_law(oracle):
+    g = json.load(open(os.path.join(GOLD, "nr_ids_cases.json")))
+    for c in g["ids"]:
+        got = oracle.ids(c["xLf"], c["Vo"], c["psi"], c["dt"], c["Vf"], 1e-6, 2)
+        VdtL = abs(c["Vf"]) * c["dt"] / c["xLf"]
+        tol = 1e-13 if VdtL >= 1e-6 else 1e-9   # Taylor branch below cnd
+        assert abs(got - c["psi1"]) <= tol * max(1.0, abs(c["psi1"]))
+
+
+def test_ids_against_ode_integration(oracle):
+    """dpsi/dt = (V0/Dc) e^{-psi} - V/Dc integrated with RK4 reproduces IDS!."""
+    xLf, Vo = 8e-3, 1e-6
+    for psi0, V, dt in [(10.0, 1e-9, 3e6), (12.0, 1e-3, 5.0), (8.0, 0.5, 0.02)]:
+        f = lambda p: (Vo / xLf) * np.exp(-p) - V / xLf
+        p, n = psi0, 20000
+        h = dt / n
+        for _ in range(n):
+            k1 = f(p); k2 = f(p + h * k1 / 2); k3 = f(p + h * k2 / 2); k4 = f(p + h * k3)
+            p += h * (k1 + 2 * k2 + 2 * k3 + k4) / 6
+        assert abs(oracle.ids(xLf, Vo, psi0, dt, V, 1e-6, 2) - p) < 1e-8
+
+
+def test_ids2_reduces_to_ids_for_equal_rates(oracle):
+    for psi, V, dt in [(10.0, 1e-9, 3e6), (12.0, 1e-3, 5.0), (9.0, 1e-12, 10.0)]:
+        a = oracle.ids2(8e-3, 1e-6, psi, 0.0, dt, V, V, 2)
+        b = oracle.ids(8e-3, 1e-6, psi, dt, V, 1e-6, 2)
+        assert abs(a - b) < 1e-14 * abs(b)
+
+
+def test_dtevol_cases(oracle):
+    XiLf = np.array([1.0, 4e-3, 4e-3, 2e-3])
+    dtmax = 50 * 24 * 3600.0
+    # creeping node 0 ignored (NFBC = 2); node 3 sets the step
+    Vf = np.array([10.0, 1e-9, 1e-12, 1e-8])
+    assert oracle.dtevol(1e9, 0.01, XiLf, 4, 2, Vf, 1) == pytest.approx(2e-3 / 1e-8, rel=1e-15)
+    # growth limited to 1.2 dt
+    assert oracle.dtevol(10.0, 0.01, XiLf, 4, 2, Vf, 1) == 12.0
+    # capped at dtmax when everything is slow
+    assert oracle.dtevol(1e9, 0.01, XiLf, 4, 2, np.full(4, 1e-20), 1) == dtmax
+    # never below dtmin; dynamic regime uses dtmin
+    assert oracle.dtevol(1e9, 0.5, XiLf, 4, 2, np.full(4, 1.0), 1) == 0.5
+    assert oracle.dtevol(1e9, 0.5, XiLf, 4, 2, Vf, 2) == 0.5
+
+
+def test_fbc_guard(oracle):
+    P = set_parameters(small(4, 2))
+    n = P.FltNglob
+    Vf = np.full(n, 1e-9)
+    Vf[P.NFBC + 1] = np.nan
+    outs = [np.zeros(n) for _ in range(6)]
+    st, _ = oracle.fbc(2, P, P.NFBC, n, *outs, np.full(n, 7.0), Vf, np.full(n, 1e-9), 0.01)
+    assert st == P.NFBC + 2   # 1 + 0-based index of the failing node
+
+
+def test_quasi_static_step_direct_vs_amg(oracle):
+    P = set_parameters(small(32, 8, material="trapezoid"))
+    a = oracle.Simulation(P, tol=1e-12, solver="amg")
+    b = oracle.Simulation(P, tol=1e-12, solver="direct")
+    for _ in range(5):
+        a.step()
+        b.step()
+    assert np.max(np.abs(a.d - b.d)) < 1e-9 * np.max(np.abs(b.d))
+    assert np.max(np.abs(a.psi - b.psi)) < 1e-10 * np.max(np.abs(b.psi))
+    assert a.dt == pytest.approx(b.dt, rel=1e-10)
