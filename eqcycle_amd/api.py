@@ -121,9 +121,19 @@ class EqContext:
         return out
 
     # ---- AMG / CG ----------------------------------------------------------
-    def amg_setup(self, theta=0.25, max_levels=10, max_coarse=10, strength_abs=1, smoother="jacobi", degree=1):
+    def amg_setup(self, hierarchy="pmg", theta=0.25, max_levels=10, max_coarse=None, strength_abs=1,
+                  smoother=None, degree=3):
+        """hierarchy "rs": ruge_stuben on K_II with Jacobi smoothing (the reference's
+        preconditioner structure); "pmg": degree-4 -> bilinear first level with
+        Chebyshev smoothing, then Ruge-Stueben (the default, far fewer iterations)."""
+        pc = {"rs": 0, "pmg": 1}[hierarchy]
+        if smoother is None:
+            smoother = "chebyshev" if pc else "jacobi"
+        if max_coarse is None:
+            max_coarse = 200 if pc else 10
         sm = {"jacobi": L.EQ_SMOOTH_JACOBI, "chebyshev": L.EQ_SMOOTH_CHEBYSHEV}[smoother]
-        self._check(self.lib.eq_amg_setup(self.h, theta, max_levels, max_coarse, strength_abs, sm, degree))
+        self._check(self.lib.eq_amg_setup(self.h, theta, max_levels, max_coarse, strength_abs, sm, degree, pc))
+        self.hierarchy = hierarchy
         return self.amg_info()
 
     def amg_info(self):
@@ -289,7 +299,7 @@ class EqContext:
     def time_kernel(self, which="ku", prec="fp64", reps=50):
         """Average device ms per launch, HIP events on the library's stream."""
         w = {"ku": L.EQ_TIME_KU, "cg_dir": L.EQ_TIME_CG_DIR, "vcycle": L.EQ_TIME_VCYCLE,
-             "cg_iter": L.EQ_TIME_CG_ITER, "newmark": L.EQ_TIME_NEWMARK}[which]
+             "cg_iter": L.EQ_TIME_CG_ITER, "newmark": L.EQ_TIME_NEWMARK, "cheb": L.EQ_TIME_CHEB}[which]
         out = C.c_double()
         self._check(self.lib.eq_time_kernel(self.h, w, PREC[prec], reps, C.byref(out)))
         return out.value

@@ -20,7 +20,6 @@
 
 namespace eq {
 
-static inline double wl(const Geom& g, int idx) { return g.w[idx & 3]; }
 
 // Sum of GLL weights of the elements that contain grid line L (of nl lines).
 static inline double wline(const Geom& g, int L, int nl) {
@@ -38,8 +37,9 @@ void assemble_kii(const Geom& g, const double* mu, HostCsr* K) {
   K->val.clear();
   K->col.reserve(nI * 13);
   K->val.reserve(nI * 13);
-  auto gx = [&](int I, int J) { return g.c1 * wline(g, J, NY) * wl(g, I) * mu[(int64_t)J * NX + I]; };
-  auto gy = [&](int I, int J) { return g.c2 * wline(g, I, NX) * wl(g, J) * mu[(int64_t)J * NX + I]; };
+  // gx / gy: line coefficient times the GLL weight of local index p in its segment
+  auto gx = [&](int I, int J, int p) { return g.c1 * wline(g, J, NY) * g.w[p] * mu[(int64_t)J * NX + I]; };
+  auto gy = [&](int I, int J, int q) { return g.c2 * wline(g, I, NX) * g.w[q] * mu[(int64_t)J * NX + I]; };
   int64_t cols[32];
   double vals[32];
   for (int J = 1; J < NY; J++) {
@@ -58,7 +58,7 @@ void assemble_kii(const Geom& g, const double* mu, HostCsr* K) {
         int i = I - 4 * s;
         for (int k = 0; k < 5; k++) {
           double v = 0;
-          for (int p = 0; p < 5; p++) v += g.D[p * 5 + i] * g.D[p * 5 + k] * gx(4 * s + p, J);
+          for (int p = 0; p < 5; p++) v += g.D[p * 5 + i] * g.D[p * 5 + k] * gx(4 * s + p, J, p);
           add((int64_t)J * NX + 4 * s + k, v);
         }
       }
@@ -69,7 +69,7 @@ void assemble_kii(const Geom& g, const double* mu, HostCsr* K) {
         int j = J - 4 * t;
         for (int l = 0; l < 5; l++) {
           double v = 0;
-          for (int q = 0; q < 5; q++) v += g.D[q * 5 + j] * g.D[q * 5 + l] * gy(I, 4 * t + q);
+          for (int q = 0; q < 5; q++) v += g.D[q * 5 + j] * g.D[q * 5 + l] * gy(I, 4 * t + q, q);
           int Jc = 4 * t + l;
           if (Jc == 0) continue;  // fault column: not an unknown
           add((int64_t)Jc * NX + I, v);
@@ -365,6 +365,81 @@ int amg_build(const HostCsr& A0, const AmgOptions& opt, HostHierarchy* H, std::s
     if (err) *err = "AMG: singular coarsest operator";
     return EQ_EINVAL;
   }
+  return EQ_OK;
+}
+
+}  // namespace eq
+
+namespace eq {
+
+// p-multigrid first step: bilinear (Q1) functions on the element vertices
+// embedded in the degree-4 GLL space.  Rows: K_II unknowns (structured
+// interior order), columns: vertices (ex, ey) with ey >= 1 (vertices on the
+// fault row carry the Dirichlet data and are not unknowns).
+void build_pmg_p0(const Geom& g, HostCsr* P) {
+  const int NX = g.NX, NY = g.NY;
+  const int nvx = g.nelx + 1;
+  const int64_t nI = (int64_t)NX * (NY - 1);
+  P->n = nI;
+  P->m = (int64_t)nvx * g.nely;
+  P->rowptr.assign(nI + 1, 0);
+  P->col.clear();
+  P->val.clear();
+  for (int J = 1; J < NY; J++) {
+    const int ey = std::min(J / 4, g.nely - 1), j = J - 4 * ey;
+    const double wy[2] = {0.5 * (1.0 - g.x[j]), 0.5 * (1.0 + g.x[j])};
+    for (int I = 0; I < NX; I++) {
+      const int ex = std::min(I / 4, g.nelx - 1), i = I - 4 * ex;
+      const double wx[2] = {0.5 * (1.0 - g.x[i]), 0.5 * (1.0 + g.x[i])};
+      int32_t cc[4];
+      double vv[4];
+      int nc = 0;
+      for (int b = 0; b < 2; b++)
+        for (int a = 0; a < 2; a++) {
+          const double w = wx[a] * wy[b];
+          const int vy = ey + b;
+          if (vy == 0 || w == 0.0) continue;
+          cc[nc] = (int32_t)((int64_t)(vy - 1) * nvx + ex + a);
+          vv[nc] = w;
+          nc++;
+        }
+      // columns ascending
+      for (int p = 1; p < nc; p++)
+        for (int q = p; q > 0 && cc[q] < cc[q - 1]; q--) {
+          std::swap(cc[q], cc[q - 1]);
+          std::swap(vv[q], vv[q - 1]);
+        }
+      for (int p = 0; p < nc; p++) {
+        P->col.push_back(cc[p]);
+        P->val.push_back(vv[p]);
+      }
+      P->rowptr[(int64_t)(J - 1) * NX + I + 1] = (int64_t)P->col.size();
+    }
+  }
+}
+
+int pmg_build(const Geom& g, const HostCsr& Kii, const AmgOptions& opt, HostHierarchy* H, std::string* err) {
+  HostLevel L0;
+  L0.A = Kii;
+  build_pmg_p0(g, &L0.P);
+  csr_transpose(L0.P, &L0.R);
+  HostCsr RA, A1;
+  csr_matmul(L0.R, Kii, &RA);
+  csr_matmul(RA, L0.P, &A1);
+  AmgOptions o = opt;
+  o.max_levels = std::max(1, opt.max_levels - 1);
+  HostHierarchy sub;
+  if (int rc = amg_build(A1, o, &sub, err)) return rc;
+  H->lev.clear();
+  L0.dinv.assign(Kii.n, 0.0);
+  for (int64_t i = 0; i < Kii.n; i++)
+    for (int64_t q = Kii.rowptr[i]; q < Kii.rowptr[i + 1]; q++)
+      if (Kii.col[q] == i) L0.dinv[i] = 1.0 / Kii.val[q];
+  L0.lambda_max = estimate_lambda_max_dinvA(Kii, L0.dinv, 40);
+  H->lev.push_back(std::move(L0));
+  for (auto& L : sub.lev) H->lev.push_back(std::move(L));
+  H->coarse_inv = std::move(sub.coarse_inv);
+  H->ncoarse = sub.ncoarse;
   return EQ_OK;
 }
 

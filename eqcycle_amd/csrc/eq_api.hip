@@ -24,6 +24,7 @@ template <typename T>
 struct CgWork {
   T *r = nullptr, *c = nullptr, *u[2] = {nullptr, nullptr}, *q = nullptr, *x0 = nullptr, *res0 = nullptr;
   T *b = nullptr, *x = nullptr;   // solve-level vectors for the mixed-precision inner solve
+  T *chx = nullptr, *chx2 = nullptr, *chr = nullptr, *chd[2] = {nullptr, nullptr};  // level-0 Chebyshev
   int cur = 0;
 };
 
@@ -93,6 +94,9 @@ struct eq_ctx {
   double* coarse_inv = nullptr;
   int64_t ncoarse = 0;
   std::vector<int64_t> amg_n, amg_nnz;
+  int l0_cheb = 0;                    // level-0 smoother: 0 Jacobi, 1 Chebyshev
+  int l0_degree = 1;
+  double l0_theta = 1, l0_delta = 1;  // Chebyshev interval centre / half-width of D^-1 K_II
   // step status
   StepStatus* st = nullptr;
   StepStatus* st_host = nullptr;
@@ -217,6 +221,7 @@ extern "C" int eq_create(const eq_mesh_desc* mesh, eq_ctx** out) {
   for (int p = 0; p < 5; p++)
     for (int i = 0; i < 5; i++) g.D[p * 5 + i] = mesh->H[p * 5 + i];   // H[i + 5 p] = h'_i(x_p)
   for (int p = 0; p < 5; p++) g.w[p] = mesh->wgll[p];
+  for (int p = 0; p < 5; p++) g.x[p] = mesh->xgll[p];
   // numbering: iglob[i,j,e] -> node (4 ex + i, 4 ey + j)
   ctx->perm.assign(ctx->N, -1);
   std::vector<char> seen(ctx->N, 0);
@@ -286,7 +291,10 @@ static int alloc_work(eq_ctx* ctx, CgWork<T>& w) {
   w.r = ctx->alloc<T>(N); w.c = ctx->alloc<T>(N); w.u[0] = ctx->alloc<T>(N); w.u[1] = ctx->alloc<T>(N);
   w.q = ctx->alloc<T>(N); w.x0 = ctx->alloc<T>(N); w.res0 = ctx->alloc<T>(N); w.b = ctx->alloc<T>(N);
   w.x = ctx->alloc<T>(N);
-  if (!w.r || !w.c || !w.u[0] || !w.u[1] || !w.q || !w.x0 || !w.res0 || !w.b || !w.x)
+  w.chx = ctx->alloc<T>(N); w.chx2 = ctx->alloc<T>(N); w.chr = ctx->alloc<T>(N);
+  w.chd[0] = ctx->alloc<T>(N); w.chd[1] = ctx->alloc<T>(N);
+  if (!w.r || !w.c || !w.u[0] || !w.u[1] || !w.q || !w.x0 || !w.res0 || !w.b || !w.x || !w.chx || !w.chx2 ||
+      !w.chr || !w.chd[0] || !w.chd[1])
     return fail(ctx, EQ_NOMEM, "device allocation failed");
   launch_fill<T>(N, w.x0, T(0), ctx->s);
   launch_fill<T>(N, w.res0, T(0), ctx->s);
@@ -514,11 +522,13 @@ static int alloc_level_vecs(eq_ctx* ctx, DevLevelVecs<T>& v, int64_t n) {
 }
 
 extern "C" int eq_amg_setup(eq_ctx* ctx, double theta, int max_levels, int64_t max_coarse, int strength_abs,
-                            int smoother, int smooth_degree) {
+                            int smoother, int smooth_degree, int pcoarsen) {
   if (!ctx) return EQ_EINVAL;
   if (!ctx->have_material) return fail(ctx, EQ_ENOTREADY, "eq_set_material first");
-  if (smoother != EQ_SMOOTH_JACOBI) return fail(ctx, EQ_EINVAL, "only the Jacobi smoother is available");
-  (void)smooth_degree;
+  if (smoother != EQ_SMOOTH_JACOBI && smoother != EQ_SMOOTH_CHEBYSHEV) return fail(ctx, EQ_EINVAL, "bad smoother");
+  if (smoother == EQ_SMOOTH_CHEBYSHEV && (smooth_degree < 1 || smooth_degree > 8))
+    return fail(ctx, EQ_EINVAL, "Chebyshev degree must be 1..8");
+  if (max_levels < 2) return fail(ctx, EQ_EINVAL, "max_levels must be >= 2");
   HostCsr Kii;
   assemble_kii(ctx->g, ctx->mu_host.data(), &Kii);
   AmgOptions opt;
@@ -528,8 +538,20 @@ extern "C" int eq_amg_setup(eq_ctx* ctx, double theta, int max_levels, int64_t m
   opt.strength_abs = strength_abs;
   HostHierarchy H;
   std::string err;
-  if (int rc = amg_build(Kii, opt, &H, &err)) return fail(ctx, rc, err);
+  if (pcoarsen) {
+    if (int rc = pmg_build(ctx->g, Kii, opt, &H, &err)) return fail(ctx, rc, err);
+  } else {
+    if (int rc = amg_build(Kii, opt, &H, &err)) return fail(ctx, rc, err);
+  }
   if (H.lev.size() < 2) return fail(ctx, EQ_EINVAL, "AMG hierarchy has a single level; use the Jacobi preconditioner");
+  ctx->l0_cheb = smoother == EQ_SMOOTH_CHEBYSHEV;
+  ctx->l0_degree = ctx->l0_cheb ? smooth_degree : 1;
+  {
+    // Chebyshev target interval [lambda_max / 10, 1.1 lambda_max] of D^-1 K_II
+    const double lmax = 1.1 * H.lev[0].lambda_max, lmin = H.lev[0].lambda_max / 10.0;
+    ctx->l0_theta = 0.5 * (lmax + lmin);
+    ctx->l0_delta = 0.5 * (lmax - lmin);
+  }
   ctx->lev.clear();
   ctx->lev.resize(H.lev.size());
   ctx->amg_n.clear();
@@ -616,6 +638,83 @@ const float* ones_of<float>(eq_ctx* c) { return c->ones32; }
 
 // One V-cycle: c = M^-1 r with partial c.r (and the cg! convergence test in
 // its first kernel).
+// Level-0 Chebyshev pre-smoothing from x = 0: k kernels, leaving x_k in chx
+// and the residual b - K_II x_k in chr.
+template <typename T>
+static void cheb_pre(eq_ctx* ctx, const T* b) {
+  CgWork<T>& w = work<T>(ctx);
+  const int k = ctx->l0_degree;
+  const double th = ctx->l0_theta, de = ctx->l0_delta, sg = th / de;
+  double rho_prev = 1.0 / sg;
+  int cur = 0;
+  for (int m = 1; m <= k; m++) {
+    ChebStep<T> st;
+    st.check = (m == 1);
+    st.rin = (m == 1) ? b : w.chr;
+    st.rout = w.chr;
+    if (m == 1) { st.src_mode = 1; st.b = b; st.s0 = (T)(1.0 / th); }
+    else { st.src = w.chd[cur]; }
+    if (m == k) {
+      st.mode = 1;
+      st.write_x_center = (k == 1);
+      st.xout = w.chx;
+    } else {
+      const double rho = 1.0 / (2.0 * sg - rho_prev);
+      st.mode = 0;
+      st.c1 = (T)(rho * rho_prev);
+      st.c2 = (T)(2.0 * rho / de);
+      rho_prev = rho;
+      st.xsrc_center = (m == 1);
+      st.xin = w.chx;
+      st.xout = w.chx;
+      st.dout = w.chd[1 - cur];
+      cur = 1 - cur;
+    }
+    launch_cheb<T>(ctx->g, st, mu_of<T>(ctx), dinv_of<T>(ctx), ctx->part_rr, ctx->npart, ctx->sc, ctx->part_rho,
+                   ctx->s);
+  }
+}
+
+// Level-0 Chebyshev post-smoothing from x = chx: k kernels; output c and
+// partial <c, r>.
+template <typename T>
+static void cheb_post(eq_ctx* ctx, const T* b, T* c) {
+  CgWork<T>& w = work<T>(ctx);
+  const int k = ctx->l0_degree;
+  const double th = ctx->l0_theta, de = ctx->l0_delta, sg = th / de;
+  double rho_prev = 1.0 / sg;
+  int cur = 0;
+  for (int m = 0; m < k; m++) {
+    ChebStep<T> st;
+    st.rin = (m == 0) ? b : w.chr;
+    st.src = (m == 0) ? w.chx : w.chd[cur];
+    if (m == 0) {
+      st.c1 = T(0);
+      st.c2 = (T)(1.0 / th);
+      st.xsrc_center = 1;
+    } else {
+      const double rho = 1.0 / (2.0 * sg - rho_prev);
+      st.c1 = (T)(rho * rho_prev);
+      st.c2 = (T)(2.0 * rho / de);
+      rho_prev = rho;
+      st.xin = w.chx2;
+    }
+    if (m == k - 1) {
+      st.mode = 2;
+      st.xout = c;
+      st.rdot = b;
+    } else {
+      st.mode = 0;
+      st.rout = w.chr;
+      st.dout = w.chd[1 - cur];
+      st.xout = w.chx2;
+      cur = 1 - cur;
+    }
+    launch_cheb<T>(ctx->g, st, mu_of<T>(ctx), dinv_of<T>(ctx), ctx->part_rr, ctx->npart, ctx->sc, ctx->part_rho,
+                   ctx->s);
+  }
+}
+
 template <typename T>
 static void vcycle(eq_ctx* ctx, const T* r, T* c) {
   CgWork<T>& w = work<T>(ctx);
@@ -624,9 +723,16 @@ static void vcycle(eq_ctx* ctx, const T* r, T* c) {
   const int L = (int)ctx->lev.size();
   DevLevel& L0 = ctx->lev[0];
   const T om0 = (T)L0.omega;
-  launch_l0_down<T>(g, r, mu_of<T>(ctx), dinv_of<T>(ctx), om0, w.x0, w.res0, ctx->part_rr, ctx->npart, ctx->sc, s);
-  // restrict to level 1
-  launch_csr<T>(CSR_SPMV, L0.R, w.res0 + g.NX, nullptr, nullptr, T(0), lvecs<T>(ctx->lev[1]).b, nullptr, ctx->sc, s);
+  T* x0 = ctx->l0_cheb ? w.chx : w.x0;
+  if (ctx->l0_cheb) {
+    cheb_pre<T>(ctx, r);
+    launch_csr<T>(CSR_SPMV, L0.R, w.chr + g.NX, nullptr, nullptr, T(0), lvecs<T>(ctx->lev[1]).b, nullptr, ctx->sc,
+                  s);
+  } else {
+    launch_l0_down<T>(g, r, mu_of<T>(ctx), dinv_of<T>(ctx), om0, w.x0, w.res0, ctx->part_rr, ctx->npart, ctx->sc, s);
+    launch_csr<T>(CSR_SPMV, L0.R, w.res0 + g.NX, nullptr, nullptr, T(0), lvecs<T>(ctx->lev[1]).b, nullptr, ctx->sc,
+                  s);
+  }
   for (int l = 1; l <= L - 2; l++) {
     DevLevel& D = ctx->lev[l];
     DevLevelVecs<T>& v = lvecs<T>(D);
@@ -642,9 +748,12 @@ static void vcycle(eq_ctx* ctx, const T* r, T* c) {
                   ctx->sc, s);
     launch_csr<T>(CSR_POST, D.A, v.x, v.b, ldinv<T>(D), (T)D.omega, v.xp, nullptr, ctx->sc, s);
   }
-  launch_csr<T>(CSR_PROLONG_ADD, L0.P, lvecs<T>(ctx->lev[1]).xp, nullptr, nullptr, T(0), w.x0 + g.NX, nullptr,
+  launch_csr<T>(CSR_PROLONG_ADD, L0.P, lvecs<T>(ctx->lev[1]).xp, nullptr, nullptr, T(0), x0 + g.NX, nullptr,
                 ctx->sc, s);
-  launch_l0_post<T>(g, w.x0, r, mu_of<T>(ctx), dinv_of<T>(ctx), om0, c, ctx->part_rho, ctx->sc, s);
+  if (ctx->l0_cheb)
+    cheb_post<T>(ctx, r, c);
+  else
+    launch_l0_post<T>(g, w.x0, r, mu_of<T>(ctx), dinv_of<T>(ctx), om0, c, ctx->part_rho, ctx->sc, s);
 }
 
 // ===========================================================================
@@ -1305,6 +1414,22 @@ static void time_body(eq_ctx* ctx, int which) {
     case EQ_TIME_VCYCLE:
       vcycle<T>(ctx, w.r, w.c);
       break;
+    case EQ_TIME_CHEB: {
+      // one middle Chebyshev step: window d, reads r, x, mu, dinv; writes r, d', x
+      ChebStep<T> st;
+      st.src = w.chd[0];
+      st.rin = w.chr;
+      st.rout = w.chr;
+      st.c1 = T(0.5);
+      st.c2 = T(1e-3);
+      st.xin = w.chx;
+      st.xout = w.chx;
+      st.dout = w.chd[1];
+      st.mode = 0;
+      launch_cheb<T>(g, st, mu_of<T>(ctx), dinv_of<T>(ctx), ctx->part_rr, ctx->npart, ctx->sc, ctx->part_rho,
+                     ctx->s);
+      break;
+    }
     default:
       cg_iteration<T>(ctx, w.x, ctx->have_amg ? EQ_PC_AMG : EQ_PC_JACOBI);
       break;
@@ -1325,6 +1450,17 @@ extern "C" int eq_time_kernel(eq_ctx* ctx, int which, int prec, int reps, double
   } else {
     launch_fill<double>(ctx->N, ctx->w64.x, 1.0, s); launch_fill<double>(ctx->N, ctx->w64.r, 1.0, s);
     launch_fill<double>(ctx->N, ctx->w64.c, 1.0, s); launch_fill<double>(ctx->N, ctx->w64.u[0], 0.0, s);
+  }
+  for (int k = 0; k < 2; k++) {
+    if (prec == EQ_PREC_FP32) {
+      launch_fill<float>(ctx->N, ctx->w32.chd[k], 1.0f, s);
+      launch_fill<float>(ctx->N, ctx->w32.chr, 1.0f, s);
+      launch_fill<float>(ctx->N, ctx->w32.chx, 1.0f, s);
+    } else {
+      launch_fill<double>(ctx->N, ctx->w64.chd[k], 1.0, s);
+      launch_fill<double>(ctx->N, ctx->w64.chr, 1.0, s);
+      launch_fill<double>(ctx->N, ctx->w64.chx, 1.0, s);
+    }
   }
   CgScal h{};
   h.rho_prev = 1.0; h.reltol = -1.0; h.maxiter = INT_MAX;

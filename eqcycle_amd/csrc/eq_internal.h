@@ -23,6 +23,7 @@ struct Geom {
   double c1, c2;      // jac/dx_dxi^2 (x term), jac/dy_deta^2 (y term)
   double D[25];       // D[p*5+i] = h'_i(xi_p)  (= H[i,p])
   double w[5];        // GLL weights
+  double x[5];        // GLL nodes
 };
 
 // Host-side RS-AMG hierarchy (CSR, int32 columns), built by amg_build().
@@ -60,5 +61,9 @@ void csr_transpose(const HostCsr& A, HostCsr* T);
 double estimate_lambda_max_dinvA(const HostCsr& A, const std::vector<double>& dinv, int iters);
 // K_II in structured interior order (reduced index r = n - NX), from nodal mu
 void assemble_kii(const Geom& g, const double* mu_struct, HostCsr* K);
+// p-multigrid: level 0 = K_II with the Q4 -> Q1 vertex prolongation, then RS on
+// the Galerkin Q1 operator.
+void build_pmg_p0(const Geom& g, HostCsr* P);
+int pmg_build(const Geom& g, const HostCsr& Kii, const AmgOptions& opt, HostHierarchy* H, std::string* err);
 
 }  // namespace eq

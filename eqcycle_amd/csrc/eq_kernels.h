@@ -85,6 +85,29 @@ template <typename T>
 void launch_l0_post(const Geom& g, const T* x0, const T* r, const T* mu, const T* dinv, T omega, T* c,
                     double* part_rho, const CgScal* sc, hipStream_t s);
 
+// One Chebyshev smoothing step on the matrix-free level (see OpCheb).
+template <typename T>
+struct ChebStep {
+  const T* src = nullptr;
+  const T* b = nullptr;
+  T s0 = 0;
+  int src_mode = 0;
+  const T* rin = nullptr;
+  T* rout = nullptr;
+  T c1 = 0, c2 = 0;
+  const T* xin = nullptr;
+  int xsrc_center = 0;
+  T* dout = nullptr;
+  T* xout = nullptr;
+  int mode = 0;
+  int write_x_center = 0;
+  const T* rdot = nullptr;
+  int check = 0;
+};
+template <typename T>
+void launch_cheb(const Geom& g, const ChebStep<T>& c, const T* mu, const T* dinv, const double* part_rr, int nrr,
+                 CgScal* sc, double* part_rho, hipStream_t s);
+
 struct DevCsr {
   int64_t n = 0, m = 0, nnz = 0;
   int32_t* rowptr = nullptr;

@@ -336,6 +336,81 @@ struct OpL0Post {
   }
 };
 
+// Chebyshev (Jacobi-preconditioned) smoothing step on the matrix-free level:
+//   w = window vector (d buffer, s0 dinv b for the first pre-smoothing step,
+//       or x for the first post-smoothing step)
+//   r = rin - K_II w
+//   mode 0: d_new = c1 w + c2 dinv r, x_new = xsrc + d_new; write r, d_new, x_new
+//   mode 1: write r only (residual after pre-smoothing; x = w if write_x_center)
+//   mode 2: x_new as in mode 0 is the smoother output c; partial <c, rdot>
+// The first kernel of a V-cycle carries the cg! convergence test.
+template <typename T>
+struct OpCheb {
+  using Pre = PreNone;
+  static constexpr int kParts = 1;
+  const T* src;
+  const T* b;
+  const T* mu;
+  const T* dinv;
+  T s0;
+  int src_mode;      // 0: src[n], 1: s0 dinv[n] b[n]
+  const T* rin;
+  T* rout;
+  T c1, c2;
+  const T* xin;
+  int xsrc_center;   // 1: x source is the window value itself
+  T* dout;
+  T* xout;
+  int mode;
+  int write_x_center;
+  const T* rdot;
+  int check;
+  const double* part_rr;
+  int nrr;
+  CgScal* sc;
+  double* part0;
+  double* part1;
+  __device__ bool prepare(Pre&) const {
+    if (sc->done) return false;
+    if (!check) return true;
+    const double rr = reduce_parts(part_rr, nrr);
+    const double resid = sqrt(rr);
+    const bool fin = (resid <= sc->reltol) || (sc->iter >= sc->maxiter);
+    if (threadIdx.x == 0 && blockIdx.x == 0 && blockIdx.y == 0) {
+      sc->resid = resid;
+      if (fin) sc->done = 1;
+    }
+    return !fin;
+  }
+  __device__ T load(const Pre&, int64_t n, int, int J) const {
+    if (J == 0) return T(0);
+    return src_mode == 1 ? s0 * dinv[n] * b[n] : src[n];
+  }
+  __device__ void epilogue(const Pre&, int64_t n, int, int J, T ku, T wc, double& a0, double&) const {
+    if (J == 0) {
+      if (mode == 0) { rout[n] = T(0); dout[n] = T(0); xout[n] = T(0); }
+      else if (mode == 1) { rout[n] = T(0); if (write_x_center) xout[n] = T(0); }
+      else xout[n] = T(0);
+      return;
+    }
+    const T r = rin[n] - ku;
+    if (mode == 1) {
+      rout[n] = r;
+      if (write_x_center) xout[n] = wc;
+      return;
+    }
+    const T dn = c1 * wc + c2 * dinv[n] * r;
+    const T xn = (xsrc_center ? wc : xin[n]) + dn;
+    xout[n] = xn;
+    if (mode == 0) {
+      rout[n] = r;
+      dout[n] = dn;
+    } else {
+      a0 += (double)xn * (double)rdot[n];
+    }
+  }
+};
+
 // --------------------------------------------------------------------------
 // fault physics (FP64, libm-equivalent device exp/log)
 // --------------------------------------------------------------------------
@@ -946,6 +1021,15 @@ void launch_l0_post(const Geom& g, const T* x0, const T* r, const T* mu, const T
   hipLaunchKernelGGL((tile_kernel<T, OpL0Post<T>>), tile_grid(g), dim3(BLOCK), 0, s, g, op);
 }
 
+
+template <typename T>
+void launch_cheb(const Geom& g, const ChebStep<T>& c, const T* mu, const T* dinv, const double* part_rr, int nrr,
+                 CgScal* sc, double* part_rho, hipStream_t s) {
+  OpCheb<T> op{c.src, c.b, mu, dinv, c.s0, c.src_mode, c.rin, c.rout, c.c1, c.c2, c.xin, c.xsrc_center, c.dout,
+               c.xout, c.mode, c.write_x_center, c.rdot, c.check, part_rr, nrr, sc, part_rho, nullptr};
+  hipLaunchKernelGGL((tile_kernel<T, OpCheb<T>>), tile_grid(g), dim3(BLOCK), 0, s, g, op);
+}
+
 template <typename T, int MODE>
 static void csr_dispatch(const DevCsr& M, const T* val, const T* x, const T* b, const T* dinv, T omega, T* y, T* xout,
                          const CgScal* sc, hipStream_t s) {
@@ -1098,6 +1182,8 @@ void launch_scale_mask(int64_t n, int nx, const double* x, double sign, double* 
   template void launch_csr<T>(CsrMode, const DevCsr&, const T*, const T*, const T*, T, T*, T*, const CgScal*,       \
                               hipStream_t);                                                                         \
   template void launch_dense<T>(int64_t, const double*, const T*, T*, const CgScal*, hipStream_t);                 \
+  template void launch_cheb<T>(const Geom&, const ChebStep<T>&, const T*, const T*, const double*, int, CgScal*,   \
+                               double*, hipStream_t);                                                              \
   template void launch_fill<T>(int64_t, T*, T, hipStream_t);
 EQ_INST(double)
 EQ_INST(float)

@@ -31,7 +31,7 @@ EXPORTS = [
     "eq_slr", "eq_dtevol", "eq_set_solver", "eq_state_init", "eq_state_get", "eq_state_set",
     "eq_step", "eq_run", "eq_time_kernel",
 ]
-EQ_TIME_KU, EQ_TIME_CG_DIR, EQ_TIME_VCYCLE, EQ_TIME_CG_ITER, EQ_TIME_NEWMARK = range(5)
+EQ_TIME_KU, EQ_TIME_CG_DIR, EQ_TIME_VCYCLE, EQ_TIME_CG_ITER, EQ_TIME_NEWMARK, EQ_TIME_CHEB = range(6)
 
 dp = C.POINTER(C.c_double)
 i64p = C.POINTER(C.c_int64)
@@ -101,7 +101,7 @@ def load():
         "eq_ku_apply": ([vp, dp, dp, C.c_int, C.c_int], C.c_int),
         "eq_kni_apply": ([vp, dp, dp, C.c_int], C.c_int),
         "eq_element_computation": ([vp, dp, dp], C.c_int),
-        "eq_amg_setup": ([vp, C.c_double, C.c_int, C.c_int64, C.c_int, C.c_int, C.c_int], C.c_int),
+        "eq_amg_setup": ([vp, C.c_double, C.c_int, C.c_int64, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
         "eq_amg_info": ([vp, C.POINTER(C.c_int), i64p, i64p, C.c_int], C.c_int),
         "eq_amg_ldiv": ([vp, dp, dp, C.c_int], C.c_int),
         "eq_cg": ([vp, dp, dp, C.c_double, C.c_int64, C.c_int, C.c_int, i64p, dp], C.c_int),

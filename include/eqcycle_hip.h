@@ -96,12 +96,17 @@ int eq_kni_apply(eq_ctx *ctx, const double *x, double *y, int prec);
 int eq_element_computation(eq_ctx *ctx, const double *u, double *a_elem);
 
 /* ---- AMG preconditioner -------------------------------------------------- */
-/* ruge_stuben(kni) + aspreconditioner(ml) (src/main.jl:200-201): Ruge-Stueben
- * hierarchy built on the host, uploaded, applied as one V-cycle on the GPU
- * with Jacobi or Chebyshev smoothing.  strength_abs = 1 is the classical
- * |a_ij| test of AlgebraicMultigrid.jl; 0 uses -a_ij. */
+/* ruge_stuben(kni) + aspreconditioner(ml) (src/main.jl:200-201): multigrid
+ * hierarchy built on the host, uploaded, applied as one V-cycle on the GPU.
+ * pcoarsen = 0: Ruge-Stueben on K_II itself, as ruge_stuben(kni) does.
+ * pcoarsen = 1: first level is the degree-4 -> bilinear (element vertex)
+ * prolongation, then Ruge-Stueben on the Galerkin bilinear operator.
+ * strength_abs = 1 is the |a_ij| strength test of AlgebraicMultigrid.jl's
+ * Classical(theta); 0 uses -a_ij.  smoother: EQ_SMOOTH_JACOBI on every level,
+ * or EQ_SMOOTH_CHEBYSHEV of smooth_degree on the matrix-free fine level
+ * (Jacobi on the assembled coarse levels). */
 int eq_amg_setup(eq_ctx *ctx, double theta, int max_levels, int64_t max_coarse, int strength_abs,
-                 int smoother, int smooth_degree);
+                 int smoother, int smooth_degree, int pcoarsen);
 /* Per-level sizes: n[l], nnz[l] for l < *nlev (arrays of capacity cap). */
 int eq_amg_info(const eq_ctx *ctx, int *nlev, int64_t *n, int64_t *nnz, int cap);
 /* x = Pl \ b for kni (ldiv!(x, p, b)); x, b in FltNI order. */
@@ -199,7 +204,8 @@ typedef struct {
 int eq_run(eq_ctx *ctx, double total_time, int64_t max_steps, eq_results *res);
 
 /* ---- measurement hook (no reference counterpart) ------------------------ */
-enum { EQ_TIME_KU = 0, EQ_TIME_CG_DIR = 1, EQ_TIME_VCYCLE = 2, EQ_TIME_CG_ITER = 3, EQ_TIME_NEWMARK = 4 };
+enum { EQ_TIME_KU = 0, EQ_TIME_CG_DIR = 1, EQ_TIME_VCYCLE = 2, EQ_TIME_CG_ITER = 3, EQ_TIME_NEWMARK = 4,
+       EQ_TIME_CHEB = 5 };
 /* Average device time (ms) of one launch of the given kernel group over reps
  * back-to-back launches, bracketed by HIP events on the context's stream.
  * Runs on scratch vectors; EQ_TIME_NEWMARK advances a copy of the state and

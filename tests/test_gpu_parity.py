@@ -95,6 +95,24 @@ def test_cg_matches_direct_solve(mesh, pc):
     assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) < 2e-11
 
 
+@pytest.mark.parametrize("hierarchy", ["rs", "pmg"])
+def test_cg_hierarchies(mesh, hierarchy):
+    """ruge_stuben-structured hierarchy and the p-multigrid one both solve to tolerance."""
+    import scipy.sparse.linalg as spla
+    from eqcycle_amd.api import EqContext
+    P, _, _, kni = mesh
+    ctx = EqContext(P, amg={"hierarchy": hierarchy})
+    A = kni.to_scipy().tocsc()
+    b = normal(31, A.shape[0])
+    xs = spla.spsolve(A, b)
+    for prec in ("fp64", "mixed"):
+        x, it, rr, ok = ctx.cg(np.zeros_like(b), b, tol=1e-10, pc="amg", prec=prec)
+        print(f"[gpu] {hierarchy} {prec} iterations {it}", flush=True)
+        assert ok
+        assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) <= 1e-10
+        assert np.linalg.norm(x - xs) / np.linalg.norm(xs) < 1e-6
+
+
 def test_cg_mixed_precision(mesh):
     import scipy.sparse.linalg as spla
     P, ctx, _, kni = mesh
@@ -225,10 +243,11 @@ def _compare_state(ctx, sim, tol):
     assert s["isolver"] == sim.isolver
 
 
-def test_quasi_static_steps_match_oracle(oracle):
+@pytest.mark.parametrize("hierarchy", ["rs", "pmg"])
+def test_quasi_static_steps_match_oracle(oracle, hierarchy):
     from eqcycle_amd.api import EqContext
     P = set_parameters(small(32, 8, material="trapezoid"))
-    ctx = EqContext(P, amg=True)
+    ctx = EqContext(P, amg={"hierarchy": hierarchy})
     ctx.set_solver(tol=1e-13, pc="amg")
     ctx.state_init()
     sim = oracle.Simulation(P, solver="direct")
@@ -269,7 +288,7 @@ def test_run_matches_oracle(oracle):
     from eqcycle_amd.api import EqContext
     P = set_parameters(small(32, 8, material="trapezoid", total_years=3))
     ctx = EqContext(P, amg=True)
-    ctx.set_solver(tol=1e-12, pc="amg")
+    ctx.set_solver(tol=1e-13, pc="amg")
     ctx.state_init()
     R = ctx.run()
     sim = oracle.Simulation(P, tol=1e-12, solver="direct")
