@@ -130,7 +130,7 @@ class EqContext:
         if smoother is None:
             smoother = "chebyshev" if pc else "jacobi"
         if max_coarse is None:
-            max_coarse = 200 if pc else 10
+            max_coarse = 2000 if pc else 10
         sm = {"jacobi": L.EQ_SMOOTH_JACOBI, "chebyshev": L.EQ_SMOOTH_CHEBYSHEV}[smoother]
         self._check(self.lib.eq_amg_setup(self.h, theta, max_levels, max_coarse, strength_abs, sm, degree, pc))
         self.hierarchy = hierarchy

@@ -94,6 +94,7 @@ struct eq_ctx {
   double* coarse_inv = nullptr;
   int64_t ncoarse = 0;
   std::vector<int64_t> amg_n, amg_nnz;
+  int pmg = 0;                        // level-0 prolongation is the bilinear vertex interpolation
   int l0_cheb = 0;                    // level-0 smoother: 0 Jacobi, 1 Chebyshev
   int l0_degree = 1;
   double l0_theta = 1, l0_delta = 1;  // Chebyshev interval centre / half-width of D^-1 K_II
@@ -564,7 +565,7 @@ extern "C" int eq_amg_setup(eq_ctx* ctx, double theta, int max_levels, int64_t m
     ctx->amg_nnz.push_back(L.A.nnz());
     D.omega = 4.0 / (3.0 * L.lambda_max);
     if (l > 0 && l + 1 < H.lev.size()) {
-      if (int rc = upload_csr(ctx, L.A, &D.A)) return rc;
+      if (int rc = upload_csr(ctx, L.A, &D.A)) return rc;  // the coarsest level uses its dense inverse
       D.dinv = ctx->alloc<double>(D.n);
       D.dinv32 = ctx->alloc<float>(D.n);
       NEED(D.dinv); NEED(D.dinv32);
@@ -580,11 +581,12 @@ extern "C" int eq_amg_setup(eq_ctx* ctx, double theta, int max_levels, int64_t m
       if (int rc = alloc_level_vecs(ctx, D.v32, D.n)) return rc;
     }
   }
-  // level 0: matrix-free operator, Jacobi weight from the assembled K_II
+  // dense coarsest inverse (row-major; one wavefront per row applies it)
   ctx->ncoarse = H.ncoarse;
   ctx->coarse_inv = ctx->alloc<double>(H.ncoarse * H.ncoarse);
   NEED(ctx->coarse_inv);
   if (int rc = h2d(ctx, ctx->coarse_inv, H.coarse_inv.data(), H.ncoarse * H.ncoarse)) return rc;
+  ctx->pmg = pcoarsen ? 1 : 0;
   ctx->have_amg = true;
   return sync_check(ctx);
 }
@@ -678,7 +680,7 @@ static void cheb_pre(eq_ctx* ctx, const T* b) {
 // Level-0 Chebyshev post-smoothing from x = chx: k kernels; output c and
 // partial <c, r>.
 template <typename T>
-static void cheb_post(eq_ctx* ctx, const T* b, T* c) {
+static void cheb_post(eq_ctx* ctx, const T* b, T* c, const T* coarse) {
   CgWork<T>& w = work<T>(ctx);
   const int k = ctx->l0_degree;
   const double th = ctx->l0_theta, de = ctx->l0_delta, sg = th / de;
@@ -688,6 +690,10 @@ static void cheb_post(eq_ctx* ctx, const T* b, T* c) {
     ChebStep<T> st;
     st.rin = (m == 0) ? b : w.chr;
     st.src = (m == 0) ? w.chx : w.chd[cur];
+    if (m == 0 && coarse) {
+      st.src_mode = 2;     // x0 = chx + P0 coarse, evaluated in the window load
+      st.coarse = coarse;
+    }
     if (m == 0) {
       st.c1 = T(0);
       st.c2 = (T)(1.0 / th);
@@ -748,10 +754,12 @@ static void vcycle(eq_ctx* ctx, const T* r, T* c) {
                   ctx->sc, s);
     launch_csr<T>(CSR_POST, D.A, v.x, v.b, ldinv<T>(D), (T)D.omega, v.xp, nullptr, ctx->sc, s);
   }
-  launch_csr<T>(CSR_PROLONG_ADD, L0.P, lvecs<T>(ctx->lev[1]).xp, nullptr, nullptr, T(0), x0 + g.NX, nullptr,
-                ctx->sc, s);
+  const bool fused_prolong = ctx->l0_cheb && ctx->pmg;
+  if (!fused_prolong)
+    launch_csr<T>(CSR_PROLONG_ADD, L0.P, lvecs<T>(ctx->lev[1]).xp, nullptr, nullptr, T(0), x0 + g.NX, nullptr,
+                  ctx->sc, s);
   if (ctx->l0_cheb)
-    cheb_post<T>(ctx, r, c);
+    cheb_post<T>(ctx, r, c, fused_prolong ? lvecs<T>(ctx->lev[1]).xp : nullptr);
   else
     launch_l0_post<T>(g, w.x0, r, mu_of<T>(ctx), dinv_of<T>(ctx), om0, c, ctx->part_rho, ctx->sc, s);
 }

@@ -103,6 +103,7 @@ struct ChebStep {
   int write_x_center = 0;
   const T* rdot = nullptr;
   int check = 0;
+  const T* coarse = nullptr;   // src_mode 2: coarse (vertex) correction added on the fly
 };
 template <typename T>
 void launch_cheb(const Geom& g, const ChebStep<T>& c, const T* mu, const T* dinv, const double* part_rr, int nrr,
@@ -124,6 +125,8 @@ void launch_csr(CsrMode mode, const DevCsr& M, const T* x, const T* b, const T* 
                 const CgScal* sc, hipStream_t s);
 template <typename T>
 void launch_dense(int64_t n, const double* inv, const T* b, T* x, const CgScal* sc, hipStream_t s);
+
+
 
 // ---- time stepping --------------------------------------------------------
 void launch_newmark(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, const double* M,

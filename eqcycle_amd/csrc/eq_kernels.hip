@@ -168,10 +168,51 @@ constexpr int WIN_J = TILE_J + 5;
 
 struct PreNone {};
 
+// Segment outputs o[0..4] of one 5-node segment (strided view, stride ST).
+template <typename T, int ST>
+__device__ __forceinline__ void seg_all(const Geom& g, const T* u, const T* m, T o[5]) {
+  T t[5];
+#pragma unroll
+  for (int p = 0; p < 5; p++) {
+    T s = 0;
+#pragma unroll
+    for (int k = 0; k < 5; k++) s += (T)g.D[p * 5 + k] * u[k * ST];
+    t[p] = s * ((T)g.w[p] * m[p * ST]);
+  }
+#pragma unroll
+  for (int i = 0; i < 5; i++) {
+    T r = 0;
+#pragma unroll
+    for (int p = 0; p < 5; p++) r += (T)g.D[p * 5 + i] * t[p];
+    o[i] = r;
+  }
+}
+
+// Output 4 only (the shared end node of the previous segment).
+template <typename T, int ST>
+__device__ __forceinline__ T seg_last(const Geom& g, const T* u, const T* m) {
+  T r = 0;
+#pragma unroll
+  for (int p = 0; p < 5; p++) {
+    T s = 0;
+#pragma unroll
+    for (int k = 0; k < 5; k++) s += (T)g.D[p * 5 + k] * u[k * ST];
+    r += (T)g.D[p * 5 + 4] * (s * ((T)g.w[p] * m[p * ST]));
+  }
+  return r;
+}
+
+// Tile kernel.  Phase 1 stages the window of the input vector (op.load) and
+// of mu in LDS.  Phase 2: each thread takes 4 consecutive nodes of one row
+// (one segment) and accumulates their x-direction terms into an LDS tile.
+// Phase 3: each thread takes 4 consecutive nodes of one column, adds the
+// y-direction terms and runs the epilogue.  Every 5-node segment is evaluated
+// once per direction (plus the shared end node of the previous segment).
 template <typename T, class Op>
 __global__ __launch_bounds__(BLOCK) void tile_kernel(Geom g, Op op) {
   __shared__ T su[WIN_J * WIN_I];
   __shared__ T sm[WIN_J * WIN_I];
+  __shared__ T sx[TILE_J * TILE_I];
   typename Op::Pre pre;
   if (!op.prepare(pre)) return;   // block-uniform
   const int I0 = blockIdx.x * TILE_I, J0 = blockIdx.y * TILE_J;
@@ -188,14 +229,46 @@ __global__ __launch_bounds__(BLOCK) void tile_kernel(Geom g, Op op) {
     sm[k] = m;
   }
   __syncthreads();
+  {
+    // x-direction: thread -> (row r, strip q of 4 nodes); TILE_I / 4 strips per row
+    constexpr int NSTRIP = TILE_I / 4;
+    for (int k = threadIdx.x; k < TILE_J * NSTRIP; k += BLOCK) {
+      const int r = k / NSTRIP, q = k - r * NSTRIP;
+      const int J = J0 + r, Is = I0 + 4 * q;
+      if (J >= g.NY || Is >= g.NX) continue;
+      const int s = Is >> 2;
+      const int lr = r + 4, lc = 4 * q + 4;
+      T o[5] = {0, 0, 0, 0, 0};
+      if (s < g.nelx) seg_all<T, 1>(g, su + lr * WIN_I + lc, sm + lr * WIN_I + lc, o);
+      if (s > 0) o[0] += seg_last<T, 1>(g, su + lr * WIN_I + lc - 4, sm + lr * WIN_I + lc - 4);
+      const T cx = (T)(g.c1 * wline(g, J, g.NY));
+#pragma unroll
+      for (int i = 0; i < 4; i++) sx[r * TILE_I + 4 * q + i] = cx * o[i];
+    }
+  }
+  __syncthreads();
   double acc0 = 0.0, acc1 = 0.0;
-  for (int k = threadIdx.x; k < TILE_I * TILE_J; k += BLOCK) {
-    const int r = k / TILE_I, c = k - r * TILE_I;
-    const int I = I0 + c, J = J0 + r;
-    if (I >= g.NX || J >= g.NY) continue;
-    const int lc = c + 4, lr = r + 4;
-    const T ku = ku_window<T, WIN_I>(g, su, sm, lc, lr, I, J);
-    op.epilogue(pre, (int64_t)J * g.NX + I, I, J, ku, su[lr * WIN_I + lc], acc0, acc1);
+  {
+    // y-direction + epilogue: thread -> (column c, strip q of 4 rows)
+    constexpr int NSTRIP = TILE_J / 4;
+    for (int k = threadIdx.x; k < TILE_I * NSTRIP; k += BLOCK) {
+      const int q = k / TILE_I, c = k - q * TILE_I;
+      const int I = I0 + c, Js = J0 + 4 * q;
+      if (I >= g.NX || Js >= g.NY) continue;
+      const int t = Js >> 2;
+      const int lc = c + 4, lr = 4 * q + 4;
+      T o[5] = {0, 0, 0, 0, 0};
+      if (t < g.nely) seg_all<T, WIN_I>(g, su + lr * WIN_I + lc, sm + lr * WIN_I + lc, o);
+      if (t > 0) o[0] += seg_last<T, WIN_I>(g, su + (lr - 4) * WIN_I + lc, sm + (lr - 4) * WIN_I + lc);
+      const T cy = (T)(g.c2 * wline(g, I, g.NX));
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int J = Js + j;
+        if (J >= g.NY) break;
+        const T ku = sx[(4 * q + j) * TILE_I + c] + cy * o[j];
+        op.epilogue(pre, (int64_t)J * g.NX + I, I, J, ku, su[(lr + j) * WIN_I + lc], acc0, acc1);
+      }
+    }
   }
   if (Op::kParts >= 1) {
     const double s = block_sum(acc0);
@@ -382,9 +455,28 @@ struct OpCheb {
     }
     return !fin;
   }
-  __device__ T load(const Pre&, int64_t n, int, int J) const {
+  const T* coarse;    // src_mode 2: window = src + P0 coarse (bilinear vertex interpolation)
+  int nelx, nely;
+  T gx[5];
+  __device__ T load(const Pre&, int64_t n, int I, int J) const {
     if (J == 0) return T(0);
-    return src_mode == 1 ? s0 * dinv[n] * b[n] : src[n];
+    if (src_mode == 1) return s0 * dinv[n] * b[n];
+    if (src_mode == 2) {
+      const int ex = min(I >> 2, nelx - 1), i = I - 4 * ex;
+      const int ey = min(J >> 2, nely - 1), j = J - 4 * ey;
+      const T ax1 = T(0.5) * (T(1) + gx[i]), ax0 = T(1) - ax1;
+      const T ay1 = T(0.5) * (T(1) + gx[j]), ay0 = T(1) - ay1;
+      const int nvx = nelx + 1;
+      T v = T(0);
+      if (ey > 0) {
+        const T* c0 = coarse + (int64_t)(ey - 1) * nvx + ex;
+        v += ay0 * (ax0 * c0[0] + ax1 * c0[1]);
+      }
+      const T* c1 = coarse + (int64_t)ey * nvx + ex;
+      v += ay1 * (ax0 * c1[0] + ax1 * c1[1]);
+      return src[n] + v;
+    }
+    return src[n];
   }
   __device__ void epilogue(const Pre&, int64_t n, int, int J, T ku, T wc, double& a0, double&) const {
     if (J == 0) {
@@ -671,16 +763,22 @@ __global__ __launch_bounds__(BLOCK) void csr_kernel(int64_t n, const int32_t* __
   }
 }
 
+// x = A_c^-1 b, one wavefront per row of the row-major inverse
 template <typename T>
 __global__ __launch_bounds__(BLOCK) void dense_kernel(int64_t n, const double* inv, const T* b, T* x,
                                                       const CgScal* sc) {
   if (sc && sc->done) return;
-  for (int64_t i = threadIdx.x; i < n; i += BLOCK) {
-    double s = 0.0;
-    for (int64_t j = 0; j < n; j++) s += inv[i * n + j] * (double)b[j];
-    x[i] = (T)s;
+  const int64_t row = (int64_t)blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  double s = 0.0;
+  if (row < n) {
+    const double* a = inv + row * n;
+    for (int64_t j = lane; j < n; j += 64) s += a[j] * (double)b[j];
   }
+  s = wave_sum(s);
+  if (row < n && lane == 0) x[row] = (T)s;
 }
+
 
 // --------------------------------------------------------------------------
 // quasi-static step pieces (src/main.jl:226-295)
@@ -1026,7 +1124,9 @@ template <typename T>
 void launch_cheb(const Geom& g, const ChebStep<T>& c, const T* mu, const T* dinv, const double* part_rr, int nrr,
                  CgScal* sc, double* part_rho, hipStream_t s) {
   OpCheb<T> op{c.src, c.b, mu, dinv, c.s0, c.src_mode, c.rin, c.rout, c.c1, c.c2, c.xin, c.xsrc_center, c.dout,
-               c.xout, c.mode, c.write_x_center, c.rdot, c.check, part_rr, nrr, sc, part_rho, nullptr};
+               c.xout, c.mode, c.write_x_center, c.rdot, c.check, part_rr, nrr, sc, part_rho, nullptr,
+               c.coarse, g.nelx, g.nely, {}};
+  for (int p = 0; p < 5; p++) op.gx[p] = (T)g.x[p];
   hipLaunchKernelGGL((tile_kernel<T, OpCheb<T>>), tile_grid(g), dim3(BLOCK), 0, s, g, op);
 }
 
@@ -1081,7 +1181,8 @@ void launch_csr(CsrMode mode, const DevCsr& M, const T* x, const T* b, const T* 
 
 template <typename T>
 void launch_dense(int64_t n, const double* inv, const T* b, T* x, const CgScal* sc, hipStream_t s) {
-  hipLaunchKernelGGL((dense_kernel<T>), dim3(1), dim3(BLOCK), 0, s, n, inv, b, x, sc);
+  const int64_t blocks = (n + BLOCK / 64 - 1) / (BLOCK / 64);
+  hipLaunchKernelGGL((dense_kernel<T>), dim3((unsigned)blocks), dim3(BLOCK), 0, s, n, inv, b, x, sc);
 }
 
 void launch_newmark(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, const double* M,
