@@ -22,7 +22,7 @@ from .lib import f64, i64, ptr
 
 log = logging.getLogger("eqcycle_amd")
 
-PREC = {"fp64": L.EQ_PREC_FP64, "fp32": L.EQ_PREC_FP32, "mixed": L.EQ_PREC_MIXED}
+PREC = {"fp64": L.EQ_PREC_FP64, "fp32": L.EQ_PREC_FP32, "mixed": L.EQ_PREC_MIXED, "f32pc": L.EQ_PREC_F32PC}
 PC = {"none": L.EQ_PC_NONE, "jacobi": L.EQ_PC_JACOBI, "amg": L.EQ_PC_AMG}
 MASK = {"all": L.EQ_MASK_ALL, "interior": L.EQ_MASK_INTERIOR, "neg": L.EQ_MASK_NEG}
 
@@ -295,6 +295,64 @@ class EqContext:
                        aSeis=bufs["aSeis"][:, :min(R.n_rec, cap_rec)].T)
         return out
 
+
+    # ---- learned initial guess ---------------------------------------------------
+    def learn_init(self, capacity=2000, channels=32, hidden=256, nbasis=64, seed=4):
+        self._check(self.lib.eq_learn_init(self.h, capacity, channels, hidden, nbasis, seed))
+        self.nbasis = nbasis
+
+    def learn_record(self, on=True):
+        self._check(self.lib.eq_learn_record(self.h, 1 if on else 0))
+
+    def learn_info(self):
+        ns, npar, nl = C.c_int64(), C.c_int64(), C.c_int64()
+        tr = C.c_int()
+        self._check(self.lib.eq_learn_info(self.h, C.byref(ns), C.byref(npar), C.byref(tr), C.byref(nl)))
+        return dict(nsnap=ns.value, nparams=npar.value, trained=bool(tr.value), learned_solves=nl.value)
+
+    def learn_gram(self):
+        m = self.learn_info()["nsnap"]
+        G = np.empty((m, m))
+        self._check(self.lib.eq_learn_gram(self.h, ptr(G)))
+        return G
+
+    def learn_set_basis(self, V, sigma):
+        V = f64(V)
+        sigma = f64(sigma)
+        self._check(self.lib.eq_learn_set_basis(self.h, ptr(V), ptr(sigma)))
+
+    def learn_train(self, epochs=200, batch=64, lr=1e-3, seed=11):
+        hist = np.zeros(max(epochs, 1))
+        self._check(self.lib.eq_learn_train(self.h, epochs, batch, lr, seed, ptr(hist)))
+        return hist[:epochs]
+
+    def learn_loss_grad(self, params, idx, use_bf16=False):
+        info = self.learn_info()
+        prm = None if params is None else np.ascontiguousarray(params, dtype=np.float32)
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        g = np.empty(info["nparams"], np.float32)
+        loss = C.c_double()
+        self._check(self.lib.eq_learn_loss_grad(self.h, ptr(prm, C.c_float), len(idx), ptr(idx, C.c_int32),
+                                                1 if use_bf16 else 0, C.byref(loss), ptr(g, C.c_float)))
+        return loss.value, g
+
+    def learn_get(self):
+        info = self.learn_info()
+        m, F = info["nsnap"], self.P.FltNglob
+        nb = self.nbasis
+        params = np.empty(info["nparams"], np.float32)
+        Phi = np.empty((nb, self.nI), np.float32)
+        X = np.empty((m, F), np.float32)
+        Y = np.empty((m, nb), np.float32)
+        self._check(self.lib.eq_learn_get(self.h, ptr(params, C.c_float), ptr(Phi, C.c_float), ptr(X, C.c_float),
+                                          ptr(Y, C.c_float)))
+        return dict(params=params, Phi=Phi, X=X, Y=Y)
+
+    def learn_predict(self, dF):
+        dF = f64(dF)
+        out = np.empty(self.nbasis)
+        self._check(self.lib.eq_learn_predict(self.h, ptr(dF), ptr(out)))
+        return out
 
     def time_kernel(self, which="ku", prec="fp64", reps=50):
         """Average device ms per launch, HIP events on the library's stream."""

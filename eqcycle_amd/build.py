@@ -25,7 +25,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-Xclang", "
 CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-I", CSRC, "-I", INCLUDE, "-I", "/opt/rocm/include",
              "-D__HIP_PLATFORM_AMD__"]
 
-HIP_SOURCES = ["eq_kernels.hip", "eq_api.hip"]
+HIP_SOURCES = ["eq_kernels.hip", "eq_api.hip", "eq_learn.hip"]
 CXX_SOURCES = ["amg_host.cpp"]
 
 

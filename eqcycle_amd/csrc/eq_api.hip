@@ -15,6 +15,7 @@
 
 #include "eq_internal.h"
 #include "eq_kernels.h"
+#include "eq_learn.h"
 
 using namespace eq;
 
@@ -104,8 +105,15 @@ struct eq_ctx {
   int* flags = nullptr;
   int64_t nr_noconv_total = 0;
   int64_t pcg_noconv = 0;
+  // learned initial guess
+  LearnState learn;
+  bool learn_on = false;
+  int learn_record_on = 0;
+  double *gp = nullptr, *gq = nullptr;
+  int64_t n_learned_solves = 0;
 
   ~eq_ctx() {
+    learn_free(&learn);
     for (void* p : allocs) (void)hipFree(p);
     if (sc_host) (void)hipHostFree(sc_host);
     if (st_host) (void)hipHostFree(st_host);
@@ -767,14 +775,20 @@ static void vcycle(eq_ctx* ctx, const T* r, T* c) {
 // ===========================================================================
 // CG in K_II form on device vectors (fault row ignored); x in/out, b input.
 // ===========================================================================
+// f32pc: apply the V-cycle in FP32 to the FP32 copy of r kept by the update
+// kernel, widen c back to FP64 and take <c, r> in FP64 (T must be double).
 template <typename T>
-static void cg_iteration(eq_ctx* ctx, T* x, int pc) {
+static void cg_iteration(eq_ctx* ctx, T* x, int pc, bool f32pc = false) {
   CgWork<T>& w = work<T>(ctx);
   const Geom& g = ctx->g;
   hipStream_t s = ctx->s;
   const int nt = tile_count(g);
   int nrho;
-  if (pc == EQ_PC_AMG) {
+  if (pc == EQ_PC_AMG && f32pc) {
+    vcycle<float>(ctx, ctx->w32.r, ctx->w32.c);
+    launch_widen_dot(ctx->N, ctx->w32.c, (const double*)w.r, (double*)w.c, ctx->sc, ctx->part_rho, s);
+    nrho = vec_blocks(ctx->N);
+  } else if (pc == EQ_PC_AMG) {
     vcycle<T>(ctx, w.r, w.c);
     nrho = nt;
   } else {
@@ -785,13 +799,13 @@ static void cg_iteration(eq_ctx* ctx, T* x, int pc) {
   launch_cg_dir<T>(g, w.c, w.u[w.cur], mu_of<T>(ctx), w.u[1 - w.cur], w.q, ctx->part_rho, nrho, ctx->sc,
                    ctx->part_uq, s);
   launch_cg_update<T>(ctx->N, x, w.r, w.u[1 - w.cur], w.q, ctx->part_rho, nrho, ctx->part_uq, nt, ctx->sc,
-                      ctx->part_rr, ctx->npart, s);
+                      ctx->part_rr, ctx->npart, f32pc ? ctx->w32.r : nullptr, s);
   w.cur = 1 - w.cur;
 }
 
 template <typename T>
 static int cg_device(eq_ctx* ctx, T* x, const T* b, double tol, int64_t maxiter, int pc, int64_t* iters,
-                     double* relres) {
+                     double* relres, bool f32pc = false) {
   CgWork<T>& w = work<T>(ctx);
   const Geom& g = ctx->g;
   hipStream_t s = ctx->s;
@@ -803,9 +817,10 @@ static int cg_device(eq_ctx* ctx, T* x, const T* b, double tol, int64_t maxiter,
   launch_cg_residual<T>(g, x, mu_of<T>(ctx), b, w.r, ctx->part_rr, ctx->part_bb, s);
   launch_cg_init_scal(ctx->sc, ctx->part_rr, ctx->part_bb, ctx->npart, tol, (int)maxiter, s);
   launch_fill<T>(ctx->N, w.u[w.cur], T(0), s);
+  if (f32pc) launch_convert<T, float>(ctx->N, w.r, ctx->w32.r, s);
   int64_t batch = std::max<int64_t>(1, ctx->last_iters - 1);
   for (;;) {
-    for (int64_t k = 0; k < batch; k++) cg_iteration<T>(ctx, x, pc);
+    for (int64_t k = 0; k < batch; k++) cg_iteration<T>(ctx, x, pc, f32pc);
     launch_cg_check(ctx->sc, ctx->part_rr, ctx->npart, s);
     HIPCHK(ctx, hipMemcpyAsync(ctx->sc_host, ctx->sc, sizeof(CgScal), hipMemcpyDeviceToHost, s));
     if (int rc = sync_check(ctx)) return rc;
@@ -860,6 +875,8 @@ static int cg_mixed(eq_ctx* ctx, double* x, const double* b, double tol, int64_t
 static int cg_any(eq_ctx* ctx, double* x, const double* b, double tol, int64_t maxiter, int pc, int prec,
                   int64_t* iters, double* relres) {
   if (prec == EQ_PREC_MIXED) return cg_mixed(ctx, x, b, tol, maxiter, pc, iters, relres);
+  if (prec == EQ_PREC_F32PC)
+    return cg_device<double>(ctx, x, b, tol, maxiter, pc, iters, relres, pc == EQ_PC_AMG);
   if (prec == EQ_PREC_FP32) {
     launch_convert<double, float>(ctx->N, x, ctx->w32.x, ctx->s);
     launch_convert<double, float>(ctx->N, b, ctx->w32.b, ctx->s);
@@ -875,7 +892,7 @@ extern "C" int eq_cg(eq_ctx* ctx, double* x, const double* b, double tol, int64_
                      int64_t* iters, double* relres) {
   if (!ctx || !x || !b) return EQ_EINVAL;
   if (!ctx->have_material) return fail(ctx, EQ_ENOTREADY, "eq_set_material first");
-  if (pc < EQ_PC_NONE || pc > EQ_PC_AMG || prec < EQ_PREC_FP64 || prec > EQ_PREC_MIXED)
+  if (pc < EQ_PC_NONE || pc > EQ_PC_AMG || prec < EQ_PREC_FP64 || prec > EQ_PREC_F32PC)
     return fail(ctx, EQ_EINVAL, "bad pc / prec");
   // cg!(x, kni, b): kni = -K_II, so solve K_II x = -b (identical CG iterates)
   double* dx = ctx->w64.x;
@@ -1110,9 +1127,10 @@ extern "C" int eq_dtevol(eq_ctx* ctx, double dt, double dtmin, const double* XiL
 extern "C" int eq_set_solver(eq_ctx* ctx, const eq_solver_opts* o) {
   if (!ctx || !o) return EQ_EINVAL;
   if (o->pc < EQ_PC_NONE || o->pc > EQ_PC_AMG) return fail(ctx, EQ_EINVAL, "bad pc");
-  if (o->prec != EQ_PREC_FP64 && o->prec != EQ_PREC_MIXED && o->prec != EQ_PREC_FP32)
-    return fail(ctx, EQ_EINVAL, "bad prec");
-  if (o->guess != EQ_GUESS_PREVIOUS) return fail(ctx, EQ_EINVAL, "learned guess is not configured");
+  if (o->prec < EQ_PREC_FP64 || o->prec > EQ_PREC_F32PC) return fail(ctx, EQ_EINVAL, "bad prec");
+  if (o->guess != EQ_GUESS_PREVIOUS && o->guess != EQ_GUESS_LEARNED) return fail(ctx, EQ_EINVAL, "bad guess");
+  if (o->guess == EQ_GUESS_LEARNED && !ctx->learn_on)
+    return fail(ctx, EQ_ENOTREADY, "learned guess: basis and network not trained (eq_learn_train)");
   ctx->opts = *o;
   return EQ_OK;
 }
@@ -1204,18 +1222,39 @@ static int read_status(eq_ctx* ctx) {
   return sync_check(ctx);
 }
 
+// Learned start for K_II x = b: x (interior = previous-step displacement,
+// fault row = new fault data) += gamma p with p = |dF| Phi net(dF / |dF|)
+// and gamma the energy-optimal step along p, so the guess is never worse (in
+// the K_II norm) than the previous-step guess it starts from.
+static void learned_correction(eq_ctx* ctx, double* x, const double* b) {
+  CgWork<double>& w = ctx->w64;
+  const Geom& g = ctx->g;
+  hipStream_t s = ctx->s;
+  const int nt = tile_count(g);
+  launch_fill<double>(ctx->npart, ctx->part_rr, 0.0, s);
+  launch_fill<double>(ctx->npart, ctx->part_bb, 0.0, s);
+  launch_cg_residual<double>(g, x, ctx->mu, b, w.r, ctx->part_rr, ctx->part_bb, s);
+  learn_guess(&ctx->learn, x, ctx->dPre, ctx->gp, s);
+  launch_guess_dot(g, ctx->gp, ctx->mu, w.r, ctx->gq, ctx->part_rho, ctx->part_uq, s);
+  launch_guess_update(ctx->N, x, w.r, ctx->gp, ctx->gq, ctx->part_rho, ctx->part_uq, nt, ctx->part_rr, ctx->npart,
+                      s);
+  ctx->n_learned_solves++;
+}
+
 static int qs_step(eq_ctx* ctx, eq_step_info* info) {
   const Geom& g = ctx->g;
   hipStream_t s = ctx->s;
   const double dt = ctx->dt;
   for (int pass = 0; pass < 2; pass++) {
     launch_qs_begin(g, ctx->fp, ctx->fs, ctx->mu, ctx->d, ctx->dPre, ctx->v, ctx->bq, dt, pass == 0, s);
+    if (pass == 0 && ctx->opts.guess == EQ_GUESS_LEARNED && ctx->learn_on) learned_correction(ctx, ctx->d, ctx->bq);
     int64_t it = 0;
     double rr = 0;
     int rc = cg_any(ctx, ctx->d, ctx->bq, ctx->opts.tol, ctx->opts.maxiter, ctx->opts.pc, ctx->opts.prec, &it, &rr);
     if (rc == EQ_PCG_NOCONV) ctx->pcg_noconv++;
     else if (rc) return rc;
     info->pcg_iters[pass] = it;
+    if (pass == 0 && ctx->learn_record_on) learn_record(&ctx->learn, ctx->d, ctx->dPre, s);
     launch_qs_fault(g, ctx->fp, ctx->fs, ctx->mu, ctx->d, ctx->v, dt, s);
   }
   launch_qs_finish(g, ctx->fp, ctx->fs, ctx->d, ctx->dPre, ctx->v, ctx->a, dt, s);
@@ -1505,4 +1544,99 @@ extern "C" int eq_time_kernel(eq_ctx* ctx, int which, int prec, int reps, double
   (void)hipEventDestroy(e1);
   *avg_ms = (double)ms / reps;
   return sync_check(ctx);
+}
+
+// ===========================================================================
+// learned initial guess
+// ===========================================================================
+#define LEARN_READY(ctx)                                                                  \
+  do {                                                                                    \
+    if (!(ctx)->learn.S) return fail((ctx), EQ_ENOTREADY, "eq_learn_init first");         \
+  } while (0)
+
+extern "C" int eq_learn_init(eq_ctx* ctx, int64_t capacity, int channels, int hidden, int nbasis, uint64_t seed) {
+  if (!ctx || capacity < 1 || channels < 1 || hidden < 1) return EQ_EINVAL;
+  if (!ctx->have_params || !ctx->have_material) return fail(ctx, EQ_ENOTREADY, "material and params first");
+  std::string err;
+  int rc = learn_init(&ctx->learn, ctx->nI, ctx->NX, ctx->NX, (int)capacity, channels, hidden, nbasis, seed, &err);
+  if (rc) return fail(ctx, rc == 7 ? EQ_NOMEM : EQ_EINVAL, err);
+  if (!ctx->gp) {
+    ctx->gp = ctx->alloc<double>(ctx->N);
+    ctx->gq = ctx->alloc<double>(ctx->N);
+    NEED(ctx->gp); NEED(ctx->gq);
+  }
+  ctx->learn_on = false;
+  ctx->learn_record_on = 0;
+  return EQ_OK;
+}
+
+extern "C" int eq_learn_record(eq_ctx* ctx, int on) {
+  if (!ctx) return EQ_EINVAL;
+  LEARN_READY(ctx);
+  ctx->learn_record_on = on ? 1 : 0;
+  return EQ_OK;
+}
+
+extern "C" int eq_learn_info(const eq_ctx* ctx, int64_t* nsnap, int64_t* nparams, int* trained, int64_t* nlearned) {
+  if (!ctx) return EQ_EINVAL;
+  if (nsnap) *nsnap = ctx->learn.nsnap;
+  if (nparams) *nparams = ctx->learn.d.nparams;
+  if (trained) *trained = ctx->learn_on ? 1 : 0;
+  if (nlearned) *nlearned = ctx->n_learned_solves;
+  return EQ_OK;
+}
+
+extern "C" int eq_learn_gram(eq_ctx* ctx, double* G) {
+  if (!ctx || !G) return EQ_EINVAL;
+  LEARN_READY(ctx);
+  std::string err;
+  if (int rc = learn_gram(&ctx->learn, G, ctx->s, &err)) return fail(ctx, rc == 7 ? EQ_NOMEM : EQ_HIP_ERR, err);
+  return EQ_OK;
+}
+
+extern "C" int eq_learn_set_basis(eq_ctx* ctx, const double* V, const double* sigma) {
+  if (!ctx || !V || !sigma) return EQ_EINVAL;
+  LEARN_READY(ctx);
+  std::string err;
+  if (int rc = learn_set_basis(&ctx->learn, V, sigma, ctx->s, &err)) return fail(ctx, rc == 7 ? EQ_NOMEM : EQ_EINVAL, err);
+  return EQ_OK;
+}
+
+extern "C" int eq_learn_train(eq_ctx* ctx, int epochs, int batch, double lr, uint64_t seed, double* hist) {
+  if (!ctx || epochs < 0 || batch < 1) return EQ_EINVAL;
+  LEARN_READY(ctx);
+  std::string err;
+  if (int rc = learn_train(&ctx->learn, epochs, batch, lr, seed, hist, ctx->s, &err))
+    return fail(ctx, rc == 7 ? EQ_NOMEM : EQ_EINVAL, err);
+  ctx->learn_on = true;
+  return EQ_OK;
+}
+
+extern "C" int eq_learn_loss_grad(eq_ctx* ctx, const float* params, int nsamp, const int32_t* idx, int use_bf16,
+                                  double* loss, float* grad) {
+  if (!ctx || !idx || !loss) return EQ_EINVAL;
+  LEARN_READY(ctx);
+  std::string err;
+  if (int rc = learn_loss_grad(&ctx->learn, params, nsamp, idx, use_bf16, loss, grad, ctx->s, &err))
+    return fail(ctx, rc == 7 ? EQ_NOMEM : EQ_EINVAL, err);
+  return EQ_OK;
+}
+
+extern "C" int eq_learn_get(eq_ctx* ctx, float* params, float* Phi, float* X, float* Y) {
+  if (!ctx) return EQ_EINVAL;
+  LEARN_READY(ctx);
+  LearnState& L = ctx->learn;
+  if (params) HIPCHK(ctx, hipMemcpyAsync(params, L.w32, sizeof(float) * L.d.nparams, hipMemcpyDeviceToHost, ctx->s));
+  if (Phi) HIPCHK(ctx, hipMemcpyAsync(Phi, L.Phi, sizeof(float) * L.nI * L.d.NB, hipMemcpyDeviceToHost, ctx->s));
+  if (X) HIPCHK(ctx, hipMemcpyAsync(X, L.X, sizeof(float) * (int64_t)L.nsnap * L.d.L0, hipMemcpyDeviceToHost, ctx->s));
+  if (Y) HIPCHK(ctx, hipMemcpyAsync(Y, L.Y, sizeof(float) * (int64_t)L.nsnap * L.d.NB, hipMemcpyDeviceToHost, ctx->s));
+  return sync_check(ctx);
+}
+
+extern "C" int eq_learn_predict(eq_ctx* ctx, const double* dF, double* coef) {
+  if (!ctx || !dF || !coef) return EQ_EINVAL;
+  LEARN_READY(ctx);
+  std::string err;
+  if (int rc = learn_predict(&ctx->learn, dF, coef, ctx->s, &err)) return fail(ctx, EQ_HIP_ERR, err);
+  return EQ_OK;
 }

@@ -72,7 +72,14 @@ void launch_cg_dir(const Geom& g, const T* c, const T* u_old, const T* mu, T* u_
                    int nrho, const CgScal* sc, double* part_uq, hipStream_t s);
 template <typename T>
 void launch_cg_update(int64_t n, T* x, T* r, const T* u, const T* q, const double* part_rho, int nrho,
-                      const double* part_uq, int nuq, CgScal* sc, double* part_rr, int npart, hipStream_t s);
+                      const double* part_uq, int nuq, CgScal* sc, double* part_rr, int npart, float* r32,
+                      hipStream_t s);
+void launch_guess_dot(const Geom& g, const double* p, const double* mu, const double* r0, double* q,
+                      double* part_pr, double* part_pq, hipStream_t s);
+void launch_guess_update(int64_t n, double* x, double* r, const double* p, const double* q, const double* part_pr,
+                         const double* part_pq, int nparts, double* part_rr, int npart_out, hipStream_t s);
+void launch_widen_dot(int64_t n, const float* c32, const double* r, double* c, const CgScal* sc, double* part,
+                      hipStream_t s);
 template <typename T>
 void launch_jacobi_pc(int64_t n, const T* dinv, const T* r, T* c, const double* part_rr, int nrr, CgScal* sc,
                       double* part_rho, hipStream_t s);

@@ -503,6 +503,26 @@ struct OpCheb {
   }
 };
 
+// Learned-guess safeguard: q = K_II p, partials <p, r0> and <p, q>.
+struct OpGuessDot {
+  using Pre = PreNone;
+  static constexpr int kParts = 2;
+  const double* p;
+  const double* mu;
+  const double* r0;
+  double* q;
+  double* part0;
+  double* part1;
+  __device__ bool prepare(Pre&) const { return true; }
+  __device__ double load(const Pre&, int64_t n, int, int J) const { return J == 0 ? 0.0 : p[n]; }
+  __device__ void epilogue(const Pre&, int64_t n, int, int J, double ku, double pc, double& a0, double& a1) const {
+    const double qv = J == 0 ? 0.0 : ku;
+    q[n] = qv;
+    a0 += pc * r0[n];
+    a1 += pc * qv;
+  }
+};
+
 // --------------------------------------------------------------------------
 // fault physics (FP64, libm-equivalent device exp/log)
 // --------------------------------------------------------------------------
@@ -644,7 +664,8 @@ struct OpNewmark {
 template <typename T>
 __global__ __launch_bounds__(BLOCK) void cg_update_kernel(int64_t n, T* x, T* r, const T* u, const T* q,
                                                           const double* part_rho, int nrho, const double* part_uq,
-                                                          int nuq, CgScal* sc, double* part_rr, int npart) {
+                                                          int nuq, CgScal* sc, double* part_rr, int npart,
+                                                          float* r32) {
   if (sc->done) return;
   // the residual partials are consumed over npart entries: clear the tail
   if (blockIdx.x == 0)
@@ -661,6 +682,7 @@ __global__ __launch_bounds__(BLOCK) void cg_update_kernel(int64_t n, T* x, T* r,
       x[i] += alpha * u[i];
       const T rv = r[i] - alpha * q[i];
       r[i] = rv;
+      if (r32) r32[i] = (float)rv;
       acc += (double)rv * (double)rv;
     }
   }
@@ -1038,6 +1060,53 @@ __global__ void fill_kernel(int64_t n, T* y, T v) {
   const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   if (i < n) y[i] = v;
 }
+// x += gamma p, r -= gamma q with gamma = <p, r0> / <p, K p> (0 if not positive
+// definite in floating point); partial ||r||^2.  The energy-norm error of the
+// guess is then never larger than that of x before the correction.
+__global__ __launch_bounds__(BLOCK) void guess_update_kernel(int64_t n, double* x, double* r, const double* p,
+                                                             const double* q, const double* part_pr,
+                                                             const double* part_pq, int nparts, double* part_rr,
+                                                             int npart_out) {
+  const double pr = reduce_parts(part_pr, nparts);
+  const double pq = reduce_parts(part_pq, nparts);
+  const double gamma = (pq > 0.0) ? pr / pq : 0.0;
+  if (blockIdx.x == 0)
+    for (int i = gridDim.x + threadIdx.x; i < npart_out; i += BLOCK) part_rr[i] = 0.0;
+  double acc = 0.0;
+  const int64_t base = (int64_t)blockIdx.x * BLOCK * 4;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int64_t i = base + k * BLOCK + threadIdx.x;
+    if (i < n) {
+      x[i] += gamma * p[i];
+      const double rv = r[i] - gamma * q[i];
+      r[i] = rv;
+      acc += rv * rv;
+    }
+  }
+  const double s = block_sum(acc);
+  if (threadIdx.x == 0) part_rr[blockIdx.x] = s;
+}
+
+// FP32 preconditioner output back to FP64 with the partial <c, r> in FP64
+__global__ __launch_bounds__(BLOCK) void widen_dot_kernel(int64_t n, const float* c32, const double* r, double* c,
+                                                          const CgScal* sc, double* part) {
+  if (sc->done) return;
+  double acc = 0.0;
+  const int64_t base = (int64_t)blockIdx.x * BLOCK * 4;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int64_t i = base + k * BLOCK + threadIdx.x;
+    if (i < n) {
+      const double cv = (double)c32[i];
+      c[i] = cv;
+      acc += cv * r[i];
+    }
+  }
+  const double s = block_sum(acc);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
 __global__ void axpy_kernel(int64_t n, double alpha, const double* x, double* y) {
   const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   if (i < n) y[i] += alpha * x[i];
@@ -1093,9 +1162,10 @@ void launch_cg_dir(const Geom& g, const T* c, const T* u_old, const T* mu, T* u_
 
 template <typename T>
 void launch_cg_update(int64_t n, T* x, T* r, const T* u, const T* q, const double* part_rho, int nrho,
-                      const double* part_uq, int nuq, CgScal* sc, double* part_rr, int npart, hipStream_t s) {
+                      const double* part_uq, int nuq, CgScal* sc, double* part_rr, int npart, float* r32,
+                      hipStream_t s) {
   hipLaunchKernelGGL((cg_update_kernel<T>), dim3(vec_blocks(n)), dim3(BLOCK), 0, s, n, x, r, u, q, part_rho, nrho,
-                     part_uq, nuq, sc, part_rr, npart);
+                     part_uq, nuq, sc, part_rr, npart, r32);
 }
 
 template <typename T>
@@ -1259,6 +1329,20 @@ void launch_fill(int64_t n, T* y, T val, hipStream_t s) {
   if (n <= 0) return;
   hipLaunchKernelGGL((fill_kernel<T>), dim3(nblk(n)), dim3(BLOCK), 0, s, n, y, val);
 }
+void launch_guess_dot(const Geom& g, const double* p, const double* mu, const double* r0, double* q,
+                      double* part_pr, double* part_pq, hipStream_t s) {
+  OpGuessDot op{p, mu, r0, q, part_pr, part_pq};
+  hipLaunchKernelGGL((tile_kernel<double, OpGuessDot>), tile_grid(g), dim3(BLOCK), 0, s, g, op);
+}
+void launch_guess_update(int64_t n, double* x, double* r, const double* p, const double* q, const double* part_pr,
+                         const double* part_pq, int nparts, double* part_rr, int npart_out, hipStream_t s) {
+  hipLaunchKernelGGL(guess_update_kernel, dim3(vec_blocks(n)), dim3(BLOCK), 0, s, n, x, r, p, q, part_pr, part_pq,
+                     nparts, part_rr, npart_out);
+}
+void launch_widen_dot(int64_t n, const float* c32, const double* r, double* c, const CgScal* sc, double* part,
+                      hipStream_t s) {
+  hipLaunchKernelGGL(widen_dot_kernel, dim3(vec_blocks(n)), dim3(BLOCK), 0, s, n, c32, r, c, sc, part);
+}
 void launch_axpy(int64_t n, double alpha, const double* x, double* y, hipStream_t s) {
   hipLaunchKernelGGL(axpy_kernel, dim3(nblk(n)), dim3(BLOCK), 0, s, n, alpha, x, y);
 }
@@ -1273,7 +1357,7 @@ void launch_scale_mask(int64_t n, int nx, const double* x, double sign, double* 
   template void launch_cg_dir<T>(const Geom&, const T*, const T*, const T*, T*, T*, const double*, int,            \
                                  const CgScal*, double*, hipStream_t);                                              \
   template void launch_cg_update<T>(int64_t, T*, T*, const T*, const T*, const double*, int, const double*, int,    \
-                                    CgScal*, double*, int, hipStream_t);                                                 \
+                                    CgScal*, double*, int, float*, hipStream_t);                                                 \
   template void launch_jacobi_pc<T>(int64_t, const T*, const T*, T*, const double*, int, CgScal*, double*,          \
                                     hipStream_t);                                                                   \
   template void launch_l0_down<T>(const Geom&, const T*, const T*, const T*, T, T*, T*, const double*, int,         \

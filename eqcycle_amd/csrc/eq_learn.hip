@@ -1,0 +1,709 @@
+// Learned initial guess: snapshot store, POD basis, conv1d + MLP network with
+// hand-written forward / backward / Adam kernels (BF16 weights, FP32 master
+// weights and accumulation), single-workgroup inference.
+//
+// Network: input = fault slip increment dF (length L0) divided by |dF|;
+// NL conv layers (C channels, kernel 5, stride 2, padding 2, ReLU); an MLP
+// flat -> H (ReLU) -> NB.  The output approximates the POD coefficients of
+// the interior displacement increment divided by |dF|: the quasi-static
+// response is linear in dF, so the scale is factored out exactly.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "eq_learn.h"
+
+namespace eq {
+
+namespace {
+constexpr int LB = 256;     // block size of the element-wise kernels
+constexpr int NBMAX = 64;   // max POD coefficients
+
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  const uint32_t u = __float_as_uint(f);
+  const uint32_t r = 0x7FFFu + ((u >> 16) & 1u);   // round to nearest even
+  return (uint16_t)((u + r) >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
+__device__ __forceinline__ float ldw(const float* w, int64_t i) { return w[i]; }
+__device__ __forceinline__ float ldw(const uint16_t* w, int64_t i) { return bf2f(w[i]); }
+
+__device__ double lblock_sum(double v) {
+  __shared__ double sh[LB / 64];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r = 0;
+  for (int k = 0; k < LB / 64; k++) r += sh[k];
+  return r;
+}
+
+inline int gblk(int64_t n) { return (int)((n + LB - 1) / LB); }
+
+#define LCHK(call)                                       \
+  do {                                                   \
+    const hipError_t e_ = (call);                        \
+    if (e_ != hipSuccess) {                              \
+      if (err) *err = hipGetErrorString(e_);             \
+      return 2;                                          \
+    }                                                    \
+  } while (0)
+
+// ---- snapshots -------------------------------------------------------------
+__global__ void record_kernel(const double* d, const double* dPre, int64_t nflt, int64_t off, int64_t nI, float* Fcol,
+                              float* Scol) {
+  const int64_t i = (int64_t)blockIdx.x * LB + threadIdx.x;
+  if (i < nflt) Fcol[i] = (float)(d[i] - dPre[i]);
+  if (i < nI) Scol[i] = (float)(d[off + i] - dPre[off + i]);
+}
+
+// G = S^T S for 64 x 64 tiles with bi <= bj; FP64 accumulation.
+constexpr int GT = 64, GR = 64;
+__global__ __launch_bounds__(256) void gram_kernel(const float* S, int64_t nI, int m, double* G) {
+  const int bi = blockIdx.y, bj = blockIdx.x;
+  if (bi > bj) return;
+  __shared__ float As[GR][GT + 1];
+  __shared__ float Bs[GR][GT + 1];
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;   // 16 x 16 threads, 4 x 4 outputs each
+  double acc[4][4] = {};
+  for (int64_t r0 = 0; r0 < nI; r0 += GR) {
+    for (int e = threadIdx.x; e < GR * GT; e += 256) {
+      const int c = e / GR, rr = e - c * GR;   // rr fastest: contiguous rows of one column
+      const int64_t row = r0 + rr;
+      const int ca = bi * GT + c, cb = bj * GT + c;
+      As[rr][c] = (row < nI && ca < m) ? S[(int64_t)ca * nI + row] : 0.f;
+      Bs[rr][c] = (row < nI && cb < m) ? S[(int64_t)cb * nI + row] : 0.f;
+    }
+    __syncthreads();
+    for (int rr = 0; rr < GR; rr++) {
+      float a[4], b[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) { a[q] = As[rr][ty * 4 + q]; b[q] = Bs[rr][tx * 4 + q]; }
+#pragma unroll
+      for (int p = 0; p < 4; p++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) acc[p][q] += (double)a[p] * (double)b[q];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int p = 0; p < 4; p++)
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int i = bi * GT + ty * 4 + p, j = bj * GT + tx * 4 + q;
+      if (i < m && j < m) G[(int64_t)i * m + j] = acc[p][q];
+    }
+}
+
+// Phi[:, k] = sum_j S[:, j] W[j, k]  (W = V diag(1/sigma), row-major [m x NB])
+__global__ __launch_bounds__(256) void basis_kernel(const float* S, int64_t nI, int m, const float* W, int NB,
+                                                    float* Phi) {
+  __shared__ float Ws[16][NBMAX];
+  const int64_t row = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  float acc[NBMAX];
+#pragma unroll
+  for (int k = 0; k < NBMAX; k++) acc[k] = 0.f;
+  for (int j0 = 0; j0 < m; j0 += 16) {
+    for (int e = threadIdx.x; e < 16 * NBMAX; e += 256) {
+      const int jj = e / NBMAX, k = e - jj * NBMAX;
+      Ws[jj][k] = (j0 + jj < m && k < NB) ? W[(int64_t)(j0 + jj) * NB + k] : 0.f;
+    }
+    __syncthreads();
+    if (row < nI) {
+      for (int jj = 0; jj < 16 && j0 + jj < m; jj++) {
+        const float s = S[(int64_t)(j0 + jj) * nI + row];
+#pragma unroll
+        for (int k = 0; k < NBMAX; k++) acc[k] += s * Ws[jj][k];
+      }
+    }
+    __syncthreads();
+  }
+  if (row < nI)
+    for (int k = 0; k < NB; k++) Phi[(int64_t)k * nI + row] = acc[k];
+}
+
+// ---- network: training kernels --------------------------------------------
+__global__ void gather_kernel(const float* X, int L0, const int32_t* idx, int B, float* out) {
+  const int64_t e = (int64_t)blockIdx.x * LB + threadIdx.x;
+  if (e >= (int64_t)B * L0) return;
+  const int b = (int)(e / L0), i = (int)(e - (int64_t)b * L0);
+  out[e] = X[(int64_t)idx[b] * L0 + i];
+}
+
+template <typename WT>
+__global__ void conv_fwd_kernel(int B, int Cin, int Lin, int Cout, int Lout, const float* in, const WT* W,
+                                const float* bias, float* out) {
+  const int64_t e = (int64_t)blockIdx.x * LB + threadIdx.x;
+  if (e >= (int64_t)B * Cout * Lout) return;
+  const int o = (int)(e % Lout);
+  const int co = (int)((e / Lout) % Cout);
+  const int b = (int)(e / ((int64_t)Lout * Cout));
+  float acc = bias[co];
+  for (int ci = 0; ci < Cin; ci++) {
+    const float* x = in + ((int64_t)b * Cin + ci) * Lin;
+    const int64_t wb = ((int64_t)co * Cin + ci) * 5;
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+      const int i = 2 * o + k - 2;
+      if (i >= 0 && i < Lin) acc += ldw(W, wb + k) * x[i];
+    }
+  }
+  out[e] = fmaxf(acc, 0.f);
+}
+
+// din = (W^T * dout) masked by in > 0 (the input is a ReLU output)
+template <typename WT>
+__global__ void conv_bwd_data_kernel(int B, int Cin, int Lin, int Cout, int Lout, const float* in, const WT* W,
+                                     const float* dout, float* din) {
+  const int64_t e = (int64_t)blockIdx.x * LB + threadIdx.x;
+  if (e >= (int64_t)B * Cin * Lin) return;
+  const int i = (int)(e % Lin);
+  const int ci = (int)((e / Lin) % Cin);
+  const int b = (int)(e / ((int64_t)Lin * Cin));
+  float acc = 0.f;
+  if (in[e] > 0.f) {
+    for (int co = 0; co < Cout; co++) {
+      const float* dy = dout + ((int64_t)b * Cout + co) * Lout;
+      const int64_t wb = ((int64_t)co * Cin + ci) * 5;
+#pragma unroll
+      for (int k = 0; k < 5; k++) {
+        const int t = i + 2 - k;
+        if (t >= 0 && (t & 1) == 0 && (t >> 1) < Lout) acc += ldw(W, wb + k) * dy[t >> 1];
+      }
+    }
+  }
+  din[e] = acc;
+}
+
+// dW[co, ci, :] and db[co] by block reduction over (b, o)
+__global__ __launch_bounds__(LB) void conv_bwd_weight_kernel(int B, int Cin, int Lin, int Cout, int Lout,
+                                                             const float* in, const float* dout, float* dW,
+                                                             float* db) {
+  const int co = blockIdx.x / Cin, ci = blockIdx.x - co * Cin;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  const int64_t tot = (int64_t)B * Lout;
+  for (int64_t e = threadIdx.x; e < tot; e += LB) {
+    const int b = (int)(e / Lout), o = (int)(e - (int64_t)b * Lout);
+    const float g = dout[((int64_t)b * Cout + co) * Lout + o];
+    const float* x = in + ((int64_t)b * Cin + ci) * Lin;
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+      const int i = 2 * o + k - 2;
+      if (i >= 0 && i < Lin) acc[k] += (double)g * (double)x[i];
+    }
+    acc[5] += g;
+  }
+  for (int k = 0; k < 6; k++) {
+    const double s = lblock_sum(acc[k]);
+    if (threadIdx.x == 0) {
+      if (k < 5) dW[((int64_t)co * Cin + ci) * 5 + k] = (float)s;
+      else if (ci == 0) db[co] = (float)s;
+    }
+  }
+}
+
+template <typename WT>
+__global__ void linear_fwd_kernel(int B, int K, int N, const float* in, const WT* W, const float* bias, float* out,
+                                  int relu) {
+  const int64_t e = (int64_t)blockIdx.x * LB + threadIdx.x;
+  if (e >= (int64_t)B * N) return;
+  const int n = (int)(e % N), b = (int)(e / N);
+  const float* x = in + (int64_t)b * K;
+  float acc = bias[n];
+  for (int k = 0; k < K; k++) acc += ldw(W, (int64_t)n * K + k) * x[k];
+  out[e] = relu ? fmaxf(acc, 0.f) : acc;
+}
+
+template <typename WT>
+__global__ void linear_bwd_data_kernel(int B, int K, int N, const float* in, const WT* W, const float* dout,
+                                       float* din) {
+  const int64_t e = (int64_t)blockIdx.x * LB + threadIdx.x;
+  if (e >= (int64_t)B * K) return;
+  const int k = (int)(e % K), b = (int)(e / K);
+  float acc = 0.f;
+  if (in[e] > 0.f)
+    for (int n = 0; n < N; n++) acc += ldw(W, (int64_t)n * K + k) * dout[(int64_t)b * N + n];
+  din[e] = acc;
+}
+
+__global__ void linear_bwd_weight_kernel(int B, int K, int N, const float* in, const float* dout, float* dW,
+                                         float* db) {
+  const int64_t e = (int64_t)blockIdx.x * LB + threadIdx.x;
+  if (e >= (int64_t)N * K) return;
+  const int k = (int)(e % K), n = (int)(e / K);
+  double acc = 0.0, accb = 0.0;
+  for (int b = 0; b < B; b++) {
+    const float g = dout[(int64_t)b * N + n];
+    acc += (double)g * (double)in[(int64_t)b * K + k];
+    accb += g;
+  }
+  dW[e] = (float)acc;
+  if (k == 0) db[n] = (float)accb;
+}
+
+// mean squared error over B x NB outputs; dout = d loss / d out
+__global__ __launch_bounds__(LB) void mse_kernel(int B, int NB, const float* out, const float* Y, const int32_t* idx,
+                                                 float* dout, double* part) {
+  const int64_t e = (int64_t)blockIdx.x * LB + threadIdx.x;
+  double l = 0.0;
+  if (e < (int64_t)B * NB) {
+    const int b = (int)(e / NB), j = (int)(e - (int64_t)b * NB);
+    const float diff = out[e] - Y[(int64_t)idx[b] * NB + j];
+    const float inv = 1.f / (float)(B * NB);
+    dout[e] = 2.f * diff * inv;
+    l = (double)diff * (double)diff / (double)(B * NB);
+  }
+  const double s = lblock_sum(l);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ void adam_kernel(int64_t n, float* w, uint16_t* w16, const float* g, float* m, float* v, float lr,
+                            float b1, float b2, float c1, float c2, float eps) {
+  const int64_t i = (int64_t)blockIdx.x * LB + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  w[i] = wi;
+  w16[i] = f2bf(wi);
+}
+
+__global__ void to_bf16_kernel(int64_t n, const float* w, uint16_t* w16) {
+  const int64_t i = (int64_t)blockIdx.x * LB + threadIdx.x;
+  if (i < n) w16[i] = f2bf(w[i]);
+}
+
+// ---- inference: one sample, one workgroup ------------------------------------
+struct InferArgs {
+  int L0, C, NL, H, NB;
+  int L[8];
+  int64_t w_off[8], b_off[8];
+};
+
+__global__ __launch_bounds__(1024) void infer_kernel(InferArgs a, const double* x, const double* base, int64_t nflt,
+                                                     const uint16_t* w16, const float* w32, float* bufA, float* bufB,
+                                                     float* coef) {
+  __shared__ double red[16];
+  __shared__ float s_norm;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  // input: dF / |dF|
+  double ss = 0.0;
+  for (int64_t i = tid; i < nflt; i += nt) {
+    const double v = x[i] - base[i];
+    ss += v * v;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) ss += __shfl_down(ss, off, 64);
+  if ((tid & 63) == 0) red[tid >> 6] = ss;
+  __syncthreads();
+  if (tid == 0) {
+    double t = 0;
+    for (int k = 0; k < nt / 64; k++) t += red[k];
+    s_norm = (float)sqrt(t);
+  }
+  __syncthreads();
+  const float nrm = s_norm;
+  const float inv = nrm > 0.f ? 1.f / nrm : 0.f;
+  for (int64_t i = tid; i < nflt; i += nt) bufA[i] = (float)(x[i] - base[i]) * inv;
+  __syncthreads();
+  float* in = bufA;
+  float* out = bufB;
+  int Cin = 1;
+  for (int l = 1; l <= a.NL; l++) {
+    const int Lin = a.L[l - 1], Lout = a.L[l];
+    const uint16_t* W = w16 + a.w_off[l - 1];
+    const float* bias = w32 + a.b_off[l - 1];
+    for (int e = tid; e < a.C * Lout; e += nt) {
+      const int co = e / Lout, o = e - co * Lout;
+      float acc = bias[co];
+      for (int ci = 0; ci < Cin; ci++) {
+        const int64_t wb = ((int64_t)co * Cin + ci) * 5;
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+          const int i = 2 * o + k - 2;
+          if (i >= 0 && i < Lin) acc += bf2f(W[wb + k]) * in[ci * Lin + i];
+        }
+      }
+      out[e] = fmaxf(acc, 0.f);
+    }
+    __syncthreads();
+    float* t = in;
+    in = out;
+    out = t;
+    Cin = a.C;
+  }
+  const int K = a.C * a.L[a.NL];
+  {
+    const uint16_t* W = w16 + a.w_off[a.NL];
+    const float* bias = w32 + a.b_off[a.NL];
+    for (int n = tid; n < a.H; n += nt) {
+      float acc = bias[n];
+      for (int k = 0; k < K; k++) acc += bf2f(W[(int64_t)n * K + k]) * in[k];
+      out[n] = fmaxf(acc, 0.f);
+    }
+  }
+  __syncthreads();
+  {
+    const uint16_t* W = w16 + a.w_off[a.NL + 1];
+    const float* bias = w32 + a.b_off[a.NL + 1];
+    for (int n = tid; n < a.NB; n += nt) {
+      float acc = bias[n];
+      for (int k = 0; k < a.H; k++) acc += bf2f(W[(int64_t)n * a.H + k]) * out[k];
+      coef[n] = acc * nrm;
+    }
+  }
+  if (tid == 0) coef[a.NB] = nrm;
+}
+
+// p[off + i] = sum_k Phi[k, i] coef[k]; p[0:off] = 0
+__global__ __launch_bounds__(LB) void guess_kernel(int64_t nI, int64_t off, int NB, const float* Phi,
+                                                   const float* coef, double* p) {
+  __shared__ float c[NBMAX];
+  if (threadIdx.x < NB) c[threadIdx.x] = coef[threadIdx.x];
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * LB + threadIdx.x;
+  if (i < off) p[i] = 0.0;
+  if (i >= nI) return;
+  float acc = 0.f;
+  for (int k = 0; k < NB; k++) acc += Phi[(int64_t)k * nI + i] * c[k];
+  p[off + i] = (double)acc;
+}
+
+template <typename T>
+T* dalloc(int64_t n) {
+  void* p = nullptr;
+  if (hipMalloc(&p, sizeof(T) * (size_t)std::max<int64_t>(n, 1)) != hipSuccess) return nullptr;
+  return (T*)p;
+}
+
+uint64_t smix(uint64_t& s) {
+  uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+double sm_uniform(uint64_t& s) { return (double)(smix(s) >> 11) * (1.0 / 9007199254740992.0); }
+
+InferArgs infer_args(const NetDims& d) {
+  InferArgs a{};
+  a.L0 = d.L0; a.C = d.C; a.NL = d.NL; a.H = d.H; a.NB = d.NB;
+  for (int l = 0; l <= d.NL; l++) a.L[l] = d.L[l];
+  for (int l = 0; l < d.NL + 2; l++) { a.w_off[l] = d.w_off[l]; a.b_off[l] = d.b_off[l]; }
+  return a;
+}
+}  // namespace
+
+void NetDims::init(int l0, int c, int nl, int h, int nb) {
+  L0 = l0; C = c; NL = nl; H = h; NB = nb;
+  L[0] = l0;
+  for (int l = 1; l <= NL; l++) L[l] = (L[l - 1] - 1) / 2 + 1;
+  int64_t p = 0;
+  for (int l = 1; l <= NL; l++) {
+    const int cin = (l == 1) ? 1 : C;
+    w_off[l - 1] = p; p += (int64_t)C * cin * 5;
+    b_off[l - 1] = p; p += C;
+  }
+  w_off[NL] = p; p += (int64_t)H * flat();
+  b_off[NL] = p; p += H;
+  w_off[NL + 1] = p; p += (int64_t)NB * H;
+  b_off[NL + 1] = p; p += NB;
+  nparams = p;
+}
+
+int learn_init(LearnState* ls, int64_t nI, int64_t nflt, int64_t offset, int cap, int C, int H, int NB, uint64_t seed,
+               std::string* err) {
+  learn_free(ls);
+  if (NB < 1 || NB > NBMAX) { *err = "nbasis must be 1..64"; return 1; }
+  if (cap < NB) { *err = "snapshot capacity must be >= nbasis"; return 1; }
+  ls->nI = nI; ls->nflt = nflt; ls->offset = offset; ls->cap = cap;
+  ls->d.init((int)nflt, C, 4, H, NB);
+  const NetDims& d = ls->d;
+  ls->S = dalloc<float>(nI * (int64_t)cap);
+  ls->F = dalloc<float>(nflt * (int64_t)cap);
+  ls->Phi = dalloc<float>(nI * (int64_t)NB);
+  ls->X = dalloc<float>(nflt * (int64_t)cap);
+  ls->Y = dalloc<float>((int64_t)NB * cap);
+  ls->w32 = dalloc<float>(d.nparams);
+  ls->w16 = dalloc<uint16_t>(d.nparams);
+  ls->grad = dalloc<float>(d.nparams);
+  ls->adam_m = dalloc<float>(d.nparams);
+  ls->adam_v = dalloc<float>(d.nparams);
+  ls->inf_act = dalloc<float>(2 * (int64_t)std::max<int64_t>(nflt, (int64_t)C * d.L[1]) + 2 * H);
+  ls->coef = dalloc<float>(NBMAX + 1);
+  if (!ls->S || !ls->F || !ls->Phi || !ls->X || !ls->Y || !ls->w32 || !ls->w16 || !ls->grad || !ls->adam_m ||
+      !ls->adam_v || !ls->inf_act || !ls->coef) {
+    *err = "device allocation failed (learned guess)";
+    learn_free(ls);
+    return 7;
+  }
+  // He-uniform weights, zero biases (splitmix64 stream from seed)
+  std::vector<float> w(d.nparams, 0.f);
+  uint64_t st = seed;
+  auto fill = [&](int64_t off, int64_t n, double fan_in) {
+    const double a = std::sqrt(6.0 / fan_in);
+    for (int64_t i = 0; i < n; i++) w[off + i] = (float)((2.0 * sm_uniform(st) - 1.0) * a);
+  };
+  for (int l = 1; l <= d.NL; l++) {
+    const int cin = (l == 1) ? 1 : d.C;
+    fill(d.w_off[l - 1], (int64_t)d.C * cin * 5, cin * 5.0);
+  }
+  fill(d.w_off[d.NL], (int64_t)d.H * d.flat(), d.flat());
+  fill(d.w_off[d.NL + 1], (int64_t)d.NB * d.H, d.H * 4.0);
+  LCHK(hipMemcpy(ls->w32, w.data(), sizeof(float) * d.nparams, hipMemcpyHostToDevice));
+  LCHK(hipMemset(ls->adam_m, 0, sizeof(float) * d.nparams));
+  LCHK(hipMemset(ls->adam_v, 0, sizeof(float) * d.nparams));
+  hipLaunchKernelGGL(to_bf16_kernel, dim3(gblk(d.nparams)), dim3(LB), 0, 0, d.nparams, ls->w32, ls->w16);
+  ls->adam_t = 0;
+  ls->nsnap = 0;
+  ls->trained = 0;
+  ls->nbasis_ready = 0;
+  return hipDeviceSynchronize() == hipSuccess ? 0 : 2;
+}
+
+void learn_free(LearnState* ls) {
+  void* ptrs[] = {ls->S, ls->F, ls->Phi, ls->X, ls->Y, ls->w32, ls->w16, ls->grad, ls->adam_m, ls->adam_v,
+                  ls->act, ls->dact, ls->out, ls->idx, ls->loss_part, ls->inf_act, ls->coef};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  NetDims d = ls->d;
+  *ls = LearnState();
+  ls->d = d;
+}
+
+void learn_record(LearnState* ls, const double* d, const double* dPre, hipStream_t s) {
+  if (ls->nsnap >= ls->cap) return;
+  const int j = ls->nsnap++;
+  const int64_t n = std::max(ls->nI, ls->nflt);
+  hipLaunchKernelGGL(record_kernel, dim3(gblk(n)), dim3(LB), 0, s, d, dPre, ls->nflt, ls->offset, ls->nI,
+                     ls->F + (int64_t)j * ls->nflt, ls->S + (int64_t)j * ls->nI);
+}
+
+int learn_gram(LearnState* ls, double* G_host, hipStream_t s, std::string* err) {
+  const int m = ls->nsnap;
+  if (m < 1) { *err = "no snapshots"; return 6; }
+  double* G = dalloc<double>((int64_t)m * m);
+  if (!G) { *err = "device allocation failed"; return 7; }
+  const int nt = (m + GT - 1) / GT;
+  hipLaunchKernelGGL(gram_kernel, dim3(nt, nt), dim3(256), 0, s, ls->S, ls->nI, m, G);
+  std::vector<double> h((int64_t)m * m);
+  LCHK(hipMemcpyAsync(h.data(), G, sizeof(double) * m * m, hipMemcpyDeviceToHost, s));
+  const hipError_t e = hipStreamSynchronize(s);
+  (void)hipFree(G);
+  if (e != hipSuccess) { *err = hipGetErrorString(e); return 2; }
+  for (int i = 0; i < m; i++)
+    for (int j = 0; j < m; j++) G_host[(int64_t)i * m + j] = (i <= j) ? h[(int64_t)i * m + j] : h[(int64_t)j * m + i];
+  return 0;
+}
+
+int learn_set_basis(LearnState* ls, const double* V, const double* sigma, hipStream_t s, std::string* err) {
+  const int m = ls->nsnap, NB = ls->d.NB, L0 = ls->d.L0;
+  if (m < NB) { *err = "fewer snapshots than basis vectors"; return 6; }
+  std::vector<float> W((int64_t)m * NB);
+  for (int j = 0; j < m; j++)
+    for (int k = 0; k < NB; k++) W[(int64_t)j * NB + k] = sigma[k] > 0 ? (float)(V[(int64_t)j * NB + k] / sigma[k]) : 0.f;
+  float* dW = dalloc<float>((int64_t)m * NB);
+  if (!dW) { *err = "device allocation failed"; return 7; }
+  LCHK(hipMemcpyAsync(dW, W.data(), sizeof(float) * W.size(), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(basis_kernel, dim3((unsigned)((ls->nI + 255) / 256)), dim3(256), 0, s, ls->S, ls->nI, m, dW, NB,
+                     ls->Phi);
+  // dataset: X_j = dF_j / |dF_j|, Y_jk = sigma_k V_jk / |dF_j|
+  std::vector<float> F((int64_t)m * L0), X((int64_t)m * L0), Y((int64_t)m * NB);
+  LCHK(hipMemcpyAsync(F.data(), ls->F, sizeof(float) * F.size(), hipMemcpyDeviceToHost, s));
+  hipError_t e = hipStreamSynchronize(s);
+  (void)hipFree(dW);
+  if (e != hipSuccess) { *err = hipGetErrorString(e); return 2; }
+  for (int j = 0; j < m; j++) {
+    double nrm = 0;
+    for (int i = 0; i < L0; i++) nrm += (double)F[(int64_t)j * L0 + i] * F[(int64_t)j * L0 + i];
+    nrm = std::sqrt(nrm);
+    const double inv = nrm > 0 ? 1.0 / nrm : 0.0;
+    for (int i = 0; i < L0; i++) X[(int64_t)j * L0 + i] = (float)(F[(int64_t)j * L0 + i] * inv);
+    for (int k = 0; k < NB; k++) Y[(int64_t)j * NB + k] = (float)(sigma[k] * V[(int64_t)j * NB + k] * inv);
+  }
+  LCHK(hipMemcpyAsync(ls->X, X.data(), sizeof(float) * X.size(), hipMemcpyHostToDevice, s));
+  LCHK(hipMemcpyAsync(ls->Y, Y.data(), sizeof(float) * Y.size(), hipMemcpyHostToDevice, s));
+  e = hipStreamSynchronize(s);
+  if (e != hipSuccess) { *err = hipGetErrorString(e); return 2; }
+  ls->nbasis_ready = 1;
+  return 0;
+}
+
+static int ensure_batch(LearnState* ls, int B) {
+  if (ls->bmax >= B) return 0;
+  for (void* p : {(void*)ls->act, (void*)ls->dact, (void*)ls->out, (void*)ls->idx, (void*)ls->loss_part})
+    if (p) (void)hipFree(p);
+  const NetDims& d = ls->d;
+  int64_t off = 0;
+  ls->act_off[0] = off; off += (int64_t)B * d.L0;
+  for (int l = 1; l <= d.NL; l++) { ls->act_off[l] = off; off += (int64_t)B * d.C * d.L[l]; }
+  ls->act_off[d.NL + 1] = off; off += (int64_t)B * d.H;
+  ls->act_off[d.NL + 2] = off;
+  ls->act = dalloc<float>(off);
+  ls->dact = dalloc<float>(off);
+  ls->out = dalloc<float>((int64_t)B * d.NB * 2);
+  ls->idx = dalloc<int32_t>(B);
+  ls->loss_part = dalloc<double>(gblk((int64_t)B * d.NB) + 1);
+  ls->bmax = B;
+  return (ls->act && ls->dact && ls->out && ls->idx && ls->loss_part) ? 0 : 7;
+}
+
+// forward + backward of one batch; fills grad, returns the loss partials in loss_part
+template <typename WT>
+static void fwd_bwd(LearnState* ls, const WT* W, const float* w32, int B, hipStream_t s) {
+  const NetDims& d = ls->d;
+  float* A = ls->act;
+  float* dA = ls->dact;
+  float* out = ls->out;
+  float* dout = ls->out + (int64_t)B * d.NB;
+  hipLaunchKernelGGL(gather_kernel, dim3(gblk((int64_t)B * d.L0)), dim3(LB), 0, s, ls->X, d.L0, ls->idx, B,
+                     A + ls->act_off[0]);
+  for (int l = 1; l <= d.NL; l++) {
+    const int cin = (l == 1) ? 1 : d.C;
+    hipLaunchKernelGGL((conv_fwd_kernel<WT>), dim3(gblk((int64_t)B * d.C * d.L[l])), dim3(LB), 0, s, B, cin,
+                       d.L[l - 1], d.C, d.L[l], A + ls->act_off[l - 1], W + d.w_off[l - 1], w32 + d.b_off[l - 1],
+                       A + ls->act_off[l]);
+  }
+  const int K = d.flat();
+  hipLaunchKernelGGL((linear_fwd_kernel<WT>), dim3(gblk((int64_t)B * d.H)), dim3(LB), 0, s, B, K, d.H,
+                     A + ls->act_off[d.NL], W + d.w_off[d.NL], w32 + d.b_off[d.NL], A + ls->act_off[d.NL + 1], 1);
+  hipLaunchKernelGGL((linear_fwd_kernel<WT>), dim3(gblk((int64_t)B * d.NB)), dim3(LB), 0, s, B, d.H, d.NB,
+                     A + ls->act_off[d.NL + 1], W + d.w_off[d.NL + 1], w32 + d.b_off[d.NL + 1], out, 0);
+  hipLaunchKernelGGL(mse_kernel, dim3(gblk((int64_t)B * d.NB)), dim3(LB), 0, s, B, d.NB, out, ls->Y, ls->idx, dout,
+                     ls->loss_part);
+  // backward
+  float* g = ls->grad;
+  hipLaunchKernelGGL(linear_bwd_weight_kernel, dim3(gblk((int64_t)d.NB * d.H)), dim3(LB), 0, s, B, d.H, d.NB,
+                     A + ls->act_off[d.NL + 1], dout, g + d.w_off[d.NL + 1], g + d.b_off[d.NL + 1]);
+  hipLaunchKernelGGL((linear_bwd_data_kernel<WT>), dim3(gblk((int64_t)B * d.H)), dim3(LB), 0, s, B, d.H, d.NB,
+                     A + ls->act_off[d.NL + 1], W + d.w_off[d.NL + 1], dout, dA + ls->act_off[d.NL + 1]);
+  hipLaunchKernelGGL(linear_bwd_weight_kernel, dim3(gblk((int64_t)d.H * K)), dim3(LB), 0, s, B, K, d.H,
+                     A + ls->act_off[d.NL], dA + ls->act_off[d.NL + 1], g + d.w_off[d.NL], g + d.b_off[d.NL]);
+  hipLaunchKernelGGL((linear_bwd_data_kernel<WT>), dim3(gblk((int64_t)B * K)), dim3(LB), 0, s, B, K, d.H,
+                     A + ls->act_off[d.NL], W + d.w_off[d.NL], dA + ls->act_off[d.NL + 1], dA + ls->act_off[d.NL]);
+  for (int l = d.NL; l >= 1; l--) {
+    const int cin = (l == 1) ? 1 : d.C;
+    hipLaunchKernelGGL(conv_bwd_weight_kernel, dim3(d.C * cin), dim3(LB), 0, s, B, cin, d.L[l - 1], d.C, d.L[l],
+                       A + ls->act_off[l - 1], dA + ls->act_off[l], g + d.w_off[l - 1], g + d.b_off[l - 1]);
+    if (l > 1)
+      hipLaunchKernelGGL((conv_bwd_data_kernel<WT>), dim3(gblk((int64_t)B * cin * d.L[l - 1])), dim3(LB), 0, s, B,
+                         cin, d.L[l - 1], d.C, d.L[l], A + ls->act_off[l - 1], W + d.w_off[l - 1],
+                         dA + ls->act_off[l], dA + ls->act_off[l - 1]);
+  }
+}
+
+int learn_train(LearnState* ls, int epochs, int batch, double lr, uint64_t seed, double* hist, hipStream_t s,
+                std::string* err) {
+  if (!ls->nbasis_ready) { *err = "POD basis not set"; return 6; }
+  const int m = ls->nsnap;
+  const int B = std::min(batch, m);
+  if (B < 1) { *err = "empty dataset"; return 6; }
+  if (ensure_batch(ls, B)) { *err = "device allocation failed"; return 7; }
+  const NetDims& d = ls->d;
+  const int nparts = gblk((int64_t)B * d.NB);
+  std::vector<int32_t> perm(m);
+  std::vector<double> parts(nparts);
+  uint64_t st = seed;
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  for (int ep = 0; ep < epochs; ep++) {
+    for (int i = 0; i < m; i++) perm[i] = i;
+    for (int i = m - 1; i > 0; i--) std::swap(perm[i], perm[smix(st) % (uint64_t)(i + 1)]);
+    double eloss = 0;
+    int nb = 0;
+    for (int b0 = 0; b0 + B <= m; b0 += B) {
+      LCHK(hipMemcpyAsync(ls->idx, perm.data() + b0, sizeof(int32_t) * B, hipMemcpyHostToDevice, s));
+      fwd_bwd<uint16_t>(ls, ls->w16, ls->w32, B, s);
+      ls->adam_t++;
+      const float c1 = 1.f - std::pow(b1, (float)ls->adam_t), c2 = 1.f - std::pow(b2, (float)ls->adam_t);
+      hipLaunchKernelGGL(adam_kernel, dim3(gblk(d.nparams)), dim3(LB), 0, s, d.nparams, ls->w32, ls->w16, ls->grad,
+                         ls->adam_m, ls->adam_v, (float)lr, b1, b2, c1, c2, eps);
+      LCHK(hipMemcpyAsync(parts.data(), ls->loss_part, sizeof(double) * nparts, hipMemcpyDeviceToHost, s));
+      const hipError_t e = hipStreamSynchronize(s);
+      if (e != hipSuccess) { *err = hipGetErrorString(e); return 2; }
+      double l = 0;
+      for (double p : parts) l += p;
+      eloss += l;
+      nb++;
+    }
+    if (hist) hist[ep] = nb ? eloss / nb : 0.0;
+  }
+  ls->trained = 1;
+  return 0;
+}
+
+int learn_loss_grad(LearnState* ls, const float* params, int nsamp, const int32_t* idx, int use_bf16, double* loss,
+                    float* grad_host, hipStream_t s, std::string* err) {
+  if (!ls->nbasis_ready) { *err = "POD basis not set"; return 6; }
+  if (nsamp < 1 || nsamp > ls->nsnap) { *err = "bad sample count"; return 1; }
+  if (ensure_batch(ls, nsamp)) { *err = "device allocation failed"; return 7; }
+  const NetDims& d = ls->d;
+  for (int i = 0; i < nsamp; i++)
+    if (idx[i] < 0 || idx[i] >= ls->nsnap) { *err = "sample index out of range"; return 1; }
+  if (params) {
+    LCHK(hipMemcpyAsync(ls->w32, params, sizeof(float) * d.nparams, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(to_bf16_kernel, dim3(gblk(d.nparams)), dim3(LB), 0, s, d.nparams, ls->w32, ls->w16);
+  }
+  LCHK(hipMemcpyAsync(ls->idx, idx, sizeof(int32_t) * nsamp, hipMemcpyHostToDevice, s));
+  if (use_bf16) fwd_bwd<uint16_t>(ls, ls->w16, ls->w32, nsamp, s);
+  else fwd_bwd<float>(ls, ls->w32, ls->w32, nsamp, s);
+  const int nparts = gblk((int64_t)nsamp * d.NB);
+  std::vector<double> parts(nparts);
+  LCHK(hipMemcpyAsync(parts.data(), ls->loss_part, sizeof(double) * nparts, hipMemcpyDeviceToHost, s));
+  if (grad_host) LCHK(hipMemcpyAsync(grad_host, ls->grad, sizeof(float) * d.nparams, hipMemcpyDeviceToHost, s));
+  const hipError_t e = hipStreamSynchronize(s);
+  if (e != hipSuccess) { *err = hipGetErrorString(e); return 2; }
+  double l = 0;
+  for (double p : parts) l += p;
+  *loss = l;
+  return 0;
+}
+
+void learn_guess(LearnState* ls, const double* x, const double* base, double* p, hipStream_t s) {
+  float* bufA = ls->inf_act;
+  float* bufB = ls->inf_act + std::max<int64_t>(ls->nflt, (int64_t)ls->d.C * ls->d.L[1]) + ls->d.H;
+  hipLaunchKernelGGL(infer_kernel, dim3(1), dim3(1024), 0, s, infer_args(ls->d), x, base, ls->nflt, ls->w16, ls->w32,
+                     bufA, bufB, ls->coef);
+  const int64_t n = std::max(ls->nI, ls->offset);
+  hipLaunchKernelGGL(guess_kernel, dim3((unsigned)((n + LB - 1) / LB)), dim3(LB), 0, s, ls->nI, ls->offset, ls->d.NB,
+                     ls->Phi, ls->coef, p);
+}
+
+int learn_get_basis(LearnState* ls, float* Phi_host, hipStream_t s) {
+  if (hipMemcpyAsync(Phi_host, ls->Phi, sizeof(float) * ls->nI * ls->d.NB, hipMemcpyDeviceToHost, s) != hipSuccess)
+    return 2;
+  return hipStreamSynchronize(s) == hipSuccess ? 0 : 2;
+}
+
+int learn_get_params(LearnState* ls, float* out, hipStream_t s) {
+  if (hipMemcpyAsync(out, ls->w32, sizeof(float) * ls->d.nparams, hipMemcpyDeviceToHost, s) != hipSuccess) return 2;
+  return hipStreamSynchronize(s) == hipSuccess ? 0 : 2;
+}
+
+int learn_predict(LearnState* ls, const double* dF_host, double* coef_host, hipStream_t s, std::string* err) {
+  double* x = dalloc<double>(ls->nflt);
+  double* z = dalloc<double>(ls->nflt);
+  if (!x || !z) { *err = "device allocation failed"; return 7; }
+  LCHK(hipMemcpyAsync(x, dF_host, sizeof(double) * ls->nflt, hipMemcpyHostToDevice, s));
+  LCHK(hipMemsetAsync(z, 0, sizeof(double) * ls->nflt, s));
+  float* bufA = ls->inf_act;
+  float* bufB = ls->inf_act + std::max<int64_t>(ls->nflt, (int64_t)ls->d.C * ls->d.L[1]) + ls->d.H;
+  hipLaunchKernelGGL(infer_kernel, dim3(1), dim3(1024), 0, s, infer_args(ls->d), x, z, ls->nflt, ls->w16, ls->w32,
+                     bufA, bufB, ls->coef);
+  std::vector<float> c(ls->d.NB);
+  LCHK(hipMemcpyAsync(c.data(), ls->coef, sizeof(float) * ls->d.NB, hipMemcpyDeviceToHost, s));
+  const hipError_t e = hipStreamSynchronize(s);
+  (void)hipFree(x);
+  (void)hipFree(z);
+  if (e != hipSuccess) { *err = hipGetErrorString(e); return 2; }
+  for (int k = 0; k < ls->d.NB; k++) coef_host[k] = c[k];
+  return 0;
+}
+
+}  // namespace eq

@@ -1,0 +1,43 @@
+"""Learned initial guess workflow for the quasi-static solve (config 4 / 5).
+
+1. ``record``: run the plain solver with snapshot recording on; each pass-1
+   quasi-static solve stores (fault increment dF, interior increment dd).
+2. ``fit_basis``: POD by the method of snapshots -- the Gram matrix S^T S is
+   formed on the GPU, eigen-decomposed here (numpy), and Phi = S V / sigma is
+   formed on the GPU; the training targets are sigma_k V_jk (Phi^T dd_j).
+3. ``train``: Adam on the conv1d + MLP network (hand-written GPU kernels).
+4. ``enable``: set eq_solver_opts.guess = learned; the previous-step guess stays
+   the fallback (energy-optimal step along the learned correction).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def record(ctx, total_time=None, max_steps=0, capacity=2000, nbasis=64, seed=4):
+    ctx.learn_init(capacity=capacity, nbasis=nbasis, seed=seed)
+    ctx.learn_record(True)
+    out = ctx.run(total_time=total_time, max_steps=max_steps, snapshots=False)
+    ctx.learn_record(False)
+    return out
+
+
+def fit_basis(ctx, nbasis=None):
+    nb = nbasis or ctx.nbasis
+    G = ctx.learn_gram()
+    w, V = np.linalg.eigh(G)
+    order = np.argsort(w)[::-1][:nb]
+    w = np.maximum(w[order], 0.0)
+    V = V[:, order]
+    sigma = np.sqrt(w)
+    ctx.learn_set_basis(np.ascontiguousarray(V), sigma)
+    energy = w.sum() / max(np.trace(G), 1e-300)
+    return dict(sigma=sigma, energy_captured=float(energy))
+
+
+def train(ctx, epochs=300, batch=64, lr=1e-3, seed=11):
+    return ctx.learn_train(epochs=epochs, batch=batch, lr=lr, seed=seed)
+
+
+def enable(ctx, tol, pc="amg", prec="fp64"):
+    ctx.set_solver(tol=tol, pc=pc, prec=prec, guess="learned")

@@ -18,7 +18,7 @@ EQ_OK, EQ_EINVAL, EQ_HIP_ERR, EQ_NR_NOCONV, EQ_NR_NAN, EQ_PCG_NOCONV, EQ_ENOTREA
 STATUS_NAMES = {0: "EQ_OK", 1: "EQ_EINVAL", 2: "EQ_HIP_ERR", 3: "EQ_NR_NOCONV", 4: "EQ_NR_NAN",
                 5: "EQ_PCG_NOCONV", 6: "EQ_ENOTREADY", 7: "EQ_NOMEM"}
 EQ_MASK_ALL, EQ_MASK_INTERIOR, EQ_MASK_NEG = 0, 1, 2
-EQ_PREC_FP64, EQ_PREC_FP32, EQ_PREC_MIXED = 0, 1, 2
+EQ_PREC_FP64, EQ_PREC_FP32, EQ_PREC_MIXED, EQ_PREC_F32PC = 0, 1, 2, 3
 EQ_PC_NONE, EQ_PC_JACOBI, EQ_PC_AMG = 0, 1, 2
 EQ_SMOOTH_JACOBI, EQ_SMOOTH_CHEBYSHEV = 0, 1
 EQ_GUESS_PREVIOUS, EQ_GUESS_LEARNED = 0, 1
@@ -29,13 +29,16 @@ EXPORTS = [
     "eq_ku_apply", "eq_kni_apply", "eq_element_computation", "eq_amg_setup", "eq_amg_info",
     "eq_amg_ldiv", "eq_cg", "eq_pcg_jacobi", "eq_ids", "eq_ids2", "eq_nrsearch", "eq_fbc",
     "eq_slr", "eq_dtevol", "eq_set_solver", "eq_state_init", "eq_state_get", "eq_state_set",
-    "eq_step", "eq_run", "eq_time_kernel",
+    "eq_step", "eq_run", "eq_time_kernel", "eq_learn_init", "eq_learn_record", "eq_learn_info",
+    "eq_learn_gram", "eq_learn_set_basis", "eq_learn_train", "eq_learn_loss_grad", "eq_learn_get",
+    "eq_learn_predict",
 ]
 EQ_TIME_KU, EQ_TIME_CG_DIR, EQ_TIME_VCYCLE, EQ_TIME_CG_ITER, EQ_TIME_NEWMARK, EQ_TIME_CHEB = range(6)
 
 dp = C.POINTER(C.c_double)
 i64p = C.POINTER(C.c_int64)
 i32p = C.POINTER(C.c_int32)
+fp = C.POINTER(C.c_float)
 
 
 class MeshDesc(C.Structure):
@@ -119,6 +122,15 @@ def load():
         "eq_step": ([vp, C.POINTER(StepInfo)], C.c_int),
         "eq_run": ([vp, C.c_double, C.c_int64, C.POINTER(Results)], C.c_int),
         "eq_time_kernel": ([vp, C.c_int, C.c_int, C.c_int, dp], C.c_int),
+        "eq_learn_init": ([vp, C.c_int64, C.c_int, C.c_int, C.c_int, C.c_uint64], C.c_int),
+        "eq_learn_record": ([vp, C.c_int], C.c_int),
+        "eq_learn_info": ([vp, i64p, i64p, C.POINTER(C.c_int), i64p], C.c_int),
+        "eq_learn_gram": ([vp, dp], C.c_int),
+        "eq_learn_set_basis": ([vp, dp, dp], C.c_int),
+        "eq_learn_train": ([vp, C.c_int, C.c_int, C.c_double, C.c_uint64, dp], C.c_int),
+        "eq_learn_loss_grad": ([vp, fp, C.c_int, i32p, C.c_int, dp, fp], C.c_int),
+        "eq_learn_get": ([vp, fp, fp, fp, fp], C.c_int),
+        "eq_learn_predict": ([vp, dp, dp], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

@@ -35,7 +35,7 @@ enum {
 };
 
 enum { EQ_MASK_ALL = 0, EQ_MASK_INTERIOR = 1, EQ_MASK_NEG = 2 };
-enum { EQ_PREC_FP64 = 0, EQ_PREC_FP32 = 1, EQ_PREC_MIXED = 2 };
+enum { EQ_PREC_FP64 = 0, EQ_PREC_FP32 = 1, EQ_PREC_MIXED = 2, EQ_PREC_F32PC = 3 };
 enum { EQ_PC_NONE = 0, EQ_PC_JACOBI = 1, EQ_PC_AMG = 2 };
 enum { EQ_SMOOTH_JACOBI = 0, EQ_SMOOTH_CHEBYSHEV = 1 };
 enum { EQ_GUESS_PREVIOUS = 0, EQ_GUESS_LEARNED = 1 };
@@ -115,8 +115,9 @@ int eq_amg_ldiv(eq_ctx *ctx, const double *b, double *x, int prec);
 /* ---- conjugate gradients ------------------------------------------------- */
 /* cg!(x, kni, b; Pl, tol, maxiter) (src/main.jl:256): stops when
  * ||b - kni x||_2 <= tol ||b||_2 or after maxiter iterations.  x, b in FltNI
- * order.  pc: EQ_PC_NONE / _JACOBI / _AMG; prec: _FP64 / _FP32 / _MIXED
- * (FP32 inner CG inside FP64 iterative refinement). */
+ * order.  pc: EQ_PC_NONE / _JACOBI / _AMG; prec: _FP64, _FP32, _MIXED (FP32
+ * inner CG inside FP64 iterative refinement) or _F32PC (FP64 CG whose AMG
+ * V-cycle, i.e. most of the K u applications, runs in FP32). */
 int eq_cg(eq_ctx *ctx, double *x, const double *b, double tol, int64_t maxiter, int pc, int prec,
           int64_t *iters, double *relres);
 /* PCG!(...) (src/PCG.jl:8-67): Jacobi CG on K_II with rhs -(K F)[FltNI];
@@ -202,6 +203,36 @@ typedef struct {
 } eq_results;
 /* while t < total_time (and it < max_steps if > 0): eq_step + bookkeeping. */
 int eq_run(eq_ctx *ctx, double total_time, int64_t max_steps, eq_results *res);
+
+/* ---- learned initial guess (no reference counterpart; the plain guess is
+ * src/main.jl:241, dnew = d[FltNI]) ----------------------------------------
+ * Snapshots of the pass-1 quasi-static solves -- fault increment dF and
+ * interior displacement increment -- are recorded on the device; a POD basis
+ * Phi comes from the snapshot Gram matrix (eigen-decomposed by the caller);
+ * a conv1d + MLP network (BF16 weights, FP32 master copy, hand-written
+ * forward / backward / Adam kernels) maps dF/|dF| to POD coefficients.  With
+ * eq_solver_opts.guess = EQ_GUESS_LEARNED the pass-1 start becomes
+ * d + gamma |dF| Phi c, gamma the energy-optimal step, so it never has a
+ * larger K_II-norm error than the previous-step guess. */
+int eq_learn_init(eq_ctx *ctx, int64_t capacity, int channels, int hidden, int nbasis, uint64_t seed);
+int eq_learn_record(eq_ctx *ctx, int on);
+int eq_learn_info(const eq_ctx *ctx, int64_t *nsnap, int64_t *nparams, int *trained, int64_t *nlearned);
+/* G[nsnap x nsnap] = S^T S of the recorded interior increments (row-major). */
+int eq_learn_gram(eq_ctx *ctx, double *G);
+/* V[nsnap x nbasis] (row-major) and sigma[nbasis] from eig(G): Phi = S V / sigma. */
+int eq_learn_set_basis(eq_ctx *ctx, const double *V, const double *sigma);
+/* Adam training (batch, lr); hist[epochs] receives the mean loss per epoch. */
+int eq_learn_train(eq_ctx *ctx, int epochs, int batch, double lr, uint64_t seed, double *hist);
+/* Loss and gradient on samples idx with the given FP32 parameters (NULL: the
+ * current ones); use_bf16 = 0 evaluates with the FP32 weights.  Overwrites the
+ * network parameters when params is given. */
+int eq_learn_loss_grad(eq_ctx *ctx, const float *params, int nsamp, const int32_t *idx, int use_bf16,
+                       double *loss, float *grad);
+/* Copies of parameters [nparams], basis [nI x nbasis] column-major, dataset
+ * inputs [nsnap x FltNglob] and targets [nsnap x nbasis]; any may be NULL. */
+int eq_learn_get(eq_ctx *ctx, float *params, float *Phi, float *X, float *Y);
+/* Network output (POD coefficients times |dF|) for one fault increment. */
+int eq_learn_predict(eq_ctx *ctx, const double *dF, double *coef);
 
 /* ---- measurement hook (no reference counterpart) ------------------------ */
 enum { EQ_TIME_KU = 0, EQ_TIME_CG_DIR = 1, EQ_TIME_VCYCLE = 2, EQ_TIME_CG_ITER = 3, EQ_TIME_NEWMARK = 4,

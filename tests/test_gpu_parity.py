@@ -105,7 +105,7 @@ def test_cg_hierarchies(mesh, hierarchy):
     A = kni.to_scipy().tocsc()
     b = normal(31, A.shape[0])
     xs = spla.spsolve(A, b)
-    for prec in ("fp64", "mixed"):
+    for prec in ("fp64", "mixed", "f32pc"):
         x, it, rr, ok = ctx.cg(np.zeros_like(b), b, tol=1e-10, pc="amg", prec=prec)
         print(f"[gpu] {hierarchy} {prec} iterations {it}", flush=True)
         assert ok
