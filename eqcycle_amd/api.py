@@ -316,10 +316,17 @@ class EqContext:
         self._check(self.lib.eq_learn_gram(self.h, ptr(G)))
         return G
 
-    def learn_set_basis(self, V, sigma):
+    def learn_set_basis(self, V, sigma, use=None):
         V = f64(V)
         sigma = f64(sigma)
-        self._check(self.lib.eq_learn_set_basis(self.h, ptr(V), ptr(sigma)))
+        u = None if use is None else np.ascontiguousarray(use, dtype=np.int32)
+        self._check(self.lib.eq_learn_set_basis(self.h, ptr(V), ptr(sigma), ptr(u, C.c_int32)))
+
+    def learn_norms(self):
+        m = self.learn_info()["nsnap"]
+        out = np.empty(2 * m)
+        self._check(self.lib.eq_learn_norms(self.h, ptr(out)))
+        return out.reshape(m, 2)
 
     def learn_train(self, epochs=200, batch=64, lr=1e-3, seed=11):
         hist = np.zeros(max(epochs, 1))

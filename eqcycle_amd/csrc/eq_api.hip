@@ -74,6 +74,7 @@ struct eq_ctx {
   std::vector<int64_t> out_seis;      // structured
   // state
   bool have_state = false;
+  double* state_blk = nullptr;        // [8 N]: d, v, a, d2, v2, a2, dPre, bq (d/v/a swap with d2/v2/a2)
   double *d = nullptr, *v = nullptr, *a = nullptr, *d2 = nullptr, *v2 = nullptr, *a2 = nullptr;
   double *dPre = nullptr, *bq = nullptr;
   FaultState fs{}, fs_api{};
@@ -1135,14 +1136,18 @@ extern "C" int eq_set_solver(eq_ctx* ctx, const eq_solver_opts* o) {
   return EQ_OK;
 }
 
+// (Re)sets the state pointers to the canonical layout of the state block: the
+// Newmark step swaps d/v/a with d2/v2/a2, so after an odd number of dynamic
+// steps ctx->d is not the block start.
 static int alloc_state(eq_ctx* ctx) {
-  if (ctx->d) return EQ_OK;
   const int64_t N = ctx->N;
-  double* blk = ctx->alloc<double>(8 * N);
-  NEED(blk);
+  if (!ctx->state_blk) {
+    ctx->state_blk = ctx->alloc<double>(8 * N);
+    NEED(ctx->state_blk);
+  }
+  double* blk = ctx->state_blk;
   ctx->d = blk; ctx->v = blk + N; ctx->a = blk + 2 * N; ctx->d2 = blk + 3 * N; ctx->v2 = blk + 4 * N;
   ctx->a2 = blk + 5 * N; ctx->dPre = blk + 6 * N; ctx->bq = blk + 7 * N;
-  launch_fill<double>(8 * N, blk, 0.0, ctx->s);
   return EQ_OK;
 }
 
@@ -1169,8 +1174,7 @@ extern "C" int eq_state_init(eq_ctx* ctx, double v_init, const double* psi_init)
       hpsi[j] = ctx->tauo[j] / (Seff[j] * ccb[j]) - fo[j] / ccb[j] - (cca[j] / ccb[j]) * std::log(2 * v_init / Vo[j]);
   }
   for (int j = 0; j < NX; j++) hVf[j] = (j < j0) ? 0.0 : 2 * v[j];
-  launch_fill<double>(6 * N, ctx->d, 0.0, ctx->s);   // d, v, a and their second buffers
-  launch_fill<double>(2 * N, ctx->dPre, 0.0, ctx->s);
+  launch_fill<double>(8 * N, ctx->state_blk, 0.0, ctx->s);   // d, v, a, their second buffers, dPre, rhs
   launch_fill<double>(11 * NX, ctx->fs.psi, 0.0, ctx->s);
   if (int rc = h2d(ctx, ctx->v, v.data(), N)) return rc;
   if (int rc = h2d(ctx, ctx->fs.psi, hpsi.data(), NX)) return rc;
@@ -1594,11 +1598,19 @@ extern "C" int eq_learn_gram(eq_ctx* ctx, double* G) {
   return EQ_OK;
 }
 
-extern "C" int eq_learn_set_basis(eq_ctx* ctx, const double* V, const double* sigma) {
+extern "C" int eq_learn_set_basis(eq_ctx* ctx, const double* V, const double* sigma, const int32_t* use) {
   if (!ctx || !V || !sigma) return EQ_EINVAL;
   LEARN_READY(ctx);
   std::string err;
-  if (int rc = learn_set_basis(&ctx->learn, V, sigma, ctx->s, &err)) return fail(ctx, rc == 7 ? EQ_NOMEM : EQ_EINVAL, err);
+  if (int rc = learn_set_basis(&ctx->learn, V, sigma, use, ctx->s, &err))
+    return fail(ctx, rc == 7 ? EQ_NOMEM : EQ_EINVAL, err);
+  return EQ_OK;
+}
+
+extern "C" int eq_learn_norms(eq_ctx* ctx, double* out) {
+  if (!ctx || !out) return EQ_EINVAL;
+  LEARN_READY(ctx);
+  if (learn_norms(&ctx->learn, out, ctx->s)) return fail(ctx, EQ_HIP_ERR, "norm copy failed");
   return EQ_OK;
 }
 

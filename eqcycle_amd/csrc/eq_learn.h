@@ -54,6 +54,8 @@ struct LearnState {
   // inference scratch
   float* inf_act = nullptr;
   float* coef = nullptr;     // [NB + 1]: coefficients, last = input scale
+  double* norms = nullptr;   // [2 x cap]: |dF|, |F| per snapshot
+  std::vector<int32_t> used; // snapshots used for the dataset
 };
 
 int learn_init(LearnState* ls, int64_t nI, int64_t nflt, int64_t offset, int cap, int C, int H, int NB,
@@ -64,7 +66,9 @@ void learn_record(LearnState* ls, const double* d, const double* dPre, hipStream
 // G = S^T S (FP64, row-major nsnap x nsnap)
 int learn_gram(LearnState* ls, double* G_host, hipStream_t s, std::string* err);
 // Phi = S V diag(1/sigma); dataset X, Y from the snapshots and sigma V
-int learn_set_basis(LearnState* ls, const double* V, const double* sigma, hipStream_t s, std::string* err);
+int learn_set_basis(LearnState* ls, const double* V, const double* sigma, const int32_t* use, hipStream_t s,
+                    std::string* err);
+int learn_norms(LearnState* ls, double* out, hipStream_t s);
 // Adam training on the dataset; loss per epoch in hist (may be null)
 int learn_train(LearnState* ls, int epochs, int batch, double lr, uint64_t seed, double* hist, hipStream_t s,
                 std::string* err);

@@ -62,6 +62,61 @@ __global__ void record_kernel(const double* d, const double* dPre, int64_t nflt,
   if (i < nI) Scol[i] = (float)(d[off + i] - dPre[off + i]);
 }
 
+// |dF| and |F| (fault row of the new displacement) of one snapshot
+__global__ __launch_bounds__(LB) void snap_norm_kernel(const double* d, const double* dPre, int64_t nflt,
+                                                       double* out) {
+  double a = 0.0, b = 0.0;
+  for (int64_t i = threadIdx.x; i < nflt; i += LB) {
+    const double df = d[i] - dPre[i];
+    a += df * df;
+    b += d[i] * d[i];
+  }
+  a = lblock_sum(a);
+  b = lblock_sum(b);
+  if (threadIdx.x == 0) {
+    out[0] = sqrt(a);
+    out[1] = sqrt(b);
+  }
+}
+
+// inference: x = (d - base)[0:nflt] / |.|, scale stored in coef[NB]
+__global__ __launch_bounds__(LB) void infer_input_kernel(const double* x, const double* base, int64_t nflt, float* a0,
+                                                         float* coef, int NB) {
+  __shared__ double s_n;
+  double ss = 0.0;
+  for (int64_t i = threadIdx.x; i < nflt; i += LB) {
+    const double v = x[i] - base[i];
+    ss += v * v;
+  }
+  ss = lblock_sum(ss);
+  if (threadIdx.x == 0) {
+    s_n = sqrt(ss);
+    coef[NB] = (float)s_n;
+  }
+  __syncthreads();
+  const double inv = s_n > 0 ? 1.0 / s_n : 0.0;
+  for (int64_t i = threadIdx.x; i < nflt; i += LB) a0[i] = (float)((x[i] - base[i]) * inv);
+}
+
+// out[n] = act(bias[n] + W[n, :] . in), one wavefront per row; scale by
+// *scale when given (the final layer multiplies back the input norm)
+__global__ __launch_bounds__(LB) void gemv_rows_kernel(int N, int K, const float* in, const uint16_t* W,
+                                                       const float* bias, float* out, int relu, const float* scale) {
+  const int row = blockIdx.x * (LB / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  float acc = 0.f;
+  if (row < N)
+    for (int k = lane; k < K; k += 64) acc += bf2f(W[(int64_t)row * K + k]) * in[k];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if (row < N && lane == 0) {
+    float v = acc + bias[row];
+    if (relu) v = fmaxf(v, 0.f);
+    if (scale) v *= *scale;
+    out[row] = v;
+  }
+}
+
 // G = S^T S for 64 x 64 tiles with bi <= bj; FP64 accumulation.
 constexpr int GT = 64, GR = 64;
 __global__ __launch_bounds__(256) void gram_kernel(const float* S, int64_t nI, int m, double* G) {
@@ -281,88 +336,6 @@ __global__ void to_bf16_kernel(int64_t n, const float* w, uint16_t* w16) {
   if (i < n) w16[i] = f2bf(w[i]);
 }
 
-// ---- inference: one sample, one workgroup ------------------------------------
-struct InferArgs {
-  int L0, C, NL, H, NB;
-  int L[8];
-  int64_t w_off[8], b_off[8];
-};
-
-__global__ __launch_bounds__(1024) void infer_kernel(InferArgs a, const double* x, const double* base, int64_t nflt,
-                                                     const uint16_t* w16, const float* w32, float* bufA, float* bufB,
-                                                     float* coef) {
-  __shared__ double red[16];
-  __shared__ float s_norm;
-  const int tid = threadIdx.x, nt = blockDim.x;
-  // input: dF / |dF|
-  double ss = 0.0;
-  for (int64_t i = tid; i < nflt; i += nt) {
-    const double v = x[i] - base[i];
-    ss += v * v;
-  }
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) ss += __shfl_down(ss, off, 64);
-  if ((tid & 63) == 0) red[tid >> 6] = ss;
-  __syncthreads();
-  if (tid == 0) {
-    double t = 0;
-    for (int k = 0; k < nt / 64; k++) t += red[k];
-    s_norm = (float)sqrt(t);
-  }
-  __syncthreads();
-  const float nrm = s_norm;
-  const float inv = nrm > 0.f ? 1.f / nrm : 0.f;
-  for (int64_t i = tid; i < nflt; i += nt) bufA[i] = (float)(x[i] - base[i]) * inv;
-  __syncthreads();
-  float* in = bufA;
-  float* out = bufB;
-  int Cin = 1;
-  for (int l = 1; l <= a.NL; l++) {
-    const int Lin = a.L[l - 1], Lout = a.L[l];
-    const uint16_t* W = w16 + a.w_off[l - 1];
-    const float* bias = w32 + a.b_off[l - 1];
-    for (int e = tid; e < a.C * Lout; e += nt) {
-      const int co = e / Lout, o = e - co * Lout;
-      float acc = bias[co];
-      for (int ci = 0; ci < Cin; ci++) {
-        const int64_t wb = ((int64_t)co * Cin + ci) * 5;
-#pragma unroll
-        for (int k = 0; k < 5; k++) {
-          const int i = 2 * o + k - 2;
-          if (i >= 0 && i < Lin) acc += bf2f(W[wb + k]) * in[ci * Lin + i];
-        }
-      }
-      out[e] = fmaxf(acc, 0.f);
-    }
-    __syncthreads();
-    float* t = in;
-    in = out;
-    out = t;
-    Cin = a.C;
-  }
-  const int K = a.C * a.L[a.NL];
-  {
-    const uint16_t* W = w16 + a.w_off[a.NL];
-    const float* bias = w32 + a.b_off[a.NL];
-    for (int n = tid; n < a.H; n += nt) {
-      float acc = bias[n];
-      for (int k = 0; k < K; k++) acc += bf2f(W[(int64_t)n * K + k]) * in[k];
-      out[n] = fmaxf(acc, 0.f);
-    }
-  }
-  __syncthreads();
-  {
-    const uint16_t* W = w16 + a.w_off[a.NL + 1];
-    const float* bias = w32 + a.b_off[a.NL + 1];
-    for (int n = tid; n < a.NB; n += nt) {
-      float acc = bias[n];
-      for (int k = 0; k < a.H; k++) acc += bf2f(W[(int64_t)n * a.H + k]) * out[k];
-      coef[n] = acc * nrm;
-    }
-  }
-  if (tid == 0) coef[a.NB] = nrm;
-}
-
 // p[off + i] = sum_k Phi[k, i] coef[k]; p[0:off] = 0
 __global__ __launch_bounds__(LB) void guess_kernel(int64_t nI, int64_t off, int NB, const float* Phi,
                                                    const float* coef, double* p) {
@@ -392,13 +365,6 @@ uint64_t smix(uint64_t& s) {
 }
 double sm_uniform(uint64_t& s) { return (double)(smix(s) >> 11) * (1.0 / 9007199254740992.0); }
 
-InferArgs infer_args(const NetDims& d) {
-  InferArgs a{};
-  a.L0 = d.L0; a.C = d.C; a.NL = d.NL; a.H = d.H; a.NB = d.NB;
-  for (int l = 0; l <= d.NL; l++) a.L[l] = d.L[l];
-  for (int l = 0; l < d.NL + 2; l++) { a.w_off[l] = d.w_off[l]; a.b_off[l] = d.b_off[l]; }
-  return a;
-}
 }  // namespace
 
 void NetDims::init(int l0, int c, int nl, int h, int nb) {
@@ -438,7 +404,8 @@ int learn_init(LearnState* ls, int64_t nI, int64_t nflt, int64_t offset, int cap
   ls->adam_v = dalloc<float>(d.nparams);
   ls->inf_act = dalloc<float>(2 * (int64_t)std::max<int64_t>(nflt, (int64_t)C * d.L[1]) + 2 * H);
   ls->coef = dalloc<float>(NBMAX + 1);
-  if (!ls->S || !ls->F || !ls->Phi || !ls->X || !ls->Y || !ls->w32 || !ls->w16 || !ls->grad || !ls->adam_m ||
+  ls->norms = dalloc<double>(2 * (int64_t)cap);
+  if (!ls->norms || !ls->S || !ls->F || !ls->Phi || !ls->X || !ls->Y || !ls->w32 || !ls->w16 || !ls->grad || !ls->adam_m ||
       !ls->adam_v || !ls->inf_act || !ls->coef) {
     *err = "device allocation failed (learned guess)";
     learn_free(ls);
@@ -470,7 +437,7 @@ int learn_init(LearnState* ls, int64_t nI, int64_t nflt, int64_t offset, int cap
 
 void learn_free(LearnState* ls) {
   void* ptrs[] = {ls->S, ls->F, ls->Phi, ls->X, ls->Y, ls->w32, ls->w16, ls->grad, ls->adam_m, ls->adam_v,
-                  ls->act, ls->dact, ls->out, ls->idx, ls->loss_part, ls->inf_act, ls->coef};
+                  ls->act, ls->dact, ls->out, ls->idx, ls->loss_part, ls->inf_act, ls->coef, ls->norms};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   NetDims d = ls->d;
@@ -484,6 +451,12 @@ void learn_record(LearnState* ls, const double* d, const double* dPre, hipStream
   const int64_t n = std::max(ls->nI, ls->nflt);
   hipLaunchKernelGGL(record_kernel, dim3(gblk(n)), dim3(LB), 0, s, d, dPre, ls->nflt, ls->offset, ls->nI,
                      ls->F + (int64_t)j * ls->nflt, ls->S + (int64_t)j * ls->nI);
+  hipLaunchKernelGGL(snap_norm_kernel, dim3(1), dim3(LB), 0, s, d, dPre, ls->nflt, ls->norms + 2 * (int64_t)j);
+}
+
+int learn_norms(LearnState* ls, double* out, hipStream_t s) {
+  if (hipMemcpyAsync(out, ls->norms, sizeof(double) * 2 * ls->nsnap, hipMemcpyDeviceToHost, s) != hipSuccess) return 2;
+  return hipStreamSynchronize(s) == hipSuccess ? 0 : 2;
 }
 
 int learn_gram(LearnState* ls, double* G_host, hipStream_t s, std::string* err) {
@@ -503,9 +476,13 @@ int learn_gram(LearnState* ls, double* G_host, hipStream_t s, std::string* err) 
   return 0;
 }
 
-int learn_set_basis(LearnState* ls, const double* V, const double* sigma, hipStream_t s, std::string* err) {
+int learn_set_basis(LearnState* ls, const double* V, const double* sigma, const int32_t* use, hipStream_t s,
+                    std::string* err) {
   const int m = ls->nsnap, NB = ls->d.NB, L0 = ls->d.L0;
-  if (m < NB) { *err = "fewer snapshots than basis vectors"; return 6; }
+  ls->used.clear();
+  for (int j = 0; j < m; j++)
+    if (!use || use[j]) ls->used.push_back(j);
+  if ((int)ls->used.size() < NB) { *err = "fewer usable snapshots than basis vectors"; return 6; }
   std::vector<float> W((int64_t)m * NB);
   for (int j = 0; j < m; j++)
     for (int k = 0; k < NB; k++) W[(int64_t)j * NB + k] = sigma[k] > 0 ? (float)(V[(int64_t)j * NB + k] / sigma[k]) : 0.f;
@@ -521,6 +498,11 @@ int learn_set_basis(LearnState* ls, const double* V, const double* sigma, hipStr
   (void)hipFree(dW);
   if (e != hipSuccess) { *err = hipGetErrorString(e); return 2; }
   for (int j = 0; j < m; j++) {
+    if (use && !use[j]) {
+      for (int i = 0; i < L0; i++) X[(int64_t)j * L0 + i] = 0.f;
+      for (int k = 0; k < NB; k++) Y[(int64_t)j * NB + k] = 0.f;
+      continue;
+    }
     double nrm = 0;
     for (int i = 0; i < L0; i++) nrm += (double)F[(int64_t)j * L0 + i] * F[(int64_t)j * L0 + i];
     nrm = std::sqrt(nrm);
@@ -602,7 +584,7 @@ static void fwd_bwd(LearnState* ls, const WT* W, const float* w32, int B, hipStr
 int learn_train(LearnState* ls, int epochs, int batch, double lr, uint64_t seed, double* hist, hipStream_t s,
                 std::string* err) {
   if (!ls->nbasis_ready) { *err = "POD basis not set"; return 6; }
-  const int m = ls->nsnap;
+  const int m = (int)ls->used.size();
   const int B = std::min(batch, m);
   if (B < 1) { *err = "empty dataset"; return 6; }
   if (ensure_batch(ls, B)) { *err = "device allocation failed"; return 7; }
@@ -613,7 +595,7 @@ int learn_train(LearnState* ls, int epochs, int batch, double lr, uint64_t seed,
   uint64_t st = seed;
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
   for (int ep = 0; ep < epochs; ep++) {
-    for (int i = 0; i < m; i++) perm[i] = i;
+    for (int i = 0; i < m; i++) perm[i] = ls->used[i];
     for (int i = m - 1; i > 0; i--) std::swap(perm[i], perm[smix(st) % (uint64_t)(i + 1)]);
     double eloss = 0;
     int nb = 0;
@@ -665,11 +647,30 @@ int learn_loss_grad(LearnState* ls, const float* params, int nsamp, const int32_
   return 0;
 }
 
-void learn_guess(LearnState* ls, const double* x, const double* base, double* p, hipStream_t s) {
+// Per-layer inference: input normalisation, conv layers (thread per output),
+// wave-per-row GEMVs for the MLP, then p = Phi coef.
+static void infer(LearnState* ls, const double* x, const double* base, hipStream_t s) {
+  const NetDims& d = ls->d;
   float* bufA = ls->inf_act;
-  float* bufB = ls->inf_act + std::max<int64_t>(ls->nflt, (int64_t)ls->d.C * ls->d.L[1]) + ls->d.H;
-  hipLaunchKernelGGL(infer_kernel, dim3(1), dim3(1024), 0, s, infer_args(ls->d), x, base, ls->nflt, ls->w16, ls->w32,
-                     bufA, bufB, ls->coef);
+  float* bufB = ls->inf_act + std::max<int64_t>(ls->nflt, (int64_t)d.C * d.L[1]) + d.H;
+  hipLaunchKernelGGL(infer_input_kernel, dim3(1), dim3(LB), 0, s, x, base, ls->nflt, bufA, ls->coef, d.NB);
+  float* in = bufA;
+  float* out = bufB;
+  for (int l = 1; l <= d.NL; l++) {
+    const int cin = (l == 1) ? 1 : d.C;
+    hipLaunchKernelGGL((conv_fwd_kernel<uint16_t>), dim3(gblk((int64_t)d.C * d.L[l])), dim3(LB), 0, s, 1, cin,
+                       d.L[l - 1], d.C, d.L[l], in, ls->w16 + d.w_off[l - 1], ls->w32 + d.b_off[l - 1], out);
+    std::swap(in, out);
+  }
+  hipLaunchKernelGGL(gemv_rows_kernel, dim3((d.H + 3) / 4), dim3(LB), 0, s, d.H, d.flat(), in, ls->w16 + d.w_off[d.NL],
+                     ls->w32 + d.b_off[d.NL], out, 1, (const float*)nullptr);
+  hipLaunchKernelGGL(gemv_rows_kernel, dim3((d.NB + 3) / 4), dim3(LB), 0, s, d.NB, d.H, out,
+                     ls->w16 + d.w_off[d.NL + 1], ls->w32 + d.b_off[d.NL + 1], ls->coef, 0,
+                     (const float*)(ls->coef + d.NB));
+}
+
+void learn_guess(LearnState* ls, const double* x, const double* base, double* p, hipStream_t s) {
+  infer(ls, x, base, s);
   const int64_t n = std::max(ls->nI, ls->offset);
   hipLaunchKernelGGL(guess_kernel, dim3((unsigned)((n + LB - 1) / LB)), dim3(LB), 0, s, ls->nI, ls->offset, ls->d.NB,
                      ls->Phi, ls->coef, p);
@@ -692,10 +693,7 @@ int learn_predict(LearnState* ls, const double* dF_host, double* coef_host, hipS
   if (!x || !z) { *err = "device allocation failed"; return 7; }
   LCHK(hipMemcpyAsync(x, dF_host, sizeof(double) * ls->nflt, hipMemcpyHostToDevice, s));
   LCHK(hipMemsetAsync(z, 0, sizeof(double) * ls->nflt, s));
-  float* bufA = ls->inf_act;
-  float* bufB = ls->inf_act + std::max<int64_t>(ls->nflt, (int64_t)ls->d.C * ls->d.L[1]) + ls->d.H;
-  hipLaunchKernelGGL(infer_kernel, dim3(1), dim3(1024), 0, s, infer_args(ls->d), x, z, ls->nflt, ls->w16, ls->w32,
-                     bufA, bufB, ls->coef);
+  infer(ls, x, z, s);
   std::vector<float> c(ls->d.NB);
   LCHK(hipMemcpyAsync(c.data(), ls->coef, sizeof(float) * ls->d.NB, hipMemcpyDeviceToHost, s));
   const hipError_t e = hipStreamSynchronize(s);

@@ -22,17 +22,24 @@ def record(ctx, total_time=None, max_steps=0, capacity=2000, nbasis=64, seed=4):
     return out
 
 
-def fit_basis(ctx, nbasis=None):
+def fit_basis(ctx, nbasis=None, tol=1e-8, noise=1e-3):
+    """POD of the usable snapshots.  A pass-1 solve to relative residual tol
+    leaves an error of order tol |F| in the displacement, so an increment with
+    |dF| < tol |F| / noise is mostly solver error; such snapshots are left out."""
     nb = nbasis or ctx.nbasis
     G = ctx.learn_gram()
-    w, V = np.linalg.eigh(G)
+    norms = ctx.learn_norms()
+    use = (norms[:, 0] > (tol / noise) * norms[:, 1]) & (norms[:, 0] > 0)
+    idx = np.nonzero(use)[0]
+    w, Vs = np.linalg.eigh(G[np.ix_(idx, idx)])
     order = np.argsort(w)[::-1][:nb]
     w = np.maximum(w[order], 0.0)
-    V = V[:, order]
+    V = np.zeros((G.shape[0], nb))
+    V[idx] = Vs[:, order]
     sigma = np.sqrt(w)
-    ctx.learn_set_basis(np.ascontiguousarray(V), sigma)
-    energy = w.sum() / max(np.trace(G), 1e-300)
-    return dict(sigma=sigma, energy_captured=float(energy))
+    ctx.learn_set_basis(np.ascontiguousarray(V), sigma, use.astype(np.int32))
+    energy = w.sum() / max(np.trace(G[np.ix_(idx, idx)]), 1e-300)
+    return dict(sigma=sigma, energy_captured=float(energy), used=int(use.sum()), total=int(len(use)))
 
 
 def train(ctx, epochs=300, batch=64, lr=1e-3, seed=11):

@@ -31,7 +31,7 @@ EXPORTS = [
     "eq_slr", "eq_dtevol", "eq_set_solver", "eq_state_init", "eq_state_get", "eq_state_set",
     "eq_step", "eq_run", "eq_time_kernel", "eq_learn_init", "eq_learn_record", "eq_learn_info",
     "eq_learn_gram", "eq_learn_set_basis", "eq_learn_train", "eq_learn_loss_grad", "eq_learn_get",
-    "eq_learn_predict",
+    "eq_learn_predict", "eq_learn_norms",
 ]
 EQ_TIME_KU, EQ_TIME_CG_DIR, EQ_TIME_VCYCLE, EQ_TIME_CG_ITER, EQ_TIME_NEWMARK, EQ_TIME_CHEB = range(6)
 
@@ -126,7 +126,8 @@ def load():
         "eq_learn_record": ([vp, C.c_int], C.c_int),
         "eq_learn_info": ([vp, i64p, i64p, C.POINTER(C.c_int), i64p], C.c_int),
         "eq_learn_gram": ([vp, dp], C.c_int),
-        "eq_learn_set_basis": ([vp, dp, dp], C.c_int),
+        "eq_learn_set_basis": ([vp, dp, dp, i32p], C.c_int),
+        "eq_learn_norms": ([vp, dp], C.c_int),
         "eq_learn_train": ([vp, C.c_int, C.c_int, C.c_double, C.c_uint64, dp], C.c_int),
         "eq_learn_loss_grad": ([vp, fp, C.c_int, i32p, C.c_int, dp, fp], C.c_int),
         "eq_learn_get": ([vp, fp, fp, fp, fp], C.c_int),

@@ -219,8 +219,11 @@ int eq_learn_record(eq_ctx *ctx, int on);
 int eq_learn_info(const eq_ctx *ctx, int64_t *nsnap, int64_t *nparams, int *trained, int64_t *nlearned);
 /* G[nsnap x nsnap] = S^T S of the recorded interior increments (row-major). */
 int eq_learn_gram(eq_ctx *ctx, double *G);
-/* V[nsnap x nbasis] (row-major) and sigma[nbasis] from eig(G): Phi = S V / sigma. */
-int eq_learn_set_basis(eq_ctx *ctx, const double *V, const double *sigma);
+/* |dF| and |F| (fault data of the solve) per snapshot: out[2 nsnap]. */
+int eq_learn_norms(eq_ctx *ctx, double *out);
+/* V[nsnap x nbasis] (row-major) and sigma[nbasis] from eig(G): Phi = S V / sigma;
+ * use[nsnap] (NULL: all) selects the snapshots that enter the training set. */
+int eq_learn_set_basis(eq_ctx *ctx, const double *V, const double *sigma, const int32_t *use);
 /* Adam training (batch, lr); hist[epochs] receives the mean loss per epoch. */
 int eq_learn_train(eq_ctx *ctx, int epochs, int batch, double lr, uint64_t seed, double *hist);
 /* Loss and gradient on samples idx with the given FP32 parameters (NULL: the

@@ -310,3 +310,28 @@ def test_run_matches_oracle(oracle):
     assert np.allclose(R["tEnd"], r.tEnd, rtol=1e-9, atol=0)
     assert m["delfafter"] < 1e-8
     assert m["d"] < 1e-8
+
+
+def test_state_init_after_dynamic_steps(oracle):
+    """Regression: re-initialising after an odd number of Newmark steps (swapped
+    state buffers) restores the initial state and reproduces a fresh run."""
+    from eqcycle_amd.api import EqContext
+    P = set_parameters(small(32, 8, material="trapezoid", total_years=3))
+    ctx = EqContext(P, amg=True)
+    ctx.set_solver(tol=1e-12, pc="amg")
+    ctx.state_init()
+    ndyn = 0
+    for _ in range(3000):
+        ndyn += ctx.step().isolver_used == 2
+        if ndyn % 2 == 1:
+            break
+    assert ndyn % 2 == 1
+    ctx.state_init()
+    s = ctx.state_get()
+    assert np.all(s["d"] == 0) and np.all(s["a"] == 0) and s["t"] == 0
+    a = [ctx.step().Vfmax for _ in range(30)]
+    fresh = EqContext(P, amg=True)
+    fresh.set_solver(tol=1e-12, pc="amg")
+    fresh.state_init()
+    b = [fresh.step().Vfmax for _ in range(30)]
+    assert a == b

@@ -23,7 +23,7 @@ def trained():
     learned.record(ctx, max_steps=160, capacity=200, nbasis=NB)
     info = ctx.learn_info()
     print(f"[gpu] recorded {info['nsnap']} snapshots, {info['nparams']} parameters", flush=True)
-    fit = learned.fit_basis(ctx)
+    fit = learned.fit_basis(ctx, tol=1e-10)
     print(f"[gpu] POD energy captured {fit['energy_captured']:.6f}", flush=True)
     return P, ctx, fit
 
