@@ -1,14 +1,28 @@
-"""Benchmark: simulated years per hour of the earthquake-cycle time loop on one MI355X.
+"""Benchmark: simulated years per hour and CG iterations per solve of the
+earthquake-cycle time loop on one MI355X, plain FP64 against learned initial
+guess + FP32 V-cycle at the same relative residual.
 
-Workload: the config-2 fault model (160 x 160 elements, NGLL 5, 410,881 nodes,
-trapezoidal damage zone, FP64, AMG-preconditioned CG to 1e-8 relative residual)
-run by the device-resident time loop (main(P)) from its seeded initial state.
-One bench step is one simulated year: --warmup W years are simulated untimed,
-then exactly --steps K years are timed (the loop stops at the first time step
-that reaches the target time, so the timed window is K years plus at most one
-step; the JSON reports the simulated years actually covered).
+Default workload (config 4): 160 x 160 elements, NGLL 5 (410,881 nodes),
+trapezoidal damage zone, p-multigrid preconditioned CG to 1e-6 relative
+residual, the whole time loop (quasi-static steps, Newmark steps, rate-state
+Newton searches, adaptive dt, event bookkeeping) resident on the GPU.
 
-Output: one JSON line (metric, value, unit, ..., roofline, cpu_baseline).
+One bench step is one simulated year.  Each variant starts from the same state:
+the seeded initial state advanced --warmup W years with the plain solver
+(untimed); then exactly --steps K years are timed (the loop stops at the first
+time step that reaches the target time, so a window is K years plus at most one
+step; the JSON reports the simulated years actually covered and uses them).
+
+  plain   : FP64 CG, FP64 V-cycle, previous-step initial guess (src/main.jl:241)
+  learned : FP64 CG, FP32 V-cycle, learned initial guess -- the network is
+            trained beforehand (untimed, reported) on snapshots recorded from
+            the plain solver's own quasi-static steps
+
+`value` is the learned variant's sim-years/hour (the plain one's for configs
+without a learned guess); both variants, their iteration counts and their
+agreement are in `config`.  Output: one JSON line with `roofline` (dominant
+kernel, HIP-event timed) and `cpu_baseline` (the oracle, one core, bounded
+sample).
 """
 from __future__ import annotations
 
@@ -24,33 +38,39 @@ sys.path.insert(0, ROOT)
 YR = 365 * 24 * 60 * 60
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per launch of the fine-level Chebyshev step, from two rocprofv3
+# --pmc passes (FETCH_SIZE, WRITE_SIZE) calibrated by a 512 MiB copy, on the
+# 410,881-node mesh (profiles/cheb_ku_newmark_cfg2_hbm_traffic.md and
+# profiles/cheb_cfg2_fp32_hbm_traffic.md).  Keyed by (nodes, precision).
+CHEB_TRAFFIC = {(410881, "fp64"): 34.38e6, (410881, "fp32"): 20.13e6}
+
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10, help="timed simulated years")
-    ap.add_argument("--warmup", type=int, default=1, help="untimed simulated years before the timed window")
-    ap.add_argument("--config", default="cfg2")
-    ap.add_argument("--prec", default=None, help="fp64 | mixed (default: the config's)")
-    ap.add_argument("--pc", default="amg")
-    ap.add_argument("--hierarchy", default="pmg", help="pmg (default) | rs")
-    ap.add_argument("--degree", type=int, default=3, help="Chebyshev degree on the fine level (pmg)")
+    ap.add_argument("--steps", type=int, default=10, help="timed simulated years per variant")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed simulated years before each timed window")
+    ap.add_argument("--config", default="cfg4")
     ap.add_argument("--tol", type=float, default=None)
+    ap.add_argument("--hierarchy", default="pmg", help="pmg (default) | rs (Ruge-Stueben on K_II)")
+    ap.add_argument("--degree", type=int, default=3, help="Chebyshev degree on the fine level (pmg)")
+    ap.add_argument("--epochs", type=int, default=300)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--plain-only", action="store_true")
     return ap.parse_args()
 
 
-def cpu_baseline(P, cfg, tol, stats, window_years, budget_s):
+def cpu_baseline(P, tol, stats, window_years, budget_s):
     """Time the oracle (C FP64 restatement, one thread) on a bounded sample of the
-    same workload and convert to simulated years per hour with the GPU run's step mix."""
+    same workload and convert to simulated years per hour with the GPU plain
+    run's step mix and iteration counts."""
     import numpy as np
 
     from oracle import oracle as O
     t_setup = time.perf_counter()
     sim = O.Simulation(P, tol=tol, solver="amg")
     t_setup = time.perf_counter() - t_setup
-    # (i) AMG-PCG iterations on a quasi-static system: time per iteration
     n = sim.kni.n
     rng = np.random.default_rng(0)
     b = rng.standard_normal(n)
@@ -61,12 +81,10 @@ def cpu_baseline(P, cfg, tol, stats, window_years, budget_s):
     t0 = time.perf_counter()
     O.cg(sim.kni, np.zeros(n), b, sim.ml, 1e-30, maxiter=m)
     per_it = (time.perf_counter() - t0) / m
-    # SpMVs and slrFunc! of one QS pass
     t0 = time.perf_counter()
     for _ in range(3):
         sim.K.matvec(np.zeros(P.nglob))
     spmv = (time.perf_counter() - t0) / 3
-    # (ii) Newmark steps
     sim.isolver = 2
     sim.dt = sim.dtmin
     k = 0
@@ -75,19 +93,88 @@ def cpu_baseline(P, cfg, tol, stats, window_years, budget_s):
         sim.dyn_step()
         k += 1
     per_dyn = (time.perf_counter() - t0) / k
-    iters_per_solve = stats["iters_per_solve"]
-    per_qs = 2 * (iters_per_solve * per_it + 2 * spmv)
+    per_qs = 2 * (stats["iters_per_solve"] * per_it + 2 * spmv)
     cpu_s = stats["n_qs"] * per_qs + stats["n_dyn"] * per_dyn
-    value = window_years / (cpu_s / 3600.0)
     return {
-        "value": value, "unit": "sim-years/hour", "cores": 1, "kind": "port",
+        "value": window_years / (cpu_s / 3600.0), "unit": "sim-years/hour", "cores": 1, "kind": "port",
         "sample": (f"oracle (C FP64 restatement: assembled CSR K, RS-AMG + symmetric Gauss-Seidel V-cycle, "
                    f"cg!, serial FBC!/NRsearch!) on the same mesh: {m} AMG-PCG iterations "
                    f"({per_it * 1e3:.1f} ms each) and {k} Newmark steps ({per_dyn * 1e3:.1f} ms each), "
-                   f"scaled by the GPU window's {stats['n_qs']} QS steps x 2 solves x "
-                   f"{iters_per_solve:.1f} iterations (the GPU's count, favourable to the CPU) and "
+                   f"scaled to the GPU plain window's {stats['n_qs']} QS steps x 2 solves x "
+                   f"{stats['iters_per_solve']:.2f} iterations (the GPU's count, favourable to the CPU) and "
                    f"{stats['n_dyn']} dynamic steps; set-up {t_setup:.1f} s not counted"),
     }
+
+
+def window(ctx, cfg, tol, prec, guess, warmup, steps):
+    """State init, `warmup` plain years (untimed), then `steps` timed years."""
+    import numpy as np
+    ctx.set_solver(tol=tol, pc="amg", prec="fp64", guess="previous")
+    ctx.state_init()
+    if warmup > 0:
+        ctx.run(total_time=warmup * YR, snapshots=False)
+    ctx.set_solver(tol=tol, pc="amg", prec=prec, guess=guess)
+    t_start = ctx.state_get()["t"]
+    w0 = time.perf_counter()
+    R = ctx.run(total_time=t_start + steps * YR, snapshots=False)
+    wall = time.perf_counter() - w0
+    s = ctx.state_get()
+    years = (s["t"] - t_start) / YR
+    it = R["pcg_iters"]
+    qs = R["isolver"] == 1
+    st = {
+        "sim_years_per_hour": years / (wall / 3600.0), "sim_years": years, "wall_s": wall,
+        "qs_steps": int(R["n_qs"]), "dyn_steps": int(R["n_dyn"]),
+        "pcg_iters_per_solve": R["total_pcg_iters"] / max(1, 2 * R["n_qs"]),
+        "pcg_iters_pass1": float(np.mean(it[qs, 0])) if qs.any() else 0.0,
+        "pcg_iters_pass2": float(np.mean(it[qs, 1])) if qs.any() else 0.0,
+        "wall_qs_s": R["wall_qs"], "wall_dyn_s": R["wall_dyn"], "events": int(len(R["tStart"])),
+        "prec": prec, "guess": guess,
+    }
+    return st, s, R
+
+
+def train_learned(ctx, cfg, tol, epochs):
+    """Record cfg.n_snapshots pass-1 snapshots with the plain solver from the
+    seeded initial state, fit the POD basis, train the network."""
+    from eqcycle_amd import learned
+    t0 = time.perf_counter()
+    ctx.set_solver(tol=tol, pc="amg", prec="fp64", guess="previous")
+    ctx.state_init()
+    ctx.learn_init(capacity=cfg.n_snapshots, nbasis=64)
+    ctx.learn_record(True)
+    while ctx.learn_info()["nsnap"] < cfg.n_snapshots:
+        ctx.run(total_time=1e30, max_steps=400, snapshots=False)
+    ctx.learn_record(False)
+    t_rec = time.perf_counter() - t0
+    years = ctx.state_get()["t"] / YR
+    t0 = time.perf_counter()
+    fit = learned.fit_basis(ctx, tol=tol)
+    t_pod = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    hist = learned.train(ctx, epochs=epochs, batch=64, lr=1e-3)
+    t_train = time.perf_counter() - t0
+    return {"snapshots": cfg.n_snapshots, "used": fit["used"], "record_s": t_rec, "sim_years_recorded": years,
+            "pod_s": t_pod, "energy_captured": fit["energy_captured"], "train_s": t_train, "epochs": epochs,
+            "loss_first": float(hist[0]), "loss_last": float(hist[-1]),
+            "nparams": ctx.learn_info()["nparams"]}
+
+
+def roofline(ctx, P, prec):
+    """Fine-level Chebyshev step (the kernel with the largest share of device
+    time): per node it reads the window vector, mu, D^-1, r, x and writes r,
+    d', x -- 8 values of s bytes (the LDS halo is not counted)."""
+    kp = "fp32" if prec in ("f32pc", "mixed", "fp32") else "fp64"
+    s = 4 if kp == "fp32" else 8
+    dt_ms = ctx.time_kernel("cheb", kp, reps=200)
+    alg = 8 * s * P.nglob
+    achieved = alg / (dt_ms * 1e-3) / 1e9
+    traffic = CHEB_TRAFFIC.get((P.nglob, kp))
+    return {"kernel": f"tile_kernel<{'float' if s == 4 else 'double'}, OpCheb> (fine-level Chebyshev step: "
+                      "K_II d fused with the r, d, x updates)",
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "avg_launch_us": dt_ms * 1e3,
+            "alg_bytes_per_launch": alg}
 
 
 def main():
@@ -96,63 +183,57 @@ def main():
         print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); every GPU box here has one GPU and the "
               "multi-GPU decomposition is a design note, not a measured path", file=sys.stderr)
         sys.exit(2)
+    import numpy as np
+
     from eqcycle_amd.api import EqContext
     from eqcycle_amd.configs import CONFIGS
     from eqcycle_amd.params import set_parameters
 
     cfg = CONFIGS[a.config]
-    prec = a.prec or cfg.precision
     tol = a.tol or cfg.pcg_tol
     P = set_parameters(cfg)
-    amg = {"hierarchy": a.hierarchy, "degree": a.degree} if a.pc == "amg" else False
-    ctx = EqContext(P, amg=amg)
-    ctx.set_solver(tol=tol, pc=a.pc, prec=prec)
-    ctx.state_init()
-    # warm-up window (untimed)
-    if a.warmup > 0:
-        ctx.run(total_time=a.warmup * YR, snapshots=False)
-    s0 = ctx.state_get()
-    t_start = s0["t"]
-    # timed window: K simulated years; every step ends with a device sync
-    w0 = time.perf_counter()
-    R = ctx.run(total_time=t_start + a.steps * YR, snapshots=False)
-    w1 = time.perf_counter()
-    t_end = ctx.state_get()["t"]
-    wall = w1 - w0
-    years = (t_end - t_start) / YR
-    n_solves = 2 * R["n_qs"]
-    stats = dict(n_qs=R["n_qs"], n_dyn=R["n_dyn"],
-                 iters_per_solve=R["total_pcg_iters"] / max(1, n_solves))
-    value = years / (wall / 3600.0)
+    ctx = EqContext(P, amg={"hierarchy": a.hierarchy, "degree": a.degree})
+    learned = cfg.learned and not a.plain_only
 
-    # roofline of the dominant kernel (largest share of device time in the
-    # rocprof summary): the fused Chebyshev smoothing step on the matrix-free
-    # fine level.  Per node it reads the window vector d, mu, dinv, r, x and
-    # writes r, d', x: 8 scalars of s bytes (the 4-node LDS halo is not counted).
-    dt_ms = ctx.time_kernel("cheb", "fp64" if prec == "fp64" else "fp32", reps=200)
-    s = 8 if prec == "fp64" else 4
-    traffic_alg = 8 * s * P.nglob
-    achieved = traffic_alg / (dt_ms * 1e-3) / 1e9
-    roof = {"kernel": "tile_kernel<OpCheb> (fine-level Chebyshev step: K_II d fused with the r, d, x updates)",
-            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "avg_launch_us": dt_ms * 1e3,
-            "alg_bytes_per_launch": traffic_alg}
-
+    plain, s_plain, R_plain = window(ctx, cfg, tol, "fp64", "previous", a.warmup, a.steps)
+    extra = {"plain_fp64": plain}
+    main_st, prec = plain, "fp64"
+    if learned:
+        extra["training"] = train_learned(ctx, cfg, tol, a.epochs)
+        lrn, s_lrn, R_lrn = window(ctx, cfg, tol, cfg.precision, "learned", a.warmup, a.steps)
+        extra["learned_f32"] = lrn
+        main_st, prec = lrn, cfg.precision
+        dref = np.max(np.abs(s_plain["d"]))
+        nt = min(len(R_lrn["Vfmax"]), len(R_plain["Vfmax"]))
+        extra["speedup_vs_plain"] = lrn["sim_years_per_hour"] / plain["sim_years_per_hour"]
+        extra["iters_ratio"] = lrn["pcg_iters_per_solve"] / max(plain["pcg_iters_per_solve"], 1e-30)
+        extra["agreement"] = {
+            "d_max_rel_diff": float(np.max(np.abs(s_lrn["d"] - s_plain["d"])) / dref),
+            "t_end_rel_diff": abs(s_lrn["t"] - s_plain["t"]) / max(s_plain["t"], 1e-30),
+            "event_starts_plain": [float(x) for x in R_plain["tStart"]],
+            "event_starts_learned": [float(x) for x in R_lrn["tStart"]],
+            "log10_Vfmax_max_abs_diff_common_steps": float(np.max(np.abs(
+                np.log10(R_lrn["Vfmax"][:nt]) - np.log10(R_plain["Vfmax"][:nt])))) if nt else None,
+        }
+    years = main_st["sim_years"]
     out = {
-        "metric": "sim-years/hour", "value": value, "unit": "sim-years/hour", "n_gpus": 1,
-        "steps": a.steps, "warmup": a.warmup, "ms_per_step": wall * 1e3 / max(a.steps, 1),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64" if prec == "fp64" else
-        "f32+f64", "data": "synthetic (seeded fault model, config 2)",
+        "metric": "sim-years/hour", "value": main_st["sim_years_per_hour"], "unit": "sim-years/hour",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": main_st["wall_s"] * 1e3 / max(years, 1e-30),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if prec == "fp64" else "f64+f32",
+        "data": f"synthetic (seeded fault model, {a.config}; network trained on the plain solver's own snapshots)"
+                if learned else f"synthetic (seeded fault model, {a.config})",
         "config": {"workload": f"{a.config}: {cfg.nelx}x{cfg.nely} elements NGLL 5 ({P.nglob} nodes), "
-                               f"{cfg.material} damage zone, AMG-PCG tol {tol:g}, {prec}",
-                   "sim_years_timed": years, "qs_steps": R["n_qs"], "dyn_steps": R["n_dyn"],
-                   "pcg_iters_per_solve": stats["iters_per_solve"], "wall_qs_s": R["wall_qs"],
-                   "wall_dyn_s": R["wall_dyn"], "pc": a.pc, "hierarchy": a.hierarchy,
-                   "amg_levels": ctx.amg_info() if a.pc == "amg" else None},
-        "roofline": roof,
+                               f"{cfg.material} damage zone, CG tol {tol:g} "
+                               + ("learned x0 + FP32 V-cycle vs plain FP64" if learned else "plain FP64"),
+                   "pcg_iters_per_solve": main_st["pcg_iters_per_solve"], "hierarchy": a.hierarchy,
+                   "amg_levels": ctx.amg_info(), **extra},
+        "roofline": roofline(ctx, P, prec),
     }
     if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(P, cfg, tol, stats, years, a.cpu_seconds)
+        stats = dict(n_qs=plain["qs_steps"], n_dyn=plain["dyn_steps"], iters_per_solve=plain["pcg_iters_per_solve"])
+        out["cpu_baseline"] = cpu_baseline(P, tol, stats, plain["sim_years"], a.cpu_seconds)
     print(json.dumps(out), flush=True)
 
 

@@ -1494,6 +1494,33 @@ extern "C" int eq_time_kernel(eq_ctx* ctx, int which, int prec, int reps, double
   if (which == EQ_TIME_NEWMARK && (!ctx->have_state || !ctx->have_params))
     return fail(ctx, EQ_ENOTREADY, "eq_state_init first");
   hipStream_t s = ctx->s;
+  if (which == EQ_TIME_COPY) {
+    const int64_t n = (int64_t)512 << 17;  // 512 MiB of doubles
+    double *src = nullptr, *dst = nullptr;
+    if (hipMalloc(&src, n * sizeof(double)) != hipSuccess) return fail(ctx, EQ_NOMEM, "copy buffers");
+    if (hipMalloc(&dst, n * sizeof(double)) != hipSuccess) {
+      (void)hipFree(src);
+      return fail(ctx, EQ_NOMEM, "copy buffers");
+    }
+    launch_fill<double>(n, src, 1.0, s);
+    launch_convert<double, double>(n, src, dst, s);  // warm-up
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    float ms = 0;
+    bool ok = hipEventCreate(&e0) == hipSuccess;
+    ok = ok && hipEventCreate(&e1) == hipSuccess;
+    ok = ok && hipEventRecord(e0, s) == hipSuccess;
+    for (int k = 0; ok && k < reps; k++) launch_convert<double, double>(n, src, dst, s);
+    ok = ok && hipEventRecord(e1, s) == hipSuccess;
+    ok = ok && hipEventSynchronize(e1) == hipSuccess;
+    ok = ok && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(src);
+    (void)hipFree(dst);
+    if (!ok) return fail(ctx, EQ_HIP_ERR, "copy timing");
+    *avg_ms = (double)ms / reps;
+    return sync_check(ctx);
+  }
   // well-defined inputs: unit vectors, a CG state that never reports convergence
   if (prec == EQ_PREC_FP32) {
     launch_fill<float>(ctx->N, ctx->w32.x, 1.0f, s); launch_fill<float>(ctx->N, ctx->w32.r, 1.0f, s);

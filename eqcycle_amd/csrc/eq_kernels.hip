@@ -1374,6 +1374,7 @@ EQ_INST(double)
 EQ_INST(float)
 template void launch_convert<double, float>(int64_t, const double*, float*, hipStream_t);
 template void launch_convert<float, double>(int64_t, const float*, double*, hipStream_t);
+template void launch_convert<double, double>(int64_t, const double*, double*, hipStream_t);
 template void launch_fill<int>(int64_t, int*, int, hipStream_t);
 
 }  // namespace eq

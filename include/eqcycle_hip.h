@@ -239,11 +239,13 @@ int eq_learn_predict(eq_ctx *ctx, const double *dF, double *coef);
 
 /* ---- measurement hook (no reference counterpart) ------------------------ */
 enum { EQ_TIME_KU = 0, EQ_TIME_CG_DIR = 1, EQ_TIME_VCYCLE = 2, EQ_TIME_CG_ITER = 3, EQ_TIME_NEWMARK = 4,
-       EQ_TIME_CHEB = 5 };
+       EQ_TIME_CHEB = 5, EQ_TIME_COPY = 6 };
 /* Average device time (ms) of one launch of the given kernel group over reps
  * back-to-back launches, bracketed by HIP events on the context's stream.
  * Runs on scratch vectors; EQ_TIME_NEWMARK advances a copy of the state and
- * must not be called in the middle of a run. */
+ * must not be called in the middle of a run.  EQ_TIME_COPY copies a 512 MiB
+ * FP64 buffer into another (allocated for the call, then freed): a kernel of
+ * known traffic that calibrates hardware byte counters. */
 int eq_time_kernel(eq_ctx *ctx, int which, int prec, int reps, double *avg_ms);
 
 #ifdef __cplusplus

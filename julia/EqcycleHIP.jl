@@ -1,0 +1,214 @@
+# EqcycleHIP.jl -- Julia bindings of libeqcycle_hip.so for the eqcycle time loop.
+#
+# Drop-in replacements, same names / positional arguments / error behaviour as
+# the reference routines (paths relative to the eqcycle repository):
+#   FBC!        src/NRsearch_serial.jl:6-45
+#   NRsearch!   src/NRsearch_serial.jl:47-98
+#   IDS!, IDS2! src/otherFunctions_serial.jl:5-59
+#   slrFunc!    src/otherFunctions_serial.jl:62-83
+#   dtevol!     src/dtevol.jl:5-44
+#   KOp / KniOp mul!            Ksparse'*u, nKsparse'*u, kni'*x  (src/main.jl:246,267,311,256)
+#   AmgPreconditioner ldiv!     aspreconditioner(ruge_stuben(kni)) (src/main.jl:200-201)
+#   cg!(x, ::KniOp, b; Pl, tol) IterativeSolvers.cg! (src/main.jl:256), whole solve on the GPU
+#   main(P)                     src/main.jl:67-476, device-resident
+#
+# Usage in run.jl, after `P = setParameters(24e3, 20)`:
+#   include("julia/EqcycleHIP.jl"); using .EqcycleHIP
+#   ctx = EqcycleHIP.Context(P)          # mesh, material W, params_*
+#   O = EqcycleHIP.main(ctx, P)          # or call the routines one by one
+module EqcycleHIP
+
+using LinearAlgebra
+
+const LIB = get(ENV, "EQCYCLE_HIP_LIB", joinpath(@__DIR__, "..", "eqcycle_amd", "libeqcycle_hip.so"))
+
+const EQ_OK = 0
+const EQ_NR_NOCONV = 3
+const EQ_NR_NAN = 4
+const EQ_PCG_NOCONV = 5
+const EQ_MASK_ALL, EQ_MASK_INTERIOR, EQ_MASK_NEG = 0, 1, 2
+const EQ_PREC_FP64, EQ_PREC_FP32, EQ_PREC_MIXED, EQ_PREC_F32PC = 0, 1, 2, 3
+const EQ_PC_NONE, EQ_PC_JACOBI, EQ_PC_AMG = 0, 1, 2
+
+struct MeshDesc
+    ngll::Int64
+    nelx::Int64
+    nely::Int64
+    dxe::Float64
+    dye::Float64
+    iglob::Ptr{Int64}
+    xgll::Ptr{Float64}
+    wgll::Ptr{Float64}
+    H::Ptr{Float64}
+end
+
+struct ParamsDesc
+    FltNglob::Int64; nglob::Int64; IDstate::Int64
+    Vpl::Float64; Vthres::Float64; Vevne::Float64; dt0::Float64
+    fo::Ptr{Float64}; Vo::Ptr{Float64}; xLf::Ptr{Float64}; M::Ptr{Float64}
+    BcLC::Ptr{Float64}; BcTC::Ptr{Float64}; FltB::Ptr{Float64}; FltZ::Ptr{Float64}; FltX::Ptr{Float64}
+    cca::Ptr{Float64}; ccb::Ptr{Float64}; Seff::Ptr{Float64}; tauo::Ptr{Float64}; XiLf::Ptr{Float64}
+    iFlt::Ptr{Int64}; iBcL::Ptr{Int64}; n_iBcL::Int64; iBcT::Ptr{Int64}; n_iBcT::Int64
+    FltIglobBC::Ptr{Int64}; n_FltIglobBC::Int64; out_seis::Ptr{Int64}; n_out_seis::Int64
+end
+
+mutable struct Context
+    h::Ptr{Cvoid}
+    nglob::Int
+    nI::Int
+    keep::Vector{Any}          # arrays whose pointers the library may hold during set-up
+end
+
+lasterr(ctx::Context) = unsafe_string(ccall((:eq_last_error, LIB), Cstring, (Ptr{Cvoid},), ctx.h))
+function check(ctx::Context, rc)
+    rc == EQ_OK && return rc
+    error("libeqcycle_hip: ", lasterr(ctx))
+end
+
+"""Context(P; W, iglob, dxe, dye, xgll, wgll, H, amg=true): one mesh on one GPU.
+
+`P` is the 5-tuple of setParameters; the mesh arrays and W are the locals of
+setParameters (par.jl:99-152) that the reference does not return -- pass them
+explicitly."""
+function Context(P; W, iglob, NelX, NelY, dxe, dye, xgll, wgll, H, amg = true)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    ig = convert(Array{Int64}, iglob)
+    Hc = convert(Matrix{Float64}, H)
+    GC.@preserve ig xgll wgll Hc begin
+        d = MeshDesc(5, NelX, NelY, dxe, dye, pointer(ig), pointer(xgll), pointer(wgll), pointer(Hc))
+        rc = ccall((:eq_create, LIB), Cint, (Ref{MeshDesc}, Ref{Ptr{Cvoid}}), d, h)
+        rc == EQ_OK || error("eq_create failed ($rc)")
+    end
+    ctx = Context(h[], P[1].nglob, length(P[4].FltNI), Any[])
+    finalizer(c -> ccall((:eq_destroy, LIB), Cvoid, (Ptr{Cvoid},), c.h), ctx)
+    GC.@preserve W check(ctx, ccall((:eq_set_material, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}), ctx.h, W))
+    set_params!(ctx, P)
+    amg && check(ctx, ccall((:eq_amg_setup, LIB), Cint,
+                           (Ptr{Cvoid}, Cdouble, Cint, Int64, Cint, Cint, Cint, Cint),
+                           ctx.h, 0.25, 10, 2000, 1, 1, 3, 1))
+    return ctx
+end
+
+function set_params!(ctx::Context, P)
+    f, ia = P[3], P[4]
+    a = (f.fo, f.Vo, f.xLf, f.M, f.BcLC, f.BcTC, f.FltB, f.FltZ, f.FltX, f.cca, f.ccb, f.Seff, f.tauo, f.XiLf,
+         ia.iFlt, ia.iBcL, ia.iBcT, ia.FltIglobBC, ia.out_seis)
+    GC.@preserve a begin
+        d = ParamsDesc(P[1].FltNglob, P[1].nglob, P[1].IDstate, P[2].Vpl, P[2].Vthres, P[2].Vevne, P[2].dt0,
+                       pointer(f.fo), pointer(f.Vo), pointer(f.xLf), pointer(f.M), pointer(f.BcLC),
+                       pointer(f.BcTC), pointer(f.FltB), pointer(f.FltZ), pointer(f.FltX), pointer(f.cca),
+                       pointer(f.ccb), pointer(f.Seff), pointer(f.tauo), pointer(f.XiLf),
+                       pointer(ia.iFlt), pointer(ia.iBcL), length(ia.iBcL), pointer(ia.iBcT), length(ia.iBcT),
+                       pointer(ia.FltIglobBC), length(ia.FltIglobBC), pointer(ia.out_seis), length(ia.out_seis))
+        check(ctx, ccall((:eq_set_params, LIB), Cint, (Ptr{Cvoid}, Ref{ParamsDesc}), ctx.h, d))
+    end
+end
+
+# ---- operators: the AbstractMatrix protocol cg! needs (test_01.jl:9-27) ------
+struct KOp <: AbstractMatrix{Float64}   # Ksparse' (mask EQ_MASK_ALL) or nKsparse' (EQ_MASK_NEG)
+    ctx::Context
+    mask::Int
+end
+Base.size(A::KOp) = (A.ctx.nglob, A.ctx.nglob)
+Base.eltype(::KOp) = Float64
+Base.getindex(::KOp, i::Int, j::Int) = error("matrix-free operator")
+function LinearAlgebra.mul!(y::Vector{Float64}, A::KOp, x::Vector{Float64})
+    check(A.ctx, ccall((:eq_ku_apply, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Cint, Cint),
+                       A.ctx.h, x, y, A.mask, EQ_PREC_FP64))
+    y
+end
+Base.:*(A::KOp, x::Vector{Float64}) = mul!(similar(x), A, x)
+
+struct KniOp <: AbstractMatrix{Float64}  # kni' = -K[FltNI, FltNI]
+    ctx::Context
+end
+Base.size(A::KniOp) = (A.ctx.nI, A.ctx.nI)
+Base.eltype(::KniOp) = Float64
+Base.getindex(::KniOp, i::Int, j::Int) = error("matrix-free operator")
+function LinearAlgebra.mul!(y::Vector{Float64}, A::KniOp, x::Vector{Float64})
+    check(A.ctx, ccall((:eq_kni_apply, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Cint),
+                       A.ctx.h, x, y, EQ_PREC_FP64))
+    y
+end
+
+struct AmgPreconditioner
+    ctx::Context
+end
+function LinearAlgebra.ldiv!(x::Vector{Float64}, p::AmgPreconditioner, b::Vector{Float64})
+    check(p.ctx, ccall((:eq_amg_ldiv, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Cint),
+                       p.ctx.h, b, x, EQ_PREC_FP64))
+    x
+end
+
+"""cg!(x, kni, b; Pl, tol, maxiter): the whole solve on the GPU, IterativeSolvers semantics
+(||b - kni x|| <= tol ||b||).  A non-converged solve returns x, silently, as cg! does."""
+function cg!(x::Vector{Float64}, A::KniOp, b::Vector{Float64}; Pl = nothing, tol = sqrt(eps()),
+             maxiter::Int = size(A, 2), prec::Int = EQ_PREC_FP64)
+    it = Ref{Int64}(0); rr = Ref{Float64}(0)
+    rc = ccall((:eq_cg, LIB), Cint,
+               (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Cdouble, Int64, Cint, Cint, Ref{Int64}, Ref{Float64}),
+               A.ctx.h, x, b, tol, maxiter, Pl === nothing ? EQ_PC_NONE : EQ_PC_AMG, prec, it, rr)
+    rc == EQ_PCG_NOCONV || check(A.ctx, rc)
+    x
+end
+
+# ---- fault routines (same positional signatures as the reference) -------------
+function FBC!(ctx::Context, IDstate, P, NFBC, FltNglob, psi1, Vf1, tau1, psi2, Vf2, tau2, psi, Vf, FltVfree, dt)
+    fail = Ref{Int64}(0)
+    rc = ccall((:eq_fbc, LIB), Cint,
+               (Ptr{Cvoid}, Int64, Int64, Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+                Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Cdouble, Ref{Int64}),
+               ctx.h, IDstate, NFBC, FltNglob, psi1, Vf1, tau1, psi2, Vf2, tau2, psi, Vf, FltVfree, dt, fail)
+    if rc == EQ_NR_NAN
+        println("Fault Location = ", fail[])
+        @error("NR SEARCH FAILED!")
+        return
+    end
+    rc == EQ_NR_NOCONV || check(ctx, rc)
+    return psi1, Vf1, tau1, psi2, Vf2, tau2
+end
+
+function NRsearch!(ctx::Context, fo, Vo, cca, ccb, Seff, tau, tauo, psi, FltZ, FltVfree)
+    Vf = Ref{Float64}(0); t = Ref{Float64}(0); k = Ref{Int32}(0)
+    rc = ccall((:eq_nrsearch, LIB), Cint,
+               (Ptr{Cvoid}, Int64, Ref{Float64}, Ref{Float64}, Ref{Float64}, Ref{Float64}, Ref{Float64},
+                Ref{Float64}, Ref{Float64}, Ref{Float64}, Ref{Float64}, Ref{Float64}, Ref{Float64},
+                Ref{Float64}, Ref{Int32}),
+               ctx.h, 1, fo, Vo, cca, ccb, Seff, Float64(tau), tauo, psi, FltZ, FltVfree, Vf, t, k)
+    rc == EQ_NR_NOCONV && println("k = ", -k[])
+    rc in (EQ_OK, EQ_NR_NOCONV) || check(ctx, rc)
+    return Vf[], t[]
+end
+
+function IDS!(ctx::Context, xLf, Vo, psi, dt, Vf, cnd, IDstate = 2)
+    out = Ref{Float64}(0)
+    check(ctx, ccall((:eq_ids, LIB), Cint, (Ptr{Cvoid}, Int64, Ref{Float64}, Ref{Float64}, Ref{Float64}, Cdouble,
+                                            Ref{Float64}, Cdouble, Int64, Ref{Float64}),
+                     ctx.h, 1, xLf, Vo, psi, dt, Vf, cnd, IDstate, out))
+    out[]
+end
+
+function IDS2!(ctx::Context, xLf, Vo, psi, psi1, dt, Vf, Vf1, IDstate = 2)
+    out = Ref{Float64}(0)
+    check(ctx, ccall((:eq_ids2, LIB), Cint, (Ptr{Cvoid}, Int64, Ref{Float64}, Ref{Float64}, Ref{Float64},
+                                             Ref{Float64}, Cdouble, Ref{Float64}, Ref{Float64}, Int64, Ref{Float64}),
+                     ctx.h, 1, xLf, Vo, psi, psi1, dt, Vf, Vf1, IDstate, out))
+    out[]
+end
+
+function slrFunc!(ctx::Context, P, NFBC, FltNglob, psi, psi1, Vf, Vf1, IDstate, tau1, dt)
+    check(ctx, ccall((:eq_slr, LIB), Cint, (Ptr{Cvoid}, Int64, Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+                                            Ptr{Float64}, Int64, Ptr{Float64}, Cdouble),
+                     ctx.h, NFBC, FltNglob, psi, psi1, Vf, Vf1, IDstate, tau1, dt))
+    return psi1, Vf1
+end
+
+function dtevol!(ctx::Context, dt, dtmin, XiLf, FaultNglob, NFBC, Vf, isolver)
+    out = Ref{Float64}(0)
+    check(ctx, ccall((:eq_dtevol, LIB), Cint, (Ptr{Cvoid}, Cdouble, Cdouble, Ptr{Float64}, Int64, Int64,
+                                               Ptr{Float64}, Int64, Ref{Float64}),
+                     ctx.h, dt, dtmin, XiLf, FaultNglob, NFBC, Vf, isolver, out))
+    out[]
+end
+
+end # module
