@@ -89,7 +89,10 @@ struct eq_ctx {
   int npart = 0;
   CgScal* sc = nullptr;
   CgScal* sc_host = nullptr;
-  int64_t last_iters = 8;
+  // CG iterations enqueued before the first convergence read-back: the last
+  // count of the same kind of solve (0: quasi-static pass 1, 1: pass 2, 2: other)
+  int64_t pred_iters[3] = {8, 8, 8};
+  int cg_slot = 2;
   // AMG
   bool have_amg = false;
   std::vector<DevLevel> lev;          // lev[0]: matrix-free fine level (A unused)
@@ -100,10 +103,19 @@ struct eq_ctx {
   int l0_cheb = 0;                    // level-0 smoother: 0 Jacobi, 1 Chebyshev
   int l0_degree = 1;
   double l0_theta = 1, l0_delta = 1;  // Chebyshev interval centre / half-width of D^-1 K_II
-  // step status
-  StepStatus* st = nullptr;
-  StepStatus* st_host = nullptr;
+  // step records: slot 0 serves single steps, a batch of dynamic steps uses
+  // slots 0..kBatch-1 (status, fault rows of d / v / tau, receiver samples)
+  static constexpr int kBatch = 64;
+  StepStatus* st = nullptr;           // [kBatch]
+  StepStatus* st_host = nullptr;      // [kBatch], pinned
+  double* rec_fault = nullptr;        // [kBatch x 3 x NX]
+  int64_t* seis_idx = nullptr;        // [seis_cap]
+  double* rec_seis = nullptr;         // [kBatch x 3 x seis_cap]
+  double* rec_seis_host = nullptr;    // pinned
+  int64_t seis_cap = 0;
+  int* run = nullptr;                 // batch flag
   int* flags = nullptr;
+  bool flags_dirty = true;            // flags not known to hold (INT_MAX, 0)
   int64_t nr_noconv_total = 0;
   int64_t pcg_noconv = 0;
   // learned initial guess
@@ -118,6 +130,7 @@ struct eq_ctx {
     for (void* p : allocs) (void)hipFree(p);
     if (sc_host) (void)hipHostFree(sc_host);
     if (st_host) (void)hipHostFree(st_host);
+    if (rec_seis_host) (void)hipHostFree(rec_seis_host);
     if (s) (void)hipStreamDestroy(s);
   }
   template <typename T>
@@ -267,14 +280,17 @@ extern "C" int eq_create(const eq_mesh_desc* mesh, eq_ctx** out) {
   c->part_rho = c->alloc<double>(c->npart);
   c->part_uq = c->alloc<double>(c->npart);
   c->sc = c->alloc<CgScal>(1);
-  c->st = c->alloc<StepStatus>(1);
+  c->st = c->alloc<StepStatus>(eq_ctx::kBatch);
+  c->rec_fault = c->alloc<double>((int64_t)eq_ctx::kBatch * 3 * c->NX);
+  c->run = c->alloc<int>(1);
   c->flags = c->alloc<int>(4);
-  if (!c->part_rr || !c->part_bb || !c->part_rho || !c->part_uq || !c->sc || !c->st || !c->flags) {
+  if (!c->part_rr || !c->part_bb || !c->part_rho || !c->part_uq || !c->sc || !c->st || !c->flags || !c->rec_fault ||
+      !c->run) {
     *out = ctx.release();
     return fail(*out, EQ_NOMEM, "device allocation failed");
   }
   if (hipHostMalloc((void**)&c->sc_host, sizeof(CgScal), hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc((void**)&c->st_host, sizeof(StepStatus), hipHostMallocDefault) != hipSuccess) {
+      hipHostMalloc((void**)&c->st_host, sizeof(StepStatus) * eq_ctx::kBatch, hipHostMallocDefault) != hipSuccess) {
     *out = ctx.release();
     return fail(*out, EQ_NOMEM, "pinned allocation failed");
   }
@@ -408,6 +424,22 @@ extern "C" int eq_set_params(eq_ctx* ctx, const eq_params_desc* p) {
     const int64_t r = p->out_seis[k] - 1;
     if (r < 0 || r >= ctx->N) return fail(ctx, EQ_EINVAL, "out_seis out of range");
     ctx->out_seis.push_back(ctx->perm[r]);
+  }
+  const int64_t nseis = (int64_t)ctx->out_seis.size();
+  if (nseis > ctx->seis_cap) {
+    ctx->seis_idx = ctx->alloc<int64_t>(nseis);
+    ctx->rec_seis = ctx->alloc<double>((int64_t)eq_ctx::kBatch * 3 * nseis);
+    NEED(ctx->seis_idx);
+    NEED(ctx->rec_seis);
+    if (ctx->rec_seis_host) (void)hipHostFree(ctx->rec_seis_host);
+    ctx->rec_seis_host = nullptr;
+    if (hipHostMalloc((void**)&ctx->rec_seis_host, sizeof(double) * eq_ctx::kBatch * 3 * nseis,
+                      hipHostMallocDefault) != hipSuccess)
+      return fail(ctx, EQ_NOMEM, "pinned allocation failed");
+    ctx->seis_cap = nseis;
+  }
+  if (nseis > 0) {
+    if (int rc = h2d(ctx, ctx->seis_idx, ctx->out_seis.data(), nseis)) return rc;
   }
   // device copies
   double* blk = ctx->alloc<double>(11 * (int64_t)NX + NY + NX);
@@ -805,21 +837,26 @@ static void cg_iteration(eq_ctx* ctx, T* x, int pc, bool f32pc = false) {
 }
 
 template <typename T>
+// r_ready: w.r already holds b - A x with its ||r||^2 partials in part_rr and
+// the ||b||^2 partials in part_bb (the learned start leaves them so).
 static int cg_device(eq_ctx* ctx, T* x, const T* b, double tol, int64_t maxiter, int pc, int64_t* iters,
-                     double* relres, bool f32pc = false) {
+                     double* relres, bool f32pc = false, bool r_ready = false) {
   CgWork<T>& w = work<T>(ctx);
   const Geom& g = ctx->g;
   hipStream_t s = ctx->s;
   if (pc == EQ_PC_AMG && !ctx->have_amg) return fail(ctx, EQ_ENOTREADY, "eq_amg_setup first");
   if (maxiter <= 0) maxiter = ctx->nI;
   if (maxiter > INT_MAX) maxiter = INT_MAX;
-  launch_fill<double>(ctx->npart, ctx->part_rr, 0.0, s);
-  launch_fill<double>(ctx->npart, ctx->part_bb, 0.0, s);
-  launch_cg_residual<T>(g, x, mu_of<T>(ctx), b, w.r, ctx->part_rr, ctx->part_bb, s);
+  if (!r_ready) {
+    launch_fill<double>(ctx->npart, ctx->part_rr, 0.0, s);
+    launch_fill<double>(ctx->npart, ctx->part_bb, 0.0, s);
+    launch_cg_residual<T>(g, x, mu_of<T>(ctx), b, w.r, ctx->part_rr, ctx->part_bb, s);
+  }
   launch_cg_init_scal(ctx->sc, ctx->part_rr, ctx->part_bb, ctx->npart, tol, (int)maxiter, s);
   launch_fill<T>(ctx->N, w.u[w.cur], T(0), s);
   if (f32pc) launch_convert<T, float>(ctx->N, w.r, ctx->w32.r, s);
-  int64_t batch = std::max<int64_t>(1, ctx->last_iters - 1);
+  int64_t& pred = ctx->pred_iters[ctx->cg_slot];
+  int64_t batch = std::max<int64_t>(1, pred);
   for (;;) {
     for (int64_t k = 0; k < batch; k++) cg_iteration<T>(ctx, x, pc, f32pc);
     launch_cg_check(ctx->sc, ctx->part_rr, ctx->npart, s);
@@ -831,7 +868,7 @@ static int cg_device(eq_ctx* ctx, T* x, const T* b, double tol, int64_t maxiter,
   const CgScal& h = *ctx->sc_host;
   if (iters) *iters = h.iter;
   if (relres) *relres = h.bnorm > 0 ? h.resid / h.bnorm : h.resid;
-  ctx->last_iters = std::max<int64_t>(1, h.iter);
+  pred = std::max<int64_t>(1, h.iter);
   return (h.resid <= h.reltol) ? EQ_OK : EQ_PCG_NOCONV;
 }
 
@@ -869,15 +906,15 @@ static int cg_mixed(eq_ctx* ctx, double* x, const double* b, double tol, int64_t
   }
   if (iters) *iters = total;
   if (relres) *relres = bnorm > 0 ? resid / bnorm : resid;
-  ctx->last_iters = std::max<int64_t>(1, total);
   return resid <= tol * bnorm ? EQ_OK : EQ_PCG_NOCONV;
 }
 
+// r_ready: see cg_device (honoured by the FP64 and F32PC modes).
 static int cg_any(eq_ctx* ctx, double* x, const double* b, double tol, int64_t maxiter, int pc, int prec,
-                  int64_t* iters, double* relres) {
+                  int64_t* iters, double* relres, bool r_ready = false) {
   if (prec == EQ_PREC_MIXED) return cg_mixed(ctx, x, b, tol, maxiter, pc, iters, relres);
   if (prec == EQ_PREC_F32PC)
-    return cg_device<double>(ctx, x, b, tol, maxiter, pc, iters, relres, pc == EQ_PC_AMG);
+    return cg_device<double>(ctx, x, b, tol, maxiter, pc, iters, relres, pc == EQ_PC_AMG, r_ready);
   if (prec == EQ_PREC_FP32) {
     launch_convert<double, float>(ctx->N, x, ctx->w32.x, ctx->s);
     launch_convert<double, float>(ctx->N, b, ctx->w32.b, ctx->s);
@@ -886,7 +923,7 @@ static int cg_any(eq_ctx* ctx, double* x, const double* b, double tol, int64_t m
     int rc2 = sync_check(ctx);
     return rc ? rc : rc2;
   }
-  return cg_device<double>(ctx, x, b, tol, maxiter, pc, iters, relres);
+  return cg_device<double>(ctx, x, b, tol, maxiter, pc, iters, relres, false, r_ready);
 }
 
 extern "C" int eq_cg(eq_ctx* ctx, double* x, const double* b, double tol, int64_t maxiter, int pc, int prec,
@@ -1065,6 +1102,7 @@ extern "C" int eq_fbc(eq_ctx* ctx, int64_t IDstate, int64_t NFBC, int64_t FltNgl
   fp.j0 = (int)(NFBC - 1);
   fp.IDstate = (int)IDstate;
   int hf[2] = {INT_MAX, 0};
+  ctx->flags_dirty = true;
   if (int rc = h2d(ctx, ctx->flags, hf, 2)) return rc;
   launch_fbc(fp, f, f.FltVfree, dt, ctx->flags, ctx->s);
   if (int rc = d2h(ctx, psi1, f.psi1, n)) return rc;
@@ -1221,9 +1259,32 @@ extern "C" int eq_state_set(eq_ctx* ctx, const double* d, const double* v, const
   return sync_check(ctx);
 }
 
-static int read_status(eq_ctx* ctx) {
-  HIPCHK(ctx, hipMemcpyAsync(ctx->st_host, ctx->st, sizeof(StepStatus), hipMemcpyDeviceToHost, ctx->s));
+// Records of slots 0..k-1 (status and receiver samples) to the host.
+static int read_status(eq_ctx* ctx, int k) {
+  HIPCHK(ctx, hipMemcpyAsync(ctx->st_host, ctx->st, sizeof(StepStatus) * k, hipMemcpyDeviceToHost, ctx->s));
+  const int64_t ns = (int64_t)ctx->out_seis.size();
+  if (ns > 0)
+    HIPCHK(ctx, hipMemcpyAsync(ctx->rec_seis_host, ctx->rec_seis, sizeof(double) * 3 * ns * k,
+                               hipMemcpyDeviceToHost, ctx->s));
   return sync_check(ctx);
+}
+
+static StepCapture capture(eq_ctx* ctx, int slot, bool batch) {
+  StepCapture c;
+  const int64_t ns = (int64_t)ctx->out_seis.size();
+  c.fault = ctx->rec_fault + (int64_t)slot * 3 * ctx->NX;
+  c.seis_idx = ctx->seis_idx;
+  c.nseis = (int)ns;
+  c.seis = ns > 0 ? ctx->rec_seis + (int64_t)slot * 3 * ns : nullptr;
+  c.run = batch ? ctx->run : nullptr;
+  return c;
+}
+
+static void clean_flags(eq_ctx* ctx) {
+  if (!ctx->flags_dirty) return;
+  launch_fill<int>(1, ctx->flags, INT_MAX, ctx->s);
+  launch_fill<int>(1, ctx->flags + 1, 0, ctx->s);
+  ctx->flags_dirty = false;
 }
 
 // Learned start for K_II x = b: x (interior = previous-step displacement,
@@ -1251,10 +1312,14 @@ static int qs_step(eq_ctx* ctx, eq_step_info* info) {
   const double dt = ctx->dt;
   for (int pass = 0; pass < 2; pass++) {
     launch_qs_begin(g, ctx->fp, ctx->fs, ctx->mu, ctx->d, ctx->dPre, ctx->v, ctx->bq, dt, pass == 0, s);
-    if (pass == 0 && ctx->opts.guess == EQ_GUESS_LEARNED && ctx->learn_on) learned_correction(ctx, ctx->d, ctx->bq);
+    const bool learned = pass == 0 && ctx->opts.guess == EQ_GUESS_LEARNED && ctx->learn_on;
+    if (learned) learned_correction(ctx, ctx->d, ctx->bq);
     int64_t it = 0;
     double rr = 0;
-    int rc = cg_any(ctx, ctx->d, ctx->bq, ctx->opts.tol, ctx->opts.maxiter, ctx->opts.pc, ctx->opts.prec, &it, &rr);
+    ctx->cg_slot = pass;
+    int rc = cg_any(ctx, ctx->d, ctx->bq, ctx->opts.tol, ctx->opts.maxiter, ctx->opts.pc, ctx->opts.prec, &it, &rr,
+                    learned);
+    ctx->cg_slot = 2;
     if (rc == EQ_PCG_NOCONV) ctx->pcg_noconv++;
     else if (rc) return rc;
     info->pcg_iters[pass] = it;
@@ -1268,7 +1333,7 @@ static int qs_step(eq_ctx* ctx, eq_step_info* info) {
 static int dyn_step(eq_ctx* ctx) {
   const double dtmin = ctx->dt0;
   launch_newmark(ctx->g, ctx->fp, ctx->fs, ctx->mu, ctx->M, ctx->d, ctx->v, ctx->a, ctx->d2, ctx->v2, ctx->a2,
-                 ctx->dt, 0.5 * dtmin, 0.5 * dtmin * dtmin, ctx->flags, ctx->s);
+                 ctx->dt, 0.5 * dtmin, 0.5 * dtmin * dtmin, ctx->flags, nullptr, ctx->s);
   std::swap(ctx->d, ctx->d2);
   std::swap(ctx->v, ctx->v2);
   std::swap(ctx->a, ctx->a2);
@@ -1284,12 +1349,14 @@ extern "C" int eq_step(eq_ctx* ctx, eq_step_info* info) {
   ctx->t += ctx->dt;
   info->dt_used = ctx->dt;
   info->isolver_used = ctx->isolver;
-  launch_fill<int>(1, ctx->flags, INT_MAX, ctx->s);
-  launch_fill<int>(1, ctx->flags + 1, 0, ctx->s);
+  clean_flags(ctx);
+  ctx->flags_dirty = true;   // until the step tail has reset them
   int rc = (ctx->isolver == 1) ? qs_step(ctx, info) : dyn_step(ctx);
   if (rc) return rc;
-  launch_step_tail(ctx->fp, ctx->v, ctx->dt, ctx->dt0, ctx->isolver, ctx->st, ctx->flags, ctx->s);
-  if (int rc2 = read_status(ctx)) return rc2;
+  launch_step_tail(ctx->fp, ctx->d, ctx->v, ctx->a, ctx->fs.tau, ctx->dt, ctx->dt0, ctx->isolver, ctx->st,
+                   ctx->flags, capture(ctx, 0, false), ctx->s);
+  if (int rc2 = read_status(ctx, 1)) return rc2;
+  ctx->flags_dirty = false;
   const StepStatus& st = *ctx->st_host;
   info->t = ctx->t;
   info->it = ctx->it;
@@ -1310,10 +1377,54 @@ extern "C" int eq_step(eq_ctx* ctx, eq_step_info* info) {
   return EQ_OK;
 }
 
+// Up to kBatch dynamic steps enqueued without a host synchronisation.  Each
+// step is the Newmark tile kernel and the step tail, which writes the step's
+// record into its own slot and clears the run flag at the first step whose
+// regime switch, Newton failure or non-finite slip rate ends the batch; the
+// steps after it exit at once.  dt is dtmin for every dynamic step
+// (dtevol!, src/dtevol.jl:5-10), so the step times are known in advance.
+// Returns the number of steps taken in *m (>= 1).
+static int dyn_batch(eq_ctx* ctx, double total_time, int64_t max_more, int* m) {
+  int k = 0;
+  for (double tt = ctx->t; k < eq_ctx::kBatch && k < max_more && tt < total_time; k++) tt += ctx->dt;
+  if (k == 0) k = 1;
+  clean_flags(ctx);
+  ctx->flags_dirty = true;
+  launch_fill<int>(1, ctx->run, 1, ctx->s);
+  const double dtmin = ctx->dt0;
+  double* bufs[2][3] = {{ctx->d, ctx->v, ctx->a}, {ctx->d2, ctx->v2, ctx->a2}};
+  for (int q = 0; q < k; q++) {
+    double** src = bufs[q & 1];
+    double** dst = bufs[(q + 1) & 1];
+    launch_newmark(ctx->g, ctx->fp, ctx->fs, ctx->mu, ctx->M, src[0], src[1], src[2], dst[0], dst[1], dst[2],
+                   ctx->dt, 0.5 * dtmin, 0.5 * dtmin * dtmin, ctx->flags, ctx->run, ctx->s);
+    launch_step_tail(ctx->fp, dst[0], dst[1], dst[2], ctx->fs.tau, ctx->dt, dtmin, 2, ctx->st + q, ctx->flags,
+                     capture(ctx, q, true), ctx->s);
+  }
+  if (int rc = read_status(ctx, k)) return rc;
+  ctx->flags_dirty = false;
+  int n = k;
+  for (int q = 0; q < k; q++) {
+    const StepStatus& st = ctx->st_host[q];
+    if (st.isolver_next != 2 || st.nr_flag != 0 || !std::isfinite(st.Vfmax)) {
+      n = q + 1;
+      break;
+    }
+  }
+  if (n & 1) {
+    std::swap(ctx->d, ctx->d2);
+    std::swap(ctx->v, ctx->v2);
+    std::swap(ctx->a, ctx->a2);
+  }
+  *m = n;
+  return EQ_OK;
+}
+
 // results bookkeeping (src/main.jl:361-436)
 extern "C" int eq_run(eq_ctx* ctx, double total_time, int64_t max_steps, eq_results* R) {
   if (!ctx || !R) return EQ_EINVAL;
   if (!ctx->have_state) return fail(ctx, EQ_ENOTREADY, "eq_state_init first");
+  if (ctx->opts.pc == EQ_PC_AMG && !ctx->have_amg) return fail(ctx, EQ_ENOTREADY, "eq_amg_setup first");
   const int64_t NX = ctx->NX;
   const double Vpl = ctx->Vpl;
   const double yr2sec = 365.0 * 24 * 60 * 60;
@@ -1322,25 +1433,21 @@ extern "C" int eq_run(eq_ctx* ctx, double total_time, int64_t max_steps, eq_resu
   const double tevneinc = 0.5;
   double tevneb = 0, tevne = 0;
   int idelevne = 0, slipstart = 0;
-  std::vector<double> dF(NX), vF(NX), tau(NX), delfref(NX, 0.0);
+  std::vector<double> fault(3 * NX), delfref(NX, 0.0), tmp(NX);
   const size_t nseis = ctx->out_seis.size();
   R->n_steps = R->n_start = R->n_end = R->n_is = R->n_seis = R->n_rec = 0;
   R->n_qs = R->n_dyn = R->total_pcg_iters = R->nr_noconv = 0;
   R->wall_qs = R->wall_dyn = 0;
-  auto fault_fields = [&]() -> int {
-    if (int rc = d2h(ctx, dF.data(), ctx->d, NX)) return rc;
-    if (int rc = d2h(ctx, vF.data(), ctx->v, NX)) return rc;
-    if (int rc = d2h(ctx, tau.data(), ctx->fs.tau, NX)) return rc;
+  // fault rows of d, v and tau after the step of record slot q
+  auto fault_fields = [&](int q) -> int {
+    if (int rc = d2h(ctx, fault.data(), ctx->rec_fault + (int64_t)q * 3 * NX, 3 * NX)) return rc;
     return sync_check(ctx);
   };
-  using clk = std::chrono::steady_clock;
-  while (ctx->t < total_time && (max_steps <= 0 || R->n_steps < max_steps)) {
-    eq_step_info info;
-    const auto t0 = clk::now();
-    int rc = eq_step(ctx, &info);
-    const double el = std::chrono::duration<double>(clk::now() - t0).count();
-    if (info.isolver_used == 1) { R->n_qs++; R->wall_qs += el; } else { R->n_dyn++; R->wall_dyn += el; }
-    if (rc) return rc;
+  auto put = [&](double* M, int64_t cap, int64_t col, const std::vector<double>& src) {
+    if (M && col < cap) std::memcpy(M + col * NX, src.data(), sizeof(double) * NX);
+  };
+  // bookkeeping of one finished step whose record is in slot q
+  auto book = [&](const eq_step_info& info, int q) -> int {
     const int64_t k = R->n_steps;
     if (k < R->cap_steps) {
       if (R->time_) R->time_[k] = info.t;
@@ -1361,11 +1468,10 @@ extern "C" int eq_run(eq_ctx* ctx, double total_time, int64_t max_steps, eq_resu
       else if ((t - tevneb) > tevne) need_seis = true;
     }
     if (need_start || need_end || need_is || need_seis)
-      if (int rc2 = fault_fields()) return rc2;
-    auto put = [&](double* M, int64_t cap, int64_t col, const std::vector<double>& src) {
-      if (M && col < cap) std::memcpy(M + col * NX, src.data(), sizeof(double) * NX);
-    };
-    std::vector<double> tmp(NX);
+      if (int rc = fault_fields(q)) return rc;
+    const double* dF = fault.data();
+    const double* vF = fault.data() + NX;
+    const double* tau = fault.data() + 2 * NX;
     if (need_start) {
       for (int64_t j = 0; j < NX; j++) delfref[j] = 2 * dF[j] + Vpl * t;
       slipstart = 1;
@@ -1374,7 +1480,7 @@ extern "C" int eq_run(eq_ctx* ctx, double total_time, int64_t max_steps, eq_resu
         if (R->tStart) R->tStart[e] = t;
         for (int64_t j = 0; j < NX; j++) tmp[j] = (tau[j] + ctx->tauo[j]) / 1e6;
         put(R->taubefore, R->cap_events, e, tmp);
-        if (R->hypo) R->hypo[e] = ctx->FltX[ctx->st_host->hypo];
+        if (R->hypo) R->hypo[e] = ctx->FltX[ctx->st_host[q].hypo];
       }
       R->n_start++;
     }
@@ -1426,22 +1532,65 @@ extern "C" int eq_run(eq_ctx* ctx, double total_time, int64_t max_steps, eq_resu
     } else {
       idelevne = 0;
     }
-    if (ctx->it % 10 == 0 && nseis > 0) {
+    if (info.it % 10 == 0 && nseis > 0) {
       const int64_t e = R->n_rec;
       if (e < R->cap_rec) {
-        for (size_t q = 0; q < nseis; q++) {
-          double vals[3];
-          const int64_t n = ctx->out_seis[q];
-          if (int rc2 = d2h(ctx, &vals[0], ctx->d + n, 1)) return rc2;
-          if (int rc2 = d2h(ctx, &vals[1], ctx->v + n, 1)) return rc2;
-          if (int rc2 = d2h(ctx, &vals[2], ctx->a + n, 1)) return rc2;
-          if (int rc2 = sync_check(ctx)) return rc2;
-          if (R->dSeis) R->dSeis[q * R->cap_rec + e] = vals[0];
-          if (R->vSeis) R->vSeis[q * R->cap_rec + e] = vals[1];
-          if (R->aSeis) R->aSeis[q * R->cap_rec + e] = vals[2];
+        const double* smp = ctx->rec_seis_host + (int64_t)q * 3 * nseis;
+        for (size_t r = 0; r < nseis; r++) {
+          if (R->dSeis) R->dSeis[r * R->cap_rec + e] = smp[3 * r];
+          if (R->vSeis) R->vSeis[r * R->cap_rec + e] = smp[3 * r + 1];
+          if (R->aSeis) R->aSeis[r * R->cap_rec + e] = smp[3 * r + 2];
         }
       }
       R->n_rec++;
+    }
+    return EQ_OK;
+  };
+  using clk = std::chrono::steady_clock;
+  while (ctx->t < total_time && (max_steps <= 0 || R->n_steps < max_steps)) {
+    const auto t0 = clk::now();
+    if (ctx->isolver == 2) {
+      // dynamic regime: a batch of steps, then the bookkeeping of each
+      int m = 0;
+      const int64_t more = max_steps > 0 ? max_steps - R->n_steps : INT64_MAX;
+      int rc = dyn_batch(ctx, total_time, more, &m);
+      R->wall_dyn += std::chrono::duration<double>(clk::now() - t0).count();
+      if (rc) return rc;
+      for (int q = 0; q < m; q++) {
+        const StepStatus& st = ctx->st_host[q];
+        eq_step_info info{};
+        ctx->it += 1;
+        ctx->t += ctx->dt;
+        info.dt_used = ctx->dt;
+        info.isolver_used = 2;
+        info.t = ctx->t;
+        info.it = ctx->it;
+        info.Vfmax = st.Vfmax;
+        info.dt_next = st.dt_next;
+        info.isolver_next = st.isolver_next;
+        info.nr_noconv = st.nr_noconv;
+        ctx->nr_noconv_total += st.nr_noconv;
+        R->n_dyn++;
+        if (q == m - 1) {
+          ctx->isolver = st.isolver_next;
+          ctx->dt = st.dt_next;
+        }
+        if (st.nr_flag) {
+          char buf[128];
+          snprintf(buf, sizeof buf, "NR SEARCH FAILED! (fault location %d)", st.nr_flag);
+          ctx->err = buf;
+          return EQ_NR_NAN;
+        }
+        if (!std::isfinite(st.Vfmax)) return fail(ctx, EQ_NR_NAN, "non-finite slip rate");
+        if (int rc2 = book(info, q)) return rc2;
+      }
+    } else {
+      eq_step_info info;
+      int rc = eq_step(ctx, &info);
+      const double el = std::chrono::duration<double>(clk::now() - t0).count();
+      if (info.isolver_used == 1) { R->n_qs++; R->wall_qs += el; } else { R->n_dyn++; R->wall_dyn += el; }
+      if (rc) return rc;
+      if (int rc2 = book(info, 0)) return rc2;
     }
   }
   return EQ_OK;
@@ -1549,7 +1698,7 @@ extern "C" int eq_time_kernel(eq_ctx* ctx, int which, int prec, int reps, double
   auto body = [&]() {
     if (which == EQ_TIME_NEWMARK) {
       launch_newmark(ctx->g, ctx->fp, ctx->fs_api, ctx->mu, ctx->M, ctx->d, ctx->v, ctx->a, ctx->d2, ctx->v2,
-                     ctx->a2, ctx->dt0, 0.5 * ctx->dt0, 0.5 * ctx->dt0 * ctx->dt0, ctx->flags, s);
+                     ctx->a2, ctx->dt0, 0.5 * ctx->dt0, 0.5 * ctx->dt0 * ctx->dt0, ctx->flags, nullptr, s);
     } else if (prec == EQ_PREC_FP32) {
       time_body<float>(ctx, which);
     } else {
@@ -1557,6 +1706,7 @@ extern "C" int eq_time_kernel(eq_ctx* ctx, int which, int prec, int reps, double
     }
   };
   if (which == EQ_TIME_NEWMARK) {
+    ctx->flags_dirty = true;
     // the fault routine works on the API scratch copy of the fault state
     HIPCHK(ctx, hipMemcpyAsync(ctx->fs_api.psi, ctx->fs.psi, sizeof(double) * 11 * ctx->NX,
                                hipMemcpyDeviceToDevice, s));

@@ -44,6 +44,19 @@ struct StepStatus {   // written by the step tail kernel
   int nr_noconv;     // Newton searches that took the failure exit
 };
 
+// Per-step record written by the step tail kernel besides its StepStatus:
+// fault rows of d and v and the fault traction (3 x nflt), and d, v, a at the
+// receiver nodes (3 x nseis).  run (may be null) is the batch flag: a batch of
+// dynamic steps stops, on the device, at the first step whose regime switch,
+// Newton failure or non-finite slip rate the tail detects.
+struct StepCapture {
+  double* fault;            // [3 * nflt] or null
+  const int64_t* seis_idx;  // structured node index of each receiver
+  int nseis;
+  double* seis;             // [3 * nseis] or null
+  int* run;
+};
+
 inline dim3 tile_grid(const Geom& g) {
   return dim3((g.NX + TILE_I - 1) / TILE_I, (g.NY + TILE_J - 1) / TILE_J);
 }
@@ -138,15 +151,18 @@ void launch_dense(int64_t n, const double* inv, const T* b, T* x, const CgScal* 
 // ---- time stepping --------------------------------------------------------
 void launch_newmark(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, const double* M,
                     const double* d, const double* v, const double* a, double* d2, double* v2, double* a2,
-                    double dt, double half_dt, double half_dt_sq, int* flags, hipStream_t s);
+                    double dt, double half_dt, double half_dt_sq, int* flags, const int* run, hipStream_t s);
 void launch_qs_begin(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, double* d, double* dPre,
                      const double* v, double* b, double dt, int first_pass, hipStream_t s);
 void launch_qs_fault(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, const double* d,
                      double* v, double dt, hipStream_t s);
 void launch_qs_finish(const Geom& g, const FaultDev& fp, FaultState fs, double* d, const double* dPre, double* v,
                       double* a, double dt, hipStream_t s);
-void launch_step_tail(const FaultDev& fp, const double* v, double dt, double dtmin, int isolver, StepStatus* st,
-                      const int* flags, hipStream_t s);
+// Vfmax, regime switch, dtevol!, Newton flags (then reset for the next step),
+// and the step record of cap.
+void launch_step_tail(const FaultDev& fp, const double* d, const double* v, const double* a, const double* tau,
+                      double dt, double dtmin, int isolver, StepStatus* st, int* flags, const StepCapture& cap,
+                      hipStream_t s);
 
 // ---- fault routines on their own (API parity entry points) ---------------
 void launch_ids(int64_t n, const double* xLf, const double* Vo, const double* psi, double dt, const double* Vf,

@@ -613,10 +613,11 @@ struct OpNewmark {
   double *d2, *v2, *a2;
   double dt, half_dt, half_dt_sq;
   int* flags;
+  const int* run;   // null, or batch flag: 0 = the batch has stopped, do nothing
   int NY;
   double* part0;
   double* part1;
-  __device__ bool prepare(Pre&) const { return true; }
+  __device__ bool prepare(Pre&) const { return run == nullptr || *run != 0; }
   __device__ double load(const Pre&, int64_t n, int, int) const { return d[n] + dt * v[n] + half_dt_sq * a[n]; }
   __device__ void epilogue(const Pre&, int64_t n, int I, int J, double ku, double dn, double&, double&) const {
     const double vo = v[n];
@@ -913,8 +914,11 @@ __device__ double dtevol_dev(const double* XiLf, const double* v, bool slip_rate
   return dtnx;
 }
 
-__global__ __launch_bounds__(BLOCK) void step_tail_kernel(FaultDev fp, const double* v, double dt, double dtmin,
-                                                          int isolver, StepStatus* st, const int* flags) {
+__global__ __launch_bounds__(BLOCK) void step_tail_kernel(FaultDev fp, const double* d, const double* v,
+                                                          const double* a, const double* tau, double dt,
+                                                          double dtmin, int isolver, StepStatus* st, int* flags,
+                                                          StepCapture cap) {
+  if (cap.run && *cap.run == 0) return;   // block-uniform: the batch stopped earlier
   __shared__ double smax[BLOCK];
   __shared__ int sidx[BLOCK];
   double m = -INFINITY;
@@ -940,13 +944,32 @@ __global__ __launch_bounds__(BLOCK) void step_tail_kernel(FaultDev fp, const dou
   const double Vfmax = 2 * smax[0] + fp.Vpl;
   const int next = ((isolver == 1 && Vfmax < 5e-3) || (isolver == 2 && Vfmax < 2e-3)) ? 1 : 2;
   const double dtn = dtevol_dev(fp.XiLf, v, true, fp.Vpl, fp.j0, fp.nflt, dt, dtmin, next);
+  if (cap.fault) {
+    for (int j = threadIdx.x; j < fp.nflt; j += BLOCK) {
+      cap.fault[j] = d[j];
+      cap.fault[fp.nflt + j] = v[j];
+      cap.fault[2 * fp.nflt + j] = tau[j];
+    }
+  }
+  if (cap.seis) {
+    for (int q = threadIdx.x; q < cap.nseis; q += BLOCK) {
+      const int64_t n = cap.seis_idx[q];
+      cap.seis[3 * q] = d[n];
+      cap.seis[3 * q + 1] = v[n];
+      cap.seis[3 * q + 2] = a[n];
+    }
+  }
   if (threadIdx.x == 0) {
+    const int nr_flag = (flags[0] == INT_MAX) ? 0 : flags[0];
     st->Vfmax = Vfmax;
     st->dt_next = dtn;
     st->isolver_next = next;
     st->hypo = sidx[0];
-    st->nr_flag = (flags[0] == INT_MAX) ? 0 : flags[0];
+    st->nr_flag = nr_flag;
     st->nr_noconv = flags[1];
+    flags[0] = INT_MAX;
+    flags[1] = 0;
+    if (cap.run && (next != isolver || nr_flag != 0 || !isfinite(Vfmax))) *cap.run = 0;
   }
 }
 
@@ -1257,8 +1280,8 @@ void launch_dense(int64_t n, const double* inv, const T* b, T* x, const CgScal* 
 
 void launch_newmark(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, const double* M,
                     const double* d, const double* v, const double* a, double* d2, double* v2, double* a2,
-                    double dt, double half_dt, double half_dt_sq, int* flags, hipStream_t s) {
-  OpNewmark op{fp, fs, mu, M, d, v, a, d2, v2, a2, dt, half_dt, half_dt_sq, flags, g.NY, nullptr, nullptr};
+                    double dt, double half_dt, double half_dt_sq, int* flags, const int* run, hipStream_t s) {
+  OpNewmark op{fp, fs, mu, M, d, v, a, d2, v2, a2, dt, half_dt, half_dt_sq, flags, run, g.NY, nullptr, nullptr};
   hipLaunchKernelGGL((tile_kernel<double, OpNewmark>), tile_grid(g), dim3(BLOCK), 0, s, g, op);
 }
 
@@ -1280,9 +1303,11 @@ void launch_qs_finish(const Geom& g, const FaultDev& fp, FaultState fs, double* 
   hipLaunchKernelGGL(qs_finish_kernel, dim3(nblk(N)), dim3(BLOCK), 0, s, g, fp, fs, d, dPre, v, a, dt);
 }
 
-void launch_step_tail(const FaultDev& fp, const double* v, double dt, double dtmin, int isolver, StepStatus* st,
-                      const int* flags, hipStream_t s) {
-  hipLaunchKernelGGL(step_tail_kernel, dim3(1), dim3(BLOCK), 0, s, fp, v, dt, dtmin, isolver, st, flags);
+void launch_step_tail(const FaultDev& fp, const double* d, const double* v, const double* a, const double* tau,
+                      double dt, double dtmin, int isolver, StepStatus* st, int* flags, const StepCapture& cap,
+                      hipStream_t s) {
+  hipLaunchKernelGGL(step_tail_kernel, dim3(1), dim3(BLOCK), 0, s, fp, d, v, a, tau, dt, dtmin, isolver, st, flags,
+                     cap);
 }
 
 void launch_ids(int64_t n, const double* xLf, const double* Vo, const double* psi, double dt, const double* Vf,

@@ -310,6 +310,41 @@ def test_run_matches_oracle(oracle):
     assert np.allclose(R["tEnd"], r.tEnd, rtol=1e-9, atol=0)
     assert m["delfafter"] < 1e-8
     assert m["d"] < 1e-8
+    # receiver traces every 10 steps and the seismic fault snapshots; slip
+    # 2 d + Vpl t carries the time_ difference (measured 4.6e-8, time_ 2.3e-8)
+    dS = np.array(r.dSeis)
+    assert R["dSeis"].shape == dS.shape
+    assert rel(R["dSeis"], dS) < 1e-8
+    assert len(R["seismic_slip"]) == len(r.seismic_slip) >= 1
+    assert rel(R["seismic_slip"], np.array(r.seismic_slip)) < 1e-7
+
+
+def test_run_batched_dynamic_steps_equal_single_steps():
+    """eq_run advances dynamic steps in device-side batches that stop at the
+    regime switch; the trajectory must equal, bit for bit, one eq_step call per
+    step (same kernels, same order), including a max_steps cut inside a batch."""
+    from eqcycle_amd.api import EqContext
+    P = set_parameters(small(32, 8, material="trapezoid", total_years=3))
+    a = EqContext(P, amg=True)
+    a.set_solver(tol=1e-12, pc="amg")
+    a.state_init()
+    R1 = a.run(max_steps=1000)            # stops inside the first event's batches
+    R2 = a.run()                          # rest of the 3 years, through the event end
+    b = EqContext(P, amg=True)
+    b.set_solver(tol=1e-12, pc="amg")
+    b.state_init()
+    n = R1["n_steps"] + R2["n_steps"]
+    infos = [b.step() for _ in range(n)]
+    assert R1["n_steps"] == 1000
+    iso = np.array([i.isolver_used for i in infos])
+    assert np.array_equal(np.concatenate([R1["isolver"], R2["isolver"]]), iso)
+    assert (iso == 2).sum() > 1000 and iso[-1] == 1      # the event ended inside the window
+    assert np.array_equal(np.concatenate([R1["Vfmax"], R2["Vfmax"]]), [i.Vfmax for i in infos])
+    assert np.array_equal(np.concatenate([R1["time_"], R2["time_"]]), [i.t for i in infos])
+    sa, sb = a.state_get(), b.state_get()
+    for k in ("d", "v", "a", "psi", "tau", "Vf"):
+        assert np.array_equal(sa[k], sb[k]), k
+    assert (sa["t"], sa["dt"], sa["it"], sa["isolver"]) == (sb["t"], sb["dt"], sb["it"], sb["isolver"])
 
 
 def test_state_init_after_dynamic_steps(oracle):
