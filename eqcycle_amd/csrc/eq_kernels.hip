@@ -202,12 +202,19 @@ __device__ __forceinline__ T seg_last(const Geom& g, const T* u, const T* m) {
   return r;
 }
 
-// Tile kernel.  Phase 1 stages the window of the input vector (op.load) and
-// of mu in LDS.  Phase 2: each thread takes 4 consecutive nodes of one row
-// (one segment) and accumulates their x-direction terms into an LDS tile.
-// Phase 3: each thread takes 4 consecutive nodes of one column, adds the
-// y-direction terms and runs the epilogue.  Every 5-node segment is evaluated
-// once per direction (plus the shared end node of the previous segment).
+// Tile kernel.  Phase 1 issues every global load of the block at once: the
+// (64+5) x (16+5) window of the input vector (op.load) and of mu, held in
+// registers, and the epilogue operands of the thread's 4 owned nodes
+// (op.fetch); only then are the window values stored to LDS.  Phase 2: each
+// thread takes 4 consecutive nodes of one row (one segment) and accumulates
+// their x-direction terms into an LDS tile.  Phase 3: each thread takes 4
+// consecutive nodes of one column, adds the y-direction terms and runs the
+// epilogue.  Every 5-node segment is evaluated once per direction (plus the
+// shared end node of the previous segment).
+static_assert(TILE_J * (TILE_I / 4) == BLOCK, "phase 2: one row strip per thread");
+static_assert(TILE_I * (TILE_J / 4) == BLOCK, "phase 3: one column strip per thread");
+constexpr int WIN_LOADS = (WIN_J * WIN_I + BLOCK - 1) / BLOCK;
+
 template <typename T, class Op>
 __global__ __launch_bounds__(BLOCK) void tile_kernel(Geom g, Op op) {
   __shared__ T su[WIN_J * WIN_I];
@@ -216,31 +223,49 @@ __global__ __launch_bounds__(BLOCK) void tile_kernel(Geom g, Op op) {
   typename Op::Pre pre;
   if (!op.prepare(pre)) return;   // block-uniform
   const int I0 = blockIdx.x * TILE_I, J0 = blockIdx.y * TILE_J;
-  for (int k = threadIdx.x; k < WIN_J * WIN_I; k += BLOCK) {
-    const int r = k / WIN_I, c = k - r * WIN_I;
-    const int I = I0 - 4 + c, J = J0 - 4 + r;
-    T u = 0, m = 0;
-    if (I >= 0 && I < g.NX && J >= 0 && J < g.NY) {
-      const int64_t n = (int64_t)J * g.NX + I;
-      u = op.load(pre, n, I, J);
-      m = op.mu[n];
+  // phase-3 ownership of this thread: column c, rows Js..Js+3
+  const int c3 = threadIdx.x % TILE_I, q3 = threadIdx.x / TILE_I;
+  const int I3 = I0 + c3, Js3 = J0 + 4 * q3;
+  typename Op::Ext ext[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++)
+    if (I3 < g.NX && Js3 + j < g.NY) op.fetch(pre, (int64_t)(Js3 + j) * g.NX + I3, I3, Js3 + j, ext[j]);
+  {
+    T ur[WIN_LOADS], mr[WIN_LOADS];
+#pragma unroll
+    for (int t = 0; t < WIN_LOADS; t++) {
+      const int k = threadIdx.x + t * BLOCK;
+      const int r = k / WIN_I, c = k - r * WIN_I;
+      const int I = I0 - 4 + c, J = J0 - 4 + r;
+      ur[t] = 0;
+      mr[t] = 0;
+      if (k < WIN_J * WIN_I && I >= 0 && I < g.NX && J >= 0 && J < g.NY) {
+        const int64_t n = (int64_t)J * g.NX + I;
+        ur[t] = op.load(pre, n, I, J);
+        mr[t] = op.mu[n];
+      }
     }
-    su[k] = u;
-    sm[k] = m;
+#pragma unroll
+    for (int t = 0; t < WIN_LOADS; t++) {
+      const int k = threadIdx.x + t * BLOCK;
+      if (k < WIN_J * WIN_I) {
+        su[k] = ur[t];
+        sm[k] = mr[t];
+      }
+    }
   }
   __syncthreads();
   {
     // x-direction: thread -> (row r, strip q of 4 nodes); TILE_I / 4 strips per row
     constexpr int NSTRIP = TILE_I / 4;
-    for (int k = threadIdx.x; k < TILE_J * NSTRIP; k += BLOCK) {
-      const int r = k / NSTRIP, q = k - r * NSTRIP;
-      const int J = J0 + r, Is = I0 + 4 * q;
-      if (J >= g.NY || Is >= g.NX) continue;
-      const int s = Is >> 2;
+    const int r = threadIdx.x / NSTRIP, q = threadIdx.x - r * NSTRIP;
+    const int J = J0 + r, Is = I0 + 4 * q;
+    if (J < g.NY && Is < g.NX) {
+      const int sg = Is >> 2;
       const int lr = r + 4, lc = 4 * q + 4;
       T o[5] = {0, 0, 0, 0, 0};
-      if (s < g.nelx) seg_all<T, 1>(g, su + lr * WIN_I + lc, sm + lr * WIN_I + lc, o);
-      if (s > 0) o[0] += seg_last<T, 1>(g, su + lr * WIN_I + lc - 4, sm + lr * WIN_I + lc - 4);
+      if (sg < g.nelx) seg_all<T, 1>(g, su + lr * WIN_I + lc, sm + lr * WIN_I + lc, o);
+      if (sg > 0) o[0] += seg_last<T, 1>(g, su + lr * WIN_I + lc - 4, sm + lr * WIN_I + lc - 4);
       const T cx = (T)(g.c1 * wline(g, J, g.NY));
 #pragma unroll
       for (int i = 0; i < 4; i++) sx[r * TILE_I + 4 * q + i] = cx * o[i];
@@ -248,25 +273,20 @@ __global__ __launch_bounds__(BLOCK) void tile_kernel(Geom g, Op op) {
   }
   __syncthreads();
   double acc0 = 0.0, acc1 = 0.0;
-  {
-    // y-direction + epilogue: thread -> (column c, strip q of 4 rows)
-    constexpr int NSTRIP = TILE_J / 4;
-    for (int k = threadIdx.x; k < TILE_I * NSTRIP; k += BLOCK) {
-      const int q = k / TILE_I, c = k - q * TILE_I;
-      const int I = I0 + c, Js = J0 + 4 * q;
-      if (I >= g.NX || Js >= g.NY) continue;
-      const int t = Js >> 2;
-      const int lc = c + 4, lr = 4 * q + 4;
-      T o[5] = {0, 0, 0, 0, 0};
-      if (t < g.nely) seg_all<T, WIN_I>(g, su + lr * WIN_I + lc, sm + lr * WIN_I + lc, o);
-      if (t > 0) o[0] += seg_last<T, WIN_I>(g, su + (lr - 4) * WIN_I + lc, sm + (lr - 4) * WIN_I + lc);
-      const T cy = (T)(g.c2 * wline(g, I, g.NX));
+  if (I3 < g.NX && Js3 < g.NY) {
+    // y-direction + epilogue
+    const int t = Js3 >> 2;
+    const int lc = c3 + 4, lr = 4 * q3 + 4;
+    T o[5] = {0, 0, 0, 0, 0};
+    if (t < g.nely) seg_all<T, WIN_I>(g, su + lr * WIN_I + lc, sm + lr * WIN_I + lc, o);
+    if (t > 0) o[0] += seg_last<T, WIN_I>(g, su + (lr - 4) * WIN_I + lc, sm + (lr - 4) * WIN_I + lc);
+    const T cy = (T)(g.c2 * wline(g, I3, g.NX));
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
-        const int J = Js + j;
-        if (J >= g.NY) break;
-        const T ku = sx[(4 * q + j) * TILE_I + c] + cy * o[j];
-        op.epilogue(pre, (int64_t)J * g.NX + I, I, J, ku, su[(lr + j) * WIN_I + lc], acc0, acc1);
+    for (int j = 0; j < 4; j++) {
+      const int J = Js3 + j;
+      if (J < g.NY) {
+        const T ku = sx[(4 * q3 + j) * TILE_I + c3] + cy * o[j];
+        op.epilogue(pre, (int64_t)J * g.NX + I3, I3, J, ku, su[(lr + j) * WIN_I + lc], ext[j], acc0, acc1);
       }
     }
   }
@@ -279,6 +299,9 @@ __global__ __launch_bounds__(BLOCK) void tile_kernel(Geom g, Op op) {
     if (threadIdx.x == 0) op.part1[blockIdx.y * gridDim.x + blockIdx.x] = s;
   }
 }
+
+// Epilogue operands with nothing to prefetch.
+struct ExtNone {};
 
 // y = sign K u  (interior: fault row of u read as 0, of y written as 0)
 template <typename T>
@@ -293,8 +316,10 @@ struct OpApply {
   double* part0;
   double* part1;
   __device__ bool prepare(Pre&) const { return true; }
+  using Ext = ExtNone;
+  __device__ void fetch(const Pre&, int64_t, int, int, Ext&) const {}
   __device__ T load(const Pre&, int64_t n, int, int J) const { return (interior && J == 0) ? T(0) : u[n]; }
-  __device__ void epilogue(const Pre&, int64_t n, int, int J, T ku, T, double&, double&) const {
+  __device__ void epilogue(const Pre&, int64_t n, int, int J, T ku, T, const Ext&, double&, double&) const {
     y[n] = (interior && J == 0) ? T(0) : sign * ku;
   }
 };
@@ -311,9 +336,11 @@ struct OpResidual {
   double* part0;
   double* part1;
   __device__ bool prepare(Pre&) const { return true; }
+  struct Ext { T b; };
+  __device__ void fetch(const Pre&, int64_t n, int, int J, Ext& e) const { e.b = J == 0 ? T(0) : b[n]; }
   __device__ T load(const Pre&, int64_t n, int, int J) const { return J == 0 ? T(0) : x[n]; }
-  __device__ void epilogue(const Pre&, int64_t n, int, int J, T ku, T, double& a0, double& a1) const {
-    const T bv = J == 0 ? T(0) : b[n];
+  __device__ void epilogue(const Pre&, int64_t n, int, int J, T ku, T, const Ext& e, double& a0, double& a1) const {
+    const T bv = e.b;
     const T rv = J == 0 ? T(0) : bv - ku;
     r[n] = rv;
     a0 += (double)rv * (double)rv;
@@ -342,8 +369,10 @@ struct OpCgDir {
     p.beta = (T)(rho / sc->rho_prev);
     return true;
   }
+  using Ext = ExtNone;
+  __device__ void fetch(const Pre&, int64_t, int, int, Ext&) const {}
   __device__ T load(const Pre& p, int64_t n, int, int J) const { return J == 0 ? T(0) : c[n] + p.beta * u_old[n]; }
-  __device__ void epilogue(const Pre&, int64_t n, int, int J, T ku, T uc, double& a0, double&) const {
+  __device__ void epilogue(const Pre&, int64_t n, int, int J, T ku, T uc, const Ext&, double& a0, double&) const {
     const T qv = J == 0 ? T(0) : ku;
     u_new[n] = uc;
     q[n] = qv;
@@ -379,10 +408,12 @@ struct OpL0Down {
     }
     return !fin;
   }
+  struct Ext { T r; };
+  __device__ void fetch(const Pre&, int64_t n, int, int J, Ext& e) const { e.r = J == 0 ? T(0) : r[n]; }
   __device__ T load(const Pre&, int64_t n, int, int J) const { return J == 0 ? T(0) : omega * dinv[n] * r[n]; }
-  __device__ void epilogue(const Pre&, int64_t n, int, int J, T ku, T uc, double&, double&) const {
+  __device__ void epilogue(const Pre&, int64_t n, int, int J, T ku, T uc, const Ext& e, double&, double&) const {
     x0[n] = uc;
-    res[n] = J == 0 ? T(0) : r[n] - ku;
+    res[n] = J == 0 ? T(0) : e.r - ku;
   }
 };
 
@@ -401,11 +432,16 @@ struct OpL0Post {
   double* part0;
   double* part1;
   __device__ bool prepare(Pre&) const { return !sc->done; }
+  struct Ext { T r, dinv; };
+  __device__ void fetch(const Pre&, int64_t n, int, int J, Ext& e) const {
+    e.r = r[n];
+    e.dinv = J == 0 ? T(0) : dinv[n];
+  }
   __device__ T load(const Pre&, int64_t n, int, int J) const { return J == 0 ? T(0) : x0[n]; }
-  __device__ void epilogue(const Pre&, int64_t n, int, int J, T ku, T uc, double& a0, double&) const {
-    const T cv = J == 0 ? T(0) : uc + omega * dinv[n] * (r[n] - ku);
+  __device__ void epilogue(const Pre&, int64_t n, int, int J, T ku, T uc, const Ext& e, double& a0, double&) const {
+    const T cv = J == 0 ? T(0) : uc + omega * e.dinv * (e.r - ku);
     c[n] = cv;
-    a0 += (double)cv * (double)r[n];
+    a0 += (double)cv * (double)e.r;
   }
 };
 
@@ -478,27 +514,37 @@ struct OpCheb {
     }
     return src[n];
   }
-  __device__ void epilogue(const Pre&, int64_t n, int, int J, T ku, T wc, double& a0, double&) const {
+  struct Ext { T rin, dinv, xin, rdot; };
+  __device__ void fetch(const Pre&, int64_t n, int, int J, Ext& e) const {
+    if (J == 0) return;
+    e.rin = rin[n];
+    if (mode != 1) {
+      e.dinv = dinv[n];
+      if (!xsrc_center) e.xin = xin[n];
+      if (mode == 2) e.rdot = rdot[n];
+    }
+  }
+  __device__ void epilogue(const Pre&, int64_t n, int, int J, T ku, T wc, const Ext& e, double& a0, double&) const {
     if (J == 0) {
       if (mode == 0) { rout[n] = T(0); dout[n] = T(0); xout[n] = T(0); }
       else if (mode == 1) { rout[n] = T(0); if (write_x_center) xout[n] = T(0); }
       else xout[n] = T(0);
       return;
     }
-    const T r = rin[n] - ku;
+    const T r = e.rin - ku;
     if (mode == 1) {
       rout[n] = r;
       if (write_x_center) xout[n] = wc;
       return;
     }
-    const T dn = c1 * wc + c2 * dinv[n] * r;
-    const T xn = (xsrc_center ? wc : xin[n]) + dn;
+    const T dn = c1 * wc + c2 * e.dinv * r;
+    const T xn = (xsrc_center ? wc : e.xin) + dn;
     xout[n] = xn;
     if (mode == 0) {
       rout[n] = r;
       dout[n] = dn;
     } else {
-      a0 += (double)xn * (double)rdot[n];
+      a0 += (double)xn * (double)e.rdot;
     }
   }
 };
@@ -514,11 +560,14 @@ struct OpGuessDot {
   double* part0;
   double* part1;
   __device__ bool prepare(Pre&) const { return true; }
+  struct Ext { double r0; };
+  __device__ void fetch(const Pre&, int64_t n, int, int, Ext& e) const { e.r0 = r0[n]; }
   __device__ double load(const Pre&, int64_t n, int, int J) const { return J == 0 ? 0.0 : p[n]; }
-  __device__ void epilogue(const Pre&, int64_t n, int, int J, double ku, double pc, double& a0, double& a1) const {
+  __device__ void epilogue(const Pre&, int64_t n, int, int J, double ku, double pc, const Ext& e, double& a0,
+                           double& a1) const {
     const double qv = J == 0 ? 0.0 : ku;
     q[n] = qv;
-    a0 += pc * r0[n];
+    a0 += pc * e.r0;
     a1 += pc * qv;
   }
 };
@@ -618,10 +667,17 @@ struct OpNewmark {
   double* part0;
   double* part1;
   __device__ bool prepare(Pre&) const { return run == nullptr || *run != 0; }
+  struct Ext { double v, a, M; };
+  __device__ void fetch(const Pre&, int64_t n, int, int, Ext& e) const {
+    e.v = v[n];
+    e.a = a[n];
+    e.M = M[n];
+  }
   __device__ double load(const Pre&, int64_t n, int, int) const { return d[n] + dt * v[n] + half_dt_sq * a[n]; }
-  __device__ void epilogue(const Pre&, int64_t n, int I, int J, double ku, double dn, double&, double&) const {
-    const double vo = v[n];
-    const double vp = vo + half_dt * a[n];
+  __device__ void epilogue(const Pre&, int64_t n, int I, int J, double ku, double dn, const Ext& e, double&,
+                           double&) const {
+    const double vo = e.v;
+    const double vp = vo + half_dt * e.a;
     double at = -ku;
     const bool fault = (J == 0);
     const bool deep = fault && (I < fp.j0);
