@@ -847,16 +847,13 @@ static int cg_device(eq_ctx* ctx, T* x, const T* b, double tol, int64_t maxiter,
   if (pc == EQ_PC_AMG && !ctx->have_amg) return fail(ctx, EQ_ENOTREADY, "eq_amg_setup first");
   if (maxiter <= 0) maxiter = ctx->nI;
   if (maxiter > INT_MAX) maxiter = INT_MAX;
-  if (!r_ready) {
-    launch_fill<double>(ctx->npart, ctx->part_rr, 0.0, s);
-    launch_fill<double>(ctx->npart, ctx->part_bb, 0.0, s);
-    launch_cg_residual<T>(g, x, mu_of<T>(ctx), b, w.r, ctx->part_rr, ctx->part_bb, s);
-  }
+  if (!r_ready) launch_cg_residual<T>(g, x, mu_of<T>(ctx), b, w.r, ctx->part_rr, ctx->part_bb, ctx->npart, s);
   launch_cg_init_scal(ctx->sc, ctx->part_rr, ctx->part_bb, ctx->npart, tol, (int)maxiter, s);
-  launch_fill<T>(ctx->N, w.u[w.cur], T(0), s);
   if (f32pc) launch_convert<T, float>(ctx->N, w.r, ctx->w32.r, s);
+  // iterations enqueued before the first read-back: the last count of the
+  // same kind of solve (0 when that one was already converged at the start)
   int64_t& pred = ctx->pred_iters[ctx->cg_slot];
-  int64_t batch = std::max<int64_t>(1, pred);
+  int64_t batch = pred;
   for (;;) {
     for (int64_t k = 0; k < batch; k++) cg_iteration<T>(ctx, x, pc, f32pc);
     launch_cg_check(ctx->sc, ctx->part_rr, ctx->npart, s);
@@ -868,7 +865,7 @@ static int cg_device(eq_ctx* ctx, T* x, const T* b, double tol, int64_t maxiter,
   const CgScal& h = *ctx->sc_host;
   if (iters) *iters = h.iter;
   if (relres) *relres = h.bnorm > 0 ? h.resid / h.bnorm : h.resid;
-  pred = std::max<int64_t>(1, h.iter);
+  pred = h.iter;
   return (h.resid <= h.reltol) ? EQ_OK : EQ_PCG_NOCONV;
 }
 
@@ -884,9 +881,7 @@ static int cg_mixed(eq_ctx* ctx, double* x, const double* b, double tol, int64_t
   double bnorm = 0, resid = 0;
   const double inner_tol = 1e-3;
   for (int outer = 0; outer < 50; outer++) {
-    launch_fill<double>(ctx->npart, ctx->part_rr, 0.0, s);
-    launch_fill<double>(ctx->npart, ctx->part_bb, 0.0, s);
-    launch_cg_residual<double>(g, x, ctx->mu, b, w.r, ctx->part_rr, ctx->part_bb, s);
+    launch_cg_residual<double>(g, x, ctx->mu, b, w.r, ctx->part_rr, ctx->part_bb, ctx->npart, s);
     launch_cg_init_scal(ctx->sc, ctx->part_rr, ctx->part_bb, ctx->npart, tol, 1, s);
     HIPCHK(ctx, hipMemcpyAsync(ctx->sc_host, ctx->sc, sizeof(CgScal), hipMemcpyDeviceToHost, s));
     if (int rc = sync_check(ctx)) return rc;
@@ -1296,9 +1291,7 @@ static void learned_correction(eq_ctx* ctx, double* x, const double* b) {
   const Geom& g = ctx->g;
   hipStream_t s = ctx->s;
   const int nt = tile_count(g);
-  launch_fill<double>(ctx->npart, ctx->part_rr, 0.0, s);
-  launch_fill<double>(ctx->npart, ctx->part_bb, 0.0, s);
-  launch_cg_residual<double>(g, x, ctx->mu, b, w.r, ctx->part_rr, ctx->part_bb, s);
+  launch_cg_residual<double>(g, x, ctx->mu, b, w.r, ctx->part_rr, ctx->part_bb, ctx->npart, s);
   learn_guess(&ctx->learn, x, ctx->dPre, ctx->gp, s);
   launch_guess_dot(g, ctx->gp, ctx->mu, w.r, ctx->gq, ctx->part_rho, ctx->part_uq, s);
   launch_guess_update(ctx->N, x, w.r, ctx->gp, ctx->gq, ctx->part_rho, ctx->part_uq, nt, ctx->part_rr, ctx->npart,

@@ -76,7 +76,7 @@ void launch_element_ku(const Geom& g, const double* u, const double* mu, double*
 // ---- CG pieces (K_II form, vectors of length N with zero fault row) ------
 template <typename T>
 void launch_cg_residual(const Geom& g, const T* x, const T* mu, const T* b, T* r, double* part_rr,
-                        double* part_bb, hipStream_t s);
+                        double* part_bb, int npart, hipStream_t s);
 void launch_cg_init_scal(CgScal* sc, const double* part_rr, const double* part_bb, int nparts, double tol,
                          int maxiter, hipStream_t s);
 void launch_cg_check(CgScal* sc, const double* part_rr, int nparts, hipStream_t s);

@@ -298,6 +298,7 @@ __global__ __launch_bounds__(BLOCK) void tile_kernel(Geom g, Op op) {
     const double s = block_sum(acc1);
     if (threadIdx.x == 0) op.part1[blockIdx.y * gridDim.x + blockIdx.x] = s;
   }
+  if (blockIdx.x == 0 && blockIdx.y == 0) op.zero_tail((int)(gridDim.x * gridDim.y));
 }
 
 // Epilogue operands with nothing to prefetch.
@@ -308,6 +309,7 @@ template <typename T>
 struct OpApply {
   using Pre = PreNone;
   static constexpr int kParts = 0;
+  __device__ void zero_tail(int) const {}
   const T* u;
   const T* mu;
   T* y;
@@ -329,6 +331,15 @@ template <typename T>
 struct OpResidual {
   using Pre = PreNone;
   static constexpr int kParts = 2;
+  // partial arrays have npart entries; entries past the grid are cleared so that
+  // consumers may sum all npart of them
+  int npart;
+  __device__ void zero_tail(int nb) const {
+    for (int k = nb + threadIdx.x; k < npart; k += BLOCK) {
+      part0[k] = 0.0;
+      part1[k] = 0.0;
+    }
+  }
   const T* x;
   const T* mu;
   const T* b;
@@ -348,11 +359,13 @@ struct OpResidual {
   }
 };
 
-// beta = rho / rho_prev; u_new = c + beta u_old; q = K_II u_new; partial u_new.q
+// beta = rho / rho_prev; u_new = c + beta u_old (u_new = c in the first
+// iteration, u_old is not read); q = K_II u_new; partial u_new.q
 template <typename T>
 struct OpCgDir {
-  struct Pre { T beta; };
+  struct Pre { T beta; bool first; };
   static constexpr int kParts = 1;
+  __device__ void zero_tail(int) const {}
   const T* c;
   const T* u_old;
   const T* mu;
@@ -367,11 +380,14 @@ struct OpCgDir {
     if (sc->done) return false;
     const double rho = reduce_parts(part_rho, nrho);
     p.beta = (T)(rho / sc->rho_prev);
+    p.first = sc->iter == 0;
     return true;
   }
   using Ext = ExtNone;
   __device__ void fetch(const Pre&, int64_t, int, int, Ext&) const {}
-  __device__ T load(const Pre& p, int64_t n, int, int J) const { return J == 0 ? T(0) : c[n] + p.beta * u_old[n]; }
+  __device__ T load(const Pre& p, int64_t n, int, int J) const {
+    return J == 0 ? T(0) : (p.first ? c[n] : c[n] + p.beta * u_old[n]);
+  }
   __device__ void epilogue(const Pre&, int64_t n, int, int J, T ku, T uc, const Ext&, double& a0, double&) const {
     const T qv = J == 0 ? T(0) : ku;
     u_new[n] = uc;
@@ -386,6 +402,7 @@ template <typename T>
 struct OpL0Down {
   using Pre = PreNone;
   static constexpr int kParts = 0;
+  __device__ void zero_tail(int) const {}
   const T* r;
   const T* mu;
   const T* dinv;
@@ -422,6 +439,7 @@ template <typename T>
 struct OpL0Post {
   using Pre = PreNone;
   static constexpr int kParts = 1;
+  __device__ void zero_tail(int) const {}
   const T* x0;
   const T* r;
   const T* mu;
@@ -457,6 +475,7 @@ template <typename T>
 struct OpCheb {
   using Pre = PreNone;
   static constexpr int kParts = 1;
+  __device__ void zero_tail(int) const {}
   const T* src;
   const T* b;
   const T* mu;
@@ -553,6 +572,7 @@ struct OpCheb {
 struct OpGuessDot {
   using Pre = PreNone;
   static constexpr int kParts = 2;
+  __device__ void zero_tail(int) const {}
   const double* p;
   const double* mu;
   const double* r0;
@@ -654,6 +674,7 @@ __device__ void d_fbc_node(const FaultDev& fp, const FaultState& fs, int j, doub
 struct OpNewmark {
   using Pre = PreNone;
   static constexpr int kParts = 0;
+  __device__ void zero_tail(int) const {}
   FaultDev fp;
   FaultState fs;
   const double* mu;
@@ -1218,8 +1239,8 @@ void launch_element_ku(const Geom& g, const double* u, const double* mu, double*
 
 template <typename T>
 void launch_cg_residual(const Geom& g, const T* x, const T* mu, const T* b, T* r, double* part_rr, double* part_bb,
-                        hipStream_t s) {
-  OpResidual<T> op{x, mu, b, r, part_rr, part_bb};
+                        int npart, hipStream_t s) {
+  OpResidual<T> op{npart, x, mu, b, r, part_rr, part_bb};
   hipLaunchKernelGGL((tile_kernel<T, OpResidual<T>>), tile_grid(g), dim3(BLOCK), 0, s, g, op);
 }
 
@@ -1434,7 +1455,8 @@ void launch_scale_mask(int64_t n, int nx, const double* x, double sign, double* 
 // explicit instantiations
 #define EQ_INST(T)                                                                                                  \
   template void launch_ku<T>(const Geom&, const T*, const T*, T*, int, int, hipStream_t);                          \
-  template void launch_cg_residual<T>(const Geom&, const T*, const T*, const T*, T*, double*, double*, hipStream_t); \
+  template void launch_cg_residual<T>(const Geom&, const T*, const T*, const T*, T*, double*, double*, int,         \
+                                      hipStream_t);                                                                 \
   template void launch_cg_dir<T>(const Geom&, const T*, const T*, const T*, T*, T*, const double*, int,            \
                                  const CgScal*, double*, hipStream_t);                                              \
   template void launch_cg_update<T>(int64_t, T*, T*, const T*, const T*, const double*, int, const double*, int,    \
