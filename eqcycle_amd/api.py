@@ -217,7 +217,8 @@ class EqContext:
     # ---- time loop ------------------------------------------------------------
     def set_solver(self, tol=1e-6, maxiter=0, pc="amg", prec="fp64", guess="previous"):
         o = L.SolverOpts(tol, maxiter, PC[pc], PREC[prec],
-                         {"previous": L.EQ_GUESS_PREVIOUS, "learned": L.EQ_GUESS_LEARNED}[guess], 1)
+                         {"previous": L.EQ_GUESS_PREVIOUS, "learned": L.EQ_GUESS_LEARNED,
+                          "galerkin": L.EQ_GUESS_GALERKIN}[guess], 1)
         self._check(self.lib.eq_set_solver(self.h, C.byref(o)))
 
     def state_init(self, v_init=None, psi_init=None):
@@ -359,6 +360,13 @@ class EqContext:
         dF = f64(dF)
         out = np.empty(self.nbasis)
         self._check(self.lib.eq_learn_predict(self.h, ptr(dF), ptr(out)))
+        return out
+
+    def learn_galerkin(self, x, b):
+        """x + Phi (Phi^T K_II Phi)^-1 Phi^T (b - K_II x); vectors in FltNI order."""
+        x, b = f64(x), f64(b)
+        out = np.empty(self.nI)
+        self._check(self.lib.eq_learn_galerkin(self.h, ptr(x), ptr(b), ptr(out)))
         return out
 
     def time_kernel(self, which="ku", prec="fp64", reps=50):

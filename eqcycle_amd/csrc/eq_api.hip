@@ -1162,9 +1162,12 @@ extern "C" int eq_set_solver(eq_ctx* ctx, const eq_solver_opts* o) {
   if (!ctx || !o) return EQ_EINVAL;
   if (o->pc < EQ_PC_NONE || o->pc > EQ_PC_AMG) return fail(ctx, EQ_EINVAL, "bad pc");
   if (o->prec < EQ_PREC_FP64 || o->prec > EQ_PREC_F32PC) return fail(ctx, EQ_EINVAL, "bad prec");
-  if (o->guess != EQ_GUESS_PREVIOUS && o->guess != EQ_GUESS_LEARNED) return fail(ctx, EQ_EINVAL, "bad guess");
+  if (o->guess != EQ_GUESS_PREVIOUS && o->guess != EQ_GUESS_LEARNED && o->guess != EQ_GUESS_GALERKIN)
+    return fail(ctx, EQ_EINVAL, "bad guess");
   if (o->guess == EQ_GUESS_LEARNED && !ctx->learn_on)
     return fail(ctx, EQ_ENOTREADY, "learned guess: basis and network not trained (eq_learn_train)");
+  if (o->guess == EQ_GUESS_GALERKIN && !ctx->learn.gal_ready)
+    return fail(ctx, EQ_ENOTREADY, "Galerkin guess: no POD basis (eq_learn_set_basis)");
   ctx->opts = *o;
   return EQ_OK;
 }
@@ -1299,6 +1302,16 @@ static void learned_correction(eq_ctx* ctx, double* x, const double* b) {
   ctx->n_learned_solves++;
 }
 
+// Galerkin start for K_II x = b on span(Phi): r = b - K_II x, x += Phi c with
+// c = (Phi^T K_II Phi)^-1 Phi^T r, the K_II-norm best correction in span(Phi).
+static void galerkin_correction(eq_ctx* ctx, double* x, const double* b) {
+  CgWork<double>& w = ctx->w64;
+  launch_cg_residual<double>(ctx->g, x, ctx->mu, b, w.r, ctx->part_rr, ctx->part_bb, ctx->npart, ctx->s);
+  learn_project(&ctx->learn, w.r, 1, ctx->s);
+  learn_expand_add(&ctx->learn, x, ctx->s);
+  ctx->n_learned_solves++;
+}
+
 static int qs_step(eq_ctx* ctx, eq_step_info* info) {
   const Geom& g = ctx->g;
   hipStream_t s = ctx->s;
@@ -1307,6 +1320,7 @@ static int qs_step(eq_ctx* ctx, eq_step_info* info) {
     launch_qs_begin(g, ctx->fp, ctx->fs, ctx->mu, ctx->d, ctx->dPre, ctx->v, ctx->bq, dt, pass == 0, s);
     const bool learned = pass == 0 && ctx->opts.guess == EQ_GUESS_LEARNED && ctx->learn_on;
     if (learned) learned_correction(ctx, ctx->d, ctx->bq);
+    if (ctx->opts.guess == EQ_GUESS_GALERKIN && ctx->learn.gal_ready) galerkin_correction(ctx, ctx->d, ctx->bq);
     int64_t it = 0;
     double rr = 0;
     ctx->cg_slot = pass;
@@ -1768,13 +1782,74 @@ extern "C" int eq_learn_gram(eq_ctx* ctx, double* G) {
   return EQ_OK;
 }
 
+// G = Phi^T K_II Phi (one K_II product per basis vector), symmetrised, then
+// inverted by Cholesky on the host for the Galerkin start.
+static int galerkin_setup(eq_ctx* ctx) {
+  LearnState& L = ctx->learn;
+  const int NB = L.d.NB;
+  std::vector<double> G((size_t)NB * NB), Ginv((size_t)NB * NB, 0.0);
+  for (int k = 0; k < NB; k++) {
+    learn_basis_column(&L, k, ctx->gp, ctx->s);
+    launch_ku<double>(ctx->g, ctx->gp, ctx->mu, ctx->gq, 1, 0, ctx->s);
+    learn_project(&L, ctx->gq, 0, ctx->s);
+    if (learn_get_proj(&L, G.data() + (size_t)k * NB, ctx->s)) return fail(ctx, EQ_HIP_ERR, "Galerkin set-up");
+  }
+  for (int i = 0; i < NB; i++)
+    for (int j = 0; j < i; j++) {
+      const double a = 0.5 * (G[(size_t)i * NB + j] + G[(size_t)j * NB + i]);
+      G[(size_t)i * NB + j] = G[(size_t)j * NB + i] = a;
+    }
+  // Cholesky G = C C^T (lower C in place), then Ginv column by column
+  std::vector<double> C = G;
+  for (int j = 0; j < NB; j++) {
+    double d = C[(size_t)j * NB + j];
+    for (int k = 0; k < j; k++) d -= C[(size_t)j * NB + k] * C[(size_t)j * NB + k];
+    if (!(d > 0.0)) return fail(ctx, EQ_EINVAL, "Phi^T K_II Phi is not positive definite");
+    d = std::sqrt(d);
+    C[(size_t)j * NB + j] = d;
+    for (int i = j + 1; i < NB; i++) {
+      double a = C[(size_t)i * NB + j];
+      for (int k = 0; k < j; k++) a -= C[(size_t)i * NB + k] * C[(size_t)j * NB + k];
+      C[(size_t)i * NB + j] = a / d;
+    }
+  }
+  std::vector<double> y(NB);
+  for (int col = 0; col < NB; col++) {
+    for (int i = 0; i < NB; i++) {   // C y = e_col
+      double a = (i == col) ? 1.0 : 0.0;
+      for (int k = 0; k < i; k++) a -= C[(size_t)i * NB + k] * y[k];
+      y[i] = a / C[(size_t)i * NB + i];
+    }
+    for (int i = NB - 1; i >= 0; i--) {   // C^T z = y
+      double a = y[i];
+      for (int k = i + 1; k < NB; k++) a -= C[(size_t)k * NB + i] * Ginv[(size_t)k * NB + col];
+      Ginv[(size_t)i * NB + col] = a / C[(size_t)i * NB + i];
+    }
+  }
+  std::string err;
+  if (learn_set_ginv(&L, Ginv.data(), ctx->s, &err)) return fail(ctx, EQ_HIP_ERR, err);
+  return EQ_OK;
+}
+
 extern "C" int eq_learn_set_basis(eq_ctx* ctx, const double* V, const double* sigma, const int32_t* use) {
   if (!ctx || !V || !sigma) return EQ_EINVAL;
   LEARN_READY(ctx);
   std::string err;
   if (int rc = learn_set_basis(&ctx->learn, V, sigma, use, ctx->s, &err))
     return fail(ctx, rc == 7 ? EQ_NOMEM : EQ_EINVAL, err);
-  return EQ_OK;
+  return galerkin_setup(ctx);
+}
+
+extern "C" int eq_learn_galerkin(eq_ctx* ctx, const double* x, const double* b, double* x_out) {
+  if (!ctx || !x || !b || !x_out) return EQ_EINVAL;
+  LEARN_READY(ctx);
+  if (!ctx->learn.gal_ready) return fail(ctx, EQ_ENOTREADY, "eq_learn_set_basis first");
+  double* dx = ctx->gp;
+  double* db = ctx->gq;
+  if (int rc = upload_ni(ctx, x, dx)) return rc;
+  if (int rc = upload_ni(ctx, b, db)) return rc;
+  galerkin_correction(ctx, dx, db);
+  return download_ni(ctx, dx, x_out);
 }
 
 extern "C" int eq_learn_norms(eq_ctx* ctx, double* out) {

@@ -56,6 +56,13 @@ struct LearnState {
   float* coef = nullptr;     // [NB + 1]: coefficients, last = input scale
   double* norms = nullptr;   // [2 x cap]: |dF|, |F| per snapshot
   std::vector<int32_t> used; // snapshots used for the dataset
+  // Galerkin start on span(Phi): c = (Phi^T K_II Phi)^-1 Phi^T r, x += Phi c
+  double* gal_part = nullptr;   // [gal_nblk x NB] partial projections
+  double* gal_c = nullptr;      // [NB] projections / coefficients
+  double* gal_Ginv = nullptr;   // [NB x NB] row-major inverse of Phi^T K_II Phi
+  float* Phi_rm = nullptr;      // [nI x NB] row-major copy of Phi (projection)
+  int gal_nblk = 0;
+  int gal_ready = 0;
 };
 
 int learn_init(LearnState* ls, int64_t nI, int64_t nflt, int64_t offset, int cap, int C, int H, int NB,
@@ -81,5 +88,16 @@ int learn_get_basis(LearnState* ls, float* Phi_host, hipStream_t s);
 int learn_get_params(LearnState* ls, float* out, hipStream_t s);
 // coefficients for one input (test entry point)
 int learn_predict(LearnState* ls, const double* dF_host, double* coef_host, hipStream_t s, std::string* err);
+
+// Galerkin start.  learn_project: gal_c = Phi^T r[off:] (solve = 0) or
+// gal_c = Ginv Phi^T r[off:] (solve = 1), FP64 accumulation in a fixed order.
+void learn_project(LearnState* ls, const double* r, int solve, hipStream_t s);
+// x[off:] += Phi gal_c
+void learn_expand_add(LearnState* ls, double* x, hipStream_t s);
+// out = column k of Phi as an N-vector (out[0:off] = 0)
+void learn_basis_column(LearnState* ls, int k, double* out, hipStream_t s);
+int learn_get_proj(LearnState* ls, double* host, hipStream_t s);
+// upload Ginv (row-major NB x NB) and mark the Galerkin start ready
+int learn_set_ginv(LearnState* ls, const double* Ginv_host, hipStream_t s, std::string* err);
 
 }  // namespace eq

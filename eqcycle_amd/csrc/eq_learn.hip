@@ -350,6 +350,92 @@ __global__ __launch_bounds__(LB) void guess_kernel(int64_t nI, int64_t off, int 
   p[off + i] = (double)acc;
 }
 
+// ---- Galerkin start on span(Phi) --------------------------------------------
+// Projection s = Phi^T r reads the row-major copy Phi_rm [nI x NB]: lane k of
+// a wave owns column k, so a row is one coalesced 256-byte load and no
+// cross-lane reduction is needed; r[off + i] is wave-uniform.
+constexpr int PROJ_RPW = 64;                  // rows per wave (many waves: bytes in flight)
+constexpr int PROJ_ROWS = 4 * PROJ_RPW;       // rows per block (4 waves)
+static_assert(LB == 4 * NBMAX, "projection kernels: four waves of NBMAX lanes");
+
+__global__ __launch_bounds__(LB) void proj_dot_kernel(int64_t nI, int64_t off, int NB, const float* Phi_rm,
+                                                      const double* r, double* part) {
+  __shared__ double sw[4][NBMAX];
+  __shared__ double sr[4][PROJ_RPW];
+  const int w = threadIdx.x >> 6, k = threadIdx.x & 63;
+  const int64_t i0 = (int64_t)blockIdx.x * PROJ_ROWS + (int64_t)w * PROJ_RPW;
+  // the wave's 64 values of r, one coalesced load, then LDS broadcasts
+  sr[w][k] = (i0 + k < nI) ? r[off + i0 + k] : 0.0;
+  __syncthreads();
+  double acc = 0.0;
+  if (k < NB) {
+    if (i0 + PROJ_RPW <= nI) {
+#pragma unroll 16
+      for (int u = 0; u < PROJ_RPW; u++) acc += (double)Phi_rm[(i0 + u) * NB + k] * sr[w][u];
+    } else {
+      for (int u = 0; i0 + u < nI; u++) acc += (double)Phi_rm[(i0 + u) * NB + k] * sr[w][u];
+    }
+  }
+  sw[w][k] = acc;
+  __syncthreads();
+  if (threadIdx.x < NB)
+    part[(int64_t)blockIdx.x * NB + threadIdx.x] =
+        (sw[0][threadIdx.x] + sw[1][threadIdx.x]) + (sw[2][threadIdx.x] + sw[3][threadIdx.x]);
+}
+
+// s[k] = sum_b part[b, k], one block per column, fixed order
+__global__ __launch_bounds__(LB) void proj_sum_kernel(int nblk, int NB, const double* part, double* s) {
+  const int k = blockIdx.x;
+  double a = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += LB) a += part[(int64_t)b * NB + k];
+  a = lblock_sum(a);
+  if (threadIdx.x == 0) s[k] = a;
+}
+
+// c = Ginv s (solve) or c = s
+__global__ __launch_bounds__(LB) void proj_solve_kernel(int NB, const double* sv, const double* Ginv, int solve,
+                                                        double* c) {
+  const int t = threadIdx.x;
+  if (t >= NB) return;
+  if (!solve) {
+    c[t] = sv[t];
+    return;
+  }
+  double a = 0.0;
+#pragma unroll 16
+  for (int j = 0; j < NB; j++) a += Ginv[t * NB + j] * sv[j];
+  c[t] = a;
+}
+
+// Phi_rm[i, k] = Phi[k, i]
+__global__ void phi_rowmajor_kernel(int64_t nI, int NB, const float* Phi, float* Phi_rm) {
+  const int64_t e = (int64_t)blockIdx.x * LB + threadIdx.x;
+  if (e >= nI * NB) return;
+  const int64_t i = e / NB;
+  const int k = (int)(e - i * NB);
+  Phi_rm[e] = Phi[(int64_t)k * nI + i];
+}
+
+// x[off + i] += sum_k Phi[k, i] c[k]
+__global__ __launch_bounds__(LB) void expand_add_kernel(int64_t nI, int64_t off, int NB, const float* Phi,
+                                                        const double* c, double* x) {
+  __shared__ double cs[NBMAX];
+  if (threadIdx.x < NB) cs[threadIdx.x] = c[threadIdx.x];
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * LB + threadIdx.x;
+  if (i >= nI) return;
+  double acc = 0.0;
+#pragma unroll 16
+  for (int k = 0; k < NB; k++) acc += (double)Phi[(int64_t)k * nI + i] * cs[k];
+  x[off + i] += acc;
+}
+
+__global__ void basis_col_kernel(int64_t nI, int64_t off, const float* col, double* out) {
+  const int64_t i = (int64_t)blockIdx.x * LB + threadIdx.x;
+  if (i < off) out[i] = 0.0;
+  if (i < nI) out[off + i] = (double)col[i];
+}
+
 template <typename T>
 T* dalloc(int64_t n) {
   void* p = nullptr;
@@ -405,7 +491,12 @@ int learn_init(LearnState* ls, int64_t nI, int64_t nflt, int64_t offset, int cap
   ls->inf_act = dalloc<float>(2 * (int64_t)std::max<int64_t>(nflt, (int64_t)C * d.L[1]) + 2 * H);
   ls->coef = dalloc<float>(NBMAX + 1);
   ls->norms = dalloc<double>(2 * (int64_t)cap);
-  if (!ls->norms || !ls->S || !ls->F || !ls->Phi || !ls->X || !ls->Y || !ls->w32 || !ls->w16 || !ls->grad || !ls->adam_m ||
+  ls->gal_nblk = (int)((nI + PROJ_ROWS - 1) / PROJ_ROWS);
+  ls->gal_part = dalloc<double>((int64_t)ls->gal_nblk * NB);
+  ls->gal_c = dalloc<double>(2 * NBMAX);   // [0, NBMAX): coefficients, [NBMAX, 2 NBMAX): projections
+  ls->Phi_rm = dalloc<float>(nI * (int64_t)NB);
+  ls->gal_Ginv = dalloc<double>((int64_t)NB * NB);
+  if (!ls->gal_part || !ls->gal_c || !ls->gal_Ginv || !ls->Phi_rm || !ls->norms || !ls->S || !ls->F || !ls->Phi || !ls->X || !ls->Y || !ls->w32 || !ls->w16 || !ls->grad || !ls->adam_m ||
       !ls->adam_v || !ls->inf_act || !ls->coef) {
     *err = "device allocation failed (learned guess)";
     learn_free(ls);
@@ -432,12 +523,14 @@ int learn_init(LearnState* ls, int64_t nI, int64_t nflt, int64_t offset, int cap
   ls->nsnap = 0;
   ls->trained = 0;
   ls->nbasis_ready = 0;
+  ls->gal_ready = 0;
   return hipDeviceSynchronize() == hipSuccess ? 0 : 2;
 }
 
 void learn_free(LearnState* ls) {
   void* ptrs[] = {ls->S, ls->F, ls->Phi, ls->X, ls->Y, ls->w32, ls->w16, ls->grad, ls->adam_m, ls->adam_v,
-                  ls->act, ls->dact, ls->out, ls->idx, ls->loss_part, ls->inf_act, ls->coef, ls->norms};
+                  ls->act, ls->dact, ls->out, ls->idx, ls->loss_part, ls->inf_act, ls->coef, ls->norms,
+                  ls->gal_part, ls->gal_c, ls->gal_Ginv, ls->Phi_rm};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   NetDims d = ls->d;
@@ -479,6 +572,7 @@ int learn_gram(LearnState* ls, double* G_host, hipStream_t s, std::string* err) 
 int learn_set_basis(LearnState* ls, const double* V, const double* sigma, const int32_t* use, hipStream_t s,
                     std::string* err) {
   const int m = ls->nsnap, NB = ls->d.NB, L0 = ls->d.L0;
+  ls->gal_ready = 0;
   ls->used.clear();
   for (int j = 0; j < m; j++)
     if (!use || use[j]) ls->used.push_back(j);
@@ -491,6 +585,7 @@ int learn_set_basis(LearnState* ls, const double* V, const double* sigma, const 
   LCHK(hipMemcpyAsync(dW, W.data(), sizeof(float) * W.size(), hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(basis_kernel, dim3((unsigned)((ls->nI + 255) / 256)), dim3(256), 0, s, ls->S, ls->nI, m, dW, NB,
                      ls->Phi);
+  hipLaunchKernelGGL(phi_rowmajor_kernel, dim3(gblk(ls->nI * NB)), dim3(LB), 0, s, ls->nI, NB, ls->Phi, ls->Phi_rm);
   // dataset: X_j = dF_j / |dF_j|, Y_jk = sigma_k V_jk / |dF_j|
   std::vector<float> F((int64_t)m * L0), X((int64_t)m * L0), Y((int64_t)m * NB);
   LCHK(hipMemcpyAsync(F.data(), ls->F, sizeof(float) * F.size(), hipMemcpyDeviceToHost, s));
@@ -701,6 +796,37 @@ int learn_predict(LearnState* ls, const double* dF_host, double* coef_host, hipS
   (void)hipFree(z);
   if (e != hipSuccess) { *err = hipGetErrorString(e); return 2; }
   for (int k = 0; k < ls->d.NB; k++) coef_host[k] = c[k];
+  return 0;
+}
+
+void learn_project(LearnState* ls, const double* r, int solve, hipStream_t s) {
+  hipLaunchKernelGGL(proj_dot_kernel, dim3(ls->gal_nblk), dim3(LB), 0, s, ls->nI, ls->offset, ls->d.NB, ls->Phi_rm,
+                     r, ls->gal_part);
+  hipLaunchKernelGGL(proj_sum_kernel, dim3(ls->d.NB), dim3(LB), 0, s, ls->gal_nblk, ls->d.NB, ls->gal_part,
+                     ls->gal_c + NBMAX);
+  hipLaunchKernelGGL(proj_solve_kernel, dim3(1), dim3(LB), 0, s, ls->d.NB, ls->gal_c + NBMAX, ls->gal_Ginv, solve,
+                     ls->gal_c);
+}
+
+void learn_expand_add(LearnState* ls, double* x, hipStream_t s) {
+  hipLaunchKernelGGL(expand_add_kernel, dim3(gblk(ls->nI)), dim3(LB), 0, s, ls->nI, ls->offset, ls->d.NB, ls->Phi,
+                     ls->gal_c, x);
+}
+
+void learn_basis_column(LearnState* ls, int k, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(basis_col_kernel, dim3(gblk(std::max(ls->nI, ls->offset))), dim3(LB), 0, s, ls->nI, ls->offset,
+                     ls->Phi + (int64_t)k * ls->nI, out);
+}
+
+int learn_get_proj(LearnState* ls, double* host, hipStream_t s) {
+  if (hipMemcpyAsync(host, ls->gal_c, sizeof(double) * ls->d.NB, hipMemcpyDeviceToHost, s) != hipSuccess) return 2;
+  return hipStreamSynchronize(s) == hipSuccess ? 0 : 2;
+}
+
+int learn_set_ginv(LearnState* ls, const double* Ginv_host, hipStream_t s, std::string* err) {
+  LCHK(hipMemcpyAsync(ls->gal_Ginv, Ginv_host, sizeof(double) * ls->d.NB * ls->d.NB, hipMemcpyHostToDevice, s));
+  LCHK(hipStreamSynchronize(s));
+  ls->gal_ready = 1;
   return 0;
 }
 

@@ -21,7 +21,7 @@ EQ_MASK_ALL, EQ_MASK_INTERIOR, EQ_MASK_NEG = 0, 1, 2
 EQ_PREC_FP64, EQ_PREC_FP32, EQ_PREC_MIXED, EQ_PREC_F32PC = 0, 1, 2, 3
 EQ_PC_NONE, EQ_PC_JACOBI, EQ_PC_AMG = 0, 1, 2
 EQ_SMOOTH_JACOBI, EQ_SMOOTH_CHEBYSHEV = 0, 1
-EQ_GUESS_PREVIOUS, EQ_GUESS_LEARNED = 0, 1
+EQ_GUESS_PREVIOUS, EQ_GUESS_LEARNED, EQ_GUESS_GALERKIN = 0, 1, 2
 
 # every symbol the header declares (checked by the CPU test-suite)
 EXPORTS = [
@@ -31,7 +31,7 @@ EXPORTS = [
     "eq_slr", "eq_dtevol", "eq_set_solver", "eq_state_init", "eq_state_get", "eq_state_set",
     "eq_step", "eq_run", "eq_time_kernel", "eq_learn_init", "eq_learn_record", "eq_learn_info",
     "eq_learn_gram", "eq_learn_set_basis", "eq_learn_train", "eq_learn_loss_grad", "eq_learn_get",
-    "eq_learn_predict", "eq_learn_norms",
+    "eq_learn_predict", "eq_learn_norms", "eq_learn_galerkin",
 ]
 EQ_TIME_KU, EQ_TIME_CG_DIR, EQ_TIME_VCYCLE, EQ_TIME_CG_ITER, EQ_TIME_NEWMARK, EQ_TIME_CHEB, EQ_TIME_COPY = range(7)
 
@@ -132,6 +132,7 @@ def load():
         "eq_learn_loss_grad": ([vp, fp, C.c_int, i32p, C.c_int, dp, fp], C.c_int),
         "eq_learn_get": ([vp, fp, fp, fp, fp], C.c_int),
         "eq_learn_predict": ([vp, dp, dp], C.c_int),
+        "eq_learn_galerkin": ([vp, dp, dp, dp], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

@@ -38,7 +38,7 @@ enum { EQ_MASK_ALL = 0, EQ_MASK_INTERIOR = 1, EQ_MASK_NEG = 2 };
 enum { EQ_PREC_FP64 = 0, EQ_PREC_FP32 = 1, EQ_PREC_MIXED = 2, EQ_PREC_F32PC = 3 };
 enum { EQ_PC_NONE = 0, EQ_PC_JACOBI = 1, EQ_PC_AMG = 2 };
 enum { EQ_SMOOTH_JACOBI = 0, EQ_SMOOTH_CHEBYSHEV = 1 };
-enum { EQ_GUESS_PREVIOUS = 0, EQ_GUESS_LEARNED = 1 };
+enum { EQ_GUESS_PREVIOUS = 0, EQ_GUESS_LEARNED = 1, EQ_GUESS_GALERKIN = 2 };
 
 typedef struct eq_ctx eq_ctx;
 
@@ -156,7 +156,7 @@ typedef struct {
   int64_t maxiter;         /* 0: nI, as cg! */
   int pc;                  /* EQ_PC_* */
   int prec;                /* EQ_PREC_FP64 or EQ_PREC_MIXED */
-  int guess;               /* EQ_GUESS_PREVIOUS (src/main.jl:241) or EQ_GUESS_LEARNED */
+  int guess;               /* EQ_GUESS_PREVIOUS (src/main.jl:241), _LEARNED or _GALERKIN */
   int record_snapshots;    /* copy fault fields at events (src/main.jl:361-422) */
 } eq_solver_opts;
 int eq_set_solver(eq_ctx *ctx, const eq_solver_opts *opts);
@@ -236,6 +236,11 @@ int eq_learn_loss_grad(eq_ctx *ctx, const float *params, int nsamp, const int32_
 int eq_learn_get(eq_ctx *ctx, float *params, float *Phi, float *X, float *Y);
 /* Network output (POD coefficients times |dF|) for one fault increment. */
 int eq_learn_predict(eq_ctx *ctx, const double *dF, double *coef);
+/* Galerkin start on the POD basis (guess EQ_GUESS_GALERKIN, both quasi-static
+ * passes): x_out = x + Phi c with c = (Phi^T K_II Phi)^-1 Phi^T (b - K_II x),
+ * the K_II-norm best correction of x within span(Phi).  Phi^T K_II Phi is
+ * formed and inverted by eq_learn_set_basis.  x, b, x_out in FltNI order. */
+int eq_learn_galerkin(eq_ctx *ctx, const double *x, const double *b, double *x_out);
 
 /* ---- measurement hook (no reference counterpart) ------------------------ */
 enum { EQ_TIME_KU = 0, EQ_TIME_CG_DIR = 1, EQ_TIME_VCYCLE = 2, EQ_TIME_CG_ITER = 3, EQ_TIME_NEWMARK = 4,

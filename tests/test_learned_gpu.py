@@ -77,3 +77,44 @@ def test_training_and_learned_guess_gain(trained):
     print(f"[gpu] pass-1 iterations plain {sum(plain)} learned {sum(lrn)}", flush=True)
     assert sum(lrn) < sum(plain)
     assert np.max(np.abs(d_lrn - d_plain)) <= 1e-7 * np.max(np.abs(d_plain))
+
+
+def test_galerkin_start_is_energy_optimal(trained, oracle):
+    """x_out = x + Phi (Phi^T K_II Phi)^-1 Phi^T (b - K_II x): applying it again
+    changes nothing (Phi^T r_out = 0), and the K_II-norm error never grows."""
+    P, ctx, _ = trained
+    K = oracle.assemble_K(P.NelX, P.NelY, P.nglob, P.iglob, P.W, P.H, P.dxe, P.dye)
+    kni = K.submatrix(P.FltNI - 1, -1.0)          # kni = -K_II (src/main.jl:195)
+    rng = np.random.default_rng(5)
+    n = ctx.nI
+    xs = rng.standard_normal(n)
+    b = -kni.matvec(xs)                           # K_II xs = b
+    x = xs + 0.1 * rng.standard_normal(n)
+    x1 = ctx.learn_galerkin(x, b)
+    x2 = ctx.learn_galerkin(x1, b)
+    # rounding of the residual is amplified by cond(Phi^T K_II Phi); a wrong
+    # inverse would leave a correction of the size of the first one
+    assert np.max(np.abs(x2 - x1)) <= 1e-6 * np.max(np.abs(x1 - x))
+
+    def enorm(e):
+        return float(np.sqrt(-e @ kni.matvec(e)))
+    assert enorm(x1 - xs) <= enorm(x - xs) * (1 + 1e-12)
+
+
+def test_galerkin_guess_gain(trained):
+    """Same 40 quasi-static steps with the Galerkin start on the POD basis
+    (both passes) against the previous-step guess: fewer CG iterations, same
+    displacements to the solver tolerance."""
+    P, ctx, _ = trained
+    s0 = ctx.state_get()
+    ctx.set_solver(tol=1e-10, pc="amg", guess="previous")
+    plain = [sum(ctx.step().pcg_iters) for _ in range(40)]
+    d_plain = ctx.state_get()["d"]
+    ctx.state_set(d=s0["d"], v=s0["v"], a=s0["a"], psi=s0["psi"], tau=s0["tau"], t=s0["t"], dt=s0["dt"],
+                  isolver=s0["isolver"])
+    ctx.set_solver(tol=1e-10, pc="amg", guess="galerkin")
+    gal = [sum(ctx.step().pcg_iters) for _ in range(40)]
+    d_gal = ctx.state_get()["d"]
+    print(f"[gpu] CG iterations (both passes) plain {sum(plain)} galerkin {sum(gal)}", flush=True)
+    assert sum(gal) < sum(plain)
+    assert np.max(np.abs(d_gal - d_plain)) <= 1e-7 * np.max(np.abs(d_plain))
