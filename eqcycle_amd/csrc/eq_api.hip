@@ -97,6 +97,7 @@ struct eq_ctx {
   bool have_amg = false;
   std::vector<DevLevel> lev;          // lev[0]: matrix-free fine level (A unused)
   double* coarse_inv = nullptr;
+  float* coarse_inv32 = nullptr;      // FP32 copy for the FP32 V-cycle
   int64_t ncoarse = 0;
   std::vector<int64_t> amg_n, amg_nnz;
   int pmg = 0;                        // level-0 prolongation is the bilinear vertex interpolation
@@ -627,6 +628,9 @@ extern "C" int eq_amg_setup(eq_ctx* ctx, double theta, int max_levels, int64_t m
   ctx->coarse_inv = ctx->alloc<double>(H.ncoarse * H.ncoarse);
   NEED(ctx->coarse_inv);
   if (int rc = h2d(ctx, ctx->coarse_inv, H.coarse_inv.data(), H.ncoarse * H.ncoarse)) return rc;
+  ctx->coarse_inv32 = ctx->alloc<float>(H.ncoarse * H.ncoarse);
+  NEED(ctx->coarse_inv32);
+  launch_convert<double, float>(H.ncoarse * H.ncoarse, ctx->coarse_inv, ctx->coarse_inv32, ctx->s);
   ctx->pmg = pcoarsen ? 1 : 0;
   ctx->have_amg = true;
   return sync_check(ctx);
@@ -672,6 +676,12 @@ template <>
 const double* dinv_of<double>(eq_ctx* c) { return c->dinv; }
 template <>
 const float* dinv_of<float>(eq_ctx* c) { return c->dinv32; }
+template <typename T>
+static const T* coarse_inv_of(eq_ctx* c);
+template <>
+const double* coarse_inv_of<double>(eq_ctx* c) { return c->coarse_inv; }
+template <>
+const float* coarse_inv_of<float>(eq_ctx* c) { return c->coarse_inv32; }
 template <typename T>
 static const T* ones_of(eq_ctx* c);
 template <>
@@ -787,7 +797,7 @@ static void vcycle(eq_ctx* ctx, const T* r, T* c) {
     launch_csr<T>(CSR_SPMV, D.R, v.res, nullptr, nullptr, T(0), lvecs<T>(ctx->lev[l + 1]).b, nullptr, ctx->sc, s);
   }
   DevLevelVecs<T>& vc = lvecs<T>(ctx->lev[L - 1]);
-  launch_dense<T>(ctx->ncoarse, ctx->coarse_inv, vc.b, vc.xp, ctx->sc, s);
+  launch_dense<T>(ctx->ncoarse, coarse_inv_of<T>(ctx), vc.b, vc.xp, ctx->sc, s);
   for (int l = L - 2; l >= 1; l--) {
     DevLevel& D = ctx->lev[l];
     DevLevelVecs<T>& v = lvecs<T>(D);

@@ -144,7 +144,7 @@ template <typename T>
 void launch_csr(CsrMode mode, const DevCsr& M, const T* x, const T* b, const T* dinv, T omega, T* y, T* xout,
                 const CgScal* sc, hipStream_t s);
 template <typename T>
-void launch_dense(int64_t n, const double* inv, const T* b, T* x, const CgScal* sc, hipStream_t s);
+void launch_dense(int64_t n, const T* inv, const T* b, T* x, const CgScal* sc, hipStream_t s);
 
 
 

@@ -865,15 +865,15 @@ __global__ __launch_bounds__(BLOCK) void csr_kernel(int64_t n, const int32_t* __
 
 // x = A_c^-1 b, one wavefront per row of the row-major inverse
 template <typename T>
-__global__ __launch_bounds__(BLOCK) void dense_kernel(int64_t n, const double* inv, const T* b, T* x,
+__global__ __launch_bounds__(BLOCK) void dense_kernel(int64_t n, const T* inv, const T* b, T* x,
                                                       const CgScal* sc) {
   if (sc && sc->done) return;
   const int64_t row = (int64_t)blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   double s = 0.0;
   if (row < n) {
-    const double* a = inv + row * n;
-    for (int64_t j = lane; j < n; j += 64) s += a[j] * (double)b[j];
+    const T* a = inv + row * n;
+    for (int64_t j = lane; j < n; j += 64) s += (double)a[j] * (double)b[j];
   }
   s = wave_sum(s);
   if (row < n && lane == 0) x[row] = (T)s;
@@ -1350,7 +1350,7 @@ void launch_csr(CsrMode mode, const DevCsr& M, const T* x, const T* b, const T* 
 }
 
 template <typename T>
-void launch_dense(int64_t n, const double* inv, const T* b, T* x, const CgScal* sc, hipStream_t s) {
+void launch_dense(int64_t n, const T* inv, const T* b, T* x, const CgScal* sc, hipStream_t s) {
   const int64_t blocks = (n + BLOCK / 64 - 1) / (BLOCK / 64);
   hipLaunchKernelGGL((dense_kernel<T>), dim3((unsigned)blocks), dim3(BLOCK), 0, s, n, inv, b, x, sc);
 }
@@ -1469,7 +1469,7 @@ void launch_scale_mask(int64_t n, int nx, const double* x, double sign, double* 
                                   const CgScal*, hipStream_t);                                                      \
   template void launch_csr<T>(CsrMode, const DevCsr&, const T*, const T*, const T*, T, T*, T*, const CgScal*,       \
                               hipStream_t);                                                                         \
-  template void launch_dense<T>(int64_t, const double*, const T*, T*, const CgScal*, hipStream_t);                 \
+  template void launch_dense<T>(int64_t, const T*, const T*, T*, const CgScal*, hipStream_t);                      \
   template void launch_cheb<T>(const Geom&, const ChebStep<T>&, const T*, const T*, const double*, int, CgScal*,   \
                                double*, hipStream_t);                                                              \
   template void launch_fill<T>(int64_t, T*, T, hipStream_t);
