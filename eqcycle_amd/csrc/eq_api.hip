@@ -114,7 +114,12 @@ struct eq_ctx {
   double* rec_seis = nullptr;         // [kBatch x 3 x seis_cap]
   double* rec_seis_host = nullptr;    // pinned
   int64_t seis_cap = 0;
-  int* run = nullptr;                 // batch flag
+  // dynamic batches: per step (max, index) of the fault velocity per bottom
+  // tile and the FBC! flags, written by the Newmark kernels themselves
+  double* vpart = nullptr;            // [kBatch x 2 nbx]
+  double* vpart_host = nullptr;       // pinned
+  int* nrflags = nullptr;             // [kBatch x 2]
+  int* nrflags_host = nullptr;        // pinned
   int* flags = nullptr;
   bool flags_dirty = true;            // flags not known to hold (INT_MAX, 0)
   int64_t nr_noconv_total = 0;
@@ -132,6 +137,8 @@ struct eq_ctx {
     if (sc_host) (void)hipHostFree(sc_host);
     if (st_host) (void)hipHostFree(st_host);
     if (rec_seis_host) (void)hipHostFree(rec_seis_host);
+    if (vpart_host) (void)hipHostFree(vpart_host);
+    if (nrflags_host) (void)hipHostFree(nrflags_host);
     if (s) (void)hipStreamDestroy(s);
   }
   template <typename T>
@@ -283,15 +290,21 @@ extern "C" int eq_create(const eq_mesh_desc* mesh, eq_ctx** out) {
   c->sc = c->alloc<CgScal>(1);
   c->st = c->alloc<StepStatus>(eq_ctx::kBatch);
   c->rec_fault = c->alloc<double>((int64_t)eq_ctx::kBatch * 3 * c->NX);
-  c->run = c->alloc<int>(1);
+  const int nbx = (int)tile_grid(g).x;
+  c->vpart = c->alloc<double>((int64_t)eq_ctx::kBatch * 2 * nbx);
+  c->nrflags = c->alloc<int>(2 * eq_ctx::kBatch);
   c->flags = c->alloc<int>(4);
   if (!c->part_rr || !c->part_bb || !c->part_rho || !c->part_uq || !c->sc || !c->st || !c->flags || !c->rec_fault ||
-      !c->run) {
+      !c->vpart || !c->nrflags) {
     *out = ctx.release();
     return fail(*out, EQ_NOMEM, "device allocation failed");
   }
   if (hipHostMalloc((void**)&c->sc_host, sizeof(CgScal), hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc((void**)&c->st_host, sizeof(StepStatus) * eq_ctx::kBatch, hipHostMallocDefault) != hipSuccess) {
+      hipHostMalloc((void**)&c->st_host, sizeof(StepStatus) * eq_ctx::kBatch, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&c->vpart_host, sizeof(double) * eq_ctx::kBatch * 2 * nbx, hipHostMallocDefault) !=
+          hipSuccess ||
+      hipHostMalloc((void**)&c->nrflags_host, sizeof(int) * 2 * eq_ctx::kBatch, hipHostMallocDefault) !=
+          hipSuccess) {
     *out = ctx.release();
     return fail(*out, EQ_NOMEM, "pinned allocation failed");
   }
@@ -427,6 +440,7 @@ extern "C" int eq_set_params(eq_ctx* ctx, const eq_params_desc* p) {
     ctx->out_seis.push_back(ctx->perm[r]);
   }
   const int64_t nseis = (int64_t)ctx->out_seis.size();
+  if (nseis > 31) return fail(ctx, EQ_EINVAL, "at most 31 receivers (out_seis)");
   if (nseis > ctx->seis_cap) {
     ctx->seis_idx = ctx->alloc<int64_t>(nseis);
     ctx->rec_seis = ctx->alloc<double>((int64_t)eq_ctx::kBatch * 3 * nseis);
@@ -1277,14 +1291,13 @@ static int read_status(eq_ctx* ctx, int k) {
   return sync_check(ctx);
 }
 
-static StepCapture capture(eq_ctx* ctx, int slot, bool batch) {
+static StepCapture capture(eq_ctx* ctx, int slot) {
   StepCapture c;
   const int64_t ns = (int64_t)ctx->out_seis.size();
   c.fault = ctx->rec_fault + (int64_t)slot * 3 * ctx->NX;
   c.seis_idx = ctx->seis_idx;
   c.nseis = (int)ns;
   c.seis = ns > 0 ? ctx->rec_seis + (int64_t)slot * 3 * ns : nullptr;
-  c.run = batch ? ctx->run : nullptr;
   return c;
 }
 
@@ -1350,7 +1363,7 @@ static int qs_step(eq_ctx* ctx, eq_step_info* info) {
 static int dyn_step(eq_ctx* ctx) {
   const double dtmin = ctx->dt0;
   launch_newmark(ctx->g, ctx->fp, ctx->fs, ctx->mu, ctx->M, ctx->d, ctx->v, ctx->a, ctx->d2, ctx->v2, ctx->a2,
-                 ctx->dt, 0.5 * dtmin, 0.5 * dtmin * dtmin, ctx->flags, nullptr, ctx->s);
+                 ctx->dt, 0.5 * dtmin, 0.5 * dtmin * dtmin, ctx->flags, NewmarkRecord{}, ctx->s);
   std::swap(ctx->d, ctx->d2);
   std::swap(ctx->v, ctx->v2);
   std::swap(ctx->a, ctx->a2);
@@ -1371,7 +1384,7 @@ extern "C" int eq_step(eq_ctx* ctx, eq_step_info* info) {
   int rc = (ctx->isolver == 1) ? qs_step(ctx, info) : dyn_step(ctx);
   if (rc) return rc;
   launch_step_tail(ctx->fp, ctx->d, ctx->v, ctx->a, ctx->fs.tau, ctx->dt, ctx->dt0, ctx->isolver, ctx->st,
-                   ctx->flags, capture(ctx, 0, false), ctx->s);
+                   ctx->flags, capture(ctx, 0), ctx->s);
   if (int rc2 = read_status(ctx, 1)) return rc2;
   ctx->flags_dirty = false;
   const StepStatus& st = *ctx->st_host;
@@ -1395,44 +1408,86 @@ extern "C" int eq_step(eq_ctx* ctx, eq_step_info* info) {
 }
 
 // Up to kBatch dynamic steps enqueued without a host synchronisation.  Each
-// step is the Newmark tile kernel and the step tail, which writes the step's
-// record into its own slot and clears the run flag at the first step whose
-// regime switch, Newton failure or non-finite slip rate ends the batch; the
-// steps after it exit at once.  dt is dtmin for every dynamic step
-// (dtevol!, src/dtevol.jl:5-10), so the step times are known in advance.
+// step is one Newmark tile kernel that also writes the step's record (fault
+// velocity maxima per tile, FBC! flags, fault rows, receiver samples); the
+// kernel of the next step reads it and stops, in every block alike, once a
+// step has ended the dynamic regime (the step tail's switch rule), failed the
+// FBC! guard or did not run, and the steps after it do nothing.  dt is dtmin
+// for every dynamic step (dtevol!, src/dtevol.jl:5-10).  The host rebuilds
+// the status of every step from the records; the last step taken gets the
+// step tail kernel, whose status (dtevol!, regime) carries the loop on.
 // Returns the number of steps taken in *m (>= 1).
 static int dyn_batch(eq_ctx* ctx, double total_time, int64_t max_more, int* m) {
   int k = 0;
   for (double tt = ctx->t; k < eq_ctx::kBatch && k < max_more && tt < total_time; k++) tt += ctx->dt;
   if (k == 0) k = 1;
-  clean_flags(ctx);
-  ctx->flags_dirty = true;
-  launch_fill<int>(1, ctx->run, 1, ctx->s);
+  hipStream_t s = ctx->s;
+  const int nbx = (int)tile_grid(ctx->g).x;
+  const int64_t ns = (int64_t)ctx->out_seis.size();
+  const int64_t NX = ctx->NX;
+  launch_fill<int>(2 * eq_ctx::kBatch, ctx->nrflags, 0, s);
+  launch_fill_strided<int>(eq_ctx::kBatch, ctx->nrflags, 2, INT_MAX, s);
   const double dtmin = ctx->dt0;
   double* bufs[2][3] = {{ctx->d, ctx->v, ctx->a}, {ctx->d2, ctx->v2, ctx->a2}};
   for (int q = 0; q < k; q++) {
     double** src = bufs[q & 1];
     double** dst = bufs[(q + 1) & 1];
+    NewmarkRecord rec;
+    rec.vprev = q > 0 ? ctx->vpart + (int64_t)(q - 1) * 2 * nbx : nullptr;
+    rec.nrprev = q > 0 ? ctx->nrflags + 2 * (q - 1) : nullptr;
+    rec.vpart_out = ctx->vpart + (int64_t)q * 2 * nbx;
+    rec.cap_fault = ctx->rec_fault + (int64_t)q * 3 * NX;
+    rec.seis_idx = ctx->seis_idx;
+    rec.nseis = (int)ns;
+    rec.cap_seis = ns > 0 ? ctx->rec_seis + (int64_t)q * 3 * ns : nullptr;
     launch_newmark(ctx->g, ctx->fp, ctx->fs, ctx->mu, ctx->M, src[0], src[1], src[2], dst[0], dst[1], dst[2],
-                   ctx->dt, 0.5 * dtmin, 0.5 * dtmin * dtmin, ctx->flags, ctx->run, ctx->s);
-    launch_step_tail(ctx->fp, dst[0], dst[1], dst[2], ctx->fs.tau, ctx->dt, dtmin, 2, ctx->st + q, ctx->flags,
-                     capture(ctx, q, true), ctx->s);
+                   ctx->dt, 0.5 * dtmin, 0.5 * dtmin * dtmin, ctx->nrflags + 2 * q, rec, s);
   }
-  if (int rc = read_status(ctx, k)) return rc;
-  ctx->flags_dirty = false;
-  int n = k;
+  HIPCHK(ctx, hipMemcpyAsync(ctx->vpart_host, ctx->vpart, sizeof(double) * 2 * nbx * k, hipMemcpyDeviceToHost, s));
+  HIPCHK(ctx, hipMemcpyAsync(ctx->nrflags_host, ctx->nrflags, sizeof(int) * 2 * k, hipMemcpyDeviceToHost, s));
+  if (ns > 0)
+    HIPCHK(ctx, hipMemcpyAsync(ctx->rec_seis_host, ctx->rec_seis, sizeof(double) * 3 * ns * k,
+                               hipMemcpyDeviceToHost, s));
+  if (int rc = sync_check(ctx)) return rc;
+  // status of each step from its record, as the step tail computes it
+  int n = 0;
   for (int q = 0; q < k; q++) {
-    const StepStatus& st = ctx->st_host[q];
-    if (st.isolver_next != 2 || st.nr_flag != 0 || !std::isfinite(st.Vfmax)) {
-      n = q + 1;
-      break;
+    const double* vp = ctx->vpart_host + (int64_t)q * 2 * nbx;
+    double mx = -INFINITY, mi = 2147483647.0;
+    bool ran = true;
+    for (int b = 0; b < nbx; b++) {
+      if (vp[2 * b + 1] < 0.0) { ran = false; break; }
+      if (vp[2 * b] > mx || (vp[2 * b] == mx && vp[2 * b + 1] < mi)) { mx = vp[2 * b]; mi = vp[2 * b + 1]; }
     }
+    if (!ran) break;
+    StepStatus& st = ctx->st_host[q];
+    st.Vfmax = 2 * mx + ctx->Vpl;
+    st.dt_next = dtmin;
+    st.isolver_next = st.Vfmax < 2e-3 ? 1 : 2;
+    st.hypo = (int)mi;
+    st.nr_flag = ctx->nrflags_host[2 * q] == INT_MAX ? 0 : ctx->nrflags_host[2 * q];
+    st.nr_noconv = ctx->nrflags_host[2 * q + 1];
+    n = q + 1;
+    if (st.isolver_next != 2 || st.nr_flag != 0 || !std::isfinite(st.Vfmax)) break;
   }
+  if (n == 0) return fail(ctx, EQ_HIP_ERR, "dynamic batch: first step has no record");
   if (n & 1) {
     std::swap(ctx->d, ctx->d2);
     std::swap(ctx->v, ctx->v2);
     std::swap(ctx->a, ctx->a2);
   }
+  // the step tail on the last step taken: dtevol! and the regime switch
+  StepCapture none{};
+  launch_step_tail(ctx->fp, ctx->d, ctx->v, ctx->a, ctx->fs.tau, ctx->dt, dtmin, 2, ctx->st + (n - 1),
+                   ctx->nrflags + 2 * (n - 1), none, s);
+  const StepStatus mine = ctx->st_host[n - 1];
+  HIPCHK(ctx, hipMemcpyAsync(ctx->st_host + (n - 1), ctx->st + (n - 1), sizeof(StepStatus), hipMemcpyDeviceToHost,
+                             s));
+  if (int rc = sync_check(ctx)) return rc;
+  const StepStatus& tail = ctx->st_host[n - 1];
+  if (tail.isolver_next != mine.isolver_next || tail.hypo != mine.hypo || tail.nr_flag != mine.nr_flag ||
+      !(tail.Vfmax == mine.Vfmax || (std::isnan(tail.Vfmax) && std::isnan(mine.Vfmax))))
+    return fail(ctx, EQ_HIP_ERR, "dynamic batch: step record disagrees with the step tail");
   *m = n;
   return EQ_OK;
 }
@@ -1715,7 +1770,7 @@ extern "C" int eq_time_kernel(eq_ctx* ctx, int which, int prec, int reps, double
   auto body = [&]() {
     if (which == EQ_TIME_NEWMARK) {
       launch_newmark(ctx->g, ctx->fp, ctx->fs_api, ctx->mu, ctx->M, ctx->d, ctx->v, ctx->a, ctx->d2, ctx->v2,
-                     ctx->a2, ctx->dt0, 0.5 * ctx->dt0, 0.5 * ctx->dt0 * ctx->dt0, ctx->flags, nullptr, s);
+                     ctx->a2, ctx->dt0, 0.5 * ctx->dt0, 0.5 * ctx->dt0 * ctx->dt0, ctx->flags, NewmarkRecord{}, s);
     } else if (prec == EQ_PREC_FP32) {
       time_body<float>(ctx, which);
     } else {

@@ -46,15 +46,12 @@ struct StepStatus {   // written by the step tail kernel
 
 // Per-step record written by the step tail kernel besides its StepStatus:
 // fault rows of d and v and the fault traction (3 x nflt), and d, v, a at the
-// receiver nodes (3 x nseis).  run (may be null) is the batch flag: a batch of
-// dynamic steps stops, on the device, at the first step whose regime switch,
-// Newton failure or non-finite slip rate the tail detects.
+// receiver nodes (3 x nseis).
 struct StepCapture {
-  double* fault;            // [3 * nflt] or null
-  const int64_t* seis_idx;  // structured node index of each receiver
-  int nseis;
-  double* seis;             // [3 * nseis] or null
-  int* run;
+  double* fault = nullptr;            // [3 * nflt] or null
+  const int64_t* seis_idx = nullptr;  // structured node index of each receiver
+  int nseis = 0;
+  double* seis = nullptr;             // [3 * nseis] or null
 };
 
 inline dim3 tile_grid(const Geom& g) {
@@ -149,9 +146,21 @@ void launch_dense(int64_t n, const T* inv, const T* b, T* x, const CgScal* sc, h
 
 
 // ---- time stepping --------------------------------------------------------
+// Step record of a Newmark step inside a batch (see OpNewmark); all null
+// outside a batch.
+struct NewmarkRecord {
+  const double* vprev = nullptr;
+  const int* nrprev = nullptr;
+  double* vpart_out = nullptr;
+  double* cap_fault = nullptr;
+  const int64_t* seis_idx = nullptr;
+  int nseis = 0;
+  double* cap_seis = nullptr;
+};
 void launch_newmark(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, const double* M,
                     const double* d, const double* v, const double* a, double* d2, double* v2, double* a2,
-                    double dt, double half_dt, double half_dt_sq, int* flags, const int* run, hipStream_t s);
+                    double dt, double half_dt, double half_dt_sq, int* flags, const NewmarkRecord& rec,
+                    hipStream_t s);
 void launch_qs_begin(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, double* d, double* dPre,
                      const double* v, double* b, double dt, int first_pass, hipStream_t s);
 void launch_qs_fault(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, const double* d,
@@ -182,6 +191,9 @@ template <typename T, typename U>
 void launch_convert(int64_t n, const T* x, U* y, hipStream_t s);
 template <typename T>
 void launch_fill(int64_t n, T* y, T val, hipStream_t s);
+// y[k * stride] = val for k < n
+template <typename T>
+void launch_fill_strided(int64_t n, T* y, int stride, T val, hipStream_t s);
 void launch_axpy(int64_t n, double alpha, const double* x, double* y, hipStream_t s);
 // y = sign * x on N-vectors, zeroing the fault row (first NX entries)
 void launch_scale_mask(int64_t n, int nx, const double* x, double sign, double* y, hipStream_t s);

@@ -290,6 +290,7 @@ __global__ __launch_bounds__(BLOCK) void tile_kernel(Geom g, Op op) {
       }
     }
   }
+  op.block_end(acc0, acc1);
   if (Op::kParts >= 1) {
     const double s = block_sum(acc0);
     if (threadIdx.x == 0) op.part0[blockIdx.y * gridDim.x + blockIdx.x] = s;
@@ -310,6 +311,7 @@ struct OpApply {
   using Pre = PreNone;
   static constexpr int kParts = 0;
   __device__ void zero_tail(int) const {}
+  __device__ void block_end(double, double) const {}
   const T* u;
   const T* mu;
   T* y;
@@ -333,6 +335,7 @@ struct OpResidual {
   static constexpr int kParts = 2;
   // partial arrays have npart entries; entries past the grid are cleared so that
   // consumers may sum all npart of them
+  __device__ void block_end(double, double) const {}
   int npart;
   __device__ void zero_tail(int nb) const {
     for (int k = nb + threadIdx.x; k < npart; k += BLOCK) {
@@ -366,6 +369,7 @@ struct OpCgDir {
   struct Pre { T beta; bool first; };
   static constexpr int kParts = 1;
   __device__ void zero_tail(int) const {}
+  __device__ void block_end(double, double) const {}
   const T* c;
   const T* u_old;
   const T* mu;
@@ -403,6 +407,7 @@ struct OpL0Down {
   using Pre = PreNone;
   static constexpr int kParts = 0;
   __device__ void zero_tail(int) const {}
+  __device__ void block_end(double, double) const {}
   const T* r;
   const T* mu;
   const T* dinv;
@@ -440,6 +445,7 @@ struct OpL0Post {
   using Pre = PreNone;
   static constexpr int kParts = 1;
   __device__ void zero_tail(int) const {}
+  __device__ void block_end(double, double) const {}
   const T* x0;
   const T* r;
   const T* mu;
@@ -476,6 +482,7 @@ struct OpCheb {
   using Pre = PreNone;
   static constexpr int kParts = 1;
   __device__ void zero_tail(int) const {}
+  __device__ void block_end(double, double) const {}
   const T* src;
   const T* b;
   const T* mu;
@@ -573,6 +580,7 @@ struct OpGuessDot {
   using Pre = PreNone;
   static constexpr int kParts = 2;
   __device__ void zero_tail(int) const {}
+  __device__ void block_end(double, double) const {}
   const double* p;
   const double* mu;
   const double* r0;
@@ -671,8 +679,18 @@ __device__ void d_fbc_node(const FaultDev& fp, const FaultState& fs, int j, doub
 // Newmark step (src/main.jl:297-344) as a tile epilogue: predictor in the
 // window load, -K d, absorbing edges, FBC!, corrector.  Reads d, v, a and
 // writes d2, v2, a2 (double-buffered state).
+//
+// In a batch of dynamic steps the kernel also keeps the step record that the
+// step tail would write: the bottom tile row leaves the maximum of the new
+// fault velocity per tile (value, first index) in vpart_out, the fault-row
+// threads copy d, v, tau of their node into cap_fault, and the owners of the
+// receiver nodes copy d, v, a into cap_seis.  The next step's kernel reads
+// the previous record (vprev, nrprev) in prepare() and stops, in every block
+// alike, when that step ended the dynamic regime (Vfmax < 2e-3, the switch
+// rule of the step tail), failed the FBC! guard or did not run; a stopped
+// kernel marks its own record as not run.
 struct OpNewmark {
-  using Pre = PreNone;
+  struct Pre { int seis; };
   static constexpr int kParts = 0;
   __device__ void zero_tail(int) const {}
   FaultDev fp;
@@ -682,12 +700,45 @@ struct OpNewmark {
   const double *d, *v, *a;
   double *d2, *v2, *a2;
   double dt, half_dt, half_dt_sq;
-  int* flags;
-  const int* run;   // null, or batch flag: 0 = the batch has stopped, do nothing
+  int* flags;               // FBC! guard / non-convergence counters of this step
   int NY;
   double* part0;
   double* part1;
-  __device__ bool prepare(Pre&) const { return run == nullptr || *run != 0; }
+  // batch record (all null outside a batch)
+  const double* vprev;      // [2 * nbx] previous step's (max, index) per bottom tile
+  const int* nrprev;        // previous step's flags
+  double* vpart_out;        // [2 * nbx] this step's
+  double* cap_fault;        // [3 * nflt] d, v, tau of the fault row
+  const int64_t* seis_idx;
+  int nseis;
+  double* cap_seis;         // [3 * nseis]
+  int nbx;
+  __device__ bool prepare(Pre& p) const {
+    p.seis = 0;
+    if (vprev) {
+      double m = -INFINITY, mi = 2147483647.0;
+      for (int b = 0; b < nbx; b++) {
+        const double o = vprev[2 * b], oi = vprev[2 * b + 1];
+        if (oi < 0.0) { mi = -1.0; break; }    // the previous step did not run
+        if (o > m || (o == m && oi < mi)) { m = o; mi = oi; }
+      }
+      const double Vfmax = 2 * m + fp.Vpl;
+      const bool stop = mi < 0.0 || nrprev[0] != INT_MAX || !isfinite(Vfmax) || Vfmax < 2e-3;
+      if (stop) {
+        if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) vpart_out[1] = -1.0;
+        return false;
+      }
+    }
+    if (cap_seis) {
+      const int I0 = blockIdx.x * TILE_I, J0 = blockIdx.y * TILE_J;
+      for (int q = 0; q < nseis; q++) {
+        const int64_t n = seis_idx[q];
+        const int J = (int)(n / fp.nflt), I = (int)(n - (int64_t)J * fp.nflt);
+        if (I >= I0 && I < I0 + TILE_I && J >= J0 && J < J0 + TILE_J) p.seis |= 1 << q;
+      }
+    }
+    return true;
+  }
   struct Ext { double v, a, M; };
   __device__ void fetch(const Pre&, int64_t n, int, int, Ext& e) const {
     e.v = v[n];
@@ -695,8 +746,8 @@ struct OpNewmark {
     e.M = M[n];
   }
   __device__ double load(const Pre&, int64_t n, int, int) const { return d[n] + dt * v[n] + half_dt_sq * a[n]; }
-  __device__ void epilogue(const Pre&, int64_t n, int I, int J, double ku, double dn, const Ext& e, double&,
-                           double&) const {
+  __device__ void epilogue(const Pre& p, int64_t n, int I, int J, double ku, double dn, const Ext& e, double& a0,
+                           double& a1) const {
     const double vo = e.v;
     const double vp = vo + half_dt * e.a;
     double at = -ku;
@@ -705,9 +756,10 @@ struct OpNewmark {
     if (deep) at = 0.0;
     if (I == 0) at = at - fp.BcLC[J] * vp;
     if (J == NY - 1) at = at - fp.BcTC[I] * vp;
+    double tau = 0.0;
     if (fault) {
       const int j = I;
-      const double Mj = M[n];
+      const double Mj = e.M;
       const double Vfree = 2 * vp + 2 * half_dt * at / Mj;
       const double Vfj = 2 * vo + fp.Vpl;
       fs.FltVfree[j] = Vfree;
@@ -719,12 +771,12 @@ struct OpNewmark {
         psi_new = fs.psi2[j];
         tau2 = fs.tau2[j];
       }
-      const double tau = deep ? 0.0 : tau2 - fp.tauo[j];
+      tau = deep ? 0.0 : tau2 - fp.tauo[j];
       fs.tau[j] = tau;
       fs.psi[j] = psi_new;
       at = at - fp.FltB[j] * tau;
     }
-    double an = at / M[n];
+    double an = at / e.M;
     double vn = vp + half_dt * an;
     if (deep) {
       vn = 0.0;
@@ -733,6 +785,40 @@ struct OpNewmark {
     d2[n] = dn;
     v2[n] = vn;
     a2[n] = an;
+    if (fault) {
+      a0 = vn;
+      a1 = (double)I;
+      if (cap_fault) {
+        cap_fault[I] = dn;
+        cap_fault[fp.nflt + I] = vn;
+        cap_fault[2 * fp.nflt + I] = tau;
+      }
+    }
+    if (p.seis) {
+      for (int q = 0; q < nseis; q++)
+        if ((p.seis >> q & 1) && seis_idx[q] == n) {
+          cap_seis[3 * q] = dn;
+          cap_seis[3 * q + 1] = vn;
+          cap_seis[3 * q + 2] = an;
+        }
+    }
+  }
+  // bottom tile row: (max, first index) of the new fault velocity over the
+  // tile's 64 fault nodes, all held by wave 0
+  __device__ void block_end(double a0, double a1) const {
+    if (!vpart_out || blockIdx.y != 0 || threadIdx.x >= 64) return;
+    // as the step tail: NaN is never the maximum; no valid value leaves (-inf, INT_MAX)
+    const bool valid = (int)(blockIdx.x * TILE_I + threadIdx.x) < fp.nflt && !isnan(a0);
+    double m = valid ? a0 : -INFINITY, mi = valid ? a1 : 2147483647.0;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const double o = __shfl_down(m, off, 64), oi = __shfl_down(mi, off, 64);
+      if (o > m || (o == m && oi < mi)) { m = o; mi = oi; }
+    }
+    if (threadIdx.x == 0) {
+      vpart_out[2 * blockIdx.x] = m;
+      vpart_out[2 * blockIdx.x + 1] = mi;
+    }
   }
 };
 
@@ -995,7 +1081,6 @@ __global__ __launch_bounds__(BLOCK) void step_tail_kernel(FaultDev fp, const dou
                                                           const double* a, const double* tau, double dt,
                                                           double dtmin, int isolver, StepStatus* st, int* flags,
                                                           StepCapture cap) {
-  if (cap.run && *cap.run == 0) return;   // block-uniform: the batch stopped earlier
   __shared__ double smax[BLOCK];
   __shared__ int sidx[BLOCK];
   double m = -INFINITY;
@@ -1046,7 +1131,6 @@ __global__ __launch_bounds__(BLOCK) void step_tail_kernel(FaultDev fp, const dou
     st->nr_noconv = flags[1];
     flags[0] = INT_MAX;
     flags[1] = 0;
-    if (cap.run && (next != isolver || nr_flag != 0 || !isfinite(Vfmax))) *cap.run = 0;
   }
 }
 
@@ -1159,6 +1243,11 @@ template <typename T>
 __global__ void fill_kernel(int64_t n, T* y, T v) {
   const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   if (i < n) y[i] = v;
+}
+template <typename T>
+__global__ void fill_strided_kernel(int64_t n, T* y, int stride, T v) {
+  const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (i < n) y[i * stride] = v;
 }
 // x += gamma p, r -= gamma q with gamma = <p, r0> / <p, K p> (0 if not positive
 // definite in floating point); partial ||r||^2.  The energy-norm error of the
@@ -1357,8 +1446,11 @@ void launch_dense(int64_t n, const T* inv, const T* b, T* x, const CgScal* sc, h
 
 void launch_newmark(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, const double* M,
                     const double* d, const double* v, const double* a, double* d2, double* v2, double* a2,
-                    double dt, double half_dt, double half_dt_sq, int* flags, const int* run, hipStream_t s) {
-  OpNewmark op{fp, fs, mu, M, d, v, a, d2, v2, a2, dt, half_dt, half_dt_sq, flags, run, g.NY, nullptr, nullptr};
+                    double dt, double half_dt, double half_dt_sq, int* flags, const NewmarkRecord& rec,
+                    hipStream_t s) {
+  OpNewmark op{fp, fs, mu, M, d, v, a, d2, v2, a2, dt, half_dt, half_dt_sq, flags, g.NY, nullptr, nullptr,
+               rec.vprev, rec.nrprev, rec.vpart_out, rec.cap_fault, rec.seis_idx, rec.nseis, rec.cap_seis,
+               (int)tile_grid(g).x};
   hipLaunchKernelGGL((tile_kernel<double, OpNewmark>), tile_grid(g), dim3(BLOCK), 0, s, g, op);
 }
 
@@ -1431,6 +1523,11 @@ void launch_fill(int64_t n, T* y, T val, hipStream_t s) {
   if (n <= 0) return;
   hipLaunchKernelGGL((fill_kernel<T>), dim3(nblk(n)), dim3(BLOCK), 0, s, n, y, val);
 }
+template <typename T>
+void launch_fill_strided(int64_t n, T* y, int stride, T val, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL((fill_strided_kernel<T>), dim3(nblk(n)), dim3(BLOCK), 0, s, n, y, stride, val);
+}
 void launch_guess_dot(const Geom& g, const double* p, const double* mu, const double* r0, double* q,
                       double* part_pr, double* part_pq, hipStream_t s) {
   OpGuessDot op{p, mu, r0, q, part_pr, part_pq};
@@ -1479,5 +1576,6 @@ template void launch_convert<double, float>(int64_t, const double*, float*, hipS
 template void launch_convert<float, double>(int64_t, const float*, double*, hipStream_t);
 template void launch_convert<double, double>(int64_t, const double*, double*, hipStream_t);
 template void launch_fill<int>(int64_t, int*, int, hipStream_t);
+template void launch_fill_strided<int>(int64_t, int*, int, int, hipStream_t);
 
 }  // namespace eq

@@ -370,3 +370,39 @@ def test_state_init_after_dynamic_steps(oracle):
     fresh.state_init()
     b = [fresh.step().Vfmax for _ in range(30)]
     assert a == b
+
+
+def test_config1_event_catalogue_matches_oracle():
+    """main(P) on config 1 (40 x 40 elements, 25,921 nodes) through its first
+    two earthquake clusters (160 of 200 simulated years) against the oracle's
+    catalogue (direct solves, tests/golden/cfg1_oracle_events.json): every
+    sub-event of the first cluster to 1e-9 relative in tStart and tEnd, the
+    second cluster's onset to 1e-6 (solver tolerance 1e-12).  The FP32
+    V-cycle at tolerance 1e-8 keeps both onsets within 0.1%."""
+    import json
+    import os
+
+    from eqcycle_amd.api import EqContext
+    from eqcycle_amd.configs import CONFIGS
+    G = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "cfg1_oracle_events.json")))
+    ts_o, te_o = np.array(G["tStart"]), np.array(G["tEnd"])
+    yr = 365 * 24 * 3600.0
+    split = 100 * yr                       # the first cluster is near 20 years, the second near 149
+    n1 = int((ts_o < split).sum())
+    assert n1 >= 1 and (ts_o > split).any()
+    P = set_parameters(CONFIGS["cfg1"])
+    ctx = EqContext(P, amg={"hierarchy": "pmg"})
+    for tol, prec, r1, r2 in ((1e-12, "fp64", 1e-9, 1e-6), (1e-8, "f32pc", 1e-6, 1e-3)):
+        ctx.set_solver(tol=tol, pc="amg", prec=prec)
+        ctx.state_init()
+        R = ctx.run(total_time=160 * yr, snapshots=False)
+        ts, te = R["tStart"], R["tEnd"]
+        print(f"[gpu] cfg1 {prec} tol {tol:g}: {R['n_steps']} steps, {len(ts)} event starts", flush=True)
+        assert int((ts < split).sum()) == n1
+        if prec == "fp64":
+            assert np.allclose(ts[:n1], ts_o[:n1], rtol=r1, atol=0)
+            assert np.allclose(te[:n1], te_o[:n1], rtol=r1, atol=0)
+        else:
+            assert abs(ts[0] - ts_o[0]) <= r1 * ts_o[0]
+        s2, s2o = ts[ts > split][0], ts_o[ts_o > split][0]
+        assert abs(s2 - s2o) <= r2 * s2o
