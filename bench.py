@@ -44,9 +44,9 @@ HBM_PEAK_GBS = 8000.0
 
 # HBM bytes per launch of the fine-level Chebyshev step, from two rocprofv3
 # --pmc passes (FETCH_SIZE, WRITE_SIZE) calibrated by a 512 MiB copy, on the
-# 410,881-node mesh (profiles/cheb_ku_newmark_cfg2_hbm_traffic.md and
-# profiles/cheb_cfg2_fp32_hbm_traffic.md).  Keyed by (nodes, precision).
-CHEB_TRAFFIC = {(410881, "fp64"): 34.38e6, (410881, "fp32"): 20.13e6}
+# 410,881-node mesh (profiles/cheb_ku_newmark_cfg2_hbm_traffic.md).  Keyed by
+# (nodes, precision).
+CHEB_TRAFFIC = {(410881, "fp64"): 34.43e6, (410881, "fp32"): 20.20e6}
 
 
 def parse():

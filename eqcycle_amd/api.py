@@ -215,10 +215,10 @@ class EqContext:
         return out.value
 
     # ---- time loop ------------------------------------------------------------
-    def set_solver(self, tol=1e-6, maxiter=0, pc="amg", prec="fp64", guess="previous"):
+    def set_solver(self, tol=1e-6, maxiter=0, pc="amg", prec="fp64", guess="previous", graphs=True):
         o = L.SolverOpts(tol, maxiter, PC[pc], PREC[prec],
                          {"previous": L.EQ_GUESS_PREVIOUS, "learned": L.EQ_GUESS_LEARNED,
-                          "galerkin": L.EQ_GUESS_GALERKIN}[guess], 1)
+                          "galerkin": L.EQ_GUESS_GALERKIN}[guess], 1, 1 if graphs else 0)
         self._check(self.lib.eq_set_solver(self.h, C.byref(o)))
 
     def state_init(self, v_init=None, psi_init=None):

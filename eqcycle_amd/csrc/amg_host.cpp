@@ -12,6 +12,7 @@
 //   AlgebraicMultigrid.jl is not part of the reference repository and its
 //   version is unpinned; this follows the published algorithm.
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -135,6 +136,79 @@ static void csr_matmul(const HostCsr& A, const HostCsr& B, HostCsr* C) {
     for (int32_t j : cur) { C->col.push_back(j); C->val.push_back(acc[j]); }
     C->rowptr[i + 1] = (int64_t)C->col.size();
   }
+}
+
+// C = alpha A + beta B (same shape), sorted union of the row patterns
+static void csr_add(double alpha, const HostCsr& A, double beta, const HostCsr& B, HostCsr* C) {
+  C->n = A.n;
+  C->m = A.m;
+  C->rowptr.assign(A.n + 1, 0);
+  C->col.clear();
+  C->val.clear();
+  for (int64_t i = 0; i < A.n; i++) {
+    int64_t p = A.rowptr[i], q = B.rowptr[i];
+    const int64_t pe = A.rowptr[i + 1], qe = B.rowptr[i + 1];
+    while (p < pe || q < qe) {
+      const int32_t ca = p < pe ? A.col[p] : INT32_MAX, cb = q < qe ? B.col[q] : INT32_MAX;
+      if (ca == cb) { C->col.push_back(ca); C->val.push_back(alpha * A.val[p++] + beta * B.val[q++]); }
+      else if (ca < cb) { C->col.push_back(ca); C->val.push_back(alpha * A.val[p++]); }
+      else { C->col.push_back(cb); C->val.push_back(beta * B.val[q++]); }
+    }
+    C->rowptr[i + 1] = (int64_t)C->col.size();
+  }
+}
+
+// A <- diag(r) A diag(c) (either may be null)
+static void csr_scale(HostCsr* A, const double* r, const double* c) {
+  for (int64_t i = 0; i < A->n; i++)
+    for (int64_t q = A->rowptr[i]; q < A->rowptr[i + 1]; q++) {
+      if (r) A->val[q] *= r[i];
+      if (c) A->val[q] *= c[A->col[q]];
+    }
+}
+
+// sort the column indices (and values) of every row
+static void csr_sort_rows(HostCsr* A) {
+  std::vector<std::pair<int32_t, double>> row;
+  for (int64_t i = 0; i < A->n; i++) {
+    row.clear();
+    for (int64_t q = A->rowptr[i]; q < A->rowptr[i + 1]; q++) row.emplace_back(A->col[q], A->val[q]);
+    std::sort(row.begin(), row.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+    int64_t q = A->rowptr[i];
+    for (const auto& e : row) { A->col[q] = e.first; A->val[q] = e.second; q++; }
+  }
+}
+
+void condense_level(const HostLevel& Lin, double omega, HostCsr* M, HostCsr* U, HostCsr* V) {
+  HostLevel L = Lin;
+  csr_sort_rows(&L.A);
+  csr_sort_rows(&L.P);
+  csr_sort_rows(&L.R);
+  const double* dinv = L.dinv.data();
+  // M = R - omega (R A) D^-1
+  HostCsr RA;
+  csr_matmul(L.R, L.A, &RA);
+  csr_scale(&RA, nullptr, dinv);
+  csr_add(1.0, L.R, -omega, RA, M);
+  // U = 2 omega D^-1 - omega^2 D^-1 A D^-1
+  HostCsr DAD = L.A, Dg;
+  csr_scale(&DAD, dinv, dinv);
+  Dg.n = Dg.m = L.A.n;
+  Dg.rowptr.resize(L.A.n + 1);
+  Dg.col.resize(L.A.n);
+  Dg.val.resize(L.A.n);
+  for (int64_t i = 0; i < L.A.n; i++) {
+    Dg.rowptr[i] = i;
+    Dg.col[i] = (int32_t)i;
+    Dg.val[i] = dinv[i];
+  }
+  Dg.rowptr[L.A.n] = L.A.n;
+  csr_add(2.0 * omega, Dg, -omega * omega, DAD, U);
+  // V = P - omega D^-1 (A P)
+  HostCsr AP;
+  csr_matmul(L.A, L.P, &AP);
+  csr_scale(&AP, dinv, nullptr);
+  csr_add(1.0, L.P, -omega, AP, V);
 }
 
 // Strong connections of row i: |a_ij| >= theta max_{k!=i} |a_ik| (abs) or

@@ -9,7 +9,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -37,6 +39,7 @@ struct DevLevelVecs {
 struct DevLevel {
   int64_t n = 0;
   DevCsr A, P, R;   // P, R link this level to the next
+  DevCsr M, U, V;   // condensed sweeps (condense_level) of levels 1 .. L-2
   double* dinv = nullptr;
   float* dinv32 = nullptr;
   double omega = 0.0;
@@ -82,7 +85,9 @@ struct eq_ctx {
   int64_t it = 0;
   int isolver = 1;
   // solver
-  eq_solver_opts opts{1e-6, 0, EQ_PC_AMG, EQ_PREC_FP64, EQ_GUESS_PREVIOUS, 1};
+  eq_solver_opts opts{1e-6, 0, EQ_PC_AMG, EQ_PREC_FP64, EQ_GUESS_PREVIOUS, 1, 0};
+  // captured CG iteration batches: (precision, f32pc, pc, iterations, parity, x) -> executable graph
+  std::map<std::tuple<int, int, int, int64_t, int, const void*>, hipGraphExec_t> cg_graphs;
   CgWork<double> w64;
   CgWork<float> w32;
   double *part_rr = nullptr, *part_bb = nullptr, *part_rho = nullptr, *part_uq = nullptr;
@@ -132,6 +137,7 @@ struct eq_ctx {
   int64_t n_learned_solves = 0;
 
   ~eq_ctx() {
+    for (auto& kv : cg_graphs) (void)hipGraphExecDestroy(kv.second);
     learn_free(&learn);
     for (void* p : allocs) (void)hipFree(p);
     if (sc_host) (void)hipHostFree(sc_host);
@@ -632,6 +638,13 @@ extern "C" int eq_amg_setup(eq_ctx* ctx, double theta, int max_levels, int64_t m
       if (int rc = upload_csr(ctx, L.P, &D.P)) return rc;
       if (int rc = upload_csr(ctx, L.R, &D.R)) return rc;
     }
+    if (l > 0 && l + 1 < H.lev.size()) {
+      HostCsr M, U, V;
+      condense_level(L, D.omega, &M, &U, &V);
+      if (int rc = upload_csr(ctx, M, &D.M)) return rc;
+      if (int rc = upload_csr(ctx, U, &D.U)) return rc;
+      if (int rc = upload_csr(ctx, V, &D.V)) return rc;
+    }
     if (l > 0) {
       if (int rc = alloc_level_vecs(ctx, D.v64, D.n)) return rc;
       if (int rc = alloc_level_vecs(ctx, D.v32, D.n)) return rc;
@@ -804,20 +817,18 @@ static void vcycle(eq_ctx* ctx, const T* r, T* c) {
     launch_csr<T>(CSR_SPMV, L0.R, w.res0 + g.NX, nullptr, nullptr, T(0), lvecs<T>(ctx->lev[1]).b, nullptr, ctx->sc,
                   s);
   }
+  // coarse levels: one kernel down (b_next = M b) and one up (xp = U b + V xp_next) per level
   for (int l = 1; l <= L - 2; l++) {
     DevLevel& D = ctx->lev[l];
-    DevLevelVecs<T>& v = lvecs<T>(D);
-    launch_csr<T>(CSR_DOWN, D.A, nullptr, v.b, ldinv<T>(D), (T)D.omega, v.res, v.x, ctx->sc, s);
-    launch_csr<T>(CSR_SPMV, D.R, v.res, nullptr, nullptr, T(0), lvecs<T>(ctx->lev[l + 1]).b, nullptr, ctx->sc, s);
+    launch_csr<T>(CSR_SPMV, D.M, lvecs<T>(D).b, nullptr, nullptr, T(0), lvecs<T>(ctx->lev[l + 1]).b, nullptr,
+                  ctx->sc, s);
   }
   DevLevelVecs<T>& vc = lvecs<T>(ctx->lev[L - 1]);
   launch_dense<T>(ctx->ncoarse, coarse_inv_of<T>(ctx), vc.b, vc.xp, ctx->sc, s);
   for (int l = L - 2; l >= 1; l--) {
     DevLevel& D = ctx->lev[l];
     DevLevelVecs<T>& v = lvecs<T>(D);
-    launch_csr<T>(CSR_PROLONG_ADD, D.P, lvecs<T>(ctx->lev[l + 1]).xp, nullptr, nullptr, T(0), v.x, nullptr,
-                  ctx->sc, s);
-    launch_csr<T>(CSR_POST, D.A, v.x, v.b, ldinv<T>(D), (T)D.omega, v.xp, nullptr, ctx->sc, s);
+    launch_csr_up<T>(D.U, D.V, v.b, lvecs<T>(ctx->lev[l + 1]).xp, v.xp, ctx->sc, s);
   }
   const bool fused_prolong = ctx->l0_cheb && ctx->pmg;
   if (!fused_prolong)
@@ -860,6 +871,38 @@ static void cg_iteration(eq_ctx* ctx, T* x, int pc, bool f32pc = false) {
   w.cur = 1 - w.cur;
 }
 
+// n CG iterations.  With opts.graphs the batch is captured once per (precision,
+// preconditioner, n, direction-buffer parity, x) as a HIP graph of the same
+// kernels and replayed, which removes the per-kernel launch cost on the host;
+// the host-side parity of the direction buffers advances as in eager launch.
+template <typename T>
+static int enqueue_iterations(eq_ctx* ctx, T* x, int pc, bool f32pc, int64_t n) {
+  if (n <= 0) return EQ_OK;
+  CgWork<T>& w = work<T>(ctx);
+  if (!ctx->opts.graphs) {
+    for (int64_t k = 0; k < n; k++) cg_iteration<T>(ctx, x, pc, f32pc);
+    return EQ_OK;
+  }
+  const auto key = std::make_tuple((int)sizeof(T), f32pc ? 1 : 0, pc, n, w.cur, (const void*)x);
+  auto it = ctx->cg_graphs.find(key);
+  if (it == ctx->cg_graphs.end()) {
+    const int cur0 = w.cur;
+    HIPCHK(ctx, hipStreamBeginCapture(ctx->s, hipStreamCaptureModeThreadLocal));
+    for (int64_t k = 0; k < n; k++) cg_iteration<T>(ctx, x, pc, f32pc);
+    hipGraph_t graph = nullptr;
+    HIPCHK(ctx, hipStreamEndCapture(ctx->s, &graph));
+    w.cur = cur0;   // capture recorded the launches without running them
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(ctx, EQ_HIP_ERR, std::string("graph instantiate: ") + hipGetErrorString(e));
+    it = ctx->cg_graphs.emplace(key, exec).first;
+  }
+  HIPCHK(ctx, hipGraphLaunch(it->second, ctx->s));
+  if (n & 1) w.cur = 1 - w.cur;
+  return EQ_OK;
+}
+
 template <typename T>
 // r_ready: w.r already holds b - A x with its ||r||^2 partials in part_rr and
 // the ||b||^2 partials in part_bb (the learned start leaves them so).
@@ -879,7 +922,7 @@ static int cg_device(eq_ctx* ctx, T* x, const T* b, double tol, int64_t maxiter,
   int64_t& pred = ctx->pred_iters[ctx->cg_slot];
   int64_t batch = pred;
   for (;;) {
-    for (int64_t k = 0; k < batch; k++) cg_iteration<T>(ctx, x, pc, f32pc);
+    if (int rc = enqueue_iterations<T>(ctx, x, pc, f32pc, batch)) return rc;
     launch_cg_check(ctx->sc, ctx->part_rr, ctx->npart, s);
     HIPCHK(ctx, hipMemcpyAsync(ctx->sc_host, ctx->sc, sizeof(CgScal), hipMemcpyDeviceToHost, s));
     if (int rc = sync_check(ctx)) return rc;

@@ -61,6 +61,14 @@ void csr_transpose(const HostCsr& A, HostCsr* T);
 double estimate_lambda_max_dinvA(const HostCsr& A, const std::vector<double>& dinv, int iters);
 // K_II in structured interior order (reduced index r = n - NX), from nodal mu
 void assemble_kii(const Geom& g, const double* mu_struct, HostCsr* K);
+// Condensed sweeps of a coarse level with one Jacobi step (weight omega, zero
+// start) before and after the coarse-grid correction:
+//   b_next = M b,            M = R - omega R A D^-1
+//   e      = U b + V e_next, U = 2 omega D^-1 - omega^2 D^-1 A D^-1,
+//                            V = P - omega D^-1 A P
+// the same linear operators as smoothing, residual, restriction,
+// prolongation and smoothing applied one after the other.
+void condense_level(const HostLevel& L, double omega, HostCsr* M, HostCsr* U, HostCsr* V);
 // p-multigrid: level 0 = K_II with the Q4 -> Q1 vertex prolongation, then RS on
 // the Galerkin Q1 operator.
 void build_pmg_p0(const Geom& g, HostCsr* P);

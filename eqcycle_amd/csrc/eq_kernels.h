@@ -140,6 +140,9 @@ enum CsrMode { CSR_SPMV = 0, CSR_DOWN = 1, CSR_PROLONG_ADD = 2, CSR_POST = 3 };
 template <typename T>
 void launch_csr(CsrMode mode, const DevCsr& M, const T* x, const T* b, const T* dinv, T omega, T* y, T* xout,
                 const CgScal* sc, hipStream_t s);
+// y = U b + V e (condensed up-sweep of a coarse level; U and V have n rows)
+template <typename T>
+void launch_csr_up(const DevCsr& U, const DevCsr& V, const T* b, const T* e, T* y, const CgScal* sc, hipStream_t s);
 template <typename T>
 void launch_dense(int64_t n, const T* inv, const T* b, T* x, const CgScal* sc, hipStream_t s);
 

@@ -949,6 +949,29 @@ __global__ __launch_bounds__(BLOCK) void csr_kernel(int64_t n, const int32_t* __
   }
 }
 
+// y = U b + V e, LPR lanes per row
+template <typename T, int LPR>
+__global__ __launch_bounds__(BLOCK) void csr_up_kernel(int64_t n, const int32_t* __restrict__ urp,
+                                                       const int32_t* __restrict__ ucol, const T* __restrict__ uval,
+                                                       const int32_t* __restrict__ vrp,
+                                                       const int32_t* __restrict__ vcol, const T* __restrict__ vval,
+                                                       const T* b, const T* e, T* y, const CgScal* sc) {
+  if (sc && sc->done) return;
+  const int64_t gid = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  const int64_t row = gid / LPR;
+  const int lane = (int)(gid % LPR);
+  double acc = 0.0;
+  if (row < n) {
+    const int u1 = urp[row + 1];
+    for (int q = urp[row] + lane; q < u1; q += LPR) acc += (double)uval[q] * (double)b[ucol[q]];
+    const int v1 = vrp[row + 1];
+    for (int q = vrp[row] + lane; q < v1; q += LPR) acc += (double)vval[q] * (double)e[vcol[q]];
+  }
+#pragma unroll
+  for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_down(acc, off, LPR);
+  if (row < n && lane == 0) y[row] = (T)acc;
+}
+
 // x = A_c^-1 b, one wavefront per row of the row-major inverse
 template <typename T>
 __global__ __launch_bounds__(BLOCK) void dense_kernel(int64_t n, const T* inv, const T* b, T* x,
@@ -1439,6 +1462,26 @@ void launch_csr(CsrMode mode, const DevCsr& M, const T* x, const T* b, const T* 
 }
 
 template <typename T>
+void launch_csr_up(const DevCsr& U, const DevCsr& V, const T* b, const T* e, T* y, const CgScal* sc, hipStream_t s) {
+  const T* uval = csr_vals<T>(U);
+  const T* vval = csr_vals<T>(V);
+  const int lpr = std::max(U.lpr, V.lpr);
+  const int64_t threads = U.n * (int64_t)lpr;
+  const dim3 grid((unsigned)((threads + BLOCK - 1) / BLOCK));
+#define EQ_UP(L)                                                                                                   \
+  hipLaunchKernelGGL((csr_up_kernel<T, L>), grid, dim3(BLOCK), 0, s, U.n, U.rowptr, U.col, uval, V.rowptr, V.col, \
+                     vval, b, e, y, sc)
+  switch (lpr) {
+    case 1: EQ_UP(1); break;
+    case 2: EQ_UP(2); break;
+    case 4: EQ_UP(4); break;
+    case 8: EQ_UP(8); break;
+    default: EQ_UP(16); break;
+  }
+#undef EQ_UP
+}
+
+template <typename T>
 void launch_dense(int64_t n, const T* inv, const T* b, T* x, const CgScal* sc, hipStream_t s) {
   const int64_t blocks = (n + BLOCK / 64 - 1) / (BLOCK / 64);
   hipLaunchKernelGGL((dense_kernel<T>), dim3((unsigned)blocks), dim3(BLOCK), 0, s, n, inv, b, x, sc);
@@ -1567,6 +1610,7 @@ void launch_scale_mask(int64_t n, int nx, const double* x, double sign, double* 
   template void launch_csr<T>(CsrMode, const DevCsr&, const T*, const T*, const T*, T, T*, T*, const CgScal*,       \
                               hipStream_t);                                                                         \
   template void launch_dense<T>(int64_t, const T*, const T*, T*, const CgScal*, hipStream_t);                      \
+  template void launch_csr_up<T>(const DevCsr&, const DevCsr&, const T*, const T*, T*, const CgScal*, hipStream_t); \
   template void launch_cheb<T>(const Geom&, const ChebStep<T>&, const T*, const T*, const double*, int, CgScal*,   \
                                double*, hipStream_t);                                                              \
   template void launch_fill<T>(int64_t, T*, T, hipStream_t);

@@ -58,7 +58,7 @@ class ParamsDesc(C.Structure):
 
 class SolverOpts(C.Structure):
     _fields_ = [("tol", C.c_double), ("maxiter", C.c_int64), ("pc", C.c_int), ("prec", C.c_int),
-                ("guess", C.c_int), ("record_snapshots", C.c_int)]
+                ("guess", C.c_int), ("record_snapshots", C.c_int), ("graphs", C.c_int)]
 
 
 class StepInfo(C.Structure):

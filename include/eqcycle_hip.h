@@ -158,6 +158,7 @@ typedef struct {
   int prec;                /* EQ_PREC_FP64 or EQ_PREC_MIXED */
   int guess;               /* EQ_GUESS_PREVIOUS (src/main.jl:241), _LEARNED or _GALERKIN */
   int record_snapshots;    /* copy fault fields at events (src/main.jl:361-422) */
+  int graphs;              /* 1: CG iteration batches replay captured HIP graphs (same kernels) */
 } eq_solver_opts;
 int eq_set_solver(eq_ctx *ctx, const eq_solver_opts *opts);
 

@@ -406,3 +406,26 @@ def test_config1_event_catalogue_matches_oracle():
             assert abs(ts[0] - ts_o[0]) <= r1 * ts_o[0]
         s2, s2o = ts[ts > split][0], ts_o[ts_o > split][0]
         assert abs(s2 - s2o) <= r2 * s2o
+
+
+@pytest.mark.parametrize("prec", ["fp64", "f32pc"])
+def test_cg_graph_replay_equals_eager_launch(prec):
+    """CG iteration batches replayed from captured HIP graphs give the same
+    trajectory, bit for bit, as launching every kernel: 400 steps replay each
+    graph many times, with host synchronisations and other kernels between
+    the replays."""
+    from eqcycle_amd.api import EqContext
+    P = set_parameters(small(32, 8, material="trapezoid", total_years=3))
+    out = []
+    for graphs in (False, True):
+        ctx = EqContext(P, amg={"hierarchy": "pmg"})
+        ctx.set_solver(tol=1e-10, pc="amg", prec=prec, graphs=graphs)
+        ctx.state_init()
+        R = ctx.run(max_steps=400)
+        out.append((R, ctx.state_get()))
+    (Ra, sa), (Rb, sb) = out
+    assert Ra["total_pcg_iters"] > 400
+    assert np.array_equal(Ra["pcg_iters"], Rb["pcg_iters"])
+    assert np.array_equal(Ra["Vfmax"], Rb["Vfmax"])
+    for k in ("d", "v", "a", "psi", "tau"):
+        assert np.array_equal(sa[k], sb[k]), k
