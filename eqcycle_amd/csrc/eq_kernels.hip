@@ -981,8 +981,18 @@ __global__ __launch_bounds__(BLOCK) void dense_kernel(int64_t n, const T* inv, c
   const int lane = threadIdx.x & 63;
   double s = 0.0;
   if (row < n) {
+    // four independent partial sums keep four loads per lane in flight
     const T* a = inv + row * n;
-    for (int64_t j = lane; j < n; j += 64) s += (double)a[j] * (double)b[j];
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int64_t j = lane;
+    for (; j + 192 < n; j += 256) {
+      s0 += (double)a[j] * (double)b[j];
+      s1 += (double)a[j + 64] * (double)b[j + 64];
+      s2 += (double)a[j + 128] * (double)b[j + 128];
+      s3 += (double)a[j + 192] * (double)b[j + 192];
+    }
+    for (; j < n; j += 64) s0 += (double)a[j] * (double)b[j];
+    s = (s0 + s1) + (s2 + s3);
   }
   s = wave_sum(s);
   if (row < n && lane == 0) x[row] = (T)s;
