@@ -126,6 +126,7 @@ struct eq_ctx {
   int* nrflags = nullptr;             // [kBatch x 2]
   int* nrflags_host = nullptr;        // pinned
   int* flags = nullptr;
+  hipEvent_t ev_spin = nullptr;       // polled by spin_sync
   bool flags_dirty = true;            // flags not known to hold (INT_MAX, 0)
   int64_t nr_noconv_total = 0;
   int64_t pcg_noconv = 0;
@@ -145,6 +146,7 @@ struct eq_ctx {
     if (rec_seis_host) (void)hipHostFree(rec_seis_host);
     if (vpart_host) (void)hipHostFree(vpart_host);
     if (nrflags_host) (void)hipHostFree(nrflags_host);
+    if (ev_spin) (void)hipEventDestroy(ev_spin);
     if (s) (void)hipStreamDestroy(s);
   }
   template <typename T>
@@ -182,6 +184,22 @@ static int sync_check(eq_ctx* ctx) {
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipStreamSynchronize(ctx->s));
   return EQ_OK;
+}
+
+// Wait for the stream by polling an event: the time loop reads a few scalars
+// back several times per step, and a busy poll answers sooner than a
+// blocking synchronisation.
+static int spin_sync(eq_ctx* ctx) {
+  HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, hipEventRecord(ctx->ev_spin, ctx->s));
+  for (;;) {
+    const hipError_t e = hipEventQuery(ctx->ev_spin);
+    if (e == hipSuccess) return EQ_OK;
+    if (e != hipErrorNotReady) {
+      ctx->err = std::string("hipEventQuery: ") + hipGetErrorString(e);
+      return EQ_HIP_ERR;
+    }
+  }
 }
 
 template <typename T>
@@ -286,6 +304,10 @@ extern "C" int eq_create(const eq_mesh_desc* mesh, eq_ctx** out) {
   for (int64_t k = 0; k < ctx->N; k++)
     if (ctx->perm[k] >= ctx->NX) ctx->fltni.push_back(ctx->perm[k]);
   if (hipStreamCreate(&ctx->s) != hipSuccess) { *out = ctx.release(); return fail(*out, EQ_HIP_ERR, "stream"); }
+  if (hipEventCreateWithFlags(&ctx->ev_spin, hipEventDisableTiming) != hipSuccess) {
+    *out = ctx.release();
+    return fail(*out, EQ_HIP_ERR, "event");
+  }
   eq_ctx* c = ctx.get();
   const int64_t N = c->N;
   c->npart = std::max(tile_count(g), vec_blocks(N)) + 1;
@@ -925,7 +947,7 @@ static int cg_device(eq_ctx* ctx, T* x, const T* b, double tol, int64_t maxiter,
     if (int rc = enqueue_iterations<T>(ctx, x, pc, f32pc, batch)) return rc;
     launch_cg_check(ctx->sc, ctx->part_rr, ctx->npart, s);
     HIPCHK(ctx, hipMemcpyAsync(ctx->sc_host, ctx->sc, sizeof(CgScal), hipMemcpyDeviceToHost, s));
-    if (int rc = sync_check(ctx)) return rc;
+    if (int rc = spin_sync(ctx)) return rc;
     if (ctx->sc_host->done) break;
     batch = std::min<int64_t>(std::max<int64_t>(2, batch / 4), 16);
   }
@@ -1331,7 +1353,7 @@ static int read_status(eq_ctx* ctx, int k) {
   if (ns > 0)
     HIPCHK(ctx, hipMemcpyAsync(ctx->rec_seis_host, ctx->rec_seis, sizeof(double) * 3 * ns * k,
                                hipMemcpyDeviceToHost, ctx->s));
-  return sync_check(ctx);
+  return spin_sync(ctx);
 }
 
 static StepCapture capture(eq_ctx* ctx, int slot) {
@@ -1491,7 +1513,7 @@ static int dyn_batch(eq_ctx* ctx, double total_time, int64_t max_more, int* m) {
   if (ns > 0)
     HIPCHK(ctx, hipMemcpyAsync(ctx->rec_seis_host, ctx->rec_seis, sizeof(double) * 3 * ns * k,
                                hipMemcpyDeviceToHost, s));
-  if (int rc = sync_check(ctx)) return rc;
+  if (int rc = spin_sync(ctx)) return rc;
   // status of each step from its record, as the step tail computes it
   int n = 0;
   for (int q = 0; q < k; q++) {
@@ -1526,7 +1548,7 @@ static int dyn_batch(eq_ctx* ctx, double total_time, int64_t max_more, int* m) {
   const StepStatus mine = ctx->st_host[n - 1];
   HIPCHK(ctx, hipMemcpyAsync(ctx->st_host + (n - 1), ctx->st + (n - 1), sizeof(StepStatus), hipMemcpyDeviceToHost,
                              s));
-  if (int rc = sync_check(ctx)) return rc;
+  if (int rc = spin_sync(ctx)) return rc;
   const StepStatus& tail = ctx->st_host[n - 1];
   if (tail.isolver_next != mine.isolver_next || tail.hypo != mine.hypo || tail.nr_flag != mine.nr_flag ||
       !(tail.Vfmax == mine.Vfmax || (std::isnan(tail.Vfmax) && std::isnan(mine.Vfmax))))
