@@ -57,7 +57,7 @@ def parse():
     ap.add_argument("--config", default="cfg4")
     ap.add_argument("--tol", type=float, default=None)
     ap.add_argument("--hierarchy", default="pmg", help="pmg (default) | rs (Ruge-Stueben on K_II)")
-    ap.add_argument("--degree", type=int, default=3, help="Chebyshev degree on the fine level (pmg)")
+    ap.add_argument("--degree", type=int, default=5, help="Chebyshev degree on the fine level (pmg)")
     ap.add_argument("--epochs", type=int, default=300)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")

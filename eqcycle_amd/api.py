@@ -122,7 +122,7 @@ class EqContext:
 
     # ---- AMG / CG ----------------------------------------------------------
     def amg_setup(self, hierarchy="pmg", theta=0.25, max_levels=10, max_coarse=None, strength_abs=1,
-                  smoother=None, degree=3):
+                  smoother=None, degree=5):
         """hierarchy "rs": ruge_stuben on K_II with Jacobi smoothing (the reference's
         preconditioner structure); "pmg": degree-4 -> bilinear first level with
         Chebyshev smoothing, then Ruge-Stueben (the default, far fewer iterations)."""

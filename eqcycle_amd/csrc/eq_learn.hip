@@ -585,8 +585,53 @@ int learn_set_basis(LearnState* ls, const double* V, const double* sigma, const 
   LCHK(hipMemcpyAsync(dW, W.data(), sizeof(float) * W.size(), hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(basis_kernel, dim3((unsigned)((ls->nI + 255) / 256)), dim3(256), 0, s, ls->S, ls->nI, m, dW, NB,
                      ls->Phi);
+  // Cholesky-QR pass: S V / sigma loses orthogonality in FP32 along the small
+  // singular values, so G = Phi^T Phi (FP64), G = L L^T, Phi <- Phi L^-T
+  // (the row-major copy is the scratch output, then the pointers swap)
+  std::vector<double> Winv((size_t)NB * NB, 0.0);
+  bool reortho = false;
+  {
+    double* G = dalloc<double>((int64_t)NB * NB);
+    if (!G) { (void)hipFree(dW); *err = "device allocation failed"; return 7; }
+    const int nt = (NB + GT - 1) / GT;
+    hipLaunchKernelGGL(gram_kernel, dim3(nt, nt), dim3(256), 0, s, ls->Phi, ls->nI, NB, G);
+    std::vector<double> h((size_t)NB * NB);
+    const hipError_t e1 = hipMemcpyAsync(h.data(), G, sizeof(double) * NB * NB, hipMemcpyDeviceToHost, s);
+    const hipError_t e2 = hipStreamSynchronize(s);
+    (void)hipFree(G);
+    if (e1 != hipSuccess || e2 != hipSuccess) { (void)hipFree(dW); *err = "Gram of the basis failed"; return 2; }
+    std::vector<double> L((size_t)NB * NB, 0.0);
+    reortho = true;
+    for (int j = 0; j < NB && reortho; j++) {
+      double d = h[(size_t)j * NB + j];   // upper triangle holds G
+      for (int k = 0; k < j; k++) d -= L[(size_t)j * NB + k] * L[(size_t)j * NB + k];
+      if (!(d > 0.0)) { reortho = false; break; }
+      d = std::sqrt(d);
+      L[(size_t)j * NB + j] = d;
+      for (int i = j + 1; i < NB; i++) {
+        double a = h[(size_t)j * NB + i];
+        for (int k = 0; k < j; k++) a -= L[(size_t)i * NB + k] * L[(size_t)j * NB + k];
+        L[(size_t)i * NB + j] = a / d;
+      }
+    }
+    if (reortho) {
+      // Winv = L^-T (upper triangular): column c solves L^T w = e_c
+      for (int c = 0; c < NB; c++)
+        for (int i = NB - 1; i >= 0; i--) {
+          double a = (i == c) ? 1.0 : 0.0;
+          for (int k = i + 1; k < NB; k++) a -= L[(size_t)k * NB + i] * Winv[(size_t)k * NB + c];
+          Winv[(size_t)i * NB + c] = a / L[(size_t)i * NB + i];
+        }
+      std::vector<float> Wf((size_t)NB * NB);
+      for (size_t q = 0; q < Wf.size(); q++) Wf[q] = (float)Winv[q];
+      LCHK(hipMemcpyAsync(dW, Wf.data(), sizeof(float) * Wf.size(), hipMemcpyHostToDevice, s));
+      hipLaunchKernelGGL(basis_kernel, dim3((unsigned)((ls->nI + 255) / 256)), dim3(256), 0, s, ls->Phi, ls->nI, NB,
+                         dW, NB, ls->Phi_rm);
+      std::swap(ls->Phi, ls->Phi_rm);
+    }
+  }
   hipLaunchKernelGGL(phi_rowmajor_kernel, dim3(gblk(ls->nI * NB)), dim3(LB), 0, s, ls->nI, NB, ls->Phi, ls->Phi_rm);
-  // dataset: X_j = dF_j / |dF_j|, Y_jk = sigma_k V_jk / |dF_j|
+  // dataset: X_j = dF_j / |dF_j|, Y_j = Phi^T s_j / |dF_j| = Winv^T (sigma_k V_jk) / |dF_j|
   std::vector<float> F((int64_t)m * L0), X((int64_t)m * L0), Y((int64_t)m * NB);
   LCHK(hipMemcpyAsync(F.data(), ls->F, sizeof(float) * F.size(), hipMemcpyDeviceToHost, s));
   hipError_t e = hipStreamSynchronize(s);
@@ -603,7 +648,15 @@ int learn_set_basis(LearnState* ls, const double* V, const double* sigma, const 
     nrm = std::sqrt(nrm);
     const double inv = nrm > 0 ? 1.0 / nrm : 0.0;
     for (int i = 0; i < L0; i++) X[(int64_t)j * L0 + i] = (float)(F[(int64_t)j * L0 + i] * inv);
-    for (int k = 0; k < NB; k++) Y[(int64_t)j * NB + k] = (float)(sigma[k] * V[(int64_t)j * NB + k] * inv);
+    for (int k = 0; k < NB; k++) {
+      double y = 0.0;
+      if (reortho) {
+        for (int i = 0; i <= k; i++) y += Winv[(size_t)i * NB + k] * sigma[i] * V[(int64_t)j * NB + i];
+      } else {
+        y = sigma[k] * V[(int64_t)j * NB + k];
+      }
+      Y[(int64_t)j * NB + k] = (float)(y * inv);
+    }
   }
   LCHK(hipMemcpyAsync(ls->X, X.data(), sizeof(float) * X.size(), hipMemcpyHostToDevice, s));
   LCHK(hipMemcpyAsync(ls->Y, Y.data(), sizeof(float) * Y.size(), hipMemcpyHostToDevice, s));

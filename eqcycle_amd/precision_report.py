@@ -56,7 +56,7 @@ def main():
 
     cfg = CONFIGS[a.config]
     P = set_parameters(cfg)
-    ctx = EqContext(P, amg={"hierarchy": "pmg", "degree": 3})
+    ctx = EqContext(P, amg={"hierarchy": "pmg", "degree": 5})
     iflt = P.iFlt - 1
     res = {}
     lines = []

@@ -36,7 +36,7 @@ def main():
 
     cfg = CONFIGS[a.config]
     P = set_parameters(cfg)
-    ctx = EqContext(P, amg={"hierarchy": "pmg", "degree": 3})
+    ctx = EqContext(P, amg={"hierarchy": "pmg", "degree": 5})
     ctx.set_solver(tol=cfg.pcg_tol, pc="amg", prec="fp64")
     ctx.state_init()
     out = {"config": a.config, "nglob": P.nglob, "prec": a.prec, "reps": a.reps, "avg_us": {}}

@@ -41,7 +41,7 @@ def main():
 
     cfg = CONFIGS[a.config]
     P = set_parameters(cfg)
-    ctx = EqContext(P, amg={"hierarchy": "pmg", "degree": 3})
+    ctx = EqContext(P, amg={"hierarchy": "pmg", "degree": 5})
     tol = cfg.pcg_tol
     guess = {"plain": "previous", "network": "learned", "galerkin": "galerkin"}[a.variant]
     prec = a.prec or ("fp64" if a.variant == "plain" else cfg.precision)

@@ -85,7 +85,7 @@ function Context(P; W, iglob, NelX, NelY, dxe, dye, xgll, wgll, H, amg = true)
     set_params!(ctx, P)
     amg && check(ctx, ccall((:eq_amg_setup, LIB), Cint,
                            (Ptr{Cvoid}, Cdouble, Cint, Int64, Cint, Cint, Cint, Cint),
-                           ctx.h, 0.25, 10, 2000, 1, 1, 3, 1))
+                           ctx.h, 0.25, 10, 2000, 1, 1, 5, 1))
     return ctx
 end
 
