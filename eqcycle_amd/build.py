@@ -36,17 +36,20 @@ def _run(cmd):
     return r
 
 
-def build(verbose: bool = False) -> str:
+def build(verbose: bool = False, defines=()) -> str:
+    """defines: extra preprocessor definitions, e.g. ("EQ_TILE_I=32",) for
+    32 x 16 tiles in the grid kernels (default 64 x 16)."""
     os.makedirs(OUT_DIR, exist_ok=True)
+    dflags = [f"-D{d}" for d in defines]
     jobs = []
     objs = []
     for src in HIP_SOURCES:
         obj = os.path.join(OUT_DIR, src.replace(".hip", ".o"))
-        jobs.append([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+        jobs.append([HIPCC, *HIP_FLAGS, *dflags, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     for src in CXX_SOURCES:
         obj = os.path.join(OUT_DIR, src.replace(".cpp", ".o"))
-        jobs.append(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+        jobs.append(["g++", *CXX_FLAGS, *dflags, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     with ThreadPoolExecutor(max_workers=4) as ex:
         for r in ex.map(_run, jobs):
@@ -59,4 +62,8 @@ def build(verbose: bool = False) -> str:
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
+    import argparse
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--define", action="append", default=[], help="preprocessor definition NAME=VALUE")
+    a = ap.parse_args()
+    print(build(verbose=True, defines=a.define))

@@ -5,8 +5,12 @@
 namespace eq {
 
 constexpr int BLOCK = 256;
-constexpr int TILE_I = 64;   // owned nodes per tile along x
-constexpr int TILE_J = 16;   // owned nodes per tile along y
+#ifndef EQ_TILE_I
+#define EQ_TILE_I 64
+#endif
+constexpr int TILE_I = EQ_TILE_I;   // owned nodes per tile along x (64 or 32)
+constexpr int TILE_J = 16;          // owned nodes per tile along y
+constexpr int TBLOCK = TILE_I * TILE_J / 4;   // threads of a tile kernel block (4 nodes each)
 
 // Device scalars of one CG solve (all FP64).
 struct CgScal {

@@ -37,6 +37,7 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// blockDim.x is 256 (BLOCK) or, in tile kernels, TBLOCK (a multiple of 64)
 __device__ double block_sum(double v) {
   __shared__ double sh[BLOCK / 64];
   v = wave_sum(v);
@@ -44,13 +45,20 @@ __device__ double block_sum(double v) {
   __syncthreads();
   if (lane == 0) sh[w] = v;
   __syncthreads();
-  double r = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  const int nw = (int)(blockDim.x >> 6);
+  double r;
+  if (nw == 4) {
+    r = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  } else {
+    r = sh[0];
+    for (int k = 1; k < nw; k++) r += sh[k];
+  }
   return r;
 }
 
 __device__ double reduce_parts(const double* p, int n) {
   double s = 0.0;
-  for (int i = threadIdx.x; i < n; i += BLOCK) s += p[i];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s += p[i];
   return block_sum(s);
 }
 
@@ -211,12 +219,13 @@ __device__ __forceinline__ T seg_last(const Geom& g, const T* u, const T* m) {
 // consecutive nodes of one column, adds the y-direction terms and runs the
 // epilogue.  Every 5-node segment is evaluated once per direction (plus the
 // shared end node of the previous segment).
-static_assert(TILE_J * (TILE_I / 4) == BLOCK, "phase 2: one row strip per thread");
-static_assert(TILE_I * (TILE_J / 4) == BLOCK, "phase 3: one column strip per thread");
-constexpr int WIN_LOADS = (WIN_J * WIN_I + BLOCK - 1) / BLOCK;
+static_assert(TILE_J * (TILE_I / 4) == TBLOCK, "phase 2: one row strip per thread");
+static_assert(TILE_I * (TILE_J / 4) == TBLOCK, "phase 3: one column strip per thread");
+static_assert(TBLOCK % 64 == 0 && TBLOCK <= BLOCK, "whole waves");
+constexpr int WIN_LOADS = (WIN_J * WIN_I + TBLOCK - 1) / TBLOCK;
 
 template <typename T, class Op>
-__global__ __launch_bounds__(BLOCK) void tile_kernel(Geom g, Op op) {
+__global__ __launch_bounds__(TBLOCK) void tile_kernel(Geom g, Op op) {
   __shared__ T su[WIN_J * WIN_I];
   __shared__ T sm[WIN_J * WIN_I];
   __shared__ T sx[TILE_J * TILE_I];
@@ -234,7 +243,7 @@ __global__ __launch_bounds__(BLOCK) void tile_kernel(Geom g, Op op) {
     T ur[WIN_LOADS], mr[WIN_LOADS];
 #pragma unroll
     for (int t = 0; t < WIN_LOADS; t++) {
-      const int k = threadIdx.x + t * BLOCK;
+      const int k = threadIdx.x + t * TBLOCK;
       const int r = k / WIN_I, c = k - r * WIN_I;
       const int I = I0 - 4 + c, J = J0 - 4 + r;
       ur[t] = 0;
@@ -247,7 +256,7 @@ __global__ __launch_bounds__(BLOCK) void tile_kernel(Geom g, Op op) {
     }
 #pragma unroll
     for (int t = 0; t < WIN_LOADS; t++) {
-      const int k = threadIdx.x + t * BLOCK;
+      const int k = threadIdx.x + t * TBLOCK;
       if (k < WIN_J * WIN_I) {
         su[k] = ur[t];
         sm[k] = mr[t];
@@ -338,7 +347,7 @@ struct OpResidual {
   __device__ void block_end(double, double) const {}
   int npart;
   __device__ void zero_tail(int nb) const {
-    for (int k = nb + threadIdx.x; k < npart; k += BLOCK) {
+    for (int k = nb + threadIdx.x; k < npart; k += blockDim.x) {
       part0[k] = 0.0;
       part1[k] = 0.0;
     }
@@ -804,11 +813,11 @@ struct OpNewmark {
     }
   }
   // bottom tile row: (max, first index) of the new fault velocity over the
-  // tile's 64 fault nodes, all held by wave 0
+  // tile's TILE_I fault nodes, all held by the first TILE_I threads (wave 0)
   __device__ void block_end(double a0, double a1) const {
     if (!vpart_out || blockIdx.y != 0 || threadIdx.x >= 64) return;
     // as the step tail: NaN is never the maximum; no valid value leaves (-inf, INT_MAX)
-    const bool valid = (int)(blockIdx.x * TILE_I + threadIdx.x) < fp.nflt && !isnan(a0);
+    const bool valid = (int)threadIdx.x < TILE_I && (int)(blockIdx.x * TILE_I + threadIdx.x) < fp.nflt && !isnan(a0);
     double m = valid ? a0 : -INFINITY, mi = valid ? a1 : 2147483647.0;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -1346,7 +1355,7 @@ static inline int nblk(int64_t n) { return (int)((n + BLOCK - 1) / BLOCK); }
 template <typename T>
 void launch_ku(const Geom& g, const T* u, const T* mu, T* y, int interior, int negate, hipStream_t s) {
   OpApply<T> op{u, mu, y, interior, negate ? T(-1) : T(1), nullptr, nullptr};
-  hipLaunchKernelGGL((tile_kernel<T, OpApply<T>>), tile_grid(g), dim3(BLOCK), 0, s, g, op);
+  hipLaunchKernelGGL((tile_kernel<T, OpApply<T>>), tile_grid(g), dim3(TBLOCK), 0, s, g, op);
 }
 
 void launch_diag(const Geom& g, const double* mu, double* diag, hipStream_t s) {
@@ -1363,7 +1372,7 @@ template <typename T>
 void launch_cg_residual(const Geom& g, const T* x, const T* mu, const T* b, T* r, double* part_rr, double* part_bb,
                         int npart, hipStream_t s) {
   OpResidual<T> op{npart, x, mu, b, r, part_rr, part_bb};
-  hipLaunchKernelGGL((tile_kernel<T, OpResidual<T>>), tile_grid(g), dim3(BLOCK), 0, s, g, op);
+  hipLaunchKernelGGL((tile_kernel<T, OpResidual<T>>), tile_grid(g), dim3(TBLOCK), 0, s, g, op);
 }
 
 void launch_cg_init_scal(CgScal* sc, const double* part_rr, const double* part_bb, int nparts, double tol,
@@ -1379,7 +1388,7 @@ template <typename T>
 void launch_cg_dir(const Geom& g, const T* c, const T* u_old, const T* mu, T* u_new, T* q, const double* part_rho,
                    int nrho, const CgScal* sc, double* part_uq, hipStream_t s) {
   OpCgDir<T> op{c, u_old, mu, u_new, q, part_rho, nrho, sc, part_uq, nullptr};
-  hipLaunchKernelGGL((tile_kernel<T, OpCgDir<T>>), tile_grid(g), dim3(BLOCK), 0, s, g, op);
+  hipLaunchKernelGGL((tile_kernel<T, OpCgDir<T>>), tile_grid(g), dim3(TBLOCK), 0, s, g, op);
 }
 
 template <typename T>
@@ -1401,14 +1410,14 @@ template <typename T>
 void launch_l0_down(const Geom& g, const T* r, const T* mu, const T* dinv, T omega, T* x0, T* res,
                     const double* part_rr, int nrr, CgScal* sc, hipStream_t s) {
   OpL0Down<T> op{r, mu, dinv, omega, x0, res, part_rr, nrr, sc, nullptr, nullptr};
-  hipLaunchKernelGGL((tile_kernel<T, OpL0Down<T>>), tile_grid(g), dim3(BLOCK), 0, s, g, op);
+  hipLaunchKernelGGL((tile_kernel<T, OpL0Down<T>>), tile_grid(g), dim3(TBLOCK), 0, s, g, op);
 }
 
 template <typename T>
 void launch_l0_post(const Geom& g, const T* x0, const T* r, const T* mu, const T* dinv, T omega, T* c,
                     double* part_rho, const CgScal* sc, hipStream_t s) {
   OpL0Post<T> op{x0, r, mu, dinv, omega, c, sc, part_rho, nullptr};
-  hipLaunchKernelGGL((tile_kernel<T, OpL0Post<T>>), tile_grid(g), dim3(BLOCK), 0, s, g, op);
+  hipLaunchKernelGGL((tile_kernel<T, OpL0Post<T>>), tile_grid(g), dim3(TBLOCK), 0, s, g, op);
 }
 
 
@@ -1419,7 +1428,7 @@ void launch_cheb(const Geom& g, const ChebStep<T>& c, const T* mu, const T* dinv
                c.xout, c.mode, c.write_x_center, c.rdot, c.check, part_rr, nrr, sc, part_rho, nullptr,
                c.coarse, g.nelx, g.nely, {}};
   for (int p = 0; p < 5; p++) op.gx[p] = (T)g.x[p];
-  hipLaunchKernelGGL((tile_kernel<T, OpCheb<T>>), tile_grid(g), dim3(BLOCK), 0, s, g, op);
+  hipLaunchKernelGGL((tile_kernel<T, OpCheb<T>>), tile_grid(g), dim3(TBLOCK), 0, s, g, op);
 }
 
 template <typename T, int MODE>
@@ -1504,7 +1513,7 @@ void launch_newmark(const Geom& g, const FaultDev& fp, FaultState fs, const doub
   OpNewmark op{fp, fs, mu, M, d, v, a, d2, v2, a2, dt, half_dt, half_dt_sq, flags, g.NY, nullptr, nullptr,
                rec.vprev, rec.nrprev, rec.vpart_out, rec.cap_fault, rec.seis_idx, rec.nseis, rec.cap_seis,
                (int)tile_grid(g).x};
-  hipLaunchKernelGGL((tile_kernel<double, OpNewmark>), tile_grid(g), dim3(BLOCK), 0, s, g, op);
+  hipLaunchKernelGGL((tile_kernel<double, OpNewmark>), tile_grid(g), dim3(TBLOCK), 0, s, g, op);
 }
 
 void launch_qs_begin(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, double* d, double* dPre,
@@ -1584,7 +1593,7 @@ void launch_fill_strided(int64_t n, T* y, int stride, T val, hipStream_t s) {
 void launch_guess_dot(const Geom& g, const double* p, const double* mu, const double* r0, double* q,
                       double* part_pr, double* part_pq, hipStream_t s) {
   OpGuessDot op{p, mu, r0, q, part_pr, part_pq};
-  hipLaunchKernelGGL((tile_kernel<double, OpGuessDot>), tile_grid(g), dim3(BLOCK), 0, s, g, op);
+  hipLaunchKernelGGL((tile_kernel<double, OpGuessDot>), tile_grid(g), dim3(TBLOCK), 0, s, g, op);
 }
 void launch_guess_update(int64_t n, double* x, double* r, const double* p, const double* q, const double* part_pr,
                          const double* part_pq, int nparts, double* part_rr, int npart_out, hipStream_t s) {
