@@ -250,8 +250,12 @@ class EqContext:
                         info.isolver_next, (info.pcg_iters[0], info.pcg_iters[1]), info.nr_noconv)
 
     def run(self, total_time=None, max_steps=0, cap_steps=200000, cap_events=400, cap_is=4000, cap_seis=30000,
-            cap_rec=190000, snapshots=True):
-        """main(P) on the GPU; returns a dict with the fields of struct results."""
+            cap_rec=190000, snapshots=True, events=False):
+        """main(P) on the GPU; returns a dict with the fields of struct results.
+        Consecutive calls continue one run (the event bookkeeping carries
+        over until state_init / state_set).  events=True keeps the per-event
+        fault matrices (taubefore, tauafter, delfafter) without the snapshot
+        series."""
         P = self.P
         F = P.FltNglob
         ns = len(P.out_seis)
@@ -259,9 +263,10 @@ class EqContext:
         bufs = dict(time_=np.zeros(cap_steps), Vfmax=np.zeros(cap_steps), isolver=np.zeros(cap_steps, np.int64),
                     pcg_iters=np.zeros(2 * cap_steps, np.int64), tStart=np.zeros(cap_events),
                     tEnd=np.zeros(cap_events), hypo=np.zeros(cap_events))
-        if snapshots:
+        if snapshots or events:
             for k in ("taubefore", "tauafter", "delfafter"):
                 bufs[k] = np.zeros((cap_events, F))
+        if snapshots:
             for k in ("is_slip", "is_slipvel", "is_stress"):
                 bufs[k] = np.zeros((cap_is, F))
             for k in ("seismic_slip", "seismic_slipvel", "seismic_stress"):
@@ -282,11 +287,12 @@ class EqContext:
                    pcg_iters=bufs["pcg_iters"][:2 * min(n, cap_steps)].reshape(-1, 2),
                    tStart=bufs["tStart"][:min(R.n_start, cap_events)], tEnd=bufs["tEnd"][:min(R.n_end, cap_events)],
                    hypo=bufs["hypo"][:min(R.n_start, cap_events)])
-        if snapshots:
+        if snapshots or events:
             out.update(taubefore=bufs["taubefore"][:min(R.n_start, cap_events)],
                        tauafter=bufs["tauafter"][:min(R.n_end, cap_events)],
-                       delfafter=bufs["delfafter"][:min(R.n_end, cap_events)],
-                       is_slip=bufs["is_slip"][:min(R.n_is, cap_is)],
+                       delfafter=bufs["delfafter"][:min(R.n_end, cap_events)])
+        if snapshots:
+            out.update(is_slip=bufs["is_slip"][:min(R.n_is, cap_is)],
                        is_slipvel=bufs["is_slipvel"][:min(R.n_is, cap_is)],
                        is_stress=bufs["is_stress"][:min(R.n_is, cap_is)],
                        seismic_slip=bufs["seismic_slip"][:min(R.n_seis, cap_seis)],

@@ -129,6 +129,13 @@ struct eq_ctx {
   hipEvent_t ev_spin = nullptr;       // polled by spin_sync
   bool flags_dirty = true;            // flags not known to hold (INT_MAX, 0)
   int64_t nr_noconv_total = 0;
+  // results bookkeeping of main(P) (src/main.jl:361-436), carried across eq_run
+  // calls so that consecutive calls continue one run; reset by eq_state_init
+  struct RunBook {
+    double tvsx = 0, tevneb = 0, tevne = 0;
+    int idelevne = 0, slipstart = 0;
+    std::vector<double> delfref;
+  } book;
   int64_t pcg_noconv = 0;
   // learned initial guess
   LearnState learn;
@@ -1276,6 +1283,15 @@ static int alloc_state(eq_ctx* ctx) {
   return EQ_OK;
 }
 
+// Fresh results bookkeeping at time ctx->t: no event in progress, next
+// interseismic snapshot at the next multiple of 5 years (src/main.jl:142-146).
+static void reset_book(eq_ctx* ctx) {
+  const double yr2sec = 365.0 * 24 * 60 * 60;
+  ctx->book = eq_ctx::RunBook();
+  ctx->book.tvsx = 5 * yr2sec * (std::floor(ctx->t / (5 * yr2sec)) + 1);
+  ctx->book.delfref.assign(ctx->NX, 0.0);
+}
+
 extern "C" int eq_state_init(eq_ctx* ctx, double v_init, const double* psi_init) {
   if (!ctx) return EQ_EINVAL;
   if (!ctx->have_material || !ctx->have_params) return fail(ctx, EQ_ENOTREADY, "material and params first");
@@ -1310,6 +1326,7 @@ extern "C" int eq_state_init(eq_ctx* ctx, double v_init, const double* psi_init)
   ctx->isolver = 1;
   ctx->nr_noconv_total = 0;
   ctx->pcg_noconv = 0;
+  reset_book(ctx);
   ctx->have_state = true;
   return sync_check(ctx);
 }
@@ -1343,6 +1360,7 @@ extern "C" int eq_state_set(eq_ctx* ctx, const double* d, const double* v, const
   ctx->t = t;
   ctx->dt = dt;
   ctx->isolver = (int)isolver;
+  reset_book(ctx);
   return sync_check(ctx);
 }
 
@@ -1565,12 +1583,17 @@ extern "C" int eq_run(eq_ctx* ctx, double total_time, int64_t max_steps, eq_resu
   const int64_t NX = ctx->NX;
   const double Vpl = ctx->Vpl;
   const double yr2sec = 365.0 * 24 * 60 * 60;
-  double tvsx = 5 * yr2sec;
-  const double tvsxinc = tvsx;
+  const double tvsxinc = 5 * yr2sec;
   const double tevneinc = 0.5;
-  double tevneb = 0, tevne = 0;
-  int idelevne = 0, slipstart = 0;
-  std::vector<double> fault(3 * NX), delfref(NX, 0.0), tmp(NX);
+  eq_ctx::RunBook& B = ctx->book;
+  double& tvsx = B.tvsx;
+  double& tevneb = B.tevneb;
+  double& tevne = B.tevne;
+  int& idelevne = B.idelevne;
+  int& slipstart = B.slipstart;
+  if ((int64_t)B.delfref.size() != NX) B.delfref.assign(NX, 0.0);
+  std::vector<double>& delfref = B.delfref;
+  std::vector<double> fault(3 * NX), tmp(NX);
   const size_t nseis = ctx->out_seis.size();
   R->n_steps = R->n_start = R->n_end = R->n_is = R->n_seis = R->n_rec = 0;
   R->n_qs = R->n_dyn = R->total_pcg_iters = R->nr_noconv = 0;

@@ -429,3 +429,24 @@ def test_cg_graph_replay_equals_eager_launch(prec):
     assert np.array_equal(Ra["Vfmax"], Rb["Vfmax"])
     for k in ("d", "v", "a", "psi", "tau"):
         assert np.array_equal(sa[k], sb[k]), k
+
+
+def test_consecutive_runs_continue_one_run():
+    """main(P) split over several eq_run calls gives the event list and step
+    history of one call: the bookkeeping carries over, the split point lies
+    inside the first event."""
+    from eqcycle_amd.api import EqContext
+    P = set_parameters(small(32, 8, material="trapezoid", total_years=3))
+    a = EqContext(P, amg=True)
+    a.set_solver(tol=1e-12, pc="amg")
+    a.state_init()
+    whole = a.run(snapshots=False, events=True)
+    a.state_init()
+    parts = [a.run(max_steps=n, snapshots=False, events=True) for n in (700, 300)]
+    parts.append(a.run(snapshots=False, events=True))
+    cat = {k: np.concatenate([p[k] for p in parts]) for k in ("time_", "Vfmax", "tStart", "tEnd", "hypo")}
+    assert sum(p["n_steps"] for p in parts) == whole["n_steps"]
+    assert len(whole["tStart"]) >= 1 and whole["tStart"][0] < parts[0]["time_"][-1] < whole["tEnd"][0]
+    for k in cat:
+        assert np.array_equal(cat[k], whole[k]), k
+    assert np.array_equal(np.concatenate([p["delfafter"] for p in parts]), whole["delfafter"])
