@@ -215,10 +215,12 @@ class EqContext:
         return out.value
 
     # ---- time loop ------------------------------------------------------------
-    def set_solver(self, tol=1e-6, maxiter=0, pc="amg", prec="fp64", guess="previous", graphs=True):
+    def set_solver(self, tol=1e-6, maxiter=0, pc="amg", prec="fp64", guess="previous", graphs=True,
+                   cheb_pairs=True):
         o = L.SolverOpts(tol, maxiter, PC[pc], PREC[prec],
                          {"previous": L.EQ_GUESS_PREVIOUS, "learned": L.EQ_GUESS_LEARNED,
-                          "galerkin": L.EQ_GUESS_GALERKIN}[guess], 1, 1 if graphs else 0)
+                          "galerkin": L.EQ_GUESS_GALERKIN}[guess], 1, 1 if graphs else 0,
+                         0 if cheb_pairs else 1)
         self._check(self.lib.eq_set_solver(self.h, C.byref(o)))
 
     def state_init(self, v_init=None, psi_init=None):
@@ -250,12 +252,15 @@ class EqContext:
                         info.isolver_next, (info.pcg_iters[0], info.pcg_iters[1]), info.nr_noconv)
 
     def run(self, total_time=None, max_steps=0, cap_steps=200000, cap_events=400, cap_is=4000, cap_seis=30000,
-            cap_rec=190000, snapshots=True, events=False):
+            cap_rec=190000, snapshots=True, events=False, keep_on_nr_failure=False):
         """main(P) on the GPU; returns a dict with the fields of struct results.
         Consecutive calls continue one run (the event bookkeeping carries
         over until state_init / state_set).  events=True keeps the per-event
         fault matrices (taubefore, tauafter, delfafter) without the snapshot
-        series."""
+        series.  A failed rate-state search (NaN slip rate or stress,
+        FBC! in src/NRsearch.jl:15-54) raises EqError, unless
+        keep_on_nr_failure: then the results up to the failing step come back
+        with the message under "error" (None when the run completed)."""
         P = self.P
         F = P.FltNglob
         ns = len(P.out_seis)
@@ -278,7 +283,8 @@ class EqContext:
         for k, v in bufs.items():
             setattr(R, k, ptr(v, C.c_int64 if v.dtype == np.int64 else C.c_double))
         T = P.Total_time if total_time is None else total_time
-        self._check(self.lib.eq_run(self.h, T, max_steps, C.byref(R)))
+        rc = self.lib.eq_run(self.h, T, max_steps, C.byref(R))
+        self._check(rc, allow=(L.EQ_NR_NAN,) if keep_on_nr_failure else ())
         n = R.n_steps
         out = dict(n_steps=n, n_qs=R.n_qs, n_dyn=R.n_dyn, total_pcg_iters=R.total_pcg_iters,
                    nr_noconv=R.nr_noconv, wall_qs=R.wall_qs, wall_dyn=R.wall_dyn,
@@ -286,7 +292,8 @@ class EqContext:
                    isolver=bufs["isolver"][:min(n, cap_steps)],
                    pcg_iters=bufs["pcg_iters"][:2 * min(n, cap_steps)].reshape(-1, 2),
                    tStart=bufs["tStart"][:min(R.n_start, cap_events)], tEnd=bufs["tEnd"][:min(R.n_end, cap_events)],
-                   hypo=bufs["hypo"][:min(R.n_start, cap_events)])
+                   hypo=bufs["hypo"][:min(R.n_start, cap_events)],
+                   error=str(EqError(rc, self._err())) if rc else None)
         if snapshots or events:
             out.update(taubefore=bufs["taubefore"][:min(R.n_start, cap_events)],
                        tauafter=bufs["tauafter"][:min(R.n_end, cap_events)],
@@ -379,7 +386,7 @@ class EqContext:
         """Average device ms per launch, HIP events on the library's stream."""
         w = {"ku": L.EQ_TIME_KU, "cg_dir": L.EQ_TIME_CG_DIR, "vcycle": L.EQ_TIME_VCYCLE,
              "cg_iter": L.EQ_TIME_CG_ITER, "newmark": L.EQ_TIME_NEWMARK, "cheb": L.EQ_TIME_CHEB,
-             "copy": L.EQ_TIME_COPY}[which]
+             "copy": L.EQ_TIME_COPY, "cheb2": L.EQ_TIME_CHEB2}[which]
         out = C.c_double()
         self._check(self.lib.eq_time_kernel(self.h, w, PREC[prec], reps, C.byref(out)))
         return out.value

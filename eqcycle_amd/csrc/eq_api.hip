@@ -23,11 +23,14 @@ using namespace eq;
 
 namespace {
 
+constexpr int kMaxChebDegree = 8;   // largest level-0 Chebyshev degree eq_amg_setup accepts
+
 template <typename T>
 struct CgWork {
   T *r = nullptr, *c = nullptr, *u[2] = {nullptr, nullptr}, *q = nullptr, *x0 = nullptr, *res0 = nullptr;
   T *b = nullptr, *x = nullptr;   // solve-level vectors for the mixed-precision inner solve
-  T *chx = nullptr, *chx2 = nullptr, *chr = nullptr, *chd[2] = {nullptr, nullptr};  // level-0 Chebyshev
+  T *chx = nullptr, *chx2 = nullptr;   // level-0 Chebyshev iterates
+  T *chr[3] = {nullptr, nullptr, nullptr}, *chd[3] = {nullptr, nullptr, nullptr};   // residuals, directions
   int cur = 0;
 };
 
@@ -366,10 +369,10 @@ static int alloc_work(eq_ctx* ctx, CgWork<T>& w) {
   w.r = ctx->alloc<T>(N); w.c = ctx->alloc<T>(N); w.u[0] = ctx->alloc<T>(N); w.u[1] = ctx->alloc<T>(N);
   w.q = ctx->alloc<T>(N); w.x0 = ctx->alloc<T>(N); w.res0 = ctx->alloc<T>(N); w.b = ctx->alloc<T>(N);
   w.x = ctx->alloc<T>(N);
-  w.chx = ctx->alloc<T>(N); w.chx2 = ctx->alloc<T>(N); w.chr = ctx->alloc<T>(N);
-  w.chd[0] = ctx->alloc<T>(N); w.chd[1] = ctx->alloc<T>(N);
+  w.chx = ctx->alloc<T>(N); w.chx2 = ctx->alloc<T>(N);
+  for (int i = 0; i < 3; i++) { w.chr[i] = ctx->alloc<T>(N); w.chd[i] = ctx->alloc<T>(N); }
   if (!w.r || !w.c || !w.u[0] || !w.u[1] || !w.q || !w.x0 || !w.res0 || !w.b || !w.x || !w.chx || !w.chx2 ||
-      !w.chr || !w.chd[0] || !w.chd[1])
+      !w.chr[0] || !w.chr[1] || !w.chr[2] || !w.chd[0] || !w.chd[1] || !w.chd[2])
     return fail(ctx, EQ_NOMEM, "device allocation failed");
   launch_fill<T>(N, w.x0, T(0), ctx->s);
   launch_fill<T>(N, w.res0, T(0), ctx->s);
@@ -747,44 +750,63 @@ const float* ones_of<float>(eq_ctx* c) { return c->ones32; }
 
 // One V-cycle: c = M^-1 r with partial c.r (and the cg! convergence test in
 // its first kernel).
-// Level-0 Chebyshev pre-smoothing from x = 0: k kernels, leaving x_k in chx
-// and the residual b - K_II x_k in chr.
+// Launch a level-0 Chebyshev sweep given as its single steps: consecutive
+// steps go to one cheb_pair launch (unless opts.cheb_single), an odd last step
+// to one OpCheb launch.  The steps rotate r and d through three buffers, so a
+// pair never writes a buffer it reads beyond its own tile.
 template <typename T>
-static void cheb_pre(eq_ctx* ctx, const T* b) {
+static void cheb_launch(eq_ctx* ctx, const ChebStep<T>* st, int k) {
+  int m = 0;
+  while (m < k) {
+    if (!ctx->opts.cheb_single && m + 1 < k && st[m].mode == 0) {
+      launch_cheb_pair<T>(ctx->g, st[m], st[m + 1], mu_of<T>(ctx), dinv_of<T>(ctx), ctx->part_rr, ctx->npart,
+                          ctx->sc, ctx->part_rho, ctx->s);
+      m += 2;
+    } else {
+      launch_cheb<T>(ctx->g, st[m], mu_of<T>(ctx), dinv_of<T>(ctx), ctx->part_rr, ctx->npart, ctx->sc,
+                     ctx->part_rho, ctx->s);
+      m += 1;
+    }
+  }
+}
+
+// Level-0 Chebyshev pre-smoothing from x = 0: k steps, leaving x_k in chx;
+// returns the buffer that holds the residual b - K_II x_k.
+template <typename T>
+static const T* cheb_pre(eq_ctx* ctx, const T* b) {
   CgWork<T>& w = work<T>(ctx);
   const int k = ctx->l0_degree;
   const double th = ctx->l0_theta, de = ctx->l0_delta, sg = th / de;
   double rho_prev = 1.0 / sg;
-  int cur = 0;
+  ChebStep<T> st[kMaxChebDegree];
   for (int m = 1; m <= k; m++) {
-    ChebStep<T> st;
-    st.check = (m == 1);
-    st.rin = (m == 1) ? b : w.chr;
-    st.rout = w.chr;
-    if (m == 1) { st.src_mode = 1; st.b = b; st.s0 = (T)(1.0 / th); }
-    else { st.src = w.chd[cur]; }
+    ChebStep<T>& c = st[m - 1];
+    c.check = (m == 1);
+    c.rin = (m == 1) ? b : w.chr[(m - 2) % 3];
+    c.rout = w.chr[(m - 1) % 3];
+    if (m == 1) { c.src_mode = 1; c.b = b; c.s0 = (T)(1.0 / th); }
+    else { c.src = w.chd[(m - 2) % 3]; }
     if (m == k) {
-      st.mode = 1;
-      st.write_x_center = (k == 1);
-      st.xout = w.chx;
+      c.mode = 1;
+      c.write_x_center = (k == 1);
+      c.xout = w.chx;
     } else {
       const double rho = 1.0 / (2.0 * sg - rho_prev);
-      st.mode = 0;
-      st.c1 = (T)(rho * rho_prev);
-      st.c2 = (T)(2.0 * rho / de);
+      c.mode = 0;
+      c.c1 = (T)(rho * rho_prev);
+      c.c2 = (T)(2.0 * rho / de);
       rho_prev = rho;
-      st.xsrc_center = (m == 1);
-      st.xin = w.chx;
-      st.xout = w.chx;
-      st.dout = w.chd[1 - cur];
-      cur = 1 - cur;
+      c.xsrc_center = (m == 1);
+      c.xin = w.chx;
+      c.xout = w.chx;
+      c.dout = w.chd[(m - 1) % 3];
     }
-    launch_cheb<T>(ctx->g, st, mu_of<T>(ctx), dinv_of<T>(ctx), ctx->part_rr, ctx->npart, ctx->sc, ctx->part_rho,
-                   ctx->s);
   }
+  cheb_launch<T>(ctx, st, k);
+  return w.chr[(k - 1) % 3];
 }
 
-// Level-0 Chebyshev post-smoothing from x = chx: k kernels; output c and
+// Level-0 Chebyshev post-smoothing from x = chx: k steps; output c and
 // partial <c, r>.
 template <typename T>
 static void cheb_post(eq_ctx* ctx, const T* b, T* c, const T* coarse) {
@@ -792,40 +814,38 @@ static void cheb_post(eq_ctx* ctx, const T* b, T* c, const T* coarse) {
   const int k = ctx->l0_degree;
   const double th = ctx->l0_theta, de = ctx->l0_delta, sg = th / de;
   double rho_prev = 1.0 / sg;
-  int cur = 0;
+  ChebStep<T> st[kMaxChebDegree];
   for (int m = 0; m < k; m++) {
-    ChebStep<T> st;
-    st.rin = (m == 0) ? b : w.chr;
-    st.src = (m == 0) ? w.chx : w.chd[cur];
+    ChebStep<T>& q = st[m];
+    q.rin = (m == 0) ? b : w.chr[(m - 1) % 3];
+    q.src = (m == 0) ? w.chx : w.chd[(m - 1) % 3];
     if (m == 0 && coarse) {
-      st.src_mode = 2;     // x0 = chx + P0 coarse, evaluated in the window load
-      st.coarse = coarse;
+      q.src_mode = 2;     // x0 = chx + P0 coarse, evaluated in the window load
+      q.coarse = coarse;
     }
     if (m == 0) {
-      st.c1 = T(0);
-      st.c2 = (T)(1.0 / th);
-      st.xsrc_center = 1;
+      q.c1 = T(0);
+      q.c2 = (T)(1.0 / th);
+      q.xsrc_center = 1;
     } else {
       const double rho = 1.0 / (2.0 * sg - rho_prev);
-      st.c1 = (T)(rho * rho_prev);
-      st.c2 = (T)(2.0 * rho / de);
+      q.c1 = (T)(rho * rho_prev);
+      q.c2 = (T)(2.0 * rho / de);
       rho_prev = rho;
-      st.xin = w.chx2;
+      q.xin = w.chx2;
     }
     if (m == k - 1) {
-      st.mode = 2;
-      st.xout = c;
-      st.rdot = b;
+      q.mode = 2;
+      q.xout = c;
+      q.rdot = b;
     } else {
-      st.mode = 0;
-      st.rout = w.chr;
-      st.dout = w.chd[1 - cur];
-      st.xout = w.chx2;
-      cur = 1 - cur;
+      q.mode = 0;
+      q.rout = w.chr[m % 3];
+      q.dout = w.chd[m % 3];
+      q.xout = w.chx2;
     }
-    launch_cheb<T>(ctx->g, st, mu_of<T>(ctx), dinv_of<T>(ctx), ctx->part_rr, ctx->npart, ctx->sc, ctx->part_rho,
-                   ctx->s);
   }
+  cheb_launch<T>(ctx, st, k);
 }
 
 template <typename T>
@@ -838,8 +858,8 @@ static void vcycle(eq_ctx* ctx, const T* r, T* c) {
   const T om0 = (T)L0.omega;
   T* x0 = ctx->l0_cheb ? w.chx : w.x0;
   if (ctx->l0_cheb) {
-    cheb_pre<T>(ctx, r);
-    launch_csr<T>(CSR_SPMV, L0.R, w.chr + g.NX, nullptr, nullptr, T(0), lvecs<T>(ctx->lev[1]).b, nullptr, ctx->sc,
+    const T* res = cheb_pre<T>(ctx, r);
+    launch_csr<T>(CSR_SPMV, L0.R, res + g.NX, nullptr, nullptr, T(0), lvecs<T>(ctx->lev[1]).b, nullptr, ctx->sc,
                   s);
   } else {
     launch_l0_down<T>(g, r, mu_of<T>(ctx), dinv_of<T>(ctx), om0, w.x0, w.res0, ctx->part_rr, ctx->npart, ctx->sc, s);
@@ -1264,6 +1284,11 @@ extern "C" int eq_set_solver(eq_ctx* ctx, const eq_solver_opts* o) {
     return fail(ctx, EQ_ENOTREADY, "learned guess: basis and network not trained (eq_learn_train)");
   if (o->guess == EQ_GUESS_GALERKIN && !ctx->learn.gal_ready)
     return fail(ctx, EQ_ENOTREADY, "Galerkin guess: no POD basis (eq_learn_set_basis)");
+  if (o->cheb_single != ctx->opts.cheb_single) {
+    // captured CG batches hold the other Chebyshev launch sequence
+    for (auto& kv : ctx->cg_graphs) (void)hipGraphExecDestroy(kv.second);
+    ctx->cg_graphs.clear();
+  }
   ctx->opts = *o;
   return EQ_OK;
 }
@@ -1778,8 +1803,8 @@ static void time_body(eq_ctx* ctx, int which) {
       // one middle Chebyshev step: window d, reads r, x, mu, dinv; writes r, d', x
       ChebStep<T> st;
       st.src = w.chd[0];
-      st.rin = w.chr;
-      st.rout = w.chr;
+      st.rin = w.chr[0];
+      st.rout = w.chr[0];
       st.c1 = T(0.5);
       st.c2 = T(1e-3);
       st.xin = w.chx;
@@ -1788,6 +1813,24 @@ static void time_body(eq_ctx* ctx, int which) {
       st.mode = 0;
       launch_cheb<T>(g, st, mu_of<T>(ctx), dinv_of<T>(ctx), ctx->part_rr, ctx->npart, ctx->sc, ctx->part_rho,
                      ctx->s);
+      break;
+    }
+    case EQ_TIME_CHEB2: {
+      // two middle Chebyshev steps in one launch: window d, reads r, x, mu, dinv; writes r'', d'', x
+      ChebStep<T> st[2];
+      for (int m = 0; m < 2; m++) {
+        st[m].src = w.chd[m];
+        st[m].rin = w.chr[m];
+        st[m].rout = w.chr[m + 1];
+        st[m].c1 = T(0.5);
+        st[m].c2 = T(1e-3);
+        st[m].xin = w.chx;
+        st[m].xout = w.chx;
+        st[m].dout = w.chd[m + 1];
+        st[m].mode = 0;
+      }
+      launch_cheb_pair<T>(g, st[0], st[1], mu_of<T>(ctx), dinv_of<T>(ctx), ctx->part_rr, ctx->npart, ctx->sc,
+                          ctx->part_rho, ctx->s);
       break;
     }
     default:
@@ -1838,17 +1881,17 @@ extern "C" int eq_time_kernel(eq_ctx* ctx, int which, int prec, int reps, double
     launch_fill<double>(ctx->N, ctx->w64.x, 1.0, s); launch_fill<double>(ctx->N, ctx->w64.r, 1.0, s);
     launch_fill<double>(ctx->N, ctx->w64.c, 1.0, s); launch_fill<double>(ctx->N, ctx->w64.u[0], 0.0, s);
   }
-  for (int k = 0; k < 2; k++) {
+  for (int k = 0; k < 3; k++) {
     if (prec == EQ_PREC_FP32) {
       launch_fill<float>(ctx->N, ctx->w32.chd[k], 1.0f, s);
-      launch_fill<float>(ctx->N, ctx->w32.chr, 1.0f, s);
-      launch_fill<float>(ctx->N, ctx->w32.chx, 1.0f, s);
+      launch_fill<float>(ctx->N, ctx->w32.chr[k], 1.0f, s);
     } else {
       launch_fill<double>(ctx->N, ctx->w64.chd[k], 1.0, s);
-      launch_fill<double>(ctx->N, ctx->w64.chr, 1.0, s);
-      launch_fill<double>(ctx->N, ctx->w64.chx, 1.0, s);
+      launch_fill<double>(ctx->N, ctx->w64.chr[k], 1.0, s);
     }
   }
+  if (prec == EQ_PREC_FP32) launch_fill<float>(ctx->N, ctx->w32.chx, 1.0f, s);
+  else launch_fill<double>(ctx->N, ctx->w64.chx, 1.0, s);
   CgScal h{};
   h.rho_prev = 1.0; h.reltol = -1.0; h.maxiter = INT_MAX;
   HIPCHK(ctx, hipMemcpyAsync(ctx->sc, &h, sizeof(CgScal), hipMemcpyHostToDevice, s));

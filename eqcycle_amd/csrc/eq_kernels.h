@@ -129,6 +129,12 @@ struct ChebStep {
 template <typename T>
 void launch_cheb(const Geom& g, const ChebStep<T>& c, const T* mu, const T* dinv, const double* part_rr, int nrr,
                  CgScal* sc, double* part_rho, hipStream_t s);
+// Steps a and b (a in mode 0, b reading a's outputs) in one launch; b's
+// outputs are written, a's stay on chip except a.xout when b is in mode 1.
+// b.rout and b.dout must not alias a.rin or a.src.
+template <typename T>
+void launch_cheb_pair(const Geom& g, const ChebStep<T>& a, const ChebStep<T>& b, const T* mu, const T* dinv,
+                      const double* part_rr, int nrr, CgScal* sc, double* part_rho, hipStream_t s);
 
 struct DevCsr {
   int64_t n = 0, m = 0, nnz = 0;

@@ -486,6 +486,34 @@ struct OpL0Post {
 //   mode 1: write r only (residual after pre-smoothing; x = w if write_x_center)
 //   mode 2: x_new as in mode 0 is the smoother output c; partial <c, rdot>
 // The first kernel of a V-cycle carries the cg! convergence test.
+// Window value of a level-0 Chebyshev step's source at node n = (I, J):
+// src_mode 0: src[n]; 1: s0 D^-1 b (the first iterate, formed on the fly);
+// 2: src[n] + P0 coarse (bilinear interpolation of the element-vertex
+// correction).  The fault row is 0.
+template <typename T>
+__device__ __forceinline__ T cheb_source(const T* src, const T* b, const T* dinv, T s0, int src_mode,
+                                         const T* coarse, int nelx, int nely, const T* gx, int64_t n, int I,
+                                         int J) {
+  if (J == 0) return T(0);
+  if (src_mode == 1) return s0 * dinv[n] * b[n];
+  if (src_mode == 2) {
+    const int ex = min(I >> 2, nelx - 1), i = I - 4 * ex;
+    const int ey = min(J >> 2, nely - 1), j = J - 4 * ey;
+    const T ax1 = T(0.5) * (T(1) + gx[i]), ax0 = T(1) - ax1;
+    const T ay1 = T(0.5) * (T(1) + gx[j]), ay0 = T(1) - ay1;
+    const int nvx = nelx + 1;
+    T v = T(0);
+    if (ey > 0) {
+      const T* c0 = coarse + (int64_t)(ey - 1) * nvx + ex;
+      v += ay0 * (ax0 * c0[0] + ax1 * c0[1]);
+    }
+    const T* c1 = coarse + (int64_t)ey * nvx + ex;
+    v += ay1 * (ax0 * c1[0] + ax1 * c1[1]);
+    return src[n] + v;
+  }
+  return src[n];
+}
+
 template <typename T>
 struct OpCheb {
   using Pre = PreNone;
@@ -530,24 +558,7 @@ struct OpCheb {
   int nelx, nely;
   T gx[5];
   __device__ T load(const Pre&, int64_t n, int I, int J) const {
-    if (J == 0) return T(0);
-    if (src_mode == 1) return s0 * dinv[n] * b[n];
-    if (src_mode == 2) {
-      const int ex = min(I >> 2, nelx - 1), i = I - 4 * ex;
-      const int ey = min(J >> 2, nely - 1), j = J - 4 * ey;
-      const T ax1 = T(0.5) * (T(1) + gx[i]), ax0 = T(1) - ax1;
-      const T ay1 = T(0.5) * (T(1) + gx[j]), ay0 = T(1) - ay1;
-      const int nvx = nelx + 1;
-      T v = T(0);
-      if (ey > 0) {
-        const T* c0 = coarse + (int64_t)(ey - 1) * nvx + ex;
-        v += ay0 * (ax0 * c0[0] + ax1 * c0[1]);
-      }
-      const T* c1 = coarse + (int64_t)ey * nvx + ex;
-      v += ay1 * (ax0 * c1[0] + ax1 * c1[1]);
-      return src[n] + v;
-    }
-    return src[n];
+    return cheb_source(src, b, dinv, s0, src_mode, coarse, nelx, nely, gx, n, I, J);
   }
   struct Ext { T rin, dinv, xin, rdot; };
   __device__ void fetch(const Pre&, int64_t n, int, int J, Ext& e) const {
@@ -1431,6 +1442,299 @@ void launch_cheb(const Geom& g, const ChebStep<T>& c, const T* mu, const T* dinv
   hipLaunchKernelGGL((tile_kernel<T, OpCheb<T>>), tile_grid(g), dim3(TBLOCK), 0, s, g, op);
 }
 
+// --------------------------------------------------------------------------
+// Two consecutive level-0 Chebyshev steps in one launch
+// --------------------------------------------------------------------------
+// The second step needs the first step's new direction d1 on the tile and on
+// the ring of nodes its stencil reaches: one element beyond the tile along the
+// owned rows (4 columns before, 1 after) and along the owned columns (4 rows
+// before, 1 after).  The first step is therefore evaluated on the tile and
+// that ring, from a source window one element wider again (columns I0-8 ..
+// I0+TILE_I+4, rows J0-8 .. J0+TILE_J+4); d1 stays in LDS, r1 and x1 of the
+// owned nodes in registers, and the second step runs on the tile from there.
+// HBM traffic per node is that of one step (source, mu, r, D^-1 and x in;
+// the second step's outputs out); the ring and the wider window are re-read
+// from L2.  Each node value is formed by the same operations in the same
+// order as in two OpCheb launches, so the results are those of two launches.
+constexpr int FW_I = TILE_I + 13;
+constexpr int FW_J = TILE_J + 13;
+constexpr int FB_I = TILE_I + 5;               // box of the first step: columns I0-4 .. I0+TILE_I
+constexpr int FB_J = TILE_J + 5;               // rows J0-4 .. J0+TILE_J
+constexpr int FW_LOADS = (FW_J * FW_I + TBLOCK - 1) / TBLOCK;
+constexpr int FB_STRIPS_I = (FB_I + 3) / 4;    // 4-node row strips per box row
+constexpr int FB_STRIPS_J = (FB_J + 3) / 4;    // 4-node column strips per box column
+constexpr int F_RING = 2 * TILE_I + 5 * (TILE_J / 4);   // column strips of the ring
+static_assert(F_RING <= TBLOCK, "one ring strip per thread");
+static_assert(FB_STRIPS_J == TILE_J / 4 + 2, "ring rows: one strip below, one above");
+
+template <typename T>
+struct ChebPair {
+  // first step (OpCheb, mode 0)
+  const T* src;
+  const T* b;
+  T s0;
+  int src_mode;
+  const T* coarse;
+  int nelx, nely;
+  T gx[5];
+  const T* rin;
+  T c1a, c2a;
+  const T* xin;
+  int xsrc_center;
+  T* x1out;          // second step in mode 1: the first step's x goes here
+  // second step (OpCheb with the first step's outputs as inputs)
+  T c1b, c2b;
+  int mode;
+  T* rout;
+  T* dout;
+  T* xout;
+  const T* rdot;
+  // both
+  const T* mu;
+  const T* dinv;
+  int check;
+  const double* part_rr;
+  int nrr;
+  CgScal* sc;
+  double* part0;
+};
+
+template <typename T>
+__global__ __launch_bounds__(TBLOCK) void cheb_pair_kernel(Geom g, ChebPair<T> op) {
+  __shared__ T sw[FW_J * FW_I];    // first step's source
+  __shared__ T smw[FW_J * FW_I];   // mu
+  __shared__ T sbx[FB_J * FB_I];   // x-direction terms: first step on the box, then second step on the tile
+  __shared__ T sd1[FB_J * FW_I];   // d1 on the box, window column numbering
+  if (op.sc->done) return;         // block-uniform, as OpCheb::prepare
+  if (op.check) {
+    const double rr = reduce_parts(op.part_rr, op.nrr);
+    const double resid = sqrt(rr);
+    const bool fin = (resid <= op.sc->reltol) || (op.sc->iter >= op.sc->maxiter);
+    if (threadIdx.x == 0 && blockIdx.x == 0 && blockIdx.y == 0) {
+      op.sc->resid = resid;
+      if (fin) op.sc->done = 1;
+    }
+    if (fin) return;
+  }
+  const int I0 = blockIdx.x * TILE_I, J0 = blockIdx.y * TILE_J;
+  const int tid = threadIdx.x;
+  const int NX = g.NX, NY = g.NY;
+  // owned column strip (both steps): column c3, rows Js3 .. Js3+3
+  const int c3 = tid % TILE_I, q3 = tid / TILE_I;
+  const int I3 = I0 + c3, Js3 = J0 + 4 * q3;
+  // ring column strip of the first step (box column hc, box strip hk), if any
+  int hc = -1, hk = 0;
+  if (tid < 2 * TILE_I) {
+    hc = 4 + tid % TILE_I;
+    hk = tid < TILE_I ? 0 : FB_STRIPS_J - 1;
+  } else if (tid < F_RING) {
+    const int h = tid - 2 * TILE_I, col = h / (TILE_J / 4);
+    hc = col < 4 ? col : FB_I - 1;
+    hk = 1 + h % (TILE_J / 4);
+  }
+  // every global load of the block first
+  T o_r[4], o_di[4], o_x[4], o_rd[4], h_r[4], h_di[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    o_r[j] = o_di[j] = o_x[j] = o_rd[j] = h_r[j] = h_di[j] = T(0);
+    const int J = Js3 + j;
+    if (I3 < NX && J > 0 && J < NY) {
+      const int64_t n = (int64_t)J * NX + I3;
+      o_r[j] = op.rin[n];
+      o_di[j] = op.dinv[n];
+      if (!op.xsrc_center) o_x[j] = op.xin[n];
+      if (op.mode == 2) o_rd[j] = op.rdot[n];
+    }
+  }
+  if (hc >= 0) {
+    const int I = I0 - 4 + hc;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int J = J0 - 4 + 4 * hk + j;
+      if (4 * hk + j < FB_J && I >= 0 && I < NX && J > 0 && J < NY) {
+        const int64_t n = (int64_t)J * NX + I;
+        h_r[j] = op.rin[n];
+        h_di[j] = op.dinv[n];
+      }
+    }
+  }
+  {
+    T ur[FW_LOADS], mr[FW_LOADS];
+#pragma unroll
+    for (int t = 0; t < FW_LOADS; t++) {
+      const int k = tid + t * TBLOCK;
+      const int r = k / FW_I, c = k - r * FW_I;
+      const int I = I0 - 8 + c, J = J0 - 8 + r;
+      ur[t] = 0;
+      mr[t] = 0;
+      if (k < FW_J * FW_I && I >= 0 && I < NX && J >= 0 && J < NY) {
+        const int64_t n = (int64_t)J * NX + I;
+        ur[t] = cheb_source(op.src, op.b, op.dinv, op.s0, op.src_mode, op.coarse, op.nelx, op.nely, op.gx, n, I,
+                            J);
+        mr[t] = op.mu[n];
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < FW_LOADS; t++) {
+      const int k = tid + t * TBLOCK;
+      if (k < FW_J * FW_I) {
+        sw[k] = ur[t];
+        smw[k] = mr[t];
+      }
+    }
+  }
+  __syncthreads();
+  // first step, x direction, on every box row
+  for (int sidx = tid; sidx < FB_J * FB_STRIPS_I; sidx += TBLOCK) {
+    const int rb = sidx / FB_STRIPS_I, k = sidx - rb * FB_STRIPS_I;
+    const int J = J0 - 4 + rb, Is = I0 - 4 + 4 * k;
+    if (J < 0 || J >= NY || Is < 0 || Is >= NX) continue;
+    const int sg = Is >> 2;
+    const int wr = rb + 4, wc = 4 * k + 4;
+    T o[5] = {0, 0, 0, 0, 0};
+    if (sg < g.nelx) seg_all<T, 1>(g, sw + wr * FW_I + wc, smw + wr * FW_I + wc, o);
+    if (sg > 0) o[0] += seg_last<T, 1>(g, sw + wr * FW_I + wc - 4, smw + wr * FW_I + wc - 4);
+    const T cx = (T)(g.c1 * wline(g, J, NY));
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+      if (4 * k + i < FB_I) sbx[rb * FB_I + 4 * k + i] = cx * o[i];
+  }
+  __syncthreads();
+  // first step, y direction and update, on one box column strip: r1 = r - K d,
+  // d1 = c1 d + c2 D^-1 r1 (0 on the fault row and outside the mesh)
+  T r1[4], x1[4];
+  auto step1 = [&](int cb, int kk, const T* rr, const T* di, bool owned) {
+    const int I = I0 - 4 + cb, Js = J0 - 4 + 4 * kk;
+    const int wc = cb + 4, wr = 4 * kk + 4;
+    const bool inI = I >= 0 && I < NX;
+    T o[5] = {0, 0, 0, 0, 0};
+    if (inI && Js >= 0 && Js < NY) {
+      const int t = Js >> 2;
+      if (t < g.nely) seg_all<T, FW_I>(g, sw + wr * FW_I + wc, smw + wr * FW_I + wc, o);
+      if (t > 0) o[0] += seg_last<T, FW_I>(g, sw + (wr - 4) * FW_I + wc, smw + (wr - 4) * FW_I + wc);
+    }
+    const T cy = inI ? (T)(g.c2 * wline(g, I, NX)) : T(0);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int rb = 4 * kk + j;
+      if (rb >= FB_J) break;
+      const int J = Js + j;
+      T dn = T(0);
+      if (owned) {
+        r1[j] = T(0);
+        x1[j] = T(0);
+      }
+      if (inI && J > 0 && J < NY) {
+        const T ku = sbx[rb * FB_I + cb] + cy * o[j];
+        const T wcv = sw[(rb + 4) * FW_I + wc];
+        const T r = rr[j] - ku;
+        dn = op.c1a * wcv + op.c2a * di[j] * r;
+        if (owned) {
+          r1[j] = r;
+          x1[j] = (op.xsrc_center ? wcv : o_x[j]) + dn;
+        }
+      }
+      sd1[rb * FW_I + wc] = dn;
+    }
+  };
+  step1(c3 + 4, q3 + 1, o_r, o_di, true);
+  if (hc >= 0) step1(hc, hk, h_r, h_di, false);
+  __syncthreads();
+  // second step, x direction, on the tile
+  {
+    constexpr int NSTRIP = TILE_I / 4;
+    const int r = tid / NSTRIP, q = tid - r * NSTRIP;
+    const int J = J0 + r, Is = I0 + 4 * q;
+    if (J < NY && Is < NX) {
+      const int sg = Is >> 2;
+      const int dr = r + 4, wc = 4 * q + 8;
+      T o[5] = {0, 0, 0, 0, 0};
+      if (sg < g.nelx) seg_all<T, 1>(g, sd1 + dr * FW_I + wc, smw + (dr + 4) * FW_I + wc, o);
+      if (sg > 0) o[0] += seg_last<T, 1>(g, sd1 + dr * FW_I + wc - 4, smw + (dr + 4) * FW_I + wc - 4);
+      const T cx = (T)(g.c1 * wline(g, J, NY));
+#pragma unroll
+      for (int i = 0; i < 4; i++) sbx[r * TILE_I + 4 * q + i] = cx * o[i];
+    }
+  }
+  __syncthreads();
+  // second step, y direction and the OpCheb epilogue of its mode
+  double acc0 = 0.0;
+  if (I3 < NX && Js3 < NY) {
+    const int t = Js3 >> 2;
+    const int wc = c3 + 8, dr = 4 * q3 + 4;
+    T o[5] = {0, 0, 0, 0, 0};
+    if (t < g.nely) seg_all<T, FW_I>(g, sd1 + dr * FW_I + wc, smw + (dr + 4) * FW_I + wc, o);
+    if (t > 0) o[0] += seg_last<T, FW_I>(g, sd1 + (dr - 4) * FW_I + wc, smw + dr * FW_I + wc);
+    const T cy = (T)(g.c2 * wline(g, I3, NX));
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int J = Js3 + j;
+      if (J >= NY) break;
+      const int64_t n = (int64_t)J * NX + I3;
+      if (J == 0) {
+        if (op.mode == 0) { op.rout[n] = T(0); op.dout[n] = T(0); op.xout[n] = T(0); }
+        else if (op.mode == 1) { op.rout[n] = T(0); op.x1out[n] = T(0); }
+        else op.xout[n] = T(0);
+        continue;
+      }
+      const T ku = sbx[(4 * q3 + j) * TILE_I + c3] + cy * o[j];
+      const T wcv = sd1[(dr + j) * FW_I + wc];
+      const T r = r1[j] - ku;
+      if (op.mode == 1) {
+        op.rout[n] = r;
+        op.x1out[n] = x1[j];
+        continue;
+      }
+      const T dn = op.c1b * wcv + op.c2b * o_di[j] * r;
+      const T xn = x1[j] + dn;
+      op.xout[n] = xn;
+      if (op.mode == 0) {
+        op.rout[n] = r;
+        op.dout[n] = dn;
+      } else {
+        acc0 += (double)xn * (double)o_rd[j];
+      }
+    }
+  }
+  const double s = block_sum(acc0);
+  if (threadIdx.x == 0) op.part0[blockIdx.y * gridDim.x + blockIdx.x] = s;
+}
+
+template <typename T>
+void launch_cheb_pair(const Geom& g, const ChebStep<T>& a, const ChebStep<T>& b, const T* mu, const T* dinv,
+                      const double* part_rr, int nrr, CgScal* sc, double* part_rho, hipStream_t s) {
+  ChebPair<T> op{};
+  op.src = a.src;
+  op.b = a.b;
+  op.s0 = a.s0;
+  op.src_mode = a.src_mode;
+  op.coarse = a.coarse;
+  op.nelx = g.nelx;
+  op.nely = g.nely;
+  for (int p = 0; p < 5; p++) op.gx[p] = (T)g.x[p];
+  op.rin = a.rin;
+  op.c1a = a.c1;
+  op.c2a = a.c2;
+  op.xin = a.xin;
+  op.xsrc_center = a.xsrc_center;
+  op.x1out = a.xout;
+  op.c1b = b.c1;
+  op.c2b = b.c2;
+  op.mode = b.mode;
+  op.rout = b.rout;
+  op.dout = b.dout;
+  op.xout = b.xout;
+  op.rdot = b.rdot;
+  op.mu = mu;
+  op.dinv = dinv;
+  op.check = a.check;
+  op.part_rr = part_rr;
+  op.nrr = nrr;
+  op.sc = sc;
+  op.part0 = part_rho;
+  hipLaunchKernelGGL((cheb_pair_kernel<T>), tile_grid(g), dim3(TBLOCK), 0, s, g, op);
+}
+
 template <typename T, int MODE>
 static void csr_dispatch(const DevCsr& M, const T* val, const T* x, const T* b, const T* dinv, T omega, T* y, T* xout,
                          const CgScal* sc, hipStream_t s) {
@@ -1630,6 +1934,8 @@ void launch_scale_mask(int64_t n, int nx, const double* x, double sign, double* 
                               hipStream_t);                                                                         \
   template void launch_dense<T>(int64_t, const T*, const T*, T*, const CgScal*, hipStream_t);                      \
   template void launch_csr_up<T>(const DevCsr&, const DevCsr&, const T*, const T*, T*, const CgScal*, hipStream_t); \
+  template void launch_cheb_pair<T>(const Geom&, const ChebStep<T>&, const ChebStep<T>&, const T*, const T*,       \
+                                    const double*, int, CgScal*, double*, hipStream_t);                           \
   template void launch_cheb<T>(const Geom&, const ChebStep<T>&, const T*, const T*, const double*, int, CgScal*,   \
                                double*, hipStream_t);                                                              \
   template void launch_fill<T>(int64_t, T*, T, hipStream_t);

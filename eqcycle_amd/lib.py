@@ -33,7 +33,8 @@ EXPORTS = [
     "eq_learn_gram", "eq_learn_set_basis", "eq_learn_train", "eq_learn_loss_grad", "eq_learn_get",
     "eq_learn_predict", "eq_learn_norms", "eq_learn_galerkin",
 ]
-EQ_TIME_KU, EQ_TIME_CG_DIR, EQ_TIME_VCYCLE, EQ_TIME_CG_ITER, EQ_TIME_NEWMARK, EQ_TIME_CHEB, EQ_TIME_COPY = range(7)
+EQ_TIME_KU, EQ_TIME_CG_DIR, EQ_TIME_VCYCLE, EQ_TIME_CG_ITER, EQ_TIME_NEWMARK, EQ_TIME_CHEB, EQ_TIME_COPY, \
+    EQ_TIME_CHEB2 = range(8)
 
 dp = C.POINTER(C.c_double)
 i64p = C.POINTER(C.c_int64)
@@ -58,7 +59,8 @@ class ParamsDesc(C.Structure):
 
 class SolverOpts(C.Structure):
     _fields_ = [("tol", C.c_double), ("maxiter", C.c_int64), ("pc", C.c_int), ("prec", C.c_int),
-                ("guess", C.c_int), ("record_snapshots", C.c_int), ("graphs", C.c_int)]
+                ("guess", C.c_int), ("record_snapshots", C.c_int), ("graphs", C.c_int),
+                ("cheb_single", C.c_int)]
 
 
 class StepInfo(C.Structure):

@@ -159,6 +159,8 @@ typedef struct {
   int guess;               /* EQ_GUESS_PREVIOUS (src/main.jl:241), _LEARNED or _GALERKIN */
   int record_snapshots;    /* copy fault fields at events (src/main.jl:361-422) */
   int graphs;              /* 1: CG iteration batches replay captured HIP graphs (same kernels) */
+  int cheb_single;         /* 1: one launch per level-0 Chebyshev step; 0 (default): two steps per
+                              launch where they follow each other (same results) */
 } eq_solver_opts;
 int eq_set_solver(eq_ctx *ctx, const eq_solver_opts *opts);
 
@@ -245,13 +247,14 @@ int eq_learn_galerkin(eq_ctx *ctx, const double *x, const double *b, double *x_o
 
 /* ---- measurement hook (no reference counterpart) ------------------------ */
 enum { EQ_TIME_KU = 0, EQ_TIME_CG_DIR = 1, EQ_TIME_VCYCLE = 2, EQ_TIME_CG_ITER = 3, EQ_TIME_NEWMARK = 4,
-       EQ_TIME_CHEB = 5, EQ_TIME_COPY = 6 };
+       EQ_TIME_CHEB = 5, EQ_TIME_COPY = 6, EQ_TIME_CHEB2 = 7 };
 /* Average device time (ms) of one launch of the given kernel group over reps
  * back-to-back launches, bracketed by HIP events on the context's stream.
  * Runs on scratch vectors; EQ_TIME_NEWMARK advances a copy of the state and
  * must not be called in the middle of a run.  EQ_TIME_COPY copies a 512 MiB
  * FP64 buffer into another (allocated for the call, then freed): a kernel of
- * known traffic that calibrates hardware byte counters. */
+ * known traffic that calibrates hardware byte counters.  EQ_TIME_CHEB is one
+ * level-0 Chebyshev step, EQ_TIME_CHEB2 two steps in one launch. */
 int eq_time_kernel(eq_ctx *ctx, int which, int prec, int reps, double *avg_ms);
 
 #ifdef __cplusplus

@@ -125,6 +125,48 @@ def test_cg_mixed_precision(mesh):
     assert np.linalg.norm(x - xs) / np.linalg.norm(xs) < 1e-6
 
 
+@pytest.mark.parametrize("degree", [2, 3, 5, 6])
+def test_cheb_pairs_equal_single_steps(degree):
+    """Two level-0 Chebyshev steps per launch give the V-cycle and the CG
+    solves of one launch per step bit for bit (FP64 and FP32 V-cycles), on a
+    mesh with several tiles in both directions and partial edge tiles."""
+    from eqcycle_amd.api import EqContext
+    P = set_parameters(small(40, 13, material="trapezoid", lognormal_sigma=0.3))
+    print(f"[gpu] cheb pairs degree {degree} nglob={P.nglob}", flush=True)
+    ctx = EqContext(P, amg={"hierarchy": "pmg", "degree": degree})
+    b = normal(41, P.nglob - P.FltNglob)
+    out = {}
+    for pairs in (True, False):
+        ctx.set_solver(tol=1e-10, pc="amg", cheb_pairs=pairs)
+        out[pairs] = [ctx.amg_ldiv(b, prec) for prec in ("fp64", "fp32")]
+        for prec in ("fp64", "f32pc"):
+            x, it, rr, ok = ctx.cg(np.zeros_like(b), b, tol=1e-10, pc="amg", prec=prec)
+            assert ok
+            out[pairs].append(x)
+    for k, (a, c) in enumerate(zip(out[True], out[False])):
+        assert np.array_equal(a, c), k
+
+
+def test_cheb_pairs_run_equals_single_steps():
+    """main(P) through the first event with paired and with single Chebyshev
+    launches (CG batches replayed as graphs): identical histories and state."""
+    from eqcycle_amd.api import EqContext
+    P = set_parameters(small(32, 8, material="trapezoid", total_years=3))
+    ctx = EqContext(P, amg={"hierarchy": "pmg", "degree": 5})
+    res = {}
+    for pairs in (True, False):
+        ctx.set_solver(tol=1e-8, pc="amg", prec="f32pc", cheb_pairs=pairs)
+        ctx.state_init()
+        R = ctx.run(snapshots=False, events=True)
+        res[pairs] = (R, ctx.state_get())
+    (Ra, sa), (Rb, sb) = res[True], res[False]
+    assert len(Ra["tStart"]) >= 1
+    for k in ("time_", "Vfmax", "pcg_iters", "tStart", "tEnd", "delfafter"):
+        assert np.array_equal(Ra[k], Rb[k]), k
+    for k in ("d", "v", "psi", "tau"):
+        assert np.array_equal(sa[k], sb[k]), k
+
+
 def test_amg_ldiv_is_spd(mesh):
     P, ctx, _, kni = mesh
     n = P.nglob - P.FltNglob
@@ -450,3 +492,42 @@ def test_consecutive_runs_continue_one_run():
     for k in cat:
         assert np.array_equal(cat[k], whole[k]), k
     assert np.array_equal(np.concatenate([p["delfafter"] for p in parts]), whole["delfafter"])
+
+
+def test_nr_failure_stops_run_like_oracle(oracle):
+    """A slip rate the rate-state search cannot follow (free slip velocity of
+    ~1e10 Pa / FltZ at the surface node: the first Newton step from tau = 0
+    overflows exp, src/NRsearch.jl:63-94) stops the run with NR SEARCH FAILED
+    at the same fault location as the oracle's FBC! (src/NRsearch.jl:15-54:
+    the first failing node in loop order; the surface node's displacement
+    reaches its neighbour through K in the same step);
+    keep_on_nr_failure returns the results up to that step."""
+    import re
+    from eqcycle_amd.api import EqContext, EqError
+    P = set_parameters(small(32, 8, material="trapezoid", total_years=3))
+    F = P.FltNglob
+    ctx = EqContext(P, amg=True)
+    ctx.set_solver(tol=1e-12, pc="amg")
+    ctx.state_init()
+    s = ctx.state_get()
+    v = s["v"].copy()
+    j = F - 1
+    v[P.iFlt[j] - 1] = 0.5 * 1e10 / P.FltZ[j]
+    sim = oracle.Simulation(P, solver="direct")
+    sim.d, sim.v, sim.a = s["d"].copy(), v.copy(), s["a"].copy()
+    sim.psi, sim.isolver, sim.t, sim.dt = s["psi"].copy(), 2, 1e7, P.dt0
+    with pytest.raises(FloatingPointError) as ei:
+        sim.step()
+    loc = int(re.search(r"fault node (\d+)$", str(ei.value)).group(1))
+    assert F - 2 <= loc <= F
+    kw = dict(d=s["d"], v=v, a=s["a"], psi=s["psi"], tau=s["tau"], t=1e7, dt=P.dt0, isolver=2)
+    ctx.state_set(**kw)
+    R = ctx.run(max_steps=5, snapshots=False, keep_on_nr_failure=True)
+    assert R["error"] is not None and R["error"].endswith(f"(fault location {loc})")
+    assert R["n_steps"] == 0
+    ctx.state_set(**kw)
+    with pytest.raises(EqError, match=f"NR SEARCH FAILED! \\(fault location {loc}\\)"):
+        ctx.step()
+    # an ordinary run reports no error
+    ctx.state_init()
+    assert ctx.run(max_steps=3, snapshots=False, keep_on_nr_failure=True)["error"] is None
