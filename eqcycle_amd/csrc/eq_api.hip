@@ -114,11 +114,13 @@ struct eq_ctx {
   double l0_theta = 1, l0_delta = 1;  // Chebyshev interval centre / half-width of D^-1 K_II
   // step records: slot 0 serves single steps, a batch of dynamic steps uses
   // slots 0..kBatch-1 (status, fault rows of d / v / tau, receiver samples)
-  static constexpr int kBatch = 64;
+  static constexpr int kBatch = 256;
   StepStatus* st = nullptr;           // [kBatch]
   StepStatus* st_host = nullptr;      // [kBatch], pinned
   double* rec_fault = nullptr;        // [kBatch x 3 x NX]
+  double* rec_fault_host = nullptr;   // pinned, same shape
   int64_t* seis_idx = nullptr;        // [seis_cap]
+  int* seis_tile = nullptr;           // [seis_cap] tile of each receiver
   double* rec_seis = nullptr;         // [kBatch x 3 x seis_cap]
   double* rec_seis_host = nullptr;    // pinned
   int64_t seis_cap = 0;
@@ -154,6 +156,7 @@ struct eq_ctx {
     if (sc_host) (void)hipHostFree(sc_host);
     if (st_host) (void)hipHostFree(st_host);
     if (rec_seis_host) (void)hipHostFree(rec_seis_host);
+    if (rec_fault_host) (void)hipHostFree(rec_fault_host);
     if (vpart_host) (void)hipHostFree(vpart_host);
     if (nrflags_host) (void)hipHostFree(nrflags_host);
     if (ev_spin) (void)hipEventDestroy(ev_spin);
@@ -342,6 +345,8 @@ extern "C" int eq_create(const eq_mesh_desc* mesh, eq_ctx** out) {
       hipHostMalloc((void**)&c->vpart_host, sizeof(double) * eq_ctx::kBatch * 2 * nbx, hipHostMallocDefault) !=
           hipSuccess ||
       hipHostMalloc((void**)&c->nrflags_host, sizeof(int) * 2 * eq_ctx::kBatch, hipHostMallocDefault) !=
+          hipSuccess ||
+      hipHostMalloc((void**)&c->rec_fault_host, sizeof(double) * eq_ctx::kBatch * 3 * c->NX, hipHostMallocDefault) !=
           hipSuccess) {
     *out = ctx.release();
     return fail(*out, EQ_NOMEM, "pinned allocation failed");
@@ -481,8 +486,10 @@ extern "C" int eq_set_params(eq_ctx* ctx, const eq_params_desc* p) {
   if (nseis > 31) return fail(ctx, EQ_EINVAL, "at most 31 receivers (out_seis)");
   if (nseis > ctx->seis_cap) {
     ctx->seis_idx = ctx->alloc<int64_t>(nseis);
+    ctx->seis_tile = ctx->alloc<int>(nseis);
     ctx->rec_seis = ctx->alloc<double>((int64_t)eq_ctx::kBatch * 3 * nseis);
     NEED(ctx->seis_idx);
+    NEED(ctx->seis_tile);
     NEED(ctx->rec_seis);
     if (ctx->rec_seis_host) (void)hipHostFree(ctx->rec_seis_host);
     ctx->rec_seis_host = nullptr;
@@ -493,6 +500,14 @@ extern "C" int eq_set_params(eq_ctx* ctx, const eq_params_desc* p) {
   }
   if (nseis > 0) {
     if (int rc = h2d(ctx, ctx->seis_idx, ctx->out_seis.data(), nseis)) return rc;
+    std::vector<int> tiles(nseis);
+    const int ntx = (int)tile_grid(ctx->g).x;
+    for (int64_t q = 0; q < nseis; q++) {
+      const int64_t n = ctx->out_seis[q];
+      tiles[q] = (int)(n / NX / TILE_J) * ntx + (int)(n % NX / TILE_I);
+    }
+    if (int rc = h2d(ctx, ctx->seis_tile, tiles.data(), nseis)) return rc;
+    if (int rc = sync_check(ctx)) return rc;
   }
   // device copies
   double* blk = ctx->alloc<double>(11 * (int64_t)NX + NY + NX);
@@ -1525,7 +1540,7 @@ extern "C" int eq_step(eq_ctx* ctx, eq_step_info* info) {
 // the status of every step from the records; the last step taken gets the
 // step tail kernel, whose status (dtevol!, regime) carries the loop on.
 // Returns the number of steps taken in *m (>= 1).
-static int dyn_batch(eq_ctx* ctx, double total_time, int64_t max_more, int* m) {
+static int dyn_batch(eq_ctx* ctx, double total_time, int64_t max_more, int* m, StepStatus* tail_expect) {
   int k = 0;
   for (double tt = ctx->t; k < eq_ctx::kBatch && k < max_more && tt < total_time; k++) tt += ctx->dt;
   if (k == 0) k = 1;
@@ -1546,6 +1561,7 @@ static int dyn_batch(eq_ctx* ctx, double total_time, int64_t max_more, int* m) {
     rec.vpart_out = ctx->vpart + (int64_t)q * 2 * nbx;
     rec.cap_fault = ctx->rec_fault + (int64_t)q * 3 * NX;
     rec.seis_idx = ctx->seis_idx;
+    rec.seis_tile = ctx->seis_tile;
     rec.nseis = (int)ns;
     rec.cap_seis = ns > 0 ? ctx->rec_seis + (int64_t)q * 3 * ns : nullptr;
     launch_newmark(ctx->g, ctx->fp, ctx->fs, ctx->mu, ctx->M, src[0], src[1], src[2], dst[0], dst[1], dst[2],
@@ -1584,19 +1600,26 @@ static int dyn_batch(eq_ctx* ctx, double total_time, int64_t max_more, int* m) {
     std::swap(ctx->v, ctx->v2);
     std::swap(ctx->a, ctx->a2);
   }
-  // the step tail on the last step taken: dtevol! and the regime switch
+  // the step tail on the last step taken: dtevol! and the regime switch; its
+  // status comes back with the caller's next synchronisation and must equal
+  // the one rebuilt here from the record (dyn_tail_check)
   StepCapture none{};
   launch_step_tail(ctx->fp, ctx->d, ctx->v, ctx->a, ctx->fs.tau, ctx->dt, dtmin, 2, ctx->st + (n - 1),
                    ctx->nrflags + 2 * (n - 1), none, s);
-  const StepStatus mine = ctx->st_host[n - 1];
+  *tail_expect = ctx->st_host[n - 1];
   HIPCHK(ctx, hipMemcpyAsync(ctx->st_host + (n - 1), ctx->st + (n - 1), sizeof(StepStatus), hipMemcpyDeviceToHost,
                              s));
-  if (int rc = spin_sync(ctx)) return rc;
+  *m = n;
+  return EQ_OK;
+}
+
+// After the synchronisation that follows dyn_batch: the step tail's status
+// against the one rebuilt from the last step's record.
+static int dyn_tail_check(eq_ctx* ctx, int n, const StepStatus& mine) {
   const StepStatus& tail = ctx->st_host[n - 1];
   if (tail.isolver_next != mine.isolver_next || tail.hypo != mine.hypo || tail.nr_flag != mine.nr_flag ||
       !(tail.Vfmax == mine.Vfmax || (std::isnan(tail.Vfmax) && std::isnan(mine.Vfmax))))
     return fail(ctx, EQ_HIP_ERR, "dynamic batch: step record disagrees with the step tail");
-  *m = n;
   return EQ_OK;
 }
 
@@ -1618,21 +1641,58 @@ extern "C" int eq_run(eq_ctx* ctx, double total_time, int64_t max_steps, eq_resu
   int& slipstart = B.slipstart;
   if ((int64_t)B.delfref.size() != NX) B.delfref.assign(NX, 0.0);
   std::vector<double>& delfref = B.delfref;
-  std::vector<double> fault(3 * NX), tmp(NX);
+  std::vector<double> tmp(NX);
   const size_t nseis = ctx->out_seis.size();
   R->n_steps = R->n_start = R->n_end = R->n_is = R->n_seis = R->n_rec = 0;
   R->n_qs = R->n_dyn = R->total_pcg_iters = R->nr_noconv = 0;
   R->wall_qs = R->wall_dyn = 0;
-  // fault rows of d, v and tau after the step of record slot q
-  auto fault_fields = [&](int q) -> int {
-    if (int rc = d2h(ctx, fault.data(), ctx->rec_fault + (int64_t)q * 3 * NX, 3 * NX)) return rc;
-    return sync_check(ctx);
+  // What the bookkeeping of a finished step (time t, maximum slip rate Vfmax)
+  // needs from its record, with the bookkeeping state S advanced as the
+  // bookkeeping advances it (src/main.jl:361-422).  Pure: a batch is first
+  // run through it on a copy to learn which records to fetch.
+  struct BookScalars { double tvsx, tevneb, tevne; int idelevne, slipstart; };
+  struct Needs {
+    bool start, end, is, seis;
+    bool any() const { return start || end || is || seis; }
+  };
+  auto needs = [&](BookScalars& S, double t, double Vfmax) -> Needs {
+    Needs n;
+    n.start = Vfmax > 1.01 * ctx->Vthres && S.slipstart == 0;
+    n.end = Vfmax < 0.99 * ctx->Vthres && S.slipstart == 1;
+    n.is = t > S.tvsx;
+    n.seis = false;
+    if (Vfmax > ctx->Vevne) {
+      if (S.idelevne == 0) n.seis = true;
+      else if ((t - S.tevneb) > S.tevne) n.seis = true;
+    }
+    if (n.start) S.slipstart = 1;
+    if (n.end) S.slipstart = 0;
+    if (n.is) S.tvsx += tvsxinc;
+    if (Vfmax > ctx->Vevne) {
+      if (S.idelevne == 0) {
+        S.idelevne = 1;
+        S.tevneb = t;
+        S.tevne = tevneinc;
+      } else if (n.seis) {
+        S.tevne += tevneinc;
+      }
+    } else {
+      S.idelevne = 0;
+    }
+    return n;
+  };
+  // fault rows of d, v and tau after the step of record slot q, in pinned memory
+  auto fetch_fault = [&](int q) -> int {
+    if (int rc = d2h(ctx, ctx->rec_fault_host + (int64_t)q * 3 * NX, ctx->rec_fault + (int64_t)q * 3 * NX, 3 * NX))
+      return rc;
+    return EQ_OK;
   };
   auto put = [&](double* M, int64_t cap, int64_t col, const std::vector<double>& src) {
     if (M && col < cap) std::memcpy(M + col * NX, src.data(), sizeof(double) * NX);
   };
-  // bookkeeping of one finished step whose record is in slot q
-  auto book = [&](const eq_step_info& info, int q) -> int {
+  // bookkeeping of one finished step whose record is in slot q; fault: its
+  // fault rows on the host, or null (fetched here when needed)
+  auto book = [&](const eq_step_info& info, int q, const double* fault) -> int {
     const int64_t k = R->n_steps;
     if (k < R->cap_steps) {
       if (R->time_) R->time_[k] = info.t;
@@ -1644,22 +1704,24 @@ extern "C" int eq_run(eq_ctx* ctx, double total_time, int64_t max_steps, eq_resu
     R->total_pcg_iters += info.pcg_iters[0] + info.pcg_iters[1];
     R->nr_noconv += info.nr_noconv;
     const double t = info.t, Vfmax = info.Vfmax;
-    const bool need_start = Vfmax > 1.01 * ctx->Vthres && slipstart == 0;
-    const bool need_end = Vfmax < 0.99 * ctx->Vthres && slipstart == 1;
-    const bool need_is = t > tvsx;
-    bool need_seis = false;
-    if (Vfmax > ctx->Vevne) {
-      if (idelevne == 0) need_seis = true;
-      else if ((t - tevneb) > tevne) need_seis = true;
+    BookScalars S{tvsx, tevneb, tevne, idelevne, slipstart};
+    const Needs nd = needs(S, t, Vfmax);
+    tvsx = S.tvsx;
+    tevneb = S.tevneb;
+    tevne = S.tevne;
+    idelevne = S.idelevne;
+    slipstart = S.slipstart;
+    const bool need_start = nd.start, need_end = nd.end, need_is = nd.is, need_seis = nd.seis;
+    if (nd.any() && !fault) {
+      if (int rc = fetch_fault(q)) return rc;
+      if (int rc = spin_sync(ctx)) return rc;
+      fault = ctx->rec_fault_host + (int64_t)q * 3 * NX;
     }
-    if (need_start || need_end || need_is || need_seis)
-      if (int rc = fault_fields(q)) return rc;
-    const double* dF = fault.data();
-    const double* vF = fault.data() + NX;
-    const double* tau = fault.data() + 2 * NX;
+    const double* dF = fault;
+    const double* vF = fault ? fault + NX : nullptr;
+    const double* tau = fault ? fault + 2 * NX : nullptr;
     if (need_start) {
       for (int64_t j = 0; j < NX; j++) delfref[j] = 2 * dF[j] + Vpl * t;
-      slipstart = 1;
       const int64_t e = R->n_start;
       if (e < R->cap_events) {
         if (R->tStart) R->tStart[e] = t;
@@ -1679,7 +1741,6 @@ extern "C" int eq_run(eq_ctx* ctx, double total_time, int64_t max_steps, eq_resu
         if (R->tEnd) R->tEnd[e] = t;
       }
       R->n_end++;
-      slipstart = 0;
     }
     if (need_is) {
       const int64_t e = R->n_is;
@@ -1692,16 +1753,8 @@ extern "C" int eq_run(eq_ctx* ctx, double total_time, int64_t max_steps, eq_resu
         put(R->is_stress, R->cap_is, e, tmp);
       }
       R->n_is++;
-      tvsx += tvsxinc;
     }
     if (Vfmax > ctx->Vevne) {
-      if (idelevne == 0) {
-        idelevne = 1;
-        tevneb = t;
-        tevne = tevneinc;
-      } else if (need_seis) {
-        tevne += tevneinc;
-      }
       if (need_seis) {
         const int64_t e = R->n_seis;
         if (e < R->cap_seis) {
@@ -1714,8 +1767,6 @@ extern "C" int eq_run(eq_ctx* ctx, double total_time, int64_t max_steps, eq_resu
         }
         R->n_seis++;
       }
-    } else {
-      idelevne = 0;
     }
     if (info.it % 10 == 0 && nseis > 0) {
       const int64_t e = R->n_rec;
@@ -1737,10 +1788,32 @@ extern "C" int eq_run(eq_ctx* ctx, double total_time, int64_t max_steps, eq_resu
     if (ctx->isolver == 2) {
       // dynamic regime: a batch of steps, then the bookkeeping of each
       int m = 0;
+      StepStatus expect{};
       const int64_t more = max_steps > 0 ? max_steps - R->n_steps : INT64_MAX;
-      int rc = dyn_batch(ctx, total_time, more, &m);
+      int rc = dyn_batch(ctx, total_time, more, &m, &expect);
+      if (rc) {
+        R->wall_dyn += std::chrono::duration<double>(clk::now() - t0).count();
+        return rc;
+      }
+      // the records this batch's bookkeeping reads come back with the step
+      // tail's status, in one synchronisation
+      bool fetched[eq_ctx::kBatch] = {};
+      {
+        BookScalars S{tvsx, tevneb, tevne, idelevne, slipstart};
+        double tt = ctx->t;
+        for (int q = 0; q < m; q++) {
+          tt += ctx->dt;
+          const double vmax = q == m - 1 ? expect.Vfmax : ctx->st_host[q].Vfmax;
+          if (needs(S, tt, vmax).any()) {
+            if (int rc2 = fetch_fault(q)) return rc2;
+            fetched[q] = true;
+          }
+        }
+      }
+      rc = spin_sync(ctx);
       R->wall_dyn += std::chrono::duration<double>(clk::now() - t0).count();
       if (rc) return rc;
+      if (int rc2 = dyn_tail_check(ctx, m, expect)) return rc2;
       for (int q = 0; q < m; q++) {
         const StepStatus& st = ctx->st_host[q];
         eq_step_info info{};
@@ -1767,7 +1840,7 @@ extern "C" int eq_run(eq_ctx* ctx, double total_time, int64_t max_steps, eq_resu
           return EQ_NR_NAN;
         }
         if (!std::isfinite(st.Vfmax)) return fail(ctx, EQ_NR_NAN, "non-finite slip rate");
-        if (int rc2 = book(info, q)) return rc2;
+        if (int rc2 = book(info, q, fetched[q] ? ctx->rec_fault_host + (int64_t)q * 3 * NX : nullptr)) return rc2;
       }
     } else {
       eq_step_info info;
@@ -1775,7 +1848,7 @@ extern "C" int eq_run(eq_ctx* ctx, double total_time, int64_t max_steps, eq_resu
       const double el = std::chrono::duration<double>(clk::now() - t0).count();
       if (info.isolver_used == 1) { R->n_qs++; R->wall_qs += el; } else { R->n_dyn++; R->wall_dyn += el; }
       if (rc) return rc;
-      if (int rc2 = book(info, 0)) return rc2;
+      if (int rc2 = book(info, 0, nullptr)) return rc2;
     }
   }
   return EQ_OK;

@@ -8,9 +8,14 @@ constexpr int BLOCK = 256;
 #ifndef EQ_TILE_I
 #define EQ_TILE_I 64
 #endif
+#ifndef EQ_TILE_NPT
+#define EQ_TILE_NPT 4
+#endif
 constexpr int TILE_I = EQ_TILE_I;   // owned nodes per tile along x (64 or 32)
 constexpr int TILE_J = 16;          // owned nodes per tile along y
-constexpr int TBLOCK = TILE_I * TILE_J / 4;   // threads of a tile kernel block (4 nodes each)
+constexpr int TILE_NPT = EQ_TILE_NPT;                // owned nodes per thread in tile_kernel (1, 2 or 4)
+constexpr int TBLOCK = TILE_I * TILE_J / TILE_NPT;   // threads of a tile_kernel block
+constexpr int PBLOCK = TILE_I * TILE_J / 4;          // threads of a cheb_pair_kernel block (4 nodes each)
 
 // Device scalars of one CG solve (all FP64).
 struct CgScal {
@@ -167,6 +172,7 @@ struct NewmarkRecord {
   double* vpart_out = nullptr;
   double* cap_fault = nullptr;
   const int64_t* seis_idx = nullptr;
+  const int* seis_tile = nullptr;     // tile index of each receiver (tile_grid order)
   int nseis = 0;
   double* cap_seis = nullptr;
 };

@@ -23,6 +23,7 @@
 #include <cfloat>
 #include <climits>
 #include <cmath>
+#include <type_traits>
 
 #include "eq_kernels.h"
 
@@ -37,9 +38,9 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// blockDim.x is 256 (BLOCK) or, in tile kernels, TBLOCK (a multiple of 64)
+// blockDim.x is a multiple of 64, at most 1024
 __device__ double block_sum(double v) {
-  __shared__ double sh[BLOCK / 64];
+  __shared__ double sh[16];
   v = wave_sum(v);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   __syncthreads();
@@ -210,34 +211,73 @@ __device__ __forceinline__ T seg_last(const Geom& g, const T* u, const T* m) {
   return r;
 }
 
+// Outputs i0 .. i0+NPT-1 of the 5-node segment at u, m (stride ST; has_seg:
+// the segment exists), plus, when i0 == 0 and has_prev, the previous
+// segment's output 4 added to the first (the shared node).  Each output is
+// formed exactly as seg_all forms it.
+template <typename T, int ST, int NPT>
+__device__ __forceinline__ void seg_part(const Geom& g, const T* u, const T* m, int i0, bool has_seg, bool has_prev,
+                                         T o[NPT]) {
+  T a[5] = {0, 0, 0, 0, 0};
+  if (has_seg) seg_all<T, ST>(g, u, m, a);
+  if (NPT == 4) {
+#pragma unroll
+    for (int j = 0; j < 4; j++) o[j] = a[j];
+  } else if (NPT == 2) {
+    o[0] = i0 ? a[2] : a[0];
+    o[1] = i0 ? a[3] : a[1];
+  } else {
+    T r = a[0];
+    r = (i0 == 1) ? a[1] : r;
+    r = (i0 == 2) ? a[2] : r;
+    r = (i0 == 3) ? a[3] : r;
+    o[0] = r;
+  }
+  if (i0 == 0 && has_prev) o[0] += seg_last<T, ST>(g, u - 4 * ST, m - 4 * ST);
+}
+
 // Tile kernel.  Phase 1 issues every global load of the block at once: the
 // (64+5) x (16+5) window of the input vector (op.load) and of mu, held in
-// registers, and the epilogue operands of the thread's 4 owned nodes
+// registers, and the epilogue operands of the thread's TILE_NPT owned nodes
 // (op.fetch); only then are the window values stored to LDS.  Phase 2: each
-// thread takes 4 consecutive nodes of one row (one segment) and accumulates
-// their x-direction terms into an LDS tile.  Phase 3: each thread takes 4
+// thread takes TILE_NPT consecutive nodes of one row and accumulates their
+// x-direction terms into an LDS tile.  Phase 3: each thread takes TILE_NPT
 // consecutive nodes of one column, adds the y-direction terms and runs the
-// epilogue.  Every 5-node segment is evaluated once per direction (plus the
-// shared end node of the previous segment).
-static_assert(TILE_J * (TILE_I / 4) == TBLOCK, "phase 2: one row strip per thread");
-static_assert(TILE_I * (TILE_J / 4) == TBLOCK, "phase 3: one column strip per thread");
-static_assert(TBLOCK % 64 == 0 && TBLOCK <= BLOCK, "whole waves");
+// epilogue.  With 4 nodes per thread every 5-node segment is evaluated once
+// per direction (plus the shared end node of the previous segment); with
+// fewer, a segment is evaluated by each of its threads, which buys more
+// waves per tile.
+static_assert(TILE_J * (TILE_I / TILE_NPT) == TBLOCK, "phase 2: one row strip per thread");
+static_assert(TILE_I * (TILE_J / TILE_NPT) == TBLOCK, "phase 3: one column strip per thread");
+static_assert(TILE_NPT == 1 || TILE_NPT == 2 || TILE_NPT == 4, "strips lie inside one segment");
+static_assert(TBLOCK % 64 == 0 && TBLOCK <= 1024, "whole waves");
 constexpr int WIN_LOADS = (WIN_J * WIN_I + TBLOCK - 1) / TBLOCK;
+
+// Op::kLatePrepare (when declared true): the window and epilogue loads are
+// issued before op.prepare, whose own reads then overlap them; load and fetch
+// must not depend on Pre.  Otherwise prepare runs first and a block it stops
+// issues no loads.
+template <class Op, class = void>
+struct LatePrepare : std::false_type {};
+template <class Op>
+struct LatePrepare<Op, std::void_t<decltype(Op::kLatePrepare)>> : std::bool_constant<Op::kLatePrepare> {};
 
 template <typename T, class Op>
 __global__ __launch_bounds__(TBLOCK) void tile_kernel(Geom g, Op op) {
+  constexpr int NPT = TILE_NPT;
+  constexpr bool late = LatePrepare<Op>::value;
   __shared__ T su[WIN_J * WIN_I];
   __shared__ T sm[WIN_J * WIN_I];
   __shared__ T sx[TILE_J * TILE_I];
-  typename Op::Pre pre;
-  if (!op.prepare(pre)) return;   // block-uniform
+  typename Op::Pre pre{};
+  if (!late && !op.prepare(pre)) return;   // block-uniform
   const int I0 = blockIdx.x * TILE_I, J0 = blockIdx.y * TILE_J;
-  // phase-3 ownership of this thread: column c, rows Js..Js+3
+  // phase-3 ownership of this thread: column c, rows Js .. Js+NPT-1
   const int c3 = threadIdx.x % TILE_I, q3 = threadIdx.x / TILE_I;
-  const int I3 = I0 + c3, Js3 = J0 + 4 * q3;
-  typename Op::Ext ext[4];
+  const int I3 = I0 + c3, Js3 = J0 + NPT * q3;
+  typename Op::Ext ext[NPT];
 #pragma unroll
-  for (int j = 0; j < 4; j++)
+  for (int j = 0; j < NPT; j++)
     if (I3 < g.NX && Js3 + j < g.NY) op.fetch(pre, (int64_t)(Js3 + j) * g.NX + I3, I3, Js3 + j, ext[j]);
   {
     T ur[WIN_LOADS], mr[WIN_LOADS];
@@ -254,6 +294,7 @@ __global__ __launch_bounds__(TBLOCK) void tile_kernel(Geom g, Op op) {
         mr[t] = op.mu[n];
       }
     }
+    if (late && !op.prepare(pre)) return;   // block-uniform
 #pragma unroll
     for (int t = 0; t < WIN_LOADS; t++) {
       const int k = threadIdx.x + t * TBLOCK;
@@ -265,36 +306,35 @@ __global__ __launch_bounds__(TBLOCK) void tile_kernel(Geom g, Op op) {
   }
   __syncthreads();
   {
-    // x-direction: thread -> (row r, strip q of 4 nodes); TILE_I / 4 strips per row
-    constexpr int NSTRIP = TILE_I / 4;
+    // x-direction: thread -> (row r, strip q of NPT nodes); TILE_I / NPT strips per row
+    constexpr int NSTRIP = TILE_I / NPT;
     const int r = threadIdx.x / NSTRIP, q = threadIdx.x - r * NSTRIP;
-    const int J = J0 + r, Is = I0 + 4 * q;
+    const int J = J0 + r, Is = I0 + NPT * q;
     if (J < g.NY && Is < g.NX) {
-      const int sg = Is >> 2;
-      const int lr = r + 4, lc = 4 * q + 4;
-      T o[5] = {0, 0, 0, 0, 0};
-      if (sg < g.nelx) seg_all<T, 1>(g, su + lr * WIN_I + lc, sm + lr * WIN_I + lc, o);
-      if (sg > 0) o[0] += seg_last<T, 1>(g, su + lr * WIN_I + lc - 4, sm + lr * WIN_I + lc - 4);
+      const int sg = Is >> 2, i0 = Is & 3;
+      const int lr = r + 4, lc = NPT * q + 4 - i0;   // window column of the segment start
+      T o[NPT];
+      seg_part<T, 1, NPT>(g, su + lr * WIN_I + lc, sm + lr * WIN_I + lc, i0, sg < g.nelx, sg > 0, o);
       const T cx = (T)(g.c1 * wline(g, J, g.NY));
 #pragma unroll
-      for (int i = 0; i < 4; i++) sx[r * TILE_I + 4 * q + i] = cx * o[i];
+      for (int i = 0; i < NPT; i++) sx[r * TILE_I + NPT * q + i] = cx * o[i];
     }
   }
   __syncthreads();
   double acc0 = 0.0, acc1 = 0.0;
   if (I3 < g.NX && Js3 < g.NY) {
     // y-direction + epilogue
-    const int t = Js3 >> 2;
-    const int lc = c3 + 4, lr = 4 * q3 + 4;
-    T o[5] = {0, 0, 0, 0, 0};
-    if (t < g.nely) seg_all<T, WIN_I>(g, su + lr * WIN_I + lc, sm + lr * WIN_I + lc, o);
-    if (t > 0) o[0] += seg_last<T, WIN_I>(g, su + (lr - 4) * WIN_I + lc, sm + (lr - 4) * WIN_I + lc);
+    const int t = Js3 >> 2, j0 = Js3 & 3;
+    const int lc = c3 + 4, lr = NPT * q3 + 4;       // window row of Js3
+    const int ls = lr - j0;                          // window row of the segment start
+    T o[NPT];
+    seg_part<T, WIN_I, NPT>(g, su + ls * WIN_I + lc, sm + ls * WIN_I + lc, j0, t < g.nely, t > 0, o);
     const T cy = (T)(g.c2 * wline(g, I3, g.NX));
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
+    for (int j = 0; j < NPT; j++) {
       const int J = Js3 + j;
       if (J < g.NY) {
-        const T ku = sx[(4 * q3 + j) * TILE_I + c3] + cy * o[j];
+        const T ku = sx[(NPT * q3 + j) * TILE_I + c3] + cy * o[j];
         op.epilogue(pre, (int64_t)J * g.NX + I3, I3, J, ku, su[(lr + j) * WIN_I + lc], ext[j], acc0, acc1);
       }
     }
@@ -712,6 +752,7 @@ __device__ void d_fbc_node(const FaultDev& fp, const FaultState& fs, int j, doub
 struct OpNewmark {
   struct Pre { int seis; };
   static constexpr int kParts = 0;
+  static constexpr bool kLatePrepare = true;   // the stop test reads the previous step's record
   __device__ void zero_tail(int) const {}
   FaultDev fp;
   FaultState fs;
@@ -730,32 +771,40 @@ struct OpNewmark {
   double* vpart_out;        // [2 * nbx] this step's
   double* cap_fault;        // [3 * nflt] d, v, tau of the fault row
   const int64_t* seis_idx;
+  const int* seis_tile;     // [nseis] tile (blockIdx.y * gridDim.x + blockIdx.x) holding each receiver
   int nseis;
   double* cap_seis;         // [3 * nseis]
   int nbx;
   __device__ bool prepare(Pre& p) const {
     p.seis = 0;
     if (vprev) {
+      // (max, lowest index) over the bottom tiles' records, one tile per lane,
+      // reduced across the wave; any record marked -1 means the previous step
+      // did not run
+      const int lane = threadIdx.x & 63;
       double m = -INFINITY, mi = 2147483647.0;
-      for (int b = 0; b < nbx; b++) {
+      int notrun = 0;
+      for (int b = lane; b < nbx; b += 64) {
         const double o = vprev[2 * b], oi = vprev[2 * b + 1];
-        if (oi < 0.0) { mi = -1.0; break; }    // the previous step did not run
+        if (oi < 0.0) notrun = 1;
+        else if (o > m || (o == m && oi < mi)) { m = o; mi = oi; }
+      }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(m, off, 64), oi = __shfl_xor(mi, off, 64);
         if (o > m || (o == m && oi < mi)) { m = o; mi = oi; }
       }
       const double Vfmax = 2 * m + fp.Vpl;
-      const bool stop = mi < 0.0 || nrprev[0] != INT_MAX || !isfinite(Vfmax) || Vfmax < 2e-3;
+      const bool stop = __any(notrun) || nrprev[0] != INT_MAX || !isfinite(Vfmax) || Vfmax < 2e-3;
       if (stop) {
         if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) vpart_out[1] = -1.0;
         return false;
       }
     }
     if (cap_seis) {
-      const int I0 = blockIdx.x * TILE_I, J0 = blockIdx.y * TILE_J;
-      for (int q = 0; q < nseis; q++) {
-        const int64_t n = seis_idx[q];
-        const int J = (int)(n / fp.nflt), I = (int)(n - (int64_t)J * fp.nflt);
-        if (I >= I0 && I < I0 + TILE_I && J >= J0 && J < J0 + TILE_J) p.seis |= 1 << q;
-      }
+      const int tile = blockIdx.y * gridDim.x + blockIdx.x;
+      for (int q = 0; q < nseis; q++)
+        if (seis_tile[q] == tile) p.seis |= 1 << q;
     }
     return true;
   }
@@ -1460,11 +1509,11 @@ constexpr int FW_I = TILE_I + 13;
 constexpr int FW_J = TILE_J + 13;
 constexpr int FB_I = TILE_I + 5;               // box of the first step: columns I0-4 .. I0+TILE_I
 constexpr int FB_J = TILE_J + 5;               // rows J0-4 .. J0+TILE_J
-constexpr int FW_LOADS = (FW_J * FW_I + TBLOCK - 1) / TBLOCK;
+constexpr int FW_LOADS = (FW_J * FW_I + PBLOCK - 1) / PBLOCK;
 constexpr int FB_STRIPS_I = (FB_I + 3) / 4;    // 4-node row strips per box row
 constexpr int FB_STRIPS_J = (FB_J + 3) / 4;    // 4-node column strips per box column
 constexpr int F_RING = 2 * TILE_I + 5 * (TILE_J / 4);   // column strips of the ring
-static_assert(F_RING <= TBLOCK, "one ring strip per thread");
+static_assert(F_RING <= PBLOCK, "one ring strip per thread");
 static_assert(FB_STRIPS_J == TILE_J / 4 + 2, "ring rows: one strip below, one above");
 
 template <typename T>
@@ -1500,7 +1549,7 @@ struct ChebPair {
 };
 
 template <typename T>
-__global__ __launch_bounds__(TBLOCK) void cheb_pair_kernel(Geom g, ChebPair<T> op) {
+__global__ __launch_bounds__(PBLOCK) void cheb_pair_kernel(Geom g, ChebPair<T> op) {
   __shared__ T sw[FW_J * FW_I];    // first step's source
   __shared__ T smw[FW_J * FW_I];   // mu
   __shared__ T sbx[FB_J * FB_I];   // x-direction terms: first step on the box, then second step on the tile
@@ -1562,7 +1611,7 @@ __global__ __launch_bounds__(TBLOCK) void cheb_pair_kernel(Geom g, ChebPair<T> o
     T ur[FW_LOADS], mr[FW_LOADS];
 #pragma unroll
     for (int t = 0; t < FW_LOADS; t++) {
-      const int k = tid + t * TBLOCK;
+      const int k = tid + t * PBLOCK;
       const int r = k / FW_I, c = k - r * FW_I;
       const int I = I0 - 8 + c, J = J0 - 8 + r;
       ur[t] = 0;
@@ -1576,7 +1625,7 @@ __global__ __launch_bounds__(TBLOCK) void cheb_pair_kernel(Geom g, ChebPair<T> o
     }
 #pragma unroll
     for (int t = 0; t < FW_LOADS; t++) {
-      const int k = tid + t * TBLOCK;
+      const int k = tid + t * PBLOCK;
       if (k < FW_J * FW_I) {
         sw[k] = ur[t];
         smw[k] = mr[t];
@@ -1585,7 +1634,7 @@ __global__ __launch_bounds__(TBLOCK) void cheb_pair_kernel(Geom g, ChebPair<T> o
   }
   __syncthreads();
   // first step, x direction, on every box row
-  for (int sidx = tid; sidx < FB_J * FB_STRIPS_I; sidx += TBLOCK) {
+  for (int sidx = tid; sidx < FB_J * FB_STRIPS_I; sidx += PBLOCK) {
     const int rb = sidx / FB_STRIPS_I, k = sidx - rb * FB_STRIPS_I;
     const int J = J0 - 4 + rb, Is = I0 - 4 + 4 * k;
     if (J < 0 || J >= NY || Is < 0 || Is >= NX) continue;
@@ -1732,7 +1781,7 @@ void launch_cheb_pair(const Geom& g, const ChebStep<T>& a, const ChebStep<T>& b,
   op.nrr = nrr;
   op.sc = sc;
   op.part0 = part_rho;
-  hipLaunchKernelGGL((cheb_pair_kernel<T>), tile_grid(g), dim3(TBLOCK), 0, s, g, op);
+  hipLaunchKernelGGL((cheb_pair_kernel<T>), tile_grid(g), dim3(PBLOCK), 0, s, g, op);
 }
 
 template <typename T, int MODE>
@@ -1815,8 +1864,8 @@ void launch_newmark(const Geom& g, const FaultDev& fp, FaultState fs, const doub
                     double dt, double half_dt, double half_dt_sq, int* flags, const NewmarkRecord& rec,
                     hipStream_t s) {
   OpNewmark op{fp, fs, mu, M, d, v, a, d2, v2, a2, dt, half_dt, half_dt_sq, flags, g.NY, nullptr, nullptr,
-               rec.vprev, rec.nrprev, rec.vpart_out, rec.cap_fault, rec.seis_idx, rec.nseis, rec.cap_seis,
-               (int)tile_grid(g).x};
+               rec.vprev, rec.nrprev, rec.vpart_out, rec.cap_fault, rec.seis_idx, rec.seis_tile, rec.nseis,
+               rec.cap_seis, (int)tile_grid(g).x};
   hipLaunchKernelGGL((tile_kernel<double, OpNewmark>), tile_grid(g), dim3(TBLOCK), 0, s, g, op);
 }
 

@@ -33,6 +33,9 @@ def main():
     ap.add_argument("--epochs", type=int, default=100)
     ap.add_argument("--warmup-years", type=float, default=20.0)
     ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--graphs", type=int, default=1,
+                    help="replay CG batches as HIP graphs (0 under rocprofv3 --kernel-trace, which crashed in "
+                         "graph capture here)")
     a = ap.parse_args()
     from eqcycle_amd import learned
     from eqcycle_amd.api import EqContext
@@ -46,7 +49,7 @@ def main():
     guess = {"plain": "previous", "network": "learned", "galerkin": "galerkin"}[a.variant]
     prec = a.prec or ("fp64" if a.variant == "plain" else cfg.precision)
     if a.variant != "plain":
-        ctx.set_solver(tol=tol, pc="amg", prec="fp64", guess="previous")
+        ctx.set_solver(tol=tol, pc="amg", prec="fp64", guess="previous", graphs=bool(a.graphs))
         ctx.state_init()
         ctx.learn_init(capacity=a.snapshots, nbasis=64)
         ctx.learn_record(True)
@@ -57,10 +60,10 @@ def main():
         if a.variant == "network":
             learned.train(ctx, epochs=a.epochs, batch=64, lr=1e-3)
     print("[profile_window] learning done; window follows", flush=True)
-    ctx.set_solver(tol=tol, pc="amg", prec="fp64", guess="previous")
+    ctx.set_solver(tol=tol, pc="amg", prec="fp64", guess="previous", graphs=bool(a.graphs))
     ctx.state_init()
     ctx.run(total_time=a.warmup_years * YR, snapshots=False)
-    ctx.set_solver(tol=tol, pc="amg", prec=prec, guess=guess)
+    ctx.set_solver(tol=tol, pc="amg", prec=prec, guess=guess, graphs=bool(a.graphs))
     ctx.time_kernel("copy", "fp64", reps=1)   # trace marker: the window starts after this copy
     t0 = time.perf_counter()
     R = ctx.run(total_time=1e30, max_steps=a.steps, snapshots=False)
