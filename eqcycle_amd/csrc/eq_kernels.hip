@@ -263,7 +263,7 @@ template <class Op>
 struct LatePrepare<Op, std::void_t<decltype(Op::kLatePrepare)>> : std::bool_constant<Op::kLatePrepare> {};
 
 template <typename T, class Op>
-__global__ __launch_bounds__(TBLOCK) void tile_kernel(Geom g, Op op) {
+__device__ __forceinline__ void tile_body(const Geom& g, const Op& op, int bx, int by, int ntx, int nty) {
   constexpr int NPT = TILE_NPT;
   constexpr bool late = LatePrepare<Op>::value;
   __shared__ T su[WIN_J * WIN_I];
@@ -271,7 +271,7 @@ __global__ __launch_bounds__(TBLOCK) void tile_kernel(Geom g, Op op) {
   __shared__ T sx[TILE_J * TILE_I];
   typename Op::Pre pre{};
   if (!late && !op.prepare(pre)) return;   // block-uniform
-  const int I0 = blockIdx.x * TILE_I, J0 = blockIdx.y * TILE_J;
+  const int I0 = bx * TILE_I, J0 = by * TILE_J;
   // phase-3 ownership of this thread: column c, rows Js .. Js+NPT-1
   const int c3 = threadIdx.x % TILE_I, q3 = threadIdx.x / TILE_I;
   const int I3 = I0 + c3, Js3 = J0 + NPT * q3;
@@ -342,13 +342,18 @@ __global__ __launch_bounds__(TBLOCK) void tile_kernel(Geom g, Op op) {
   op.block_end(acc0, acc1);
   if (Op::kParts >= 1) {
     const double s = block_sum(acc0);
-    if (threadIdx.x == 0) op.part0[blockIdx.y * gridDim.x + blockIdx.x] = s;
+    if (threadIdx.x == 0) op.part0[by * ntx + bx] = s;
   }
   if (Op::kParts >= 2) {
     const double s = block_sum(acc1);
-    if (threadIdx.x == 0) op.part1[blockIdx.y * gridDim.x + blockIdx.x] = s;
+    if (threadIdx.x == 0) op.part1[by * ntx + bx] = s;
   }
-  if (blockIdx.x == 0 && blockIdx.y == 0) op.zero_tail((int)(gridDim.x * gridDim.y));
+  if (bx == 0 && by == 0) op.zero_tail(ntx * nty);
+}
+
+template <typename T, class Op>
+__global__ __launch_bounds__(TBLOCK) void tile_kernel(Geom g, Op op) {
+  tile_body<T, Op>(g, op, blockIdx.x, blockIdx.y, gridDim.x, gridDim.y);
 }
 
 // Epilogue operands with nothing to prefetch.
@@ -771,38 +776,41 @@ struct OpNewmark {
   double* vpart_out;        // [2 * nbx] this step's
   double* cap_fault;        // [3 * nflt] d, v, tau of the fault row
   const int64_t* seis_idx;
-  const int* seis_tile;     // [nseis] tile (blockIdx.y * gridDim.x + blockIdx.x) holding each receiver
+  const int* seis_tile;     // [nseis] tile (row-major over the tile grid) holding each receiver
   int nseis;
   double* cap_seis;         // [3 * nseis]
   int nbx;
+  // false: the previous step of the batch asked to stop (regime switch, FBC!
+  // guard, non-finite or not run); block-uniform
+  __device__ bool go() const {
+    if (!vprev) return true;
+    // (max, lowest index) over the fault records, one record per lane,
+    // reduced across the wave; a record marked -1 means the previous step
+    // did not run
+    const int lane = threadIdx.x & 63;
+    double m = -INFINITY, mi = 2147483647.0;
+    int notrun = 0;
+    for (int b = lane; b < nbx; b += 64) {
+      const double o = vprev[2 * b], oi = vprev[2 * b + 1];
+      if (oi < 0.0) notrun = 1;
+      else if (o > m || (o == m && oi < mi)) { m = o; mi = oi; }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const double o = __shfl_xor(m, off, 64), oi = __shfl_xor(mi, off, 64);
+      if (o > m || (o == m && oi < mi)) { m = o; mi = oi; }
+    }
+    const double Vfmax = 2 * m + fp.Vpl;
+    const bool stop = __any(notrun) || nrprev[0] != INT_MAX || !isfinite(Vfmax) || Vfmax < 2e-3;
+    if (stop && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) vpart_out[1] = -1.0;
+    return !stop;
+  }
+  // tile blocks sit at blockIdx.y >= 1 (row 0 of the grid runs the fault row)
   __device__ bool prepare(Pre& p) const {
     p.seis = 0;
-    if (vprev) {
-      // (max, lowest index) over the bottom tiles' records, one tile per lane,
-      // reduced across the wave; any record marked -1 means the previous step
-      // did not run
-      const int lane = threadIdx.x & 63;
-      double m = -INFINITY, mi = 2147483647.0;
-      int notrun = 0;
-      for (int b = lane; b < nbx; b += 64) {
-        const double o = vprev[2 * b], oi = vprev[2 * b + 1];
-        if (oi < 0.0) notrun = 1;
-        else if (o > m || (o == m && oi < mi)) { m = o; mi = oi; }
-      }
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) {
-        const double o = __shfl_xor(m, off, 64), oi = __shfl_xor(mi, off, 64);
-        if (o > m || (o == m && oi < mi)) { m = o; mi = oi; }
-      }
-      const double Vfmax = 2 * m + fp.Vpl;
-      const bool stop = __any(notrun) || nrprev[0] != INT_MAX || !isfinite(Vfmax) || Vfmax < 2e-3;
-      if (stop) {
-        if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) vpart_out[1] = -1.0;
-        return false;
-      }
-    }
+    if (!go()) return false;
     if (cap_seis) {
-      const int tile = blockIdx.y * gridDim.x + blockIdx.x;
+      const int tile = (blockIdx.y - 1) * gridDim.x + blockIdx.x;
       for (int q = 0; q < nseis; q++)
         if (seis_tile[q] == tile) p.seis |= 1 << q;
     }
@@ -815,8 +823,15 @@ struct OpNewmark {
     e.M = M[n];
   }
   __device__ double load(const Pre&, int64_t n, int, int) const { return d[n] + dt * v[n] + half_dt_sq * a[n]; }
+  // the fault row belongs to the fault blocks
   __device__ void epilogue(const Pre& p, int64_t n, int I, int J, double ku, double dn, const Ext& e, double& a0,
                            double& a1) const {
+    if (J != 0) node(p, n, I, J, ku, dn, e, a0, a1);
+  }
+  // corrector of one node (src/main.jl:302-338): K d, absorbing edges, FBC!
+  // on the fault row, a = f / M, v += dt/2 a
+  __device__ void node(const Pre& p, int64_t n, int I, int J, double ku, double dn, const Ext& e, double& a0,
+                       double& a1) const {
     const double vo = e.v;
     const double vp = vo + half_dt * e.a;
     double at = -ku;
@@ -872,24 +887,89 @@ struct OpNewmark {
         }
     }
   }
-  // bottom tile row: (max, first index) of the new fault velocity over the
-  // tile's TILE_I fault nodes, all held by the first TILE_I threads (wave 0)
-  __device__ void block_end(double a0, double a1) const {
-    if (!vpart_out || blockIdx.y != 0 || threadIdx.x >= 64) return;
-    // as the step tail: NaN is never the maximum; no valid value leaves (-inf, INT_MAX)
-    const bool valid = (int)threadIdx.x < TILE_I && (int)(blockIdx.x * TILE_I + threadIdx.x) < fp.nflt && !isnan(a0);
-    double m = valid ? a0 : -INFINITY, mi = valid ? a1 : 2147483647.0;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      const double o = __shfl_down(m, off, 64), oi = __shfl_down(mi, off, 64);
-      if (o > m || (o == m && oi < mi)) { m = o; mi = oi; }
-    }
-    if (threadIdx.x == 0) {
-      vpart_out[2 * blockIdx.x] = m;
-      vpart_out[2 * blockIdx.x + 1] = mi;
-    }
-  }
+  __device__ void block_end(double, double) const {}
 };
+
+// K u at fault node (I, 0) of the predictor field u = d + dt v + dt^2/2 a,
+// read from global memory: the row-0 segment(s) along x and the column
+// segment of rows 0..4, each value formed as the tile kernel forms it
+// (seg_all / seg_last outputs, x term rounded before the y term is added).
+__device__ __forceinline__ double newmark_ku_fault(const Geom& g, const OpNewmark& op, int I) {
+  const OpNewmark::Pre pre{};
+  double u[5], m[5];
+  double ax = 0.0;
+  const int i = I & 3, s = I >> 2;
+  if (s < g.nelx) {
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+      u[k] = op.load(pre, 4 * s + k, 4 * s + k, 0);
+      m[k] = op.mu[4 * s + k];
+    }
+    ax = seg_eval<double, 1>(g, u, m, i);
+  }
+  if (i == 0 && s > 0) {
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+      u[k] = op.load(pre, 4 * s - 4 + k, 4 * s - 4 + k, 0);
+      m[k] = op.mu[4 * s - 4 + k];
+    }
+    ax += seg_last<double, 1>(g, u, m);
+  }
+  double ay = 0.0;
+  if (g.nely > 0) {
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+      const int64_t n = (int64_t)k * g.NX + I;
+      u[k] = op.load(pre, n, I, k);
+      m[k] = op.mu[n];
+    }
+    double o[5];
+    seg_all<double, 1>(g, u, m, o);
+    ay = o[0];
+  }
+  const double sx = __dmul_rn(g.c1 * wline(g, 0, g.NY), ax);
+  return sx + (g.c2 * wline(g, I, g.NX)) * ay;
+}
+
+// Whole Newmark step.  Grid row 0: fault blocks, one fault node per thread,
+// dispatched first so that the FBC! Newton searches run beside the bulk
+// instead of after it; they also write the per-TILE_I fault-velocity records.
+// Rows 1.. : the tiles of the mesh (tile_body with OpNewmark, fault row left
+// to the fault blocks).
+__global__ __launch_bounds__(TBLOCK) void newmark_kernel(Geom g, OpNewmark op) {
+  if (blockIdx.y > 0) {
+    tile_body<double, OpNewmark>(g, op, blockIdx.x, blockIdx.y - 1, gridDim.x, gridDim.y - 1);
+    return;
+  }
+  if (blockIdx.x * TBLOCK >= g.NX) return;   // block-uniform
+  if (!op.go()) return;                      // block-uniform
+  const int I = blockIdx.x * TBLOCK + threadIdx.x;
+  double a0 = 0.0, a1 = 0.0;
+  if (I < g.NX) {
+    OpNewmark::Pre pre{};
+    pre.seis = op.cap_seis ? (int)((1u << op.nseis) - 1u) : 0;
+    OpNewmark::Ext e;
+    op.fetch(pre, I, I, 0, e);
+    const double dn = op.load(pre, I, I, 0);
+    const double ku = newmark_ku_fault(g, op, I);
+    op.node(pre, I, I, 0, ku, dn, e, a0, a1);
+  }
+  if (!op.vpart_out) return;
+  // (max, first index) of the new fault velocity per TILE_I fault nodes; as
+  // the step tail: NaN is never the maximum, no valid value leaves (-inf, INT_MAX)
+  const bool valid = I < op.fp.nflt && !isnan(a0);
+  double m = valid ? a0 : -INFINITY, mi = valid ? a1 : 2147483647.0;
+#pragma unroll
+  for (int off = TILE_I / 2; off > 0; off >>= 1) {
+    const double o = __shfl_down(m, off, TILE_I), oi = __shfl_down(mi, off, TILE_I);
+    if (o > m || (o == m && oi < mi)) { m = o; mi = oi; }
+  }
+  const int b = I / TILE_I;
+  if (I % TILE_I == 0 && b < op.nbx) {
+    op.vpart_out[2 * b] = m;
+    op.vpart_out[2 * b + 1] = mi;
+  }
+}
 
 // --------------------------------------------------------------------------
 // element-wise kernels
@@ -1866,7 +1946,8 @@ void launch_newmark(const Geom& g, const FaultDev& fp, FaultState fs, const doub
   OpNewmark op{fp, fs, mu, M, d, v, a, d2, v2, a2, dt, half_dt, half_dt_sq, flags, g.NY, nullptr, nullptr,
                rec.vprev, rec.nrprev, rec.vpart_out, rec.cap_fault, rec.seis_idx, rec.seis_tile, rec.nseis,
                rec.cap_seis, (int)tile_grid(g).x};
-  hipLaunchKernelGGL((tile_kernel<double, OpNewmark>), tile_grid(g), dim3(TBLOCK), 0, s, g, op);
+  const dim3 tg = tile_grid(g);
+  hipLaunchKernelGGL(newmark_kernel, dim3(tg.x, tg.y + 1), dim3(TBLOCK), 0, s, g, op);
 }
 
 void launch_qs_begin(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, double* d, double* dPre,
