@@ -42,11 +42,12 @@ sys.path.insert(0, ROOT)
 YR = 365 * 24 * 60 * 60
 HBM_PEAK_GBS = 8000.0
 
-# HBM bytes per launch of the fine-level Chebyshev step, from two rocprofv3
-# --pmc passes (FETCH_SIZE, WRITE_SIZE) calibrated by a 512 MiB copy, on the
-# 410,881-node mesh (profiles/cheb_ku_newmark_cfg2_hbm_traffic.md).  Keyed by
-# (nodes, precision).
-CHEB_TRAFFIC = {(410881, "fp64"): 34.43e6, (410881, "fp32"): 20.20e6}
+# HBM bytes per launch from two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE)
+# calibrated by a 512 MiB copy, on the 410,881-node mesh
+# (profiles/newmark_cheb_pair_cfg4_hbm_traffic.md).  Keyed by nodes, and
+# precision for the Chebyshev pair.
+NEWMARK_TRAFFIC = {410881: 38.17e6}
+CHEB_PAIR_TRAFFIC = {(410881, "fp64"): 43.27e6, (410881, "fp32"): 25.41e6}
 
 
 def parse():
@@ -165,19 +166,35 @@ def train_learned(ctx, cfg, tol, epochs):
 
 
 def roofline(ctx, P, prec):
-    """Fine-level Chebyshev step (the kernel with the largest share of device
-    time): per node it reads the window vector, mu, D^-1, r, x and writes r,
-    d', x -- 8 values of s bytes (the LDS halo is not counted)."""
+    """The Newmark step, the kernel with the largest share of device time in
+    this window (one launch per dynamic step, ~20,000 steps against ~10,000
+    Chebyshev pair launches): per node it reads d, v, a, M, mu and writes d,
+    v, a -- 8 FP64 values (the LDS halo and the fault-row stencil reads are not
+    counted).  Timed with HIP events on the library's stream."""
+    dt_ms = ctx.time_kernel("newmark", "fp64", reps=200)
+    alg = 8 * 8 * P.nglob
+    achieved = alg / (dt_ms * 1e-3) / 1e9
+    return {"kernel": "newmark_kernel (whole Newmark step: predictor, -K d, absorbing edges, FBC! on the fault "
+                      "row in blocks of their own, corrector)",
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": NEWMARK_TRAFFIC.get(P.nglob), "avg_launch_us": dt_ms * 1e3,
+            "alg_bytes_per_launch": alg}
+
+
+def roofline_cheb_pair(ctx, P, prec):
+    """Two fine-level Chebyshev steps in one launch (the quasi-static solve's
+    largest kernel), in the V-cycle precision of the value's variant.  The
+    algorithmic bytes are those of one step, 8 values of s bytes per node
+    (window vector, mu, D^-1, r, x in; r, d, x out): the first step's outputs
+    never leave the chip."""
     kp = "fp32" if prec in ("f32pc", "mixed", "fp32") else "fp64"
     s = 4 if kp == "fp32" else 8
-    dt_ms = ctx.time_kernel("cheb", kp, reps=200)
+    dt_ms = ctx.time_kernel("cheb2", kp, reps=200)
     alg = 8 * s * P.nglob
     achieved = alg / (dt_ms * 1e-3) / 1e9
-    traffic = CHEB_TRAFFIC.get((P.nglob, kp))
-    return {"kernel": f"tile_kernel<{'float' if s == 4 else 'double'}, OpCheb> (fine-level Chebyshev step: "
-                      "K_II d fused with the r, d, x updates)",
-            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "avg_launch_us": dt_ms * 1e3,
+    return {"kernel": f"cheb_pair_kernel<{'float' if s == 4 else 'double'}>", "bound": "hbm",
+            "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "traffic": CHEB_PAIR_TRAFFIC.get((P.nglob, kp)), "avg_launch_us": dt_ms * 1e3,
             "alg_bytes_per_launch": alg}
 
 
@@ -241,6 +258,7 @@ def main():
                    "pcg_iters_per_solve": main_st["pcg_iters_per_solve"], "hierarchy": a.hierarchy,
                    "amg_levels": ctx.amg_info(), **extra},
         "roofline": roofline(ctx, P, prec),
+        "roofline_cheb_pair": roofline_cheb_pair(ctx, P, prec),
     }
     if not a.no_cpu_baseline:
         stats = dict(n_qs=plain["qs_steps"], n_dyn=plain["dyn_steps"], iters_per_solve=plain["pcg_iters_per_solve"])
