@@ -208,6 +208,19 @@ class EqContext:
                                     IDstate, ptr(f64(tau1)), dt))
         return p1, v1
 
+    def newmark_step(self, d, v, a, psi, dt=None):
+        """The dynamic branch of the loop body (src/main.jl:297-344) on copies
+        of d, v, a (reference numbering) and psi; returns (rc, dict with d, v,
+        a, psi, tau, Vf, FltVfree after the step).  rc is EQ_NR_NAN when the
+        FBC! guard fired (the message names the fault location)."""
+        P = self.P
+        d, v, a, psi = (f64(z).copy() for z in (d, v, a, psi))
+        tau, Vf, Vfree = (np.empty(P.FltNglob) for _ in range(3))
+        rc = self._check(self.lib.eq_newmark_step(self.h, ptr(d), ptr(v), ptr(a), ptr(psi), ptr(tau), ptr(Vf),
+                                                  ptr(Vfree), P.dt0 if dt is None else dt),
+                         allow=(L.EQ_NR_NAN, L.EQ_NR_NOCONV))
+        return rc, dict(d=d, v=v, a=a, psi=psi, tau=tau, Vf=Vf, FltVfree=Vfree)
+
     def dtevol(self, dt, dtmin, XiLf, FaultNglob, NFBC, Vf, isolver):
         out = C.c_double()
         self._check(self.lib.eq_dtevol(self.h, dt, dtmin, ptr(f64(XiLf)), FaultNglob, NFBC, ptr(f64(Vf)),

@@ -84,6 +84,7 @@ struct eq_ctx {
   double *d = nullptr, *v = nullptr, *a = nullptr, *d2 = nullptr, *v2 = nullptr, *a2 = nullptr;
   double *dPre = nullptr, *bq = nullptr;
   FaultState fs{}, fs_api{};
+  double* api_nodal = nullptr;        // [6 N] d, v, a in and out of eq_newmark_step
   double t = 0, dt = 0;
   int64_t it = 0;
   int isolver = 1;
@@ -1240,6 +1241,57 @@ extern "C" int eq_fbc(eq_ctx* ctx, int64_t IDstate, int64_t NFBC, int64_t FltNgl
   if (int rc = d2h(ctx, hf, ctx->flags, 2)) return rc;
   if (int rc = sync_check(ctx)) return rc;
   if (fail_node) *fail_node = (hf[0] == INT_MAX) ? 0 : hf[0];
+  if (hf[0] != INT_MAX) {
+    char buf[128];
+    snprintf(buf, sizeof buf, "NR SEARCH FAILED! (fault location %d)", hf[0]);
+    ctx->err = buf;
+    return EQ_NR_NAN;
+  }
+  if (hf[1] > 0) {
+    ctx->err = "NRsearch! hit |delta| > 1e10 or k == 1000 (last iterate returned)";
+    return EQ_NR_NOCONV;
+  }
+  return EQ_OK;
+}
+
+// The dynamic branch of the loop body on caller arrays: one launch of the
+// Newmark kernel the device loop uses, on a scratch copy of the state.
+extern "C" int eq_newmark_step(eq_ctx* ctx, double* d, double* v, double* a, double* psi, double* tau, double* Vf,
+                               double* FltVfree, double dt) {
+  if (!ctx || !d || !v || !a || !psi) return EQ_EINVAL;
+  if (!ctx->have_material || !ctx->have_params) return fail(ctx, EQ_ENOTREADY, "eq_set_material and eq_set_params first");
+  if (!(dt > 0)) return fail(ctx, EQ_EINVAL, "dt must be positive");
+  const int64_t N = ctx->N, NX = ctx->NX;
+  if (!ctx->api_nodal) {
+    ctx->api_nodal = ctx->alloc<double>(6 * N);
+    NEED(ctx->api_nodal);
+  }
+  double* in = ctx->api_nodal;
+  double* out = ctx->api_nodal + 3 * N;
+  if (int rc = upload_ref(ctx, d, in)) return rc;
+  if (int rc = upload_ref(ctx, v, in + N)) return rc;
+  if (int rc = upload_ref(ctx, a, in + 2 * N)) return rc;
+  FaultState& f = ctx->fs_api;
+  // nodes deeper than NFBC keep their psi (FBC! does not visit them)
+  if (int rc = h2d(ctx, f.psi, psi, NX)) return rc;
+  if (int rc = h2d(ctx, f.psi2, psi, NX)) return rc;
+  int hf[2] = {INT_MAX, 0};
+  ctx->flags_dirty = true;
+  if (int rc = h2d(ctx, ctx->flags, hf, 2)) return rc;
+  launch_newmark(ctx->g, ctx->fp, f, ctx->mu, ctx->M, in, in + N, in + 2 * N, out, out + N, out + 2 * N, dt,
+                 0.5 * ctx->dt0, 0.5 * ctx->dt0 * ctx->dt0, ctx->flags, NewmarkRecord{}, ctx->s);
+  if (int rc = d2h(ctx, hf, ctx->flags, 2)) return rc;
+  if (int rc = download_ref(ctx, out, d)) return rc;
+  if (int rc = download_ref(ctx, out + N, v)) return rc;
+  if (int rc = download_ref(ctx, out + 2 * N, a)) return rc;
+  if (int rc = d2h(ctx, psi, f.psi, NX)) return rc;
+  if (tau)
+    if (int rc = d2h(ctx, tau, f.tau, NX)) return rc;
+  if (Vf)
+    if (int rc = d2h(ctx, Vf, f.Vf, NX)) return rc;
+  if (FltVfree)
+    if (int rc = d2h(ctx, FltVfree, f.FltVfree, NX)) return rc;
+  if (int rc = sync_check(ctx)) return rc;
   if (hf[0] != INT_MAX) {
     char buf[128];
     snprintf(buf, sizeof buf, "NR SEARCH FAILED! (fault location %d)", hf[0]);

@@ -164,6 +164,17 @@ typedef struct {
 } eq_solver_opts;
 int eq_set_solver(eq_ctx *ctx, const eq_solver_opts *opts);
 
+/* The dynamic branch of the loop body (src/main.jl:297-344) on caller
+ * arrays: predictor, a = -K d with the absorbing edges, FBC! on the fault row
+ * (src/NRsearch_serial.jl:6-45), corrector.  d, v, a [nglob] (reference
+ * numbering) are advanced by one step of length dt (the loop's dtmin); psi
+ * [FltNglob] becomes psi2 (nodes deeper than NFBC keep theirs); tau receives
+ * tau2 - tauo, Vf the slip rate 2 v[iFlt] + Vpl before the step, FltVfree
+ * the free slip velocity (each of the three may be NULL).  Status as
+ * eq_fbc. */
+int eq_newmark_step(eq_ctx *ctx, double *d, double *v, double *a, double *psi, double *tau, double *Vf,
+                    double *FltVfree, double dt);
+
 /* Initial state (src/main.jl:99-190): v = v_init everywhere then -Vpl/2,
  * d = a = 0, psi = psi_init (NULL: the formula of src/main.jl:125). */
 int eq_state_init(eq_ctx *ctx, double v_init, const double *psi_init);

@@ -211,4 +211,22 @@ function dtevol!(ctx::Context, dt, dtmin, XiLf, FaultNglob, NFBC, Vf, isolver)
     out[]
 end
 
+# The dynamic branch of main.jl's loop body (src/main.jl:297-344) in one call:
+# d, v, a (nglob) and psi (FltNglob) are advanced in place; tau and Vf receive
+# the new fault traction and the pre-step slip rate.  Returns nothing after
+# the reference's NR SEARCH FAILED! report, as FBC! does.
+function newmark_step!(ctx::Context, d, v, a, psi, tau, Vf, FltVfree, dt)
+    rc = ccall((:eq_newmark_step, LIB), Cint,
+               (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+                Ptr{Float64}, Cdouble),
+               ctx.h, d, v, a, psi, tau, Vf, FltVfree, dt)
+    if rc == EQ_NR_NAN
+        println(lasterr(ctx))
+        @error("NR SEARCH FAILED!")
+        return
+    end
+    rc == EQ_NR_NOCONV || check(ctx, rc)
+    return d, v, a, psi, tau
+end
+
 end # module

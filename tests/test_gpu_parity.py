@@ -531,3 +531,42 @@ def test_nr_failure_stops_run_like_oracle(oracle):
     # an ordinary run reports no error
     ctx.state_init()
     assert ctx.run(max_steps=3, snapshots=False, keep_on_nr_failure=True)["error"] is None
+
+
+def test_newmark_step_entry_point_matches_loop_and_oracle(oracle):
+    """eq_newmark_step (the dynamic branch, src/main.jl:297-344, on caller
+    arrays) from a mid-event state: bit for bit the loop's own next step, and
+    the oracle's Newmark step from the same state to 1e-10."""
+    from eqcycle_amd.api import EqContext
+    P = set_parameters(small(32, 8, material="trapezoid", total_years=3))
+    ctx = EqContext(P, amg=True)
+    ctx.set_solver(tol=1e-12, pc="amg")
+    ctx.state_init()
+    ndyn = 0
+    for _ in range(800):
+        info = ctx.step()
+        ndyn += info.isolver_used == 2
+        if ndyn >= 20 and info.isolver_next == 2:
+            break
+    s = ctx.state_get()
+    assert s["isolver"] == 2 and ndyn >= 20
+    rc, out = ctx.newmark_step(s["d"], s["v"], s["a"], s["psi"], dt=s["dt"])
+    assert rc == 0
+    info = ctx.step()
+    assert info.isolver_used == 2
+    s2 = ctx.state_get()
+    j0 = P.NFBC - 1
+    for k in ("d", "v", "a"):
+        assert np.array_equal(out[k], s2[k]), k
+    assert np.array_equal(out["psi"][j0:], s2["psi"][j0:])
+    assert np.array_equal(out["tau"], s2["tau"])
+    sim = oracle.Simulation(P, solver="direct")
+    sim.d, sim.v, sim.a = s["d"].copy(), s["v"].copy(), s["a"].copy()
+    sim.psi, sim.dt = s["psi"].copy(), s["dt"]
+    sim.dyn_step()
+    for k in ("d", "v"):
+        assert rel(out[k], getattr(sim, k)) < 1e-10, k
+    assert rel(out["a"], sim.a) < 1e-9
+    assert rel(out["psi"][j0:], sim.psi[j0:]) < 1e-10
+    assert rel(out["tau"], sim.tau) < 1e-10
+    assert rel(out["FltVfree"], sim.FltVfree) < 1e-10
