@@ -143,6 +143,29 @@ class EqContext:
         self._check(self.lib.eq_amg_info(self.h, C.byref(nl), ptr(n, C.c_int64), ptr(nnz, C.c_int64), 32))
         return list(zip(n[:nl.value].tolist(), nnz[:nl.value].tolist()))
 
+    def amg_upload(self, A, P, omega=None, smoother="jacobi", degree=1):
+        """Use a host-built hierarchy (ruge_stuben(kni), src/main.jl:200-201):
+        A = [A_0 .. A_{L-1}] and P = [P_0 .. P_{L-2}] as scipy sparse matrices,
+        level 0 in FltNI order; omega: Jacobi weights per level (None:
+        estimated).  See eq_amg_upload."""
+        import scipy.sparse as sp
+        keep, descs = [], []
+        for M in list(A) + list(P):
+            M = sp.csr_matrix(M)
+            M.sort_indices()
+            rp = np.ascontiguousarray(M.indptr, dtype=np.int64)
+            cl = np.ascontiguousarray(M.indices, dtype=np.int64)
+            vl = np.ascontiguousarray(M.data, dtype=np.float64)
+            keep += [rp, cl, vl]
+            descs.append(L.CsrDesc(M.shape[0], M.shape[1], ptr(rp, C.c_int64), ptr(cl, C.c_int64), ptr(vl)))
+        nA = len(A)
+        Aarr = (L.CsrDesc * nA)(*descs[:nA])
+        Parr = (L.CsrDesc * max(1, len(P)))(*descs[nA:])
+        om = None if omega is None else f64(omega)
+        sm = {"jacobi": L.EQ_SMOOTH_JACOBI, "chebyshev": L.EQ_SMOOTH_CHEBYSHEV}[smoother]
+        self._check(self.lib.eq_amg_upload(self.h, nA, Aarr, Parr, ptr(om), sm, degree))
+        self.amg_levels = nA
+
     def amg_ldiv(self, b, prec="fp64"):
         b = f64(b)
         x = np.empty_like(b)

@@ -423,12 +423,25 @@ int amg_build(const HostCsr& A0, const AmgOptions& opt, HostHierarchy* H, std::s
   if (sgn < 0)
     for (auto& L : H->lev)
       for (auto& v : L.A.val) v = -v;
-  for (auto& L : H->lev) {
+  return finish_hierarchy(H, nullptr, err);
+}
+
+// D^-1, lambda_max(D^-1 A) (or the caller's smoothing weights) and the dense
+// coarsest inverse of a hierarchy whose operators and prolongations are set.
+int finish_hierarchy(HostHierarchy* H, const double* omega, std::string* err) {
+  for (size_t l = 0; l < H->lev.size(); l++) {
+    HostLevel& L = H->lev[l];
     L.dinv.assign(L.A.n, 0.0);
     for (int64_t i = 0; i < L.A.n; i++)
       for (int64_t q = L.A.rowptr[i]; q < L.A.rowptr[i + 1]; q++)
         if (L.A.col[q] == i) L.dinv[i] = 1.0 / L.A.val[q];
-    L.lambda_max = estimate_lambda_max_dinvA(L.A, L.dinv, 30);
+    for (int64_t i = 0; i < L.A.n; i++)
+      if (!(L.dinv[i] > 0) || !std::isfinite(L.dinv[i])) {
+        if (err) *err = "AMG: level " + std::to_string(l) + " has a zero or negative diagonal entry";
+        return EQ_EINVAL;
+      }
+    // omega = 4 / (3 lambda_max) is the Jacobi weight the V-cycle uses
+    L.lambda_max = omega ? 4.0 / (3.0 * omega[l]) : estimate_lambda_max_dinvA(L.A, L.dinv, 30);
   }
   const HostCsr& Ac = H->lev.back().A;
   H->ncoarse = Ac.n;

@@ -632,29 +632,14 @@ static int alloc_level_vecs(eq_ctx* ctx, DevLevelVecs<T>& v, int64_t n) {
   return EQ_OK;
 }
 
-extern "C" int eq_amg_setup(eq_ctx* ctx, double theta, int max_levels, int64_t max_coarse, int strength_abs,
-                            int smoother, int smooth_degree, int pcoarsen) {
-  if (!ctx) return EQ_EINVAL;
-  if (!ctx->have_material) return fail(ctx, EQ_ENOTREADY, "eq_set_material first");
-  if (smoother != EQ_SMOOTH_JACOBI && smoother != EQ_SMOOTH_CHEBYSHEV) return fail(ctx, EQ_EINVAL, "bad smoother");
-  if (smoother == EQ_SMOOTH_CHEBYSHEV && (smooth_degree < 1 || smooth_degree > 8))
-    return fail(ctx, EQ_EINVAL, "Chebyshev degree must be 1..8");
-  if (max_levels < 2) return fail(ctx, EQ_EINVAL, "max_levels must be >= 2");
-  HostCsr Kii;
-  assemble_kii(ctx->g, ctx->mu_host.data(), &Kii);
-  AmgOptions opt;
-  opt.theta = theta;
-  opt.max_levels = max_levels;
-  opt.max_coarse = max_coarse;
-  opt.strength_abs = strength_abs;
-  HostHierarchy H;
-  std::string err;
-  if (pcoarsen) {
-    if (int rc = pmg_build(ctx->g, Kii, opt, &H, &err)) return fail(ctx, rc, err);
-  } else {
-    if (int rc = amg_build(Kii, opt, &H, &err)) return fail(ctx, rc, err);
-  }
+// Device copy of a host hierarchy: level 0 stays matrix-free (K_II from mu)
+// with Jacobi or Chebyshev smoothing; levels 1 .. L-2 as condensed sweeps;
+// the last level's dense inverse.
+static int upload_hierarchy(eq_ctx* ctx, const HostHierarchy& H, int smoother, int smooth_degree, int pcoarsen) {
   if (H.lev.size() < 2) return fail(ctx, EQ_EINVAL, "AMG hierarchy has a single level; use the Jacobi preconditioner");
+  ctx->have_amg = false;
+  for (auto& kv : ctx->cg_graphs) (void)hipGraphExecDestroy(kv.second);   // they hold the old levels
+  ctx->cg_graphs.clear();
   ctx->l0_cheb = smoother == EQ_SMOOTH_CHEBYSHEV;
   ctx->l0_degree = ctx->l0_cheb ? smooth_degree : 1;
   {
@@ -709,6 +694,132 @@ extern "C" int eq_amg_setup(eq_ctx* ctx, double theta, int max_levels, int64_t m
   ctx->pmg = pcoarsen ? 1 : 0;
   ctx->have_amg = true;
   return sync_check(ctx);
+}
+
+extern "C" int eq_amg_setup(eq_ctx* ctx, double theta, int max_levels, int64_t max_coarse, int strength_abs,
+                            int smoother, int smooth_degree, int pcoarsen) {
+  if (!ctx) return EQ_EINVAL;
+  if (!ctx->have_material) return fail(ctx, EQ_ENOTREADY, "eq_set_material first");
+  if (smoother != EQ_SMOOTH_JACOBI && smoother != EQ_SMOOTH_CHEBYSHEV) return fail(ctx, EQ_EINVAL, "bad smoother");
+  if (smoother == EQ_SMOOTH_CHEBYSHEV && (smooth_degree < 1 || smooth_degree > 8))
+    return fail(ctx, EQ_EINVAL, "Chebyshev degree must be 1..8");
+  if (max_levels < 2) return fail(ctx, EQ_EINVAL, "max_levels must be >= 2");
+  HostCsr Kii;
+  assemble_kii(ctx->g, ctx->mu_host.data(), &Kii);
+  AmgOptions opt;
+  opt.theta = theta;
+  opt.max_levels = max_levels;
+  opt.max_coarse = max_coarse;
+  opt.strength_abs = strength_abs;
+  HostHierarchy H;
+  std::string err;
+  if (pcoarsen) {
+    if (int rc = pmg_build(ctx->g, Kii, opt, &H, &err)) return fail(ctx, rc, err);
+  } else {
+    if (int rc = amg_build(Kii, opt, &H, &err)) return fail(ctx, rc, err);
+  }
+  return upload_hierarchy(ctx, H, smoother, smooth_degree, pcoarsen);
+}
+
+// A hierarchy the caller built (the reference's ruge_stuben(kni)).
+extern "C" int eq_amg_upload(eq_ctx* ctx, int nlev, const eq_csr* A, const eq_csr* P, const double* omega,
+                             int smoother, int smooth_degree) {
+  if (!ctx || !A || !P) return EQ_EINVAL;
+  if (!ctx->have_material) return fail(ctx, EQ_ENOTREADY, "eq_set_material first");
+  if (nlev < 2 || nlev > 32) return fail(ctx, EQ_EINVAL, "nlev must be 2..32");
+  if (smoother != EQ_SMOOTH_JACOBI && smoother != EQ_SMOOTH_CHEBYSHEV) return fail(ctx, EQ_EINVAL, "bad smoother");
+  if (smoother == EQ_SMOOTH_CHEBYSHEV && (smooth_degree < 1 || smooth_degree > 8))
+    return fail(ctx, EQ_EINVAL, "Chebyshev degree must be 1..8");
+  if (omega)
+    for (int l = 0; l < nlev; l++)
+      if (!(omega[l] > 0) || !std::isfinite(omega[l])) return fail(ctx, EQ_EINVAL, "omega must be positive");
+  const int64_t nI = ctx->nI, NX = ctx->NX;
+  auto take = [&](const eq_csr& c, int64_t n, int64_t m, const std::string& what, HostCsr* out) -> int {
+    if (c.n != n || c.m != m || !c.rowptr) return fail(ctx, EQ_EINVAL, what + ": wrong shape");
+    if (c.rowptr[0] != 0) return fail(ctx, EQ_EINVAL, what + ": rowptr[0] must be 0");
+    for (int64_t i = 0; i < n; i++)
+      if (c.rowptr[i + 1] < c.rowptr[i]) return fail(ctx, EQ_EINVAL, what + ": rowptr not monotone");
+    const int64_t nnz = c.rowptr[n];
+    if (nnz > 0 && (!c.col || !c.val)) return fail(ctx, EQ_EINVAL, what + ": missing col / val");
+    if (m > INT32_MAX) return fail(ctx, EQ_EINVAL, what + ": too many columns");
+    out->n = n;
+    out->m = m;
+    out->rowptr.assign(c.rowptr, c.rowptr + n + 1);
+    out->col.resize(nnz);
+    out->val.assign(c.val, c.val + nnz);
+    for (int64_t q = 0; q < nnz; q++) {
+      if (c.col[q] < 0 || c.col[q] >= m) return fail(ctx, EQ_EINVAL, what + ": column index out of range");
+      out->col[q] = (int32_t)c.col[q];
+    }
+    return EQ_OK;
+  };
+  std::vector<int64_t> sz(nlev);
+  for (int l = 0; l < nlev; l++) sz[l] = A[l].n;
+  if (sz[0] != nI) return fail(ctx, EQ_EINVAL, "A[0] must be kni, of size nI");
+  HostHierarchy H;
+  H.lev.resize(nlev);
+  // level 0: the caller's A_0 (FltNI order) must be kni = -K_II or K_II of this mesh
+  HostCsr A0, Kii;
+  if (int rc = take(A[0], nI, nI, "A[0]", &A0)) return rc;
+  assemble_kii(ctx->g, ctx->mu_host.data(), &Kii);
+  std::vector<int64_t> rmap(nI);   // FltNI index -> structured interior index
+  for (int64_t k = 0; k < nI; k++) rmap[k] = ctx->fltni[k] - NX;
+  double sgn = 1.0;
+  {
+    std::vector<double> x(nI), xs(nI), y0(nI, 0.0), y1(nI, 0.0);
+    uint64_t st = 0x9e3779b97f4a7c15ull;
+    for (int64_t k = 0; k < nI; k++) {
+      st = st * 6364136223846793005ull + 1442695040888963407ull;
+      x[k] = (double)(st >> 11) / 9007199254740992.0 - 0.5;
+      xs[rmap[k]] = x[k];
+    }
+    for (int64_t i = 0; i < nI; i++) {
+      for (int64_t q = A0.rowptr[i]; q < A0.rowptr[i + 1]; q++) y0[i] += A0.val[q] * x[A0.col[q]];
+      for (int64_t q = Kii.rowptr[i]; q < Kii.rowptr[i + 1]; q++) y1[i] += Kii.val[q] * xs[Kii.col[q]];
+    }
+    double dot = 0.0, ymax = 0.0;
+    for (int64_t k = 0; k < nI; k++) {
+      dot += y0[k] * y1[rmap[k]];
+      ymax = std::max(ymax, std::fabs(y1[k]));
+    }
+    sgn = dot < 0 ? -1.0 : 1.0;
+    double e = 0.0;
+    for (int64_t k = 0; k < nI; k++) e = std::max(e, std::fabs(sgn * y0[k] - y1[rmap[k]]));
+    if (!(e <= 1e-10 * ymax))
+      return fail(ctx, EQ_EINVAL, "A[0] is not kni (= -K[FltNI, FltNI]) or K[FltNI, FltNI] of this mesh");
+  }
+  H.lev[0].A = std::move(Kii);
+  {
+    HostCsr P0;
+    if (int rc = take(P[0], nI, sz[1], "P[0]", &P0)) return rc;
+    // rows to the structured interior order
+    HostCsr& Q = H.lev[0].P;
+    Q.n = nI;
+    Q.m = P0.m;
+    std::vector<int64_t> inv(nI);
+    for (int64_t k = 0; k < nI; k++) inv[rmap[k]] = k;
+    Q.rowptr.assign(nI + 1, 0);
+    for (int64_t r = 0; r < nI; r++) Q.rowptr[r + 1] = Q.rowptr[r] + (P0.rowptr[inv[r] + 1] - P0.rowptr[inv[r]]);
+    Q.col.resize(P0.col.size());
+    Q.val.resize(P0.val.size());
+    for (int64_t r = 0; r < nI; r++) {
+      const int64_t k = inv[r];
+      std::copy(P0.col.begin() + P0.rowptr[k], P0.col.begin() + P0.rowptr[k + 1], Q.col.begin() + Q.rowptr[r]);
+      std::copy(P0.val.begin() + P0.rowptr[k], P0.val.begin() + P0.rowptr[k + 1], Q.val.begin() + Q.rowptr[r]);
+    }
+    csr_transpose(Q, &H.lev[0].R);
+  }
+  for (int l = 1; l < nlev; l++) {
+    if (int rc = take(A[l], sz[l], sz[l], "A[" + std::to_string(l) + "]", &H.lev[l].A)) return rc;
+    for (auto& v : H.lev[l].A.val) v *= sgn;
+    if (l + 1 < nlev) {
+      if (int rc = take(P[l], sz[l], sz[l + 1], "P[" + std::to_string(l) + "]", &H.lev[l].P)) return rc;
+      csr_transpose(H.lev[l].P, &H.lev[l].R);
+    }
+  }
+  std::string err;
+  if (int rc = finish_hierarchy(&H, omega, &err)) return fail(ctx, rc, err);
+  return upload_hierarchy(ctx, H, smoother, smooth_degree, 0);
 }
 
 extern "C" int eq_amg_info(const eq_ctx* ctx, int* nlev, int64_t* n, int64_t* nnz, int cap) {

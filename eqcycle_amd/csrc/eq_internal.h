@@ -57,6 +57,10 @@ struct HostHierarchy {
 
 // amg_host.cpp
 int amg_build(const HostCsr& A, const AmgOptions& opt, HostHierarchy* out, std::string* err);
+// D^-1, lambda_max(D^-1 A) (omega: the caller's Jacobi weights per level, or
+// null to estimate) and the dense coarsest inverse, once every level's A, P
+// and R are set.
+int finish_hierarchy(HostHierarchy* H, const double* omega, std::string* err);
 void csr_transpose(const HostCsr& A, HostCsr* T);
 double estimate_lambda_max_dinvA(const HostCsr& A, const std::vector<double>& dinv, int iters);
 // K_II in structured interior order (reduced index r = n - NX), from nodal mu

@@ -31,7 +31,7 @@ EXPORTS = [
     "eq_slr", "eq_dtevol", "eq_set_solver", "eq_state_init", "eq_state_get", "eq_state_set",
     "eq_step", "eq_run", "eq_time_kernel", "eq_learn_init", "eq_learn_record", "eq_learn_info",
     "eq_learn_gram", "eq_learn_set_basis", "eq_learn_train", "eq_learn_loss_grad", "eq_learn_get",
-    "eq_learn_predict", "eq_learn_norms", "eq_learn_galerkin", "eq_newmark_step",
+    "eq_learn_predict", "eq_learn_norms", "eq_learn_galerkin", "eq_newmark_step", "eq_amg_upload",
 ]
 EQ_TIME_KU, EQ_TIME_CG_DIR, EQ_TIME_VCYCLE, EQ_TIME_CG_ITER, EQ_TIME_NEWMARK, EQ_TIME_CHEB, EQ_TIME_COPY, \
     EQ_TIME_CHEB2 = range(8)
@@ -61,6 +61,11 @@ class SolverOpts(C.Structure):
     _fields_ = [("tol", C.c_double), ("maxiter", C.c_int64), ("pc", C.c_int), ("prec", C.c_int),
                 ("guess", C.c_int), ("record_snapshots", C.c_int), ("graphs", C.c_int),
                 ("cheb_single", C.c_int)]
+
+
+class CsrDesc(C.Structure):
+    _fields_ = [("n", C.c_int64), ("m", C.c_int64), ("rowptr", C.POINTER(C.c_int64)),
+                ("col", C.POINTER(C.c_int64)), ("val", C.POINTER(C.c_double))]
 
 
 class StepInfo(C.Structure):
@@ -119,6 +124,7 @@ def load():
         "eq_dtevol": ([vp, C.c_double, C.c_double, dp, C.c_int64, C.c_int64, dp, C.c_int64, dp], C.c_int),
         "eq_set_solver": ([vp, C.POINTER(SolverOpts)], C.c_int),
         "eq_newmark_step": ([vp, dp, dp, dp, dp, dp, dp, dp, C.c_double], C.c_int),
+        "eq_amg_upload": ([vp, C.c_int, C.POINTER(CsrDesc), C.POINTER(CsrDesc), dp, C.c_int, C.c_int], C.c_int),
         "eq_state_init": ([vp, C.c_double, dp], C.c_int),
         "eq_state_get": ([vp, dp, dp, dp, dp, dp, dp, dp, dp, i64p, i64p], C.c_int),
         "eq_state_set": ([vp, dp, dp, dp, dp, dp, C.c_double, C.c_double, C.c_int64], C.c_int),

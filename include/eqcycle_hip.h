@@ -107,6 +107,27 @@ int eq_element_computation(eq_ctx *ctx, const double *u, double *a_elem);
  * (Jacobi on the assembled coarse levels). */
 int eq_amg_setup(eq_ctx *ctx, double theta, int max_levels, int64_t max_coarse, int strength_abs,
                  int smoother, int smooth_degree, int pcoarsen);
+/* A sparse matrix in 0-based CSR: rowptr[n + 1], col and val[rowptr[n]]. */
+typedef struct {
+  int64_t n, m;
+  const int64_t *rowptr;
+  const int64_t *col;
+  const double *val;
+} eq_csr;
+/* Use a hierarchy built on the host by the caller, as the reference's
+ * ruge_stuben(kni) builds it (src/main.jl:200-201), instead of eq_amg_setup's
+ * own: A[0 .. nlev-1] the level operators, P[0 .. nlev-2] the prolongations
+ * (A[l+1] = P[l]^T A[l] P[l]), rows and columns of level 0 in FltNI order.
+ * A[0] must be kni (= -K[FltNI, FltNI]) or K[FltNI, FltNI] of this mesh (checked
+ * by a product): the device applies level 0 matrix-free and the coarser
+ * levels as given (negated with A[0]), with one Jacobi sweep (weight omega[l],
+ * or 4 / (3 lambda_max(D^-1 A)) estimated when omega is NULL) before and after
+ * each coarse correction, or Chebyshev of smooth_degree on level 0, and a
+ * dense solve on the last level.  From a Julia ml = ruge_stuben(kni): A[l] is
+ * ml.levels[l].A (or ml.final_A), and the CSC arrays of ml.levels[l].R are the
+ * CSR arrays of P[l]. */
+int eq_amg_upload(eq_ctx *ctx, int nlev, const eq_csr *A, const eq_csr *P, const double *omega, int smoother,
+                  int smooth_degree);
 /* Per-level sizes: n[l], nnz[l] for l < *nlev (arrays of capacity cap). */
 int eq_amg_info(const eq_ctx *ctx, int *nlev, int64_t *n, int64_t *nnz, int cap);
 /* x = Pl \ b for kni (ldiv!(x, p, b)); x, b in FltNI order. */

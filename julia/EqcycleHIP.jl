@@ -19,6 +19,7 @@
 module EqcycleHIP
 
 using LinearAlgebra
+using SparseArrays
 
 const LIB = get(ENV, "EQCYCLE_HIP_LIB", joinpath(@__DIR__, "..", "eqcycle_amd", "libeqcycle_hip.so"))
 
@@ -29,6 +30,7 @@ const EQ_PCG_NOCONV = 5
 const EQ_MASK_ALL, EQ_MASK_INTERIOR, EQ_MASK_NEG = 0, 1, 2
 const EQ_PREC_FP64, EQ_PREC_FP32, EQ_PREC_MIXED, EQ_PREC_F32PC = 0, 1, 2, 3
 const EQ_PC_NONE, EQ_PC_JACOBI, EQ_PC_AMG = 0, 1, 2
+const EQ_SMOOTH_JACOBI, EQ_SMOOTH_CHEBYSHEV = 0, 1
 
 struct MeshDesc
     ngll::Int64
@@ -105,6 +107,37 @@ function set_params!(ctx::Context, P)
 end
 
 # ---- operators: the AbstractMatrix protocol cg! needs (test_01.jl:9-27) ------
+struct EqCsr
+    n::Int64
+    m::Int64
+    rowptr::Ptr{Int64}
+    col::Ptr{Int64}
+    val::Ptr{Float64}
+end
+
+# Keep the reference's own hierarchy: ml = ruge_stuben(kni) (src/main.jl:200)
+# goes to the device level by level.  A symmetric level operator's CSC arrays
+# are its CSR arrays; the CSC arrays of R = P' are the CSR arrays of P.
+function upload_amg!(ctx::Context, ml; smoother = EQ_SMOOTH_JACOBI, degree = 1, omega = nothing)
+    As = SparseMatrixCSC[lvl.A for lvl in ml.levels]
+    push!(As, ml.final_A)
+    keep = Any[]
+    function desc(M::SparseMatrixCSC, n, m)
+        rp = Int64.(M.colptr) .- 1
+        cl = Int64.(M.rowval) .- 1
+        v = Vector{Float64}(M.nzval)
+        push!(keep, rp, cl, v)
+        EqCsr(n, m, pointer(rp), pointer(cl), pointer(v))
+    end
+    Ad = [desc(A, size(A, 1), size(A, 2)) for A in As]
+    Pd = [desc(lvl.R, size(lvl.R, 2), size(lvl.R, 1)) for lvl in ml.levels]
+    om = omega === nothing ? Ptr{Float64}(C_NULL) : pointer(omega)
+    GC.@preserve keep omega check(ctx, ccall((:eq_amg_upload, LIB), Cint,
+                                             (Ptr{Cvoid}, Cint, Ptr{EqCsr}, Ptr{EqCsr}, Ptr{Float64}, Cint, Cint),
+                                             ctx.h, length(Ad), Ad, Pd, om, smoother, degree))
+    return AmgPreconditioner(ctx)
+end
+
 struct KOp <: AbstractMatrix{Float64}   # Ksparse' (mask EQ_MASK_ALL) or nKsparse' (EQ_MASK_NEG)
     ctx::Context
     mask::Int

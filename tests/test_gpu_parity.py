@@ -167,6 +167,52 @@ def test_cheb_pairs_run_equals_single_steps():
         assert np.array_equal(sa[k], sb[k]), k
 
 
+def test_amg_upload_matches_numpy_vcycle(mesh, oracle):
+    """eq_amg_upload with the oracle's Ruge-Stueben hierarchy of kni (what the
+    reference's ruge_stuben(kni) hands over, src/main.jl:200): the device
+    V-cycle equals the same V-cycle in numpy (one Jacobi sweep of weight
+    omega_l before and after each coarse correction, dense solve on the last
+    level) to 1e-12, and CG with it, or with Chebyshev on level 0, solves to
+    tolerance."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    from eqcycle_amd.api import EqContext
+    P, _, _, kni = mesh
+    ctx = EqContext(P)
+    ml = oracle.Amg(kni)
+    L = ml.nlev
+    assert L >= 3
+    A = [ml.level_csr(l, 0) for l in range(L)]
+    Pm = [ml.level_csr(l, 1) for l in range(L - 1)]
+    om = []
+    for Al in A:
+        s = sp.diags(1.0 / np.sqrt(np.abs(Al.diagonal())))
+        S = (s @ (-Al) @ s).toarray()                 # similar to D^-1 A (A negative definite)
+        om.append(4.0 / (3.0 * np.linalg.eigvalsh(S)[-1]))
+    ctx.amg_upload(A, Pm, omega=om)
+    b = normal(43, A[0].shape[0])
+
+    def vcycle(l, rhs):
+        if l == L - 1:
+            return np.linalg.solve(A[l].toarray(), rhs)
+        dinv = 1.0 / A[l].diagonal()
+        x = om[l] * dinv * rhs
+        e = vcycle(l + 1, Pm[l].T @ (rhs - A[l] @ x))
+        x = x + Pm[l] @ e
+        return x + om[l] * dinv * (rhs - A[l] @ x)
+
+    assert rel(ctx.amg_ldiv(b), vcycle(0, b)) < 1e-12
+    Ad = kni.to_scipy().tocsc()
+    for smoother, degree in (("jacobi", 1), ("chebyshev", 3)):
+        ctx.amg_upload(A, Pm, omega=om if smoother == "jacobi" else None, smoother=smoother, degree=degree)
+        x, it, rr, ok = ctx.cg(np.zeros_like(b), b, tol=1e-10, pc="amg")
+        print(f"[gpu] uploaded hierarchy ({L} levels), {smoother}: {it} iterations", flush=True)
+        assert ok and np.linalg.norm(b - Ad @ x) <= 1e-10 * np.linalg.norm(b)
+    # a level-0 operator that is not this mesh's kni is refused
+    with pytest.raises(Exception, match="not kni"):
+        ctx.amg_upload([1.01 * A[0]] + A[1:], Pm)
+
+
 def test_amg_ldiv_is_spd(mesh):
     P, ctx, _, kni = mesh
     n = P.nglob - P.FltNglob
