@@ -24,7 +24,8 @@ log = logging.getLogger("eqcycle_amd")
 
 PREC = {"fp64": L.EQ_PREC_FP64, "fp32": L.EQ_PREC_FP32, "mixed": L.EQ_PREC_MIXED, "f32pc": L.EQ_PREC_F32PC}
 PC = {"none": L.EQ_PC_NONE, "jacobi": L.EQ_PC_JACOBI, "amg": L.EQ_PC_AMG}
-MASK = {"all": L.EQ_MASK_ALL, "interior": L.EQ_MASK_INTERIOR, "neg": L.EQ_MASK_NEG}
+MASK = {"all": L.EQ_MASK_ALL, "interior": L.EQ_MASK_INTERIOR, "neg": L.EQ_MASK_NEG,
+        "fault_rows": L.EQ_MASK_FAULT_ROWS}
 
 
 class EqError(RuntimeError):

@@ -551,12 +551,22 @@ extern "C" int eq_set_params(eq_ctx* ctx, const eq_params_desc* p) {
 extern "C" int eq_ku_apply(eq_ctx* ctx, const double* u, double* y, int mask, int prec) {
   if (!ctx || !u || !y) return EQ_EINVAL;
   if (!ctx->have_material) return fail(ctx, EQ_ENOTREADY, "eq_set_material first");
-  if (mask < EQ_MASK_ALL || mask > EQ_MASK_NEG) return fail(ctx, EQ_EINVAL, "bad mask");
+  if (mask < EQ_MASK_ALL || mask > EQ_MASK_FAULT_ROWS) return fail(ctx, EQ_EINVAL, "bad mask");
   const int interior = mask == EQ_MASK_INTERIOR, neg = mask == EQ_MASK_NEG;
   double* du = ctx->w64.x;
   double* dy = ctx->w64.q;
   if (int rc = upload_ref(ctx, u, du)) return rc;
-  if (prec == EQ_PREC_FP32) {
+  if (mask == EQ_MASK_FAULT_ROWS) {
+    launch_fill<double>(ctx->N, dy, 0.0, ctx->s);
+    if (prec == EQ_PREC_FP32) {
+      launch_convert<double, float>(ctx->N, du, ctx->w32.x, ctx->s);
+      launch_fill<float>(ctx->N, ctx->w32.q, 0.0f, ctx->s);
+      launch_ku_rows<float>(ctx->g, ctx->w32.x, ctx->mu32, ctx->w32.q, 5, ctx->s);
+      launch_convert<float, double>(ctx->N, ctx->w32.q, dy, ctx->s);
+    } else {
+      launch_ku_rows<double>(ctx->g, du, ctx->mu, dy, 5, ctx->s);
+    }
+  } else if (prec == EQ_PREC_FP32) {
     launch_convert<double, float>(ctx->N, du, ctx->w32.x, ctx->s);
     launch_ku<float>(ctx->g, ctx->w32.x, ctx->mu32, ctx->w32.q, interior, neg, ctx->s);
     launch_convert<float, double>(ctx->N, ctx->w32.q, dy, ctx->s);

@@ -134,6 +134,10 @@ struct ChebStep {
 template <typename T>
 void launch_cheb(const Geom& g, const ChebStep<T>& c, const T* mu, const T* dinv, const double* part_rr, int nrr,
                  CgScal* sc, double* part_rho, hipStream_t s);
+// y = K u on the first nrows grid rows (the element row along the fault for
+// nrows = 5), from global memory; other entries of y untouched.
+template <typename T>
+void launch_ku_rows(const Geom& g, const T* u, const T* mu, T* y, int nrows, hipStream_t s);
 // Steps a and b (a in mode 0, b reading a's outputs) in one launch; b's
 // outputs are written, a's stay on chip except a.xout when b is in mode 1.
 // b.rout and b.dout must not alias a.rin or a.src.

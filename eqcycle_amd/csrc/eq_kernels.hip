@@ -1571,6 +1571,22 @@ void launch_cheb(const Geom& g, const ChebStep<T>& c, const T* mu, const T* dinv
   hipLaunchKernelGGL((tile_kernel<T, OpCheb<T>>), tile_grid(g), dim3(TBLOCK), 0, s, g, op);
 }
 
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void ku_rows_kernel(Geom g, const T* u, const T* mu, T* y, int nrows) {
+  const int64_t k = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (k >= (int64_t)nrows * g.NX) return;
+  const int J = (int)(k / g.NX), I = (int)(k - (int64_t)J * g.NX);
+  y[k] = ku_global<T>(g, u, mu, I, J);
+}
+
+template <typename T>
+void launch_ku_rows(const Geom& g, const T* u, const T* mu, T* y, int nrows, hipStream_t s) {
+  nrows = nrows < g.NY ? nrows : g.NY;
+  const int64_t n = (int64_t)nrows * g.NX;
+  hipLaunchKernelGGL((ku_rows_kernel<T>), dim3((unsigned)((n + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, s, g, u, mu, y,
+                     nrows);
+}
+
 // --------------------------------------------------------------------------
 // Two consecutive level-0 Chebyshev steps in one launch
 // --------------------------------------------------------------------------
@@ -2064,6 +2080,7 @@ void launch_scale_mask(int64_t n, int nx, const double* x, double sign, double* 
                               hipStream_t);                                                                         \
   template void launch_dense<T>(int64_t, const T*, const T*, T*, const CgScal*, hipStream_t);                      \
   template void launch_csr_up<T>(const DevCsr&, const DevCsr&, const T*, const T*, T*, const CgScal*, hipStream_t); \
+  template void launch_ku_rows<T>(const Geom&, const T*, const T*, T*, int, hipStream_t);                         \
   template void launch_cheb_pair<T>(const Geom&, const ChebStep<T>&, const ChebStep<T>&, const T*, const T*,       \
                                     const double*, int, CgScal*, double*, hipStream_t);                           \
   template void launch_cheb<T>(const Geom&, const ChebStep<T>&, const T*, const T*, const double*, int, CgScal*,   \

@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(HERE, "libeqcycle_hip.so")
 EQ_OK, EQ_EINVAL, EQ_HIP_ERR, EQ_NR_NOCONV, EQ_NR_NAN, EQ_PCG_NOCONV, EQ_ENOTREADY, EQ_NOMEM = range(8)
 STATUS_NAMES = {0: "EQ_OK", 1: "EQ_EINVAL", 2: "EQ_HIP_ERR", 3: "EQ_NR_NOCONV", 4: "EQ_NR_NAN",
                 5: "EQ_PCG_NOCONV", 6: "EQ_ENOTREADY", 7: "EQ_NOMEM"}
-EQ_MASK_ALL, EQ_MASK_INTERIOR, EQ_MASK_NEG = 0, 1, 2
+EQ_MASK_ALL, EQ_MASK_INTERIOR, EQ_MASK_NEG, EQ_MASK_FAULT_ROWS = 0, 1, 2, 3
 EQ_PREC_FP64, EQ_PREC_FP32, EQ_PREC_MIXED, EQ_PREC_F32PC = 0, 1, 2, 3
 EQ_PC_NONE, EQ_PC_JACOBI, EQ_PC_AMG = 0, 1, 2
 EQ_SMOOTH_JACOBI, EQ_SMOOTH_CHEBYSHEV = 0, 1

@@ -34,7 +34,7 @@ enum {
   EQ_NOMEM = 7
 };
 
-enum { EQ_MASK_ALL = 0, EQ_MASK_INTERIOR = 1, EQ_MASK_NEG = 2 };
+enum { EQ_MASK_ALL = 0, EQ_MASK_INTERIOR = 1, EQ_MASK_NEG = 2, EQ_MASK_FAULT_ROWS = 3 };
 enum { EQ_PREC_FP64 = 0, EQ_PREC_FP32 = 1, EQ_PREC_MIXED = 2, EQ_PREC_F32PC = 3 };
 enum { EQ_PC_NONE = 0, EQ_PC_JACOBI = 1, EQ_PC_AMG = 2 };
 enum { EQ_SMOOTH_JACOBI = 0, EQ_SMOOTH_CHEBYSHEV = 1 };
@@ -88,7 +88,10 @@ int eq_set_params(eq_ctx *ctx, const eq_params_desc *p);
 /* y = K u with K = stiffness_assembly (src/Kassemble.jl:3-63), matrix free;
  * replaces Ksparse'*u (src/main.jl:246,267).  mask: EQ_MASK_ALL (K u),
  * EQ_MASK_NEG (-K u = nKsparse'*u, src/main.jl:311), EQ_MASK_INTERIOR (fault
- * entries of u read as 0 and of y set to 0).  prec: EQ_PREC_FP64 / _FP32. */
+ * entries of u read as 0 and of y set to 0), EQ_MASK_FAULT_ROWS (K u on the
+ * nodes of the element row along the fault only, 0 elsewhere: all that
+ * (K F)[FltNI] with F on the fault, src/main.jl:246, and a[iFlt] of K d,
+ * :267-272, read).  prec: EQ_PREC_FP64 / _FP32. */
 int eq_ku_apply(eq_ctx *ctx, const double *u, double *y, int mask, int prec);
 /* y = kni * x, kni = -K[FltNI,FltNI] (src/main.jl:195,211); x, y in FltNI order. */
 int eq_kni_apply(eq_ctx *ctx, const double *x, double *y, int prec);

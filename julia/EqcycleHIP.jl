@@ -27,7 +27,7 @@ const EQ_OK = 0
 const EQ_NR_NOCONV = 3
 const EQ_NR_NAN = 4
 const EQ_PCG_NOCONV = 5
-const EQ_MASK_ALL, EQ_MASK_INTERIOR, EQ_MASK_NEG = 0, 1, 2
+const EQ_MASK_ALL, EQ_MASK_INTERIOR, EQ_MASK_NEG, EQ_MASK_FAULT_ROWS = 0, 1, 2, 3
 const EQ_PREC_FP64, EQ_PREC_FP32, EQ_PREC_MIXED, EQ_PREC_F32PC = 0, 1, 2, 3
 const EQ_PC_NONE, EQ_PC_JACOBI, EQ_PC_AMG = 0, 1, 2
 const EQ_SMOOTH_JACOBI, EQ_SMOOTH_CHEBYSHEV = 0, 1

@@ -46,6 +46,24 @@ def test_ku_fp64(mesh):
     assert rel(ctx.ku(u, mask="interior"), yi) < 1e-13
 
 
+def test_ku_fault_rows(mesh):
+    """mask EQ_MASK_FAULT_ROWS: K u on the element row along the fault (all
+    that (K F)[FltNI] and a[iFlt] read, src/main.jl:246,267-272), 0 elsewhere."""
+    P, ctx, K, _ = mesh
+    u = normal(9, P.nglob)
+    y = K.matvec(u)
+    rows = P.y <= P.dye * (1 + 1e-9)
+    assert rows.sum() == 5 * P.FltNglob
+    for prec, tol in (("fp64", 1e-13), ("fp32", 1e-5)):
+        yf = ctx.ku(u, mask="fault_rows", prec=prec)
+        assert np.all(yf[~rows] == 0.0)
+        assert np.linalg.norm(yf[rows] - y[rows]) / np.linalg.norm(y[rows]) < tol
+    F = np.zeros(P.nglob)
+    F[P.iFlt - 1] = normal(11, P.FltNglob)
+    rhs = ctx.ku(F, mask="fault_rows")[P.FltNI - 1]
+    assert rel(rhs, K.matvec(F)[P.FltNI - 1]) < 1e-13
+
+
 def test_ku_constant_nullspace(mesh):
     P, ctx, _, _ = mesh
     y = ctx.ku(np.ones(P.nglob))
