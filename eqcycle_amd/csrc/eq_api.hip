@@ -888,14 +888,19 @@ const float* ones_of<float>(eq_ctx* c) { return c->ones32; }
 // One V-cycle: c = M^-1 r with partial c.r (and the cg! convergence test in
 // its first kernel).
 // Launch a level-0 Chebyshev sweep given as its single steps: consecutive
-// steps go to one cheb_pair launch (unless opts.cheb_single), an odd last step
-// to one OpCheb launch.  The steps rotate r and d through three buffers, so a
+// steps go to one cheb_pair launch (unless opts.cheb_single or a large FP64
+// mesh), an odd last step to one OpCheb launch.  The steps rotate r and d through three buffers, so a
 // pair never writes a buffer it reads beyond its own tile.
 template <typename T>
 static void cheb_launch(eq_ctx* ctx, const ChebStep<T>* st, int k) {
+  // A pair moves one step's HBM bytes but holds 2x the LDS of a step (FP64:
+  // 2 blocks per CU).  It wins while one round of blocks covers the mesh and
+  // in FP32; on large FP64 meshes (config 5: 92 us against 2 x 44 us) single
+  // steps stream better.
+  const bool pairs = !ctx->opts.cheb_single && (sizeof(T) == 4 || tile_count(ctx->g) <= 1024);
   int m = 0;
   while (m < k) {
-    if (!ctx->opts.cheb_single && m + 1 < k && st[m].mode == 0) {
+    if (pairs && m + 1 < k && st[m].mode == 0) {
       launch_cheb_pair<T>(ctx->g, st[m], st[m + 1], mu_of<T>(ctx), dinv_of<T>(ctx), ctx->part_rr, ctx->npart,
                           ctx->sc, ctx->part_rho, ctx->s);
       m += 2;
