@@ -505,6 +505,47 @@ void build_pmg_p0(const Geom& g, HostCsr* P) {
   }
 }
 
+// Geometric semi-coarsening of a structured vertex grid (nx columns, ny
+// unknown rows above the Dirichlet fault row, index row * nx + column) in one
+// direction: every other line is kept (and the last), a dropped line takes the
+// mean of its two neighbours (linear interpolation on a uniform grid); in y
+// the Dirichlet row below the first unknown row counts as a kept line of zeros.
+static void semi_coarsen_p(int nx, int ny, bool in_x, HostCsr* P, int* cnx, int* cny) {
+  std::vector<int> keep;
+  const int nl = in_x ? nx : ny;
+  // kept lines: x columns 0, 2, 4, ... and nx - 1; y rows (1-based above the
+  // fault) 2, 4, ... and ny, i.e. 0-based unknown rows 1, 3, ... and ny - 1
+  std::vector<int> pos(nl, -1);
+  for (int l = 0; l < nl; l++) {
+    const bool kept = in_x ? (l % 2 == 0 || l == nl - 1) : (l % 2 == 1 || l == nl - 1);
+    if (kept) { pos[l] = (int)keep.size(); keep.push_back(l); }
+  }
+  const int nk = (int)keep.size();
+  *cnx = in_x ? nk : nx;
+  *cny = in_x ? ny : nk;
+  P->n = (int64_t)nx * ny;
+  P->m = (int64_t)(*cnx) * (*cny);
+  P->rowptr.assign(P->n + 1, 0);
+  P->col.clear();
+  P->val.clear();
+  for (int yy = 0; yy < ny; yy++)
+    for (int xx = 0; xx < nx; xx++) {
+      const int l = in_x ? xx : yy;
+      auto coarse = [&](int lc) {
+        return in_x ? (int64_t)yy * (*cnx) + pos[lc] : (int64_t)pos[lc] * (*cnx) + xx;
+      };
+      if (pos[l] >= 0) {
+        P->col.push_back((int32_t)coarse(l));
+        P->val.push_back(1.0);
+      } else {
+        // neighbours l - 1 and l + 1; below y row 0 lies the Dirichlet row
+        if (l - 1 >= 0) { P->col.push_back((int32_t)coarse(l - 1)); P->val.push_back(0.5); }
+        if (l + 1 < nl) { P->col.push_back((int32_t)coarse(l + 1)); P->val.push_back(0.5); }
+      }
+      P->rowptr[(int64_t)yy * nx + xx + 1] = (int64_t)P->col.size();
+    }
+}
+
 int pmg_build(const Geom& g, const HostCsr& Kii, const AmgOptions& opt, HostHierarchy* H, std::string* err) {
   HostLevel L0;
   L0.A = Kii;
@@ -513,10 +554,6 @@ int pmg_build(const Geom& g, const HostCsr& Kii, const AmgOptions& opt, HostHier
   HostCsr RA, A1;
   csr_matmul(L0.R, Kii, &RA);
   csr_matmul(RA, L0.P, &A1);
-  AmgOptions o = opt;
-  o.max_levels = std::max(1, opt.max_levels - 1);
-  HostHierarchy sub;
-  if (int rc = amg_build(A1, o, &sub, err)) return rc;
   H->lev.clear();
   L0.dinv.assign(Kii.n, 0.0);
   for (int64_t i = 0; i < Kii.n; i++)
@@ -524,6 +561,40 @@ int pmg_build(const Geom& g, const HostCsr& Kii, const AmgOptions& opt, HostHier
       if (Kii.col[q] == i) L0.dinv[i] = 1.0 / Kii.val[q];
   L0.lambda_max = estimate_lambda_max_dinvA(Kii, L0.dinv, 40);
   H->lev.push_back(std::move(L0));
+  // Elongated elements: the Q1 operator is anisotropic (and has positive
+  // off-diagonal entries), which defeats point smoothing with RS coarsening;
+  // semi-coarsen the vertex grid geometrically in the finer direction until
+  // its spacing is within a factor 1.6 of the other, then hand over to RS.
+  int nx = g.nelx + 1, ny = g.nely;
+  double hx = 1.0 / g.c1, hy = 1.0;   // element aspect dx / dy = dye / dxe scaled
+  HostCsr Acur = std::move(A1);
+  while (Acur.n > opt.max_coarse && (int)H->lev.size() < opt.max_levels - 1) {
+    const bool in_x = hx * 1.6 <= hy, in_y = hy * 1.6 <= hx;
+    if (!in_x && !in_y) break;
+    if ((in_x ? nx : ny) < 3) break;
+    HostLevel L;
+    L.A = std::move(Acur);
+    int cnx, cny;
+    semi_coarsen_p(nx, ny, in_x, &L.P, &cnx, &cny);
+    csr_transpose(L.P, &L.R);
+    HostCsr RA2, Ac;
+    csr_matmul(L.R, L.A, &RA2);
+    csr_matmul(RA2, L.P, &Ac);
+    L.dinv.assign(L.A.n, 0.0);
+    for (int64_t i = 0; i < L.A.n; i++)
+      for (int64_t q = L.A.rowptr[i]; q < L.A.rowptr[i + 1]; q++)
+        if (L.A.col[q] == i) L.dinv[i] = 1.0 / L.A.val[q];
+    L.lambda_max = estimate_lambda_max_dinvA(L.A, L.dinv, 30);
+    H->lev.push_back(std::move(L));
+    Acur = std::move(Ac);
+    if (in_x) hx *= 2; else hy *= 2;
+    nx = cnx;
+    ny = cny;
+  }
+  AmgOptions o = opt;
+  o.max_levels = std::max(1, opt.max_levels - (int)H->lev.size());
+  HostHierarchy sub;
+  if (int rc = amg_build(Acur, o, &sub, err)) return rc;
   for (auto& L : sub.lev) H->lev.push_back(std::move(L));
   H->coarse_inv = std::move(sub.coarse_inv);
   H->ncoarse = sub.ncoarse;

@@ -21,6 +21,10 @@
 
 using namespace eq;
 
+#ifndef EQ_CHEB_LOWER
+#define EQ_CHEB_LOWER 10.0
+#endif
+
 namespace {
 
 constexpr int kMaxChebDegree = 8;   // largest level-0 Chebyshev degree eq_amg_setup accepts
@@ -653,8 +657,8 @@ static int upload_hierarchy(eq_ctx* ctx, const HostHierarchy& H, int smoother, i
   ctx->l0_cheb = smoother == EQ_SMOOTH_CHEBYSHEV;
   ctx->l0_degree = ctx->l0_cheb ? smooth_degree : 1;
   {
-    // Chebyshev target interval [lambda_max / 10, 1.1 lambda_max] of D^-1 K_II
-    const double lmax = 1.1 * H.lev[0].lambda_max, lmin = H.lev[0].lambda_max / 10.0;
+    // Chebyshev target interval [lambda_max / EQ_CHEB_LOWER, 1.1 lambda_max] of D^-1 K_II
+    const double lmax = 1.1 * H.lev[0].lambda_max, lmin = H.lev[0].lambda_max / EQ_CHEB_LOWER;
     ctx->l0_theta = 0.5 * (lmax + lmin);
     ctx->l0_delta = 0.5 * (lmax - lmin);
   }

@@ -131,6 +131,24 @@ def test_cg_hierarchies(mesh, hierarchy):
         assert np.linalg.norm(x - xs) / np.linalg.norm(xs) < 1e-6
 
 
+@pytest.mark.parametrize("nelx, nely, bound", [(60, 30, 12), (30, 60, 12), (160, 80, 12), (96, 24, 20)])
+def test_pmg_elongated_elements(nelx, nely, bound):
+    """p-MG on elongated elements (aspect 2 either way, and 4): the Q1 level
+    is semi-coarsened geometrically before Ruge-Stueben, and CG from zero
+    needs about as many iterations as on square elements (6 there; RS alone
+    on the anisotropic Q1 operator needed 27 on 80 x 40 and 151 on 320 x 160).
+    Aspect 4 (Q1 couplings of opposite sign, 1.2 against -2.6) stays harder."""
+    import scipy.sparse.linalg as spla
+    from eqcycle_amd.api import EqContext
+    P = set_parameters(small(nelx, nely))
+    ctx = EqContext(P, amg={"hierarchy": "pmg"})
+    n = P.nglob - P.FltNglob
+    b = normal(5, n)
+    x, it, rr, ok = ctx.cg(np.zeros(n), b, tol=1e-6, pc="amg")
+    print(f"[gpu] pmg {nelx}x{nely}: {it} iterations, levels {[l[0] for l in ctx.amg_info()]}", flush=True)
+    assert ok and it <= bound
+
+
 def test_cg_mixed_precision(mesh):
     import scipy.sparse.linalg as spla
     P, ctx, _, kni = mesh
