@@ -211,6 +211,21 @@ __device__ __forceinline__ T seg_last(const Geom& g, const T* u, const T* m) {
   return r;
 }
 
+// XCD-aware tile order.  Consecutive workgroups go to the 8 XCDs in turn;
+// workgroup b takes tile xcd_tile(b), which gives each XCD a contiguous run of
+// tiles in row-major order, so the halos of neighbouring tiles meet in one L2.
+// A bijection of 0 .. ntiles-1 (EQ_XCD_ORDER=0: the identity).
+#ifndef EQ_XCD_ORDER
+#define EQ_XCD_ORDER 1
+#endif
+__device__ __forceinline__ int xcd_tile(int b, int ntiles) {
+  if (!EQ_XCD_ORDER) return b;
+  constexpr int kXcd = 8;
+  const int base = ntiles / kXcd, rem = ntiles % kXcd;
+  const int x = b % kXcd, k = b / kXcd;
+  return x * base + min(x, rem) + k;
+}
+
 // Outputs i0 .. i0+NPT-1 of the 5-node segment at u, m (stride ST; has_seg:
 // the segment exists), plus, when i0 == 0 and has_prev, the previous
 // segment's output 4 added to the first (the shared node).  Each output is
@@ -353,7 +368,8 @@ __device__ __forceinline__ void tile_body(const Geom& g, const Op& op, int bx, i
 
 template <typename T, class Op>
 __global__ __launch_bounds__(TBLOCK) void tile_kernel(Geom g, Op op) {
-  tile_body<T, Op>(g, op, blockIdx.x, blockIdx.y, gridDim.x, gridDim.y);
+  const int t = xcd_tile(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+  tile_body<T, Op>(g, op, t % gridDim.x, t / gridDim.x, gridDim.x, gridDim.y);
 }
 
 // Epilogue operands with nothing to prefetch.
@@ -805,12 +821,13 @@ struct OpNewmark {
     if (stop && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) vpart_out[1] = -1.0;
     return !stop;
   }
-  // tile blocks sit at blockIdx.y >= 1 (row 0 of the grid runs the fault row)
+  // tile blocks sit at blockIdx.y >= 1 (row 0 of the grid runs the fault row),
+  // in the XCD-aware order of xcd_tile
   __device__ bool prepare(Pre& p) const {
     p.seis = 0;
     if (!go()) return false;
     if (cap_seis) {
-      const int tile = (blockIdx.y - 1) * gridDim.x + blockIdx.x;
+      const int tile = xcd_tile((blockIdx.y - 1) * gridDim.x + blockIdx.x, gridDim.x * (gridDim.y - 1));
       for (int q = 0; q < nseis; q++)
         if (seis_tile[q] == tile) p.seis |= 1 << q;
     }
@@ -938,7 +955,8 @@ __device__ __forceinline__ double newmark_ku_fault(const Geom& g, const OpNewmar
 // to the fault blocks).
 __global__ __launch_bounds__(TBLOCK) void newmark_kernel(Geom g, OpNewmark op) {
   if (blockIdx.y > 0) {
-    tile_body<double, OpNewmark>(g, op, blockIdx.x, blockIdx.y - 1, gridDim.x, gridDim.y - 1);
+    const int t = xcd_tile((blockIdx.y - 1) * gridDim.x + blockIdx.x, gridDim.x * (gridDim.y - 1));
+    tile_body<double, OpNewmark>(g, op, t % gridDim.x, t / gridDim.x, gridDim.x, gridDim.y - 1);
     return;
   }
   if (blockIdx.x * TBLOCK >= g.NX) return;   // block-uniform
@@ -1661,7 +1679,8 @@ __global__ __launch_bounds__(PBLOCK) void cheb_pair_kernel(Geom g, ChebPair<T> o
     }
     if (fin) return;
   }
-  const int I0 = blockIdx.x * TILE_I, J0 = blockIdx.y * TILE_J;
+  const int tile = xcd_tile(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+  const int I0 = (tile % gridDim.x) * TILE_I, J0 = (tile / gridDim.x) * TILE_J;
   const int tid = threadIdx.x;
   const int NX = g.NX, NY = g.NY;
   // owned column strip (both steps): column c3, rows Js3 .. Js3+3
@@ -1842,7 +1861,7 @@ __global__ __launch_bounds__(PBLOCK) void cheb_pair_kernel(Geom g, ChebPair<T> o
     }
   }
   const double s = block_sum(acc0);
-  if (threadIdx.x == 0) op.part0[blockIdx.y * gridDim.x + blockIdx.x] = s;
+  if (threadIdx.x == 0) op.part0[tile] = s;
 }
 
 template <typename T>
