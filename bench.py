@@ -46,8 +46,8 @@ HBM_PEAK_GBS = 8000.0
 # calibrated by a 512 MiB copy, on the 410,881-node mesh
 # (profiles/newmark_cheb_pair_cfg4_hbm_traffic.md).  Keyed by nodes, and
 # precision for the Chebyshev pair.
-NEWMARK_TRAFFIC = {410881: 38.17e6}
-CHEB_PAIR_TRAFFIC = {(410881, "fp64"): 43.27e6, (410881, "fp32"): 25.41e6}
+NEWMARK_TRAFFIC = {410881: 27.63e6}
+CHEB_PAIR_TRAFFIC = {(410881, "fp64"): 28.07e6, (410881, "fp32"): 14.28e6}
 
 
 def parse():
