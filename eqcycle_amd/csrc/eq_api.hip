@@ -89,6 +89,14 @@ struct eq_ctx {
   double *dPre = nullptr, *bq = nullptr;
   FaultState fs{}, fs_api{};
   double* api_nodal = nullptr;        // [6 N] d, v, a in and out of eq_newmark_step
+  // quasi-static pass 2 run ahead of its convergence read-back: its CG batch,
+  // the pass's fault update, the finish and the step tail go out together;
+  // psi, Vf and the fault row of d before the fault update are kept to redo
+  // them if the batch fell short
+  double* spec_save = nullptr;        // [3 NX]
+  bool spec_pending = false;
+  bool spec_f32pc = false;
+  int64_t spec_batch = 0;
   double t = 0, dt = 0;
   int64_t it = 0;
   int isolver = 1;
@@ -1098,37 +1106,63 @@ static int enqueue_iterations(eq_ctx* ctx, T* x, int pc, bool f32pc, int64_t n) 
   return EQ_OK;
 }
 
+// CG set-up: r = b - K_II x (unless r_ready), the device scalars, and the
+// FP32 copy of r for an FP32 V-cycle.
 template <typename T>
-// r_ready: w.r already holds b - A x with its ||r||^2 partials in part_rr and
-// the ||b||^2 partials in part_bb (the learned start leaves them so).
-static int cg_device(eq_ctx* ctx, T* x, const T* b, double tol, int64_t maxiter, int pc, int64_t* iters,
-                     double* relres, bool f32pc = false, bool r_ready = false) {
+static int cg_begin(eq_ctx* ctx, T* x, const T* b, double tol, int64_t maxiter, int pc, bool f32pc, bool r_ready) {
   CgWork<T>& w = work<T>(ctx);
-  const Geom& g = ctx->g;
   hipStream_t s = ctx->s;
   if (pc == EQ_PC_AMG && !ctx->have_amg) return fail(ctx, EQ_ENOTREADY, "eq_amg_setup first");
   if (maxiter <= 0) maxiter = ctx->nI;
   if (maxiter > INT_MAX) maxiter = INT_MAX;
-  if (!r_ready) launch_cg_residual<T>(g, x, mu_of<T>(ctx), b, w.r, ctx->part_rr, ctx->part_bb, ctx->npart, s);
+  if (!r_ready) launch_cg_residual<T>(ctx->g, x, mu_of<T>(ctx), b, w.r, ctx->part_rr, ctx->part_bb, ctx->npart, s);
   launch_cg_init_scal(ctx->sc, ctx->part_rr, ctx->part_bb, ctx->npart, tol, (int)maxiter, s);
   if (f32pc) launch_convert<T, float>(ctx->N, w.r, ctx->w32.r, s);
-  // iterations enqueued before the first read-back: the last count of the
-  // same kind of solve (0 when that one was already converged at the start)
-  int64_t& pred = ctx->pred_iters[ctx->cg_slot];
-  int64_t batch = pred;
-  for (;;) {
-    if (int rc = enqueue_iterations<T>(ctx, x, pc, f32pc, batch)) return rc;
-    launch_cg_check(ctx->sc, ctx->part_rr, ctx->npart, s);
-    HIPCHK(ctx, hipMemcpyAsync(ctx->sc_host, ctx->sc, sizeof(CgScal), hipMemcpyDeviceToHost, s));
-    if (int rc = spin_sync(ctx)) return rc;
-    if (ctx->sc_host->done) break;
+  return EQ_OK;
+}
+
+// A batch of iterations, the convergence test and the copy of the scalars to
+// the host; nothing waits.
+template <typename T>
+static int cg_batch(eq_ctx* ctx, T* x, int pc, bool f32pc, int64_t batch) {
+  if (int rc = enqueue_iterations<T>(ctx, x, pc, f32pc, batch)) return rc;
+  launch_cg_check(ctx->sc, ctx->part_rr, ctx->npart, ctx->s);
+  HIPCHK(ctx, hipMemcpyAsync(ctx->sc_host, ctx->sc, sizeof(CgScal), hipMemcpyDeviceToHost, ctx->s));
+  return EQ_OK;
+}
+
+// After the host has the scalars of a first batch of `batch` iterations:
+// further, smaller batches until the test passes.  The iterates do not depend
+// on where the batches end (a converged solve turns later kernels into exits).
+template <typename T>
+static int cg_rest(eq_ctx* ctx, T* x, int pc, bool f32pc, int64_t batch) {
+  while (!ctx->sc_host->done) {
     batch = std::min<int64_t>(std::max<int64_t>(2, batch / 4), 16);
+    if (int rc = cg_batch<T>(ctx, x, pc, f32pc, batch)) return rc;
+    if (int rc = spin_sync(ctx)) return rc;
   }
+  return EQ_OK;
+}
+
+static int cg_end(eq_ctx* ctx, int64_t* iters, double* relres) {
   const CgScal& h = *ctx->sc_host;
   if (iters) *iters = h.iter;
   if (relres) *relres = h.bnorm > 0 ? h.resid / h.bnorm : h.resid;
-  pred = h.iter;
+  ctx->pred_iters[ctx->cg_slot] = h.iter;
   return (h.resid <= h.reltol) ? EQ_OK : EQ_PCG_NOCONV;
+}
+
+// r_ready: w.r already holds b - A x with its ||r||^2 partials in part_rr and
+// the ||b||^2 partials in part_bb (the learned start leaves them so).
+template <typename T>
+static int cg_device(eq_ctx* ctx, T* x, const T* b, double tol, int64_t maxiter, int pc, int64_t* iters,
+                     double* relres, bool f32pc = false, bool r_ready = false) {
+  if (int rc = cg_begin<T>(ctx, x, b, tol, maxiter, pc, f32pc, r_ready)) return rc;
+  int64_t batch = ctx->pred_iters[ctx->cg_slot];
+  if (int rc = cg_batch<T>(ctx, x, pc, f32pc, batch)) return rc;
+  if (int rc = spin_sync(ctx)) return rc;
+  if (int rc = cg_rest<T>(ctx, x, pc, f32pc, batch)) return rc;
+  return cg_end(ctx, iters, relres);
 }
 
 // Mixed precision: FP64 residual / update around FP32 inner PCG.
@@ -1502,6 +1536,10 @@ static int alloc_state(eq_ctx* ctx) {
   double* blk = ctx->state_blk;
   ctx->d = blk; ctx->v = blk + N; ctx->a = blk + 2 * N; ctx->d2 = blk + 3 * N; ctx->v2 = blk + 4 * N;
   ctx->a2 = blk + 5 * N; ctx->dPre = blk + 6 * N; ctx->bq = blk + 7 * N;
+  if (!ctx->spec_save) {
+    ctx->spec_save = ctx->alloc<double>(3 * ctx->NX);
+    NEED(ctx->spec_save);
+  }
   return EQ_OK;
 }
 
@@ -1651,6 +1689,25 @@ static int qs_step(eq_ctx* ctx, eq_step_info* info) {
     if (ctx->opts.guess == EQ_GUESS_GALERKIN && ctx->learn.gal_ready) galerkin_correction(ctx, ctx->d, ctx->bq);
     int64_t it = 0;
     double rr = 0;
+    const int prec = ctx->opts.prec;
+    if (pass == 1 && !ctx->opts.sync_pass2 && (prec == EQ_PREC_FP64 || prec == EQ_PREC_F32PC) && ctx->spec_save) {
+      // pass 2 ahead of its read-back (eq_step finishes it)
+      const bool f32pc = prec == EQ_PREC_F32PC && ctx->opts.pc == EQ_PC_AMG;
+      ctx->cg_slot = 1;
+      int rc = cg_begin<double>(ctx, ctx->d, ctx->bq, ctx->opts.tol, ctx->opts.maxiter, ctx->opts.pc, f32pc, false);
+      if (!rc) rc = cg_batch<double>(ctx, ctx->d, ctx->opts.pc, f32pc, ctx->pred_iters[1]);
+      ctx->cg_slot = 2;
+      if (rc) return rc;
+      const size_t nb = sizeof(double) * ctx->NX;
+      HIPCHK(ctx, hipMemcpyAsync(ctx->spec_save, ctx->fs.psi, nb, hipMemcpyDeviceToDevice, s));
+      HIPCHK(ctx, hipMemcpyAsync(ctx->spec_save + ctx->NX, ctx->fs.Vf, nb, hipMemcpyDeviceToDevice, s));
+      HIPCHK(ctx, hipMemcpyAsync(ctx->spec_save + 2 * ctx->NX, ctx->d, nb, hipMemcpyDeviceToDevice, s));
+      ctx->spec_pending = true;
+      ctx->spec_f32pc = f32pc;
+      ctx->spec_batch = ctx->pred_iters[1];
+      launch_qs_fault(g, ctx->fp, ctx->fs, ctx->mu, ctx->d, ctx->v, dt, s);
+      continue;
+    }
     ctx->cg_slot = pass;
     int rc = cg_any(ctx, ctx->d, ctx->bq, ctx->opts.tol, ctx->opts.maxiter, ctx->opts.pc, ctx->opts.prec, &it, &rr,
                     learned);
@@ -1690,7 +1747,37 @@ extern "C" int eq_step(eq_ctx* ctx, eq_step_info* info) {
   if (rc) return rc;
   launch_step_tail(ctx->fp, ctx->d, ctx->v, ctx->a, ctx->fs.tau, ctx->dt, ctx->dt0, ctx->isolver, ctx->st,
                    ctx->flags, capture(ctx, 0), ctx->s);
+  const bool spec = ctx->spec_pending;
+  ctx->spec_pending = false;
   if (int rc2 = read_status(ctx, 1)) return rc2;
+  if (spec) {
+    // pass 2's scalars came back with the status
+    if (!ctx->sc_host->done) {
+      // the batch fell short: restore what the fault update read, finish the
+      // solve, and redo the fault update, the finish and the step tail
+      hipStream_t s = ctx->s;
+      const size_t nb = sizeof(double) * ctx->NX;
+      HIPCHK(ctx, hipMemcpyAsync(ctx->fs.psi, ctx->spec_save, nb, hipMemcpyDeviceToDevice, s));
+      HIPCHK(ctx, hipMemcpyAsync(ctx->fs.Vf, ctx->spec_save + ctx->NX, nb, hipMemcpyDeviceToDevice, s));
+      HIPCHK(ctx, hipMemcpyAsync(ctx->d, ctx->spec_save + 2 * ctx->NX, nb, hipMemcpyDeviceToDevice, s));
+      ctx->cg_slot = 1;
+      int rc3 = cg_rest<double>(ctx, ctx->d, ctx->opts.pc, ctx->spec_f32pc, ctx->spec_batch);
+      ctx->cg_slot = 2;
+      if (rc3) return rc3;
+      launch_qs_fault(ctx->g, ctx->fp, ctx->fs, ctx->mu, ctx->d, ctx->v, ctx->dt, s);
+      launch_qs_finish(ctx->g, ctx->fp, ctx->fs, ctx->d, ctx->dPre, ctx->v, ctx->a, ctx->dt, s);
+      // (the first step tail left the flags cleared; nothing in this step sets them)
+      launch_step_tail(ctx->fp, ctx->d, ctx->v, ctx->a, ctx->fs.tau, ctx->dt, ctx->dt0, ctx->isolver, ctx->st,
+                       ctx->flags, capture(ctx, 0), s);
+      if (int rc4 = read_status(ctx, 1)) return rc4;
+    }
+    ctx->cg_slot = 1;
+    int64_t it = 0;
+    const int rc5 = cg_end(ctx, &it, nullptr);
+    ctx->cg_slot = 2;
+    info->pcg_iters[1] = it;
+    if (rc5 == EQ_PCG_NOCONV) ctx->pcg_noconv++;
+  }
   ctx->flags_dirty = false;
   const StepStatus& st = *ctx->st_host;
   info->t = ctx->t;

@@ -186,6 +186,10 @@ typedef struct {
   int cheb_single;         /* 1: one launch per level-0 Chebyshev step; 0 (default): two steps per
                               launch where they follow each other, in FP32 and on FP64 meshes of up
                               to 1024 tiles of 64 x 16 nodes (same results) */
+  int sync_pass2;          /* 1: the host waits for the second quasi-static solve to converge before
+                              the fault update; 0 (default, FP64 and F32PC): the fault update, the
+                              step finish and the status read-back follow the solve's first batch of
+                              iterations and are redone when that batch fell short (same results) */
 } eq_solver_opts;
 int eq_set_solver(eq_ctx *ctx, const eq_solver_opts *opts);
 

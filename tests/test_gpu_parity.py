@@ -203,6 +203,29 @@ def test_cheb_pairs_run_equals_single_steps():
         assert np.array_equal(sa[k], sb[k]), k
 
 
+@pytest.mark.parametrize("prec", ["fp64", "f32pc"])
+def test_early_tail_run_equals_synchronous(prec):
+    """main(P) through the first event with the second quasi-static solve's
+    fault update and status read-back sent ahead of its convergence test (and
+    redone when the first batch fell short) and with the host waiting for
+    convergence first: identical histories, iteration counts and state."""
+    from eqcycle_amd.api import EqContext
+    P = set_parameters(small(32, 8, material="trapezoid", total_years=3))
+    ctx = EqContext(P, amg={"hierarchy": "pmg", "degree": 5})
+    res = {}
+    for early in (True, False):
+        ctx.set_solver(tol=1e-8, pc="amg", prec=prec, early_tail=early)
+        ctx.state_init()
+        R = ctx.run(snapshots=False, events=True)
+        res[early] = (R, ctx.state_get())
+    (Ra, sa), (Rb, sb) = res[True], res[False]
+    assert len(Ra["tStart"]) >= 1
+    for k in ("time_", "Vfmax", "pcg_iters", "tStart", "tEnd", "delfafter", "n_qs", "n_dyn"):
+        assert np.array_equal(Ra[k], Rb[k]), k
+    for k in ("d", "v", "a", "psi", "tau", "Vf"):
+        assert np.array_equal(sa[k], sb[k]), k
+
+
 def test_amg_upload_matches_numpy_vcycle(mesh, oracle):
     """eq_amg_upload with the oracle's Ruge-Stueben hierarchy of kni (what the
     reference's ruge_stuben(kni) hands over, src/main.jl:200): the device
