@@ -961,9 +961,10 @@ static const T* cheb_pre(eq_ctx* ctx, const T* b) {
 }
 
 // Level-0 Chebyshev post-smoothing from x = chx: k steps; output c and
-// partial <c, r>.
+// partial <c, r> (with c64: c widened to FP64 into c64 and partial <c, r64>).
 template <typename T>
-static void cheb_post(eq_ctx* ctx, const T* b, T* c, const T* coarse) {
+static void cheb_post(eq_ctx* ctx, const T* b, T* c, const T* coarse, double* c64 = nullptr,
+                      const double* r64 = nullptr) {
   CgWork<T>& w = work<T>(ctx);
   const int k = ctx->l0_degree;
   const double th = ctx->l0_theta, de = ctx->l0_delta, sg = th / de;
@@ -992,6 +993,8 @@ static void cheb_post(eq_ctx* ctx, const T* b, T* c, const T* coarse) {
       q.mode = 2;
       q.xout = c;
       q.rdot = b;
+      q.xout64 = c64;
+      q.rdot64 = r64;
     } else {
       q.mode = 0;
       q.rout = w.chr[m % 3];
@@ -1002,8 +1005,10 @@ static void cheb_post(eq_ctx* ctx, const T* b, T* c, const T* coarse) {
   cheb_launch<T>(ctx, st, k);
 }
 
+// c64/r64 (matrix-free Chebyshev level 0 only): the last smoothing step
+// writes c widened to FP64 into c64 and the partial <c, r64>.
 template <typename T>
-static void vcycle(eq_ctx* ctx, const T* r, T* c) {
+static void vcycle(eq_ctx* ctx, const T* r, T* c, double* c64 = nullptr, const double* r64 = nullptr) {
   CgWork<T>& w = work<T>(ctx);
   const Geom& g = ctx->g;
   hipStream_t s = ctx->s;
@@ -1038,7 +1043,7 @@ static void vcycle(eq_ctx* ctx, const T* r, T* c) {
     launch_csr<T>(CSR_PROLONG_ADD, L0.P, lvecs<T>(ctx->lev[1]).xp, nullptr, nullptr, T(0), x0 + g.NX, nullptr,
                   ctx->sc, s);
   if (ctx->l0_cheb)
-    cheb_post<T>(ctx, r, c, fused_prolong ? lvecs<T>(ctx->lev[1]).xp : nullptr);
+    cheb_post<T>(ctx, r, c, fused_prolong ? lvecs<T>(ctx->lev[1]).xp : nullptr, c64, r64);
   else
     launch_l0_post<T>(g, w.x0, r, mu_of<T>(ctx), dinv_of<T>(ctx), om0, c, ctx->part_rho, ctx->sc, s);
 }
@@ -1055,7 +1060,11 @@ static void cg_iteration(eq_ctx* ctx, T* x, int pc, bool f32pc = false) {
   hipStream_t s = ctx->s;
   const int nt = tile_count(g);
   int nrho;
-  if (pc == EQ_PC_AMG && f32pc) {
+  if (pc == EQ_PC_AMG && f32pc && ctx->l0_cheb) {
+    // the last smoothing step widens c and takes <c, r> in FP64 per tile
+    vcycle<float>(ctx, ctx->w32.r, ctx->w32.c, (double*)w.c, (const double*)w.r);
+    nrho = nt;
+  } else if (pc == EQ_PC_AMG && f32pc) {
     vcycle<float>(ctx, ctx->w32.r, ctx->w32.c);
     launch_widen_dot(ctx->N, ctx->w32.c, (const double*)w.r, (double*)w.c, ctx->sc, ctx->part_rho, s);
     nrho = vec_blocks(ctx->N);

@@ -130,6 +130,10 @@ struct ChebStep {
   const T* rdot = nullptr;
   int check = 0;
   const T* coarse = nullptr;   // src_mode 2: coarse (vertex) correction added on the fly
+  // mode 2 with an FP32 step: c widened to FP64 goes to xout64 (instead of
+  // xout) and the partial is <c, rdot64> with the FP64 residual
+  double* xout64 = nullptr;
+  const double* rdot64 = nullptr;
 };
 template <typename T>
 void launch_cheb(const Geom& g, const ChebStep<T>& c, const T* mu, const T* dinv, const double* part_rr, int nrr,

@@ -546,6 +546,8 @@ struct OpL0Post {
 //   mode 0: d_new = c1 w + c2 dinv r, x_new = xsrc + d_new; write r, d_new, x_new
 //   mode 1: write r only (residual after pre-smoothing; x = w if write_x_center)
 //   mode 2: x_new as in mode 0 is the smoother output c; partial <c, rdot>
+//           (with xout64/rdot64: c widened to FP64 into xout64, partial
+//           <c, rdot64>)
 // The first kernel of a V-cycle carries the cg! convergence test.
 // Window value of a level-0 Chebyshev step's source at node n = (I, J):
 // src_mode 0: src[n]; 1: s0 D^-1 b (the first iterate, formed on the fly);
@@ -597,6 +599,8 @@ struct OpCheb {
   int mode;
   int write_x_center;
   const T* rdot;
+  double* xout64;
+  const double* rdot64;
   int check;
   const double* part_rr;
   int nrr;
@@ -621,20 +625,21 @@ struct OpCheb {
   __device__ T load(const Pre&, int64_t n, int I, int J) const {
     return cheb_source(src, b, dinv, s0, src_mode, coarse, nelx, nely, gx, n, I, J);
   }
-  struct Ext { T rin, dinv, xin, rdot; };
+  struct Ext { T rin, dinv, xin; double rdot; };
   __device__ void fetch(const Pre&, int64_t n, int, int J, Ext& e) const {
     if (J == 0) return;
     e.rin = rin[n];
     if (mode != 1) {
       e.dinv = dinv[n];
       if (!xsrc_center) e.xin = xin[n];
-      if (mode == 2) e.rdot = rdot[n];
+      if (mode == 2) e.rdot = rdot64 ? rdot64[n] : (double)rdot[n];
     }
   }
   __device__ void epilogue(const Pre&, int64_t n, int, int J, T ku, T wc, const Ext& e, double& a0, double&) const {
     if (J == 0) {
       if (mode == 0) { rout[n] = T(0); dout[n] = T(0); xout[n] = T(0); }
       else if (mode == 1) { rout[n] = T(0); if (write_x_center) xout[n] = T(0); }
+      else if (xout64) xout64[n] = 0.0;
       else xout[n] = T(0);
       return;
     }
@@ -646,12 +651,14 @@ struct OpCheb {
     }
     const T dn = c1 * wc + c2 * e.dinv * r;
     const T xn = (xsrc_center ? wc : e.xin) + dn;
-    xout[n] = xn;
     if (mode == 0) {
+      xout[n] = xn;
       rout[n] = r;
       dout[n] = dn;
     } else {
-      a0 += (double)xn * (double)e.rdot;
+      if (xout64) xout64[n] = (double)xn;
+      else xout[n] = xn;
+      a0 += (double)xn * e.rdot;
     }
   }
 };
@@ -1583,7 +1590,8 @@ template <typename T>
 void launch_cheb(const Geom& g, const ChebStep<T>& c, const T* mu, const T* dinv, const double* part_rr, int nrr,
                  CgScal* sc, double* part_rho, hipStream_t s) {
   OpCheb<T> op{c.src, c.b, mu, dinv, c.s0, c.src_mode, c.rin, c.rout, c.c1, c.c2, c.xin, c.xsrc_center, c.dout,
-               c.xout, c.mode, c.write_x_center, c.rdot, c.check, part_rr, nrr, sc, part_rho, nullptr,
+               c.xout, c.mode, c.write_x_center, c.rdot, c.xout64, c.rdot64, c.check, part_rr, nrr, sc, part_rho,
+               nullptr,
                c.coarse, g.nelx, g.nely, {}};
   for (int p = 0; p < 5; p++) op.gx[p] = (T)g.x[p];
   hipLaunchKernelGGL((tile_kernel<T, OpCheb<T>>), tile_grid(g), dim3(TBLOCK), 0, s, g, op);
@@ -1652,6 +1660,8 @@ struct ChebPair {
   T* dout;
   T* xout;
   const T* rdot;
+  double* xout64;          // mode 2 widened output and FP64 dot operand (see ChebStep)
+  const double* rdot64;
   // both
   const T* mu;
   const T* dinv;
@@ -1697,17 +1707,19 @@ __global__ __launch_bounds__(PBLOCK) void cheb_pair_kernel(Geom g, ChebPair<T> o
     hk = 1 + h % (TILE_J / 4);
   }
   // every global load of the block first
-  T o_r[4], o_di[4], o_x[4], o_rd[4], h_r[4], h_di[4];
+  T o_r[4], o_di[4], o_x[4], h_r[4], h_di[4];
+  double o_rd[4];
 #pragma unroll
   for (int j = 0; j < 4; j++) {
-    o_r[j] = o_di[j] = o_x[j] = o_rd[j] = h_r[j] = h_di[j] = T(0);
+    o_r[j] = o_di[j] = o_x[j] = h_r[j] = h_di[j] = T(0);
+    o_rd[j] = 0.0;
     const int J = Js3 + j;
     if (I3 < NX && J > 0 && J < NY) {
       const int64_t n = (int64_t)J * NX + I3;
       o_r[j] = op.rin[n];
       o_di[j] = op.dinv[n];
       if (!op.xsrc_center) o_x[j] = op.xin[n];
-      if (op.mode == 2) o_rd[j] = op.rdot[n];
+      if (op.mode == 2) o_rd[j] = op.rdot64 ? op.rdot64[n] : (double)op.rdot[n];
     }
   }
   if (hc >= 0) {
@@ -1838,6 +1850,7 @@ __global__ __launch_bounds__(PBLOCK) void cheb_pair_kernel(Geom g, ChebPair<T> o
       if (J == 0) {
         if (op.mode == 0) { op.rout[n] = T(0); op.dout[n] = T(0); op.xout[n] = T(0); }
         else if (op.mode == 1) { op.rout[n] = T(0); op.x1out[n] = T(0); }
+        else if (op.xout64) op.xout64[n] = 0.0;
         else op.xout[n] = T(0);
         continue;
       }
@@ -1851,12 +1864,14 @@ __global__ __launch_bounds__(PBLOCK) void cheb_pair_kernel(Geom g, ChebPair<T> o
       }
       const T dn = op.c1b * wcv + op.c2b * o_di[j] * r;
       const T xn = x1[j] + dn;
-      op.xout[n] = xn;
       if (op.mode == 0) {
+        op.xout[n] = xn;
         op.rout[n] = r;
         op.dout[n] = dn;
       } else {
-        acc0 += (double)xn * (double)o_rd[j];
+        if (op.xout64) op.xout64[n] = (double)xn;
+        else op.xout[n] = xn;
+        acc0 += (double)xn * o_rd[j];
       }
     }
   }
@@ -1889,6 +1904,8 @@ void launch_cheb_pair(const Geom& g, const ChebStep<T>& a, const ChebStep<T>& b,
   op.dout = b.dout;
   op.xout = b.xout;
   op.rdot = b.rdot;
+  op.xout64 = b.xout64;
+  op.rdot64 = b.rdot64;
   op.mu = mu;
   op.dinv = dinv;
   op.check = a.check;
