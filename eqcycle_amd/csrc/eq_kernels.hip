@@ -720,10 +720,14 @@ __device__ int d_nrsearch(double fo, double Vo, double cca, double ccb, double S
   const double fact = 1. + (Vo / Vw) * exp(-psi);
   const double eps = 0.001 * cca * Seff;
   const double help = -(fo + ccb * psi) / cca;
+  // fa = fact tau / (sigma a) as tau times a loop constant: one division
+  // fewer in the iteration's dependency chain (the iterates agree with the
+  // divided form to rounding)
+  const double kfa = fact / (Seff * cca);
   int k = 0;
   double delta = INFINITY, Vf = 0.0;
   while (fabs(delta) > eps) {
-    const double fa = fact * tau / (Seff * cca);
+    const double fa = kfa * tau;
     const double help1 = exp(help + fa), help2 = exp(help - fa);
     Vf = Vo * (help1 - help2);
     const double Vfprime = fact * (Vo / (cca * Seff)) * (help1 + help2);
@@ -736,7 +740,7 @@ __device__ int d_nrsearch(double fo, double Vo, double cca, double ccb, double S
       return -k;
     }
   }
-  const double fa = fact * tau / (Seff * cca);
+  const double fa = kfa * tau;
   Vf_out = Vo * (exp(help + fa) - exp(help - fa));
   tau_out = tau;
   return k;
