@@ -1124,9 +1124,12 @@ static int cg_begin(eq_ctx* ctx, T* x, const T* b, double tol, int64_t maxiter, 
   if (pc == EQ_PC_AMG && !ctx->have_amg) return fail(ctx, EQ_ENOTREADY, "eq_amg_setup first");
   if (maxiter <= 0) maxiter = ctx->nI;
   if (maxiter > INT_MAX) maxiter = INT_MAX;
-  if (!r_ready) launch_cg_residual<T>(ctx->g, x, mu_of<T>(ctx), b, w.r, ctx->part_rr, ctx->part_bb, ctx->npart, s);
+  // (the residual kernel also leaves the FP32 copy of r for an FP32 V-cycle)
+  if (!r_ready)
+    launch_cg_residual<T>(ctx->g, x, mu_of<T>(ctx), b, w.r, ctx->part_rr, ctx->part_bb, ctx->npart, s,
+                          f32pc ? ctx->w32.r : nullptr);
   launch_cg_init_scal(ctx->sc, ctx->part_rr, ctx->part_bb, ctx->npart, tol, (int)maxiter, s);
-  if (f32pc) launch_convert<T, float>(ctx->N, w.r, ctx->w32.r, s);
+  if (f32pc && r_ready) launch_convert<T, float>(ctx->N, w.r, ctx->w32.r, s);
   return EQ_OK;
 }
 
@@ -1679,9 +1682,12 @@ static void learned_correction(eq_ctx* ctx, double* x, const double* b) {
 
 // Galerkin start for K_II x = b on span(Phi): r = b - K_II x, x += Phi c with
 // c = (Phi^T K_II Phi)^-1 Phi^T r, the K_II-norm best correction in span(Phi).
-static void galerkin_correction(eq_ctx* ctx, double* x, const double* b) {
+// have_r: w64.r already holds b - K_II x (up to the rounding of its
+// recursive updates; the solve that follows starts from the true residual).
+static void galerkin_correction(eq_ctx* ctx, double* x, const double* b, bool have_r = false) {
   CgWork<double>& w = ctx->w64;
-  launch_cg_residual<double>(ctx->g, x, ctx->mu, b, w.r, ctx->part_rr, ctx->part_bb, ctx->npart, ctx->s);
+  if (!have_r)
+    launch_cg_residual<double>(ctx->g, x, ctx->mu, b, w.r, ctx->part_rr, ctx->part_bb, ctx->npart, ctx->s);
   learn_project(&ctx->learn, w.r, 1, ctx->s);
   learn_expand_add(&ctx->learn, x, ctx->s);
   ctx->n_learned_solves++;
@@ -1692,10 +1698,15 @@ static int qs_step(eq_ctx* ctx, eq_step_info* info) {
   hipStream_t s = ctx->s;
   const double dt = ctx->dt;
   for (int pass = 0; pass < 2; pass++) {
-    launch_qs_begin(g, ctx->fp, ctx->fs, ctx->mu, ctx->d, ctx->dPre, ctx->v, ctx->bq, dt, pass == 0, s);
+    const bool galerkin = ctx->opts.guess == EQ_GUESS_GALERKIN && ctx->learn.gal_ready;
+    // second pass: the first solve left b1 - K_II d in w64.r (FP64 CG); only
+    // b rows 1..4 change, so the Galerkin start takes b2 - K_II d from it
+    const bool upd_r = pass == 1 && galerkin && (ctx->opts.prec == EQ_PREC_FP64 || ctx->opts.prec == EQ_PREC_F32PC);
+    launch_qs_begin(g, ctx->fp, ctx->fs, ctx->mu, ctx->d, ctx->dPre, ctx->v, ctx->bq, dt, pass == 0, s,
+                    upd_r ? ctx->w64.r : nullptr);
     const bool learned = pass == 0 && ctx->opts.guess == EQ_GUESS_LEARNED && ctx->learn_on;
     if (learned) learned_correction(ctx, ctx->d, ctx->bq);
-    if (ctx->opts.guess == EQ_GUESS_GALERKIN && ctx->learn.gal_ready) galerkin_correction(ctx, ctx->d, ctx->bq);
+    if (galerkin) galerkin_correction(ctx, ctx->d, ctx->bq, upd_r);
     int64_t it = 0;
     double rr = 0;
     const int prec = ctx->opts.prec;

@@ -82,7 +82,7 @@ void launch_element_ku(const Geom& g, const double* u, const double* mu, double*
 // ---- CG pieces (K_II form, vectors of length N with zero fault row) ------
 template <typename T>
 void launch_cg_residual(const Geom& g, const T* x, const T* mu, const T* b, T* r, double* part_rr,
-                        double* part_bb, int npart, hipStream_t s);
+                        double* part_bb, int npart, hipStream_t s, float* r32 = nullptr);
 void launch_cg_init_scal(CgScal* sc, const double* part_rr, const double* part_bb, int nparts, double tol,
                          int maxiter, hipStream_t s);
 void launch_cg_check(CgScal* sc, const double* part_rr, int nparts, hipStream_t s);
@@ -189,7 +189,8 @@ void launch_newmark(const Geom& g, const FaultDev& fp, FaultState fs, const doub
                     double dt, double half_dt, double half_dt_sq, int* flags, const NewmarkRecord& rec,
                     hipStream_t s);
 void launch_qs_begin(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, double* d, double* dPre,
-                     const double* v, double* b, double dt, int first_pass, hipStream_t s);
+                     const double* v, double* b, double dt, int first_pass, hipStream_t s,
+                     double* r_upd = nullptr);
 void launch_qs_fault(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, const double* d,
                      double* v, double dt, hipStream_t s);
 void launch_qs_finish(const Geom& g, const FaultDev& fp, FaultState fs, double* d, const double* dPre, double* v,

@@ -419,6 +419,7 @@ struct OpResidual {
   T* r;
   double* part0;
   double* part1;
+  float* r32;        // optional FP32 copy of r (for an FP32 V-cycle)
   __device__ bool prepare(Pre&) const { return true; }
   struct Ext { T b; };
   __device__ void fetch(const Pre&, int64_t n, int, int J, Ext& e) const { e.b = J == 0 ? T(0) : b[n]; }
@@ -427,6 +428,7 @@ struct OpResidual {
     const T bv = e.b;
     const T rv = J == 0 ? T(0) : bv - ku;
     r[n] = rv;
+    if (r32) r32[n] = (float)rv;
     a0 += (double)rv * (double)rv;
     a1 += (double)bv * (double)bv;
   }
@@ -1182,10 +1184,11 @@ __global__ __launch_bounds__(BLOCK) void dense_kernel(int64_t n, const T* inv, c
 // --------------------------------------------------------------------------
 // pass 1: dPre = d everywhere; fault: Vf0 = Vf = 2v + Vpl.  Both passes:
 // F = dPre + v dt on the fault, d[fault] = F, b rows 1..4 = -(K F) (only the
-// bottom element row couples to the fault).
+// bottom element row couples to the fault).  r_upd (optional) holds
+// b_old - K_II x and is moved to b_new - K_II x (x unchanged).
 __global__ __launch_bounds__(BLOCK) void qs_begin_kernel(Geom g, FaultDev fp, FaultState fs, const double* mu,
                                                          double* d, double* dPre, const double* v, double* b,
-                                                         double dt, int first_pass) {
+                                                         double dt, int first_pass, double* r_upd) {
   const int64_t N = (int64_t)g.NX * g.NY;
   const int64_t n = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   if (first_pass) {
@@ -1208,7 +1211,10 @@ __global__ __launch_bounds__(BLOCK) void qs_begin_kernel(Geom g, FaultDev fp, Fa
   for (int J = 1; J <= 4; J++) {
     double s = 0.0;
     for (int q = 0; q < 5; q++) s += g.D[q * 5 + J] * (g.w[q] * mu[(int64_t)q * g.NX + I]) * g.D[q * 5 + 0];
-    b[(int64_t)J * g.NX + I] = -(cy * s) * F;
+    const int64_t m = (int64_t)J * g.NX + I;
+    const double bn = -(cy * s) * F;
+    if (r_upd) r_upd[m] += bn - b[m];
+    b[m] = bn;
   }
 }
 
@@ -1539,8 +1545,8 @@ void launch_element_ku(const Geom& g, const double* u, const double* mu, double*
 
 template <typename T>
 void launch_cg_residual(const Geom& g, const T* x, const T* mu, const T* b, T* r, double* part_rr, double* part_bb,
-                        int npart, hipStream_t s) {
-  OpResidual<T> op{npart, x, mu, b, r, part_rr, part_bb};
+                        int npart, hipStream_t s, float* r32) {
+  OpResidual<T> op{npart, x, mu, b, r, part_rr, part_bb, r32};
   hipLaunchKernelGGL((tile_kernel<T, OpResidual<T>>), tile_grid(g), dim3(TBLOCK), 0, s, g, op);
 }
 
@@ -2007,10 +2013,10 @@ void launch_newmark(const Geom& g, const FaultDev& fp, FaultState fs, const doub
 }
 
 void launch_qs_begin(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, double* d, double* dPre,
-                     const double* v, double* b, double dt, int first_pass, hipStream_t s) {
+                     const double* v, double* b, double dt, int first_pass, hipStream_t s, double* r_upd) {
   const int64_t N = first_pass ? (int64_t)g.NX * g.NY : g.NX;
   hipLaunchKernelGGL(qs_begin_kernel, dim3(nblk(N)), dim3(BLOCK), 0, s, g, fp, fs, mu, d, dPre, v, b, dt,
-                     first_pass);
+                     first_pass, r_upd);
 }
 
 void launch_qs_fault(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, const double* d, double* v,
@@ -2105,7 +2111,7 @@ void launch_scale_mask(int64_t n, int nx, const double* x, double sign, double* 
 #define EQ_INST(T)                                                                                                  \
   template void launch_ku<T>(const Geom&, const T*, const T*, T*, int, int, hipStream_t);                          \
   template void launch_cg_residual<T>(const Geom&, const T*, const T*, const T*, T*, double*, double*, int,         \
-                                      hipStream_t);                                                                 \
+                                      hipStream_t, float*);                                                         \
   template void launch_cg_dir<T>(const Geom&, const T*, const T*, const T*, T*, T*, const double*, int,            \
                                  const CgScal*, double*, hipStream_t);                                              \
   template void launch_cg_update<T>(int64_t, T*, T*, const T*, const T*, const double*, int, const double*, int,    \
