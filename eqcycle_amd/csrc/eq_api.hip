@@ -95,6 +95,11 @@ struct eq_ctx {
   // them if the batch fell short
   double* spec_save = nullptr;        // [3 NX]
   bool spec_pending = false;
+  // w64.r holds b - K_II d of the last quasi-static solve, bq its b, and
+  // neither nor d's interior has changed since (any other entry point or a
+  // dynamic step clears it): the next step's Galerkin start can move that
+  // residual by the change of b instead of recomputing it
+  bool qs_r_valid = false;
   bool spec_f32pc = false;
   int64_t spec_batch = 0;
   double t = 0, dt = 0;
@@ -399,6 +404,7 @@ static int alloc_work(eq_ctx* ctx, CgWork<T>& w) {
 
 extern "C" int eq_set_material(eq_ctx* ctx, const double* W) {
   if (!ctx || !W) return EQ_EINVAL;
+  ctx->qs_r_valid = false;
   const int NX = ctx->NX;
   std::vector<double> mu(ctx->N, NAN);
   const double* w = ctx->g.w;
@@ -453,6 +459,7 @@ static FaultState alloc_fault_state(eq_ctx* ctx, int64_t n) {
 
 extern "C" int eq_set_params(eq_ctx* ctx, const eq_params_desc* p) {
   if (!ctx || !p) return EQ_EINVAL;
+  ctx->qs_r_valid = false;
   const int NX = ctx->NX, NY = ctx->NY;
   if (p->FltNglob != NX || p->nglob != ctx->N) return fail(ctx, EQ_EINVAL, "params do not match the mesh");
   const double* need[] = {p->fo, p->Vo, p->xLf, p->M, p->FltB, p->FltZ, p->FltX, p->cca, p->ccb, p->Seff, p->tauo,
@@ -562,6 +569,7 @@ extern "C" int eq_set_params(eq_ctx* ctx, const eq_params_desc* p) {
 // ===========================================================================
 extern "C" int eq_ku_apply(eq_ctx* ctx, const double* u, double* y, int mask, int prec) {
   if (!ctx || !u || !y) return EQ_EINVAL;
+  ctx->qs_r_valid = false;
   if (!ctx->have_material) return fail(ctx, EQ_ENOTREADY, "eq_set_material first");
   if (mask < EQ_MASK_ALL || mask > EQ_MASK_FAULT_ROWS) return fail(ctx, EQ_EINVAL, "bad mask");
   const int interior = mask == EQ_MASK_INTERIOR, neg = mask == EQ_MASK_NEG;
@@ -591,6 +599,7 @@ extern "C" int eq_ku_apply(eq_ctx* ctx, const double* u, double* y, int mask, in
 
 extern "C" int eq_kni_apply(eq_ctx* ctx, const double* x, double* y, int prec) {
   if (!ctx || !x || !y) return EQ_EINVAL;
+  ctx->qs_r_valid = false;
   if (!ctx->have_material) return fail(ctx, EQ_ENOTREADY, "eq_set_material first");
   double* dx = ctx->w64.x;
   double* dy = ctx->w64.q;
@@ -608,6 +617,7 @@ extern "C" int eq_kni_apply(eq_ctx* ctx, const double* x, double* y, int prec) {
 
 extern "C" int eq_element_computation(eq_ctx* ctx, const double* u, double* a_elem) {
   if (!ctx || !u || !a_elem) return EQ_EINVAL;
+  ctx->qs_r_valid = false;
   if (!ctx->have_material) return fail(ctx, EQ_ENOTREADY, "eq_set_material first");
   double* du = ctx->w64.x;
   if (int rc = upload_ref(ctx, u, du)) return rc;
@@ -721,6 +731,7 @@ static int upload_hierarchy(eq_ctx* ctx, const HostHierarchy& H, int smoother, i
 extern "C" int eq_amg_setup(eq_ctx* ctx, double theta, int max_levels, int64_t max_coarse, int strength_abs,
                             int smoother, int smooth_degree, int pcoarsen) {
   if (!ctx) return EQ_EINVAL;
+  ctx->qs_r_valid = false;
   if (!ctx->have_material) return fail(ctx, EQ_ENOTREADY, "eq_set_material first");
   if (smoother != EQ_SMOOTH_JACOBI && smoother != EQ_SMOOTH_CHEBYSHEV) return fail(ctx, EQ_EINVAL, "bad smoother");
   if (smoother == EQ_SMOOTH_CHEBYSHEV && (smooth_degree < 1 || smooth_degree > 8))
@@ -747,6 +758,7 @@ extern "C" int eq_amg_setup(eq_ctx* ctx, double theta, int max_levels, int64_t m
 extern "C" int eq_amg_upload(eq_ctx* ctx, int nlev, const eq_csr* A, const eq_csr* P, const double* omega,
                              int smoother, int smooth_degree) {
   if (!ctx || !A || !P) return EQ_EINVAL;
+  ctx->qs_r_valid = false;
   if (!ctx->have_material) return fail(ctx, EQ_ENOTREADY, "eq_set_material first");
   if (nlev < 2 || nlev > 32) return fail(ctx, EQ_EINVAL, "nlev must be 2..32");
   if (smoother != EQ_SMOOTH_JACOBI && smoother != EQ_SMOOTH_CHEBYSHEV) return fail(ctx, EQ_EINVAL, "bad smoother");
@@ -1232,6 +1244,7 @@ static int cg_any(eq_ctx* ctx, double* x, const double* b, double tol, int64_t m
 extern "C" int eq_cg(eq_ctx* ctx, double* x, const double* b, double tol, int64_t maxiter, int pc, int prec,
                      int64_t* iters, double* relres) {
   if (!ctx || !x || !b) return EQ_EINVAL;
+  ctx->qs_r_valid = false;
   if (!ctx->have_material) return fail(ctx, EQ_ENOTREADY, "eq_set_material first");
   if (pc < EQ_PC_NONE || pc > EQ_PC_AMG || prec < EQ_PREC_FP64 || prec > EQ_PREC_F32PC)
     return fail(ctx, EQ_EINVAL, "bad pc / prec");
@@ -1250,6 +1263,7 @@ extern "C" int eq_cg(eq_ctx* ctx, double* x, const double* b, double tol, int64_
 
 extern "C" int eq_amg_ldiv(eq_ctx* ctx, const double* b, double* x, int prec) {
   if (!ctx || !b || !x) return EQ_EINVAL;
+  ctx->qs_r_valid = false;
   if (!ctx->have_amg) return fail(ctx, EQ_ENOTREADY, "eq_amg_setup first");
   // x = (V-cycle of kni) b = -(V-cycle of K_II) b
   double* db = ctx->w64.b;
@@ -1273,6 +1287,7 @@ extern "C" int eq_amg_ldiv(eq_ctx* ctx, const double* b, double* x, int prec) {
 
 extern "C" int eq_pcg_jacobi(eq_ctx* ctx, const double* F, double* dnew, double tol, int64_t maxit, int64_t* iters) {
   if (!ctx || !F || !dnew) return EQ_EINVAL;
+  ctx->qs_r_valid = false;
   if (!ctx->have_material) return fail(ctx, EQ_ENOTREADY, "eq_set_material first");
   // Fnew = -(K F)[FltNI]
   double* dF = ctx->w64.x;
@@ -1316,6 +1331,7 @@ struct DevTmp {
 extern "C" int eq_ids(eq_ctx* ctx, int64_t n, const double* xLf, const double* Vo, const double* psi, double dt,
                       const double* Vf, double cnd, int64_t IDstate, double* psi1) {
   if (!ctx || n < 0 || !xLf || !Vo || !psi || !Vf || !psi1) return EQ_EINVAL;
+  ctx->qs_r_valid = false;
   if (IDstate != 1 && IDstate != 2) return fail(ctx, EQ_EINVAL, "IDstate must be 1 or 2");
   DevTmp t;
   double *a = t.get<double>(n), *b = t.get<double>(n), *c = t.get<double>(n), *d = t.get<double>(n),
@@ -1331,6 +1347,7 @@ extern "C" int eq_ids2(eq_ctx* ctx, int64_t n, const double* xLf, const double* 
                        const double* psi1, double dt, const double* Vf, const double* Vf1, int64_t IDstate,
                        double* psi2) {
   if (!ctx || n < 0 || !xLf || !Vo || !psi || !psi1 || !Vf || !Vf1 || !psi2) return EQ_EINVAL;
+  ctx->qs_r_valid = false;
   if (IDstate != 1 && IDstate != 2) return fail(ctx, EQ_EINVAL, "IDstate must be 1 or 2");
   DevTmp t;
   double *a = t.get<double>(n), *b = t.get<double>(n), *c = t.get<double>(n), *d = t.get<double>(n),
@@ -1349,6 +1366,7 @@ extern "C" int eq_nrsearch(eq_ctx* ctx, int64_t n, const double* fo, const doubl
   if (!ctx || n < 0 || !fo || !Vo || !cca || !ccb || !Seff || !tau || !tauo || !psi || !FltZ || !FltVfree ||
       !Vf_out || !tau_out)
     return EQ_EINVAL;
+  ctx->qs_r_valid = false;
   DevTmp t;
   const double* in[10] = {fo, Vo, cca, ccb, Seff, tau, tauo, psi, FltZ, FltVfree};
   double* din[10];
@@ -1389,6 +1407,7 @@ extern "C" int eq_fbc(eq_ctx* ctx, int64_t IDstate, int64_t NFBC, int64_t FltNgl
                       double* tau1, double* psi2, double* Vf2, double* tau2, const double* psi, const double* Vf,
                       const double* FltVfree, double dt, int64_t* fail_node) {
   if (!ctx || !psi1 || !Vf1 || !tau1 || !psi2 || !Vf2 || !tau2 || !psi || !Vf || !FltVfree) return EQ_EINVAL;
+  ctx->qs_r_valid = false;
   if (int rc = check_fault_call(ctx, NFBC, FltNglob)) return rc;
   const int64_t n = FltNglob;
   FaultState& f = ctx->fs_api;
@@ -1435,6 +1454,7 @@ extern "C" int eq_fbc(eq_ctx* ctx, int64_t IDstate, int64_t NFBC, int64_t FltNgl
 extern "C" int eq_newmark_step(eq_ctx* ctx, double* d, double* v, double* a, double* psi, double* tau, double* Vf,
                                double* FltVfree, double dt) {
   if (!ctx || !d || !v || !a || !psi) return EQ_EINVAL;
+  ctx->qs_r_valid = false;
   if (!ctx->have_material || !ctx->have_params) return fail(ctx, EQ_ENOTREADY, "eq_set_material and eq_set_params first");
   if (!(dt > 0)) return fail(ctx, EQ_EINVAL, "dt must be positive");
   const int64_t N = ctx->N, NX = ctx->NX;
@@ -1484,6 +1504,7 @@ extern "C" int eq_newmark_step(eq_ctx* ctx, double* d, double* v, double* a, dou
 extern "C" int eq_slr(eq_ctx* ctx, int64_t NFBC, int64_t FltNglob, const double* psi, double* psi1, const double* Vf,
                       double* Vf1, int64_t IDstate, const double* tau1, double dt) {
   if (!ctx || !psi || !psi1 || !Vf || !Vf1 || !tau1) return EQ_EINVAL;
+  ctx->qs_r_valid = false;
   if (int rc = check_fault_call(ctx, NFBC, FltNglob)) return rc;
   const int64_t n = FltNglob;
   FaultState& f = ctx->fs_api;
@@ -1504,6 +1525,7 @@ extern "C" int eq_slr(eq_ctx* ctx, int64_t NFBC, int64_t FltNglob, const double*
 extern "C" int eq_dtevol(eq_ctx* ctx, double dt, double dtmin, const double* XiLf, int64_t FaultNglob, int64_t NFBC,
                          const double* Vf, int64_t isolver, double* dt_out) {
   if (!ctx || !XiLf || !Vf || !dt_out || FaultNglob < 0 || NFBC < 1) return EQ_EINVAL;
+  ctx->qs_r_valid = false;
   DevTmp t;
   double *x = t.get<double>(FaultNglob), *v = t.get<double>(FaultNglob), *o = t.get<double>(1);
   UP(x, XiLf, FaultNglob);
@@ -1519,6 +1541,7 @@ extern "C" int eq_dtevol(eq_ctx* ctx, double dt, double dtmin, const double* XiL
 // ===========================================================================
 extern "C" int eq_set_solver(eq_ctx* ctx, const eq_solver_opts* o) {
   if (!ctx || !o) return EQ_EINVAL;
+  ctx->qs_r_valid = false;
   if (o->pc < EQ_PC_NONE || o->pc > EQ_PC_AMG) return fail(ctx, EQ_EINVAL, "bad pc");
   if (o->prec < EQ_PREC_FP64 || o->prec > EQ_PREC_F32PC) return fail(ctx, EQ_EINVAL, "bad prec");
   if (o->guess != EQ_GUESS_PREVIOUS && o->guess != EQ_GUESS_LEARNED && o->guess != EQ_GUESS_GALERKIN)
@@ -1566,6 +1589,7 @@ static void reset_book(eq_ctx* ctx) {
 
 extern "C" int eq_state_init(eq_ctx* ctx, double v_init, const double* psi_init) {
   if (!ctx) return EQ_EINVAL;
+  ctx->qs_r_valid = false;
   if (!ctx->have_material || !ctx->have_params) return fail(ctx, EQ_ENOTREADY, "material and params first");
   if (int rc = alloc_state(ctx)) return rc;
   const int64_t N = ctx->N, NX = ctx->NX;
@@ -1623,6 +1647,7 @@ extern "C" int eq_state_get(eq_ctx* ctx, double* d, double* v, double* a, double
 extern "C" int eq_state_set(eq_ctx* ctx, const double* d, const double* v, const double* a, const double* psi,
                             const double* tau, double t, double dt, int64_t isolver) {
   if (!ctx) return EQ_EINVAL;
+  ctx->qs_r_valid = false;
   if (!ctx->have_state) return fail(ctx, EQ_ENOTREADY, "eq_state_init first");
   if (d) if (int rc = upload_ref(ctx, d, ctx->d)) return rc;
   if (v) if (int rc = upload_ref(ctx, v, ctx->v)) return rc;
@@ -1693,15 +1718,18 @@ static void galerkin_correction(eq_ctx* ctx, double* x, const double* b, bool ha
   ctx->n_learned_solves++;
 }
 
-static int qs_step(eq_ctx* ctx, eq_step_info* info) {
+// r_valid: the last solve's residual may be reused (eq_ctx::qs_r_valid).
+static int qs_step(eq_ctx* ctx, eq_step_info* info, bool r_valid) {
   const Geom& g = ctx->g;
   hipStream_t s = ctx->s;
   const double dt = ctx->dt;
   for (int pass = 0; pass < 2; pass++) {
     const bool galerkin = ctx->opts.guess == EQ_GUESS_GALERKIN && ctx->learn.gal_ready;
-    // second pass: the first solve left b1 - K_II d in w64.r (FP64 CG); only
-    // b rows 1..4 change, so the Galerkin start takes b2 - K_II d from it
-    const bool upd_r = pass == 1 && galerkin && (ctx->opts.prec == EQ_PREC_FP64 || ctx->opts.prec == EQ_PREC_F32PC);
+    // the previous solve (this step's first, or the last step's second when
+    // r_valid) left b_old - K_II d in w64.r (FP64 CG); only b rows 1..4
+    // change, so the Galerkin start takes b_new - K_II d from it
+    const bool upd_r = (pass == 1 || r_valid) && galerkin &&
+                       (ctx->opts.prec == EQ_PREC_FP64 || ctx->opts.prec == EQ_PREC_F32PC);
     launch_qs_begin(g, ctx->fp, ctx->fs, ctx->mu, ctx->d, ctx->dPre, ctx->v, ctx->bq, dt, pass == 0, s,
                     upd_r ? ctx->w64.r : nullptr);
     const bool learned = pass == 0 && ctx->opts.guess == EQ_GUESS_LEARNED && ctx->learn_on;
@@ -1763,7 +1791,9 @@ extern "C" int eq_step(eq_ctx* ctx, eq_step_info* info) {
   info->isolver_used = ctx->isolver;
   clean_flags(ctx);
   ctx->flags_dirty = true;   // until the step tail has reset them
-  int rc = (ctx->isolver == 1) ? qs_step(ctx, info) : dyn_step(ctx);
+  const bool r_valid = ctx->qs_r_valid;
+  ctx->qs_r_valid = false;
+  int rc = (ctx->isolver == 1) ? qs_step(ctx, info, r_valid) : dyn_step(ctx);
   if (rc) return rc;
   launch_step_tail(ctx->fp, ctx->d, ctx->v, ctx->a, ctx->fs.tau, ctx->dt, ctx->dt0, ctx->isolver, ctx->st,
                    ctx->flags, capture(ctx, 0), ctx->s);
@@ -1799,6 +1829,7 @@ extern "C" int eq_step(eq_ctx* ctx, eq_step_info* info) {
     if (rc5 == EQ_PCG_NOCONV) ctx->pcg_noconv++;
   }
   ctx->flags_dirty = false;
+  ctx->qs_r_valid = info->isolver_used == 1 && (ctx->opts.prec == EQ_PREC_FP64 || ctx->opts.prec == EQ_PREC_F32PC);
   const StepStatus& st = *ctx->st_host;
   info->t = ctx->t;
   info->it = ctx->it;
@@ -1830,6 +1861,7 @@ extern "C" int eq_step(eq_ctx* ctx, eq_step_info* info) {
 // step tail kernel, whose status (dtevol!, regime) carries the loop on.
 // Returns the number of steps taken in *m (>= 1).
 static int dyn_batch(eq_ctx* ctx, double total_time, int64_t max_more, int* m, StepStatus* tail_expect) {
+  ctx->qs_r_valid = false;
   int k = 0;
   for (double tt = ctx->t; k < eq_ctx::kBatch && k < max_more && tt < total_time; k++) tt += ctx->dt;
   if (k == 0) k = 1;
@@ -2203,6 +2235,7 @@ static void time_body(eq_ctx* ctx, int which) {
 
 extern "C" int eq_time_kernel(eq_ctx* ctx, int which, int prec, int reps, double* avg_ms) {
   if (!ctx || !avg_ms || reps < 1) return EQ_EINVAL;
+  ctx->qs_r_valid = false;
   if (!ctx->have_material) return fail(ctx, EQ_ENOTREADY, "eq_set_material first");
   if (which == EQ_TIME_VCYCLE && !ctx->have_amg) return fail(ctx, EQ_ENOTREADY, "eq_amg_setup first");
   if (which == EQ_TIME_NEWMARK && (!ctx->have_state || !ctx->have_params))
@@ -2302,6 +2335,7 @@ extern "C" int eq_time_kernel(eq_ctx* ctx, int which, int prec, int reps, double
 
 extern "C" int eq_learn_init(eq_ctx* ctx, int64_t capacity, int channels, int hidden, int nbasis, uint64_t seed) {
   if (!ctx || capacity < 1 || channels < 1 || hidden < 1) return EQ_EINVAL;
+  ctx->qs_r_valid = false;
   if (!ctx->have_params || !ctx->have_material) return fail(ctx, EQ_ENOTREADY, "material and params first");
   std::string err;
   int rc = learn_init(&ctx->learn, ctx->nI, ctx->NX, ctx->NX, (int)capacity, channels, hidden, nbasis, seed, &err);
@@ -2318,6 +2352,7 @@ extern "C" int eq_learn_init(eq_ctx* ctx, int64_t capacity, int channels, int hi
 
 extern "C" int eq_learn_record(eq_ctx* ctx, int on) {
   if (!ctx) return EQ_EINVAL;
+  ctx->qs_r_valid = false;
   LEARN_READY(ctx);
   ctx->learn_record_on = on ? 1 : 0;
   return EQ_OK;
@@ -2334,6 +2369,7 @@ extern "C" int eq_learn_info(const eq_ctx* ctx, int64_t* nsnap, int64_t* nparams
 
 extern "C" int eq_learn_gram(eq_ctx* ctx, double* G) {
   if (!ctx || !G) return EQ_EINVAL;
+  ctx->qs_r_valid = false;
   LEARN_READY(ctx);
   std::string err;
   if (int rc = learn_gram(&ctx->learn, G, ctx->s, &err)) return fail(ctx, rc == 7 ? EQ_NOMEM : EQ_HIP_ERR, err);
@@ -2391,6 +2427,7 @@ static int galerkin_setup(eq_ctx* ctx) {
 
 extern "C" int eq_learn_set_basis(eq_ctx* ctx, const double* V, const double* sigma, const int32_t* use) {
   if (!ctx || !V || !sigma) return EQ_EINVAL;
+  ctx->qs_r_valid = false;
   LEARN_READY(ctx);
   std::string err;
   if (int rc = learn_set_basis(&ctx->learn, V, sigma, use, ctx->s, &err))
@@ -2400,6 +2437,7 @@ extern "C" int eq_learn_set_basis(eq_ctx* ctx, const double* V, const double* si
 
 extern "C" int eq_learn_galerkin(eq_ctx* ctx, const double* x, const double* b, double* x_out) {
   if (!ctx || !x || !b || !x_out) return EQ_EINVAL;
+  ctx->qs_r_valid = false;
   LEARN_READY(ctx);
   if (!ctx->learn.gal_ready) return fail(ctx, EQ_ENOTREADY, "eq_learn_set_basis first");
   double* dx = ctx->gp;
@@ -2419,6 +2457,7 @@ extern "C" int eq_learn_norms(eq_ctx* ctx, double* out) {
 
 extern "C" int eq_learn_train(eq_ctx* ctx, int epochs, int batch, double lr, uint64_t seed, double* hist) {
   if (!ctx || epochs < 0 || batch < 1) return EQ_EINVAL;
+  ctx->qs_r_valid = false;
   LEARN_READY(ctx);
   std::string err;
   if (int rc = learn_train(&ctx->learn, epochs, batch, lr, seed, hist, ctx->s, &err))
@@ -2430,6 +2469,7 @@ extern "C" int eq_learn_train(eq_ctx* ctx, int epochs, int batch, double lr, uin
 extern "C" int eq_learn_loss_grad(eq_ctx* ctx, const float* params, int nsamp, const int32_t* idx, int use_bf16,
                                   double* loss, float* grad) {
   if (!ctx || !idx || !loss) return EQ_EINVAL;
+  ctx->qs_r_valid = false;
   LEARN_READY(ctx);
   std::string err;
   if (int rc = learn_loss_grad(&ctx->learn, params, nsamp, idx, use_bf16, loss, grad, ctx->s, &err))
@@ -2450,6 +2490,7 @@ extern "C" int eq_learn_get(eq_ctx* ctx, float* params, float* Phi, float* X, fl
 
 extern "C" int eq_learn_predict(eq_ctx* ctx, const double* dF, double* coef) {
   if (!ctx || !dF || !coef) return EQ_EINVAL;
+  ctx->qs_r_valid = false;
   LEARN_READY(ctx);
   std::string err;
   if (int rc = learn_predict(&ctx->learn, dF, coef, ctx->s, &err)) return fail(ctx, EQ_HIP_ERR, err);
