@@ -1298,31 +1298,55 @@ __global__ __launch_bounds__(BLOCK) void step_tail_kernel(FaultDev fp, const dou
                                                           const double* a, const double* tau, double dt,
                                                           double dtmin, int isolver, StepStatus* st, int* flags,
                                                           StepCapture cap) {
-  __shared__ double smax[BLOCK];
-  __shared__ int sidx[BLOCK];
-  double m = -INFINITY;
+  // one pass over the fault: the maximum of v (first index on ties) and the
+  // quasi-static dtevol! candidate min XiLf/|Vf| over the cells that qualify,
+  // reduced across each wave by shuffles and across the four waves in LDS
+  // (both reductions are order-independent)
+  const double dtmax = 50 * 24 * 60 * 60;
+  double m = -INFINITY, dtnx = dtmax;
   int mi = INT_MAX;
   for (int j = threadIdx.x; j < fp.nflt; j += BLOCK) {
     const double vj = v[j];
     if (vj > m) { m = vj; mi = j; }
-  }
-  smax[threadIdx.x] = m;
-  sidx[threadIdx.x] = mi;
-  __syncthreads();
-  for (int s = BLOCK / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) {
-      const double o = smax[threadIdx.x + s];
-      const int oi = sidx[threadIdx.x + s];
-      if (o > smax[threadIdx.x] || (o == smax[threadIdx.x] && oi < sidx[threadIdx.x])) {
-        smax[threadIdx.x] = o;
-        sidx[threadIdx.x] = oi;
+    if (j >= fp.j0) {
+      const double Vc = 2 * vj + fp.Vpl;
+      if (fabs(Vc) * dtmax > fp.XiLf[j]) {
+        const double dtcell = fp.XiLf[j] / fabs(Vc);
+        if (dtcell < dtnx) dtnx = dtcell;
       }
     }
-    __syncthreads();
   }
-  const double Vfmax = 2 * smax[0] + fp.Vpl;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double om = __shfl_down(m, off, 64);
+    const int oi = __shfl_down(mi, off, 64);
+    if (om > m || (om == m && oi < mi)) { m = om; mi = oi; }
+    dtnx = fmin(dtnx, __shfl_down(dtnx, off, 64));
+  }
+  __shared__ double sm[BLOCK / 64], sdt[BLOCK / 64];
+  __shared__ int si[BLOCK / 64];
+  if ((threadIdx.x & 63) == 0) {
+    sm[threadIdx.x >> 6] = m;
+    si[threadIdx.x >> 6] = mi;
+    sdt[threadIdx.x >> 6] = dtnx;
+  }
+  __syncthreads();
+  m = sm[0];
+  mi = si[0];
+  dtnx = sdt[0];
+  for (int w = 1; w < BLOCK / 64; w++) {
+    if (sm[w] > m || (sm[w] == m && si[w] < mi)) { m = sm[w]; mi = si[w]; }
+    dtnx = fmin(dtnx, sdt[w]);
+  }
+  const double Vfmax = 2 * m + fp.Vpl;
   const int next = ((isolver == 1 && Vfmax < 5e-3) || (isolver == 2 && Vfmax < 2e-3)) ? 1 : 2;
-  const double dtn = dtevol_dev(fp.XiLf, v, true, fp.Vpl, fp.j0, fp.nflt, dt, dtmin, next);
+  double dtn = dtmin;
+  if (next == 1) {
+    // dtevol! (src/dtevol.jl:5-44)
+    if (dtmin > dtnx) dtnx = dtmin;
+    if (dtnx > 1.2 * dt) dtnx = 1.2 * dt;
+    dtn = dtnx;
+  }
   if (cap.fault) {
     for (int j = threadIdx.x; j < fp.nflt; j += BLOCK) {
       cap.fault[j] = d[j];
@@ -1343,7 +1367,7 @@ __global__ __launch_bounds__(BLOCK) void step_tail_kernel(FaultDev fp, const dou
     st->Vfmax = Vfmax;
     st->dt_next = dtn;
     st->isolver_next = next;
-    st->hypo = sidx[0];
+    st->hypo = mi;
     st->nr_flag = nr_flag;
     st->nr_noconv = flags[1];
     flags[0] = INT_MAX;
