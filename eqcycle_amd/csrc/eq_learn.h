@@ -61,7 +61,6 @@ struct LearnState {
   double* gal_c = nullptr;      // [NB] projections / coefficients
   double* gal_Ginv = nullptr;   // [NB x NB] row-major inverse of Phi^T K_II Phi
   float* Phi_rm = nullptr;      // [nI x NB] row-major copy of Phi (projection)
-  unsigned* gal_arrivals = nullptr;  // blocks of the projection sum done (0 between launches)
   int gal_nblk = 0;
   int gal_ready = 0;
 };

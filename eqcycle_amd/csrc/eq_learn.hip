@@ -383,36 +383,28 @@ __global__ __launch_bounds__(LB) void proj_dot_kernel(int64_t nI, int64_t off, i
         (sw[0][threadIdx.x] + sw[1][threadIdx.x]) + (sw[2][threadIdx.x] + sw[3][threadIdx.x]);
 }
 
-// s[k] = sum_b part[b, k], one block per column, fixed order.  The block
-// that finishes last (by the arrival counter, which it resets for the next
-// launch) forms c = Ginv s (solve) or c = s: four lanes per row of Ginv,
-// sixteen products each, then a fixed-order sum of the four.
-__global__ __launch_bounds__(LB) void proj_sum_solve_kernel(int nblk, int NB, const double* part, double* s,
-                                                            const double* Ginv, int solve, double* c,
-                                                            unsigned* arrivals) {
-  static_assert(LB == 4 * NBMAX, "four lanes per coefficient");
-  __shared__ int last;
-  __shared__ double sv[NBMAX];
-  __shared__ double sp[LB];
+// s[k] = sum_b part[b, k], one block per column, fixed order
+__global__ __launch_bounds__(LB) void proj_sum_kernel(int nblk, int NB, const double* part, double* s) {
   const int k = blockIdx.x;
   double a = 0.0;
   for (int b = threadIdx.x; b < nblk; b += LB) a += part[(int64_t)b * NB + k];
   a = lblock_sum(a);
-  if (threadIdx.x == 0) {
-    s[k] = a;
-    __threadfence();   // s[k] visible device-wide before the arrival is counted
-    last = atomicAdd(arrivals, 1u) == (unsigned)(NB - 1);
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();     // every block's s[k] is visible past this point
-  if (threadIdx.x < NBMAX) sv[threadIdx.x] = threadIdx.x < NB ? s[threadIdx.x] : 0.0;
-  if (threadIdx.x == 0) *arrivals = 0u;
+  if (threadIdx.x == 0) s[k] = a;
+}
+
+// c = Ginv s (solve) or c = s: four lanes per row of Ginv, each with its
+// sixteen loads issued before the products, then a fixed-order sum of the four
+__global__ __launch_bounds__(LB) void proj_solve_kernel(int NB, const double* s, const double* Ginv, int solve,
+                                                        double* c) {
+  static_assert(LB == 4 * NBMAX, "four lanes per coefficient");
+  __shared__ double sv[NBMAX];
+  __shared__ double sp[LB];
   const int t = threadIdx.x >> 2, q = threadIdx.x & 3;
   const int j0 = q * (NBMAX / 4);
   double gr[NBMAX / 4];
 #pragma unroll
   for (int u = 0; u < NBMAX / 4; u++) gr[u] = (solve && t < NB && j0 + u < NB) ? Ginv[t * NB + j0 + u] : 0.0;
+  if (threadIdx.x < NBMAX) sv[threadIdx.x] = threadIdx.x < NB ? s[threadIdx.x] : 0.0;
   __syncthreads();
   double p = 0.0;
 #pragma unroll
@@ -514,9 +506,7 @@ int learn_init(LearnState* ls, int64_t nI, int64_t nflt, int64_t offset, int cap
   ls->gal_c = dalloc<double>(2 * NBMAX);   // [0, NBMAX): coefficients, [NBMAX, 2 NBMAX): projections
   ls->Phi_rm = dalloc<float>(nI * (int64_t)NB);
   ls->gal_Ginv = dalloc<double>((int64_t)NB * NB);
-  ls->gal_arrivals = dalloc<unsigned>(1);
-  if (ls->gal_arrivals) LCHK(hipMemset(ls->gal_arrivals, 0, sizeof(unsigned)));
-  if (!ls->gal_part || !ls->gal_c || !ls->gal_Ginv || !ls->gal_arrivals || !ls->Phi_rm || !ls->norms || !ls->S || !ls->F || !ls->Phi || !ls->X || !ls->Y || !ls->w32 || !ls->w16 || !ls->grad || !ls->adam_m ||
+  if (!ls->gal_part || !ls->gal_c || !ls->gal_Ginv || !ls->Phi_rm || !ls->norms || !ls->S || !ls->F || !ls->Phi || !ls->X || !ls->Y || !ls->w32 || !ls->w16 || !ls->grad || !ls->adam_m ||
       !ls->adam_v || !ls->inf_act || !ls->coef) {
     *err = "device allocation failed (learned guess)";
     learn_free(ls);
@@ -550,7 +540,7 @@ int learn_init(LearnState* ls, int64_t nI, int64_t nflt, int64_t offset, int cap
 void learn_free(LearnState* ls) {
   void* ptrs[] = {ls->S, ls->F, ls->Phi, ls->X, ls->Y, ls->w32, ls->w16, ls->grad, ls->adam_m, ls->adam_v,
                   ls->act, ls->dact, ls->out, ls->idx, ls->loss_part, ls->inf_act, ls->coef, ls->norms,
-                  ls->gal_part, ls->gal_c, ls->gal_Ginv, ls->Phi_rm, ls->gal_arrivals};
+                  ls->gal_part, ls->gal_c, ls->gal_Ginv, ls->Phi_rm};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   NetDims d = ls->d;
@@ -875,8 +865,10 @@ int learn_predict(LearnState* ls, const double* dF_host, double* coef_host, hipS
 void learn_project(LearnState* ls, const double* r, int solve, hipStream_t s) {
   hipLaunchKernelGGL(proj_dot_kernel, dim3(ls->gal_nblk), dim3(LB), 0, s, ls->nI, ls->offset, ls->d.NB, ls->Phi_rm,
                      r, ls->gal_part);
-  hipLaunchKernelGGL(proj_sum_solve_kernel, dim3(ls->d.NB), dim3(LB), 0, s, ls->gal_nblk, ls->d.NB, ls->gal_part,
-                     ls->gal_c + NBMAX, ls->gal_Ginv, solve, ls->gal_c, ls->gal_arrivals);
+  hipLaunchKernelGGL(proj_sum_kernel, dim3(ls->d.NB), dim3(LB), 0, s, ls->gal_nblk, ls->d.NB, ls->gal_part,
+                     ls->gal_c + NBMAX);
+  hipLaunchKernelGGL(proj_solve_kernel, dim3(1), dim3(LB), 0, s, ls->d.NB, ls->gal_c + NBMAX, ls->gal_Ginv, solve,
+                     ls->gal_c);
 }
 
 void learn_expand_add(LearnState* ls, double* x, hipStream_t s) {
