@@ -47,7 +47,7 @@ HBM_PEAK_GBS = 8000.0
 # (profiles/newmark_cheb_pair_cfg4_hbm_traffic.md).  Keyed by nodes, and
 # precision for the Chebyshev pair.
 NEWMARK_TRAFFIC = {410881: 27.63e6}
-CHEB_PAIR_TRAFFIC = {(410881, "fp64"): 28.07e6, (410881, "fp32"): 14.28e6}
+CHEB_PAIR_TRAFFIC = {(410881, "fp64"): 28.07e6, (410881, "fp32"): 14.29e6}
 
 
 def parse():
