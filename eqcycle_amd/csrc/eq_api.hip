@@ -1746,10 +1746,7 @@ static int qs_step(eq_ctx* ctx, eq_step_info* info, bool r_valid) {
       if (!rc) rc = cg_batch<double>(ctx, ctx->d, ctx->opts.pc, f32pc, ctx->pred_iters[1]);
       ctx->cg_slot = 2;
       if (rc) return rc;
-      const size_t nb = sizeof(double) * ctx->NX;
-      HIPCHK(ctx, hipMemcpyAsync(ctx->spec_save, ctx->fs.psi, nb, hipMemcpyDeviceToDevice, s));
-      HIPCHK(ctx, hipMemcpyAsync(ctx->spec_save + ctx->NX, ctx->fs.Vf, nb, hipMemcpyDeviceToDevice, s));
-      HIPCHK(ctx, hipMemcpyAsync(ctx->spec_save + 2 * ctx->NX, ctx->d, nb, hipMemcpyDeviceToDevice, s));
+      launch_copy3(ctx->NX, ctx->fs.psi, ctx->fs.Vf, ctx->d, ctx->spec_save, true, s);
       ctx->spec_pending = true;
       ctx->spec_f32pc = f32pc;
       ctx->spec_batch = ctx->pred_iters[1];
@@ -1806,10 +1803,7 @@ extern "C" int eq_step(eq_ctx* ctx, eq_step_info* info) {
       // the batch fell short: restore what the fault update read, finish the
       // solve, and redo the fault update, the finish and the step tail
       hipStream_t s = ctx->s;
-      const size_t nb = sizeof(double) * ctx->NX;
-      HIPCHK(ctx, hipMemcpyAsync(ctx->fs.psi, ctx->spec_save, nb, hipMemcpyDeviceToDevice, s));
-      HIPCHK(ctx, hipMemcpyAsync(ctx->fs.Vf, ctx->spec_save + ctx->NX, nb, hipMemcpyDeviceToDevice, s));
-      HIPCHK(ctx, hipMemcpyAsync(ctx->d, ctx->spec_save + 2 * ctx->NX, nb, hipMemcpyDeviceToDevice, s));
+      launch_copy3(ctx->NX, ctx->fs.psi, ctx->fs.Vf, ctx->d, ctx->spec_save, false, s);
       ctx->cg_slot = 1;
       int rc3 = cg_rest<double>(ctx, ctx->d, ctx->opts.pc, ctx->spec_f32pc, ctx->spec_batch);
       ctx->cg_slot = 2;

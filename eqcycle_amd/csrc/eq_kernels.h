@@ -222,6 +222,8 @@ void launch_fill(int64_t n, T* y, T val, hipStream_t s);
 // y[k * stride] = val for k < n
 template <typename T>
 void launch_fill_strided(int64_t n, T* y, int stride, T val, hipStream_t s);
+// gather: y = [x0[0..n) | x1[0..n) | x2[0..n)]; otherwise the reverse (one launch)
+void launch_copy3(int64_t n, double* x0, double* x1, double* x2, double* y, bool gather, hipStream_t s);
 void launch_axpy(int64_t n, double alpha, const double* x, double* y, hipStream_t s);
 // y = sign * x on N-vectors, zeroing the fault row (first NX entries)
 void launch_scale_mask(int64_t n, int nx, const double* x, double sign, double* y, hipStream_t s);

@@ -1490,6 +1490,15 @@ __global__ void fill_strided_kernel(int64_t n, T* y, int stride, T v) {
   const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   if (i < n) y[i * stride] = v;
 }
+// y[k n + i] = x_k[i] (k = 0, 1, 2), or the reverse
+__global__ void copy3_kernel(int64_t n, double* x0, double* x1, double* x2, double* y, int gather) {
+  const int64_t e = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (e >= 3 * n) return;
+  const int64_t k = e / n, i = e - k * n;
+  double* x = k == 0 ? x0 : (k == 1 ? x1 : x2);
+  if (gather) y[e] = x[i];
+  else x[i] = y[e];
+}
 // x += gamma p, r -= gamma q with gamma = <p, r0> / <p, K p> (0 if not positive
 // definite in floating point); partial ||r||^2.  The energy-norm error of the
 // guess is then never larger than that of x before the correction.
@@ -2109,6 +2118,10 @@ template <typename T>
 void launch_fill_strided(int64_t n, T* y, int stride, T val, hipStream_t s) {
   if (n <= 0) return;
   hipLaunchKernelGGL((fill_strided_kernel<T>), dim3(nblk(n)), dim3(BLOCK), 0, s, n, y, stride, val);
+}
+void launch_copy3(int64_t n, double* x0, double* x1, double* x2, double* y, bool gather, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(copy3_kernel, dim3(nblk(3 * n)), dim3(BLOCK), 0, s, n, x0, x1, x2, y, gather ? 1 : 0);
 }
 void launch_guess_dot(const Geom& g, const double* p, const double* mu, const double* r0, double* q,
                       double* part_pr, double* part_pq, hipStream_t s) {
