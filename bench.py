@@ -201,8 +201,8 @@ def roofline_cheb_pair(ctx, P, prec):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); every GPU box here has one GPU and the "
-              "multi-GPU decomposition is a design note, not a measured path", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (got --gpus {a.gpus}): the library drives one device, and the "
+              "decomposition over several GPUs is a design note, not a measured path", file=sys.stderr)
         sys.exit(2)
     import numpy as np
 
