@@ -1,8 +1,8 @@
 /*
  * eqoracle.c -- CPU FP64 restatement of the eqcycle time-stepping arithmetic.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker that the HIP library is
- * compared against (and the CPU baseline that bench.py times).  It is never
+ * TEST INFRASTRUCTURE ONLY.  This file is the reference that the HIP library's
+ * results are compared against (and the CPU baseline that bench.py times).  It is never
  * linked into, loaded by or called from the product library.
  *
  * Every function names the reference file:line it restates.  Index arrays are

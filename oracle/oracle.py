@@ -1,7 +1,7 @@
 """CPU FP64 oracle for the eqcycle time-stepping hot path.
 
 TEST INFRASTRUCTURE ONLY: imported by ``tests/``, ``__graft_entry__.smoke()``
-and the ``cpu_baseline`` leg of ``bench.py``, and only as the checker.  The
+and the ``cpu_baseline`` leg of ``bench.py``, and only to check results.  The
 product library never imports, links or calls anything here.
 
 What it restates (reference file:line):

@@ -1,4 +1,4 @@
-"""CPU checks of the learned-guess reference network (the checker of the GPU kernels)."""
+"""CPU checks of the learned-guess reference network (the reference the GPU kernels are compared against)."""
 import numpy as np
 
 from oracle.learned_ref import NetRef
