@@ -409,6 +409,36 @@ def test_quasi_static_steps_match_oracle(oracle, hierarchy):
         _compare_state(ctx, sim, 1e-9)
 
 
+@pytest.mark.parametrize("nel", [(1, 1), (2, 3), (1, 6), (6, 1), (17, 5), (65, 3)])
+def test_small_and_ragged_meshes(oracle, nel):
+    """Meshes of one element, one element column or row, and sizes that leave
+    every tile or tile row partial (a 64 x 16 node tile covers 16 x 4
+    elements): K u against the assembled K, CG against a direct solve, and
+    quasi-static steps of the loop against the oracle's."""
+    import scipy.sparse.linalg as spla
+    from eqcycle_amd.api import EqContext
+    P = set_parameters(small(*nel, material="trapezoid", lognormal_sigma=0.3))
+    print(f"[gpu] mesh {nel} nglob={P.nglob}", flush=True)
+    ctx = EqContext(P, amg={"hierarchy": "pmg"})
+    K = oracle.assemble_K(P.NelX, P.NelY, P.nglob, P.iglob, P.W, P.H, P.dxe, P.dye)
+    u = normal(21, P.nglob)
+    assert rel(ctx.ku(u), K.matvec(u)) < 1e-13
+    A = K.submatrix(P.FltNI - 1, -1.0).to_scipy().tocsc()
+    b = normal(22, A.shape[0])
+    xs = spla.spsolve(A, b)
+    for prec in ("fp64", "f32pc"):
+        x, it, rr, ok = ctx.cg(np.zeros_like(b), b, tol=1e-11, pc="amg", prec=prec)
+        assert ok and np.linalg.norm(x - xs) <= 1e-7 * np.linalg.norm(xs)
+    ctx.set_solver(tol=1e-13, pc="amg")
+    ctx.state_init()
+    sim = oracle.Simulation(P, solver="direct")
+    for k in range(8):
+        info = ctx.step()
+        sim.step()
+        assert info.isolver_used == sim.res.isolver[-1]
+        _compare_state(ctx, sim, 1e-9)
+
+
 def test_dynamic_steps_match_oracle(oracle):
     """Step both sides into the first earthquake (about step 256 on this mesh)
     and compare every Newmark step."""
