@@ -31,3 +31,29 @@ def test_create_fails_loudly_without_gpu():
     from eqcycle_amd.params import set_parameters
     with pytest.raises(EqError):
         EqContext(set_parameters(small(2, 2)))
+
+
+def test_null_context_is_rejected():
+    """Every entry point that takes a context returns EQ_EINVAL for a null one
+    before touching a device, as the reference's functions report misuse by
+    `@error` and `return nothing` rather than crashing."""
+    import ctypes as C
+    lib = L.load()
+
+    def dummy(t):
+        if t in (C.c_int, C.c_int32, C.c_int64, C.c_uint32, C.c_uint64):
+            return 0
+        if t in (C.c_double, C.c_float):
+            return 0.0
+        return None          # pointers and structs by pointer
+
+    checked = 0
+    for name in L.EXPORTS:
+        if name in ("eq_create", "eq_destroy", "eq_last_error"):
+            continue
+        f = getattr(lib, name)
+        assert f(*[dummy(t) for t in f.argtypes]) == L.EQ_EINVAL, name
+        checked += 1
+    assert checked == len(L.EXPORTS) - 3
+    assert lib.eq_last_error(None) == b"null context"
+    lib.eq_destroy(None)
