@@ -1333,6 +1333,7 @@ extern "C" int eq_ids(eq_ctx* ctx, int64_t n, const double* xLf, const double* V
   if (!ctx || n < 0 || !xLf || !Vo || !psi || !Vf || !psi1) return EQ_EINVAL;
   ctx->qs_r_valid = false;
   if (IDstate != 1 && IDstate != 2) return fail(ctx, EQ_EINVAL, "IDstate must be 1 or 2");
+  if (n == 0) return EQ_OK;
   DevTmp t;
   double *a = t.get<double>(n), *b = t.get<double>(n), *c = t.get<double>(n), *d = t.get<double>(n),
          *o = t.get<double>(n);
@@ -1349,6 +1350,7 @@ extern "C" int eq_ids2(eq_ctx* ctx, int64_t n, const double* xLf, const double* 
   if (!ctx || n < 0 || !xLf || !Vo || !psi || !psi1 || !Vf || !Vf1 || !psi2) return EQ_EINVAL;
   ctx->qs_r_valid = false;
   if (IDstate != 1 && IDstate != 2) return fail(ctx, EQ_EINVAL, "IDstate must be 1 or 2");
+  if (n == 0) return EQ_OK;
   DevTmp t;
   double *a = t.get<double>(n), *b = t.get<double>(n), *c = t.get<double>(n), *d = t.get<double>(n),
          *e = t.get<double>(n), *f = t.get<double>(n), *o = t.get<double>(n);
@@ -1367,6 +1369,7 @@ extern "C" int eq_nrsearch(eq_ctx* ctx, int64_t n, const double* fo, const doubl
       !Vf_out || !tau_out)
     return EQ_EINVAL;
   ctx->qs_r_valid = false;
+  if (n == 0) return EQ_OK;
   DevTmp t;
   const double* in[10] = {fo, Vo, cca, ccb, Seff, tau, tauo, psi, FltZ, FltVfree};
   double* din[10];

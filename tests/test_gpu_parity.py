@@ -360,6 +360,17 @@ def test_fbc_and_slr(mesh, oracle):
     assert rel(v1g, v1o) < 1e-11
 
 
+def test_zero_length_fault_routines(mesh):
+    """Empty inputs to the element-wise fault routines return empty outputs
+    (no launch, no error), as the reference's loops over empty ranges do."""
+    P, ctx, _, _ = mesh
+    e = np.zeros(0)
+    assert ctx.ids(e, e, e, 1.0, e, 1e-5).shape == (0,)
+    assert ctx.ids2(e, e, e, e, 1.0, e, e).shape == (0,)
+    out = ctx.nrsearch(e, e, e, e, e, e, e, e, e, e)
+    assert all(np.asarray(o).shape == (0,) for o in out)
+
+
 def test_fbc_guard_returns_nothing(mesh):
     """The FBC! NaN guard (src/NRsearch_serial.jl:19-32) maps to None."""
     from eqcycle_amd.api import FBC
