@@ -95,11 +95,12 @@ struct eq_ctx {
   // them if the batch fell short
   double* spec_save = nullptr;        // [3 NX]
   bool spec_pending = false;
-  // w64.r holds b - K_II d of the last quasi-static solve, bq its b, and
-  // neither nor d's interior has changed since (any other entry point or a
-  // dynamic step clears it): the next step's Galerkin start can move that
-  // residual by the change of b instead of recomputing it
+  // d's interior solves K_II d = bq of the last quasi-static solve to its
+  // tolerance, and neither has changed since (any other entry point or a
+  // dynamic step clears it): the next step's Galerkin start may project
+  // just the change of b
   bool qs_r_valid = false;
+  double* gal_db = nullptr;           // [4 NX] change of b on rows 1..4
   bool spec_f32pc = false;
   int64_t spec_batch = 0;
   double t = 0, dt = 0;
@@ -1578,6 +1579,10 @@ static int alloc_state(eq_ctx* ctx) {
     ctx->spec_save = ctx->alloc<double>(3 * ctx->NX);
     NEED(ctx->spec_save);
   }
+  if (!ctx->gal_db) {
+    ctx->gal_db = ctx->alloc<double>(4 * ctx->NX);
+    NEED(ctx->gal_db);
+  }
   return EQ_OK;
 }
 
@@ -1710,13 +1715,19 @@ static void learned_correction(eq_ctx* ctx, double* x, const double* b) {
 
 // Galerkin start for K_II x = b on span(Phi): r = b - K_II x, x += Phi c with
 // c = (Phi^T K_II Phi)^-1 Phi^T r, the K_II-norm best correction in span(Phi).
-// have_r: w64.r already holds b - K_II x (up to the rounding of its
-// recursive updates; the solve that follows starts from the true residual).
-static void galerkin_correction(eq_ctx* ctx, double* x, const double* b, bool have_r = false) {
-  CgWork<double>& w = ctx->w64;
-  if (!have_r)
+// db (when x solved K_II x = b_old to the solver tolerance and b - b_old
+// lives on rows 1..4 only): r = (b - b_old) + (b_old - K_II x), and the
+// second part, at tolerance level, is left to CG, so c comes from the 4 NX
+// rows of Phi that meet b - b_old instead of a pass over the whole basis.
+// The solve that follows starts from the true residual either way.
+static void galerkin_correction(eq_ctx* ctx, double* x, const double* b, const double* db = nullptr) {
+  if (db) {
+    learn_project_rows(&ctx->learn, db, 4 * (int64_t)ctx->NX, ctx->s);
+  } else {
+    CgWork<double>& w = ctx->w64;
     launch_cg_residual<double>(ctx->g, x, ctx->mu, b, w.r, ctx->part_rr, ctx->part_bb, ctx->npart, ctx->s);
-  learn_project(&ctx->learn, w.r, 1, ctx->s);
+    learn_project(&ctx->learn, w.r, 1, ctx->s);
+  }
   learn_expand_add(&ctx->learn, x, ctx->s);
   ctx->n_learned_solves++;
 }
@@ -1729,15 +1740,13 @@ static int qs_step(eq_ctx* ctx, eq_step_info* info, bool r_valid) {
   for (int pass = 0; pass < 2; pass++) {
     const bool galerkin = ctx->opts.guess == EQ_GUESS_GALERKIN && ctx->learn.gal_ready;
     // the previous solve (this step's first, or the last step's second when
-    // r_valid) left b_old - K_II d in w64.r (FP64 CG); only b rows 1..4
-    // change, so the Galerkin start takes b_new - K_II d from it
-    const bool upd_r = (pass == 1 || r_valid) && galerkin &&
-                       (ctx->opts.prec == EQ_PREC_FP64 || ctx->opts.prec == EQ_PREC_F32PC);
+    // r_valid) left d solving K_II d = b_old; only b rows 1..4 change
+    const bool upd = (pass == 1 || r_valid) && galerkin && !ctx->opts.galerkin_full;
     launch_qs_begin(g, ctx->fp, ctx->fs, ctx->mu, ctx->d, ctx->dPre, ctx->v, ctx->bq, dt, pass == 0, s,
-                    upd_r ? ctx->w64.r : nullptr);
+                    upd ? ctx->gal_db : nullptr);
     const bool learned = pass == 0 && ctx->opts.guess == EQ_GUESS_LEARNED && ctx->learn_on;
     if (learned) learned_correction(ctx, ctx->d, ctx->bq);
-    if (galerkin) galerkin_correction(ctx, ctx->d, ctx->bq, upd_r);
+    if (galerkin) galerkin_correction(ctx, ctx->d, ctx->bq, upd ? ctx->gal_db : nullptr);
     int64_t it = 0;
     double rr = 0;
     const int prec = ctx->opts.prec;
@@ -1826,7 +1835,7 @@ extern "C" int eq_step(eq_ctx* ctx, eq_step_info* info) {
     if (rc5 == EQ_PCG_NOCONV) ctx->pcg_noconv++;
   }
   ctx->flags_dirty = false;
-  ctx->qs_r_valid = info->isolver_used == 1 && (ctx->opts.prec == EQ_PREC_FP64 || ctx->opts.prec == EQ_PREC_F32PC);
+  ctx->qs_r_valid = info->isolver_used == 1;
   const StepStatus& st = *ctx->st_host;
   info->t = ctx->t;
   info->it = ctx->it;

@@ -190,7 +190,7 @@ void launch_newmark(const Geom& g, const FaultDev& fp, FaultState fs, const doub
                     hipStream_t s);
 void launch_qs_begin(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, double* d, double* dPre,
                      const double* v, double* b, double dt, int first_pass, hipStream_t s,
-                     double* r_upd = nullptr);
+                     double* db = nullptr);
 void launch_qs_fault(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, const double* d,
                      double* v, double dt, hipStream_t s);
 void launch_qs_finish(const Geom& g, const FaultDev& fp, FaultState fs, double* d, const double* dPre, double* v,

@@ -1184,11 +1184,11 @@ __global__ __launch_bounds__(BLOCK) void dense_kernel(int64_t n, const T* inv, c
 // --------------------------------------------------------------------------
 // pass 1: dPre = d everywhere; fault: Vf0 = Vf = 2v + Vpl.  Both passes:
 // F = dPre + v dt on the fault, d[fault] = F, b rows 1..4 = -(K F) (only the
-// bottom element row couples to the fault).  r_upd (optional) holds
-// b_old - K_II x and is moved to b_new - K_II x (x unchanged).
+// bottom element row couples to the fault).  db (optional, [4 NX]) receives
+// b_new - b_old on rows 1..4 (row J at db[(J - 1) NX + I]).
 __global__ __launch_bounds__(BLOCK) void qs_begin_kernel(Geom g, FaultDev fp, FaultState fs, const double* mu,
                                                          double* d, double* dPre, const double* v, double* b,
-                                                         double dt, int first_pass, double* r_upd) {
+                                                         double dt, int first_pass, double* db) {
   const int64_t N = (int64_t)g.NX * g.NY;
   const int64_t n = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   if (first_pass) {
@@ -1213,7 +1213,7 @@ __global__ __launch_bounds__(BLOCK) void qs_begin_kernel(Geom g, FaultDev fp, Fa
     for (int q = 0; q < 5; q++) s += g.D[q * 5 + J] * (g.w[q] * mu[(int64_t)q * g.NX + I]) * g.D[q * 5 + 0];
     const int64_t m = (int64_t)J * g.NX + I;
     const double bn = -(cy * s) * F;
-    if (r_upd) r_upd[m] += bn - b[m];
+    if (db) db[(int64_t)(J - 1) * g.NX + I] = bn - b[m];
     b[m] = bn;
   }
 }
@@ -2046,10 +2046,10 @@ void launch_newmark(const Geom& g, const FaultDev& fp, FaultState fs, const doub
 }
 
 void launch_qs_begin(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, double* d, double* dPre,
-                     const double* v, double* b, double dt, int first_pass, hipStream_t s, double* r_upd) {
+                     const double* v, double* b, double dt, int first_pass, hipStream_t s, double* db) {
   const int64_t N = first_pass ? (int64_t)g.NX * g.NY : g.NX;
   hipLaunchKernelGGL(qs_begin_kernel, dim3(nblk(N)), dim3(BLOCK), 0, s, g, fp, fs, mu, d, dPre, v, b, dt,
-                     first_pass, r_upd);
+                     first_pass, db);
 }
 
 void launch_qs_fault(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, const double* d, double* v,

@@ -92,6 +92,9 @@ int learn_predict(LearnState* ls, const double* dF_host, double* coef_host, hipS
 // Galerkin start.  learn_project: gal_c = Phi^T r[off:] (solve = 0) or
 // gal_c = Ginv Phi^T r[off:] (solve = 1), FP64 accumulation in a fixed order.
 void learn_project(LearnState* ls, const double* r, int solve, hipStream_t s);
+// gal_c = Ginv Phi[0:nrows, :]^T v: the projection of a vector that is zero
+// beyond the first nrows interior rows (v holds just those rows)
+void learn_project_rows(LearnState* ls, const double* v, int64_t nrows, hipStream_t s);
 // x[off:] += Phi gal_c
 void learn_expand_add(LearnState* ls, double* x, hipStream_t s);
 // out = column k of Phi as an N-vector (out[0:off] = 0)

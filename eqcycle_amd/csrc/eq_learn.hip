@@ -871,6 +871,18 @@ void learn_project(LearnState* ls, const double* r, int solve, hipStream_t s) {
                      ls->gal_c);
 }
 
+void learn_project_rows(LearnState* ls, const double* v, int64_t nrows, hipStream_t s) {
+  nrows = std::min<int64_t>(nrows, ls->nI);
+  const int nblk = (int)((nrows + PROJ_ROWS - 1) / PROJ_ROWS);
+  if (nblk == 0) return;
+  hipLaunchKernelGGL(proj_dot_kernel, dim3(nblk), dim3(LB), 0, s, nrows, (int64_t)0, ls->d.NB, ls->Phi_rm, v,
+                     ls->gal_part);
+  hipLaunchKernelGGL(proj_sum_kernel, dim3(ls->d.NB), dim3(LB), 0, s, nblk, ls->d.NB, ls->gal_part,
+                     ls->gal_c + NBMAX);
+  hipLaunchKernelGGL(proj_solve_kernel, dim3(1), dim3(LB), 0, s, ls->d.NB, ls->gal_c + NBMAX, ls->gal_Ginv, 1,
+                     ls->gal_c);
+}
+
 void learn_expand_add(LearnState* ls, double* x, hipStream_t s) {
   hipLaunchKernelGGL(expand_add_kernel, dim3(gblk(ls->nI)), dim3(LB), 0, s, ls->nI, ls->offset, ls->d.NB, ls->Phi,
                      ls->gal_c, x);

@@ -190,6 +190,10 @@ typedef struct {
                               the fault update; 0 (default, FP64 and F32PC): the fault update, the
                               step finish and the status read-back follow the solve's first batch of
                               iterations and are redone when that batch fell short (same results) */
+  int galerkin_full;       /* 1: every Galerkin start projects the full residual b - K_II x; 0
+                              (default): when the previous solve left x solving the old b and only
+                              b's rows 1..4 changed, just that change is projected (the old
+                              residual, at tolerance level, is left to CG) */
 } eq_solver_opts;
 int eq_set_solver(eq_ctx *ctx, const eq_solver_opts *opts);
 

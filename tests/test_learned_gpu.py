@@ -121,28 +121,25 @@ def test_galerkin_guess_gain(trained):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "f32pc"])
-def test_galerkin_residual_reuse_matches_recomputed(trained, prec):
-    """The Galerkin start of a quasi-static solve takes b - K_II d from the
-    previous solve's residual moved by the change of b.  The same 40 steps
-    with that reuse cut after every step (a set_solver call between steps
-    makes the next step recompute the residual): the same iteration counts
-    and displacements that agree to far below the solver tolerance."""
+def test_galerkin_change_of_b_start_matches_full_projection(trained, prec):
+    """When the previous solve left d solving K_II d = b_old to tolerance and
+    only rows 1..4 of b have changed, the Galerkin start projects just that
+    change of b (4 NX rows of Phi) instead of the full residual.  The same 40
+    steps with the full residual and projection in every solve: iteration
+    counts within 1 % and displacements that agree far below the solver
+    tolerance."""
     P, ctx, _ = trained
     s0 = ctx.state_get()
 
-    def run(cut):
+    def run(shortcut):
         ctx.state_set(d=s0["d"], v=s0["v"], a=s0["a"], psi=s0["psi"], tau=s0["tau"], t=s0["t"], dt=s0["dt"],
                       isolver=s0["isolver"])
-        ctx.set_solver(tol=1e-10, pc="amg", prec=prec, guess="galerkin")
-        its = []
-        for _ in range(40):
-            its.append(ctx.step().pcg_iters)
-            if cut:
-                ctx.set_solver(tol=1e-10, pc="amg", prec=prec, guess="galerkin")
+        ctx.set_solver(tol=1e-10, pc="amg", prec=prec, guess="galerkin", galerkin_shortcut=shortcut)
+        its = [sum(ctx.step().pcg_iters) for _ in range(40)]
         return its, ctx.state_get()["d"]
 
-    it_a, d_a = run(False)
-    it_b, d_b = run(True)
-    print(f"[gpu] {prec}: iterations reuse {np.sum(it_a)} recomputed {np.sum(it_b)}", flush=True)
-    assert it_a == it_b
-    assert np.max(np.abs(d_a - d_b)) <= 1e-12 * np.max(np.abs(d_b))
+    it_a, d_a = run(True)
+    it_b, d_b = run(False)
+    print(f"[gpu] {prec}: iterations change-of-b start {np.sum(it_a)}, full projection {np.sum(it_b)}", flush=True)
+    assert abs(np.sum(it_a) - np.sum(it_b)) <= 0.01 * np.sum(it_b)
+    assert np.max(np.abs(d_a - d_b)) <= 1e-9 * np.max(np.abs(d_b))
