@@ -61,6 +61,8 @@ struct LearnState {
   double* gal_c = nullptr;      // [NB] projections / coefficients
   double* gal_Ginv = nullptr;   // [NB x NB] row-major inverse of Phi^T K_II Phi
   float* Phi_rm = nullptr;      // [nI x NB] row-major copy of Phi (projection)
+  double* gal_T = nullptr;      // [NB x gal_trows] Ginv Phi^T on the first gal_trows interior rows
+  int64_t gal_trows = 0;
   int gal_nblk = 0;
   int gal_ready = 0;
 };

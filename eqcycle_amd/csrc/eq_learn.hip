@@ -417,6 +417,29 @@ __global__ __launch_bounds__(LB) void proj_solve_kernel(int NB, const double* s,
   }
 }
 
+// T[k, i] = sum_j Ginv[k, j] Phi[i, j] over the first nrows interior rows
+// (the operator that maps a vector living on those rows to its Galerkin
+// coefficients)
+__global__ __launch_bounds__(LB) void gal_rows_op_kernel(int64_t nrows, int NB, const double* Ginv,
+                                                         const float* Phi_rm, double* T) {
+  const int64_t e = (int64_t)blockIdx.x * LB + threadIdx.x;
+  if (e >= nrows * NB) return;
+  const int64_t i = e / NB;
+  const int k = (int)(e - i * NB);
+  double a = 0.0;
+  for (int j = 0; j < NB; j++) a += Ginv[k * NB + j] * (double)Phi_rm[i * NB + j];
+  T[(int64_t)k * nrows + i] = a;
+}
+
+// c[k] = sum_i T[k, i] v[i], one block per coefficient, fixed order
+__global__ __launch_bounds__(LB) void gal_coef_kernel(int64_t nrows, const double* T, const double* v, double* c) {
+  const int k = blockIdx.x;
+  double a = 0.0;
+  for (int64_t i = threadIdx.x; i < nrows; i += LB) a += T[(int64_t)k * nrows + i] * v[i];
+  a = lblock_sum(a);
+  if (threadIdx.x == 0) c[k] = a;
+}
+
 // Phi_rm[i, k] = Phi[k, i]
 __global__ void phi_rowmajor_kernel(int64_t nI, int NB, const float* Phi, float* Phi_rm) {
   const int64_t e = (int64_t)blockIdx.x * LB + threadIdx.x;
@@ -540,7 +563,7 @@ int learn_init(LearnState* ls, int64_t nI, int64_t nflt, int64_t offset, int cap
 void learn_free(LearnState* ls) {
   void* ptrs[] = {ls->S, ls->F, ls->Phi, ls->X, ls->Y, ls->w32, ls->w16, ls->grad, ls->adam_m, ls->adam_v,
                   ls->act, ls->dact, ls->out, ls->idx, ls->loss_part, ls->inf_act, ls->coef, ls->norms,
-                  ls->gal_part, ls->gal_c, ls->gal_Ginv, ls->Phi_rm};
+                  ls->gal_part, ls->gal_c, ls->gal_Ginv, ls->Phi_rm, ls->gal_T};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   NetDims d = ls->d;
@@ -873,6 +896,10 @@ void learn_project(LearnState* ls, const double* r, int solve, hipStream_t s) {
 
 void learn_project_rows(LearnState* ls, const double* v, int64_t nrows, hipStream_t s) {
   nrows = std::min<int64_t>(nrows, ls->nI);
+  if (ls->gal_T && nrows == ls->gal_trows) {
+    hipLaunchKernelGGL(gal_coef_kernel, dim3(ls->d.NB), dim3(LB), 0, s, nrows, ls->gal_T, v, ls->gal_c);
+    return;
+  }
   const int nblk = (int)((nrows + PROJ_ROWS - 1) / PROJ_ROWS);
   if (nblk == 0) return;
   hipLaunchKernelGGL(proj_dot_kernel, dim3(nblk), dim3(LB), 0, s, nrows, (int64_t)0, ls->d.NB, ls->Phi_rm, v,
@@ -900,6 +927,15 @@ int learn_get_proj(LearnState* ls, double* host, hipStream_t s) {
 
 int learn_set_ginv(LearnState* ls, const double* Ginv_host, hipStream_t s, std::string* err) {
   LCHK(hipMemcpyAsync(ls->gal_Ginv, Ginv_host, sizeof(double) * ls->d.NB * ls->d.NB, hipMemcpyHostToDevice, s));
+  // Ginv Phi^T restricted to the rows next to the fault (rows 1..4 of the
+  // grid: the first 4 nflt interior rows), where a change of b lives
+  const int64_t nrows = std::min<int64_t>(4 * ls->nflt, ls->nI);
+  if (!ls->gal_T) ls->gal_T = dalloc<double>(nrows * ls->d.NB);
+  if (!ls->gal_T) { *err = "device allocation failed (Galerkin rows operator)"; return 2; }
+  ls->gal_trows = nrows;
+  hipLaunchKernelGGL(gal_rows_op_kernel, dim3((unsigned)((nrows * ls->d.NB + LB - 1) / LB)), dim3(LB), 0, s, nrows,
+                     ls->d.NB, ls->gal_Ginv, ls->Phi_rm, ls->gal_T);
+  LCHK(hipGetLastError());
   LCHK(hipStreamSynchronize(s));
   ls->gal_ready = 1;
   return 0;
