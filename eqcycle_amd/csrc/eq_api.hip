@@ -1478,7 +1478,9 @@ extern "C" int eq_newmark_step(eq_ctx* ctx, double* d, double* v, double* a, dou
   int hf[2] = {INT_MAX, 0};
   ctx->flags_dirty = true;
   if (int rc = h2d(ctx, ctx->flags, hf, 2)) return rc;
-  launch_newmark(ctx->g, ctx->fp, f, ctx->mu, ctx->M, in, in + N, in + 2 * N, out, out + N, out + 2 * N, dt,
+  // the new d is the predictor (into out), the next predictor is scratch (into in)
+  launch_newmark_predict(N, in, in + N, in + 2 * N, dt, 0.5 * ctx->dt0 * ctx->dt0, out, ctx->s);
+  launch_newmark(ctx->g, ctx->fp, f, ctx->mu, ctx->M, out, in + N, in + 2 * N, in, out + N, out + 2 * N, dt,
                  0.5 * ctx->dt0, 0.5 * ctx->dt0 * ctx->dt0, ctx->flags, NewmarkRecord{}, ctx->s);
   if (int rc = d2h(ctx, hf, ctx->flags, 2)) return rc;
   if (int rc = download_ref(ctx, out, d)) return rc;
@@ -1781,7 +1783,9 @@ static int qs_step(eq_ctx* ctx, eq_step_info* info, bool r_valid) {
 
 static int dyn_step(eq_ctx* ctx) {
   const double dtmin = ctx->dt0;
-  launch_newmark(ctx->g, ctx->fp, ctx->fs, ctx->mu, ctx->M, ctx->d, ctx->v, ctx->a, ctx->d2, ctx->v2, ctx->a2,
+  // predictor (the new d) into d2; the next predictor lands in d, which the swap retires
+  launch_newmark_predict(ctx->N, ctx->d, ctx->v, ctx->a, ctx->dt, 0.5 * dtmin * dtmin, ctx->d2, ctx->s);
+  launch_newmark(ctx->g, ctx->fp, ctx->fs, ctx->mu, ctx->M, ctx->d2, ctx->v, ctx->a, ctx->d, ctx->v2, ctx->a2,
                  ctx->dt, 0.5 * dtmin, 0.5 * dtmin * dtmin, ctx->flags, NewmarkRecord{}, ctx->s);
   std::swap(ctx->d, ctx->d2);
   std::swap(ctx->v, ctx->v2);
@@ -1879,6 +1883,10 @@ static int dyn_batch(eq_ctx* ctx, double total_time, int64_t max_more, int* m, S
   launch_fill_strided<int>(eq_ctx::kBatch, ctx->nrflags, 2, INT_MAX, s);
   const double dtmin = ctx->dt0;
   double* bufs[2][3] = {{ctx->d, ctx->v, ctx->a}, {ctx->d2, ctx->v2, ctx->a2}};
+  // step q reads the predictor (its new d) from the other set and v, a from
+  // its own, and writes v, a into the other set and the next predictor into
+  // its own d; after q + 1 steps the state is set (q + 1) & 1, as before
+  launch_newmark_predict(ctx->N, ctx->d, ctx->v, ctx->a, ctx->dt, 0.5 * dtmin * dtmin, ctx->d2, s);
   for (int q = 0; q < k; q++) {
     double** src = bufs[q & 1];
     double** dst = bufs[(q + 1) & 1];
@@ -1891,7 +1899,7 @@ static int dyn_batch(eq_ctx* ctx, double total_time, int64_t max_more, int* m, S
     rec.seis_tile = ctx->seis_tile;
     rec.nseis = (int)ns;
     rec.cap_seis = ns > 0 ? ctx->rec_seis + (int64_t)q * 3 * ns : nullptr;
-    launch_newmark(ctx->g, ctx->fp, ctx->fs, ctx->mu, ctx->M, src[0], src[1], src[2], dst[0], dst[1], dst[2],
+    launch_newmark(ctx->g, ctx->fp, ctx->fs, ctx->mu, ctx->M, dst[0], src[1], src[2], src[0], dst[1], dst[2],
                    ctx->dt, 0.5 * dtmin, 0.5 * dtmin * dtmin, ctx->nrflags + 2 * q, rec, s);
   }
   HIPCHK(ctx, hipMemcpyAsync(ctx->vpart_host, ctx->vpart, sizeof(double) * 2 * nbx * k, hipMemcpyDeviceToHost, s));
@@ -2301,8 +2309,11 @@ extern "C" int eq_time_kernel(eq_ctx* ctx, int which, int prec, int reps, double
   launch_fill<double>(ctx->npart, ctx->part_uq, 1.0, s);
   auto body = [&]() {
     if (which == EQ_TIME_NEWMARK) {
-      launch_newmark(ctx->g, ctx->fp, ctx->fs_api, ctx->mu, ctx->M, ctx->d, ctx->v, ctx->a, ctx->d2, ctx->v2,
-                     ctx->a2, ctx->dt0, 0.5 * ctx->dt0, 0.5 * ctx->dt0 * ctx->dt0, ctx->flags, NewmarkRecord{}, s);
+      // the same step from the same state each time, outputs in scratch
+      double* w = ctx->api_nodal;
+      launch_newmark(ctx->g, ctx->fp, ctx->fs_api, ctx->mu, ctx->M, w, ctx->v, ctx->a, w + ctx->N, w + 2 * ctx->N,
+                     w + 3 * ctx->N, ctx->dt0, 0.5 * ctx->dt0, 0.5 * ctx->dt0 * ctx->dt0, ctx->flags, NewmarkRecord{},
+                     s);
     } else if (prec == EQ_PREC_FP32) {
       time_body<float>(ctx, which);
     } else {
@@ -2314,6 +2325,11 @@ extern "C" int eq_time_kernel(eq_ctx* ctx, int which, int prec, int reps, double
     // the fault routine works on the API scratch copy of the fault state
     HIPCHK(ctx, hipMemcpyAsync(ctx->fs_api.psi, ctx->fs.psi, sizeof(double) * 11 * ctx->NX,
                                hipMemcpyDeviceToDevice, s));
+    if (!ctx->api_nodal) {
+      ctx->api_nodal = ctx->alloc<double>(6 * ctx->N);
+      NEED(ctx->api_nodal);
+    }
+    launch_newmark_predict(ctx->N, ctx->d, ctx->v, ctx->a, ctx->dt0, 0.5 * ctx->dt0 * ctx->dt0, ctx->api_nodal, s);
   }
   body();  // warm-up
   hipEvent_t e0, e1;

@@ -184,10 +184,16 @@ struct NewmarkRecord {
   int nseis = 0;
   double* cap_seis = nullptr;
 };
+// One Newmark step (src/main.jl:297-344) from the predicted displacement P
+// (= d + dt v + dt^2/2 a, the step's new d) and the previous v, a; writes
+// the new v, a into v2, a2 and the next step's predictor into P2.
 void launch_newmark(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, const double* M,
-                    const double* d, const double* v, const double* a, double* d2, double* v2, double* a2,
+                    const double* P, const double* v, const double* a, double* P2, double* v2, double* a2,
                     double dt, double half_dt, double half_dt_sq, int* flags, const NewmarkRecord& rec,
                     hipStream_t s);
+// P = d + dt v + dt^2/2 a on n nodes
+void launch_newmark_predict(int64_t n, const double* d, const double* v, const double* a, double dt,
+                            double half_dt_sq, double* P, hipStream_t s);
 void launch_qs_begin(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, double* d, double* dPre,
                      const double* v, double* b, double dt, int first_pass, hipStream_t s,
                      double* db = nullptr);

@@ -783,6 +783,12 @@ __device__ void d_fbc_node(const FaultDev& fp, const FaultState& fs, int j, doub
 // alike, when that step ended the dynamic regime (Vfmax < 2e-3, the switch
 // rule of the step tail), failed the FBC! guard or did not run; a stopped
 // kernel marks its own record as not run.
+// Newmark predictor d + dt v + dt^2/2 a (src/main.jl:299-300), one
+// expression for every place that forms it, so that all agree bit for bit
+__device__ __forceinline__ double newmark_predict(double d, double v, double a, double dt, double half_dt_sq) {
+  return d + dt * v + half_dt_sq * a;
+}
+
 struct OpNewmark {
   struct Pre { int seis; };
   static constexpr int kParts = 0;
@@ -792,8 +798,11 @@ struct OpNewmark {
   FaultState fs;
   const double* mu;
   const double* M;
-  const double *d, *v, *a;
-  double *d2, *v2, *a2;
+  // P: this step's predicted displacement d + dt v + dt^2/2 a (the step's
+  // new d); v, a: the previous step's.  Out: v2, a2 and P2, the next step's
+  // predictor formed from the new state as the previous step formed P.
+  const double *P, *v, *a;
+  double *P2, *v2, *a2;
   double dt, half_dt, half_dt_sq;
   int* flags;               // FBC! guard / non-convergence counters of this step
   int NY;
@@ -852,7 +861,7 @@ struct OpNewmark {
     e.a = a[n];
     e.M = M[n];
   }
-  __device__ double load(const Pre&, int64_t n, int, int) const { return d[n] + dt * v[n] + half_dt_sq * a[n]; }
+  __device__ double load(const Pre&, int64_t n, int, int) const { return P[n]; }
   // the fault row belongs to the fault blocks
   __device__ void epilogue(const Pre& p, int64_t n, int I, int J, double ku, double dn, const Ext& e, double& a0,
                            double& a1) const {
@@ -896,7 +905,7 @@ struct OpNewmark {
       vn = 0.0;
       an = 0.0;
     }
-    d2[n] = dn;
+    P2[n] = newmark_predict(dn, vn, an, dt, half_dt_sq);
     v2[n] = vn;
     a2[n] = an;
     if (fault) {
@@ -920,8 +929,7 @@ struct OpNewmark {
   __device__ void block_end(double, double) const {}
 };
 
-// K u at fault node (I, 0) of the predictor field u = d + dt v + dt^2/2 a,
-// read from global memory: the row-0 segment(s) along x and the column
+// K u at fault node (I, 0) of the predictor field P, read from global memory: the row-0 segment(s) along x and the column
 // segment of rows 0..4, each value formed as the tile kernel forms it
 // (seg_all / seg_last outputs, x term rounded before the y term is added).
 __device__ __forceinline__ double newmark_ku_fault(const Geom& g, const OpNewmark& op, int I) {
@@ -2034,11 +2042,22 @@ void launch_dense(int64_t n, const T* inv, const T* b, T* x, const CgScal* sc, h
   hipLaunchKernelGGL((dense_kernel<T>), dim3((unsigned)blocks), dim3(BLOCK), 0, s, n, inv, b, x, sc);
 }
 
+__global__ void newmark_predict_kernel(int64_t n, const double* d, const double* v, const double* a, double dt,
+                                       double half_dt_sq, double* P) {
+  const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (i < n) P[i] = newmark_predict(d[i], v[i], a[i], dt, half_dt_sq);
+}
+void launch_newmark_predict(int64_t n, const double* d, const double* v, const double* a, double dt,
+                            double half_dt_sq, double* P, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(newmark_predict_kernel, dim3(nblk(n)), dim3(BLOCK), 0, s, n, d, v, a, dt, half_dt_sq, P);
+}
+
 void launch_newmark(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, const double* M,
-                    const double* d, const double* v, const double* a, double* d2, double* v2, double* a2,
+                    const double* P, const double* v, const double* a, double* P2, double* v2, double* a2,
                     double dt, double half_dt, double half_dt_sq, int* flags, const NewmarkRecord& rec,
                     hipStream_t s) {
-  OpNewmark op{fp, fs, mu, M, d, v, a, d2, v2, a2, dt, half_dt, half_dt_sq, flags, g.NY, nullptr, nullptr,
+  OpNewmark op{fp, fs, mu, M, P, v, a, P2, v2, a2, dt, half_dt, half_dt_sq, flags, g.NY, nullptr, nullptr,
                rec.vprev, rec.nrprev, rec.vpart_out, rec.cap_fault, rec.seis_idx, rec.seis_tile, rec.nseis,
                rec.cap_seis, (int)tile_grid(g).x};
   const dim3 tg = tile_grid(g);
