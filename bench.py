@@ -46,7 +46,7 @@ HBM_PEAK_GBS = 8000.0
 # calibrated by a 512 MiB copy, on the 410,881-node mesh
 # (profiles/newmark_cheb_pair_cfg4_hbm_traffic.md).  Keyed by nodes, and
 # precision for the Chebyshev pair.
-NEWMARK_TRAFFIC = {410881: 27.63e6}
+NEWMARK_TRAFFIC = {410881: 27.19e6}
 CHEB_PAIR_TRAFFIC = {(410881, "fp64"): 28.07e6, (410881, "fp32"): 14.29e6}
 
 
@@ -168,14 +168,15 @@ def train_learned(ctx, cfg, tol, epochs):
 def roofline(ctx, P, prec):
     """The Newmark step, the kernel with the largest share of device time in
     this window (one launch per dynamic step, ~20,000 steps against ~10,000
-    Chebyshev pair launches): per node it reads d, v, a, M, mu and writes d,
-    v, a -- 8 FP64 values (the LDS halo and the fault-row stencil reads are not
-    counted).  Timed with HIP events on the library's stream."""
+    Chebyshev pair launches): per node it reads the predicted displacement
+    (the step's new d), v, a, M, mu and writes v, a and the next step's
+    predictor -- 8 FP64 values (the LDS halo and the fault-row stencil reads
+    are not counted).  Timed with HIP events on the library's stream."""
     dt_ms = ctx.time_kernel("newmark", "fp64", reps=200)
     alg = 8 * 8 * P.nglob
     achieved = alg / (dt_ms * 1e-3) / 1e9
-    return {"kernel": "newmark_kernel (whole Newmark step: predictor, -K d, absorbing edges, FBC! on the fault "
-                      "row in blocks of their own, corrector)",
+    return {"kernel": "newmark_kernel (whole Newmark step from the carried predictor: -K d, absorbing edges, "
+                      "FBC! on the fault row in blocks of their own, corrector, next predictor)",
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": NEWMARK_TRAFFIC.get(P.nglob), "avg_launch_us": dt_ms * 1e3,
             "alg_bytes_per_launch": alg}
