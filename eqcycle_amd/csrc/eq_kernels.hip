@@ -770,9 +770,10 @@ __device__ void d_fbc_node(const FaultDev& fp, const FaultState& fs, int j, doub
   tau2_out = tau2;
 }
 
-// Newmark step (src/main.jl:297-344) as a tile epilogue: predictor in the
-// window load, -K d, absorbing edges, FBC!, corrector.  Reads d, v, a and
-// writes d2, v2, a2 (double-buffered state).
+// Newmark step (src/main.jl:297-344) as a tile epilogue: -K d of the
+// predicted displacement P (loaded as the window), absorbing edges, FBC!,
+// corrector, and the next step's predictor.  Reads P, v, a and writes
+// P2, v2, a2 (double-buffered state; P is the step's new d).
 //
 // In a batch of dynamic steps the kernel also keeps the step record that the
 // step tail would write: the bottom tile row leaves the maximum of the new
