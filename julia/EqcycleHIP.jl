@@ -15,7 +15,7 @@
 # Usage in run.jl, after `P = setParameters(24e3, 20)`:
 #   include("julia/EqcycleHIP.jl"); using .EqcycleHIP
 #   ctx = EqcycleHIP.Context(P)          # mesh, material W, params_*
-#   O = EqcycleHIP.main(ctx, P)          # or call the routines one by one
+#   O = EqcycleHIP.main(ctx, P; v_init)  # or call the routines one by one
 module EqcycleHIP
 
 using LinearAlgebra
@@ -260,6 +260,126 @@ function newmark_step!(ctx::Context, d, v, a, psi, tau, Vf, FltVfree, dt)
     end
     rc == EQ_NR_NOCONV || check(ctx, rc)
     return d, v, a, psi, tau
+end
+
+# ---- device-resident loop: main(P) (src/main.jl:67-476) ---------------------
+const EQ_GUESS_PREVIOUS, EQ_GUESS_LEARNED, EQ_GUESS_GALERKIN = 0, 1, 2
+
+struct SolverOpts                      # eq_solver_opts, field for field
+    tol::Float64
+    maxiter::Int64
+    pc::Cint
+    prec::Cint
+    guess::Cint
+    record_snapshots::Cint
+    graphs::Cint
+    cheb_single::Cint
+    sync_pass2::Cint
+    galerkin_full::Cint
+end
+
+"""set_solver!(ctx; tol = 1e-6, ...): the cg! settings of the loop's quasi-static
+solves (src/main.jl:256: tol = 1e-6, previous-step start)."""
+function set_solver!(ctx::Context; tol = 1e-6, maxiter = 0, pc = EQ_PC_AMG, prec = EQ_PREC_FP64,
+                     guess = EQ_GUESS_PREVIOUS, record_snapshots = true, graphs = true)
+    o = SolverOpts(tol, maxiter, pc, prec, guess, record_snapshots, graphs, 0, 0, 0)
+    check(ctx, ccall((:eq_set_solver, LIB), Cint, (Ptr{Cvoid}, Ref{SolverOpts}), ctx.h, o))
+end
+
+"""state_init!(ctx, v_init; psi_init = nothing): the initial state of
+src/main.jl:99-190 (v = v_init, then v .-= Vpl/2; d = a = 0; psi = psi_init,
+or the formula of src/main.jl:125 when nothing)."""
+function state_init!(ctx::Context, v_init; psi_init = nothing)
+    psi = psi_init === nothing ? nothing : convert(Vector{Float64}, psi_init)
+    GC.@preserve psi check(ctx, ccall((:eq_state_init, LIB), Cint, (Ptr{Cvoid}, Cdouble, Ptr{Float64}),
+                                      ctx.h, v_init, psi === nothing ? Ptr{Float64}(C_NULL) : pointer(psi)))
+end
+
+struct StepInfo                        # eq_step_info, field for field
+    t::Float64
+    dt_used::Float64
+    dt_next::Float64
+    Vfmax::Float64
+    it::Int64
+    isolver_used::Int64
+    isolver_next::Int64
+    pcg_iters::NTuple{2,Int64}
+    nr_noconv::Int64
+end
+
+"""step!(ctx): one pass of the loop body (src/main.jl:221-451) without the
+output bookkeeping; nothing after the reference's NR SEARCH FAILED! report."""
+function step!(ctx::Context)
+    info = Ref{StepInfo}()
+    rc = ccall((:eq_step, LIB), Cint, (Ptr{Cvoid}, Ref{StepInfo}), ctx.h, info)
+    if rc == EQ_NR_NAN
+        println(lasterr(ctx))
+        @error("NR SEARCH FAILED!")
+        return
+    end
+    check(ctx, rc)
+    return info[]
+end
+
+mutable struct Results                 # eq_results, field for field
+    cap_steps::Int64; cap_events::Int64; cap_is::Int64; cap_seis::Int64; cap_rec::Int64
+    time_::Ptr{Float64}; Vfmax::Ptr{Float64}; isolver::Ptr{Int64}; pcg_iters::Ptr{Int64}
+    tStart::Ptr{Float64}; tEnd::Ptr{Float64}; hypo::Ptr{Float64}
+    taubefore::Ptr{Float64}; tauafter::Ptr{Float64}; delfafter::Ptr{Float64}
+    is_slip::Ptr{Float64}; is_slipvel::Ptr{Float64}; is_stress::Ptr{Float64}
+    seismic_slip::Ptr{Float64}; seismic_slipvel::Ptr{Float64}; seismic_stress::Ptr{Float64}
+    dSeis::Ptr{Float64}; vSeis::Ptr{Float64}; aSeis::Ptr{Float64}
+    n_steps::Int64; n_start::Int64; n_end::Int64; n_is::Int64; n_seis::Int64; n_rec::Int64
+    n_qs::Int64; n_dyn::Int64; total_pcg_iters::Int64; nr_noconv::Int64
+    wall_qs::Float64; wall_dyn::Float64
+end
+
+"""main(ctx, P; v_init, psi_init = nothing, total_time = P[2].Total_time, ...):
+the whole loop of src/main.jl:67-476 on the device, from the initial state
+(state_init!) to total_time.  Returns the fields of struct results
+(src/main.jl:479-498) under the reference's names, fault matrices as
+[FltNglob x n] and receiver traces as [n x receivers].  After NR SEARCH
+FAILED! the results up to that step are returned."""
+function main(ctx::Context, P; v_init, psi_init = nothing, total_time = P[2].Total_time,
+              cap_steps = 200_000, cap_events = 400, cap_is = 4_000, cap_seis = 30_000, cap_rec = 190_000,
+              prec = EQ_PREC_FP64, guess = EQ_GUESS_PREVIOUS)
+    F = P[1].FltNglob
+    nr = length(P[4].out_seis)
+    set_solver!(ctx; prec = prec, guess = guess)
+    state_init!(ctx, v_init; psi_init = psi_init)
+    time_ = zeros(cap_steps); Vfmax = zeros(cap_steps); iso = zeros(Int64, cap_steps)
+    iters = zeros(Int64, 2 * cap_steps)
+    tStart = zeros(cap_events); tEnd = zeros(cap_events); hypo = zeros(cap_events)
+    taub = zeros(F, cap_events); taua = zeros(F, cap_events); delf = zeros(F, cap_events)
+    is_slip = zeros(F, cap_is); is_slipvel = zeros(F, cap_is); is_stress = zeros(F, cap_is)
+    s_slip = zeros(F, cap_seis); s_slipvel = zeros(F, cap_seis); s_stress = zeros(F, cap_seis)
+    dS = zeros(cap_rec, nr); vS = zeros(cap_rec, nr); aS = zeros(cap_rec, nr)
+    bufs = (time_, Vfmax, iso, iters, tStart, tEnd, hypo, taub, taua, delf, is_slip, is_slipvel, is_stress,
+            s_slip, s_slipvel, s_stress, dS, vS, aS)
+    R = GC.@preserve bufs begin
+        r = Results(cap_steps, cap_events, cap_is, cap_seis, cap_rec,
+                    pointer(time_), pointer(Vfmax), pointer(iso), pointer(iters),
+                    pointer(tStart), pointer(tEnd), pointer(hypo), pointer(taub), pointer(taua), pointer(delf),
+                    pointer(is_slip), pointer(is_slipvel), pointer(is_stress),
+                    pointer(s_slip), pointer(s_slipvel), pointer(s_stress),
+                    pointer(dS), pointer(vS), pointer(aS), 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0.0, 0.0)
+        rc = ccall((:eq_run, LIB), Cint, (Ptr{Cvoid}, Cdouble, Int64, Ref{Results}), ctx.h, total_time, 0, r)
+        if rc == EQ_NR_NAN
+            println(lasterr(ctx))
+            @error("NR SEARCH FAILED!")
+        else
+            check(ctx, rc)
+        end
+        r
+    end
+    ns = min(R.n_steps, cap_steps); ne = min(R.n_end, cap_events); nst = min(R.n_start, cap_events)
+    nis = min(R.n_is, cap_is); nse = min(R.n_seis, cap_seis); nrc = min(R.n_rec, cap_rec)
+    return (seismic_stress = s_stress[:, 1:nse], seismic_slipvel = s_slipvel[:, 1:nse],
+            seismic_slip = s_slip[:, 1:nse], index_eq = Int64[], is_stress = is_stress[:, 1:nis],
+            is_slipvel = is_slipvel[:, 1:nis], is_slip = is_slip[:, 1:nis], dSeis = dS[1:nrc, :],
+            vSeis = vS[1:nrc, :], aSeis = aS[1:nrc, :], tStart = tStart[1:nst], tEnd = tEnd[1:ne],
+            taubefore = taub[:, 1:nst], tauafter = taua[:, 1:ne], delfafter = delf[:, 1:ne],
+            hypo = hypo[1:nst], time_ = time_[1:ns], Vfmax = Vfmax[1:ns])
 end
 
 end # module
