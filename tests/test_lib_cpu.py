@@ -57,3 +57,34 @@ def test_null_context_is_rejected():
     assert checked == len(L.EXPORTS) - 3
     assert lib.eq_last_error(None) == b"null context"
     lib.eq_destroy(None)
+
+
+def _c_struct_fields(hdr, name):
+    body = re.search(r"typedef struct \{([^{}]*)\}\s*" + name + ";", hdr).group(1)
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    out = []
+    for decl in body.split(";"):
+        decl = decl.strip()
+        if not decl:
+            continue
+        names = re.sub(r"^(const\s+)?\w+\s*", "", decl)
+        for nm in names.split(","):
+            out.append(re.sub(r"\[.*\]", "", nm).strip(" *"))
+    return out
+
+
+def _jl_struct_fields(src, name):
+    body = re.search(r"struct " + name + r"\b[^\n]*\n(.*?)\nend", src, flags=re.S).group(1)
+    return re.findall(r"(\w+)::", body)
+
+
+def test_julia_binding_matches_header():
+    """The Julia binding calls only exported entry points, and its mirrors of
+    the C structs list the same fields in the same order."""
+    hdr = open(os.path.join(ROOT, "include", "eqcycle_hip.h")).read()
+    jl = open(os.path.join(ROOT, "julia", "EqcycleHIP.jl")).read()
+    called = set(re.findall(r"\(:(eq_\w+), LIB\)", jl))
+    assert called and called <= set(L.EXPORTS), called - set(L.EXPORTS)
+    for c_name, jl_name in (("eq_solver_opts", "SolverOpts"), ("eq_step_info", "StepInfo"),
+                            ("eq_results", "Results")):
+        assert _jl_struct_fields(jl, jl_name) == _c_struct_fields(hdr, c_name), c_name
