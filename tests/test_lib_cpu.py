@@ -85,6 +85,7 @@ def test_julia_binding_matches_header():
     jl = open(os.path.join(ROOT, "julia", "EqcycleHIP.jl")).read()
     called = set(re.findall(r"\(:(eq_\w+), LIB\)", jl))
     assert called and called <= set(L.EXPORTS), called - set(L.EXPORTS)
-    for c_name, jl_name in (("eq_solver_opts", "SolverOpts"), ("eq_step_info", "StepInfo"),
+    for c_name, jl_name in (("eq_mesh_desc", "MeshDesc"), ("eq_params_desc", "ParamsDesc"), ("eq_csr", "EqCsr"),
+                            ("eq_solver_opts", "SolverOpts"), ("eq_step_info", "StepInfo"),
                             ("eq_results", "Results")):
         assert _jl_struct_fields(jl, jl_name) == _c_struct_fields(hdr, c_name), c_name
