@@ -511,6 +511,40 @@ def test_run_matches_oracle(oracle):
     assert rel(R["seismic_slip"], np.array(r.seismic_slip)) < 1e-7
 
 
+def test_fast_path_run_matches_oracle(oracle):
+    """The bench's fast path through the first event against the oracle's
+    direct-solve loop: Galerkin start on a POD basis learned from this mesh's
+    own quasi-static steps, FP32 V-cycle, change-of-b starts, early step tail,
+    Chebyshev pairs, graph-replayed CG batches and device-batched dynamic
+    steps, at CG tolerance 1e-12."""
+    from eqcycle_amd import learned
+    from eqcycle_amd.api import EqContext
+    P = set_parameters(small(32, 8, material="trapezoid", total_years=3))
+    ctx = EqContext(P, amg={"hierarchy": "pmg"})
+    ctx.set_solver(tol=1e-12, pc="amg")
+    ctx.state_init()
+    learned.record(ctx, max_steps=150, capacity=160, nbasis=24)
+    learned.fit_basis(ctx, tol=1e-12)
+    ctx.set_solver(tol=1e-12, pc="amg", prec="f32pc", guess="galerkin")
+    ctx.state_init()
+    R = ctx.run()
+    sim = oracle.Simulation(P, tol=1e-12, solver="direct")
+    sim.run()
+    r = sim.res
+    m = dict(time_=rel(R["time_"], r.time_), Vfmax=rel(R["Vfmax"], r.Vfmax),
+             delfafter=rel(R["delfafter"], np.array(r.delfafter)), d=rel(ctx.state_get()["d"], sim.d))
+    print(f"[gpu] fast-path run parity {m}, {R['n_qs']} QS / {R['n_dyn']} dynamic steps, "
+          f"{R['total_pcg_iters'] / max(1, 2 * R['n_qs']):.2f} CG iterations per solve", flush=True)
+    assert R["n_steps"] == len(r.time_)
+    assert len(R["tStart"]) == len(r.tStart) >= 1
+    assert np.allclose(R["tStart"], r.tStart, rtol=1e-8, atol=0)
+    assert np.allclose(R["tEnd"], r.tEnd, rtol=1e-8, atol=0)
+    assert m["time_"] < 1e-6
+    assert m["Vfmax"] < 1e-5
+    assert m["delfafter"] < 1e-6
+    assert m["d"] < 1e-6
+
+
 def test_run_batched_dynamic_steps_equal_single_steps():
     """eq_run advances dynamic steps in device-side batches that stop at the
     regime switch; the trajectory must equal, bit for bit, one eq_step call per
