@@ -107,7 +107,7 @@ struct eq_ctx {
   int64_t it = 0;
   int isolver = 1;
   // solver
-  eq_solver_opts opts{1e-6, 0, EQ_PC_AMG, EQ_PREC_FP64, EQ_GUESS_PREVIOUS, 1, 0};
+  eq_solver_opts opts{1e-6, 0, EQ_PC_AMG, EQ_PREC_FP64, EQ_GUESS_PREVIOUS, 1, 0, 0, 0, 0};
   // captured CG iteration batches: (precision, f32pc, pc, iterations, parity, x) -> executable graph
   std::map<std::tuple<int, int, int, int64_t, int, const void*>, hipGraphExec_t> cg_graphs;
   CgWork<double> w64;
@@ -873,12 +873,6 @@ template <>
 DevLevelVecs<double>& lvecs<double>(DevLevel& L) { return L.v64; }
 template <>
 DevLevelVecs<float>& lvecs<float>(DevLevel& L) { return L.v32; }
-template <typename T>
-static const T* ldinv(const DevLevel& L);
-template <>
-const double* ldinv<double>(const DevLevel& L) { return L.dinv; }
-template <>
-const float* ldinv<float>(const DevLevel& L) { return L.dinv32; }
 template <typename T>
 static CgWork<T>& work(eq_ctx* c);
 template <>
@@ -2515,6 +2509,6 @@ extern "C" int eq_learn_predict(eq_ctx* ctx, const double* dF, double* coef) {
   ctx->qs_r_valid = false;
   LEARN_READY(ctx);
   std::string err;
-  if (int rc = learn_predict(&ctx->learn, dF, coef, ctx->s, &err)) return fail(ctx, EQ_HIP_ERR, err);
+  if (learn_predict(&ctx->learn, dF, coef, ctx->s, &err) != 0) return fail(ctx, EQ_HIP_ERR, err);
   return EQ_OK;
 }
