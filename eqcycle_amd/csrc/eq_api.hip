@@ -207,6 +207,14 @@ struct eq_ctx {
     }                                                             \
   } while (0)
 
+// Order of Julia's maximum/findmax (src/main.jl:348,367): NaN is the largest
+// value, and among equal values (NaN included) the lowest index wins.
+static bool findmax_before_host(double o, double oi, double m, double mi) {
+  if (std::isnan(o)) return !std::isnan(m) || oi < mi;
+  if (std::isnan(m)) return false;
+  return o > m || (o == m && oi < mi);
+}
+
 static int fail(eq_ctx* ctx, int code, const std::string& msg) {
   if (ctx) ctx->err = msg;
   return code;
@@ -1850,7 +1858,6 @@ extern "C" int eq_step(eq_ctx* ctx, eq_step_info* info) {
     ctx->err = buf;
     return EQ_NR_NAN;
   }
-  if (!std::isfinite(st.Vfmax)) return fail(ctx, EQ_NR_NAN, "non-finite slip rate");
   return EQ_OK;
 }
 
@@ -1910,7 +1917,7 @@ static int dyn_batch(eq_ctx* ctx, double total_time, int64_t max_more, int* m, S
     bool ran = true;
     for (int b = 0; b < nbx; b++) {
       if (vp[2 * b + 1] < 0.0) { ran = false; break; }
-      if (vp[2 * b] > mx || (vp[2 * b] == mx && vp[2 * b + 1] < mi)) { mx = vp[2 * b]; mi = vp[2 * b + 1]; }
+      if (findmax_before_host(vp[2 * b], vp[2 * b + 1], mx, mi)) { mx = vp[2 * b]; mi = vp[2 * b + 1]; }
     }
     if (!ran) break;
     StepStatus& st = ctx->st_host[q];
@@ -1921,7 +1928,7 @@ static int dyn_batch(eq_ctx* ctx, double total_time, int64_t max_more, int* m, S
     st.nr_flag = ctx->nrflags_host[2 * q] == INT_MAX ? 0 : ctx->nrflags_host[2 * q];
     st.nr_noconv = ctx->nrflags_host[2 * q + 1];
     n = q + 1;
-    if (st.isolver_next != 2 || st.nr_flag != 0 || !std::isfinite(st.Vfmax)) break;
+    if (st.isolver_next != 2 || st.nr_flag != 0) break;
   }
   if (n == 0) return fail(ctx, EQ_HIP_ERR, "dynamic batch: first step has no record");
   if (n & 1) {
@@ -2168,8 +2175,7 @@ extern "C" int eq_run(eq_ctx* ctx, double total_time, int64_t max_steps, eq_resu
           ctx->err = buf;
           return EQ_NR_NAN;
         }
-        if (!std::isfinite(st.Vfmax)) return fail(ctx, EQ_NR_NAN, "non-finite slip rate");
-        if (int rc2 = book(info, q, fetched[q] ? ctx->rec_fault_host + (int64_t)q * 3 * NX : nullptr)) return rc2;
+              if (int rc2 = book(info, q, fetched[q] ? ctx->rec_fault_host + (int64_t)q * 3 * NX : nullptr)) return rc2;
       }
     } else {
       eq_step_info info;

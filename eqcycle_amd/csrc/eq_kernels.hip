@@ -690,6 +690,17 @@ struct OpGuessDot {
   }
 };
 
+// Order of Julia's maximum/findmax over the fault velocities (Vfmax and the
+// hypocentre, src/main.jl:348,367): NaN is the largest value, and among equal
+// values (NaN included) the lowest index wins.  True when (o, oi) replaces
+// the running (m, mi).
+template <typename I>
+__device__ __forceinline__ bool findmax_before(double o, I oi, double m, I mi) {
+  if (isnan(o)) return !isnan(m) || oi < mi;
+  if (isnan(m)) return false;
+  return o > m || (o == m && oi < mi);
+}
+
 // --------------------------------------------------------------------------
 // fault physics (FP64, libm-equivalent device exp/log)
 // --------------------------------------------------------------------------
@@ -715,7 +726,20 @@ __device__ double d_ids2(double xLf, double Vo, double psi, double psi1, double 
 // NRsearch! (src/NRsearch_serial.jl:47-98): Newton on
 //   g(tau) = Z Vfree - Z Vf(tau) + tauo - tau,  Vf = Vo (e^{h+fa} - e^{h-fa}),
 // stopping when |delta| <= 0.001 a sigma; returns k (negative on the
-// |delta| > 1e10 / k == 1000 exit, which returns the current iterate).
+// |delta| > 1e10 / k == 1000 exit, which returns the current iterate: Vf at
+// the last point evaluated, tau after its step).
+//
+// The reference iterates in BigFloat (:9,51-54), where e^{h+fa} never
+// overflows.  At a fast-slipping node the first step from tau = 0 lands far
+// above the root; there, with M = h + |fa|, e^M dwarfs every other term, the
+// step is c = -sign(fa) Vo / (fact Vo/(a sigma)) to far below FP64 rounding,
+// and |fa| falls by 1 per step, for hundreds of steps.  While
+// M >= max(h + 40, ln(A / (1e-20 Z Vo))), A = |Z Vfree + tauo - tau| + 1000|c|
+// (a bound of the numerator's other term over any run of steps), the
+// neglected terms stay below 1e-20 of the kept ones, so the run of steps down
+// to that bound is taken at once (k += n, tau += n c: the same iteration count
+// and, to rounding, the same iterate).  The other steps run as written, at M
+// below about 65, far from the FP64 range of exp.
 __device__ int d_nrsearch(double fo, double Vo, double cca, double ccb, double Seff, double tau, double tauo,
                           double psi, double FltZ, double FltVfree, double& Vf_out, double& tau_out) {
   const double Vw = 1e10;
@@ -726,13 +750,38 @@ __device__ int d_nrsearch(double fo, double Vo, double cca, double ccb, double S
   // fewer in the iteration's dependency chain (the iterates agree with the
   // divided form to rounding)
   const double kfa = fact / (Seff * cca);
+  const double vfc = fact * (Vo / (cca * Seff));   // Vf' = vfc (help1 + help2)
   int k = 0;
   double delta = INFINITY, Vf = 0.0;
   while (fabs(delta) > eps) {
     const double fa = kfa * tau;
+    const double M = fmax(help + fa, help - fa);
+    if (isfinite(M) && M >= help + 40.0) {
+      const double c = copysign(Vo / vfc, -fa);
+      const double Ta = log((fabs(FltZ * FltVfree + tauo - tau) + 1000.0 * fabs(c)) / (1e-20 * FltZ * Vo));
+      if (M >= Ta) {   // false for a NaN Ta
+        // the steps at M, M - dM, ... down to max(h + 40, Ta); one step if it
+        // ends the loop
+        double nd = 1.0;
+        if (fabs(c) > eps && fabs(c) <= 1e10) nd = floor((M - fmax(help + 40.0, Ta)) / fabs(kfa * c)) + 1.0;
+        if (nd > 1000 - k) nd = 1000 - k;
+        const int n = (int)nd;
+        const double tau_last = tau + (n - 1) * c;
+        tau = tau + n * c;
+        k += n;
+        delta = c;
+        if (fabs(delta) > 1e10 || k == 1000) {
+          const double fl = kfa * tau_last;
+          Vf_out = Vo * (exp(help + fl) - exp(help - fl));
+          tau_out = tau;
+          return -k;
+        }
+        continue;
+      }
+    }
     const double help1 = exp(help + fa), help2 = exp(help - fa);
     Vf = Vo * (help1 - help2);
-    const double Vfprime = fact * (Vo / (cca * Seff)) * (help1 + help2);
+    const double Vfprime = vfc * (help1 + help2);
     delta = (FltZ * FltVfree - FltZ * Vf + tauo - tau) / (1 + FltZ * Vfprime);
     tau = tau + delta;
     k = k + 1;
@@ -832,15 +881,17 @@ struct OpNewmark {
     for (int b = lane; b < nbx; b += 64) {
       const double o = vprev[2 * b], oi = vprev[2 * b + 1];
       if (oi < 0.0) notrun = 1;
-      else if (o > m || (o == m && oi < mi)) { m = o; mi = oi; }
+      else if (findmax_before(o, oi, m, mi)) { m = o; mi = oi; }
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
       const double o = __shfl_xor(m, off, 64), oi = __shfl_xor(mi, off, 64);
-      if (o > m || (o == m && oi < mi)) { m = o; mi = oi; }
+      if (findmax_before(o, oi, m, mi)) { m = o; mi = oi; }
     }
     const double Vfmax = 2 * m + fp.Vpl;
-    const bool stop = __any(notrun) || nrprev[0] != INT_MAX || !isfinite(Vfmax) || Vfmax < 2e-3;
+    // a NaN Vfmax keeps the dynamic regime (NaN < 2e-3 is false), as in the
+    // reference; the step's own FBC! guard then stops on the NaN slip rate
+    const bool stop = __any(notrun) || nrprev[0] != INT_MAX || Vfmax < 2e-3;
     if (stop && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) vpart_out[1] = -1.0;
     return !stop;
   }
@@ -995,14 +1046,15 @@ __global__ __launch_bounds__(TBLOCK) void newmark_kernel(Geom g, OpNewmark op) {
     op.node(pre, I, I, 0, ku, dn, e, a0, a1);
   }
   if (!op.vpart_out) return;
-  // (max, first index) of the new fault velocity per TILE_I fault nodes; as
-  // the step tail: NaN is never the maximum, no valid value leaves (-inf, INT_MAX)
-  const bool valid = I < op.fp.nflt && !isnan(a0);
+  // (max, first index) of the new fault velocity per TILE_I fault nodes, in
+  // findmax order as the step tail; a tile with no fault node leaves
+  // (-inf, INT_MAX)
+  const bool valid = I < op.fp.nflt;
   double m = valid ? a0 : -INFINITY, mi = valid ? a1 : 2147483647.0;
 #pragma unroll
   for (int off = TILE_I / 2; off > 0; off >>= 1) {
     const double o = __shfl_down(m, off, TILE_I), oi = __shfl_down(mi, off, TILE_I);
-    if (o > m || (o == m && oi < mi)) { m = o; mi = oi; }
+    if (findmax_before(o, oi, m, mi)) { m = o; mi = oi; }
   }
   const int b = I / TILE_I;
   if (I % TILE_I == 0 && b < op.nbx) {
@@ -1316,7 +1368,7 @@ __global__ __launch_bounds__(BLOCK) void step_tail_kernel(FaultDev fp, const dou
   int mi = INT_MAX;
   for (int j = threadIdx.x; j < fp.nflt; j += BLOCK) {
     const double vj = v[j];
-    if (vj > m) { m = vj; mi = j; }
+    if (findmax_before(vj, j, m, mi)) { m = vj; mi = j; }
     if (j >= fp.j0) {
       const double Vc = 2 * vj + fp.Vpl;
       if (fabs(Vc) * dtmax > fp.XiLf[j]) {
@@ -1329,7 +1381,7 @@ __global__ __launch_bounds__(BLOCK) void step_tail_kernel(FaultDev fp, const dou
   for (int off = 32; off > 0; off >>= 1) {
     const double om = __shfl_down(m, off, 64);
     const int oi = __shfl_down(mi, off, 64);
-    if (om > m || (om == m && oi < mi)) { m = om; mi = oi; }
+    if (findmax_before(om, oi, m, mi)) { m = om; mi = oi; }
     dtnx = fmin(dtnx, __shfl_down(dtnx, off, 64));
   }
   __shared__ double sm[BLOCK / 64], sdt[BLOCK / 64];
@@ -1344,7 +1396,7 @@ __global__ __launch_bounds__(BLOCK) void step_tail_kernel(FaultDev fp, const dou
   mi = si[0];
   dtnx = sdt[0];
   for (int w = 1; w < BLOCK / 64; w++) {
-    if (sm[w] > m || (sm[w] == m && si[w] < mi)) { m = sm[w]; mi = si[w]; }
+    if (findmax_before(sm[w], si[w], m, mi)) { m = sm[w]; mi = si[w]; }
     dtnx = fmin(dtnx, sdt[w]);
   }
   const double Vfmax = 2 * m + fp.Vpl;

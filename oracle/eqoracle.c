@@ -632,27 +632,66 @@ double orc_ids2(double xLf, double Vo, double psi, double psi1, double dt, doubl
     return log(exp(psi - VdtL) + (Vo / (0.5 * fabs(Vf + Vf1))) * (1 - exp(-VdtL)));
 }
 
-/* NRsearch! (src/NRsearch_serial.jl:47-98) in FP64; returns the iteration count */
+/* NRsearch! (src/NRsearch_serial.jl:47-98) in FP64; returns the iteration count, */
+/* negative on the |delta| > 1e10 / k == 1000 exit, which returns the current    */
+/* iterate (Vf at the last point evaluated, tau after its step).                  */
+/*                                                                                */
+/* The reference keeps fa, help1, help2, delta and tau in BigFloat (:9,51-54), so */
+/* its iterates never overflow.  At a fast-slipping node the first step from      */
+/* tau = 0 lands far above the root.  There e^M, M = help + |fa|, dwarfs every    */
+/* other term of the step: delta equals c = -sign(fa) Vo / (fact Vo/(a sigma))    */
+/* to far below FP64 rounding, and |fa| falls by |c| fact/(sigma a) = 1 per step, */
+/* for hundreds of steps.  While M >= max(help + 40, ln(A / (1e-20 Z Vo))), with   */
+/* A = |Z Vfree + tauo - tau| + 1000 |c| bounding the numerator's other term over */
+/* any run of steps, the neglected terms stay below 1e-20 of the kept ones        */
+/* (e^{-2|fa|} <= e^{-80} and A e^{-M} <= 1e-20 Z Vo), so that run of steps is    */
+/* taken at once: k += n, tau += n c.  Every other step runs as written, at M     */
+/* below about 65, where FP64 exp is far from overflow.                           */
 int orc_nrsearch(double fo, double Vo, double cca, double ccb, double Seff, double tau, double tauo,
                  double psi, double FltZ, double FltVfree, double *Vf_out, double *tau_out) {
     const double Vw = 1e10;
     double fact = 1. + (Vo / Vw) * exp(-psi);
     double eps = 0.001 * cca * Seff;
+    double help = -(fo + ccb * psi) / cca;
+    double vfc = fact * (Vo / (cca * Seff));   /* Vf' = vfc (help1 + help2) */
     int k = 0;
     double delta = INFINITY, Vf = 0.0;
     while (fabs(delta) > eps) {
         double fa = fact * tau / (Seff * cca);
-        double help = -(fo + ccb * psi) / cca;
+        double M = fmax(help + fa, help - fa);
+        if (isfinite(M) && M >= help + 40.0) {
+            double c = copysign(Vo / vfc, -fa);
+            double Ta = log((fabs(FltZ * FltVfree + tauo - tau) + 1000.0 * fabs(c)) / (1e-20 * FltZ * Vo));
+            if (M >= Ta) {   /* false for a NaN Ta */
+                /* the steps at M, M - dM, ... down to max(help + 40, Ta); one step
+                   if it ends the loop */
+                double Tlin = fmax(help + 40.0, Ta);
+                double nd = 1.0;
+                if (fabs(c) > eps && fabs(c) <= 1e10) nd = floor((M - Tlin) / fabs(fact * c / (Seff * cca))) + 1.0;
+                if (nd > 1000 - k) nd = 1000 - k;
+                int n = (int)nd;
+                double tau_last = tau + (n - 1) * c;
+                tau = tau + n * c;
+                k += n;
+                delta = c;
+                if (fabs(delta) > 1e10 || k == 1000) {
+                    double fl = fact * tau_last / (Seff * cca);
+                    *Vf_out = Vo * (exp(help + fl) - exp(help - fl));
+                    *tau_out = tau;
+                    return -k;
+                }
+                continue;
+            }
+        }
         double help1 = exp(help + fa), help2 = exp(help - fa);
         Vf = Vo * (help1 - help2);
-        double Vfprime = fact * (Vo / (cca * Seff)) * (help1 + help2);
+        double Vfprime = vfc * (help1 + help2);
         delta = (FltZ * FltVfree - FltZ * Vf + tauo - tau) / (1 + FltZ * Vfprime);
         tau = tau + delta;
         k = k + 1;
         if (fabs(delta) > 1e10 || k == 1000) { *Vf_out = Vf; *tau_out = tau; return -k; }
     }
     double fa = fact * tau / (Seff * cca);
-    double help = -(fo + ccb * psi) / cca;
     Vf = Vo * (exp(help + fa) - exp(help - fa));
     *Vf_out = Vf; *tau_out = tau;
     return k;

@@ -292,19 +292,23 @@ def nrsearch_mp(fo, Vo, cca, ccb, Seff, tau, tauo, psi, FltZ, FltVfree, prec=256
     """NRsearch! with the BigFloat variables at 256-bit precision (src/NRsearch_serial.jl:47-98).
 
     fact, eps and help stay Float64 as in the reference; fa, help1, help2,
-    delta and tau are multi-precision (mpmath stands in for MPFR)."""
+    delta and tau are multi-precision (mpmath stands in for MPFR).  A Float64
+    starting tau (the second call, tau0 = tau1, :37-38) stays Float64 until the
+    first update, so the first fa is formed in Float64 and then widened, as the
+    typed assignment ``fa::BigFloat = fact*tau/(Seff*cca)`` does; the first
+    call's tau0 is BigFloat 0 (:9,13)."""
     import mpmath as mp
     with mp.workprec(prec):
         Vw = 1e10
         fact = 1.0 + (Vo / Vw) * float(np.exp(-psi))
         eps = 0.001 * cca * Seff
         k = 0
-        tau = mp.mpf(tau)
+        tau = float(tau)
         delta = mp.inf
         Vf = mp.mpf(0)
         help_ = -(fo + ccb * psi) / cca
         while abs(delta) > eps:
-            fa = fact * tau / (Seff * cca)
+            fa = mp.mpf(fact * tau / (Seff * cca))
             help1 = mp.exp(help_ + fa)
             help2 = mp.exp(help_ - fa)
             Vf = Vo * (help1 - help2)

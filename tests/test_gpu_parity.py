@@ -5,6 +5,8 @@ summation order differs; exp/log come from different libraries), FP32 K u at
 1e-5, linear solves against a sparse direct solve at the CG tolerance times
 a condition allowance, time steps at 1e-10 relative per step.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -338,6 +340,30 @@ def test_nrsearch(mesh, oracle):
         assert kg[j] == ko
         assert abs(tg[j] - to) <= 1e-12 * max(abs(to), 1.0)
         assert abs(Vg[j] - vo) <= 1e-11 * max(abs(vo), 1e-300)
+
+
+def test_nrsearch_fast_slip_matches_bigfloat_sequence(mesh):
+    """Device NRsearch! on the 256-bit fast-slip fixtures
+    (tests/golden/nr_ids_cases.json, nrsearch_fast: 1-130 m/s, FltZ up to
+    2.4e7, the k == 1000 and |delta| > 1e10 exits): the same iteration count
+    and exit as the BigFloat sequence of src/NRsearch_serial.jl:47-98, tau to
+    1e-11, the slip rate to 1e-9 (infinite where the reference's Float64
+    conversion is)."""
+    import json
+    P, ctx, _, _ = mesh
+    G = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "nr_ids_cases.json")))
+    cases = G["nrsearch_fast"]
+    col = {k: np.array([c[k] for c in cases], dtype=np.float64)
+           for k in ("fo", "Vo", "cca", "ccb", "Seff", "tau0", "tauo", "psi", "FltZ", "FltVfree")}
+    Vg, tg, kg = ctx.nrsearch(col["fo"], col["Vo"], col["cca"], col["ccb"], col["Seff"], col["tau0"], col["tauo"],
+                              col["psi"], col["FltZ"], col["FltVfree"])
+    for i, c in enumerate(cases):
+        assert kg[i] == c["k"], (i, c)
+        assert abs(tg[i] - c["tau"]) <= 1e-11 * max(abs(c["tau"]), 1.0), (i, c)
+        if np.isinf(c["Vf"]) or np.isinf(Vg[i]):
+            assert Vg[i] == c["Vf"], (i, c)
+        else:
+            assert abs(Vg[i] - c["Vf"]) <= 1e-9 * abs(c["Vf"]), (i, c)
 
 
 def test_fbc_and_slr(mesh, oracle):
@@ -679,13 +705,17 @@ def test_consecutive_runs_continue_one_run():
 
 
 def test_nr_failure_stops_run_like_oracle(oracle):
-    """A slip rate the rate-state search cannot follow (free slip velocity of
-    ~1e10 Pa / FltZ at the surface node: the first Newton step from tau = 0
-    overflows exp, src/NRsearch.jl:63-94) stops the run with NR SEARCH FAILED
-    at the same fault location as the oracle's FBC! (src/NRsearch.jl:15-54:
-    the first failing node in loop order; the surface node's displacement
-    reaches its neighbour through K in the same step);
-    keep_on_nr_failure returns the results up to that step."""
+    """A surface slip rate the rate-state search cannot follow, as the BigFloat
+    search of src/NRsearch_serial.jl:47-98 behaves on it: free slip velocity
+    ~390 m/s at the surface node.  The first search from tau = 0 takes the
+    k == 1000 exit with an infinite rate and a finite tau, so the guard of
+    :19 (which tests the step's incoming Vf and tau1) lets it through;
+    IDS2! then gives psi2 = -inf and the second search NaN.  The step ends
+    with a NaN traction at that node and Vfmax = NaN, which keeps the dynamic
+    regime (src/main.jl:439-443); the next step's guard fires at the first
+    node, in loop order, that the NaN reached through K -- the lowest node
+    of the surface element.  GPU and oracle take the same two steps;
+    keep_on_nr_failure returns the results up to the failing step."""
     import re
     from eqcycle_amd.api import EqContext, EqError
     P = set_parameters(small(32, 8, material="trapezoid", total_years=3))
@@ -700,16 +730,23 @@ def test_nr_failure_stops_run_like_oracle(oracle):
     sim = oracle.Simulation(P, solver="direct")
     sim.d, sim.v, sim.a = s["d"].copy(), v.copy(), s["a"].copy()
     sim.psi, sim.isolver, sim.t, sim.dt = s["psi"].copy(), 2, 1e7, P.dt0
+    sim.step()
+    assert np.isnan(sim.tau[j]) and sim.isolver == 2
+    assert np.isfinite(sim.tau[:j]).all()
     with pytest.raises(FloatingPointError) as ei:
         sim.step()
     loc = int(re.search(r"fault node (\d+)$", str(ei.value)).group(1))
-    assert F - 2 <= loc <= F
+    assert loc == F - 4     # 1-based: the surface element spans nodes F-4 .. F
     kw = dict(d=s["d"], v=v, a=s["a"], psi=s["psi"], tau=s["tau"], t=1e7, dt=P.dt0, isolver=2)
     ctx.state_set(**kw)
     R = ctx.run(max_steps=5, snapshots=False, keep_on_nr_failure=True)
     assert R["error"] is not None and R["error"].endswith(f"(fault location {loc})")
-    assert R["n_steps"] == 0
+    assert R["n_steps"] == 1 and np.isnan(R["Vfmax"][0])
     ctx.state_set(**kw)
+    info = ctx.step()
+    assert np.isnan(info.Vfmax) and info.isolver_next == 2
+    g = ctx.state_get()
+    assert np.isnan(g["tau"][j]) and np.isfinite(g["tau"][:j]).all()
     with pytest.raises(EqError, match=f"NR SEARCH FAILED! \\(fault location {loc}\\)"):
         ctx.step()
     # an ordinary run reports no error

@@ -177,12 +177,51 @@ def test_nrsearch_fp64_matches_bigfloat_sequence(oracle):
         assert abs(Vf - c["Vf"]) <= 1e-8 * abs(c["Vf"])
 
 
+def _same_rate(got, want, rtol):
+    if np.isinf(want) or np.isinf(got):
+        return got == want
+    return abs(got - want) <= rtol * abs(want)
+
+
+def test_nrsearch_fast_slip_matches_bigfloat_sequence(oracle):
+    """Fast slip (1-130 m/s, FltZ up to 2.4e7): the FP64 search follows the
+    BigFloat sequence of src/NRsearch_serial.jl:47-98 where e^{help+fa}
+    exceeds the FP64 range -- the same iteration count, the same exit (k ==
+    1000 with an infinite rate, |delta| > 1e10), tau to 1e-12."""
+    g = json.load(open(os.path.join(GOLD, "nr_ids_cases.json")))
+    cases = g["nrsearch_fast"]
+    assert max(c["k"] for c in cases) > 700 and any(c["k"] == -1000 for c in cases)
+    assert any(c["k"] == -1 for c in cases)
+    for c in cases:
+        Vf, tau, k = oracle.nrsearch(c["fo"], c["Vo"], c["cca"], c["ccb"], c["Seff"], c["tau0"], c["tauo"],
+                                     c["psi"], c["FltZ"], c["FltVfree"])
+        assert k == c["k"], c
+        assert abs(tau - c["tau"]) <= 1e-12 * max(abs(c["tau"]), 1.0), c
+        assert _same_rate(Vf, c["Vf"], 1e-9), c
+
+
+def test_nrsearch_fast_slip_step_taken_as_written_is_overflow(oracle):
+    """The plain FP64 form of the same Newton step (every iterate evaluated
+    with exp, as the dormant src/NRsearch.jl:59-104 does) overflows on the
+    named surface-node case: exp(help + fa) after the first step is out of
+    range, so that form cannot reproduce the BigFloat sequence there."""
+    g = json.load(open(os.path.join(GOLD, "nr_ids_cases.json")))
+    c = next(c for c in g["nrsearch_fast"] if c["name"].startswith("surface node"))
+    fact = 1 + (c["Vo"] / 1e10) * np.exp(-c["psi"])
+    help_ = -(c["fo"] + c["ccb"] * c["psi"]) / c["cca"]
+    Vf0 = 0.0   # at tau = 0, help1 == help2
+    Vfp0 = fact * (c["Vo"] / (c["cca"] * c["Seff"])) * 2 * np.exp(help_)
+    tau1 = (c["FltZ"] * c["FltVfree"] - c["FltZ"] * Vf0 + c["tauo"]) / (1 + c["FltZ"] * Vfp0)
+    assert help_ + fact * tau1 / (c["Seff"] * c["cca"]) > np.log(np.finfo(float).max)
+    assert c["k"] == 713 and abs(c["Vf"] - 36.8594945) < 1e-6
+
+
 def test_nrsearch_solution_properties(oracle):
     """At the returned tau: Vf follows the regularised law exactly, whose inverse is
     tau = (a sigma / fact) asinh(Vf e^{-h} / (2 V0)); the Newton residual is below
     the step tolerance's scale."""
     g = json.load(open(os.path.join(GOLD, "nr_ids_cases.json")))
-    for c in g["nrsearch"]:
+    for c in g["nrsearch"] + [c for c in g["nrsearch_fast"] if c["k"] > 0]:
         Vf, tau, k = oracle.nrsearch(c["fo"], c["Vo"], c["cca"], c["ccb"], c["Seff"], c["tau0"], c["tauo"],
                                      c["psi"], c["FltZ"], c["FltVfree"])
         fact = 1 + (c["Vo"] / 1e10) * np.exp(-c["psi"])
@@ -190,7 +229,7 @@ def test_nrsearch_solution_properties(oracle):
         tau_inv = (c["cca"] * c["Seff"] / fact) * np.arcsinh(Vf * np.exp(-h) / (2 * c["Vo"]))
         assert abs(tau_inv - tau) < 1e-6 * max(abs(tau), 1.0)
         resid = c["FltZ"] * c["FltVfree"] - c["FltZ"] * Vf + c["tauo"] - tau
-        assert abs(resid) < 0.001 * c["cca"] * c["Seff"] * (1 + c["FltZ"] * Vf / (c["cca"] * c["Seff"])) * 10
+        assert abs(resid) < 0.001 * c["cca"] * c["Seff"] * (1 + c["FltZ"] * abs(Vf) / (c["cca"] * c["Seff"])) * 10
 
 
 def test_ids_matches_exact_aging_law(oracle):
