@@ -424,6 +424,18 @@ class EqContext:
         self._check(self.lib.eq_learn_galerkin(self.h, ptr(x), ptr(b), ptr(out)))
         return out
 
+    def launch_timing(self, on: bool):
+        """Bracket every launch of the fine-level kernels with HIP events on the
+        library's stream (CG batches launch eagerly meanwhile); off stops."""
+        self._check(self.lib.eq_launch_timing(self.h, 1 if on else 0))
+
+    def launch_timing_get(self):
+        """{kernel name: (launches, total device ms)} since launch_timing(True)."""
+        n = np.zeros(L.EQ_TK_COUNT, np.int64)
+        ms = np.zeros(L.EQ_TK_COUNT)
+        self._check(self.lib.eq_launch_timing_get(self.h, ptr(n, C.c_int64), ptr(ms)))
+        return {name: (int(n[k]), float(ms[k])) for k, name in enumerate(L.TK_NAMES)}
+
     def time_kernel(self, which="ku", prec="fp64", reps=50):
         """Average device ms per launch, HIP events on the library's stream."""
         w = {"ku": L.EQ_TIME_KU, "cg_dir": L.EQ_TIME_CG_DIR, "vcycle": L.EQ_TIME_VCYCLE,

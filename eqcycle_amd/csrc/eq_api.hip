@@ -167,8 +167,20 @@ struct eq_ctx {
   int learn_record_on = 0;
   double *gp = nullptr, *gq = nullptr;
   int64_t n_learned_solves = 0;
+  // launch timing (eq_launch_timing): event pairs around launches of the
+  // fine-level kernels, summed per kind when the pool fills or is read
+  struct LaunchTiming {
+    static constexpr int kSlots = 4096;
+    bool on = false;
+    std::vector<hipEvent_t> ev;       // [2 kSlots]
+    int kind[kSlots] = {};
+    int used = 0;
+    int64_t count[EQ_TK_COUNT] = {};
+    double ms[EQ_TK_COUNT] = {};
+  } lt;
 
   ~eq_ctx() {
+    for (hipEvent_t e : lt.ev) (void)hipEventDestroy(e);
     for (auto& kv : cg_graphs) (void)hipGraphExecDestroy(kv.second);
     learn_free(&learn);
     for (void* p : allocs) (void)hipFree(p);
@@ -206,6 +218,33 @@ struct eq_ctx {
       return EQ_NOMEM;                                            \
     }                                                             \
   } while (0)
+
+// Launch timing: each bracketed launch carries an event pair that its
+// dispatch stamps (hipExtLaunchKernelGGL).  Pool full or read: wait for the
+// last event and sum per kind.
+static int lt_harvest(eq_ctx* ctx) {
+  auto& L = ctx->lt;
+  if (L.used == 0) return EQ_OK;
+  HIPCHK(ctx, hipEventSynchronize(L.ev[2 * L.used - 1]));
+  for (int i = 0; i < L.used; i++) {
+    float ms = 0.f;
+    HIPCHK(ctx, hipEventElapsedTime(&ms, L.ev[2 * i], L.ev[2 * i + 1]));
+    L.count[L.kind[i]]++;
+    L.ms[L.kind[i]] += ms;
+  }
+  L.used = 0;
+  return EQ_OK;
+}
+// The events of the next bracketed launch ({} when timing is off)
+static LaunchEvents lt_slot(eq_ctx* ctx, int kind) {
+  auto& L = ctx->lt;
+  if (!L.on) return {};
+  if (L.used == L.kSlots) (void)lt_harvest(ctx);
+  L.kind[L.used] = kind;
+  LaunchEvents e{L.ev[2 * L.used], L.ev[2 * L.used + 1]};
+  L.used++;
+  return e;
+}
 
 // Order of Julia's maximum/findmax (src/main.jl:348,367): NaN is the largest
 // value, and among equal values (NaN included) the lowest index wins.
@@ -929,11 +968,12 @@ static void cheb_launch(eq_ctx* ctx, const ChebStep<T>* st, int k) {
   while (m < k) {
     if (pairs && m + 1 < k && st[m].mode == 0) {
       launch_cheb_pair<T>(ctx->g, st[m], st[m + 1], mu_of<T>(ctx), dinv_of<T>(ctx), ctx->part_rr, ctx->npart,
-                          ctx->sc, ctx->part_rho, ctx->s);
+                          ctx->sc, ctx->part_rho, ctx->s,
+                          lt_slot(ctx, sizeof(T) == 4 ? EQ_TK_CHEB_PAIR32 : EQ_TK_CHEB_PAIR64));
       m += 2;
     } else {
       launch_cheb<T>(ctx->g, st[m], mu_of<T>(ctx), dinv_of<T>(ctx), ctx->part_rr, ctx->npart, ctx->sc,
-                     ctx->part_rho, ctx->s);
+                     ctx->part_rho, ctx->s, lt_slot(ctx, sizeof(T) == 4 ? EQ_TK_CHEB_STEP32 : EQ_TK_CHEB_STEP64));
       m += 1;
     }
   }
@@ -1092,9 +1132,9 @@ static void cg_iteration(eq_ctx* ctx, T* x, int pc, bool f32pc = false) {
     nrho = vec_blocks(ctx->N);
   }
   launch_cg_dir<T>(g, w.c, w.u[w.cur], mu_of<T>(ctx), w.u[1 - w.cur], w.q, ctx->part_rho, nrho, ctx->sc,
-                   ctx->part_uq, s);
+                   ctx->part_uq, s, lt_slot(ctx, EQ_TK_CG_DIR));
   launch_cg_update<T>(ctx->N, x, w.r, w.u[1 - w.cur], w.q, ctx->part_rho, nrho, ctx->part_uq, nt, ctx->sc,
-                      ctx->part_rr, ctx->npart, f32pc ? ctx->w32.r : nullptr, s);
+                      ctx->part_rr, ctx->npart, f32pc ? ctx->w32.r : nullptr, s, lt_slot(ctx, EQ_TK_CG_UPDATE));
   w.cur = 1 - w.cur;
 }
 
@@ -1106,7 +1146,7 @@ template <typename T>
 static int enqueue_iterations(eq_ctx* ctx, T* x, int pc, bool f32pc, int64_t n) {
   if (n <= 0) return EQ_OK;
   CgWork<T>& w = work<T>(ctx);
-  if (!ctx->opts.graphs) {
+  if (!ctx->opts.graphs || ctx->lt.on) {
     for (int64_t k = 0; k < n; k++) cg_iteration<T>(ctx, x, pc, f32pc);
     return EQ_OK;
   }
@@ -1788,7 +1828,8 @@ static int dyn_step(eq_ctx* ctx) {
   // predictor (the new d) into d2; the next predictor lands in d, which the swap retires
   launch_newmark_predict(ctx->N, ctx->d, ctx->v, ctx->a, ctx->dt, 0.5 * dtmin * dtmin, ctx->d2, ctx->s);
   launch_newmark(ctx->g, ctx->fp, ctx->fs, ctx->mu, ctx->M, ctx->d2, ctx->v, ctx->a, ctx->d, ctx->v2, ctx->a2,
-                 ctx->dt, 0.5 * dtmin, 0.5 * dtmin * dtmin, ctx->flags, NewmarkRecord{}, ctx->s);
+                 ctx->dt, 0.5 * dtmin, 0.5 * dtmin * dtmin, ctx->flags, NewmarkRecord{}, ctx->s,
+                 lt_slot(ctx, EQ_TK_NEWMARK));
   std::swap(ctx->d, ctx->d2);
   std::swap(ctx->v, ctx->v2);
   std::swap(ctx->a, ctx->a2);
@@ -1901,7 +1942,8 @@ static int dyn_batch(eq_ctx* ctx, double total_time, int64_t max_more, int* m, S
     rec.nseis = (int)ns;
     rec.cap_seis = ns > 0 ? ctx->rec_seis + (int64_t)q * 3 * ns : nullptr;
     launch_newmark(ctx->g, ctx->fp, ctx->fs, ctx->mu, ctx->M, dst[0], src[1], src[2], src[0], dst[1], dst[2],
-                   ctx->dt, 0.5 * dtmin, 0.5 * dtmin * dtmin, ctx->nrflags + 2 * q, rec, s);
+                   ctx->dt, 0.5 * dtmin, 0.5 * dtmin * dtmin, ctx->nrflags + 2 * q, rec, s,
+                   lt_slot(ctx, EQ_TK_NEWMARK));
   }
   HIPCHK(ctx, hipMemcpyAsync(ctx->vpart_host, ctx->vpart, sizeof(double) * 2 * nbx * k, hipMemcpyDeviceToHost, s));
   HIPCHK(ctx, hipMemcpyAsync(ctx->nrflags_host, ctx->nrflags, sizeof(int) * 2 * k, hipMemcpyDeviceToHost, s));
@@ -2516,5 +2558,33 @@ extern "C" int eq_learn_predict(eq_ctx* ctx, const double* dF, double* coef) {
   LEARN_READY(ctx);
   std::string err;
   if (learn_predict(&ctx->learn, dF, coef, ctx->s, &err) != 0) return fail(ctx, EQ_HIP_ERR, err);
+  return EQ_OK;
+}
+
+extern "C" int eq_launch_timing(eq_ctx* ctx, int on) {
+  if (!ctx) return EQ_EINVAL;
+  auto& L = ctx->lt;
+  if (on) {
+    if (L.ev.empty()) {
+      L.ev.resize(2 * L.kSlots, nullptr);
+      for (hipEvent_t& e : L.ev) HIPCHK(ctx, hipEventCreate(&e));
+    }
+    L.used = 0;
+    for (int k = 0; k < EQ_TK_COUNT; k++) { L.count[k] = 0; L.ms[k] = 0.0; }
+    L.on = true;
+    return EQ_OK;
+  }
+  const int rc = lt_harvest(ctx);
+  L.on = false;
+  return rc;
+}
+
+extern "C" int eq_launch_timing_get(eq_ctx* ctx, int64_t* counts, double* total_ms) {
+  if (!ctx || !counts || !total_ms) return EQ_EINVAL;
+  if (int rc = lt_harvest(ctx)) return rc;
+  for (int k = 0; k < EQ_TK_COUNT; k++) {
+    counts[k] = ctx->lt.count[k];
+    total_ms[k] = ctx->lt.ms[k];
+  }
   return EQ_OK;
 }

@@ -4,6 +4,13 @@
 
 namespace eq {
 
+// Optional events of one launch: when set, the launch goes through
+// hipExtLaunchKernelGGL and the kernel's own dispatch stamps its start and
+// end (no marker packets around it).
+struct LaunchEvents {
+  hipEvent_t start = nullptr, stop = nullptr;
+};
+
 constexpr int BLOCK = 256;
 #ifndef EQ_TILE_I
 #define EQ_TILE_I 64
@@ -88,11 +95,11 @@ void launch_cg_init_scal(CgScal* sc, const double* part_rr, const double* part_b
 void launch_cg_check(CgScal* sc, const double* part_rr, int nparts, hipStream_t s);
 template <typename T>
 void launch_cg_dir(const Geom& g, const T* c, const T* u_old, const T* mu, T* u_new, T* q, const double* part_rho,
-                   int nrho, const CgScal* sc, double* part_uq, hipStream_t s);
+                   int nrho, const CgScal* sc, double* part_uq, hipStream_t s, LaunchEvents le = {});
 template <typename T>
 void launch_cg_update(int64_t n, T* x, T* r, const T* u, const T* q, const double* part_rho, int nrho,
                       const double* part_uq, int nuq, CgScal* sc, double* part_rr, int npart, float* r32,
-                      hipStream_t s);
+                      hipStream_t s, LaunchEvents le = {});
 void launch_guess_dot(const Geom& g, const double* p, const double* mu, const double* r0, double* q,
                       double* part_pr, double* part_pq, hipStream_t s);
 void launch_guess_update(int64_t n, double* x, double* r, const double* p, const double* q, const double* part_pr,
@@ -137,7 +144,7 @@ struct ChebStep {
 };
 template <typename T>
 void launch_cheb(const Geom& g, const ChebStep<T>& c, const T* mu, const T* dinv, const double* part_rr, int nrr,
-                 CgScal* sc, double* part_rho, hipStream_t s);
+                 CgScal* sc, double* part_rho, hipStream_t s, LaunchEvents le = {});
 // y = K u on the first nrows grid rows (the element row along the fault for
 // nrows = 5), from global memory; other entries of y untouched.
 template <typename T>
@@ -147,7 +154,8 @@ void launch_ku_rows(const Geom& g, const T* u, const T* mu, T* y, int nrows, hip
 // b.rout and b.dout must not alias a.rin or a.src.
 template <typename T>
 void launch_cheb_pair(const Geom& g, const ChebStep<T>& a, const ChebStep<T>& b, const T* mu, const T* dinv,
-                      const double* part_rr, int nrr, CgScal* sc, double* part_rho, hipStream_t s);
+                      const double* part_rr, int nrr, CgScal* sc, double* part_rho, hipStream_t s,
+                      LaunchEvents le = {});
 
 struct DevCsr {
   int64_t n = 0, m = 0, nnz = 0;
@@ -190,7 +198,7 @@ struct NewmarkRecord {
 void launch_newmark(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, const double* M,
                     const double* P, const double* v, const double* a, double* P2, double* v2, double* a2,
                     double dt, double half_dt, double half_dt_sq, int* flags, const NewmarkRecord& rec,
-                    hipStream_t s);
+                    hipStream_t s, LaunchEvents le = {});
 // P = d + dt v + dt^2/2 a on n nodes
 void launch_newmark_predict(int64_t n, const double* d, const double* v, const double* a, double dt,
                             double half_dt_sq, double* P, hipStream_t s);

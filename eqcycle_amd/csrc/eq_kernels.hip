@@ -18,6 +18,7 @@
 // Reductions: every reducing kernel writes one FP64 partial per workgroup; the
 // consumer kernel re-reduces the partials in a fixed order in each workgroup,
 // so scalars never round-trip through the host and results are reproducible.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -699,6 +700,15 @@ __device__ __forceinline__ bool findmax_before(double o, I oi, double m, I mi) {
   if (isnan(o)) return !isnan(m) || oi < mi;
   if (isnan(m)) return false;
   return o > m || (o == m && oi < mi);
+}
+
+// One launch, through hipExtLaunchKernelGGL when events are given
+template <typename F, typename... Args>
+static void launch_timed(F kernel, dim3 grid, dim3 block, hipStream_t s, LaunchEvents le, Args... args) {
+  if (le.start)
+    hipExtLaunchKernelGGL(kernel, grid, block, 0, s, le.start, le.stop, 0, args...);
+  else
+    hipLaunchKernelGGL(kernel, grid, block, 0, s, args...);
 }
 
 // --------------------------------------------------------------------------
@@ -1655,17 +1665,17 @@ void launch_cg_check(CgScal* sc, const double* part_rr, int nparts, hipStream_t 
 
 template <typename T>
 void launch_cg_dir(const Geom& g, const T* c, const T* u_old, const T* mu, T* u_new, T* q, const double* part_rho,
-                   int nrho, const CgScal* sc, double* part_uq, hipStream_t s) {
+                   int nrho, const CgScal* sc, double* part_uq, hipStream_t s, LaunchEvents le) {
   OpCgDir<T> op{c, u_old, mu, u_new, q, part_rho, nrho, sc, part_uq, nullptr};
-  hipLaunchKernelGGL((tile_kernel<T, OpCgDir<T>>), tile_grid(g), dim3(TBLOCK), 0, s, g, op);
+  launch_timed(tile_kernel<T, OpCgDir<T>>, tile_grid(g), dim3(TBLOCK), s, le, g, op);
 }
 
 template <typename T>
 void launch_cg_update(int64_t n, T* x, T* r, const T* u, const T* q, const double* part_rho, int nrho,
                       const double* part_uq, int nuq, CgScal* sc, double* part_rr, int npart, float* r32,
-                      hipStream_t s) {
-  hipLaunchKernelGGL((cg_update_kernel<T>), dim3(vec_blocks(n)), dim3(BLOCK), 0, s, n, x, r, u, q, part_rho, nrho,
-                     part_uq, nuq, sc, part_rr, npart, r32);
+                      hipStream_t s, LaunchEvents le) {
+  launch_timed(cg_update_kernel<T>, dim3(vec_blocks(n)), dim3(BLOCK), s, le, n, x, r, u, q, part_rho, nrho,
+               part_uq, nuq, sc, part_rr, npart, r32);
 }
 
 template <typename T>
@@ -1692,13 +1702,13 @@ void launch_l0_post(const Geom& g, const T* x0, const T* r, const T* mu, const T
 
 template <typename T>
 void launch_cheb(const Geom& g, const ChebStep<T>& c, const T* mu, const T* dinv, const double* part_rr, int nrr,
-                 CgScal* sc, double* part_rho, hipStream_t s) {
+                 CgScal* sc, double* part_rho, hipStream_t s, LaunchEvents le) {
   OpCheb<T> op{c.src, c.b, mu, dinv, c.s0, c.src_mode, c.rin, c.rout, c.c1, c.c2, c.xin, c.xsrc_center, c.dout,
                c.xout, c.mode, c.write_x_center, c.rdot, c.xout64, c.rdot64, c.check, part_rr, nrr, sc, part_rho,
                nullptr,
                c.coarse, g.nelx, g.nely, {}};
   for (int p = 0; p < 5; p++) op.gx[p] = (T)g.x[p];
-  hipLaunchKernelGGL((tile_kernel<T, OpCheb<T>>), tile_grid(g), dim3(TBLOCK), 0, s, g, op);
+  launch_timed(tile_kernel<T, OpCheb<T>>, tile_grid(g), dim3(TBLOCK), s, le, g, op);
 }
 
 template <typename T>
@@ -1985,7 +1995,8 @@ __global__ __launch_bounds__(PBLOCK) void cheb_pair_kernel(Geom g, ChebPair<T> o
 
 template <typename T>
 void launch_cheb_pair(const Geom& g, const ChebStep<T>& a, const ChebStep<T>& b, const T* mu, const T* dinv,
-                      const double* part_rr, int nrr, CgScal* sc, double* part_rho, hipStream_t s) {
+                      const double* part_rr, int nrr, CgScal* sc, double* part_rho, hipStream_t s,
+                      LaunchEvents le) {
   ChebPair<T> op{};
   op.src = a.src;
   op.b = a.b;
@@ -2017,7 +2028,7 @@ void launch_cheb_pair(const Geom& g, const ChebStep<T>& a, const ChebStep<T>& b,
   op.nrr = nrr;
   op.sc = sc;
   op.part0 = part_rho;
-  hipLaunchKernelGGL((cheb_pair_kernel<T>), tile_grid(g), dim3(PBLOCK), 0, s, g, op);
+  launch_timed(cheb_pair_kernel<T>, tile_grid(g), dim3(PBLOCK), s, le, g, op);
 }
 
 template <typename T, int MODE>
@@ -2109,12 +2120,12 @@ void launch_newmark_predict(int64_t n, const double* d, const double* v, const d
 void launch_newmark(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, const double* M,
                     const double* P, const double* v, const double* a, double* P2, double* v2, double* a2,
                     double dt, double half_dt, double half_dt_sq, int* flags, const NewmarkRecord& rec,
-                    hipStream_t s) {
+                    hipStream_t s, LaunchEvents le) {
   OpNewmark op{fp, fs, mu, M, P, v, a, P2, v2, a2, dt, half_dt, half_dt_sq, flags, g.NY, nullptr, nullptr,
                rec.vprev, rec.nrprev, rec.vpart_out, rec.cap_fault, rec.seis_idx, rec.seis_tile, rec.nseis,
                rec.cap_seis, (int)tile_grid(g).x};
   const dim3 tg = tile_grid(g);
-  hipLaunchKernelGGL(newmark_kernel, dim3(tg.x, tg.y + 1), dim3(TBLOCK), 0, s, g, op);
+  launch_timed(newmark_kernel, dim3(tg.x, tg.y + 1), dim3(TBLOCK), s, le, g, op);
 }
 
 void launch_qs_begin(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, double* d, double* dPre,
@@ -2222,9 +2233,9 @@ void launch_scale_mask(int64_t n, int nx, const double* x, double sign, double* 
   template void launch_cg_residual<T>(const Geom&, const T*, const T*, const T*, T*, double*, double*, int,         \
                                       hipStream_t, float*);                                                         \
   template void launch_cg_dir<T>(const Geom&, const T*, const T*, const T*, T*, T*, const double*, int,            \
-                                 const CgScal*, double*, hipStream_t);                                              \
+                                 const CgScal*, double*, hipStream_t, LaunchEvents);                                \
   template void launch_cg_update<T>(int64_t, T*, T*, const T*, const T*, const double*, int, const double*, int,    \
-                                    CgScal*, double*, int, float*, hipStream_t);                                                 \
+                                    CgScal*, double*, int, float*, hipStream_t, LaunchEvents);                      \
   template void launch_jacobi_pc<T>(int64_t, const T*, const T*, T*, const double*, int, CgScal*, double*,          \
                                     hipStream_t);                                                                   \
   template void launch_l0_down<T>(const Geom&, const T*, const T*, const T*, T, T*, T*, const double*, int,         \
@@ -2237,9 +2248,9 @@ void launch_scale_mask(int64_t n, int nx, const double* x, double sign, double* 
   template void launch_csr_up<T>(const DevCsr&, const DevCsr&, const T*, const T*, T*, const CgScal*, hipStream_t); \
   template void launch_ku_rows<T>(const Geom&, const T*, const T*, T*, int, hipStream_t);                         \
   template void launch_cheb_pair<T>(const Geom&, const ChebStep<T>&, const ChebStep<T>&, const T*, const T*,       \
-                                    const double*, int, CgScal*, double*, hipStream_t);                           \
+                                    const double*, int, CgScal*, double*, hipStream_t, LaunchEvents);             \
   template void launch_cheb<T>(const Geom&, const ChebStep<T>&, const T*, const T*, const double*, int, CgScal*,   \
-                               double*, hipStream_t);                                                              \
+                               double*, hipStream_t, LaunchEvents);                                                \
   template void launch_fill<T>(int64_t, T*, T, hipStream_t);
 EQ_INST(double)
 EQ_INST(float)

@@ -32,9 +32,14 @@ EXPORTS = [
     "eq_step", "eq_run", "eq_time_kernel", "eq_learn_init", "eq_learn_record", "eq_learn_info",
     "eq_learn_gram", "eq_learn_set_basis", "eq_learn_train", "eq_learn_loss_grad", "eq_learn_get",
     "eq_learn_predict", "eq_learn_norms", "eq_learn_galerkin", "eq_newmark_step", "eq_amg_upload",
+    "eq_launch_timing", "eq_launch_timing_get",
 ]
 EQ_TIME_KU, EQ_TIME_CG_DIR, EQ_TIME_VCYCLE, EQ_TIME_CG_ITER, EQ_TIME_NEWMARK, EQ_TIME_CHEB, EQ_TIME_COPY, \
     EQ_TIME_CHEB2 = range(8)
+TK_NAMES = ["newmark_kernel", "cheb_pair_kernel<float>", "cheb_pair_kernel<double>",
+            "tile_kernel<float, OpCheb>", "tile_kernel<double, OpCheb>", "tile_kernel<T, OpCgDir>",
+            "cg_update_kernel<T>"]
+EQ_TK_COUNT = len(TK_NAMES)
 
 dp = C.POINTER(C.c_double)
 i64p = C.POINTER(C.c_int64)
@@ -143,6 +148,8 @@ def load():
         "eq_learn_get": ([vp, fp, fp, fp, fp], C.c_int),
         "eq_learn_predict": ([vp, dp, dp], C.c_int),
         "eq_learn_galerkin": ([vp, dp, dp, dp], C.c_int),
+        "eq_launch_timing": ([vp, C.c_int], C.c_int),
+        "eq_launch_timing_get": ([vp, i64p, dp], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

@@ -301,6 +301,30 @@ enum { EQ_TIME_KU = 0, EQ_TIME_CG_DIR = 1, EQ_TIME_VCYCLE = 2, EQ_TIME_CG_ITER =
  * level-0 Chebyshev step, EQ_TIME_CHEB2 two steps in one launch. */
 int eq_time_kernel(eq_ctx *ctx, int which, int prec, int reps, double *avg_ms);
 
+/* Kinds of launch that eq_launch_timing brackets: the fine-level kernels of
+ * the time loop (the Newmark step and every kernel that streams the whole
+ * mesh inside a CG iteration). */
+enum { EQ_TK_NEWMARK = 0,     /* newmark_kernel: one dynamic step */
+       EQ_TK_CHEB_PAIR32 = 1, /* cheb_pair_kernel<float>: two fine-level Chebyshev steps */
+       EQ_TK_CHEB_PAIR64 = 2, /* cheb_pair_kernel<double> */
+       EQ_TK_CHEB_STEP32 = 3, /* one fine-level Chebyshev step, FP32 */
+       EQ_TK_CHEB_STEP64 = 4, /* one fine-level Chebyshev step, FP64 */
+       EQ_TK_CG_DIR = 5,      /* p = c + beta p, q = K_II p (any precision) */
+       EQ_TK_CG_UPDATE = 6,   /* x += alpha p, r -= alpha q (any precision) */
+       EQ_TK_COUNT = 7 };
+/* on = 1: clear the counters and give every launch of the kinds above a pair
+ * of HIP events on the context's stream until on = 0.  The launch goes
+ * through hipExtLaunchKernelGGL, so the kernel's own dispatch stamps its start
+ * and end (no marker packets between the kernels).  While on, CG iteration
+ * batches are launched one kernel at a time instead of replaying captured
+ * graphs (the same kernels), and the host waits on the events whenever their
+ * pool of 4096 pairs fills. */
+int eq_launch_timing(eq_ctx *ctx, int on);
+/* Launches bracketed and their summed device time (ms) per kind since the
+ * last eq_launch_timing(ctx, 1): counts[EQ_TK_COUNT], total_ms[EQ_TK_COUNT].
+ * Waits for the bracketed launches to finish. */
+int eq_launch_timing_get(eq_ctx *ctx, int64_t *counts, double *total_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -10,6 +10,7 @@
  * Sparse matrices are CSR with int64 indices.
  */
 #include <math.h>
+#include <omp.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -132,7 +133,16 @@ int orc_assemble_K(int64_t nel, int64_t nglob, const int64_t *iglob, const doubl
     return 0;
 }
 
+/* Threads of the OpenMP loops below (SpMV, the coarse dense product, CG's  */
+/* dots and updates).  One thread, the default, is the serial order; more   */
+/* threads change only the summation order of the dot products.  The        */
+/* Gauss-Seidel sweeps stay sequential, as in the reference's smoother.     */
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+
+#define ORC_PAR_MIN 16384   /* shorter loops run on one thread */
+
 void orc_spmv(const orc_csr *A, const double *x, double *y) {
+#pragma omp parallel for schedule(static) if (A->n >= ORC_PAR_MIN)
     for (int64_t i = 0; i < A->n; i++) {
         double s = 0.0;
         for (int64_t q = A->rowptr[i]; q < A->rowptr[i + 1]; q++) s += A->val[q] * x[A->col[q]];
@@ -516,6 +526,7 @@ static void vcycle(orc_amg *ml, int lvl, double *x, const double *b) {
     int64_t nc = ml->A[lvl + 1].n;
     memset(cx, 0, sizeof(double) * nc);
     if (lvl + 1 == ml->nlev - 1) {
+#pragma omp parallel for schedule(static) if (nc >= 256)
         for (int64_t i = 0; i < nc; i++) {
             double s = 0.0;
             for (int64_t j = 0; j < nc; j++) s += ml->coarse_inv[i * nc + j] * cb[j];
@@ -548,6 +559,7 @@ void orc_amg_apply(orc_amg *ml, const double *b, double *x) {
 /* r = b - A x, stop when ||r||_2 <= tol ||b||_2 or after maxiter updates.    */
 static double dot(int64_t n, const double *a, const double *b) {
     double s = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : s) if (n >= ORC_PAR_MIN)
     for (int64_t i = 0; i < n; i++) s += a[i] * b[i];
     return s;
 }
@@ -557,6 +569,7 @@ int orc_cg(const orc_csr *A, double *x, const double *b, orc_amg *Pl, double tol
     int64_t n = A->n;
     double *r = malloc(sizeof(double) * n), *c = malloc(sizeof(double) * n), *u = calloc(n, sizeof(double));
     orc_spmv(A, x, c);
+#pragma omp parallel for schedule(static) if (n >= ORC_PAR_MIN)
     for (int64_t i = 0; i < n; i++) r[i] = b[i] - c[i];
     double residual = sqrt(dot(n, r, r));
     double nb = sqrt(dot(n, b, b));
@@ -568,9 +581,11 @@ int orc_cg(const orc_csr *A, double *x, const double *b, orc_amg *Pl, double tol
         double rho_prev = rho;
         rho = dot(n, c, r);
         double beta = rho / rho_prev;
+#pragma omp parallel for schedule(static) if (n >= ORC_PAR_MIN)
         for (int64_t i = 0; i < n; i++) u[i] = c[i] + beta * u[i];
         orc_spmv(A, u, c);
         double alpha = rho / dot(n, u, c);
+#pragma omp parallel for schedule(static) if (n >= ORC_PAR_MIN)
         for (int64_t i = 0; i < n; i++) { x[i] += alpha * u[i]; r[i] -= alpha * c[i]; }
         residual = sqrt(dot(n, r, r));
         it++;

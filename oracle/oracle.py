@@ -52,7 +52,7 @@ class CSR(C.Structure):
 def build(verbose: bool = False) -> str:
     """Compile eqoracle.c with gcc into oracle/_build/libeqoracle.so."""
     os.makedirs(os.path.dirname(LIB_PATH), exist_ok=True)
-    cmd = ["gcc", "-O2", "-fPIC", "-shared", "-std=c11", "-o", LIB_PATH,
+    cmd = ["gcc", "-O2", "-fPIC", "-shared", "-std=c11", "-fopenmp", "-o", LIB_PATH,
            os.path.join(HERE, "eqoracle.c"), "-lm"]
     subprocess.run(cmd, check=True, capture_output=not verbose)
     return LIB_PATH
@@ -95,8 +95,15 @@ def lib():
     L.orc_dtevol.argtypes = [C.c_double, C.c_double, f64p, C.c_int64, C.c_int64, f64p, C.c_int]
     L.orc_dtevol.restype = C.c_double
     L.orc_splitmix64.argtypes = [C.c_uint64, C.c_int64, u64p]
+    L.orc_set_threads.argtypes = [C.c_int]
+    L.orc_set_threads(1)
     _lib = L
     return L
+
+
+def set_threads(n: int) -> None:
+    """Threads of the C restatement's OpenMP loops (1 = serial order, the default)."""
+    lib().orc_set_threads(int(n))
 
 
 # ---------------------------------------------------------------------------
