@@ -245,6 +245,17 @@ class EqContext:
                          allow=(L.EQ_NR_NAN, L.EQ_NR_NOCONV))
         return rc, dict(d=d, v=v, a=a, psi=psi, tau=tau, Vf=Vf, FltVfree=Vfree)
 
+    def newmark_step_noalloc(self, d, v, a, psi, dt=None):
+        """newmark_step through eq_newmark_step_noalloc (no device allocation:
+        its scratch is the library's file-scope device array)."""
+        P = self.P
+        d, v, a, psi = (f64(z).copy() for z in (d, v, a, psi))
+        tau, Vf, Vfree = (np.empty(P.FltNglob) for _ in range(3))
+        rc = self._check(self.lib.eq_newmark_step_noalloc(self.h, ptr(d), ptr(v), ptr(a), ptr(psi), ptr(tau),
+                                                          ptr(Vf), ptr(Vfree), P.dt0 if dt is None else dt),
+                         allow=(L.EQ_NR_NAN, L.EQ_NR_NOCONV))
+        return rc, dict(d=d, v=v, a=a, psi=psi, tau=tau, Vf=Vf, FltVfree=Vfree)
+
     def dtevol(self, dt, dtmin, XiLf, FaultNglob, NFBC, Vf, isolver):
         out = C.c_double()
         self._check(self.lib.eq_dtevol(self.h, dt, dtmin, ptr(f64(XiLf)), FaultNglob, NFBC, ptr(f64(Vf)),

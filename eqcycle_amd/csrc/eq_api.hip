@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <memory>
 #include <tuple>
 #include <string>
@@ -89,6 +90,8 @@ struct eq_ctx {
   double *dPre = nullptr, *bq = nullptr;
   FaultState fs{}, fs_api{};
   double* api_nodal = nullptr;        // [6 N] d, v, a in and out of eq_newmark_step
+  int32_t* perm32 = nullptr;          // perm on the device (eq_newmark_step_noalloc)
+  void* scratch_dev = nullptr;        // this device's address of the file-scope scratch array
   // quasi-static pass 2 run ahead of its convergence read-back: its CG batch,
   // the pass's fault update, the finish and the step tail go out together;
   // psi, Vf and the fault row of d before the fault update are kept to redo
@@ -324,6 +327,42 @@ static int download_ni(eq_ctx* ctx, const double* dev, double* x) {
 // ===========================================================================
 // context
 // ===========================================================================
+// ---------------------------------------------------------------------------
+// Scratch of eq_newmark_step_noalloc: one file-scope device array, which each
+// GPU that loads the library holds until the process ends.  Layout: the
+// double-buffered nodal state (6 x kNoallocMaxNodes doubles), the fault
+// staging block (11 x kNoallocMaxFault doubles), the FBC! guard counters.
+// In host code `scratch` names a host-side stand-in; the device address comes
+// from hipGetSymbolAddress, once per device, in scratch_address.
+// ---------------------------------------------------------------------------
+constexpr int64_t kNoallocMaxNodes = 1 << 20;   // 1,048,576 nodes
+constexpr int64_t kNoallocMaxFault = 1 << 13;   // 8,192 fault nodes
+constexpr size_t kNoallocFaultOff = sizeof(double) * 6 * kNoallocMaxNodes;
+constexpr size_t kNoallocFlagsOff = kNoallocFaultOff + sizeof(double) * 11 * kNoallocMaxFault;
+constexpr size_t kNoallocBytes = kNoallocFlagsOff + 256;
+static __device__ __align__(256) unsigned char scratch[kNoallocBytes];
+
+static std::mutex g_scratch_mu;
+static void* g_scratch_addr[64] = {};   // per device ordinal
+
+// The current device's address of `scratch` (looked up on the first call per
+// device, with that device current).
+static int scratch_address(eq_ctx* ctx) {
+  int dev = -1;
+  HIPCHK(ctx, hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return fail(ctx, EQ_HIP_ERR, "device ordinal out of range for the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_addr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(ctx, EQ_HIP_ERR, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_addr[dev] = p;
+  }
+  ctx->scratch_dev = g_scratch_addr[dev];
+  return EQ_OK;
+}
+
 extern "C" int eq_create(const eq_mesh_desc* mesh, eq_ctx** out) {
   if (!out) return EQ_EINVAL;
   *out = nullptr;
@@ -417,6 +456,16 @@ extern "C" int eq_create(const eq_mesh_desc* mesh, eq_ctx** out) {
     *out = ctx.release();
     return fail(*out, EQ_NOMEM, "pinned allocation failed");
   }
+  // the reference -> structured permutation on the device, and the scratch
+  // address of eq_newmark_step_noalloc (which allocates nothing itself)
+  if (N < INT32_MAX) {
+    std::vector<int32_t> p32(c->perm.begin(), c->perm.end());
+    c->perm32 = c->alloc<int32_t>(N);
+    if (!c->perm32) { *out = ctx.release(); return fail(*out, EQ_NOMEM, "device allocation failed"); }
+    if (int rc = h2d(c, c->perm32, p32.data(), N)) { *out = ctx.release(); return rc; }
+    if (int rc = sync_check(c)) { *out = ctx.release(); return rc; }
+  }
+  if (int rc = scratch_address(c)) { *out = ctx.release(); return rc; }
   *out = ctx.release();
   return EQ_OK;
 }
@@ -1528,6 +1577,79 @@ extern "C" int eq_newmark_step(eq_ctx* ctx, double* d, double* v, double* a, dou
   if (int rc = download_ref(ctx, out, d)) return rc;
   if (int rc = download_ref(ctx, out + N, v)) return rc;
   if (int rc = download_ref(ctx, out + 2 * N, a)) return rc;
+  if (int rc = d2h(ctx, psi, f.psi, NX)) return rc;
+  if (tau)
+    if (int rc = d2h(ctx, tau, f.tau, NX)) return rc;
+  if (Vf)
+    if (int rc = d2h(ctx, Vf, f.Vf, NX)) return rc;
+  if (FltVfree)
+    if (int rc = d2h(ctx, FltVfree, f.FltVfree, NX)) return rc;
+  if (int rc = sync_check(ctx)) return rc;
+  if (hf[0] != INT_MAX) {
+    char buf[128];
+    snprintf(buf, sizeof buf, "NR SEARCH FAILED! (fault location %d)", hf[0]);
+    ctx->err = buf;
+    return EQ_NR_NAN;
+  }
+  if (hf[1] > 0) {
+    ctx->err = "NRsearch! hit |delta| > 1e10 or k == 1000 (last iterate returned)";
+    return EQ_NR_NOCONV;
+  }
+  return EQ_OK;
+}
+
+// eq_newmark_step with every device buffer in the file-scope scratch array:
+// no allocation of any kind.  The caller's arrays (reference order) land in
+// the output half of the scratch state, are permuted on the device into the
+// input half, and the result goes back the same way; the fault staging block
+// and the guard counters are cleared first, since the array keeps the last
+// call's contents.  Calls that overlap on one GPU share the array and race.
+extern "C" int eq_newmark_step_noalloc(eq_ctx* ctx, double* d, double* v, double* a, double* psi, double* tau,
+                                       double* Vf, double* FltVfree, double dt) {
+  if (!ctx || !d || !v || !a || !psi) return EQ_EINVAL;
+  ctx->qs_r_valid = false;
+  if (!ctx->have_material || !ctx->have_params) return fail(ctx, EQ_ENOTREADY, "eq_set_material and eq_set_params first");
+  if (!(dt > 0)) return fail(ctx, EQ_EINVAL, "dt must be positive");
+  const int64_t N = ctx->N, NX = ctx->NX;
+  if (N > kNoallocMaxNodes || NX > kNoallocMaxFault) {
+    char buf[200];
+    snprintf(buf, sizeof buf,
+             "mesh of %lld nodes (%lld on the fault) exceeds the allocation-free step's scratch: at most %lld nodes "
+             "and %lld fault nodes",
+             (long long)N, (long long)NX, (long long)kNoallocMaxNodes, (long long)kNoallocMaxFault);
+    return fail(ctx, EQ_EINVAL, buf);
+  }
+  if (!ctx->scratch_dev || !ctx->perm32) return fail(ctx, EQ_ENOTREADY, "scratch address not set up");
+  unsigned char* base = (unsigned char*)ctx->scratch_dev;
+  double* in = (double*)base;
+  double* out = in + 3 * N;
+  double* fblk = (double*)(base + kNoallocFaultOff);
+  int* flags = (int*)(base + kNoallocFlagsOff);
+  FaultState f{};
+  f.psi = fblk; f.psi1 = fblk + NX; f.psi2 = fblk + 2 * NX; f.Vf = fblk + 3 * NX; f.Vf0 = fblk + 4 * NX;
+  f.Vf1 = fblk + 5 * NX; f.Vf2 = fblk + 6 * NX; f.tau = fblk + 7 * NX; f.tau1 = fblk + 8 * NX;
+  f.tau2 = fblk + 9 * NX; f.FltVfree = fblk + 10 * NX;
+  hipStream_t s = ctx->s;
+  launch_fill<double>(11 * NX, fblk, 0.0, s);
+  if (int rc = h2d(ctx, out, d, N)) return rc;
+  if (int rc = h2d(ctx, out + N, v, N)) return rc;
+  if (int rc = h2d(ctx, out + 2 * N, a, N)) return rc;
+  for (int q = 0; q < 3; q++) launch_permute(N, ctx->perm32, out + q * N, in + q * N, 1, s);
+  // nodes deeper than NFBC keep their psi (FBC! does not visit them)
+  if (int rc = h2d(ctx, f.psi, psi, NX)) return rc;
+  if (int rc = h2d(ctx, f.psi2, psi, NX)) return rc;
+  int hf[2] = {INT_MAX, 0};
+  if (int rc = h2d(ctx, flags, hf, 2)) return rc;
+  // as eq_newmark_step: the new d is the predictor (into out), the next
+  // predictor is scratch (into in)
+  launch_newmark_predict(N, in, in + N, in + 2 * N, dt, 0.5 * ctx->dt0 * ctx->dt0, out, s);
+  launch_newmark(ctx->g, ctx->fp, f, ctx->mu, ctx->M, out, in + N, in + 2 * N, in, out + N, out + 2 * N, dt,
+                 0.5 * ctx->dt0, 0.5 * ctx->dt0 * ctx->dt0, flags, NewmarkRecord{}, s);
+  for (int q = 0; q < 3; q++) launch_permute(N, ctx->perm32, out + q * N, in + q * N, 0, s);
+  if (int rc = d2h(ctx, hf, flags, 2)) return rc;
+  if (int rc = d2h(ctx, d, in, N)) return rc;
+  if (int rc = d2h(ctx, v, in + N, N)) return rc;
+  if (int rc = d2h(ctx, a, in + 2 * N, N)) return rc;
   if (int rc = d2h(ctx, psi, f.psi, NX)) return rc;
   if (tau)
     if (int rc = d2h(ctx, tau, f.tau, NX)) return rc;

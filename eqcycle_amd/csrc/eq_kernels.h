@@ -199,6 +199,10 @@ void launch_newmark(const Geom& g, const FaultDev& fp, FaultState fs, const doub
                     const double* P, const double* v, const double* a, double* P2, double* v2, double* a2,
                     double dt, double half_dt, double half_dt_sq, int* flags, const NewmarkRecord& rec,
                     hipStream_t s, LaunchEvents le = {});
+// Reference (MeshBox) order <-> structured order on the device:
+// to_structured: dst[perm[k]] = src[k]; else dst[k] = src[perm[k]]
+void launch_permute(int64_t n, const int32_t* perm, const double* src, double* dst, int to_structured,
+                    hipStream_t s);
 // P = d + dt v + dt^2/2 a on n nodes
 void launch_newmark_predict(int64_t n, const double* d, const double* v, const double* a, double dt,
                             double half_dt_sq, double* P, hipStream_t s);

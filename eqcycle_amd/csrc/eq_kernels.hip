@@ -2111,6 +2111,21 @@ __global__ void newmark_predict_kernel(int64_t n, const double* d, const double*
   const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   if (i < n) P[i] = newmark_predict(d[i], v[i], a[i], dt, half_dt_sq);
 }
+__global__ void permute_kernel(int64_t n, const int32_t* perm, const double* src, double* dst, int to_structured) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  if (to_structured)
+    dst[perm[k]] = src[k];
+  else
+    dst[k] = src[perm[k]];
+}
+
+void launch_permute(int64_t n, const int32_t* perm, const double* src, double* dst, int to_structured,
+                    hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(permute_kernel, dim3(nblk(n)), dim3(BLOCK), 0, s, n, perm, src, dst, to_structured);
+}
+
 void launch_newmark_predict(int64_t n, const double* d, const double* v, const double* a, double dt,
                             double half_dt_sq, double* P, hipStream_t s) {
   if (n <= 0) return;

@@ -32,7 +32,7 @@ EXPORTS = [
     "eq_step", "eq_run", "eq_time_kernel", "eq_learn_init", "eq_learn_record", "eq_learn_info",
     "eq_learn_gram", "eq_learn_set_basis", "eq_learn_train", "eq_learn_loss_grad", "eq_learn_get",
     "eq_learn_predict", "eq_learn_norms", "eq_learn_galerkin", "eq_newmark_step", "eq_amg_upload",
-    "eq_launch_timing", "eq_launch_timing_get",
+    "eq_launch_timing", "eq_launch_timing_get", "eq_newmark_step_noalloc",
 ]
 EQ_TIME_KU, EQ_TIME_CG_DIR, EQ_TIME_VCYCLE, EQ_TIME_CG_ITER, EQ_TIME_NEWMARK, EQ_TIME_CHEB, EQ_TIME_COPY, \
     EQ_TIME_CHEB2 = range(8)
@@ -130,6 +130,7 @@ def load():
         "eq_dtevol": ([vp, C.c_double, C.c_double, dp, C.c_int64, C.c_int64, dp, C.c_int64, dp], C.c_int),
         "eq_set_solver": ([vp, C.POINTER(SolverOpts)], C.c_int),
         "eq_newmark_step": ([vp, dp, dp, dp, dp, dp, dp, dp, C.c_double], C.c_int),
+        "eq_newmark_step_noalloc": ([vp, dp, dp, dp, dp, dp, dp, dp, C.c_double], C.c_int),
         "eq_amg_upload": ([vp, C.c_int, C.POINTER(CsrDesc), C.POINTER(CsrDesc), dp, C.c_int, C.c_int], C.c_int),
         "eq_state_init": ([vp, C.c_double, dp], C.c_int),
         "eq_state_get": ([vp, dp, dp, dp, dp, dp, dp, dp, dp, i64p, i64p], C.c_int),

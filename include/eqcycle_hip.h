@@ -208,6 +208,19 @@ int eq_set_solver(eq_ctx *ctx, const eq_solver_opts *opts);
 int eq_newmark_step(eq_ctx *ctx, double *d, double *v, double *a, double *psi, double *tau, double *Vf,
                     double *FltVfree, double dt);
 
+/* eq_newmark_step for callers that cannot allocate device memory (code that
+ * runs under HIP graph capture, serving loops, harnesses that forbid
+ * allocation).  No counterpart in the reference.  Same arguments, results and
+ * status as eq_newmark_step; every device buffer it needs beyond the context's
+ * own set-up data (the double-buffered nodal state, the fault staging arrays,
+ * the FBC! guard counters) lies in one file-scope device array of 48.7 MiB,
+ * which each GPU that loads the library holds until the process ends.  Limit:
+ * meshes of at most 1,048,576 nodes and 8,192 fault nodes (larger ones return
+ * EQ_EINVAL with the limit in eq_last_error).  The array is shared by every
+ * call on one GPU, so calls that overlap on a device race. */
+int eq_newmark_step_noalloc(eq_ctx *ctx, double *d, double *v, double *a, double *psi, double *tau, double *Vf,
+                            double *FltVfree, double dt);
+
 /* Initial state (src/main.jl:99-190): v = v_init everywhere then -Vpl/2,
  * d = a = 0, psi = psi_init (NULL: the formula of src/main.jl:125). */
 int eq_state_init(eq_ctx *ctx, double v_init, const double *psi_init);
