@@ -20,7 +20,11 @@ What it restates (reference file:line):
 * ``element_ku``         -- element_computation! (src/PCG.jl:71-86), numpy
 
 Pinning: see tests/test_oracle.py (GLL table, closed forms, brute force,
-direct sparse solves, mpmath replays).  Parity unpinned: the AMG C/F split and
+direct sparse solves, mpmath replays of the BigFloat Newton sequence up to
+130 m/s, the Newmark step against the central-difference recurrence on an
+eigenmode and against the equation of motion, slrFunc! against the inverse
+friction law and the exact aging law, the quasi-static step against
+equilibrium, traction and the average-rate rule).  Parity unpinned: the AMG C/F split and
 iteration counts (AlgebraicMultigrid.jl / IterativeSolvers.jl are absent and
 unversioned), and reference full-run trajectories (none are stored).
 """

@@ -296,3 +296,247 @@ def test_quasi_static_step_direct_vs_amg(oracle):
     assert np.max(np.abs(a.d - b.d)) < 1e-9 * np.max(np.abs(b.d))
     assert np.max(np.abs(a.psi - b.psi)) < 1e-10 * np.max(np.abs(b.psi))
     assert a.dt == pytest.approx(b.dt, rel=1e-10)
+
+
+# ---------------------------------------------------------------------------
+# Newmark step (src/main.jl:297-344)
+# ---------------------------------------------------------------------------
+def _damping(P):
+    """Absorbing-edge coefficients C per node (BoundaryMatrix! 'L' and 'T')."""
+    C = np.zeros(P.nglob)
+    np.add.at(C, P.iBcL - 1, P.BcLC)
+    np.add.at(C, P.iBcT - 1, P.BcTC)
+    return C
+
+
+def test_newmark_step_is_central_difference_on_an_eigenmode(oracle):
+    """Every fault node creeping (FBC! visits none, the fault row is held at
+    zero) and no absorbing edges: the step is the central difference
+    M d'' = -K d on the interior.  From d0 = phi, a generalized eigenvector
+    (K_II phi = w^2 M_II phi), v0 = 0 and a0 = -M^-1 K d0, it must give
+    d_n = cos(n theta) phi with cos(theta) = 1 - dt^2 w^2 / 2.  A wrong
+    predictor or corrector coefficient, a dropped a-term or a sign error on
+    K changes theta or breaks the recurrence."""
+    import dataclasses
+
+    import scipy.linalg as sla
+    P = set_parameters(small(4, 3, creep_depth=-1.0, total_years=1))
+    assert P.NFBC - 1 == P.FltNglob
+    P = dataclasses.replace(P, BcLC=np.zeros_like(P.BcLC), BcTC=np.zeros_like(P.BcTC))
+    sim = oracle.Simulation(P, solver="direct")
+    K = sim.K.to_scipy().tocsr()
+    ni = P.FltNI - 1
+    Kii = K[ni][:, ni].toarray()
+    w2, V = sla.eigh(Kii, np.diag(P.M[ni]))
+    for mode in (1, 7):
+        phi = V[:, mode] / np.max(np.abs(V[:, mode]))
+        dt = sim.dtmin
+        cos_t = 1 - 0.5 * dt * dt * w2[mode]
+        assert -1 < cos_t < 1
+        theta = np.arccos(cos_t)
+        d0 = np.zeros(P.nglob)
+        d0[ni] = phi
+        a0 = -(K @ d0) / P.M
+        a0[P.iFlt - 1] = 0.0
+        sim.d, sim.v, sim.a = d0.copy(), np.zeros(P.nglob), a0
+        sim.dt = dt
+        for n in range(1, 121):
+            sim.dyn_step()
+            want = np.cos(n * theta) * phi
+            assert np.max(np.abs(sim.d[ni] - want)) < 1e-11, (mode, n)
+        assert np.all(sim.d[P.iFlt - 1] == 0.0)
+
+
+def test_newmark_step_satisfies_the_equation_of_motion(oracle):
+    """After a step from a state inside an event (absorbing edges and FBC!
+    active): with M0 = M - (dt/2) C the mass without the absorbing-edge
+    terms that par.jl:175-176 fold into M, the new state satisfies
+    M0 a + C v + K d + B tau = 0 at every node the velocity boundary does
+    not hold (B tau: FltB times the traction tau2 - tauo on the fault).  A
+    sign error on the absorbing or fault term, damping taken at the wrong
+    velocity, or M without its boundary terms leaves a residual of the size
+    of those terms."""
+    P = set_parameters(small(32, 8, material="trapezoid", total_years=3))
+    sim = oracle.Simulation(P, tol=1e-12, solver="direct")
+    for _ in range(2000):
+        sim.step()
+        if sim.isolver == 2 and np.max(2 * sim.v[P.iFlt - 1] + P.Vpl) > 0.05:
+            break
+    assert sim.isolver == 2
+    C = _damping(P)
+    M0 = P.M - sim.half_dt * C
+    K = sim.K.to_scipy().tocsr()
+    held = np.zeros(P.nglob, bool)
+    held[P.FltIglobBC - 1] = True
+    for _ in range(5):
+        sim.dyn_step()
+        Btau = np.zeros(P.nglob)
+        Btau[P.iFlt - 1] = P.FltB * sim.tau
+        R = M0 * sim.a + C * sim.v + K @ sim.d + Btau
+        scale = np.max(np.abs(K @ sim.d)) + np.max(np.abs(Btau))
+        assert np.max(np.abs(R[~held])) < 1e-12 * scale
+        assert np.all(sim.v[held] == 0) and np.all(sim.a[held] == 0)
+        assert np.max(np.abs(C * sim.v)) > 1e-6 * scale and np.max(np.abs(Btau)) > 0
+
+
+# ---------------------------------------------------------------------------
+# slrFunc! (src/otherFunctions_serial.jl:62-83)
+# ---------------------------------------------------------------------------
+def test_slr_inverts_the_regularised_friction_law(oracle):
+    """At every searched node: psi1 is the aging-law state after dt at the
+    incoming rate (exact closed form, 60 digits; Taylor branch below
+    VdtL = 1e-6), and the returned Vf1 is the rate of the regularised law at
+    psi1 and tau1 + tauo, i.e. tau1 + tauo = a sigma asinh(Vf1 e^{-h(psi1)}
+    / (2 V0)) with h = -(f0 + b psi1)/a.  Nodes deeper than NFBC keep their
+    inputs.  Using psi instead of psi1, or tau1 without tauo, fails."""
+    import mpmath as mp
+    P = set_parameters(small(16, 4, material="trapezoid", total_years=1))
+    n = P.FltNglob
+    rng = np.random.default_rng(5)
+    psi = np.log(P.Vo / 10 ** rng.uniform(-12, -2, n))
+    Vf = 10 ** rng.uniform(-11, 0, n)
+    tau1 = rng.normal(0, 2e6, n)
+    psi1, Vf1 = np.full(n, -7.0), np.full(n, -7.0)
+    dt = 3600.0
+    oracle.slr(P, P.NFBC, n, psi, psi1, Vf, Vf1, 2, tau1, dt)
+    j0 = P.NFBC - 1
+    assert np.all(psi1[:j0] == -7.0) and np.all(Vf1[:j0] == -7.0)
+    mp.mp.dps = 60
+    for j in range(j0, n):
+        VdtL = abs(Vf[j]) * dt / P.xLf[j]
+        exact = float(mp.log(mp.exp(mp.mpf(psi[j]) - mp.mpf(VdtL)) + (mp.mpf(P.Vo[j]) / abs(Vf[j]))
+                             * (1 - mp.exp(-mp.mpf(VdtL)))))
+        tolpsi = 1e-13 if VdtL >= 1e-6 else 1e-12 + VdtL ** 2
+        assert abs(psi1[j] - exact) <= tolpsi * max(1.0, abs(exact)), j
+        h = -(P.fo[j] + P.ccb[j] * psi1[j]) / P.cca[j]
+        tau_inv = P.cca[j] * P.Seff[j] * np.arcsinh(Vf1[j] * np.exp(-h) / (2 * P.Vo[j]))
+        assert abs(tau_inv - (tau1[j] + P.tauo[j])) <= 1e-9 * abs(P.tauo[j]), j
+
+
+# ---------------------------------------------------------------------------
+# quasi-static step (src/main.jl:226-295)
+# ---------------------------------------------------------------------------
+def test_quasi_static_step_invariants(oracle, monkeypatch):
+    """One quasi-static step (two passes of the fixed point) against what
+    src/main.jl:226-295 fixes, observing the two slrFunc! calls:
+    * equilibrium of the interior, (K d)[FltNI] = 0 to the solve's tolerance;
+    * the on-fault displacement of pass p is dPre + dt (Vf_p - Vpl)/2 with
+      Vf_1 = Vf0 and Vf_2 = (Vf0 + Vf1 of pass 1)/2 (the average-rate rule);
+    * tau = -(K d)[iFlt] / FltB (an independent scipy product), 0 on the
+      creeping nodes;
+    * the final fault velocity is (Vf1 of pass 2 - Vpl)/2 and satisfies the
+      friction law at the final psi and tau;
+    * the interior velocity is (d - dPre)/dt; a = 0; creeping nodes d = v = 0."""
+    P = set_parameters(small(16, 8, material="trapezoid", total_years=1))
+    sim = oracle.Simulation(P, tol=1e-12, solver="direct")
+    for _ in range(30):
+        sim.step()
+    assert sim.isolver == 1
+    calls = []
+    real_slr = oracle.slr
+
+    def spy(P_, NFBC, FltNglob, psi, psi1, Vf, Vf1, IDstate, tau1, dt):
+        out = real_slr(P_, NFBC, FltNglob, psi, psi1, Vf, Vf1, IDstate, tau1, dt)
+        calls.append(dict(Vf_in=Vf.copy(), Vf1=Vf1.copy(), tau1=tau1.copy()))
+        return out
+
+    monkeypatch.setattr(oracle, "slr", spy)
+    iF, ni = P.iFlt - 1, P.FltNI - 1
+    d0, v0, dt = sim.d.copy(), sim.v.copy(), sim.dt
+    Vf0 = 2 * v0[iF] + P.Vpl
+    sim.qs_step()
+    assert len(calls) == 2
+    K = sim.K.to_scipy().tocsr()
+    Kd = K @ sim.d
+    F = np.zeros(P.nglob)
+    F[iF] = sim.d[iF]
+    assert np.linalg.norm(Kd[ni]) <= 1e-10 * np.linalg.norm((K @ F)[ni])
+    creep = np.zeros(P.FltNglob, bool)
+    creep[: P.NFBC - 1] = True
+    Vf1_p1 = calls[0]["Vf1"].copy()
+    Vf1_p1[creep] = P.Vpl
+    Vmid = (Vf0 + Vf1_p1) / 2
+    assert np.allclose(calls[0]["Vf_in"], Vf0, rtol=0, atol=0)
+    assert np.allclose(calls[1]["Vf_in"], Vmid, rtol=1e-15, atol=0)
+    want_dF = d0[iF] + dt * 0.5 * (Vmid - P.Vpl)
+    want_dF[creep] = 0.0
+    assert np.allclose(sim.d[iF], want_dF, rtol=1e-14, atol=1e-20)
+    tau_want = -Kd[iF] / P.FltB
+    tau_want[creep] = 0.0
+    assert np.allclose(sim.tau, tau_want, rtol=1e-12, atol=1e-9 * np.max(np.abs(tau_want)))
+    Vfin = 2 * sim.v[iF] + P.Vpl
+    sr = ~creep
+    assert np.allclose(Vfin[sr], calls[1]["Vf1"][sr], rtol=1e-14, atol=0)
+    h = -(P.fo + P.ccb * sim.psi) / P.cca
+    tau_inv = P.cca * P.Seff * np.arcsinh(Vfin * np.exp(-h) / (2 * P.Vo))
+    assert np.allclose(tau_inv[sr], (sim.tau + P.tauo)[sr], rtol=1e-10, atol=0)
+    assert np.allclose(sim.v[ni], (sim.d[ni] - d0[ni]) / dt, rtol=1e-15, atol=0)
+    assert np.all(sim.a == 0)
+    held = P.FltIglobBC - 1
+    assert np.all(sim.d[held] == 0) and np.all(sim.v[held] == 0)
+
+
+def test_oracle_pins_reject_planted_mistakes(oracle):
+    """The Newmark, slrFunc! and quasi-static pins above fail on plausible
+    mistakes planted in the oracle (here, on the CPU): a predictor
+    coefficient dt^2 instead of dt^2/2, the absorbing-edge term with the
+    wrong sign, the fault traction weighted by 2 FltB, and the friction law
+    evaluated at psi instead of psi1."""
+    import dataclasses
+
+    import scipy.linalg as sla
+    # 1. dt^2 a instead of dt^2/2 a in the predictor breaks the eigenmode recurrence
+    P = set_parameters(small(4, 3, creep_depth=-1.0, total_years=1))
+    P = dataclasses.replace(P, BcLC=np.zeros_like(P.BcLC), BcTC=np.zeros_like(P.BcTC))
+    sim = oracle.Simulation(P, solver="direct")
+    K = sim.K.to_scipy().tocsr()
+    ni = P.FltNI - 1
+    w2, V = sla.eigh(K[ni][:, ni].toarray(), np.diag(P.M[ni]))
+    phi = V[:, 1] / np.max(np.abs(V[:, 1]))
+    theta = np.arccos(1 - 0.5 * sim.dtmin ** 2 * w2[1])
+    d0 = np.zeros(P.nglob)
+    d0[ni] = phi
+    a0 = -(K @ d0) / P.M
+    a0[P.iFlt - 1] = 0.0
+    sim.d, sim.v, sim.a, sim.dt = d0.copy(), np.zeros(P.nglob), a0, sim.dtmin
+    sim.half_dt_sq = sim.dtmin ** 2
+    err = 0.0
+    for n in range(1, 121):
+        sim.dyn_step()
+        err = max(err, np.max(np.abs(sim.d[ni] - np.cos(n * theta) * phi)))
+    assert err > 1e-4
+    # 2. absorbing term with the wrong sign / 3. traction weighted by 2 FltB
+    P = set_parameters(small(32, 8, material="trapezoid", total_years=3))
+    ref = oracle.Simulation(P, tol=1e-12, solver="direct")
+    for _ in range(2000):
+        ref.step()
+        if ref.isolver == 2 and np.max(2 * ref.v[P.iFlt - 1] + P.Vpl) > 0.05:
+            break
+    C = _damping(P)
+    M0 = P.M - ref.half_dt * C
+    held = np.zeros(P.nglob, bool)
+    held[P.FltIglobBC - 1] = True
+    for bad in (dataclasses.replace(P, BcLC=-P.BcLC, BcTC=-P.BcTC),
+                dataclasses.replace(P, FltB=2 * P.FltB)):
+        sim = oracle.Simulation(bad, tol=1e-12, solver="direct")
+        sim.d, sim.v, sim.a, sim.psi, sim.dt = ref.d.copy(), ref.v.copy(), ref.a.copy(), ref.psi.copy(), ref.dt
+        sim.isolver = 2
+        sim.dyn_step()
+        Btau = np.zeros(P.nglob)
+        Btau[P.iFlt - 1] = P.FltB * sim.tau
+        Kd = sim.K.to_scipy().tocsr() @ sim.d
+        R = M0 * sim.a + C * sim.v + Kd + Btau
+        assert np.max(np.abs(R[~held])) > 1e-6 * (np.max(np.abs(Kd)) + np.max(np.abs(Btau)))
+    # 4. the friction law at psi instead of psi1 does not invert slrFunc!'s rates
+    P = set_parameters(small(16, 4, material="trapezoid", total_years=1))
+    n = P.FltNglob
+    rng = np.random.default_rng(5)
+    psi = np.log(P.Vo / 10 ** rng.uniform(-12, -2, n))
+    Vf = 10 ** rng.uniform(-11, 0, n)
+    tau1 = rng.normal(0, 2e6, n)
+    psi1, Vf1 = np.zeros(n), np.zeros(n)
+    oracle.slr(P, P.NFBC, n, psi, psi1, Vf, Vf1, 2, tau1, 3600.0)
+    j = np.arange(P.NFBC - 1, n)
+    h = -(P.fo + P.ccb * psi) / P.cca
+    tau_inv = P.cca * P.Seff * np.arcsinh(Vf1 * np.exp(-h) / (2 * P.Vo))
+    assert np.max(np.abs(tau_inv - (tau1 + P.tauo))[j]) > 1e-3 * np.max(np.abs(P.tauo))
