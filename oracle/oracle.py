@@ -409,8 +409,12 @@ class Simulation:
         if solver == "amg":
             self.ml = Amg(self.kni, **(amg_kw or {}))
         elif solver == "direct":
+            # K_II is symmetric positive definite: a symmetric minimum-degree
+            # ordering without pivoting keeps the LU fill near Cholesky's
+            # (71 M entries instead of 400 M on the 160 x 160 mesh)
             import scipy.sparse.linalg as spla
-            self.lu = spla.splu(self.kni.to_scipy().tocsc())
+            self.lu = spla.splu(self.kni.to_scipy().tocsc(), permc_spec="MMD_AT_PLUS_A", diag_pivot_thresh=0.0,
+                                options=dict(SymmetricMode=True))
         else:
             raise ValueError(solver)
         # state (src/main.jl:88-190)
