@@ -6,8 +6,13 @@ returns ``nothing`` (src/NRsearch_serial.jl:19-32, src/PCG.jl:55-63) the
 mirror returns ``None``; anything else that fails raises :class:`EqError`.
 
     ctx = EqContext(P)                       # P from eqcycle_amd.params.set_parameters
-    psi1, Vf1, tau1, psi2, Vf2, tau2 = FBC(ctx, IDstate, P, NFBC, ...)
+    psi1, Vf1, tau1, psi2, Vf2, tau2 = FBC(IDstate, P, NFBC, ...)   # on the current context
     dnew = cg(ctx, dnew, kni_rhs, tol=1e-6)  # cg!(dnew, kni, rhs; Pl=p, tol)
+
+FBC, NRsearch, IDS, IDS2, slrFunc and dtevol take exactly the reference's
+arguments and run on the current context (the last EqContext created, or the
+one given to use_context); FBC and slrFunc read the fault parameters from
+their P argument on every call.
 """
 from __future__ import annotations
 
@@ -62,6 +67,7 @@ class EqContext:
         self.nglob, self.nI, self.nfault = (x.value for x in n)
         if amg:
             self.amg_setup(**(amg if isinstance(amg, dict) else {}))
+        use_context(self)
 
     def _set_params(self, P):
         arrs = {k: f64(getattr(P, k)) for k in ["fo", "Vo", "xLf", "M", "BcLC", "BcTC", "FltB", "FltZ", "FltX",
@@ -216,20 +222,34 @@ class EqContext:
                     allow=(L.EQ_NR_NOCONV,))
         return Vf, t, k
 
-    def fbc(self, IDstate, NFBC, FltNglob, psi1, Vf1, tau1, psi2, Vf2, tau2, psi, Vf, FltVfree, dt):
+    @staticmethod
+    def _fault_params(P):
+        """eq_fault_params of a params_farray-like object P (None: the context's)."""
+        if P is None:
+            return None, []
+        keep = [f64(getattr(P, k)) for k in ("fo", "Vo", "xLf", "cca", "ccb", "Seff", "tauo", "FltZ")]
+        return C.byref(L.FaultParams(*[ptr(a) for a in keep])), keep
+
+    def fbc(self, IDstate, NFBC, FltNglob, psi1, Vf1, tau1, psi2, Vf2, tau2, psi, Vf, FltVfree, dt, P=None):
+        """FBC! with the fault parameters of P (None: those given at set-up)."""
         outs = [f64(z).copy() for z in (psi1, Vf1, tau1, psi2, Vf2, tau2)]
         ins = [f64(z) for z in (psi, Vf, FltVfree)]
         fail = C.c_int64()
-        rc = self._check(self.lib.eq_fbc(self.h, IDstate, NFBC, FltNglob, *[ptr(z) for z in outs],
+        pp, keep = self._fault_params(P)
+        rc = self._check(self.lib.eq_fbc(self.h, IDstate, pp, NFBC, FltNglob, *[ptr(z) for z in outs],
                                          *[ptr(z) for z in ins], dt, C.byref(fail)),
                          allow=(L.EQ_NR_NAN, L.EQ_NR_NOCONV))
+        del keep
         return rc, fail.value, outs
 
-    def slr(self, NFBC, FltNglob, psi, psi1, Vf, Vf1, IDstate, tau1, dt):
+    def slr(self, NFBC, FltNglob, psi, psi1, Vf, Vf1, IDstate, tau1, dt, P=None):
+        """slrFunc! with the fault parameters of P (None: those given at set-up)."""
         p1 = f64(psi1).copy()
         v1 = f64(Vf1).copy()
-        self._check(self.lib.eq_slr(self.h, NFBC, FltNglob, ptr(f64(psi)), ptr(p1), ptr(f64(Vf)), ptr(v1),
+        pp, keep = self._fault_params(P)
+        self._check(self.lib.eq_slr(self.h, pp, NFBC, FltNglob, ptr(f64(psi)), ptr(p1), ptr(f64(Vf)), ptr(v1),
                                     IDstate, ptr(f64(tau1)), dt))
+        del keep
         return p1, v1
 
     def newmark_step(self, d, v, a, psi, dt=None):
@@ -471,12 +491,31 @@ class StepInfo:
 
 
 # ---------------------------------------------------------------------------
-# reference-named functions (positional signatures of the Julia routines)
+# reference-named functions: the positional signatures of the Julia routines,
+# argument for argument, on the current context (use_context; the last
+# EqContext created otherwise), so that a call site needs no edit
 # ---------------------------------------------------------------------------
-def FBC(ctx: EqContext, IDstate, P, NFBC, FltNglob, psi1, Vf1, tau1, psi2, Vf2, tau2, psi, Vf, FltVfree, dt):
-    """FBC!(IDstate, P, NFBC, FltNglob, psi1, Vf1, tau1, psi2, Vf2, tau2, psi, Vf, FltVfree, dt)
-    (src/NRsearch_serial.jl:6-45).  Returns the 6-tuple, or None when the NaN / Vf guard fires."""
-    rc, fail, outs = ctx.fbc(IDstate, NFBC, FltNglob, psi1, Vf1, tau1, psi2, Vf2, tau2, psi, Vf, FltVfree, dt)
+_current: list = [None]
+
+
+def use_context(ctx: EqContext) -> EqContext:
+    """Make ctx the context the reference-named functions run on."""
+    _current[0] = ctx
+    return ctx
+
+
+def current_context() -> EqContext:
+    if _current[0] is None:
+        raise EqError(L.EQ_ENOTREADY, "no EqContext: create one (or call use_context) first")
+    return _current[0]
+
+
+def FBC(IDstate, P, NFBC, FltNglob, psi1, Vf1, tau1, psi2, Vf2, tau2, psi, Vf, FltVfree, dt):
+    """FBC!(IDstate, P::params_farray, NFBC, FltNglob, psi1, Vf1, tau1, psi2, Vf2, tau2, psi, Vf, FltVfree,
+    dt) (src/NRsearch_serial.jl:6-45), with P's fault parameters.  Returns the 6-tuple, or None when the
+    NaN / Vf guard fires (the reference's @error and bare return)."""
+    rc, fail, outs = current_context().fbc(IDstate, NFBC, FltNglob, psi1, Vf1, tau1, psi2, Vf2, tau2, psi, Vf,
+                                           FltVfree, dt, P=P)
     if rc == L.EQ_NR_NAN:
         log.error("NR SEARCH FAILED! (fault location %d)", fail)
         return None
@@ -486,9 +525,9 @@ def FBC(ctx: EqContext, IDstate, P, NFBC, FltNglob, psi1, Vf1, tau1, psi2, Vf2, 
     return tuple(outs)
 
 
-def NRsearch(ctx: EqContext, fo, Vo, cca, ccb, Seff, tau, tauo, psi, FltZ, FltVfree):
+def NRsearch(fo, Vo, cca, ccb, Seff, tau, tauo, psi, FltZ, FltVfree):
     """NRsearch!(fo, Vo, cca, ccb, Seff, tau, tauo, psi, FltZ, FltVfree) -> (Vf, tau) (src/NRsearch_serial.jl:47-98)."""
-    Vf, t, k = ctx.nrsearch(fo, Vo, cca, ccb, Seff, tau, tauo, psi, FltZ, FltVfree)
+    Vf, t, k = current_context().nrsearch(fo, Vo, cca, ccb, Seff, tau, tauo, psi, FltZ, FltVfree)
     for kk in np.atleast_1d(k)[np.atleast_1d(k) < 0]:
         log.info("k = %d", -kk)
     if np.ndim(fo) == 0:
@@ -496,21 +535,22 @@ def NRsearch(ctx: EqContext, fo, Vo, cca, ccb, Seff, tau, tauo, psi, FltZ, FltVf
     return Vf, t
 
 
-def IDS(ctx: EqContext, xLf, Vo, psi, dt, Vf, cnd, IDstate=2):
+def IDS(xLf, Vo, psi, dt, Vf, cnd, IDstate=2):
     """IDS!(xLf, Vo, psi, dt, Vf, cnd, IDstate) (src/otherFunctions_serial.jl:5-32)."""
-    r = ctx.ids(xLf, Vo, psi, dt, Vf, cnd, IDstate)
+    r = current_context().ids(xLf, Vo, psi, dt, Vf, cnd, IDstate)
     return float(r[0]) if np.ndim(psi) == 0 else r
 
 
-def IDS2(ctx: EqContext, xLf, Vo, psi, psi1, dt, Vf, Vf1, IDstate=2):
+def IDS2(xLf, Vo, psi, psi1, dt, Vf, Vf1, IDstate=2):
     """IDS2!(xLf, Vo, psi, psi1, dt, Vf, Vf1, IDstate) (src/otherFunctions_serial.jl:35-59)."""
-    r = ctx.ids2(xLf, Vo, psi, psi1, dt, Vf, Vf1, IDstate)
+    r = current_context().ids2(xLf, Vo, psi, psi1, dt, Vf, Vf1, IDstate)
     return float(r[0]) if np.ndim(psi) == 0 else r
 
 
-def slrFunc(ctx: EqContext, P, NFBC, FltNglob, psi, psi1, Vf, Vf1, IDstate, tau1, dt):
-    """slrFunc!(P, NFBC, FltNglob, psi, psi1, Vf, Vf1, IDstate, tau1, dt) -> (psi1, Vf1)."""
-    p1, v1 = ctx.slr(NFBC, FltNglob, psi, psi1, Vf, Vf1, IDstate, tau1, dt)
+def slrFunc(P, NFBC, FltNglob, psi, psi1, Vf, Vf1, IDstate, tau1, dt):
+    """slrFunc!(P::params_farray, NFBC, FltNglob, psi, psi1, Vf, Vf1, IDstate, tau1, dt) -> (psi1, Vf1)
+    (src/otherFunctions_serial.jl:62-83), with P's fault parameters."""
+    p1, v1 = current_context().slr(NFBC, FltNglob, psi, psi1, Vf, Vf1, IDstate, tau1, dt, P=P)
     if isinstance(psi1, np.ndarray):
         psi1[:] = p1
     if isinstance(Vf1, np.ndarray):
@@ -518,9 +558,9 @@ def slrFunc(ctx: EqContext, P, NFBC, FltNglob, psi, psi1, Vf, Vf1, IDstate, tau1
     return p1, v1
 
 
-def dtevol(ctx: EqContext, dt, dtmin, XiLf, FaultNglob, NFBC, Vf, isolver):
+def dtevol(dt, dtmin, XiLf, FaultNglob, NFBC, Vf, isolver):
     """dtevol!(dt, dtmin, XiLf, FaultNglob, NFBC, Vf, isolver) -> dt (src/dtevol.jl:5-44)."""
-    return ctx.dtevol(dt, dtmin, XiLf, FaultNglob, NFBC, Vf, isolver)
+    return current_context().dtevol(dt, dtmin, XiLf, FaultNglob, NFBC, Vf, isolver)
 
 
 def cg(ctx: EqContext, x, b, tol, maxiter=0, pc="amg", prec="fp64"):
@@ -540,7 +580,7 @@ def PCG(ctx: EqContext, F, dnew, tol=1e-5, maxit=8000):
     return d
 
 
-def main(P, tol=1e-6, pc="amg", prec="fp64", total_time=None, max_steps=0, amg=None):
+def main(P, *, tol=1e-6, pc="amg", prec="fp64", total_time=None, max_steps=0, amg=None):
     """main(P) (src/main.jl:67-476) on the GPU; returns the results dict."""
     ctx = EqContext(P, amg=(amg or {}) if pc == "amg" else False)
     ctx.set_solver(tol=tol, pc=pc, prec=prec)

@@ -89,6 +89,7 @@ struct eq_ctx {
   double *d = nullptr, *v = nullptr, *a = nullptr, *d2 = nullptr, *v2 = nullptr, *a2 = nullptr;
   double *dPre = nullptr, *bq = nullptr;
   FaultState fs{}, fs_api{};
+  double* fp_api_blk = nullptr;       // [8 NX] a caller's eq_fault_params on the device
   double* api_nodal = nullptr;        // [6 N] d, v, a in and out of eq_newmark_step
   int32_t* perm32 = nullptr;          // perm on the device (eq_newmark_step_noalloc)
   void* scratch_dev = nullptr;        // this device's address of the file-scope scratch array
@@ -655,7 +656,8 @@ extern "C" int eq_set_params(eq_ctx* ctx, const eq_params_desc* p) {
   if (!ctx->fs.psi) {
     ctx->fs = alloc_fault_state(ctx, NX);
     ctx->fs_api = alloc_fault_state(ctx, NX);
-    if (!ctx->fs.psi || !ctx->fs_api.psi) return fail(ctx, EQ_NOMEM, "device allocation failed");
+    ctx->fp_api_blk = ctx->alloc<double>(8 * NX);
+    if (!ctx->fs.psi || !ctx->fs_api.psi || !ctx->fp_api_blk) return fail(ctx, EQ_NOMEM, "device allocation failed");
   }
   ctx->have_params = true;
   return sync_check(ctx);
@@ -1498,9 +1500,34 @@ static int check_fault_call(eq_ctx* ctx, int64_t NFBC, int64_t FltNglob) {
   return EQ_OK;
 }
 
-extern "C" int eq_fbc(eq_ctx* ctx, int64_t IDstate, int64_t NFBC, int64_t FltNglob, double* psi1, double* Vf1,
-                      double* tau1, double* psi2, double* Vf2, double* tau2, const double* psi, const double* Vf,
-                      const double* FltVfree, double dt, int64_t* fail_node) {
+// The FaultDev of a fault-routine call: the context's, with the caller's
+// params_farray arrays (copied to the device) in place of its copies when P
+// is given.  need_z: the routine reads FltZ.
+static int api_fault_params(eq_ctx* ctx, const eq_fault_params* P, bool need_z, FaultDev* fp) {
+  *fp = ctx->fp;
+  if (!P) return EQ_OK;
+  const double* src[8] = {P->fo, P->Vo, P->xLf, P->cca, P->ccb, P->Seff, P->tauo, P->FltZ};
+  for (int k = 0; k < 8; k++)
+    if (!src[k] && (k < 7 || need_z)) return fail(ctx, EQ_EINVAL, "eq_fault_params: missing array");
+  const int64_t n = ctx->FltNglob;
+  double* b = ctx->fp_api_blk;
+  for (int k = 0; k < 8; k++)
+    if (src[k])
+      if (int rc = h2d(ctx, b + k * n, src[k], n)) return rc;
+  fp->fo = b;
+  fp->Vo = b + n;
+  fp->xLf = b + 2 * n;
+  fp->cca = b + 3 * n;
+  fp->ccb = b + 4 * n;
+  fp->Seff = b + 5 * n;
+  fp->tauo = b + 6 * n;
+  if (src[7]) fp->FltZ = b + 7 * n;
+  return EQ_OK;
+}
+
+extern "C" int eq_fbc(eq_ctx* ctx, int64_t IDstate, const eq_fault_params* P, int64_t NFBC, int64_t FltNglob,
+                      double* psi1, double* Vf1, double* tau1, double* psi2, double* Vf2, double* tau2,
+                      const double* psi, const double* Vf, const double* FltVfree, double dt, int64_t* fail_node) {
   if (!ctx || !psi1 || !Vf1 || !tau1 || !psi2 || !Vf2 || !tau2 || !psi || !Vf || !FltVfree) return EQ_EINVAL;
   ctx->qs_r_valid = false;
   if (int rc = check_fault_call(ctx, NFBC, FltNglob)) return rc;
@@ -1515,7 +1542,8 @@ extern "C" int eq_fbc(eq_ctx* ctx, int64_t IDstate, int64_t NFBC, int64_t FltNgl
   if (int rc = h2d(ctx, f.psi, psi, n)) return rc;
   if (int rc = h2d(ctx, f.Vf, Vf, n)) return rc;
   if (int rc = h2d(ctx, f.FltVfree, FltVfree, n)) return rc;
-  FaultDev fp = ctx->fp;
+  FaultDev fp{};
+  if (int rc = api_fault_params(ctx, P, true, &fp)) return rc;
   fp.j0 = (int)(NFBC - 1);
   fp.IDstate = (int)IDstate;
   int hf[2] = {INT_MAX, 0};
@@ -1671,8 +1699,8 @@ extern "C" int eq_newmark_step_noalloc(eq_ctx* ctx, double* d, double* v, double
   return EQ_OK;
 }
 
-extern "C" int eq_slr(eq_ctx* ctx, int64_t NFBC, int64_t FltNglob, const double* psi, double* psi1, const double* Vf,
-                      double* Vf1, int64_t IDstate, const double* tau1, double dt) {
+extern "C" int eq_slr(eq_ctx* ctx, const eq_fault_params* P, int64_t NFBC, int64_t FltNglob, const double* psi,
+                      double* psi1, const double* Vf, double* Vf1, int64_t IDstate, const double* tau1, double dt) {
   if (!ctx || !psi || !psi1 || !Vf || !Vf1 || !tau1) return EQ_EINVAL;
   ctx->qs_r_valid = false;
   if (int rc = check_fault_call(ctx, NFBC, FltNglob)) return rc;
@@ -1683,7 +1711,8 @@ extern "C" int eq_slr(eq_ctx* ctx, int64_t NFBC, int64_t FltNglob, const double*
   if (int rc = h2d(ctx, f.Vf, Vf, n)) return rc;
   if (int rc = h2d(ctx, f.Vf1, Vf1, n)) return rc;
   if (int rc = h2d(ctx, f.tau1, tau1, n)) return rc;
-  FaultDev fp = ctx->fp;
+  FaultDev fp{};
+  if (int rc = api_fault_params(ctx, P, false, &fp)) return rc;
   fp.j0 = (int)(NFBC - 1);
   fp.IDstate = (int)IDstate;
   launch_slr(fp, f, dt, ctx->s);

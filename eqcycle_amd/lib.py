@@ -62,6 +62,11 @@ class ParamsDesc(C.Structure):
                 ("out_seis", i64p), ("n_out_seis", C.c_int64)]
 
 
+class FaultParams(C.Structure):
+    """eq_fault_params: the params_farray fields FBC! and slrFunc! read."""
+    _fields_ = [(k, dp) for k in ("fo", "Vo", "xLf", "cca", "ccb", "Seff", "tauo", "FltZ")]
+
+
 class SolverOpts(C.Structure):
     _fields_ = [("tol", C.c_double), ("maxiter", C.c_int64), ("pc", C.c_int), ("prec", C.c_int),
                 ("guess", C.c_int), ("record_snapshots", C.c_int), ("graphs", C.c_int),
@@ -125,8 +130,10 @@ def load():
         "eq_ids": ([vp, C.c_int64, dp, dp, dp, C.c_double, dp, C.c_double, C.c_int64, dp], C.c_int),
         "eq_ids2": ([vp, C.c_int64, dp, dp, dp, dp, C.c_double, dp, dp, C.c_int64, dp], C.c_int),
         "eq_nrsearch": ([vp, C.c_int64] + [dp] * 12 + [i32p], C.c_int),
-        "eq_fbc": ([vp, C.c_int64, C.c_int64, C.c_int64] + [dp] * 9 + [C.c_double, i64p], C.c_int),
-        "eq_slr": ([vp, C.c_int64, C.c_int64, dp, dp, dp, dp, C.c_int64, dp, C.c_double], C.c_int),
+        "eq_fbc": ([vp, C.c_int64, C.POINTER(FaultParams), C.c_int64, C.c_int64] + [dp] * 9 + [C.c_double, i64p],
+                   C.c_int),
+        "eq_slr": ([vp, C.POINTER(FaultParams), C.c_int64, C.c_int64, dp, dp, dp, dp, C.c_int64, dp, C.c_double],
+                   C.c_int),
         "eq_dtevol": ([vp, C.c_double, C.c_double, dp, C.c_int64, C.c_int64, dp, C.c_int64, dp], C.c_int),
         "eq_set_solver": ([vp, C.POINTER(SolverOpts)], C.c_int),
         "eq_newmark_step": ([vp, dp, dp, dp, dp, dp, dp, dp, C.c_double], C.c_int),

@@ -162,13 +162,23 @@ int eq_nrsearch(eq_ctx *ctx, int64_t n, const double *fo, const double *Vo, cons
                 const double *ccb, const double *Seff, const double *tau, const double *tauo,
                 const double *psi, const double *FltZ, const double *FltVfree, double *Vf_out,
                 double *tau_out, int32_t *k_out);
-/* FBC! (src/NRsearch_serial.jl:6-45) over j = NFBC..FltNglob using the
- * context's params; on the guard returns EQ_NR_NAN and *fail_node = j. */
-int eq_fbc(eq_ctx *ctx, int64_t IDstate, int64_t NFBC, int64_t FltNglob, double *psi1, double *Vf1,
-           double *tau1, double *psi2, double *Vf2, double *tau2, const double *psi, const double *Vf,
+/* The params_farray fields (par.jl:262-289) that FBC! and slrFunc! read,
+ * as the caller holds them (FltNglob values each); the argument P::params_farray
+ * of FBC! and slrFunc! (src/NRsearch_serial.jl:6, src/otherFunctions_serial.jl:62).
+ * slrFunc! does not read FltZ (may be NULL there). */
+typedef struct {
+  const double *fo, *Vo, *xLf, *cca, *ccb, *Seff, *tauo, *FltZ;
+} eq_fault_params;
+/* FBC!(IDstate, P, NFBC, FltNglob, psi1, Vf1, tau1, psi2, Vf2, tau2, psi, Vf,
+ * FltVfree, dt) (src/NRsearch_serial.jl:6-45) over j = NFBC..FltNglob with the
+ * caller's P (copied to the device on each call; NULL: the context's copy
+ * from eq_set_params); on the guard returns EQ_NR_NAN and *fail_node = j. */
+int eq_fbc(eq_ctx *ctx, int64_t IDstate, const eq_fault_params *P, int64_t NFBC, int64_t FltNglob, double *psi1,
+           double *Vf1, double *tau1, double *psi2, double *Vf2, double *tau2, const double *psi, const double *Vf,
            const double *FltVfree, double dt, int64_t *fail_node);
-/* slrFunc! (src/otherFunctions_serial.jl:62-83). */
-int eq_slr(eq_ctx *ctx, int64_t NFBC, int64_t FltNglob, const double *psi, double *psi1,
+/* slrFunc!(P, NFBC, FltNglob, psi, psi1, Vf, Vf1, IDstate, tau1, dt)
+ * (src/otherFunctions_serial.jl:62-83), P as for eq_fbc. */
+int eq_slr(eq_ctx *ctx, const eq_fault_params *P, int64_t NFBC, int64_t FltNglob, const double *psi, double *psi1,
            const double *Vf, double *Vf1, int64_t IDstate, const double *tau1, double dt);
 /* dtevol! (src/dtevol.jl:5-44). */
 int eq_dtevol(eq_ctx *ctx, double dt, double dtmin, const double *XiLf, int64_t FaultNglob, int64_t NFBC,

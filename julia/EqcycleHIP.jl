@@ -14,8 +14,12 @@
 #
 # Usage in run.jl, after `P = setParameters(24e3, 20)`:
 #   include("julia/EqcycleHIP.jl"); using .EqcycleHIP
-#   ctx = EqcycleHIP.Context(P)          # mesh, material W, params_*
-#   O = EqcycleHIP.main(ctx, P; v_init)  # or call the routines one by one
+#   EqcycleHIP.Context(P; W, iglob, ...) # mesh, material W, params_*; becomes the current context
+#   O = EqcycleHIP.main(P)               # the reference's main(P), unchanged call site
+# FBC!, NRsearch!, IDS!, IDS2!, slrFunc!, dtevol! and main(P) take exactly the
+# reference's arguments and run on the current context (use_context!(ctx) to
+# switch); FBC! and slrFunc! read the fault parameters from their P argument.
+# Every routine also has a method with a leading ctx::Context.
 module EqcycleHIP
 
 using LinearAlgebra
@@ -54,11 +58,26 @@ struct ParamsDesc
     FltIglobBC::Ptr{Int64}; n_FltIglobBC::Int64; out_seis::Ptr{Int64}; n_out_seis::Int64
 end
 
+struct FaultParams                     # eq_fault_params: the params_farray fields FBC! / slrFunc! read
+    fo::Ptr{Float64}; Vo::Ptr{Float64}; xLf::Ptr{Float64}; cca::Ptr{Float64}
+    ccb::Ptr{Float64}; Seff::Ptr{Float64}; tauo::Ptr{Float64}; FltZ::Ptr{Float64}
+end
+FaultParams(P) = FaultParams(pointer(P.fo), pointer(P.Vo), pointer(P.xLf), pointer(P.cca), pointer(P.ccb),
+                             pointer(P.Seff), pointer(P.tauo), pointer(P.FltZ))
+
 mutable struct Context
     h::Ptr{Cvoid}
     nglob::Int
     nI::Int
     keep::Vector{Any}          # arrays whose pointers the library may hold during set-up
+end
+
+# the context the reference-signature methods run on (the last one created)
+const CURRENT = Ref{Union{Context,Nothing}}(nothing)
+use_context!(ctx::Context) = (CURRENT[] = ctx)
+function current()
+    CURRENT[] === nothing && error("EqcycleHIP: create a Context first")
+    return CURRENT[]
 end
 
 lasterr(ctx::Context) = unsafe_string(ccall((:eq_last_error, LIB), Cstring, (Ptr{Cvoid},), ctx.h))
@@ -88,7 +107,7 @@ function Context(P; W, iglob, NelX, NelY, dxe, dye, xgll, wgll, H, amg = true)
     amg && check(ctx, ccall((:eq_amg_setup, LIB), Cint,
                            (Ptr{Cvoid}, Cdouble, Cint, Int64, Cint, Cint, Cint, Cint),
                            ctx.h, 0.25, 10, 2000, 1, 1, 5, 1))
-    return ctx
+    return use_context!(ctx)
 end
 
 function set_params!(ctx::Context, P)
@@ -188,10 +207,12 @@ end
 # ---- fault routines (same positional signatures as the reference) -------------
 function FBC!(ctx::Context, IDstate, P, NFBC, FltNglob, psi1, Vf1, tau1, psi2, Vf2, tau2, psi, Vf, FltVfree, dt)
     fail = Ref{Int64}(0)
-    rc = ccall((:eq_fbc, LIB), Cint,
-               (Ptr{Cvoid}, Int64, Int64, Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
-                Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Cdouble, Ref{Int64}),
-               ctx.h, IDstate, NFBC, FltNglob, psi1, Vf1, tau1, psi2, Vf2, tau2, psi, Vf, FltVfree, dt, fail)
+    rc = GC.@preserve P ccall((:eq_fbc, LIB), Cint,
+               (Ptr{Cvoid}, Int64, Ref{FaultParams}, Int64, Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+                Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Cdouble,
+                Ref{Int64}),
+               ctx.h, IDstate, FaultParams(P), NFBC, FltNglob, psi1, Vf1, tau1, psi2, Vf2, tau2, psi, Vf,
+               FltVfree, dt, fail)
     if rc == EQ_NR_NAN
         println("Fault Location = ", fail[])
         @error("NR SEARCH FAILED!")
@@ -213,7 +234,7 @@ function NRsearch!(ctx::Context, fo, Vo, cca, ccb, Seff, tau, tauo, psi, FltZ, F
     return Vf[], t[]
 end
 
-function IDS!(ctx::Context, xLf, Vo, psi, dt, Vf, cnd, IDstate = 2)
+function IDS!(ctx::Context, xLf, Vo, psi, dt, Vf, cnd, IDstate)
     out = Ref{Float64}(0)
     check(ctx, ccall((:eq_ids, LIB), Cint, (Ptr{Cvoid}, Int64, Ref{Float64}, Ref{Float64}, Ref{Float64}, Cdouble,
                                             Ref{Float64}, Cdouble, Int64, Ref{Float64}),
@@ -221,7 +242,7 @@ function IDS!(ctx::Context, xLf, Vo, psi, dt, Vf, cnd, IDstate = 2)
     out[]
 end
 
-function IDS2!(ctx::Context, xLf, Vo, psi, psi1, dt, Vf, Vf1, IDstate = 2)
+function IDS2!(ctx::Context, xLf, Vo, psi, psi1, dt, Vf, Vf1, IDstate)
     out = Ref{Float64}(0)
     check(ctx, ccall((:eq_ids2, LIB), Cint, (Ptr{Cvoid}, Int64, Ref{Float64}, Ref{Float64}, Ref{Float64},
                                              Ref{Float64}, Cdouble, Ref{Float64}, Ref{Float64}, Int64, Ref{Float64}),
@@ -230,9 +251,10 @@ function IDS2!(ctx::Context, xLf, Vo, psi, psi1, dt, Vf, Vf1, IDstate = 2)
 end
 
 function slrFunc!(ctx::Context, P, NFBC, FltNglob, psi, psi1, Vf, Vf1, IDstate, tau1, dt)
-    check(ctx, ccall((:eq_slr, LIB), Cint, (Ptr{Cvoid}, Int64, Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
-                                            Ptr{Float64}, Int64, Ptr{Float64}, Cdouble),
-                     ctx.h, NFBC, FltNglob, psi, psi1, Vf, Vf1, IDstate, tau1, dt))
+    GC.@preserve P check(ctx, ccall((:eq_slr, LIB), Cint,
+                                    (Ptr{Cvoid}, Ref{FaultParams}, Int64, Int64, Ptr{Float64}, Ptr{Float64},
+                                     Ptr{Float64}, Ptr{Float64}, Int64, Ptr{Float64}, Cdouble),
+                                    ctx.h, FaultParams(P), NFBC, FltNglob, psi, psi1, Vf, Vf1, IDstate, tau1, dt))
     return psi1, Vf1
 end
 
@@ -243,6 +265,19 @@ function dtevol!(ctx::Context, dt, dtmin, XiLf, FaultNglob, NFBC, Vf, isolver)
                      ctx.h, dt, dtmin, XiLf, FaultNglob, NFBC, Vf, isolver, out))
     out[]
 end
+
+# The reference's own signatures, on the current context: call sites of
+# src/main.jl and src/NRsearch_serial.jl need no edit.
+FBC!(IDstate, P, NFBC, FltNglob, psi1, Vf1, tau1, psi2, Vf2, tau2, psi, Vf, FltVfree, dt) =
+    FBC!(current(), IDstate, P, NFBC, FltNglob, psi1, Vf1, tau1, psi2, Vf2, tau2, psi, Vf, FltVfree, dt)
+NRsearch!(fo, Vo, cca, ccb, Seff, tau, tauo, psi, FltZ, FltVfree) =
+    NRsearch!(current(), fo, Vo, cca, ccb, Seff, tau, tauo, psi, FltZ, FltVfree)
+IDS!(xLf, Vo, psi, dt, Vf, cnd, IDstate = 2) = IDS!(current(), xLf, Vo, psi, dt, Vf, cnd, IDstate)
+IDS2!(xLf, Vo, psi, psi1, dt, Vf, Vf1, IDstate = 2) = IDS2!(current(), xLf, Vo, psi, psi1, dt, Vf, Vf1, IDstate)
+slrFunc!(P, NFBC, FltNglob, psi, psi1, Vf, Vf1, IDstate, tau1, dt) =
+    slrFunc!(current(), P, NFBC, FltNglob, psi, psi1, Vf, Vf1, IDstate, tau1, dt)
+dtevol!(dt, dtmin, XiLf, FaultNglob, NFBC, Vf, isolver) =
+    dtevol!(current(), dt, dtmin, XiLf, FaultNglob, NFBC, Vf, isolver)
 
 # The dynamic branch of main.jl's loop body (src/main.jl:297-344) in one call:
 # d, v, a (nglob) and psi (FltNglob) are advanced in place; tau and Vf receive
@@ -381,5 +416,10 @@ function main(ctx::Context, P; v_init, psi_init = nothing, total_time = P[2].Tot
             taubefore = taub[:, 1:nst], tauafter = taua[:, 1:ne], delfafter = delf[:, 1:ne],
             hypo = hypo[1:nst], time_ = time_[1:ns], Vfmax = Vfmax[1:ns])
 end
+
+"""main(P): src/main.jl:67-476 with the reference's call signature, on the
+current context.  The initial velocity is the reference's v .= 0.5e-3
+(src/main.jl:101-102) and psi its formula (:125)."""
+main(P) = main(current(), P; v_init = 0.5e-3)
 
 end # module

@@ -405,8 +405,46 @@ def test_fbc_guard_returns_nothing(mesh):
     n = P.FltNglob
     Vf = Vf.copy()
     Vf[P.NFBC + 2] = np.nan
-    out = FBC(ctx, 2, P, P.NFBC, n, *[np.zeros(n) for _ in range(6)], psi, Vf, np.full(n, 1e-9), 0.01)
+    from eqcycle_amd.api import use_context
+    use_context(ctx)
+    out = FBC(2, P, P.NFBC, n, *[np.zeros(n) for _ in range(6)], psi, Vf, np.full(n, 1e-9), 0.01)
     assert out is None
+
+
+def test_fault_routines_read_the_callers_params(mesh, oracle):
+    """FBC!(IDstate, P, ...) and slrFunc!(P, ...) with the reference's
+    arguments read the caller's P (src/NRsearch_serial.jl:6,
+    src/otherFunctions_serial.jl:62): a P whose tauo, cca and FltZ differ
+    from the context's gives the oracle's result for that P, not the one
+    for the parameters given at set-up."""
+    import dataclasses
+
+    from eqcycle_amd.api import FBC, slrFunc, use_context
+    P, ctx, _, _ = mesh
+    use_context(ctx)
+    psi, Vf, rng = _fault_inputs(P, 21)
+    n = P.FltNglob
+    Vfree = 10 ** rng.uniform(-10, 0, n)
+    P2 = dataclasses.replace(P, tauo=P.tauo * 1.05, cca=P.cca * 1.1, FltZ=P.FltZ * 0.8)
+    dt = 0.01
+    want = [np.zeros(n) for _ in range(6)]
+    assert oracle.fbc(2, P2, P2.NFBC, n, *want, psi, Vf, Vfree, dt)[0] == 0
+    got = FBC(2, P2, P.NFBC, n, *[np.zeros(n) for _ in range(6)], psi, Vf, Vfree, dt)
+    for a, b in zip(got, want):
+        assert rel(a, b) < 1e-11
+    old = [np.zeros(n) for _ in range(6)]
+    oracle.fbc(2, P, P.NFBC, n, *old, psi, Vf, Vfree, dt)
+    assert rel(got[2], old[2]) > 1e-3
+    tau1 = rng.normal(0, 1e6, n)
+    p1o, v1o = np.zeros(n), np.zeros(n)
+    oracle.slr(P2, P2.NFBC, n, psi, p1o, Vf, v1o, 2, tau1, 3600.0)
+    p1g, v1g = slrFunc(P2, P.NFBC, n, psi, np.zeros(n), Vf, np.zeros(n), 2, tau1, 3600.0)
+    assert rel(p1g, p1o) < 1e-13 and rel(v1g, v1o) < 1e-11
+    # the parameters given at set-up are untouched for calls without P
+    p1c, v1c = ctx.slr(P.NFBC, n, psi, np.zeros(n), Vf, np.zeros(n), 2, tau1, 3600.0)
+    p1r, v1r = np.zeros(n), np.zeros(n)
+    oracle.slr(P, P.NFBC, n, psi, p1r, Vf, v1r, 2, tau1, 3600.0)
+    assert rel(v1c, v1r) < 1e-11
 
 
 def test_dtevol(mesh, oracle):

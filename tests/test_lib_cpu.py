@@ -87,5 +87,45 @@ def test_julia_binding_matches_header():
     assert called and called <= set(L.EXPORTS), called - set(L.EXPORTS)
     for c_name, jl_name in (("eq_mesh_desc", "MeshDesc"), ("eq_params_desc", "ParamsDesc"), ("eq_csr", "EqCsr"),
                             ("eq_solver_opts", "SolverOpts"), ("eq_step_info", "StepInfo"),
-                            ("eq_results", "Results")):
+                            ("eq_results", "Results"), ("eq_fault_params", "FaultParams")):
         assert _jl_struct_fields(jl, jl_name) == _c_struct_fields(hdr, c_name), c_name
+
+
+# The reference routines' signatures (name, positional arguments, defaults):
+# src/NRsearch_serial.jl:6,47, src/otherFunctions_serial.jl:5,35,62,
+# src/dtevol.jl:5, src/main.jl:67.
+REFERENCE_SIGNATURES = {
+    "FBC!": ("IDstate", "P", "NFBC", "FltNglob", "psi1", "Vf1", "tau1", "psi2", "Vf2", "tau2", "psi", "Vf",
+             "FltVfree", "dt"),
+    "NRsearch!": ("fo", "Vo", "cca", "ccb", "Seff", "tau", "tauo", "psi", "FltZ", "FltVfree"),
+    "IDS!": ("xLf", "Vo", "psi", "dt", "Vf", "cnd", "IDstate = 2"),
+    "IDS2!": ("xLf", "Vo", "psi", "psi1", "dt", "Vf", "Vf1", "IDstate = 2"),
+    "slrFunc!": ("P", "NFBC", "FltNglob", "psi", "psi1", "Vf", "Vf1", "IDstate", "tau1", "dt"),
+    "dtevol!": ("dt", "dtmin", "XiLf", "FaultNglob", "NFBC", "Vf", "isolver"),
+    "main": ("P",),
+}
+
+
+def test_julia_binding_has_the_reference_signatures():
+    """Each routine has a method with the reference's positional arguments
+    (same names and order, same defaults), so a call site needs no edit."""
+    jl = open(os.path.join(ROOT, "julia", "EqcycleHIP.jl")).read()
+    for name, args in REFERENCE_SIGNATURES.items():
+        pat = r"^" + re.escape(name) + r"\(([^)]*)\)\s*="
+        found = [tuple(a.strip() for a in m.split(",")) for m in re.findall(pat, jl, flags=re.M)]
+        assert args in found, (name, found)
+
+
+def test_python_mirror_has_the_reference_signatures():
+    import inspect
+
+    from eqcycle_amd import api
+    for name, args in REFERENCE_SIGNATURES.items():
+        f = getattr(api, name.rstrip("!"))
+        got = []
+        for p in inspect.signature(f).parameters.values():
+            got.append(p.name if p.default is inspect.Parameter.empty else f"{p.name} = {p.default}")
+        if name == "main":
+            got = got[:1]     # main(P) plus keyword-only extras of the device loop
+        assert tuple(got) == args, (name, got)
+
