@@ -23,6 +23,8 @@ host synchronisation, and the timed region is the sum of the K loop runs.
             cycle, c the Galerkin (K_II-norm optimal) coefficients, both passes
   network : FP64 CG, FP32 V-cycle, pass-1 start from the conv1d + MLP network's
             predicted POD coefficients with an energy-optimal step
+  linear  : as network, with the coefficients from the linear least-squares
+            map of the same input (the bar the network has to beat)
   (snapshot recording, POD and network training run beforehand, untimed and
   reported)
 
@@ -123,6 +125,9 @@ def checkpoint(ctx, cfg, tol, learned, epochs, graphs):
         info.update(snapshots=cfg.n_snapshots, used=fit["used"], pod_s=t_pod,
                     energy_captured=fit["energy_captured"], train_s=time.perf_counter() - t1, epochs=epochs,
                     loss_first=float(hist[0]), loss_last=float(hist[-1]), nparams=ctx.learn_info()["nparams"])
+        t1 = time.perf_counter()
+        lin = L.fit_linear(ctx, fit["use"])
+        info.update(linear_fit_s=time.perf_counter() - t1, linear_train_rel_residual=lin["train_rel_residual"])
     return s, info
 
 
@@ -327,9 +332,12 @@ def main():
         extra["training"] = training
         lrn, s_lrn, R_lrn = variant(ctx, ck, tol, cfg.precision, "learned", a.warmup, a.steps, a.graphs)
         print(f"network: {lrn['sim_years_per_hour']:.0f}", file=sys.stderr, flush=True)
+        lin, s_lin, R_lin = variant(ctx, ck, tol, cfg.precision, "linear", a.warmup, a.steps, a.graphs)
+        print(f"linear: {lin['sim_years_per_hour']:.0f}", file=sys.stderr, flush=True)
         gal, s_gal, R_gal = variant(ctx, ck, tol, cfg.precision, "galerkin", a.warmup, a.steps, a.graphs)
         print(f"galerkin: {gal['sim_years_per_hour']:.0f}", file=sys.stderr, flush=True)
         extra["network_f32"] = lrn
+        extra["linear_f32"] = lin
         extra["galerkin_f32"] = gal
         main_st, prec, guess = gal, cfg.precision, "galerkin"
 
@@ -345,6 +353,7 @@ def main():
             }
         extra["galerkin_vs_plain"] = versus(gal, s_gal, R_gal)
         extra["network_vs_plain"] = versus(lrn, s_lrn, R_lrn)
+        extra["linear_vs_plain"] = versus(lin, s_lin, R_lin)
     roof, ktable, w_acc = kernel_time(ctx, ck, tol, prec, guess, P, main_st["wall_per_pass_s"])
     roof["newmark_isolated_us"] = ctx.time_kernel("newmark", "fp64", reps=200) * 1e3
     print(f"dominant kernel {roof['kernel']}: {roof['frac']:.3f} of HBM peak", file=sys.stderr, flush=True)
@@ -362,7 +371,8 @@ def main():
                                 f"(years {ck['t'] / YR:.1f} to {ck['t'] / YR + SEGMENT_YEARS:.1f} from the plain "
                                 f"solver's checkpoint, third event cluster inside); "
                                 + ("value: x0 from the learned POD basis (Galerkin coefficients) + FP32 V-cycle, "
-                                   "plain FP64 and network-predicted x0 alongside" if learned else "plain FP64")),
+                                   "plain FP64, network-predicted and linear-predicted x0 alongside" if learned
+                                   else "plain FP64")),
                    "pcg_iters_per_solve": main_st["pcg_iters_per_solve"], "hierarchy": a.hierarchy,
                    "graphs": bool(a.graphs), "amg_levels": ctx.amg_info(), **extra,
                    "kernel_time": {"variant": f"{guess}/{prec}", "pass_wall_s_with_events": w_acc,

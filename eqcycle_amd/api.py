@@ -292,7 +292,8 @@ class EqContext:
         solve of the same x projects only the change of b (rows 1..4)."""
         o = L.SolverOpts(tol, maxiter, PC[pc], PREC[prec],
                          {"previous": L.EQ_GUESS_PREVIOUS, "learned": L.EQ_GUESS_LEARNED,
-                          "galerkin": L.EQ_GUESS_GALERKIN}[guess], 1, 1 if graphs else 0,
+                          "galerkin": L.EQ_GUESS_GALERKIN, "linear": L.EQ_GUESS_LINEAR}[guess], 1,
+                         1 if graphs else 0,
                          0 if cheb_pairs else 1, 0 if early_tail else 1, 0 if galerkin_shortcut else 1)
         self._check(self.lib.eq_set_solver(self.h, C.byref(o)))
 
@@ -446,6 +447,19 @@ class EqContext:
         dF = f64(dF)
         out = np.empty(self.nbasis)
         self._check(self.lib.eq_learn_predict(self.h, ptr(dF), ptr(out)))
+        return out
+
+    def learn_set_linear(self, A, b):
+        """The linear predictor coef = |dF| (A dF/|dF| + b), A [nbasis x FltNglob]."""
+        A = np.ascontiguousarray(A, dtype=np.float32)
+        b = np.ascontiguousarray(b, dtype=np.float32)
+        assert A.shape == (self.nbasis, self.P.FltNglob) and b.shape == (self.nbasis,)
+        self._check(self.lib.eq_learn_set_linear(self.h, ptr(A, C.c_float), ptr(b, C.c_float)))
+
+    def learn_predict_linear(self, dF):
+        dF = f64(dF)
+        out = np.empty(self.nbasis)
+        self._check(self.lib.eq_learn_predict_linear(self.h, ptr(dF), ptr(out)))
         return out
 
     def learn_galerkin(self, x, b):

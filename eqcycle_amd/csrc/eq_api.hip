@@ -1743,10 +1743,13 @@ extern "C" int eq_set_solver(eq_ctx* ctx, const eq_solver_opts* o) {
   ctx->qs_r_valid = false;
   if (o->pc < EQ_PC_NONE || o->pc > EQ_PC_AMG) return fail(ctx, EQ_EINVAL, "bad pc");
   if (o->prec < EQ_PREC_FP64 || o->prec > EQ_PREC_F32PC) return fail(ctx, EQ_EINVAL, "bad prec");
-  if (o->guess != EQ_GUESS_PREVIOUS && o->guess != EQ_GUESS_LEARNED && o->guess != EQ_GUESS_GALERKIN)
+  if (o->guess != EQ_GUESS_PREVIOUS && o->guess != EQ_GUESS_LEARNED && o->guess != EQ_GUESS_GALERKIN &&
+      o->guess != EQ_GUESS_LINEAR)
     return fail(ctx, EQ_EINVAL, "bad guess");
   if (o->guess == EQ_GUESS_LEARNED && !ctx->learn_on)
     return fail(ctx, EQ_ENOTREADY, "learned guess: basis and network not trained (eq_learn_train)");
+  if (o->guess == EQ_GUESS_LINEAR && !ctx->learn.lin_ready)
+    return fail(ctx, EQ_ENOTREADY, "linear guess: no linear predictor (eq_learn_set_linear)");
   if (o->guess == EQ_GUESS_GALERKIN && !ctx->learn.gal_ready)
     return fail(ctx, EQ_ENOTREADY, "Galerkin guess: no POD basis (eq_learn_set_basis)");
   if (o->cheb_single != ctx->opts.cheb_single) {
@@ -1895,13 +1898,16 @@ static void clean_flags(eq_ctx* ctx) {
 // fault row = new fault data) += gamma p with p = |dF| Phi net(dF / |dF|)
 // and gamma the energy-optimal step along p, so the guess is never worse (in
 // the K_II norm) than the previous-step guess it starts from.
-static void learned_correction(eq_ctx* ctx, double* x, const double* b) {
+static void learned_correction(eq_ctx* ctx, double* x, const double* b, bool linear) {
   CgWork<double>& w = ctx->w64;
   const Geom& g = ctx->g;
   hipStream_t s = ctx->s;
   const int nt = tile_count(g);
   launch_cg_residual<double>(g, x, ctx->mu, b, w.r, ctx->part_rr, ctx->part_bb, ctx->npart, s);
-  learn_guess(&ctx->learn, x, ctx->dPre, ctx->gp, s);
+  if (linear)
+    learn_guess_linear(&ctx->learn, x, ctx->dPre, ctx->gp, s);
+  else
+    learn_guess(&ctx->learn, x, ctx->dPre, ctx->gp, s);
   launch_guess_dot(g, ctx->gp, ctx->mu, w.r, ctx->gq, ctx->part_rho, ctx->part_uq, s);
   launch_guess_update(ctx->N, x, w.r, ctx->gp, ctx->gq, ctx->part_rho, ctx->part_uq, nt, ctx->part_rr, ctx->npart,
                       s);
@@ -1940,7 +1946,8 @@ static int qs_step(eq_ctx* ctx, eq_step_info* info, bool r_valid) {
     launch_qs_begin(g, ctx->fp, ctx->fs, ctx->mu, ctx->d, ctx->dPre, ctx->v, ctx->bq, dt, pass == 0, s,
                     upd ? ctx->gal_db : nullptr);
     const bool learned = pass == 0 && ctx->opts.guess == EQ_GUESS_LEARNED && ctx->learn_on;
-    if (learned) learned_correction(ctx, ctx->d, ctx->bq);
+    const bool linear = pass == 0 && ctx->opts.guess == EQ_GUESS_LINEAR && ctx->learn.lin_ready;
+    if (learned || linear) learned_correction(ctx, ctx->d, ctx->bq, linear);
     if (galerkin) galerkin_correction(ctx, ctx->d, ctx->bq, upd ? ctx->gal_db : nullptr);
     int64_t it = 0;
     double rr = 0;
@@ -2708,7 +2715,27 @@ extern "C" int eq_learn_predict(eq_ctx* ctx, const double* dF, double* coef) {
   ctx->qs_r_valid = false;
   LEARN_READY(ctx);
   std::string err;
-  if (learn_predict(&ctx->learn, dF, coef, ctx->s, &err) != 0) return fail(ctx, EQ_HIP_ERR, err);
+  if (learn_predict(&ctx->learn, dF, coef, 0, ctx->s, &err) != 0) return fail(ctx, EQ_HIP_ERR, err);
+  return EQ_OK;
+}
+
+extern "C" int eq_learn_set_linear(eq_ctx* ctx, const float* A, const float* b) {
+  if (!ctx || !A || !b) return EQ_EINVAL;
+  ctx->qs_r_valid = false;
+  LEARN_READY(ctx);
+  if (!ctx->learn.nbasis_ready) return fail(ctx, EQ_ENOTREADY, "linear predictor: no POD basis (eq_learn_set_basis)");
+  std::string err;
+  if (learn_set_linear(&ctx->learn, A, b, ctx->s, &err) != 0) return fail(ctx, EQ_NOMEM, err);
+  return EQ_OK;
+}
+
+extern "C" int eq_learn_predict_linear(eq_ctx* ctx, const double* dF, double* coef) {
+  if (!ctx || !dF || !coef) return EQ_EINVAL;
+  ctx->qs_r_valid = false;
+  LEARN_READY(ctx);
+  if (!ctx->learn.lin_ready) return fail(ctx, EQ_ENOTREADY, "eq_learn_set_linear first");
+  std::string err;
+  if (learn_predict(&ctx->learn, dF, coef, 1, ctx->s, &err) != 0) return fail(ctx, EQ_HIP_ERR, err);
   return EQ_OK;
 }
 

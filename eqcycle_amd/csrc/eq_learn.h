@@ -65,6 +65,11 @@ struct LearnState {
   int64_t gal_trows = 0;
   int gal_nblk = 0;
   int gal_ready = 0;
+  // linear least-squares predictor of the same coefficients from the same
+  // input: coef = |dF| (A dF/|dF| + b)
+  float* lin_A = nullptr;       // [NB x L0] row-major
+  float* lin_b = nullptr;       // [NB]
+  int lin_ready = 0;
 };
 
 int learn_init(LearnState* ls, int64_t nI, int64_t nflt, int64_t offset, int cap, int C, int H, int NB,
@@ -86,10 +91,15 @@ int learn_loss_grad(LearnState* ls, const float* params, int nsamp, const int32_
                     float* grad_host, hipStream_t s, std::string* err);
 // p[off:] = |dF| Phi net(dF / |dF|), p[0:off] = 0, with dF = x[0:nflt] - base[0:nflt]
 void learn_guess(LearnState* ls, const double* x, const double* base, double* p, hipStream_t s);
+// the same with the linear predictor's coefficients
+void learn_guess_linear(LearnState* ls, const double* x, const double* base, double* p, hipStream_t s);
+// upload the linear predictor (A [NB x L0] row-major, b [NB])
+int learn_set_linear(LearnState* ls, const float* A_host, const float* b_host, hipStream_t s, std::string* err);
 int learn_get_basis(LearnState* ls, float* Phi_host, hipStream_t s);
 int learn_get_params(LearnState* ls, float* out, hipStream_t s);
-// coefficients for one input (test entry point)
-int learn_predict(LearnState* ls, const double* dF_host, double* coef_host, hipStream_t s, std::string* err);
+// coefficients for one input (test entry point); linear: the linear predictor's
+int learn_predict(LearnState* ls, const double* dF_host, double* coef_host, int linear, hipStream_t s,
+                  std::string* err);
 
 // Galerkin start.  learn_project: gal_c = Phi^T r[off:] (solve = 0) or
 // gal_c = Ginv Phi^T r[off:] (solve = 1), FP64 accumulation in a fixed order.

@@ -336,6 +336,64 @@ __global__ void to_bf16_kernel(int64_t n, const float* w, uint16_t* w16) {
   if (i < n) w16[i] = f2bf(w[i]);
 }
 
+// ---- inference: one workgroup per launch -----------------------------------
+constexpr int IB = 1024;   // threads of an inference workgroup
+
+// a0[i] = (x[i] - base[i]) / |x - base| over the fault; returns |x - base|
+// (every thread) and leaves it in coef[NB]
+__device__ double infer_input(const double* x, const double* base, int64_t nflt, float* a0, float* coef, int NB) {
+  __shared__ double sred[IB / 64];
+  __shared__ double s_n;
+  double ss = 0.0;
+  for (int64_t i = threadIdx.x; i < nflt; i += IB) {
+    const double v = x[i] - base[i];
+    ss += v * v;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) ss += __shfl_down(ss, off, 64);
+  if ((threadIdx.x & 63) == 0) sred[threadIdx.x >> 6] = ss;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int k = 0; k < IB / 64; k++) t += sred[k];
+    s_n = sqrt(t);
+    coef[NB] = (float)s_n;
+  }
+  __syncthreads();
+  const double nrm = s_n;
+  const double inv = nrm > 0 ? 1.0 / nrm : 0.0;
+  for (int64_t i = threadIdx.x; i < nflt; i += IB) a0[i] = (float)((x[i] - base[i]) * inv);
+  __syncthreads();
+  return nrm;
+}
+
+// out[row] = act(bias[row] + W[row, :] . in) (times scale when scaled), one
+// wave per row, lanes striding the row, as a shuffle-reduced sum
+template <typename WT>
+__device__ void rows_gemv(int N, int K, const float* in, const WT* W, const float* bias, float* out, int relu,
+                          bool scaled, float scale) {
+  const int lane = threadIdx.x & 63;
+  for (int row = threadIdx.x >> 6; row < N; row += IB / 64) {
+    float acc = 0.f;
+    for (int k = lane; k < K; k += 64) acc += ldw(W, (int64_t)row * K + k) * in[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    if (lane == 0) {
+      float v = acc + bias[row];
+      if (relu) v = fmaxf(v, 0.f);
+      if (scaled) v *= scale;
+      out[row] = v;
+    }
+  }
+}
+
+// The linear predictor in one launch: coef = |dF| (A dF/|dF| + b).
+__global__ __launch_bounds__(IB) void lin_infer_kernel(int64_t nflt, int NB, const double* x, const double* base,
+                                                       const float* A, const float* b, float* buf, float* coef) {
+  const double nrm = infer_input(x, base, nflt, buf, coef, NB);
+  rows_gemv(NB, (int)nflt, buf, A, b, coef, 0, true, (float)nrm);
+}
+
 // p[off + i] = sum_k Phi[k, i] coef[k]; p[0:off] = 0
 __global__ __launch_bounds__(LB) void guess_kernel(int64_t nI, int64_t off, int NB, const float* Phi,
                                                    const float* coef, double* p) {
@@ -563,7 +621,7 @@ int learn_init(LearnState* ls, int64_t nI, int64_t nflt, int64_t offset, int cap
 void learn_free(LearnState* ls) {
   void* ptrs[] = {ls->S, ls->F, ls->Phi, ls->X, ls->Y, ls->w32, ls->w16, ls->grad, ls->adam_m, ls->adam_v,
                   ls->act, ls->dact, ls->out, ls->idx, ls->loss_part, ls->inf_act, ls->coef, ls->norms,
-                  ls->gal_part, ls->gal_c, ls->gal_Ginv, ls->Phi_rm, ls->gal_T};
+                  ls->gal_part, ls->gal_c, ls->gal_Ginv, ls->Phi_rm, ls->gal_T, ls->lin_A, ls->lin_b};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   NetDims d = ls->d;
@@ -828,12 +886,22 @@ int learn_loss_grad(LearnState* ls, const float* params, int nsamp, const int32_
   return 0;
 }
 
-// Per-layer inference: input normalisation, conv layers (thread per output),
-// wave-per-row GEMVs for the MLP, then p = Phi coef.
-static void infer(LearnState* ls, const double* x, const double* base, hipStream_t s) {
+// Coefficients for the input x - base into ls->coef (coef[NB] = |x - base|).
+// The linear predictor: one launch.  The network: per layer, input
+// normalisation, conv layers (thread per output), wave-per-row GEMVs for the
+// MLP -- 7 launches spread over the chip.  The network's whole forward in one
+// workgroup (activations in global scratch between barriers) measured about
+// 280 us per call on the 641-node fault, against tens of microseconds for
+// the per-layer launches, so the launches stay.
+static void infer(LearnState* ls, const double* x, const double* base, int linear, hipStream_t s) {
   const NetDims& d = ls->d;
   float* bufA = ls->inf_act;
   float* bufB = ls->inf_act + std::max<int64_t>(ls->nflt, (int64_t)d.C * d.L[1]) + d.H;
+  if (linear) {
+    hipLaunchKernelGGL(lin_infer_kernel, dim3(1), dim3(IB), 0, s, ls->nflt, d.NB, x, base, ls->lin_A, ls->lin_b,
+                       bufA, ls->coef);
+    return;
+  }
   hipLaunchKernelGGL(infer_input_kernel, dim3(1), dim3(LB), 0, s, x, base, ls->nflt, bufA, ls->coef, d.NB);
   float* in = bufA;
   float* out = bufB;
@@ -850,11 +918,34 @@ static void infer(LearnState* ls, const double* x, const double* base, hipStream
                      (const float*)(ls->coef + d.NB));
 }
 
-void learn_guess(LearnState* ls, const double* x, const double* base, double* p, hipStream_t s) {
-  infer(ls, x, base, s);
+static void expand_guess(LearnState* ls, double* p, hipStream_t s) {
   const int64_t n = std::max(ls->nI, ls->offset);
   hipLaunchKernelGGL(guess_kernel, dim3((unsigned)((n + LB - 1) / LB)), dim3(LB), 0, s, ls->nI, ls->offset, ls->d.NB,
                      ls->Phi, ls->coef, p);
+}
+
+void learn_guess(LearnState* ls, const double* x, const double* base, double* p, hipStream_t s) {
+  infer(ls, x, base, 0, s);
+  expand_guess(ls, p, s);
+}
+
+void learn_guess_linear(LearnState* ls, const double* x, const double* base, double* p, hipStream_t s) {
+  infer(ls, x, base, 1, s);
+  expand_guess(ls, p, s);
+}
+
+int learn_set_linear(LearnState* ls, const float* A_host, const float* b_host, hipStream_t s, std::string* err) {
+  const int64_t nA = (int64_t)ls->d.NB * ls->nflt;
+  if (!ls->lin_A) {
+    ls->lin_A = dalloc<float>(nA);
+    ls->lin_b = dalloc<float>(ls->d.NB);
+    if (!ls->lin_A || !ls->lin_b) { *err = "device allocation failed (linear predictor)"; return 7; }
+  }
+  LCHK(hipMemcpyAsync(ls->lin_A, A_host, sizeof(float) * nA, hipMemcpyHostToDevice, s));
+  LCHK(hipMemcpyAsync(ls->lin_b, b_host, sizeof(float) * ls->d.NB, hipMemcpyHostToDevice, s));
+  LCHK(hipStreamSynchronize(s));
+  ls->lin_ready = 1;
+  return 0;
 }
 
 int learn_get_basis(LearnState* ls, float* Phi_host, hipStream_t s) {
@@ -868,13 +959,14 @@ int learn_get_params(LearnState* ls, float* out, hipStream_t s) {
   return hipStreamSynchronize(s) == hipSuccess ? 0 : 2;
 }
 
-int learn_predict(LearnState* ls, const double* dF_host, double* coef_host, hipStream_t s, std::string* err) {
+int learn_predict(LearnState* ls, const double* dF_host, double* coef_host, int linear, hipStream_t s,
+                  std::string* err) {
   double* x = dalloc<double>(ls->nflt);
   double* z = dalloc<double>(ls->nflt);
   if (!x || !z) { *err = "device allocation failed"; return 7; }
   LCHK(hipMemcpyAsync(x, dF_host, sizeof(double) * ls->nflt, hipMemcpyHostToDevice, s));
   LCHK(hipMemsetAsync(z, 0, sizeof(double) * ls->nflt, s));
-  infer(ls, x, z, s);
+  infer(ls, x, z, linear, s);
   std::vector<float> c(ls->d.NB);
   LCHK(hipMemcpyAsync(c.data(), ls->coef, sizeof(float) * ls->d.NB, hipMemcpyDeviceToHost, s));
   const hipError_t e = hipStreamSynchronize(s);

@@ -8,6 +8,10 @@
 3. ``train``: Adam on the conv1d + MLP network (hand-written GPU kernels).
 4. ``enable``: set eq_solver_opts.guess = learned; the previous-step guess stays
    the fallback (energy-optimal step along the learned correction).
+5. ``fit_linear``: the linear least-squares map from the same input to the
+   same coefficients (guess = linear), the bar the network has to beat: the
+   quasi-static response is linear in dF, so a linear predictor in POD space
+   is the natural competitor.
 """
 from __future__ import annotations
 
@@ -39,7 +43,23 @@ def fit_basis(ctx, nbasis=None, tol=1e-8, noise=1e-3):
     sigma = np.sqrt(w)
     ctx.learn_set_basis(np.ascontiguousarray(V), sigma, use.astype(np.int32))
     energy = w.sum() / max(np.trace(G[np.ix_(idx, idx)]), 1e-300)
-    return dict(sigma=sigma, energy_captured=float(energy), used=int(use.sum()), total=int(len(use)))
+    return dict(sigma=sigma, energy_captured=float(energy), used=int(use.sum()), total=int(len(use)), use=use)
+
+
+def fit_linear(ctx, use):
+    """Least squares [X 1] [A^T; b^T] = Y over the snapshots in use (X: the
+    network's normalised inputs, Y: its targets), in FP64 on the host, handed
+    to the device as the linear predictor.  Returns the fit's relative
+    residual on the training set."""
+    data = ctx.learn_get()
+    X = data["X"][use].astype(np.float64)
+    Y = data["Y"][use].astype(np.float64)
+    Xa = np.hstack([X, np.ones((X.shape[0], 1))])
+    beta, *_ = np.linalg.lstsq(Xa, Y, rcond=None)
+    A, b = beta[:-1].T, beta[-1]
+    ctx.learn_set_linear(A, b)
+    res = np.linalg.norm(Xa @ beta - Y) / max(np.linalg.norm(Y), 1e-300)
+    return dict(train_rel_residual=float(res), rows=int(X.shape[0]))
 
 
 def train(ctx, epochs=300, batch=64, lr=1e-3, seed=11):

@@ -21,7 +21,7 @@ EQ_MASK_ALL, EQ_MASK_INTERIOR, EQ_MASK_NEG, EQ_MASK_FAULT_ROWS = 0, 1, 2, 3
 EQ_PREC_FP64, EQ_PREC_FP32, EQ_PREC_MIXED, EQ_PREC_F32PC = 0, 1, 2, 3
 EQ_PC_NONE, EQ_PC_JACOBI, EQ_PC_AMG = 0, 1, 2
 EQ_SMOOTH_JACOBI, EQ_SMOOTH_CHEBYSHEV = 0, 1
-EQ_GUESS_PREVIOUS, EQ_GUESS_LEARNED, EQ_GUESS_GALERKIN = 0, 1, 2
+EQ_GUESS_PREVIOUS, EQ_GUESS_LEARNED, EQ_GUESS_GALERKIN, EQ_GUESS_LINEAR = 0, 1, 2, 3
 
 # every symbol the header declares (checked by the CPU test-suite)
 EXPORTS = [
@@ -32,7 +32,8 @@ EXPORTS = [
     "eq_step", "eq_run", "eq_time_kernel", "eq_learn_init", "eq_learn_record", "eq_learn_info",
     "eq_learn_gram", "eq_learn_set_basis", "eq_learn_train", "eq_learn_loss_grad", "eq_learn_get",
     "eq_learn_predict", "eq_learn_norms", "eq_learn_galerkin", "eq_newmark_step", "eq_amg_upload",
-    "eq_launch_timing", "eq_launch_timing_get", "eq_newmark_step_noalloc",
+    "eq_launch_timing", "eq_launch_timing_get", "eq_newmark_step_noalloc", "eq_learn_set_linear",
+    "eq_learn_predict_linear",
 ]
 EQ_TIME_KU, EQ_TIME_CG_DIR, EQ_TIME_VCYCLE, EQ_TIME_CG_ITER, EQ_TIME_NEWMARK, EQ_TIME_CHEB, EQ_TIME_COPY, \
     EQ_TIME_CHEB2 = range(8)
@@ -155,6 +156,8 @@ def load():
         "eq_learn_loss_grad": ([vp, fp, C.c_int, i32p, C.c_int, dp, fp], C.c_int),
         "eq_learn_get": ([vp, fp, fp, fp, fp], C.c_int),
         "eq_learn_predict": ([vp, dp, dp], C.c_int),
+        "eq_learn_set_linear": ([vp, fp, fp], C.c_int),
+        "eq_learn_predict_linear": ([vp, dp, dp], C.c_int),
         "eq_learn_galerkin": ([vp, dp, dp, dp], C.c_int),
         "eq_launch_timing": ([vp, C.c_int], C.c_int),
         "eq_launch_timing_get": ([vp, i64p, dp], C.c_int),

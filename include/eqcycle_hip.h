@@ -38,7 +38,7 @@ enum { EQ_MASK_ALL = 0, EQ_MASK_INTERIOR = 1, EQ_MASK_NEG = 2, EQ_MASK_FAULT_ROW
 enum { EQ_PREC_FP64 = 0, EQ_PREC_FP32 = 1, EQ_PREC_MIXED = 2, EQ_PREC_F32PC = 3 };
 enum { EQ_PC_NONE = 0, EQ_PC_JACOBI = 1, EQ_PC_AMG = 2 };
 enum { EQ_SMOOTH_JACOBI = 0, EQ_SMOOTH_CHEBYSHEV = 1 };
-enum { EQ_GUESS_PREVIOUS = 0, EQ_GUESS_LEARNED = 1, EQ_GUESS_GALERKIN = 2 };
+enum { EQ_GUESS_PREVIOUS = 0, EQ_GUESS_LEARNED = 1, EQ_GUESS_GALERKIN = 2, EQ_GUESS_LINEAR = 3 };
 
 typedef struct eq_ctx eq_ctx;
 
@@ -190,7 +190,7 @@ typedef struct {
   int64_t maxiter;         /* 0: nI, as cg! */
   int pc;                  /* EQ_PC_* */
   int prec;                /* EQ_PREC_FP64 or EQ_PREC_MIXED */
-  int guess;               /* EQ_GUESS_PREVIOUS (src/main.jl:241), _LEARNED or _GALERKIN */
+  int guess;               /* EQ_GUESS_PREVIOUS (src/main.jl:241), _LEARNED, _GALERKIN or _LINEAR */
   int record_snapshots;    /* copy fault fields at events (src/main.jl:361-422) */
   int graphs;              /* 1: CG iteration batches replay captured HIP graphs (same kernels) */
   int cheb_single;         /* 1: one launch per level-0 Chebyshev step; 0 (default): two steps per
@@ -306,6 +306,14 @@ int eq_learn_loss_grad(eq_ctx *ctx, const float *params, int nsamp, const int32_
 int eq_learn_get(eq_ctx *ctx, float *params, float *Phi, float *X, float *Y);
 /* Network output (POD coefficients times |dF|) for one fault increment. */
 int eq_learn_predict(eq_ctx *ctx, const double *dF, double *coef);
+/* The linear least-squares predictor of the same coefficients from the same
+ * input, the bar the network has to beat: coef = |dF| (A dF/|dF| + b), A
+ * [nbasis x FltNglob] row-major and b [nbasis] fitted by the caller on the
+ * dataset (eq_learn_get).  With eq_solver_opts.guess = EQ_GUESS_LINEAR the
+ * pass-1 start is that of EQ_GUESS_LEARNED with these coefficients. */
+int eq_learn_set_linear(eq_ctx *ctx, const float *A, const float *b);
+/* The linear predictor's output for one fault increment. */
+int eq_learn_predict_linear(eq_ctx *ctx, const double *dF, double *coef);
 /* Galerkin start on the POD basis (guess EQ_GUESS_GALERKIN, both quasi-static
  * passes): x_out = x + Phi c with c = (Phi^T K_II Phi)^-1 Phi^T (b - K_II x),
  * the K_II-norm best correction of x within span(Phi).  Phi^T K_II Phi is

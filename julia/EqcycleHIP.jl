@@ -298,7 +298,7 @@ function newmark_step!(ctx::Context, d, v, a, psi, tau, Vf, FltVfree, dt)
 end
 
 # ---- device-resident loop: main(P) (src/main.jl:67-476) ---------------------
-const EQ_GUESS_PREVIOUS, EQ_GUESS_LEARNED, EQ_GUESS_GALERKIN = 0, 1, 2
+const EQ_GUESS_PREVIOUS, EQ_GUESS_LEARNED, EQ_GUESS_GALERKIN, EQ_GUESS_LINEAR = 0, 1, 2, 3
 
 struct SolverOpts                      # eq_solver_opts, field for field
     tol::Float64

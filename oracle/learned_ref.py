@@ -45,6 +45,16 @@ class NetRef:
         idx = 2 * np.arange(Lout)[:, None] + np.arange(5)[None, :]
         return xp[:, :, idx]
 
+    def forward(self, p, X):
+        """Network outputs [B x NB] for inputs X [B x L0]."""
+        P = self.unpack(p)
+        a = np.asarray(X, dtype=np.float64)[:, None, :]
+        for l in range(1, self.NL + 1):
+            z = np.einsum("bcok,dck->bdo", self._windows(a), P[f"w{l}"]) + P[f"b{l}"][None, :, None]
+            a = np.maximum(z, 0.0)
+        h = np.maximum(a.reshape(a.shape[0], -1) @ P["fc1w"].T + P["fc1b"], 0.0)
+        return h @ P["fc2w"].T + P["fc2b"]
+
     def loss_grad(self, p, X, Y):
         P = self.unpack(p)
         B = X.shape[0]

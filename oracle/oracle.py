@@ -409,12 +409,20 @@ class Simulation:
         if solver == "amg":
             self.ml = Amg(self.kni, **(amg_kw or {}))
         elif solver == "direct":
-            # K_II is symmetric positive definite: a symmetric minimum-degree
-            # ordering without pivoting keeps the LU fill near Cholesky's
-            # (71 M entries instead of 400 M on the 160 x 160 mesh)
+            # SuperLU's default column ordering up to 100k unknowns; above,
+            # K_II being symmetric positive definite, a symmetric minimum-degree
+            # ordering without pivoting keeps the fill near Cholesky's (71 M
+            # entries instead of 400 M on the 160 x 160 mesh).  Either is a
+            # direct solve; through an event the two orderings' trajectories
+            # part by ~5e-9 in d (the 32 x 8 test mesh), the loop's own
+            # round-off sensitivity.
             import scipy.sparse.linalg as spla
-            self.lu = spla.splu(self.kni.to_scipy().tocsc(), permc_spec="MMD_AT_PLUS_A", diag_pivot_thresh=0.0,
-                                options=dict(SymmetricMode=True))
+            A = self.kni.to_scipy().tocsc()
+            if A.shape[0] <= 100_000:
+                self.lu = spla.splu(A)
+            else:
+                self.lu = spla.splu(A, permc_spec="MMD_AT_PLUS_A", diag_pivot_thresh=0.0,
+                                    options=dict(SymmetricMode=True))
         else:
             raise ValueError(solver)
         # state (src/main.jl:88-190)

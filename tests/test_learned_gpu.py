@@ -143,3 +143,69 @@ def test_galerkin_change_of_b_start_matches_full_projection(trained, prec):
     print(f"[gpu] {prec}: iterations change-of-b start {np.sum(it_a)}, full projection {np.sum(it_b)}", flush=True)
     assert abs(np.sum(it_a) - np.sum(it_b)) <= 0.01 * np.sum(it_b)
     assert np.max(np.abs(d_a - d_b)) <= 1e-9 * np.max(np.abs(d_b))
+
+
+def _bf16(x):
+    """Round FP32 values to BF16 (nearest even) and back, as the kernels store weights."""
+    u = np.asarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32)
+
+
+def test_one_launch_network_inference_matches_fp64_reference(trained):
+    """The network's whole forward in one launch (eq_learn_predict): |dF| times
+    the FP64 reference network on dF/|dF|, with the weights rounded to BF16 as
+    the kernels hold them (biases FP32), to FP32 accumulation accuracy."""
+    from oracle.learned_ref import NetRef
+    P, ctx, _ = trained
+    data = ctx.learn_get()
+    net = NetRef(P.FltNglob, C=32, NL=4, H=256, NB=NB)
+    p = data["params"].astype(np.float64)
+    u = net.unpack(p)
+    for k in u:
+        if "w" in k:
+            u[k] = _bf16(u[k]).astype(np.float64)
+    pb = net.pack(u)
+    rng = np.random.default_rng(8)
+    for _ in range(3):
+        dF = rng.standard_normal(P.FltNglob) * 1e-3
+        got = ctx.learn_predict(dF)
+        n = np.linalg.norm(dF)
+        want = n * net.forward(pb, (dF / n)[None, :])[0]
+        assert np.linalg.norm(got - want) <= 1e-5 * np.linalg.norm(want)
+
+
+def test_linear_predictor_against_network_and_previous_guess(trained):
+    """The linear least-squares predictor of the POD coefficients from the
+    network's input (the bar the network must beat): its one-launch device
+    output equals |dF| (A dF/|dF| + b) computed here; and over the same 40
+    steps from the same state, the pass-1 iterations of the previous-step
+    guess, the network start and the linear start (both energy-safeguarded,
+    so neither needs more iterations than the guess it corrects)."""
+    from eqcycle_amd import learned
+    P, ctx, fit = trained
+    lin = learned.fit_linear(ctx, fit["use"])
+    print(f"[gpu] linear fit: {lin}", flush=True)
+    data = ctx.learn_get()
+    X, Y = data["X"][fit["use"]].astype(np.float64), data["Y"][fit["use"]].astype(np.float64)
+    Xa = np.hstack([X, np.ones((X.shape[0], 1))])
+    beta = np.linalg.lstsq(Xa, Y, rcond=None)[0]
+    A, b = beta[:-1].T.astype(np.float32).astype(np.float64), beta[-1].astype(np.float32).astype(np.float64)
+    dF = np.random.default_rng(9).standard_normal(P.FltNglob) * 1e-3
+    n = np.linalg.norm(dF)
+    want = n * (A @ (dF / n).astype(np.float32).astype(np.float64) + b)
+    got = ctx.learn_predict_linear(dF)
+    assert np.linalg.norm(got - want) <= 1e-5 * np.linalg.norm(want)
+    s0 = ctx.state_get()
+    its, dfin = {}, {}
+    for guess in ("previous", "learned", "linear"):
+        ctx.state_set(d=s0["d"], v=s0["v"], a=s0["a"], psi=s0["psi"], tau=s0["tau"], t=s0["t"], dt=s0["dt"],
+                      isolver=s0["isolver"])
+        ctx.set_solver(tol=1e-10, pc="amg", guess=guess)
+        its[guess] = sum(ctx.step().pcg_iters[0] for _ in range(40))
+        dfin[guess] = ctx.state_get()["d"]
+    print(f"[gpu] pass-1 iterations over 40 steps: {its}", flush=True)
+    assert its["linear"] <= its["previous"] and its["learned"] <= its["previous"]
+    dref = np.max(np.abs(dfin["previous"]))
+    assert np.max(np.abs(dfin["linear"] - dfin["previous"])) <= 1e-7 * dref
+
