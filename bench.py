@@ -91,13 +91,14 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU budget per thread setting")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--plain-only", action="store_true")
+    ap.add_argument("--nbasis", type=int, default=64, help="POD basis vectors of the learned starts")
     return ap.parse_args()
 
 
 # ---------------------------------------------------------------------------
 # GPU side
 # ---------------------------------------------------------------------------
-def checkpoint(ctx, cfg, tol, learned, epochs, graphs):
+def checkpoint(ctx, cfg, tol, learned, epochs, graphs, nbasis=64):
     """The plain solver from the seeded state to CHECKPOINT_YEARS; with the
     learned start, its first cfg.n_snapshots pass-1 solves are the training
     snapshots (POD basis and network fitted afterwards)."""
@@ -107,7 +108,7 @@ def checkpoint(ctx, cfg, tol, learned, epochs, graphs):
     info = {}
     t0 = time.perf_counter()
     if learned:
-        ctx.learn_init(capacity=cfg.n_snapshots, nbasis=64)
+        ctx.learn_init(capacity=cfg.n_snapshots, nbasis=nbasis)
         ctx.learn_record(True)
         while ctx.learn_info()["nsnap"] < cfg.n_snapshots and ctx.state_get()["t"] < CHECKPOINT_YEARS * YR:
             ctx.run(total_time=CHECKPOINT_YEARS * YR, max_steps=400, snapshots=False)
@@ -322,7 +323,7 @@ def main():
     ctx = EqContext(P, amg={"hierarchy": a.hierarchy, "degree": a.degree})
     learned = cfg.learned and not a.plain_only
 
-    ck, training = checkpoint(ctx, cfg, tol, learned, a.epochs, a.graphs)
+    ck, training = checkpoint(ctx, cfg, tol, learned, a.epochs, a.graphs, a.nbasis)
     print(f"checkpoint at {ck['t'] / YR:.3f} years", file=sys.stderr, flush=True)
     plain, s_plain, R_plain = variant(ctx, ck, tol, "fp64", "previous", a.warmup, a.steps, a.graphs)
     print(f"plain: {plain['sim_years_per_hour']:.0f} sim-years/hour", file=sys.stderr, flush=True)

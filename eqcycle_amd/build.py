@@ -36,34 +36,38 @@ def _run(cmd):
     return r
 
 
-def build(verbose: bool = False, defines=()) -> str:
+def build(verbose: bool = False, defines=(), out: str | None = None) -> str:
     """defines: extra preprocessor definitions, e.g. ("EQ_TILE_I=32",) for
-    32 x 16 tiles in the grid kernels (default 64 x 16)."""
-    os.makedirs(OUT_DIR, exist_ok=True)
+    32 x 16 tiles in the grid kernels (default 64 x 16); out: where to write
+    the library (default: the package's own, which the package loads)."""
+    lib = out or LIB
+    obj_dir = OUT_DIR if out is None else os.path.join(os.path.dirname(os.path.abspath(out)), "_build")
+    os.makedirs(obj_dir, exist_ok=True)
     dflags = [f"-D{d}" for d in defines]
     jobs = []
     objs = []
     for src in HIP_SOURCES:
-        obj = os.path.join(OUT_DIR, src.replace(".hip", ".o"))
+        obj = os.path.join(obj_dir, src.replace(".hip", ".o"))
         jobs.append([HIPCC, *HIP_FLAGS, *dflags, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     for src in CXX_SOURCES:
-        obj = os.path.join(OUT_DIR, src.replace(".cpp", ".o"))
+        obj = os.path.join(obj_dir, src.replace(".cpp", ".o"))
         jobs.append(["g++", *CXX_FLAGS, *dflags, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     with ThreadPoolExecutor(max_workers=4) as ex:
         for r in ex.map(_run, jobs):
             if verbose and (r.stdout or r.stderr):
                 print(r.stdout, r.stderr, file=sys.stderr)
-    tmp = LIB + ".tmp"
+    tmp = lib + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp, *objs])
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib)
+    return lib
 
 
 if __name__ == "__main__":
     import argparse
     ap = argparse.ArgumentParser()
     ap.add_argument("--define", action="append", default=[], help="preprocessor definition NAME=VALUE")
+    ap.add_argument("--out", default=None, help="library path (default: the package's)")
     a = ap.parse_args()
-    print(build(verbose=True, defines=a.define))
+    print(build(verbose=True, defines=a.define, out=a.out))

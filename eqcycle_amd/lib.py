@@ -12,7 +12,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libeqcycle_hip.so")
+# EQCYCLE_HIP_LIB: another build of the library (e.g. one with other
+# build-time options), as the Julia binding allows
+LIB_PATH = os.environ.get("EQCYCLE_HIP_LIB", os.path.join(HERE, "libeqcycle_hip.so"))
 
 EQ_OK, EQ_EINVAL, EQ_HIP_ERR, EQ_NR_NOCONV, EQ_NR_NAN, EQ_PCG_NOCONV, EQ_ENOTREADY, EQ_NOMEM = range(8)
 STATUS_NAMES = {0: "EQ_OK", 1: "EQ_EINVAL", 2: "EQ_HIP_ERR", 3: "EQ_NR_NOCONV", 4: "EQ_NR_NAN",

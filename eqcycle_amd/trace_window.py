@@ -5,9 +5,10 @@ rocprofv3 kernel trace.
 
 profile_window.py runs one 512 MiB copy (`convert_kernel<double, double>`)
 right before its window; every dispatch after the last such copy belongs to
-the window.  Prints the window's span and busy time, then per kernel the call
-count, total and average duration, and, for the Newmark kernel, the gaps
-between consecutive launches (host synchronisations of the dynamic batches).
+the window.  Prints the window's span and busy time, the idle time between dispatches
+(host synchronisations), then per kernel the call count, total and average
+duration, and, for the Newmark kernel, the gaps between consecutive launches
+(host synchronisations of the dynamic batches).
 """
 from __future__ import annotations
 
@@ -37,6 +38,10 @@ def main():
     print("|---|---|---|---|---|")
     for k, (c, t) in sorted(agg.items(), key=lambda x: -x[1][1])[:25]:
         print(f"| `{k}` | {c} | {t:.1f} | {t / c:.2f} | {100 * t / busy:.1f} |")
+    idle = sorted(w[i + 1][0] - w[i][1] for i in range(len(w) - 1))
+    big = [g for g in idle if g > 5000]
+    print(f"idle between dispatches: {sum(g for g in idle if g > 0) / 1e3:.1f} us in total; "
+          f"{len(big)} gaps over 5 us hold {sum(big) / 1e3:.1f} us")
     nm = [(s, e) for s, e, n in w if "newmark" in n]
     gaps = sorted(nm[i + 1][0] - nm[i][1] for i in range(len(nm) - 1))
     if gaps:
