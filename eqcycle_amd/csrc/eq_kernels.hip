@@ -750,6 +750,27 @@ __device__ double d_ids2(double xLf, double Vo, double psi, double psi1, double 
 // to that bound is taken at once (k += n, tau += n c: the same iteration count
 // and, to rounding, the same iterate).  The other steps run as written, at M
 // below about 65, far from the FP64 range of exp.
+// Vo (exp(h + fa) - exp(h - fa)) and vfc (exp(h + fa) + exp(h - fa)) of the
+// search.  For |fa| >= 20 the smaller exponential is below 2^-54 of the larger
+// (e^-40 = 4.2e-18), so adding or subtracting it cannot change the rounded
+// FP64 sum: it is not evaluated, and the results are the two-exponential
+// form's bit for bit.
+__device__ __forceinline__ void nr_rate(double help, double fa, double Vo, double vfc, double& Vf, double& Vfp) {
+  if (fa >= 20.0) {
+    const double e = exp(help + fa);
+    Vf = Vo * e;
+    Vfp = vfc * e;
+  } else if (fa <= -20.0) {
+    const double e = exp(help - fa);
+    Vf = Vo * (-e);
+    Vfp = vfc * e;
+  } else {
+    const double help1 = exp(help + fa), help2 = exp(help - fa);
+    Vf = Vo * (help1 - help2);
+    Vfp = vfc * (help1 + help2);
+  }
+}
+
 __device__ int d_nrsearch(double fo, double Vo, double cca, double ccb, double Seff, double tau, double tauo,
                           double psi, double FltZ, double FltVfree, double& Vf_out, double& tau_out) {
   const double Vw = 1e10;
@@ -761,13 +782,18 @@ __device__ int d_nrsearch(double fo, double Vo, double cca, double ccb, double S
   // divided form to rounding)
   const double kfa = fact / (Seff * cca);
   const double vfc = fact * (Vo / (cca * Seff));   // Vf' = vfc (help1 + help2)
+  // |c| and a lower bound of the jump threshold Ta below (|FltZ FltVfree +
+  // tauo - tau| >= 0), less a margin far above log's rounding: while M is
+  // below it, M >= Ta is false and Ta is not evaluated
+  const double cabs = Vo / vfc;
+  const double Ta_lo = log(1000.0 * cabs / (1e-20 * FltZ * Vo)) - 1e-6;
   int k = 0;
   double delta = INFINITY, Vf = 0.0;
   while (fabs(delta) > eps) {
     const double fa = kfa * tau;
     const double M = fmax(help + fa, help - fa);
-    if (isfinite(M) && M >= help + 40.0) {
-      const double c = copysign(Vo / vfc, -fa);
+    if (isfinite(M) && M >= help + 40.0 && !(M < Ta_lo)) {
+      const double c = copysign(cabs, -fa);
       const double Ta = log((fabs(FltZ * FltVfree + tauo - tau) + 1000.0 * fabs(c)) / (1e-20 * FltZ * Vo));
       if (M >= Ta) {   // false for a NaN Ta
         // the steps at M, M - dM, ... down to max(h + 40, Ta); one step if it
@@ -781,17 +807,16 @@ __device__ int d_nrsearch(double fo, double Vo, double cca, double ccb, double S
         k += n;
         delta = c;
         if (fabs(delta) > 1e10 || k == 1000) {
-          const double fl = kfa * tau_last;
-          Vf_out = Vo * (exp(help + fl) - exp(help - fl));
+          double Vfp;
+          nr_rate(help, kfa * tau_last, Vo, vfc, Vf_out, Vfp);
           tau_out = tau;
           return -k;
         }
         continue;
       }
     }
-    const double help1 = exp(help + fa), help2 = exp(help - fa);
-    Vf = Vo * (help1 - help2);
-    const double Vfprime = vfc * (help1 + help2);
+    double Vfprime;
+    nr_rate(help, fa, Vo, vfc, Vf, Vfprime);
     delta = (FltZ * FltVfree - FltZ * Vf + tauo - tau) / (1 + FltZ * Vfprime);
     tau = tau + delta;
     k = k + 1;
@@ -801,8 +826,8 @@ __device__ int d_nrsearch(double fo, double Vo, double cca, double ccb, double S
       return -k;
     }
   }
-  const double fa = kfa * tau;
-  Vf_out = Vo * (exp(help + fa) - exp(help - fa));
+  double Vfp;
+  nr_rate(help, kfa * tau, Vo, vfc, Vf_out, Vfp);
   tau_out = tau;
   return k;
 }
