@@ -505,6 +505,51 @@ class StepInfo:
 
 
 # ---------------------------------------------------------------------------
+# host-side Ruge-Stueben set-up (no GPU)
+# ---------------------------------------------------------------------------
+def rs_hierarchy(A, theta=0.25, max_levels=10, max_coarse=10, negate=True):
+    """The Ruge-Stueben hierarchy the library's "rs" set-up builds from the SPD
+    matrix A (coarsening -A as ruge_stuben(kni) does, src/main.jl:195-200, or A
+    itself with negate=False; see eq_rs_build): {"A": [A_l], "P": [P_l],
+    "split": [C mask of level l]}, scipy CSR matrices with A's sign."""
+    import scipy.sparse as sp
+    lib = L.load()
+    M = sp.csr_matrix(A)
+    M.sort_indices()
+    rp = np.ascontiguousarray(M.indptr, dtype=np.int64)
+    cl = np.ascontiguousarray(M.indices, dtype=np.int64)
+    vl = np.ascontiguousarray(M.data, dtype=np.float64)
+    desc = L.CsrDesc(M.shape[0], M.shape[1], ptr(rp, C.c_int64), ptr(cl, C.c_int64), ptr(vl))
+    h = C.c_void_p()
+    rc = lib.eq_rs_build(C.byref(desc), theta, max_levels, max_coarse, 1 if negate else 0, C.byref(h))
+    if rc:
+        raise EqError(rc, f"eq_rs_build: {L.STATUS_NAMES.get(rc, rc)}")
+    try:
+        nlev = C.c_int()
+        cap = 64
+        n, nna, nnp = (np.zeros(cap, np.int64) for _ in range(3))
+        lib.eq_rs_info(h, C.byref(nlev), ptr(n, C.c_int64), ptr(nna, C.c_int64), ptr(nnp, C.c_int64), cap)
+        out = {"A": [], "P": [], "split": []}
+        for l in range(nlev.value):
+            for which, nnz in ((0, nna[l]), (1, nnp[l])):
+                if which == 1 and l + 1 >= nlev.value:
+                    continue
+                r = np.zeros(n[l] + 1, np.int64)
+                c = np.zeros(max(1, nnz), np.int64)
+                v = np.zeros(max(1, nnz))
+                lib.eq_rs_level(h, l, which, ptr(r, C.c_int64), ptr(c, C.c_int64), ptr(v))
+                m = n[l] if which == 0 else n[l + 1]
+                out["A" if which == 0 else "P"].append(sp.csr_matrix((v[:nnz], c[:nnz], r), shape=(n[l], m)))
+            if l + 1 < nlev.value:
+                s_ = np.zeros(n[l], np.int32)
+                lib.eq_rs_split(h, l, ptr(s_, C.c_int32))
+                out["split"].append(s_.astype(bool))
+        return out
+    finally:
+        lib.eq_rs_free(h)
+
+
+# ---------------------------------------------------------------------------
 # reference-named functions: the positional signatures of the Julia routines,
 # argument for argument, on the current context (use_context; the last
 # EqContext created otherwise), so that a call site needs no edit

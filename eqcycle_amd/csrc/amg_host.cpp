@@ -391,8 +391,10 @@ double estimate_lambda_max_dinvA(const HostCsr& A, const std::vector<double>& di
   return lam;
 }
 
-int amg_build(const HostCsr& A0, const AmgOptions& opt, HostHierarchy* H, std::string* err) {
+int amg_build(const HostCsr& A0, const AmgOptions& opt, HostHierarchy* H, std::string* err,
+              std::vector<std::vector<int>>* splits) {
   H->lev.clear();
+  if (splits) splits->clear();
   HostLevel L0;
   L0.A = A0;
   const double sgn = opt.build_on_negated ? -1.0 : 1.0;
@@ -410,6 +412,10 @@ int amg_build(const HostCsr& A0, const AmgOptions& opt, HostHierarchy* H, std::s
     if (cur.P.m == 0 || cur.P.m == cur.A.n) {
       cur.P = HostCsr();
       break;
+    }
+    if (splits) {
+      splits->emplace_back(split.size());
+      for (size_t i = 0; i < split.size(); i++) splits->back()[i] = split[i] == C_PT;
     }
     csr_transpose(cur.P, &cur.R);
     HostCsr RA, Ac;
@@ -602,3 +608,80 @@ int pmg_build(const Geom& g, const HostCsr& Kii, const AmgOptions& opt, HostHier
 }
 
 }  // namespace eq
+
+// ---------------------------------------------------------------------------
+// Host-side Ruge-Stueben set-up exported for inspection (no GPU involved):
+// the hierarchy eq_amg_setup builds with its "rs" choice, from a caller's
+// matrix.
+// ---------------------------------------------------------------------------
+struct eq_rs {
+  eq::HostHierarchy H;
+  std::vector<std::vector<int>> split;
+};
+
+extern "C" int eq_rs_build(const eq_csr* A, double theta, int max_levels, int64_t max_coarse, int negate,
+                           eq_rs** out) {
+  if (!A || !out || !A->rowptr || A->n <= 0 || A->m != A->n || !(theta >= 0) || max_levels < 1) return EQ_EINVAL;
+  *out = nullptr;
+  const int64_t n = A->n;
+  if (A->rowptr[0] != 0) return EQ_EINVAL;
+  for (int64_t i = 0; i < n; i++)
+    if (A->rowptr[i + 1] < A->rowptr[i]) return EQ_EINVAL;
+  const int64_t nnz = A->rowptr[n];
+  if (nnz > 0 && (!A->col || !A->val)) return EQ_EINVAL;
+  eq::HostCsr C;
+  C.n = C.m = n;
+  C.rowptr.assign(A->rowptr, A->rowptr + n + 1);
+  C.col.resize(nnz);
+  C.val.assign(A->val, A->val + nnz);
+  for (int64_t q = 0; q < nnz; q++) {
+    if (A->col[q] < 0 || A->col[q] >= n) return EQ_EINVAL;
+    C.col[q] = (int32_t)A->col[q];
+  }
+  eq::AmgOptions o;
+  o.theta = theta;
+  o.max_levels = max_levels;
+  o.max_coarse = max_coarse;
+  o.build_on_negated = negate != 0;
+  auto* h = new (std::nothrow) eq_rs;
+  if (!h) return EQ_NOMEM;
+  std::string err;
+  if (int rc = eq::amg_build(C, o, &h->H, &err, &h->split)) {
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return EQ_OK;
+}
+
+extern "C" int eq_rs_info(const eq_rs* h, int* nlev, int64_t* n, int64_t* nnz_a, int64_t* nnz_p, int cap) {
+  if (!h || !nlev) return EQ_EINVAL;
+  *nlev = (int)h->H.lev.size();
+  for (int l = 0; l < *nlev && l < cap; l++) {
+    const eq::HostLevel& L = h->H.lev[l];
+    if (n) n[l] = L.A.n;
+    if (nnz_a) nnz_a[l] = L.A.nnz();
+    if (nnz_p) nnz_p[l] = (l + 1 < *nlev) ? L.P.nnz() : 0;
+  }
+  return EQ_OK;
+}
+
+extern "C" int eq_rs_level(const eq_rs* h, int l, int which, int64_t* rowptr, int64_t* col, double* val) {
+  if (!h || l < 0 || l >= (int)h->H.lev.size() || (which != 0 && which != 1)) return EQ_EINVAL;
+  if (which == 1 && l + 1 >= (int)h->H.lev.size()) return EQ_EINVAL;
+  const eq::HostCsr& M = which == 0 ? h->H.lev[l].A : h->H.lev[l].P;
+  for (int64_t i = 0; i <= M.n; i++) rowptr[i] = M.rowptr[i];
+  for (int64_t q = 0; q < M.nnz(); q++) {
+    col[q] = M.col[q];
+    val[q] = M.val[q];
+  }
+  return EQ_OK;
+}
+
+extern "C" int eq_rs_split(const eq_rs* h, int l, int* is_c) {
+  if (!h || !is_c || l < 0 || l >= (int)h->split.size()) return EQ_EINVAL;
+  std::copy(h->split[l].begin(), h->split[l].end(), is_c);
+  return EQ_OK;
+}
+
+extern "C" void eq_rs_free(eq_rs* h) { delete h; }

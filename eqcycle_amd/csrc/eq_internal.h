@@ -56,7 +56,9 @@ struct HostHierarchy {
 };
 
 // amg_host.cpp
-int amg_build(const HostCsr& A, const AmgOptions& opt, HostHierarchy* out, std::string* err);
+// splits (optional): the C/F split of each level that has a coarser one, 1 = C
+int amg_build(const HostCsr& A, const AmgOptions& opt, HostHierarchy* out, std::string* err,
+              std::vector<std::vector<int>>* splits = nullptr);
 // D^-1, lambda_max(D^-1 A) (omega: the caller's Jacobi weights per level, or
 // null to estimate) and the dense coarsest inverse, once every level's A, P
 // and R are set.

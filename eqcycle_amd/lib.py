@@ -35,7 +35,7 @@ EXPORTS = [
     "eq_learn_gram", "eq_learn_set_basis", "eq_learn_train", "eq_learn_loss_grad", "eq_learn_get",
     "eq_learn_predict", "eq_learn_norms", "eq_learn_galerkin", "eq_newmark_step", "eq_amg_upload",
     "eq_launch_timing", "eq_launch_timing_get", "eq_newmark_step_noalloc", "eq_learn_set_linear",
-    "eq_learn_predict_linear",
+    "eq_learn_predict_linear", "eq_rs_build", "eq_rs_info", "eq_rs_level", "eq_rs_split", "eq_rs_free",
 ]
 EQ_TIME_KU, EQ_TIME_CG_DIR, EQ_TIME_VCYCLE, EQ_TIME_CG_ITER, EQ_TIME_NEWMARK, EQ_TIME_CHEB, EQ_TIME_COPY, \
     EQ_TIME_CHEB2 = range(8)
@@ -163,6 +163,11 @@ def load():
         "eq_learn_galerkin": ([vp, dp, dp, dp], C.c_int),
         "eq_launch_timing": ([vp, C.c_int], C.c_int),
         "eq_launch_timing_get": ([vp, i64p, dp], C.c_int),
+        "eq_rs_build": ([C.POINTER(CsrDesc), C.c_double, C.c_int, C.c_int64, C.c_int, C.POINTER(vp)], C.c_int),
+        "eq_rs_info": ([vp, C.POINTER(C.c_int), i64p, i64p, i64p, C.c_int], C.c_int),
+        "eq_rs_level": ([vp, C.c_int, C.c_int, i64p, i64p, dp], C.c_int),
+        "eq_rs_split": ([vp, C.c_int, i32p], C.c_int),
+        "eq_rs_free": ([vp], None),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

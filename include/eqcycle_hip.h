@@ -131,6 +131,23 @@ typedef struct {
  * CSR arrays of P[l]. */
 int eq_amg_upload(eq_ctx *ctx, int nlev, const eq_csr *A, const eq_csr *P, const double *omega, int smoother,
                   int smooth_degree);
+/* Host-side Ruge-Stueben set-up, for inspection; needs no GPU and no context.
+ * Builds the hierarchy eq_amg_setup(pcoarsen = 0) builds from K_II, from the
+ * caller's SPD matrix A (n x n CSR): with negate = 1 on -A, as
+ * ruge_stuben(kni) coarsens kni = -K[FltNI, FltNI] (src/main.jl:195-200), with
+ * negate = 0 on A itself; the level operators come back with A's sign.  No
+ * counterpart in the reference (its hierarchy lives in AlgebraicMultigrid.jl's
+ * ml object); exposes what the set-up chose. */
+typedef struct eq_rs eq_rs;
+int eq_rs_build(const eq_csr *A, double theta, int max_levels, int64_t max_coarse, int negate, eq_rs **out);
+/* nlev, and per level l < cap: n[l], nnz of A_l, nnz of P_l (0 on the last) */
+int eq_rs_info(const eq_rs *h, int *nlev, int64_t *n, int64_t *nnz_a, int64_t *nnz_p, int cap);
+/* which = 0: A_l (n_l x n_l), 1: P_l (n_l x n_{l+1}), as CSR into the caller's
+ * arrays (rowptr[n_l + 1], col and val of the nnz eq_rs_info reports) */
+int eq_rs_level(const eq_rs *h, int l, int which, int64_t *rowptr, int64_t *col, double *val);
+/* is_c[i] = 1 for the C points of level l (l < nlev - 1), 0 for F points */
+int eq_rs_split(const eq_rs *h, int l, int *is_c);
+void eq_rs_free(eq_rs *h);
 /* Per-level sizes: n[l], nnz[l] for l < *nlev (arrays of capacity cap). */
 int eq_amg_info(const eq_ctx *ctx, int *nlev, int64_t *n, int64_t *nnz, int cap);
 /* x = Pl \ b for kni (ldiv!(x, p, b)); x, b in FltNI order. */

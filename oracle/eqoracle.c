@@ -255,6 +255,7 @@ typedef struct {
     int64_t ncoarse;
     double *work[ORC_MAXLEV][3]; /* res, coarse_b, coarse_x */
     int smoother;                /* 0 = symmetric GS */
+    int *split[ORC_MAXLEV];      /* C_NODE / F_NODE of each level with a coarser one */
 } orc_amg;
 
 /* strength: S_ij kept when |a_ij| >= theta * max_{k != i} |a_ik|; diagonal kept */
@@ -440,9 +441,9 @@ orc_amg *orc_amg_build(const orc_csr *A0, double theta, int max_levels, int64_t 
         int *split = malloc(sizeof(int) * ml->A[lev].n);
         rs_split(&S, &T, split);
         direct_interp(&ml->A[lev], &S, split, &ml->P[lev]);
-        free(split);
         orc_csr_free(&S); orc_csr_free(&T);
-        if (ml->P[lev].m == 0 || ml->P[lev].m == ml->A[lev].n) { orc_csr_free(&ml->P[lev]); break; }
+        if (ml->P[lev].m == 0 || ml->P[lev].m == ml->A[lev].n) { orc_csr_free(&ml->P[lev]); free(split); break; }
+        ml->split[lev] = split;
         csr_transpose(&ml->P[lev], &ml->R[lev]);
         orc_csr RA;
         csr_matmul(&ml->R[lev], &ml->A[lev], &RA);
@@ -473,6 +474,7 @@ void orc_amg_free(orc_amg *ml) {
     if (!ml) return;
     for (int l = 0; l < ml->nlev; l++) {
         orc_csr_free(&ml->A[l]); orc_csr_free(&ml->P[l]); orc_csr_free(&ml->R[l]);
+        free(ml->split[l]);
         for (int k = 0; k < 3; k++) free(ml->work[l][k]);
     }
     free(ml->coarse_inv);
@@ -484,15 +486,7 @@ int64_t orc_amg_level_n(const orc_amg *ml, int l) { return ml->A[l].n; }
 int64_t orc_amg_level_nnz(const orc_amg *ml, int l) { return nnz_of(&ml->A[l]); }
 int64_t orc_amg_level_pnnz(const orc_amg *ml, int l) { return (l + 1 < ml->nlev) ? nnz_of(&ml->P[l]) : 0; }
 void orc_amg_get_split(const orc_amg *ml, int l, int *is_c) {
-    /* C points are the rows of P with a single unit entry whose column is new */
-    const orc_csr *P = &ml->P[l];
-    int64_t next = 0;
-    for (int64_t i = 0; i < P->n; i++) {
-        is_c[i] = 0;
-        if (P->rowptr[i + 1] - P->rowptr[i] == 1 && P->val[P->rowptr[i]] == 1.0 && P->col[P->rowptr[i]] == next) {
-            is_c[i] = 1; next++;
-        }
-    }
+    for (int64_t i = 0; i < ml->A[l].n; i++) is_c[i] = ml->split[l][i] == C_NODE;
 }
 /* copy level l's P (or A) out: which 0 = A, 1 = P */
 void orc_amg_get_csr(const orc_amg *ml, int l, int which, int64_t *rowptr, int64_t *col, double *val) {

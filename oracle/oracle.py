@@ -24,9 +24,13 @@ direct sparse solves, mpmath replays of the BigFloat Newton sequence up to
 130 m/s, the Newmark step against the central-difference recurrence on an
 eigenmode and against the equation of motion, slrFunc! against the inverse
 friction law and the exact aging law, the quasi-static step against
-equilibrium, traction and the average-rate rule).  Parity unpinned: the AMG C/F split and
-iteration counts (AlgebraicMultigrid.jl / IterativeSolvers.jl are absent and
-unversioned), and reference full-run trajectories (none are stored).
+equilibrium, traction and the average-rate rule; the Ruge-Stueben set-up against
+the first-pass split invariants, constant-preserving direct interpolation,
+Galerkin coarse operators and the V-cycle's spectrum in tests/test_amg_cpu.py).
+Parity unpinned: which of the admissible C/F splits AlgebraicMultigrid.jl's
+tie-breaking picks, and CG iteration counts (AlgebraicMultigrid.jl /
+IterativeSolvers.jl are absent and unversioned), and reference full-run
+trajectories (none are stored).
 """
 from __future__ import annotations
 
@@ -85,6 +89,8 @@ def lib():
     L.orc_amg_level_pnnz.argtypes = [C.c_void_p, C.c_int]
     L.orc_amg_level_pnnz.restype = C.c_int64
     L.orc_amg_get_csr.argtypes = [C.c_void_p, C.c_int, C.c_int, i64p, i64p, f64p]
+    L.orc_amg_get_split.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int)]
+    L.orc_amg_get_split.restype = None
     L.orc_cg.argtypes = [C.POINTER(CSR), f64p, f64p, C.c_void_p, C.c_double, C.c_int64,
                          C.POINTER(C.c_int64), C.POINTER(C.c_double)]
     L.orc_pcg_jacobi.argtypes = [C.POINTER(CSR), f64p, f64p, f64p, C.c_double, C.c_int64, C.POINTER(C.c_int64)]
@@ -253,6 +259,12 @@ class Amg:
         vl = np.zeros(max(nnz, 1))
         L.orc_amg_get_csr(self.h, l, which, rp, cl, vl)
         return sp.csr_matrix((vl[:nnz], cl[:nnz], rp), shape=(n, m))
+
+    def split(self, l):
+        """C mask of level l's C/F split (l < nlev - 1)."""
+        isc = np.zeros(lib().orc_amg_level_n(self.h, l), np.intc)
+        lib().orc_amg_get_split(self.h, l, isc.ctypes.data_as(C.POINTER(C.c_int)))
+        return isc.astype(bool)
 
     def apply(self, b):
         b = np.ascontiguousarray(b, dtype=np.float64)

@@ -47,16 +47,18 @@ def test_null_context_is_rejected():
             return 0.0
         return None          # pointers and structs by pointer
 
+    skip = ("eq_create", "eq_destroy", "eq_last_error", "eq_rs_free")
     checked = 0
     for name in L.EXPORTS:
-        if name in ("eq_create", "eq_destroy", "eq_last_error"):
+        if name in skip:
             continue
         f = getattr(lib, name)
         assert f(*[dummy(t) for t in f.argtypes]) == L.EQ_EINVAL, name
         checked += 1
-    assert checked == len(L.EXPORTS) - 3
+    assert checked == len(L.EXPORTS) - len(skip)
     assert lib.eq_last_error(None) == b"null context"
     lib.eq_destroy(None)
+    lib.eq_rs_free(None)
 
 
 def _c_struct_fields(hdr, name):
