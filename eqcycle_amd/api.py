@@ -443,6 +443,13 @@ class EqContext:
                                           ptr(Y, C.c_float)))
         return dict(params=params, Phi=Phi, X=X, Y=Y)
 
+    def learn_snapshots(self):
+        """The recorded interior increments, [nsnap x nI] (one snapshot a row)."""
+        m = self.learn_info()["nsnap"]
+        S = np.empty((m, self.nI), np.float32)
+        self._check(self.lib.eq_learn_snapshots(self.h, ptr(S, C.c_float)))
+        return S
+
     def learn_predict(self, dF):
         dF = f64(dF)
         out = np.empty(self.nbasis)

@@ -2710,6 +2710,14 @@ extern "C" int eq_learn_get(eq_ctx* ctx, float* params, float* Phi, float* X, fl
   return sync_check(ctx);
 }
 
+extern "C" int eq_learn_snapshots(eq_ctx* ctx, float* S) {
+  if (!ctx || !S) return EQ_EINVAL;
+  LEARN_READY(ctx);
+  LearnState& L = ctx->learn;
+  HIPCHK(ctx, hipMemcpyAsync(S, L.S, sizeof(float) * L.nI * (int64_t)L.nsnap, hipMemcpyDeviceToHost, ctx->s));
+  return sync_check(ctx);
+}
+
 extern "C" int eq_learn_predict(eq_ctx* ctx, const double* dF, double* coef) {
   if (!ctx || !dF || !coef) return EQ_EINVAL;
   ctx->qs_r_valid = false;

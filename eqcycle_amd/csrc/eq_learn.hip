@@ -118,41 +118,122 @@ __global__ __launch_bounds__(LB) void gemv_rows_kernel(int N, int K, const float
 }
 
 // G = S^T S for 64 x 64 tiles with bi <= bj; FP64 accumulation.
-constexpr int GT = 64, GR = 64;
-__global__ __launch_bounds__(256) void gram_kernel(const float* S, int64_t nI, int m, double* G) {
+// Gram matrix G = S^T S of the snapshot columns (S: nI x m, column-major, FP32)
+// on the matrix cores: V_MFMA_F64_16X16X4_F64 with the FP32 entries widened,
+// so every product is exact and the sums run in FP64 as in an FP64 GEMM.  A
+// workgroup (4 waves) owns a GM x GM tile of the upper triangle; each wave a
+// GW x GW quarter as GB x GB blocks of 16 x 16.  Rows of S stream through LDS
+// in chunks of GK, the next chunk's global loads in flight while the current
+// one feeds the MFMAs.  Few output tiles (a Gram of the 64 basis vectors)
+// split the rows over gridDim.z workgroups per tile, each writing its partial
+// Gram at G + z m^2; gram_sum_kernel adds the partials in z order.
+//   Operand layout of the 16x16x4 FP64 MFMA (lane l, 64 lanes):
+//   A (16 x 4): A[l % 16][l / 16];  B (4 x 16): B[l / 16][l % 16];
+//   D (16 x 16, four FP64 registers r): D[4 r + l / 16][l % 16] (accumulator
+//   groups of one row, unlike the FP32 16x16 forms' groups of four).
+typedef double gram_d4 __attribute__((ext_vector_type(4)));
+constexpr int GM = 128;                      // output tile side
+constexpr int GW = GM / 2, GB = GW / 16;     // a wave's quarter, in 16 x 16 blocks per side
+constexpr int GK = 32;                       // rows of S per LDS chunk
+constexpr int GLD = (GK * GM) / 256;         // loads per thread per operand and chunk
+static_assert(GLD * 256 == GK * GM, "whole loads");
+
+__global__ __launch_bounds__(256) void gram_kernel(const float* S, int64_t nI, int m, int64_t rows_per_split,
+                                                   double* G) {
   const int bi = blockIdx.y, bj = blockIdx.x;
-  if (bi > bj) return;
-  __shared__ float As[GR][GT + 1];
-  __shared__ float Bs[GR][GT + 1];
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;   // 16 x 16 threads, 4 x 4 outputs each
-  double acc[4][4] = {};
-  for (int64_t r0 = 0; r0 < nI; r0 += GR) {
-    for (int e = threadIdx.x; e < GR * GT; e += 256) {
-      const int c = e / GR, rr = e - c * GR;   // rr fastest: contiguous rows of one column
+  if (bi > bj) return;   // block-uniform: the upper triangle only
+  const int64_t rbeg = (int64_t)blockIdx.z * rows_per_split;
+  const int64_t rend = std::min<int64_t>(nI, rbeg + rows_per_split);
+  G += (int64_t)blockIdx.z * m * m;
+  __shared__ float As[GK][GM + 1];
+  __shared__ float Bs[GK][GM + 1];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wi = (w >> 1) * GW, wj = (w & 1) * GW;   // the wave's quarter of the tile
+  const int li = lane & 15, lk = lane >> 4;
+  gram_d4 acc[GB][GB];
+#pragma unroll
+  for (int a = 0; a < GB; a++)
+#pragma unroll
+    for (int b = 0; b < GB; b++) acc[a][b] = gram_d4{0.0, 0.0, 0.0, 0.0};
+  // chunk loads: element e = tid + 256 q is column e / GK, row e % GK of the
+  // chunk (consecutive threads read consecutive rows of one column)
+  float pa[GLD], pb[GLD];
+  auto fetch = [&](int64_t r0) {
+#pragma unroll
+    for (int q = 0; q < GLD; q++) {
+      const int e = tid + 256 * q, c = e / GK, rr = e - c * GK;
       const int64_t row = r0 + rr;
-      const int ca = bi * GT + c, cb = bj * GT + c;
-      As[rr][c] = (row < nI && ca < m) ? S[(int64_t)ca * nI + row] : 0.f;
-      Bs[rr][c] = (row < nI && cb < m) ? S[(int64_t)cb * nI + row] : 0.f;
+      const int ca = bi * GM + c, cb = bj * GM + c;
+      pa[q] = (row < rend && ca < m) ? S[(int64_t)ca * nI + row] : 0.f;
+      pb[q] = (row < rend && cb < m) ? S[(int64_t)cb * nI + row] : 0.f;
+    }
+  };
+  fetch(rbeg);
+  for (int64_t r0 = rbeg; r0 < rend; r0 += GK) {
+#pragma unroll
+    for (int q = 0; q < GLD; q++) {
+      const int e = tid + 256 * q, c = e / GK, rr = e - c * GK;
+      As[rr][c] = pa[q];
+      Bs[rr][c] = pb[q];
     }
     __syncthreads();
-    for (int rr = 0; rr < GR; rr++) {
-      float a[4], b[4];
+    if (r0 + GK < rend) fetch(r0 + GK);
 #pragma unroll
-      for (int q = 0; q < 4; q++) { a[q] = As[rr][ty * 4 + q]; b[q] = Bs[rr][tx * 4 + q]; }
+    for (int ks = 0; ks < GK / 4; ks++) {
+      const int k = 4 * ks + lk;
+      double a[GB], b[GB];
 #pragma unroll
-      for (int p = 0; p < 4; p++)
+      for (int t = 0; t < GB; t++) {
+        a[t] = (double)As[k][wi + 16 * t + li];
+        b[t] = (double)Bs[k][wj + 16 * t + li];
+      }
 #pragma unroll
-        for (int q = 0; q < 4; q++) acc[p][q] += (double)a[p] * (double)b[q];
+      for (int ta = 0; ta < GB; ta++)
+#pragma unroll
+        for (int tb = 0; tb < GB; tb++)
+          acc[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ta], b[tb], acc[ta][tb], 0, 0, 0);
     }
     __syncthreads();
   }
 #pragma unroll
-  for (int p = 0; p < 4; p++)
+  for (int ta = 0; ta < GB; ta++)
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const int i = bi * GT + ty * 4 + p, j = bj * GT + tx * 4 + q;
-      if (i < m && j < m) G[(int64_t)i * m + j] = acc[p][q];
-    }
+    for (int tb = 0; tb < GB; tb++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = bi * GM + wi + 16 * ta + 4 * r + lk, j = bj * GM + wj + 16 * tb + li;
+        if (i < m && j < m) G[(int64_t)i * m + j] = acc[ta][tb][r];
+      }
+}
+
+// G[i, j] = sum_z part[z][i, j] (z in order) on the upper triangle
+__global__ void gram_sum_kernel(int m, int nz, const double* part, double* G) {
+  const int64_t e = (int64_t)blockIdx.x * LB + threadIdx.x;
+  if (e >= (int64_t)m * m) return;
+  const int i = (int)(e / m), j = (int)(e - (int64_t)i * m);
+  if (i > j) return;
+  double s = 0.0;
+  for (int z = 0; z < nz; z++) s += part[(int64_t)z * m * m + e];
+  G[e] = s;
+}
+
+// Upper triangle of S^T S (S: nI x m column-major) into G (m x m, row-major,
+// device); partials from the row split in scratch (nz m^2 doubles, nz from
+// gram_splits) when more than one.
+inline int gram_splits(int64_t nI, int m) {
+  const int nt = (m + GM - 1) / GM, tiles = nt * (nt + 1) / 2;
+  int nz = 1;
+  while (tiles * nz < 512 && (nI + GK - 1) / GK >= 32 * (int64_t)nz * 2) nz *= 2;
+  return nz;
+}
+hipError_t launch_gram(const float* S, int64_t nI, int m, double* G, double* part, hipStream_t s) {
+  const int nt = (m + GM - 1) / GM, nz = gram_splits(nI, m);
+  int64_t rps = (nI + nz - 1) / nz;
+  rps = (rps + GK - 1) / GK * GK;
+  hipLaunchKernelGGL(gram_kernel, dim3(nt, nt, nz), dim3(256), 0, s, S, nI, m, rps, nz > 1 ? part : G);
+  if (nz > 1)
+    hipLaunchKernelGGL(gram_sum_kernel, dim3(gblk((int64_t)m * m)), dim3(LB), 0, s, m, nz, part, G);
+  return hipGetLastError();
 }
 
 // Phi[:, k] = sum_j S[:, j] W[j, k]  (W = V diag(1/sigma), row-major [m x NB])
@@ -646,13 +727,13 @@ int learn_norms(LearnState* ls, double* out, hipStream_t s) {
 int learn_gram(LearnState* ls, double* G_host, hipStream_t s, std::string* err) {
   const int m = ls->nsnap;
   if (m < 1) { *err = "no snapshots"; return 6; }
-  double* G = dalloc<double>((int64_t)m * m);
+  const int nz = gram_splits(ls->nI, m);
+  double* G = dalloc<double>((int64_t)m * m * (nz > 1 ? nz + 1 : 1));
   if (!G) { *err = "device allocation failed"; return 7; }
-  const int nt = (m + GT - 1) / GT;
-  hipLaunchKernelGGL(gram_kernel, dim3(nt, nt), dim3(256), 0, s, ls->S, ls->nI, m, G);
   std::vector<double> h((int64_t)m * m);
-  LCHK(hipMemcpyAsync(h.data(), G, sizeof(double) * m * m, hipMemcpyDeviceToHost, s));
-  const hipError_t e = hipStreamSynchronize(s);
+  hipError_t e = launch_gram(ls->S, ls->nI, m, G, G + (int64_t)m * m, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(h.data(), G, sizeof(double) * m * m, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
   (void)hipFree(G);
   if (e != hipSuccess) { *err = hipGetErrorString(e); return 2; }
   for (int i = 0; i < m; i++)
@@ -682,12 +763,13 @@ int learn_set_basis(LearnState* ls, const double* V, const double* sigma, const 
   std::vector<double> Winv((size_t)NB * NB, 0.0);
   bool reortho = false;
   {
-    double* G = dalloc<double>((int64_t)NB * NB);
+    const int nz = gram_splits(ls->nI, NB);
+    double* G = dalloc<double>((int64_t)NB * NB * (nz > 1 ? nz + 1 : 1));
     if (!G) { (void)hipFree(dW); *err = "device allocation failed"; return 7; }
-    const int nt = (NB + GT - 1) / GT;
-    hipLaunchKernelGGL(gram_kernel, dim3(nt, nt), dim3(256), 0, s, ls->Phi, ls->nI, NB, G);
+    const hipError_t e0 = launch_gram(ls->Phi, ls->nI, NB, G, G + (int64_t)NB * NB, s);
     std::vector<double> h((size_t)NB * NB);
-    const hipError_t e1 = hipMemcpyAsync(h.data(), G, sizeof(double) * NB * NB, hipMemcpyDeviceToHost, s);
+    const hipError_t e1 = e0 != hipSuccess ? e0 : hipMemcpyAsync(h.data(), G, sizeof(double) * NB * NB,
+                                                                 hipMemcpyDeviceToHost, s);
     const hipError_t e2 = hipStreamSynchronize(s);
     (void)hipFree(G);
     if (e1 != hipSuccess || e2 != hipSuccess) { (void)hipFree(dW); *err = "Gram of the basis failed"; return 2; }

@@ -35,7 +35,7 @@ EXPORTS = [
     "eq_learn_gram", "eq_learn_set_basis", "eq_learn_train", "eq_learn_loss_grad", "eq_learn_get",
     "eq_learn_predict", "eq_learn_norms", "eq_learn_galerkin", "eq_newmark_step", "eq_amg_upload",
     "eq_launch_timing", "eq_launch_timing_get", "eq_newmark_step_noalloc", "eq_learn_set_linear",
-    "eq_learn_predict_linear", "eq_rs_build", "eq_rs_info", "eq_rs_level", "eq_rs_split", "eq_rs_free",
+    "eq_learn_predict_linear", "eq_learn_snapshots", "eq_rs_build", "eq_rs_info", "eq_rs_level", "eq_rs_split", "eq_rs_free",
 ]
 EQ_TIME_KU, EQ_TIME_CG_DIR, EQ_TIME_VCYCLE, EQ_TIME_CG_ITER, EQ_TIME_NEWMARK, EQ_TIME_CHEB, EQ_TIME_COPY, \
     EQ_TIME_CHEB2 = range(8)
@@ -157,6 +157,7 @@ def load():
         "eq_learn_train": ([vp, C.c_int, C.c_int, C.c_double, C.c_uint64, dp], C.c_int),
         "eq_learn_loss_grad": ([vp, fp, C.c_int, i32p, C.c_int, dp, fp], C.c_int),
         "eq_learn_get": ([vp, fp, fp, fp, fp], C.c_int),
+        "eq_learn_snapshots": ([vp, fp], C.c_int),
         "eq_learn_predict": ([vp, dp, dp], C.c_int),
         "eq_learn_set_linear": ([vp, fp, fp], C.c_int),
         "eq_learn_predict_linear": ([vp, dp, dp], C.c_int),

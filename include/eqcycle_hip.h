@@ -321,6 +321,9 @@ int eq_learn_loss_grad(eq_ctx *ctx, const float *params, int nsamp, const int32_
 /* Copies of parameters [nparams], basis [nI x nbasis] column-major, dataset
  * inputs [nsnap x FltNglob] and targets [nsnap x nbasis]; any may be NULL. */
 int eq_learn_get(eq_ctx *ctx, float *params, float *Phi, float *X, float *Y);
+/* Copy of the recorded interior increments S [nI x nsnap], column-major (one
+ * snapshot after the other), whose Gram eq_learn_gram returns. */
+int eq_learn_snapshots(eq_ctx *ctx, float *S);
 /* Network output (POD coefficients times |dF|) for one fault increment. */
 int eq_learn_predict(eq_ctx *ctx, const double *dF, double *coef);
 /* The linear least-squares predictor of the same coefficients from the same

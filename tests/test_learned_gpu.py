@@ -36,6 +36,21 @@ def test_pod_basis_is_orthonormal(trained):
     assert fit["energy_captured"] > 0.99
 
 
+def test_gram_on_matrix_cores_matches_fp64_product(trained):
+    """eq_learn_gram (FP64 MFMA over the widened FP32 snapshots) against the
+    FP64 product S^T S: every product is exact, so only the summation order
+    differs; the snapshot count spans several 64-column tiles, so off-diagonal
+    tiles and the operand layout are exercised."""
+    P, ctx, fit = trained
+    S = ctx.learn_snapshots().astype(np.float64)
+    G = ctx.learn_gram()
+    ref = S @ S.T
+    print(f"[gpu] Gram {G.shape}, max rel diff {np.max(np.abs(G - ref)) / np.max(np.abs(ref)):.2e}", flush=True)
+    assert S.shape[0] > 64
+    assert np.max(np.abs(G - ref)) <= 1e-13 * np.max(np.abs(ref))
+    assert np.array_equal(G, G.T)
+
+
 def test_gpu_gradient_matches_fp64_reference(trained):
     from oracle.learned_ref import NetRef
     P, ctx, _ = trained
