@@ -446,8 +446,11 @@ extern "C" int eq_create(const eq_mesh_desc* mesh, eq_ctx** out) {
     *out = ctx.release();
     return fail(*out, EQ_NOMEM, "device allocation failed");
   }
-  if (hipHostMalloc((void**)&c->sc_host, sizeof(CgScal), hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc((void**)&c->st_host, sizeof(StepStatus) * eq_ctx::kBatch, hipHostMallocDefault) != hipSuccess ||
+  // sc_host and st_host (and rec_seis_host) are also written by kernels
+  // directly (coherent: no GPU caching), read by the host after a
+  // synchronisation
+  if (hipHostMalloc((void**)&c->sc_host, sizeof(CgScal), hipHostMallocCoherent) != hipSuccess ||
+      hipHostMalloc((void**)&c->st_host, sizeof(StepStatus) * eq_ctx::kBatch, hipHostMallocCoherent) != hipSuccess ||
       hipHostMalloc((void**)&c->vpart_host, sizeof(double) * eq_ctx::kBatch * 2 * nbx, hipHostMallocDefault) !=
           hipSuccess ||
       hipHostMalloc((void**)&c->nrflags_host, sizeof(int) * 2 * eq_ctx::kBatch, hipHostMallocDefault) !=
@@ -612,7 +615,7 @@ extern "C" int eq_set_params(eq_ctx* ctx, const eq_params_desc* p) {
     if (ctx->rec_seis_host) (void)hipHostFree(ctx->rec_seis_host);
     ctx->rec_seis_host = nullptr;
     if (hipHostMalloc((void**)&ctx->rec_seis_host, sizeof(double) * eq_ctx::kBatch * 3 * nseis,
-                      hipHostMallocDefault) != hipSuccess)
+                      hipHostMallocCoherent) != hipSuccess)
       return fail(ctx, EQ_NOMEM, "pinned allocation failed");
     ctx->seis_cap = nseis;
   }
@@ -1239,13 +1242,12 @@ static int cg_begin(eq_ctx* ctx, T* x, const T* b, double tol, int64_t maxiter, 
   return EQ_OK;
 }
 
-// A batch of iterations, the convergence test and the copy of the scalars to
-// the host; nothing waits.
+// A batch of iterations and the convergence test, which also leaves the
+// scalars in the pinned host copy; nothing waits.
 template <typename T>
 static int cg_batch(eq_ctx* ctx, T* x, int pc, bool f32pc, int64_t batch) {
   if (int rc = enqueue_iterations<T>(ctx, x, pc, f32pc, batch)) return rc;
-  launch_cg_check(ctx->sc, ctx->part_rr, ctx->npart, ctx->s);
-  HIPCHK(ctx, hipMemcpyAsync(ctx->sc_host, ctx->sc, sizeof(CgScal), hipMemcpyDeviceToHost, ctx->s));
+  launch_cg_check(ctx->sc, ctx->part_rr, ctx->npart, ctx->s, ctx->sc_host);
   return EQ_OK;
 }
 
@@ -1868,24 +1870,25 @@ extern "C" int eq_state_set(eq_ctx* ctx, const double* d, const double* v, const
 }
 
 // Records of slots 0..k-1 (status and receiver samples) to the host.
-static int read_status(eq_ctx* ctx, int k) {
-  HIPCHK(ctx, hipMemcpyAsync(ctx->st_host, ctx->st, sizeof(StepStatus) * k, hipMemcpyDeviceToHost, ctx->s));
-  const int64_t ns = (int64_t)ctx->out_seis.size();
-  if (ns > 0)
-    HIPCHK(ctx, hipMemcpyAsync(ctx->rec_seis_host, ctx->rec_seis, sizeof(double) * 3 * ns * k,
-                               hipMemcpyDeviceToHost, ctx->s));
-  return spin_sync(ctx);
-}
-
-static StepCapture capture(eq_ctx* ctx, int slot) {
+// mirror: the step tail also writes its status and receiver samples into the
+// pinned host copies (slot 0), which sync_status then leaves in place
+static StepCapture capture(eq_ctx* ctx, int slot, bool mirror = false) {
   StepCapture c;
   const int64_t ns = (int64_t)ctx->out_seis.size();
   c.fault = ctx->rec_fault + (int64_t)slot * 3 * ctx->NX;
   c.seis_idx = ctx->seis_idx;
   c.nseis = (int)ns;
   c.seis = ns > 0 ? ctx->rec_seis + (int64_t)slot * 3 * ns : nullptr;
+  if (mirror) {
+    c.seis_mirror = ns > 0 ? ctx->rec_seis_host : nullptr;
+    c.st_mirror = ctx->st_host;
+  }
   return c;
 }
+
+// The status of a step whose tail wrote the pinned copies (capture with
+// mirror): only the synchronisation.
+static int sync_status(eq_ctx* ctx) { return spin_sync(ctx); }
 
 static void clean_flags(eq_ctx* ctx) {
   if (!ctx->flags_dirty) return;
@@ -2010,10 +2013,10 @@ extern "C" int eq_step(eq_ctx* ctx, eq_step_info* info) {
   int rc = (ctx->isolver == 1) ? qs_step(ctx, info, r_valid) : dyn_step(ctx);
   if (rc) return rc;
   launch_step_tail(ctx->fp, ctx->d, ctx->v, ctx->a, ctx->fs.tau, ctx->dt, ctx->dt0, ctx->isolver, ctx->st,
-                   ctx->flags, capture(ctx, 0), ctx->s);
+                   ctx->flags, capture(ctx, 0, true), ctx->s);
   const bool spec = ctx->spec_pending;
   ctx->spec_pending = false;
-  if (int rc2 = read_status(ctx, 1)) return rc2;
+  if (int rc2 = sync_status(ctx)) return rc2;
   if (spec) {
     // pass 2's scalars came back with the status
     if (!ctx->sc_host->done) {
@@ -2029,8 +2032,8 @@ extern "C" int eq_step(eq_ctx* ctx, eq_step_info* info) {
       launch_qs_finish(ctx->g, ctx->fp, ctx->fs, ctx->d, ctx->dPre, ctx->v, ctx->a, ctx->dt, s);
       // (the first step tail left the flags cleared; nothing in this step sets them)
       launch_step_tail(ctx->fp, ctx->d, ctx->v, ctx->a, ctx->fs.tau, ctx->dt, ctx->dt0, ctx->isolver, ctx->st,
-                       ctx->flags, capture(ctx, 0), s);
-      if (int rc4 = read_status(ctx, 1)) return rc4;
+                       ctx->flags, capture(ctx, 0, true), s);
+      if (int rc4 = sync_status(ctx)) return rc4;
     }
     ctx->cg_slot = 1;
     int64_t it = 0;

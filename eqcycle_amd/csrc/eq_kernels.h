@@ -68,6 +68,8 @@ struct StepCapture {
   const int64_t* seis_idx = nullptr;  // structured node index of each receiver
   int nseis = 0;
   double* seis = nullptr;             // [3 * nseis] or null
+  double* seis_mirror = nullptr;      // pinned host copy of seis, or null
+  StepStatus* st_mirror = nullptr;    // pinned host copy of the status, or null
 };
 
 inline dim3 tile_grid(const Geom& g) {
@@ -92,7 +94,8 @@ void launch_cg_residual(const Geom& g, const T* x, const T* mu, const T* b, T* r
                         double* part_bb, int npart, hipStream_t s, float* r32 = nullptr);
 void launch_cg_init_scal(CgScal* sc, const double* part_rr, const double* part_bb, int nparts, double tol,
                          int maxiter, hipStream_t s);
-void launch_cg_check(CgScal* sc, const double* part_rr, int nparts, hipStream_t s);
+// mirror: pinned host copy of the scalars, written after the check (or null)
+void launch_cg_check(CgScal* sc, const double* part_rr, int nparts, hipStream_t s, CgScal* mirror = nullptr);
 template <typename T>
 void launch_cg_dir(const Geom& g, const T* c, const T* u_old, const T* mu, T* u_new, T* q, const double* part_rho,
                    int nrho, const CgScal* sc, double* part_uq, hipStream_t s, LaunchEvents le = {});

@@ -1178,14 +1178,18 @@ __global__ void cg_init_scal_kernel(CgScal* sc, const double* part_rr, const dou
   }
 }
 
-__global__ void cg_check_kernel(CgScal* sc, const double* part_rr, int nparts) {
-  if (sc->done) return;
-  const double rr = reduce_parts(part_rr, nparts);
-  if (threadIdx.x == 0) {
-    const double resid = sqrt(rr);
-    sc->resid = resid;
-    if (resid <= sc->reltol || sc->iter >= sc->maxiter) sc->done = 1;
+// With a mirror (pinned host memory) the scalars also go there, so the host
+// reads them after its synchronisation with no copy on the stream.
+__global__ void cg_check_kernel(CgScal* sc, const double* part_rr, int nparts, CgScal* mirror) {
+  if (!sc->done) {   // block-uniform
+    const double rr = reduce_parts(part_rr, nparts);
+    if (threadIdx.x == 0) {
+      const double resid = sqrt(rr);
+      sc->resid = resid;
+      if (resid <= sc->reltol || sc->iter >= sc->maxiter) sc->done = 1;
+    }
   }
+  if (mirror && threadIdx.x == 0) *mirror = *sc;
 }
 
 // --------------------------------------------------------------------------
@@ -1453,19 +1457,27 @@ __global__ __launch_bounds__(BLOCK) void step_tail_kernel(FaultDev fp, const dou
   if (cap.seis) {
     for (int q = threadIdx.x; q < cap.nseis; q += BLOCK) {
       const int64_t n = cap.seis_idx[q];
-      cap.seis[3 * q] = d[n];
-      cap.seis[3 * q + 1] = v[n];
-      cap.seis[3 * q + 2] = a[n];
+      const double dn = d[n], vn = v[n], an = a[n];
+      cap.seis[3 * q] = dn;
+      cap.seis[3 * q + 1] = vn;
+      cap.seis[3 * q + 2] = an;
+      if (cap.seis_mirror) {
+        cap.seis_mirror[3 * q] = dn;
+        cap.seis_mirror[3 * q + 1] = vn;
+        cap.seis_mirror[3 * q + 2] = an;
+      }
     }
   }
   if (threadIdx.x == 0) {
-    const int nr_flag = (flags[0] == INT_MAX) ? 0 : flags[0];
-    st->Vfmax = Vfmax;
-    st->dt_next = dtn;
-    st->isolver_next = next;
-    st->hypo = mi;
-    st->nr_flag = nr_flag;
-    st->nr_noconv = flags[1];
+    StepStatus o{};
+    o.Vfmax = Vfmax;
+    o.dt_next = dtn;
+    o.isolver_next = next;
+    o.hypo = mi;
+    o.nr_flag = (flags[0] == INT_MAX) ? 0 : flags[0];
+    o.nr_noconv = flags[1];
+    *st = o;
+    if (cap.st_mirror) *cap.st_mirror = o;
     flags[0] = INT_MAX;
     flags[1] = 0;
   }
@@ -1684,8 +1696,8 @@ void launch_cg_init_scal(CgScal* sc, const double* part_rr, const double* part_b
   hipLaunchKernelGGL(cg_init_scal_kernel, dim3(1), dim3(BLOCK), 0, s, sc, part_rr, part_bb, nparts, tol, maxiter);
 }
 
-void launch_cg_check(CgScal* sc, const double* part_rr, int nparts, hipStream_t s) {
-  hipLaunchKernelGGL(cg_check_kernel, dim3(1), dim3(BLOCK), 0, s, sc, part_rr, nparts);
+void launch_cg_check(CgScal* sc, const double* part_rr, int nparts, hipStream_t s, CgScal* mirror) {
+  hipLaunchKernelGGL(cg_check_kernel, dim3(1), dim3(BLOCK), 0, s, sc, part_rr, nparts, mirror);
 }
 
 template <typename T>
