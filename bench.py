@@ -143,7 +143,7 @@ def one_pass(ctx, ck, t_end):
 def variant(ctx, ck, tol, prec, guess, warmup, steps, graphs):
     """warmup + steps passes over the cycle; statistics of the timed ones."""
     import numpy as np
-    ctx.set_solver(tol=tol, pc="amg", prec=prec, guess=guess, graphs=bool(graphs))
+    ctx.set_solver(tol=tol, pc="amg", prec=prec, guess=guess, graphs=bool(graphs), basis_fp16=guess == "galerkin")
     t_end = ck["t"] + SEGMENT_YEARS * YR
     for _ in range(warmup):
         one_pass(ctx, ck, t_end)
@@ -179,7 +179,7 @@ def kernel_time(ctx, ck, tol, prec, guess, P, wall_pass):
     """One more pass of the variant with every fine-level launch bracketed by
     HIP events (CG batches launch eagerly: same kernels); device time per
     kernel, the dominant one, and its roofline."""
-    ctx.set_solver(tol=tol, pc="amg", prec=prec, guess=guess, graphs=False)
+    ctx.set_solver(tol=tol, pc="amg", prec=prec, guess=guess, graphs=False, basis_fp16=guess == "galerkin")
     ctx.launch_timing(True)
     w, R = one_pass(ctx, ck, ck["t"] + SEGMENT_YEARS * YR)
     kt = ctx.launch_timing_get()
@@ -371,7 +371,8 @@ def main():
                                 f"{cfg.material} damage zone, CG tol {tol:g}; one step = one earthquake cycle "
                                 f"(years {ck['t'] / YR:.1f} to {ck['t'] / YR + SEGMENT_YEARS:.1f} from the plain "
                                 f"solver's checkpoint, third event cluster inside); "
-                                + ("value: x0 from the learned POD basis (Galerkin coefficients) + FP32 V-cycle, "
+                                + ("value: x0 from the learned POD basis (Galerkin coefficients, expansion from "
+                                   "its FP16 copy) + FP32 V-cycle, "
                                    "plain FP64, network-predicted and linear-predicted x0 alongside" if learned
                                    else "plain FP64")),
                    "pcg_iters_per_solve": main_st["pcg_iters_per_solve"], "hierarchy": a.hierarchy,

@@ -157,7 +157,8 @@ def main():
             if name == "plain":
                 ctx.set_solver(tol=tol, pc="amg", prec="fp64", guess="previous", graphs=bool(a.graphs))
             else:
-                ctx.set_solver(tol=tol, pc="amg", prec=cfg.precision, guess="galerkin", graphs=bool(a.graphs))
+                ctx.set_solver(tol=tol, pc="amg", prec=cfg.precision, guess="galerkin", graphs=bool(a.graphs),
+                               basis_fp16=True)
             if window == "learned":
                 ctx.state_set(d=S["d"], v=S["v"], a=S["a"], psi=S["psi"], tau=S["tau"], t=S["t"], dt=S["dt"],
                               isolver=S["isolver"])

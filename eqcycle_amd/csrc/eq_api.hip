@@ -1932,7 +1932,7 @@ static void galerkin_correction(eq_ctx* ctx, double* x, const double* b, const d
     launch_cg_residual<double>(ctx->g, x, ctx->mu, b, w.r, ctx->part_rr, ctx->part_bb, ctx->npart, ctx->s);
     learn_project(&ctx->learn, w.r, 1, ctx->s);
   }
-  learn_expand_add(&ctx->learn, x, ctx->s);
+  learn_expand_add(&ctx->learn, x, ctx->s, ctx->opts.basis_fp16 != 0);
   ctx->n_learned_solves++;
 }
 
@@ -1963,11 +1963,12 @@ static int qs_step(eq_ctx* ctx, eq_step_info* info, bool r_valid) {
       if (!rc) rc = cg_batch<double>(ctx, ctx->d, ctx->opts.pc, f32pc, ctx->pred_iters[1]);
       ctx->cg_slot = 2;
       if (rc) return rc;
-      launch_copy3(ctx->NX, ctx->fs.psi, ctx->fs.Vf, ctx->d, ctx->spec_save, true, s);
+      // the fault update saves psi, Vf and the fault row of d first (restored
+      // if the batch falls short)
       ctx->spec_pending = true;
       ctx->spec_f32pc = f32pc;
       ctx->spec_batch = ctx->pred_iters[1];
-      launch_qs_fault(g, ctx->fp, ctx->fs, ctx->mu, ctx->d, ctx->v, dt, s);
+      launch_qs_fault(g, ctx->fp, ctx->fs, ctx->mu, ctx->d, ctx->v, dt, s, ctx->spec_save);
       continue;
     }
     ctx->cg_slot = pass;

@@ -212,8 +212,9 @@ void launch_newmark_predict(int64_t n, const double* d, const double* v, const d
 void launch_qs_begin(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, double* d, double* dPre,
                      const double* v, double* b, double dt, int first_pass, hipStream_t s,
                      double* db = nullptr);
+// save: psi, Vf and the fault row of d before the update (3 nflt), or null
 void launch_qs_fault(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, const double* d,
-                     double* v, double dt, hipStream_t s);
+                     double* v, double dt, hipStream_t s, double* save = nullptr);
 void launch_qs_finish(const Geom& g, const FaultDev& fp, FaultState fs, double* d, const double* dPre, double* v,
                       double* a, double dt, hipStream_t s);
 // Vfmax, regime switch, dtevol!, Newton flags (then reset for the next step),

@@ -1319,10 +1319,17 @@ __global__ __launch_bounds__(BLOCK) void qs_begin_kernel(Geom g, FaultDev fp, Fa
 }
 
 // tau1 = -(K d)[fault]/FltB (0 on creeping nodes), slrFunc!, Vf and fault v update
+// save (or null): [psi | Vf | fault row of d], each nflt long, as they were
+// before this update (the state a redone second pass restores)
 __global__ __launch_bounds__(BLOCK) void qs_fault_kernel(Geom g, FaultDev fp, FaultState fs, const double* mu,
-                                                         const double* d, double* v, double dt) {
+                                                         const double* d, double* v, double dt, double* save) {
   const int j = blockIdx.x * BLOCK + threadIdx.x;
   if (j >= fp.nflt) return;
+  if (save) {
+    save[j] = fs.psi[j];
+    save[fp.nflt + j] = fs.Vf[j];
+    save[2 * fp.nflt + j] = d[j];
+  }
   const bool deep = j < fp.j0;
   const double a = deep ? 0.0 : ku_global<double>(g, d, mu, j, 0);
   const double tau1 = -a / fp.FltB[j];
@@ -2188,8 +2195,8 @@ void launch_qs_begin(const Geom& g, const FaultDev& fp, FaultState fs, const dou
 }
 
 void launch_qs_fault(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, const double* d, double* v,
-                     double dt, hipStream_t s) {
-  hipLaunchKernelGGL(qs_fault_kernel, dim3(nblk(fp.nflt)), dim3(BLOCK), 0, s, g, fp, fs, mu, d, v, dt);
+                     double dt, hipStream_t s, double* save) {
+  hipLaunchKernelGGL(qs_fault_kernel, dim3(nblk(fp.nflt)), dim3(BLOCK), 0, s, g, fp, fs, mu, d, v, dt, save);
 }
 
 void launch_qs_finish(const Geom& g, const FaultDev& fp, FaultState fs, double* d, const double* dPre, double* v,

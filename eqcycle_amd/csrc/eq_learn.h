@@ -61,6 +61,7 @@ struct LearnState {
   double* gal_c = nullptr;      // [NB] projections / coefficients
   double* gal_Ginv = nullptr;   // [NB x NB] row-major inverse of Phi^T K_II Phi
   float* Phi_rm = nullptr;      // [nI x NB] row-major copy of Phi (projection)
+  _Float16* Phi16 = nullptr;    // [nI x NB] FP16 copy of Phi for the expansion, or null
   double* gal_T = nullptr;      // [NB x gal_trows] Ginv Phi^T on the first gal_trows interior rows
   int64_t gal_trows = 0;
   int gal_nblk = 0;
@@ -108,7 +109,8 @@ void learn_project(LearnState* ls, const double* r, int solve, hipStream_t s);
 // beyond the first nrows interior rows (v holds just those rows)
 void learn_project_rows(LearnState* ls, const double* v, int64_t nrows, hipStream_t s);
 // x[off:] += Phi gal_c
-void learn_expand_add(LearnState* ls, double* x, hipStream_t s);
+// fp16: read the FP16 copy of the basis (when set)
+void learn_expand_add(LearnState* ls, double* x, hipStream_t s, bool fp16 = false);
 // out = column k of Phi as an N-vector (out[0:off] = 0)
 void learn_basis_column(LearnState* ls, int k, double* out, hipStream_t s);
 int learn_get_proj(LearnState* ls, double* host, hipStream_t s);

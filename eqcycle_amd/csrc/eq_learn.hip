@@ -579,6 +579,11 @@ __global__ __launch_bounds__(LB) void gal_coef_kernel(int64_t nrows, const doubl
   if (threadIdx.x == 0) c[k] = a;
 }
 
+__global__ void to_f16_kernel(int64_t n, const float* x, _Float16* y) {
+  const int64_t e = (int64_t)blockIdx.x * LB + threadIdx.x;
+  if (e < n) y[e] = (_Float16)x[e];
+}
+
 // Phi_rm[i, k] = Phi[k, i]
 __global__ void phi_rowmajor_kernel(int64_t nI, int NB, const float* Phi, float* Phi_rm) {
   const int64_t e = (int64_t)blockIdx.x * LB + threadIdx.x;
@@ -589,7 +594,8 @@ __global__ void phi_rowmajor_kernel(int64_t nI, int NB, const float* Phi, float*
 }
 
 // x[off + i] += sum_k Phi[k, i] c[k]
-__global__ __launch_bounds__(LB) void expand_add_kernel(int64_t nI, int64_t off, int NB, const float* Phi,
+template <typename PT>
+__global__ __launch_bounds__(LB) void expand_add_kernel(int64_t nI, int64_t off, int NB, const PT* Phi,
                                                         const double* c, double* x) {
   __shared__ double cs[NBMAX];
   if (threadIdx.x < NB) cs[threadIdx.x] = c[threadIdx.x];
@@ -667,8 +673,9 @@ int learn_init(LearnState* ls, int64_t nI, int64_t nflt, int64_t offset, int cap
   ls->gal_part = dalloc<double>((int64_t)ls->gal_nblk * NB);
   ls->gal_c = dalloc<double>(2 * NBMAX);   // [0, NBMAX): coefficients, [NBMAX, 2 NBMAX): projections
   ls->Phi_rm = dalloc<float>(nI * (int64_t)NB);
+  ls->Phi16 = dalloc<_Float16>(nI * (int64_t)NB);
   ls->gal_Ginv = dalloc<double>((int64_t)NB * NB);
-  if (!ls->gal_part || !ls->gal_c || !ls->gal_Ginv || !ls->Phi_rm || !ls->norms || !ls->S || !ls->F || !ls->Phi || !ls->X || !ls->Y || !ls->w32 || !ls->w16 || !ls->grad || !ls->adam_m ||
+  if (!ls->gal_part || !ls->gal_c || !ls->gal_Ginv || !ls->Phi_rm || !ls->Phi16 || !ls->norms || !ls->S || !ls->F || !ls->Phi || !ls->X || !ls->Y || !ls->w32 || !ls->w16 || !ls->grad || !ls->adam_m ||
       !ls->adam_v || !ls->inf_act || !ls->coef) {
     *err = "device allocation failed (learned guess)";
     learn_free(ls);
@@ -702,7 +709,7 @@ int learn_init(LearnState* ls, int64_t nI, int64_t nflt, int64_t offset, int cap
 void learn_free(LearnState* ls) {
   void* ptrs[] = {ls->S, ls->F, ls->Phi, ls->X, ls->Y, ls->w32, ls->w16, ls->grad, ls->adam_m, ls->adam_v,
                   ls->act, ls->dact, ls->out, ls->idx, ls->loss_part, ls->inf_act, ls->coef, ls->norms,
-                  ls->gal_part, ls->gal_c, ls->gal_Ginv, ls->Phi_rm, ls->gal_T, ls->lin_A, ls->lin_b};
+                  ls->gal_part, ls->gal_c, ls->gal_Ginv, ls->Phi_rm, ls->gal_T, ls->lin_A, ls->lin_b, ls->Phi16};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   NetDims d = ls->d;
@@ -804,6 +811,9 @@ int learn_set_basis(LearnState* ls, const double* V, const double* sigma, const 
     }
   }
   hipLaunchKernelGGL(phi_rowmajor_kernel, dim3(gblk(ls->nI * NB)), dim3(LB), 0, s, ls->nI, NB, ls->Phi, ls->Phi_rm);
+  // the FP16 copy the Galerkin expansion may read (eq_solver_opts.basis_fp16)
+  hipLaunchKernelGGL(to_f16_kernel, dim3(gblk(ls->nI * NB)), dim3(LB), 0, s, ls->nI * (int64_t)NB, ls->Phi,
+                     ls->Phi16);
   // dataset: X_j = dF_j / |dF_j|, Y_j = Phi^T s_j / |dF_j| = Winv^T (sigma_k V_jk) / |dF_j|
   std::vector<float> F((int64_t)m * L0), X((int64_t)m * L0), Y((int64_t)m * NB);
   LCHK(hipMemcpyAsync(F.data(), ls->F, sizeof(float) * F.size(), hipMemcpyDeviceToHost, s));
@@ -1084,9 +1094,13 @@ void learn_project_rows(LearnState* ls, const double* v, int64_t nrows, hipStrea
                      ls->gal_c);
 }
 
-void learn_expand_add(LearnState* ls, double* x, hipStream_t s) {
-  hipLaunchKernelGGL(expand_add_kernel, dim3(gblk(ls->nI)), dim3(LB), 0, s, ls->nI, ls->offset, ls->d.NB, ls->Phi,
-                     ls->gal_c, x);
+void learn_expand_add(LearnState* ls, double* x, hipStream_t s, bool fp16) {
+  if (fp16 && ls->Phi16)
+    hipLaunchKernelGGL(expand_add_kernel<_Float16>, dim3(gblk(ls->nI)), dim3(LB), 0, s, ls->nI, ls->offset, ls->d.NB,
+                       ls->Phi16, ls->gal_c, x);
+  else
+    hipLaunchKernelGGL(expand_add_kernel<float>, dim3(gblk(ls->nI)), dim3(LB), 0, s, ls->nI, ls->offset, ls->d.NB,
+                       ls->Phi, ls->gal_c, x);
 }
 
 void learn_basis_column(LearnState* ls, int k, double* out, hipStream_t s) {

@@ -43,7 +43,7 @@ def main():
     ck, _ = bench.checkpoint(ctx, cfg, tol, cfg.learned, 300, a.graphs)
     guess = {"plain": "previous", "network": "learned", "linear": "linear", "galerkin": "galerkin"}[a.variant]
     prec = "fp64" if a.variant == "plain" else cfg.precision
-    ctx.set_solver(tol=tol, pc="amg", prec=prec, guess=guess, graphs=bool(a.graphs))
+    ctx.set_solver(tol=tol, pc="amg", prec=prec, guess=guess, graphs=bool(a.graphs), basis_fp16=guess == "galerkin")
     t_end = ck["t"] + bench.SEGMENT_YEARS * bench.YR
     bench.one_pass(ctx, ck, t_end)
     print("[profile_window] checkpoint and warm-up pass done; the measured pass follows", flush=True)

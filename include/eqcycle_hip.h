@@ -221,6 +221,9 @@ typedef struct {
                               (default): when the previous solve left x solving the old b and only
                               b's rows 1..4 changed, just that change is projected (the old
                               residual, at tolerance level, is left to CG) */
+  int basis_fp16;          /* 1: the Galerkin start's expansion x += Phi c reads an FP16 copy of the
+                              basis (half the bytes; the rounding, 5e-4 of the increment Phi c, is
+                              left to CG, which runs to the same tolerance); 0: the FP32 basis */
 } eq_solver_opts;
 int eq_set_solver(eq_ctx *ctx, const eq_solver_opts *opts);
 

@@ -311,6 +311,7 @@ struct SolverOpts                      # eq_solver_opts, field for field
     cheb_single::Cint
     sync_pass2::Cint
     galerkin_full::Cint
+    basis_fp16::Cint
 end
 
 """set_solver!(ctx; tol = 1e-6, ...): the cg! settings of the loop's quasi-static

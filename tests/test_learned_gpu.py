@@ -160,6 +160,31 @@ def test_galerkin_change_of_b_start_matches_full_projection(trained, prec):
     assert np.max(np.abs(d_a - d_b)) <= 1e-9 * np.max(np.abs(d_b))
 
 
+@pytest.mark.parametrize("tol", [1e-6, 1e-10])
+def test_fp16_basis_expansion(trained, tol):
+    """The Galerkin expansion x += Phi c from the FP16 copy of the basis
+    (eq_solver_opts.basis_fp16): 40 steps from the same state with and
+    without it need the same CG iterations within 5 %, and, since each solve
+    still runs to its tolerance, the displacements agree to that tolerance
+    (the expansion's rounding, 5e-4 of the increment, is what CG removes)."""
+    P, ctx, _ = trained
+    s0 = ctx.state_get()
+
+    def run(fp16):
+        ctx.state_set(d=s0["d"], v=s0["v"], a=s0["a"], psi=s0["psi"], tau=s0["tau"], t=s0["t"], dt=s0["dt"],
+                      isolver=s0["isolver"])
+        ctx.set_solver(tol=tol, pc="amg", prec="f32pc", guess="galerkin", basis_fp16=fp16)
+        its = [sum(ctx.step().pcg_iters) for _ in range(40)]
+        return its, ctx.state_get()["d"]
+
+    it_a, d_a = run(False)
+    it_b, d_b = run(True)
+    print(f"[gpu] tol {tol:g}: iterations FP32 basis {np.sum(it_a)}, FP16 basis {np.sum(it_b)}, "
+          f"d rel diff {np.max(np.abs(d_a - d_b)) / np.max(np.abs(d_a)):.2e}", flush=True)
+    assert np.sum(it_b) <= 1.05 * np.sum(it_a) + 1
+    assert np.max(np.abs(d_a - d_b)) <= 20 * tol * np.max(np.abs(d_a))
+
+
 def _bf16(x):
     """Round FP32 values to BF16 (nearest even) and back, as the kernels store weights."""
     u = np.asarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
