@@ -56,7 +56,7 @@ def test_ku_fault_rows(mesh):
     y = K.matvec(u)
     rows = P.y <= P.dye * (1 + 1e-9)
     assert rows.sum() == 5 * P.FltNglob
-    for prec, tol in (("fp64", 1e-13), ("fp32", 1e-5)):
+    for prec, tol in (("fp64", 1e-13), ("fp32", 1e-6)):
         yf = ctx.ku(u, mask="fault_rows", prec=prec)
         assert np.all(yf[~rows] == 0.0)
         assert np.linalg.norm(yf[rows] - y[rows]) / np.linalg.norm(y[rows]) < tol
@@ -74,11 +74,15 @@ def test_ku_constant_nullspace(mesh):
 
 
 def test_ku_fp32(mesh):
+    """FP32 K u (FP32 operands and sums) against the oracle's FP64 product: 1e-6
+    relative in the 2-norm, as on the benchmark meshes (test_gpu_configs)."""
     P, ctx, K, _ = mesh
     u = normal(3, P.nglob)
     y = K.matvec(u)
     y32 = ctx.ku(u, prec="fp32")
-    assert np.linalg.norm(y32 - y) / np.linalg.norm(y) < 1e-5
+    err = np.linalg.norm(y32 - y) / np.linalg.norm(y)
+    print(f"[gpu] FP32 K u rel err {err:.2e}", flush=True)
+    assert err < 1e-6
 
 
 def test_kni(mesh):
