@@ -175,7 +175,8 @@ def main():
         summary = {"config": a.config, "nodes": int(P.nglob), "tol": tol, "window": window,
                    "window_years": a.window_years, "cycles_requested": a.cycles, **learning}
         if len(wl) == 2:
-            p, g = wl
+            # the plain run and the learned one, in whichever order --runs gave them
+            p, g = sorted(wl, key=lambda r: 0 if r["run"].endswith("/plain") else 1)
             n = min(len(p["cycle_onsets_yr"]), len(g["cycle_onsets_yr"]))
             summary.update({
                 "t_end_yr": [p["t_end_yr"], g["t_end_yr"]],
