@@ -51,6 +51,8 @@ struct LearnState {
   float* out = nullptr;
   int32_t* idx = nullptr;
   double* loss_part = nullptr;
+  float* gemm_part = nullptr;  // split-K partials of the layer GEMMs
+  int64_t gemm_part_cap = 0;
   // inference scratch
   float* inf_act = nullptr;
   float* coef = nullptr;     // [NB + 1]: coefficients, last = input scale

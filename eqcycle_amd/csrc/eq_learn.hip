@@ -271,6 +271,271 @@ __global__ void gather_kernel(const float* X, int L0, const int32_t* idx, int B,
   out[e] = X[(int64_t)idx[b] * L0 + i];
 }
 
+// ---- training GEMMs on the matrix cores --------------------------------------
+// C[M x N] = A[M x K] B[K x N] with V_MFMA_F32_16X16X4_F32: FP32 operands (the
+// BF16 weights widen exactly), FP32 accumulation, so the products are those of
+// an FP32 GEMM.  The operands are functors (im2col gathers for the
+// convolutions, transposed views for the linear layers) and the output goes
+// through an epilogue functor.  A workgroup (4 waves) owns a 64 x 64 tile, each
+// wave a 32 x 32 quarter as 2 x 2 blocks of 16 x 16; K streams through LDS in
+// chunks of 16.  gridDim.z > 1 splits K: each split writes its FP32 partial
+// tile to part + z M N and gemm_finish_kernel sums the splits in order (FP64)
+// and applies the epilogue.
+//   Operand layout of the 16x16x4 FP32 MFMA (lane l): A[l % 16][l / 16],
+//   B[l / 16][l % 16], D (four registers r) D[4 (l / 16) + r][l % 16].
+typedef float gemm_f4 __attribute__((ext_vector_type(4)));
+constexpr int QM = 64, QK = 16;
+
+template <class AOp, class BOp, class Ep>
+__global__ __launch_bounds__(256) void mfma_gemm_kernel(int M, int N, int K, int kps, AOp a, BOp b, Ep ep,
+                                                        float* part) {
+  __shared__ float As[QK][QM + 4];
+  __shared__ float Bs[QK][QM + 4];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wi = (w >> 1) * 32, wj = (w & 1) * 32, li = lane & 15, lk = lane >> 4;
+  const int m0 = blockIdx.y * QM, n0 = blockIdx.x * QM;
+  const int kbeg = blockIdx.z * kps, kend = min(K, kbeg + kps);
+  gemm_f4 acc[2][2];
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 2; y++) acc[x][y] = gemm_f4{0.f, 0.f, 0.f, 0.f};
+  // element e = tid + 256 q of a chunk: row / column e % 64, k = e / 64; the
+  // next chunk's operands are fetched into registers while this one computes
+  constexpr int QL = (QM * QK) / 256;
+  float pa[QL], pb[QL];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < QL; q++) {
+      const int e = tid + 256 * q, mm = e & 63, kk = e >> 6;
+      const int k = k0 + kk;
+      pa[q] = (k < kend && m0 + mm < M) ? a(m0 + mm, k) : 0.f;
+      pb[q] = (k < kend && n0 + mm < N) ? b(k, n0 + mm) : 0.f;
+    }
+  };
+  fetch(kbeg);
+  for (int k0 = kbeg; k0 < kend; k0 += QK) {
+#pragma unroll
+    for (int q = 0; q < QL; q++) {
+      const int e = tid + 256 * q, mm = e & 63, kk = e >> 6;
+      As[kk][mm] = pa[q];
+      Bs[kk][mm] = pb[q];
+    }
+    __syncthreads();
+    if (k0 + QK < kend) fetch(k0 + QK);
+#pragma unroll
+    for (int ks = 0; ks < QK / 4; ks++) {
+      const int k = 4 * ks + lk;
+      const float a0 = As[k][wi + li], a1 = As[k][wi + 16 + li];
+      const float b0 = Bs[k][wj + li], b1 = Bs[k][wj + 16 + li];
+      acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc[1][1], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int ta = 0; ta < 2; ta++)
+#pragma unroll
+    for (int tb = 0; tb < 2; tb++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = m0 + wi + 16 * ta + 4 * lk + r, j = n0 + wj + 16 * tb + li;
+        if (i < M && j < N) {
+          if (part) part[((int64_t)blockIdx.z * M + i) * N + j] = acc[ta][tb][r];
+          else ep(i, j, acc[ta][tb][r]);
+        }
+      }
+}
+
+// Few splits: a thread per output element adds them in order.  Many: one wave
+// per element, lane z adding splits z, z + 64, ..., then a shuffle tree (fixed
+// order) summing the lanes; both in FP64.
+template <class Ep>
+__global__ __launch_bounds__(LB) void gemm_sum_kernel(int M, int N, int nz, const float* part, Ep ep) {
+  const int64_t e = (int64_t)blockIdx.x * LB + threadIdx.x;
+  if (e >= (int64_t)M * N) return;
+  double v = 0.0;
+  for (int z = 0; z < nz; z++) v += part[(int64_t)z * M * N + e];
+  ep((int)(e / N), (int)(e % N), (float)v);
+}
+
+template <class Ep>
+__global__ __launch_bounds__(LB) void gemm_finish_kernel(int M, int N, int nz, const float* part, Ep ep) {
+  const int64_t e = (int64_t)blockIdx.x * (LB / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (e >= (int64_t)M * N) return;   // wave-uniform
+  double v = 0.0;
+  for (int z = lane; z < nz; z += 64) v += part[(int64_t)z * M * N + e];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if (lane == 0) ep((int)(e / N), (int)(e % N), (float)v);
+}
+
+// Launch C = A B with the K split so that the grid has about 256 workgroups
+// (each split at least 64 deep; the partials must fit part_cap floats).
+template <class AOp, class BOp, class Ep>
+void gemm(int M, int N, int K, AOp a, BOp b, Ep ep, float* part, int64_t part_cap, hipStream_t s) {
+  const int tm = (M + QM - 1) / QM, tn = (N + QM - 1) / QM, tiles = tm * tn;
+  int nz = 1;
+  while (tiles * nz * 2 <= 256 && K >= 64 * nz * 2 && (int64_t)nz * 2 * M * N <= part_cap) nz *= 2;
+  int kps = (K + nz - 1) / nz;
+  kps = (kps + QK - 1) / QK * QK;
+  nz = (K + kps - 1) / kps;
+  if (nz <= 1) {
+    hipLaunchKernelGGL((mfma_gemm_kernel<AOp, BOp, Ep>), dim3(tn, tm, 1), dim3(256), 0, s, M, N, K, K, a, b, ep,
+                       (float*)nullptr);
+    return;
+  }
+  hipLaunchKernelGGL((mfma_gemm_kernel<AOp, BOp, Ep>), dim3(tn, tm, nz), dim3(256), 0, s, M, N, K, kps, a, b, ep,
+                     part);
+  if (nz <= 8)
+    hipLaunchKernelGGL((gemm_sum_kernel<Ep>), dim3(gblk((int64_t)M * N)), dim3(LB), 0, s, M, N, nz, part, ep);
+  else
+    hipLaunchKernelGGL((gemm_finish_kernel<Ep>), dim3((unsigned)(((int64_t)M * N + LB / 64 - 1) / (LB / 64))),
+                       dim3(LB), 0, s, M, N, nz, part, ep);
+}
+
+// operands --------------------------------------------------------------------
+struct RowMat {   // A(r, c) = p[r ld + c]
+  const float* p;
+  int ld;
+  __device__ float operator()(int r, int c) const { return p[(int64_t)r * ld + c]; }
+};
+struct ColMat {   // A(r, c) = p[c ld + r] (the transpose of a row-major matrix)
+  const float* p;
+  int ld;
+  __device__ float operator()(int r, int c) const { return p[(int64_t)c * ld + r]; }
+};
+template <typename WT>
+struct WRow {     // weights, row-major [rows x ld]: W(r, c)
+  const WT* p;
+  int ld;
+  __device__ float operator()(int r, int c) const { return ldw(p, (int64_t)r * ld + c); }
+};
+template <typename WT>
+struct WCol {     // the transpose of row-major weights: W^T(r, c) = W(c, r)
+  const WT* p;
+  int ld;
+  __device__ float operator()(int r, int c) const { return ldw(p, (int64_t)c * ld + r); }
+};
+struct RowMatOnes {   // row-major [rows x n] with a column of ones appended at c = n
+  const float* p;
+  int n;
+  __device__ float operator()(int r, int c) const { return c < n ? p[(int64_t)r * n + c] : 1.f; }
+};
+// im2col of a conv layer's input x [B x Cin x Lin] for stride 2, padding 2,
+// kernel 5: row (b, o), column (ci, k) -> x[b, ci, 2o + k - 2] (0 outside)
+struct ConvIn {
+  const float* x;
+  int Cin, Lin, Lout;
+  __device__ float operator()(int m, int kk) const {
+    const int b = m / Lout, o = m - b * Lout, ci = kk / 5, k = kk - 5 * ci;
+    const int i = 2 * o + k - 2;
+    return (i >= 0 && i < Lin) ? x[((int64_t)b * Cin + ci) * Lin + i] : 0.f;
+  }
+};
+// the same gather with rows and columns exchanged, plus a column of ones at
+// (ci, k) = Cin * 5 (the bias gradient): B((b, o), (ci, k))
+struct ConvInT {
+  const float* x;
+  int Cin, Lin, Lout;
+  __device__ float operator()(int kk, int n) const {
+    if (n == 5 * Cin) return 1.f;
+    const int b = kk / Lout, o = kk - b * Lout, ci = n / 5, k = n - 5 * ci;
+    const int i = 2 * o + k - 2;
+    return (i >= 0 && i < Lin) ? x[((int64_t)b * Cin + ci) * Lin + i] : 0.f;
+  }
+};
+// dout [B x Cout x Lout] as A(co, (b, o))
+struct ConvGradT {
+  const float* g;
+  int Cout, Lout;
+  __device__ float operator()(int co, int kk) const {
+    const int b = kk / Lout, o = kk - b * Lout;
+    return g[((int64_t)b * Cout + co) * Lout + o];
+  }
+};
+// the transposed convolution's gather of dout: row (b, i), column (co, k) ->
+// dout[b, co, (i + 2 - k) / 2] when that index is whole and in range
+struct ConvGradIm {
+  const float* g;
+  int Cout, Lin, Lout;
+  __device__ float operator()(int m, int kk) const {
+    const int b = m / Lin, i = m - b * Lin, co = kk / 5, k = kk - 5 * co;
+    const int t = i + 2 - k;
+    return (t >= 0 && (t & 1) == 0 && (t >> 1) < Lout) ? g[((int64_t)b * Cout + co) * Lout + (t >> 1)] : 0.f;
+  }
+};
+// conv weights W [Cout x Cin x 5] as B((co, k), ci) = W[co, ci, k]
+template <typename WT>
+struct ConvWT {
+  const WT* p;
+  int Cin;
+  __device__ float operator()(int kk, int ci) const {
+    const int co = kk / 5, k = kk - 5 * co;
+    return ldw(p, ((int64_t)co * Cin + ci) * 5 + k);
+  }
+};
+
+// epilogues -------------------------------------------------------------------
+struct EpDense {   // out[i N + j] = f(v + bias[j]), f = ReLU or identity
+  float* out;
+  const float* bias;
+  int N, relu;
+  __device__ void operator()(int i, int j, float v) const {
+    v += bias[j];
+    out[(int64_t)i * N + j] = relu ? fmaxf(v, 0.f) : v;
+  }
+};
+struct EpMasked {  // out[i N + j] = v where the layer input (a ReLU output) is positive
+  float* out;
+  const float* in;
+  int N;
+  __device__ void operator()(int i, int j, float v) const {
+    const int64_t e = (int64_t)i * N + j;
+    out[e] = in[e] > 0.f ? v : 0.f;
+  }
+};
+struct EpGrad {    // weight gradient [M x N]; column N is the bias gradient
+  float* dW;
+  float* db;
+  int N;
+  __device__ void operator()(int i, int j, float v) const {
+    if (j < N) dW[(int64_t)i * N + j] = v;
+    else db[i] = v;
+  }
+};
+struct EpConv {    // conv output [B x Cout x Lout] from row (b, o), column co, with bias and ReLU
+  float* out;
+  const float* bias;
+  int Cout, Lout;
+  __device__ void operator()(int m, int co, float v) const {
+    const int b = m / Lout, o = m - b * Lout;
+    out[((int64_t)b * Cout + co) * Lout + o] = fmaxf(v + bias[co], 0.f);
+  }
+};
+struct EpConvGradIn {   // din [B x Cin x Lin] from row (b, i), column ci, masked by the input
+  float* din;
+  const float* in;
+  int Cin, Lin;
+  __device__ void operator()(int m, int ci, float v) const {
+    const int b = m / Lin, i = m - b * Lin;
+    const int64_t e = ((int64_t)b * Cin + ci) * Lin + i;
+    din[e] = in[e] > 0.f ? v : 0.f;
+  }
+};
+struct EpConvGradW {    // dW [Cout x Cin x 5] from row co, column (ci, k); column Cin * 5 is db
+  float* dW;
+  float* db;
+  int n;
+  __device__ void operator()(int co, int j, float v) const {
+    if (j < n) dW[(int64_t)co * n + j] = v;
+    else db[co] = v;
+  }
+};
+
 template <typename WT>
 __global__ void conv_fwd_kernel(int B, int Cin, int Lin, int Cout, int Lout, const float* in, const WT* W,
                                 const float* bias, float* out) {
@@ -290,96 +555,6 @@ __global__ void conv_fwd_kernel(int B, int Cin, int Lin, int Cout, int Lout, con
     }
   }
   out[e] = fmaxf(acc, 0.f);
-}
-
-// din = (W^T * dout) masked by in > 0 (the input is a ReLU output)
-template <typename WT>
-__global__ void conv_bwd_data_kernel(int B, int Cin, int Lin, int Cout, int Lout, const float* in, const WT* W,
-                                     const float* dout, float* din) {
-  const int64_t e = (int64_t)blockIdx.x * LB + threadIdx.x;
-  if (e >= (int64_t)B * Cin * Lin) return;
-  const int i = (int)(e % Lin);
-  const int ci = (int)((e / Lin) % Cin);
-  const int b = (int)(e / ((int64_t)Lin * Cin));
-  float acc = 0.f;
-  if (in[e] > 0.f) {
-    for (int co = 0; co < Cout; co++) {
-      const float* dy = dout + ((int64_t)b * Cout + co) * Lout;
-      const int64_t wb = ((int64_t)co * Cin + ci) * 5;
-#pragma unroll
-      for (int k = 0; k < 5; k++) {
-        const int t = i + 2 - k;
-        if (t >= 0 && (t & 1) == 0 && (t >> 1) < Lout) acc += ldw(W, wb + k) * dy[t >> 1];
-      }
-    }
-  }
-  din[e] = acc;
-}
-
-// dW[co, ci, :] and db[co] by block reduction over (b, o)
-__global__ __launch_bounds__(LB) void conv_bwd_weight_kernel(int B, int Cin, int Lin, int Cout, int Lout,
-                                                             const float* in, const float* dout, float* dW,
-                                                             float* db) {
-  const int co = blockIdx.x / Cin, ci = blockIdx.x - co * Cin;
-  double acc[6] = {0, 0, 0, 0, 0, 0};
-  const int64_t tot = (int64_t)B * Lout;
-  for (int64_t e = threadIdx.x; e < tot; e += LB) {
-    const int b = (int)(e / Lout), o = (int)(e - (int64_t)b * Lout);
-    const float g = dout[((int64_t)b * Cout + co) * Lout + o];
-    const float* x = in + ((int64_t)b * Cin + ci) * Lin;
-#pragma unroll
-    for (int k = 0; k < 5; k++) {
-      const int i = 2 * o + k - 2;
-      if (i >= 0 && i < Lin) acc[k] += (double)g * (double)x[i];
-    }
-    acc[5] += g;
-  }
-  for (int k = 0; k < 6; k++) {
-    const double s = lblock_sum(acc[k]);
-    if (threadIdx.x == 0) {
-      if (k < 5) dW[((int64_t)co * Cin + ci) * 5 + k] = (float)s;
-      else if (ci == 0) db[co] = (float)s;
-    }
-  }
-}
-
-template <typename WT>
-__global__ void linear_fwd_kernel(int B, int K, int N, const float* in, const WT* W, const float* bias, float* out,
-                                  int relu) {
-  const int64_t e = (int64_t)blockIdx.x * LB + threadIdx.x;
-  if (e >= (int64_t)B * N) return;
-  const int n = (int)(e % N), b = (int)(e / N);
-  const float* x = in + (int64_t)b * K;
-  float acc = bias[n];
-  for (int k = 0; k < K; k++) acc += ldw(W, (int64_t)n * K + k) * x[k];
-  out[e] = relu ? fmaxf(acc, 0.f) : acc;
-}
-
-template <typename WT>
-__global__ void linear_bwd_data_kernel(int B, int K, int N, const float* in, const WT* W, const float* dout,
-                                       float* din) {
-  const int64_t e = (int64_t)blockIdx.x * LB + threadIdx.x;
-  if (e >= (int64_t)B * K) return;
-  const int k = (int)(e % K), b = (int)(e / K);
-  float acc = 0.f;
-  if (in[e] > 0.f)
-    for (int n = 0; n < N; n++) acc += ldw(W, (int64_t)n * K + k) * dout[(int64_t)b * N + n];
-  din[e] = acc;
-}
-
-__global__ void linear_bwd_weight_kernel(int B, int K, int N, const float* in, const float* dout, float* dW,
-                                         float* db) {
-  const int64_t e = (int64_t)blockIdx.x * LB + threadIdx.x;
-  if (e >= (int64_t)N * K) return;
-  const int k = (int)(e % K), n = (int)(e / K);
-  double acc = 0.0, accb = 0.0;
-  for (int b = 0; b < B; b++) {
-    const float g = dout[(int64_t)b * N + n];
-    acc += (double)g * (double)in[(int64_t)b * K + k];
-    accb += g;
-  }
-  dW[e] = (float)acc;
-  if (k == 0) db[n] = (float)accb;
 }
 
 // mean squared error over B x NB outputs; dout = d loss / d out
@@ -709,7 +884,8 @@ int learn_init(LearnState* ls, int64_t nI, int64_t nflt, int64_t offset, int cap
 void learn_free(LearnState* ls) {
   void* ptrs[] = {ls->S, ls->F, ls->Phi, ls->X, ls->Y, ls->w32, ls->w16, ls->grad, ls->adam_m, ls->adam_v,
                   ls->act, ls->dact, ls->out, ls->idx, ls->loss_part, ls->inf_act, ls->coef, ls->norms,
-                  ls->gal_part, ls->gal_c, ls->gal_Ginv, ls->Phi_rm, ls->gal_T, ls->lin_A, ls->lin_b, ls->Phi16};
+                  ls->gal_part, ls->gal_c, ls->gal_Ginv, ls->Phi_rm, ls->gal_T, ls->lin_A, ls->lin_b, ls->Phi16,
+                  ls->gemm_part};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   NetDims d = ls->d;
@@ -851,7 +1027,8 @@ int learn_set_basis(LearnState* ls, const double* V, const double* sigma, const 
 
 static int ensure_batch(LearnState* ls, int B) {
   if (ls->bmax >= B) return 0;
-  for (void* p : {(void*)ls->act, (void*)ls->dact, (void*)ls->out, (void*)ls->idx, (void*)ls->loss_part})
+  for (void* p : {(void*)ls->act, (void*)ls->dact, (void*)ls->out, (void*)ls->idx, (void*)ls->loss_part,
+                  (void*)ls->gemm_part})
     if (p) (void)hipFree(p);
   const NetDims& d = ls->d;
   int64_t off = 0;
@@ -864,51 +1041,59 @@ static int ensure_batch(LearnState* ls, int B) {
   ls->out = dalloc<float>((int64_t)B * d.NB * 2);
   ls->idx = dalloc<int32_t>(B);
   ls->loss_part = dalloc<double>(gblk((int64_t)B * d.NB) + 1);
+  ls->gemm_part_cap = (int64_t)1 << 21;   // split-K partials of the layer GEMMs (8 MB)
+  ls->gemm_part = dalloc<float>(ls->gemm_part_cap);
   ls->bmax = B;
-  return (ls->act && ls->dact && ls->out && ls->idx && ls->loss_part) ? 0 : 7;
+  return (ls->act && ls->dact && ls->out && ls->idx && ls->loss_part && ls->gemm_part) ? 0 : 7;
 }
 
-// forward + backward of one batch; fills grad, returns the loss partials in loss_part
+// forward + backward of one batch; fills grad, returns the loss partials in
+// loss_part.  Every layer is a GEMM on the matrix cores (mfma_gemm_kernel):
+// the convolutions through their im2col gathers, the weight gradients with a
+// column of ones that gives the bias gradient.
 template <typename WT>
-static void fwd_bwd(LearnState* ls, const WT* W, const float* w32, int B, hipStream_t s) {
+static void fwd_bwd(LearnState* ls, const WT* W, const float* w32, int B, const int32_t* idx, double* loss_part,
+                    hipStream_t s) {
   const NetDims& d = ls->d;
   float* A = ls->act;
   float* dA = ls->dact;
   float* out = ls->out;
   float* dout = ls->out + (int64_t)B * d.NB;
-  hipLaunchKernelGGL(gather_kernel, dim3(gblk((int64_t)B * d.L0)), dim3(LB), 0, s, ls->X, d.L0, ls->idx, B,
+  float* part = ls->gemm_part;
+  const int64_t cap = ls->gemm_part_cap;
+  hipLaunchKernelGGL(gather_kernel, dim3(gblk((int64_t)B * d.L0)), dim3(LB), 0, s, ls->X, d.L0, idx, B,
                      A + ls->act_off[0]);
   for (int l = 1; l <= d.NL; l++) {
-    const int cin = (l == 1) ? 1 : d.C;
-    hipLaunchKernelGGL((conv_fwd_kernel<WT>), dim3(gblk((int64_t)B * d.C * d.L[l])), dim3(LB), 0, s, B, cin,
-                       d.L[l - 1], d.C, d.L[l], A + ls->act_off[l - 1], W + d.w_off[l - 1], w32 + d.b_off[l - 1],
-                       A + ls->act_off[l]);
+    const int cin = (l == 1) ? 1 : d.C, Lin = d.L[l - 1], Lout = d.L[l];
+    gemm(B * Lout, d.C, 5 * cin, ConvIn{A + ls->act_off[l - 1], cin, Lin, Lout},
+         WCol<WT>{W + d.w_off[l - 1], 5 * cin}, EpConv{A + ls->act_off[l], w32 + d.b_off[l - 1], d.C, Lout}, part,
+         cap, s);
   }
   const int K = d.flat();
-  hipLaunchKernelGGL((linear_fwd_kernel<WT>), dim3(gblk((int64_t)B * d.H)), dim3(LB), 0, s, B, K, d.H,
-                     A + ls->act_off[d.NL], W + d.w_off[d.NL], w32 + d.b_off[d.NL], A + ls->act_off[d.NL + 1], 1);
-  hipLaunchKernelGGL((linear_fwd_kernel<WT>), dim3(gblk((int64_t)B * d.NB)), dim3(LB), 0, s, B, d.H, d.NB,
-                     A + ls->act_off[d.NL + 1], W + d.w_off[d.NL + 1], w32 + d.b_off[d.NL + 1], out, 0);
-  hipLaunchKernelGGL(mse_kernel, dim3(gblk((int64_t)B * d.NB)), dim3(LB), 0, s, B, d.NB, out, ls->Y, ls->idx, dout,
-                     ls->loss_part);
+  gemm(B, d.H, K, RowMat{A + ls->act_off[d.NL], K}, WCol<WT>{W + d.w_off[d.NL], K},
+       EpDense{A + ls->act_off[d.NL + 1], w32 + d.b_off[d.NL], d.H, 1}, part, cap, s);
+  gemm(B, d.NB, d.H, RowMat{A + ls->act_off[d.NL + 1], d.H}, WCol<WT>{W + d.w_off[d.NL + 1], d.H},
+       EpDense{out, w32 + d.b_off[d.NL + 1], d.NB, 0}, part, cap, s);
+  hipLaunchKernelGGL(mse_kernel, dim3(gblk((int64_t)B * d.NB)), dim3(LB), 0, s, B, d.NB, out, ls->Y, idx, dout,
+                     loss_part);
   // backward
   float* g = ls->grad;
-  hipLaunchKernelGGL(linear_bwd_weight_kernel, dim3(gblk((int64_t)d.NB * d.H)), dim3(LB), 0, s, B, d.H, d.NB,
-                     A + ls->act_off[d.NL + 1], dout, g + d.w_off[d.NL + 1], g + d.b_off[d.NL + 1]);
-  hipLaunchKernelGGL((linear_bwd_data_kernel<WT>), dim3(gblk((int64_t)B * d.H)), dim3(LB), 0, s, B, d.H, d.NB,
-                     A + ls->act_off[d.NL + 1], W + d.w_off[d.NL + 1], dout, dA + ls->act_off[d.NL + 1]);
-  hipLaunchKernelGGL(linear_bwd_weight_kernel, dim3(gblk((int64_t)d.H * K)), dim3(LB), 0, s, B, K, d.H,
-                     A + ls->act_off[d.NL], dA + ls->act_off[d.NL + 1], g + d.w_off[d.NL], g + d.b_off[d.NL]);
-  hipLaunchKernelGGL((linear_bwd_data_kernel<WT>), dim3(gblk((int64_t)B * K)), dim3(LB), 0, s, B, K, d.H,
-                     A + ls->act_off[d.NL], W + d.w_off[d.NL], dA + ls->act_off[d.NL + 1], dA + ls->act_off[d.NL]);
+  gemm(d.NB, d.H + 1, B, ColMat{dout, d.NB}, RowMatOnes{A + ls->act_off[d.NL + 1], d.H},
+       EpGrad{g + d.w_off[d.NL + 1], g + d.b_off[d.NL + 1], d.H}, part, cap, s);
+  gemm(B, d.H, d.NB, RowMat{dout, d.NB}, WRow<WT>{W + d.w_off[d.NL + 1], d.H},
+       EpMasked{dA + ls->act_off[d.NL + 1], A + ls->act_off[d.NL + 1], d.H}, part, cap, s);
+  gemm(d.H, K + 1, B, ColMat{dA + ls->act_off[d.NL + 1], d.H}, RowMatOnes{A + ls->act_off[d.NL], K},
+       EpGrad{g + d.w_off[d.NL], g + d.b_off[d.NL], K}, part, cap, s);
+  gemm(B, K, d.H, RowMat{dA + ls->act_off[d.NL + 1], d.H}, WRow<WT>{W + d.w_off[d.NL], K},
+       EpMasked{dA + ls->act_off[d.NL], A + ls->act_off[d.NL], K}, part, cap, s);
   for (int l = d.NL; l >= 1; l--) {
-    const int cin = (l == 1) ? 1 : d.C;
-    hipLaunchKernelGGL(conv_bwd_weight_kernel, dim3(d.C * cin), dim3(LB), 0, s, B, cin, d.L[l - 1], d.C, d.L[l],
-                       A + ls->act_off[l - 1], dA + ls->act_off[l], g + d.w_off[l - 1], g + d.b_off[l - 1]);
+    const int cin = (l == 1) ? 1 : d.C, Lin = d.L[l - 1], Lout = d.L[l];
+    gemm(d.C, 5 * cin + 1, B * Lout, ConvGradT{dA + ls->act_off[l], d.C, Lout},
+         ConvInT{A + ls->act_off[l - 1], cin, Lin, Lout}, EpConvGradW{g + d.w_off[l - 1], g + d.b_off[l - 1], 5 * cin},
+         part, cap, s);
     if (l > 1)
-      hipLaunchKernelGGL((conv_bwd_data_kernel<WT>), dim3(gblk((int64_t)B * cin * d.L[l - 1])), dim3(LB), 0, s, B,
-                         cin, d.L[l - 1], d.C, d.L[l], A + ls->act_off[l - 1], W + d.w_off[l - 1],
-                         dA + ls->act_off[l], dA + ls->act_off[l - 1]);
+      gemm(B * Lin, cin, 5 * d.C, ConvGradIm{dA + ls->act_off[l], d.C, Lin, Lout}, ConvWT<WT>{W + d.w_off[l - 1], cin},
+           EpConvGradIn{dA + ls->act_off[l - 1], A + ls->act_off[l - 1], cin, Lin}, part, cap, s);
   }
 }
 
@@ -921,31 +1106,51 @@ int learn_train(LearnState* ls, int epochs, int batch, double lr, uint64_t seed,
   if (ensure_batch(ls, B)) { *err = "device allocation failed"; return 7; }
   const NetDims& d = ls->d;
   const int nparts = gblk((int64_t)B * d.NB);
-  std::vector<int32_t> perm(m);
-  std::vector<double> parts(nparts);
+  const int nbatch = m / B;
+  // every epoch's shuffled order up front, and every batch's loss partials
+  // read back once at the end: the steps run with no host round trip
+  std::vector<int32_t> perm((size_t)epochs * m);
   uint64_t st = seed;
-  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
   for (int ep = 0; ep < epochs; ep++) {
-    for (int i = 0; i < m; i++) perm[i] = ls->used[i];
-    for (int i = m - 1; i > 0; i--) std::swap(perm[i], perm[smix(st) % (uint64_t)(i + 1)]);
-    double eloss = 0;
-    int nb = 0;
-    for (int b0 = 0; b0 + B <= m; b0 += B) {
-      LCHK(hipMemcpyAsync(ls->idx, perm.data() + b0, sizeof(int32_t) * B, hipMemcpyHostToDevice, s));
-      fwd_bwd<uint16_t>(ls, ls->w16, ls->w32, B, s);
+    int32_t* pe = perm.data() + (size_t)ep * m;
+    for (int i = 0; i < m; i++) pe[i] = ls->used[i];
+    for (int i = m - 1; i > 0; i--) std::swap(pe[i], pe[smix(st) % (uint64_t)(i + 1)]);
+  }
+  int32_t* perm_dev = dalloc<int32_t>((int64_t)epochs * m);
+  double* parts_dev = dalloc<double>((int64_t)epochs * nbatch * nparts);
+  if (!perm_dev || !parts_dev) {
+    if (perm_dev) (void)hipFree(perm_dev);
+    if (parts_dev) (void)hipFree(parts_dev);
+    *err = "device allocation failed";
+    return 7;
+  }
+  hipError_t e = hipMemcpyAsync(perm_dev, perm.data(), sizeof(int32_t) * perm.size(), hipMemcpyHostToDevice, s);
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  for (int ep = 0; ep < epochs && e == hipSuccess; ep++)
+    for (int q = 0; q < nbatch; q++) {
+      fwd_bwd<uint16_t>(ls, ls->w16, ls->w32, B, perm_dev + (int64_t)ep * m + (int64_t)q * B,
+                        parts_dev + ((int64_t)ep * nbatch + q) * nparts, s);
       ls->adam_t++;
       const float c1 = 1.f - std::pow(b1, (float)ls->adam_t), c2 = 1.f - std::pow(b2, (float)ls->adam_t);
       hipLaunchKernelGGL(adam_kernel, dim3(gblk(d.nparams)), dim3(LB), 0, s, d.nparams, ls->w32, ls->w16, ls->grad,
                          ls->adam_m, ls->adam_v, (float)lr, b1, b2, c1, c2, eps);
-      LCHK(hipMemcpyAsync(parts.data(), ls->loss_part, sizeof(double) * nparts, hipMemcpyDeviceToHost, s));
-      const hipError_t e = hipStreamSynchronize(s);
-      if (e != hipSuccess) { *err = hipGetErrorString(e); return 2; }
-      double l = 0;
-      for (double p : parts) l += p;
-      eloss += l;
-      nb++;
     }
-    if (hist) hist[ep] = nb ? eloss / nb : 0.0;
+  std::vector<double> parts((size_t)epochs * nbatch * nparts);
+  if (e == hipSuccess) e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(parts.data(), parts_dev, sizeof(double) * parts.size(), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(perm_dev);
+  (void)hipFree(parts_dev);
+  if (e != hipSuccess) { *err = hipGetErrorString(e); return 2; }
+  for (int ep = 0; ep < epochs; ep++) {
+    double eloss = 0;
+    for (int q = 0; q < nbatch; q++) {
+      double l = 0;
+      const double* pq = parts.data() + ((size_t)ep * nbatch + q) * nparts;
+      for (int k = 0; k < nparts; k++) l += pq[k];
+      eloss += l;
+    }
+    if (hist) hist[ep] = nbatch ? eloss / nbatch : 0.0;
   }
   ls->trained = 1;
   return 0;
@@ -964,8 +1169,8 @@ int learn_loss_grad(LearnState* ls, const float* params, int nsamp, const int32_
     hipLaunchKernelGGL(to_bf16_kernel, dim3(gblk(d.nparams)), dim3(LB), 0, s, d.nparams, ls->w32, ls->w16);
   }
   LCHK(hipMemcpyAsync(ls->idx, idx, sizeof(int32_t) * nsamp, hipMemcpyHostToDevice, s));
-  if (use_bf16) fwd_bwd<uint16_t>(ls, ls->w16, ls->w32, nsamp, s);
-  else fwd_bwd<float>(ls, ls->w32, ls->w32, nsamp, s);
+  if (use_bf16) fwd_bwd<uint16_t>(ls, ls->w16, ls->w32, nsamp, ls->idx, ls->loss_part, s);
+  else fwd_bwd<float>(ls, ls->w32, ls->w32, nsamp, ls->idx, ls->loss_part, s);
   const int nparts = gblk((int64_t)nsamp * d.NB);
   std::vector<double> parts(nparts);
   LCHK(hipMemcpyAsync(parts.data(), ls->loss_part, sizeof(double) * nparts, hipMemcpyDeviceToHost, s));

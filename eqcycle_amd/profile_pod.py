@@ -6,7 +6,8 @@ Runs the plain FP64 solver from the seeded state until cfg.n_snapshots pass-1
 solves are recorded (config 4: 2,000 snapshots of 410,240 interior nodes),
 then --reps times the snapshot Gram S^T S (gram_kernel on the matrix cores)
 and once the whole POD fit (Gram, host eigendecomposition, basis and its
-Gram for the Cholesky-QR pass), printing wall times.  CG batches launch one
+Gram for the Cholesky-QR pass), then (--epochs > 0) trains the network that
+many epochs (batches of 64), printing wall times.  CG batches launch one
 kernel at a time (no HIP stream capture), so the program can run under
 rocprofv3 --kernel-trace.
 """
@@ -25,6 +26,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--config", default="cfg4")
     ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--epochs", type=int, default=0, help="> 0: then train the network this many epochs")
     a = ap.parse_args()
     from eqcycle_amd import learned
     from eqcycle_amd.api import EqContext
@@ -52,6 +54,11 @@ def main():
     fit = learned.fit_basis(ctx, tol=cfg.pcg_tol)
     out = {"config": a.config, "snapshots": m, "nI": ctx.nI, "gram_wall_s": walls,
            "fit_basis_wall_s": time.perf_counter() - t, "energy_captured": fit["energy_captured"]}
+    if a.epochs > 0:
+        t = time.perf_counter()
+        hist = learned.train(ctx, epochs=a.epochs, batch=64, lr=1e-3)
+        out.update(train_epochs=a.epochs, train_wall_s=time.perf_counter() - t, loss_first=float(hist[0]),
+                   loss_last=float(hist[-1]))
     print(json.dumps(out), flush=True)
 
 

@@ -70,6 +70,16 @@ def test_gpu_gradient_matches_fp64_reference(trained):
     loss_b, grad_b = ctx.learn_loss_grad(None, idx, use_bf16=True)
     assert abs(loss_b - loss_r) <= 3e-2 * abs(loss_r)
     assert np.linalg.norm(grad_b - grad_r) / np.linalg.norm(grad_r) < 5e-2
+    # and exactly the FP64 network at the BF16-rounded weights (biases stay FP32)
+    pw = net.unpack(params)
+    for name, _ in net.layout:
+        if not name.startswith("b") and not name.endswith("b"):
+            pw[name] = _bf16(pw[name].astype(np.float32)).astype(np.float64)
+    loss_rb, grad_rb = net.loss_grad(net.pack(pw), data["X"][idx].astype(np.float64),
+                                     data["Y"][idx].astype(np.float64))
+    errb = np.linalg.norm(grad_b - grad_rb) / np.linalg.norm(grad_rb)
+    print(f"[gpu] gradient rel err bf16 weights {errb:.2e}", flush=True)
+    assert abs(loss_b - loss_rb) <= 1e-5 * abs(loss_rb) and errb < 1e-4
 
 
 def test_training_and_learned_guess_gain(trained):
