@@ -236,33 +236,6 @@ hipError_t launch_gram(const float* S, int64_t nI, int m, double* G, double* par
   return hipGetLastError();
 }
 
-// Phi[:, k] = sum_j S[:, j] W[j, k]  (W = V diag(1/sigma), row-major [m x NB])
-__global__ __launch_bounds__(256) void basis_kernel(const float* S, int64_t nI, int m, const float* W, int NB,
-                                                    float* Phi) {
-  __shared__ float Ws[16][NBMAX];
-  const int64_t row = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  float acc[NBMAX];
-#pragma unroll
-  for (int k = 0; k < NBMAX; k++) acc[k] = 0.f;
-  for (int j0 = 0; j0 < m; j0 += 16) {
-    for (int e = threadIdx.x; e < 16 * NBMAX; e += 256) {
-      const int jj = e / NBMAX, k = e - jj * NBMAX;
-      Ws[jj][k] = (j0 + jj < m && k < NB) ? W[(int64_t)(j0 + jj) * NB + k] : 0.f;
-    }
-    __syncthreads();
-    if (row < nI) {
-      for (int jj = 0; jj < 16 && j0 + jj < m; jj++) {
-        const float s = S[(int64_t)(j0 + jj) * nI + row];
-#pragma unroll
-        for (int k = 0; k < NBMAX; k++) acc[k] += s * Ws[jj][k];
-      }
-    }
-    __syncthreads();
-  }
-  if (row < nI)
-    for (int k = 0; k < NB; k++) Phi[(int64_t)k * nI + row] = acc[k];
-}
-
 // ---- network: training kernels --------------------------------------------
 __global__ void gather_kernel(const float* X, int L0, const int32_t* idx, int B, float* out) {
   const int64_t e = (int64_t)blockIdx.x * LB + threadIdx.x;
@@ -525,6 +498,11 @@ struct EpConvGradIn {   // din [B x Cin x Lin] from row (b, i), column ci, maske
     const int64_t e = ((int64_t)b * Cin + ci) * Lin + i;
     din[e] = in[e] > 0.f ? v : 0.f;
   }
+};
+struct EpColOut {       // column-major output: out[j ld + i] = v
+  float* out;
+  int64_t ld;
+  __device__ void operator()(int i, int j, float v) const { out[(int64_t)j * ld + i] = v; }
 };
 struct EpConvGradW {    // dW [Cout x Cin x 5] from row co, column (ci, k); column Cin * 5 is db
   float* dW;
@@ -938,8 +916,8 @@ int learn_set_basis(LearnState* ls, const double* V, const double* sigma, const 
   float* dW = dalloc<float>((int64_t)m * NB);
   if (!dW) { *err = "device allocation failed"; return 7; }
   LCHK(hipMemcpyAsync(dW, W.data(), sizeof(float) * W.size(), hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(basis_kernel, dim3((unsigned)((ls->nI + 255) / 256)), dim3(256), 0, s, ls->S, ls->nI, m, dW, NB,
-                     ls->Phi);
+  // Phi = S W on the matrix cores (FP32 GEMM, [nI x m] [m x NB])
+  gemm((int)ls->nI, NB, m, ColMat{ls->S, (int)ls->nI}, RowMat{dW, NB}, EpColOut{ls->Phi, ls->nI}, nullptr, 0, s);
   // Cholesky-QR pass: S V / sigma loses orthogonality in FP32 along the small
   // singular values, so G = Phi^T Phi (FP64), G = L L^T, Phi <- Phi L^-T
   // (the row-major copy is the scratch output, then the pointers swap)
@@ -981,8 +959,8 @@ int learn_set_basis(LearnState* ls, const double* V, const double* sigma, const 
       std::vector<float> Wf((size_t)NB * NB);
       for (size_t q = 0; q < Wf.size(); q++) Wf[q] = (float)Winv[q];
       LCHK(hipMemcpyAsync(dW, Wf.data(), sizeof(float) * Wf.size(), hipMemcpyHostToDevice, s));
-      hipLaunchKernelGGL(basis_kernel, dim3((unsigned)((ls->nI + 255) / 256)), dim3(256), 0, s, ls->Phi, ls->nI, NB,
-                         dW, NB, ls->Phi_rm);
+      gemm((int)ls->nI, NB, NB, ColMat{ls->Phi, (int)ls->nI}, RowMat{dW, NB}, EpColOut{ls->Phi_rm, ls->nI}, nullptr, 0,
+           s);
       std::swap(ls->Phi, ls->Phi_rm);
     }
   }
