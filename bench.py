@@ -61,18 +61,26 @@ TRAFFIC = {("newmark_kernel", 410881): 27.19e6,
            ("cheb_pair_kernel<float>", 410881): 14.29e6}
 
 
-def alg_bytes(kernel, nglob, prec):
+def alg_bytes(kernel, nglob, prec, smoother="tiles"):
     """Algorithmic HBM bytes of one launch: the values per node that the
     kernel must read and write once (halo re-reads are not counted)."""
     s = 4 if "float" in kernel else 8
+    # single Chebyshev steps: with tile sweeps only the residual after pre-smoothing
+    # (x, mu, b in; r out), otherwise a whole step
+    step = 4 * s if smoother == "tiles" else 8 * s
     per_node = {
         "newmark_kernel": 64,                    # predictor, v, a, M, mu in; v, a, next predictor out
         "cheb_pair_kernel<float>": 8 * s,        # window vector, mu, D^-1, r, x in; r, d, x out (one step's)
         "cheb_pair_kernel<double>": 8 * s,
-        "tile_kernel<float, OpCheb>": 8 * s,
-        "tile_kernel<double, OpCheb>": 8 * s,
+        "tile_kernel<float, OpCheb>": step,
+        "tile_kernel<double, OpCheb>": step,
         "tile_kernel<T, OpCgDir>": 5 * 8,        # c, p, mu in; p, q out (FP64 CG vectors)
         "cg_update_kernel<T>": 6 * 8 + (4 if prec == "f32pc" else 0),   # x, r, p, q in; x, r out (+ FP32 r)
+        # a whole smoothing sweep per launch, pre and post taking turns (mean of the two):
+        #   pre: b, mu, D^-1 in; x out.  post: x, b, mu, D^-1 in; c out; and with the FP32 V-cycle
+        #   of f32pc the FP64 residual in and c widened to FP64 out (4 s + 8 + 8).
+        "cheb_sweep_kernel<float>": (4 * s + (4 * s + 16 if prec == "f32pc" else 5 * s)) // 2,
+        "cheb_sweep_kernel<double>": (4 * s + 5 * s) // 2,
     }[kernel]
     return per_node * nglob
 
@@ -86,6 +94,8 @@ def parse():
     ap.add_argument("--tol", type=float, default=None)
     ap.add_argument("--hierarchy", default="pmg", help="pmg (default) | rs (Ruge-Stueben on K_II)")
     ap.add_argument("--degree", type=int, default=5, help="Chebyshev degree on the fine level (pmg)")
+    ap.add_argument("--smoother", default="tiles",
+                    help="fine-level smoother: tiles (one launch per Chebyshev sweep) | chebyshev | jacobi")
     ap.add_argument("--epochs", type=int, default=300)
     ap.add_argument("--graphs", type=int, default=1, help="replay CG batches as HIP graphs (1) or launch eagerly")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU budget per thread setting")
@@ -175,7 +185,7 @@ def variant(ctx, ck, tol, prec, guess, warmup, steps, graphs):
     return st, s, R
 
 
-def kernel_time(ctx, ck, tol, prec, guess, P, wall_pass):
+def kernel_time(ctx, ck, tol, prec, guess, P, wall_pass, smoother):
     """One more pass of the variant with every fine-level launch bracketed by
     HIP events (CG batches launch eagerly: same kernels); device time per
     kernel, the dominant one, and its roofline."""
@@ -189,7 +199,7 @@ def kernel_time(ctx, ck, tol, prec, guess, P, wall_pass):
         if n == 0:
             continue
         avg_us = ms * 1e3 / n
-        alg = alg_bytes(name, P.nglob, prec)
+        alg = alg_bytes(name, P.nglob, prec, smoother)
         table[name] = {"launches": n, "total_ms": ms, "avg_us": avg_us, "share_of_pass_wall": ms / 1e3 / wall_pass,
                        "alg_bytes_per_launch": alg, "achieved_GBs": alg / (avg_us * 1e-6) / 1e9}
     dom = max(table, key=lambda k: table[k]["total_ms"])
@@ -320,7 +330,7 @@ def main():
     cfg = CONFIGS[a.config]
     tol = a.tol or cfg.pcg_tol
     P = set_parameters(cfg)
-    ctx = EqContext(P, amg={"hierarchy": a.hierarchy, "degree": a.degree})
+    ctx = EqContext(P, amg={"hierarchy": a.hierarchy, "degree": a.degree, "smoother": a.smoother})
     learned = cfg.learned and not a.plain_only
 
     ck, training = checkpoint(ctx, cfg, tol, learned, a.epochs, a.graphs, a.nbasis)
@@ -355,7 +365,7 @@ def main():
         extra["galerkin_vs_plain"] = versus(gal, s_gal, R_gal)
         extra["network_vs_plain"] = versus(lrn, s_lrn, R_lrn)
         extra["linear_vs_plain"] = versus(lin, s_lin, R_lin)
-    roof, ktable, w_acc = kernel_time(ctx, ck, tol, prec, guess, P, main_st["wall_per_pass_s"])
+    roof, ktable, w_acc = kernel_time(ctx, ck, tol, prec, guess, P, main_st["wall_per_pass_s"], a.smoother)
     roof["newmark_isolated_us"] = ctx.time_kernel("newmark", "fp64", reps=200) * 1e3
     print(f"dominant kernel {roof['kernel']}: {roof['frac']:.3f} of HBM peak", file=sys.stderr, flush=True)
     out = {
@@ -375,7 +385,7 @@ def main():
                                    "its FP16 copy) + FP32 V-cycle, "
                                    "plain FP64, network-predicted and linear-predicted x0 alongside" if learned
                                    else "plain FP64")),
-                   "pcg_iters_per_solve": main_st["pcg_iters_per_solve"], "hierarchy": a.hierarchy,
+                   "pcg_iters_per_solve": main_st["pcg_iters_per_solve"], "hierarchy": a.hierarchy, "smoother": a.smoother,
                    "graphs": bool(a.graphs), "amg_levels": ctx.amg_info(), **extra,
                    "kernel_time": {"variant": f"{guess}/{prec}", "pass_wall_s_with_events": w_acc,
                                    "kernels": ktable}},

@@ -28,6 +28,8 @@ from .lib import f64, i64, ptr
 log = logging.getLogger("eqcycle_amd")
 
 PREC = {"fp64": L.EQ_PREC_FP64, "fp32": L.EQ_PREC_FP32, "mixed": L.EQ_PREC_MIXED, "f32pc": L.EQ_PREC_F32PC}
+SMOOTHERS = {"jacobi": L.EQ_SMOOTH_JACOBI, "chebyshev": L.EQ_SMOOTH_CHEBYSHEV,
+             "tiles": L.EQ_SMOOTH_CHEBYSHEV_TILES}
 PC = {"none": L.EQ_PC_NONE, "jacobi": L.EQ_PC_JACOBI, "amg": L.EQ_PC_AMG}
 MASK = {"all": L.EQ_MASK_ALL, "interior": L.EQ_MASK_INTERIOR, "neg": L.EQ_MASK_NEG,
         "fault_rows": L.EQ_MASK_FAULT_ROWS}
@@ -132,16 +134,27 @@ class EqContext:
                   smoother=None, degree=5):
         """hierarchy "rs": ruge_stuben on K_II with Jacobi smoothing (the reference's
         preconditioner structure); "pmg": degree-4 -> bilinear first level with
-        Chebyshev smoothing, then Ruge-Stueben (the default, far fewer iterations)."""
+        Chebyshev smoothing, then Ruge-Stueben (the default, far fewer iterations).
+        smoother "tiles" (pmg default): each Chebyshev sweep in one launch per tile
+        and its 2-node ring; "chebyshev": the same sweep on the whole mesh, one
+        launch per one or two steps; "jacobi"."""
         pc = {"rs": 0, "pmg": 1}[hierarchy]
         if smoother is None:
-            smoother = "chebyshev" if pc else "jacobi"
+            smoother = "tiles" if pc else "jacobi"
         if max_coarse is None:
             max_coarse = 2000 if pc else 10
-        sm = {"jacobi": L.EQ_SMOOTH_JACOBI, "chebyshev": L.EQ_SMOOTH_CHEBYSHEV}[smoother]
+        sm = SMOOTHERS[smoother]
         self._check(self.lib.eq_amg_setup(self.h, theta, max_levels, max_coarse, strength_abs, sm, degree, pc))
         self.hierarchy = hierarchy
         return self.amg_info()
+
+    def amg_l0_info(self):
+        """(smoother name, Chebyshev degree, lo, hi): the fine-level smoother and its
+        target interval of D^-1 K_II."""
+        sm, deg, lo, hi = C.c_int(), C.c_int(), C.c_double(), C.c_double()
+        self._check(self.lib.eq_amg_l0_info(self.h, C.byref(sm), C.byref(deg), C.byref(lo), C.byref(hi)))
+        name = {v: k for k, v in SMOOTHERS.items()}[sm.value]
+        return name, deg.value, lo.value, hi.value
 
     def amg_info(self):
         nl = C.c_int()
@@ -169,7 +182,7 @@ class EqContext:
         Aarr = (L.CsrDesc * nA)(*descs[:nA])
         Parr = (L.CsrDesc * max(1, len(P)))(*descs[nA:])
         om = None if omega is None else f64(omega)
-        sm = {"jacobi": L.EQ_SMOOTH_JACOBI, "chebyshev": L.EQ_SMOOTH_CHEBYSHEV}[smoother]
+        sm = SMOOTHERS[smoother]
         self._check(self.lib.eq_amg_upload(self.h, nA, Aarr, Parr, ptr(om), sm, degree))
         self.amg_levels = nA
 

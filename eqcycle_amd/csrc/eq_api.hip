@@ -29,6 +29,7 @@ using namespace eq;
 namespace {
 
 constexpr int kMaxChebDegree = 8;   // largest level-0 Chebyshev degree eq_amg_setup accepts
+static_assert(kMaxChebDegree <= kSweepMaxDeg, "tile sweeps hold every step's coefficients");
 
 template <typename T>
 struct CgWork {
@@ -133,6 +134,7 @@ struct eq_ctx {
   std::vector<int64_t> amg_n, amg_nnz;
   int pmg = 0;                        // level-0 prolongation is the bilinear vertex interpolation
   int l0_cheb = 0;                    // level-0 smoother: 0 Jacobi, 1 Chebyshev
+  int l0_sweep = 0;                   // Chebyshev as whole tile-local sweeps (one launch per sweep)
   int l0_degree = 1;
   double l0_theta = 1, l0_delta = 1;  // Chebyshev interval centre / half-width of D^-1 K_II
   // step records: slot 0 serves single steps, a batch of dynamic steps uses
@@ -774,7 +776,8 @@ static int upload_hierarchy(eq_ctx* ctx, const HostHierarchy& H, int smoother, i
   ctx->have_amg = false;
   for (auto& kv : ctx->cg_graphs) (void)hipGraphExecDestroy(kv.second);   // they hold the old levels
   ctx->cg_graphs.clear();
-  ctx->l0_cheb = smoother == EQ_SMOOTH_CHEBYSHEV;
+  ctx->l0_cheb = smoother == EQ_SMOOTH_CHEBYSHEV || smoother == EQ_SMOOTH_CHEBYSHEV_TILES;
+  ctx->l0_sweep = smoother == EQ_SMOOTH_CHEBYSHEV_TILES;
   ctx->l0_degree = ctx->l0_cheb ? smooth_degree : 1;
   {
     // Chebyshev target interval [lambda_max / EQ_CHEB_LOWER, 1.1 lambda_max] of D^-1 K_II
@@ -835,8 +838,9 @@ extern "C" int eq_amg_setup(eq_ctx* ctx, double theta, int max_levels, int64_t m
   if (!ctx) return EQ_EINVAL;
   ctx->qs_r_valid = false;
   if (!ctx->have_material) return fail(ctx, EQ_ENOTREADY, "eq_set_material first");
-  if (smoother != EQ_SMOOTH_JACOBI && smoother != EQ_SMOOTH_CHEBYSHEV) return fail(ctx, EQ_EINVAL, "bad smoother");
-  if (smoother == EQ_SMOOTH_CHEBYSHEV && (smooth_degree < 1 || smooth_degree > 8))
+  if (smoother != EQ_SMOOTH_JACOBI && smoother != EQ_SMOOTH_CHEBYSHEV && smoother != EQ_SMOOTH_CHEBYSHEV_TILES)
+    return fail(ctx, EQ_EINVAL, "bad smoother");
+  if (smoother != EQ_SMOOTH_JACOBI && (smooth_degree < 1 || smooth_degree > kMaxChebDegree))
     return fail(ctx, EQ_EINVAL, "Chebyshev degree must be 1..8");
   if (max_levels < 2) return fail(ctx, EQ_EINVAL, "max_levels must be >= 2");
   HostCsr Kii;
@@ -863,8 +867,9 @@ extern "C" int eq_amg_upload(eq_ctx* ctx, int nlev, const eq_csr* A, const eq_cs
   ctx->qs_r_valid = false;
   if (!ctx->have_material) return fail(ctx, EQ_ENOTREADY, "eq_set_material first");
   if (nlev < 2 || nlev > 32) return fail(ctx, EQ_EINVAL, "nlev must be 2..32");
-  if (smoother != EQ_SMOOTH_JACOBI && smoother != EQ_SMOOTH_CHEBYSHEV) return fail(ctx, EQ_EINVAL, "bad smoother");
-  if (smoother == EQ_SMOOTH_CHEBYSHEV && (smooth_degree < 1 || smooth_degree > 8))
+  if (smoother != EQ_SMOOTH_JACOBI && smoother != EQ_SMOOTH_CHEBYSHEV && smoother != EQ_SMOOTH_CHEBYSHEV_TILES)
+    return fail(ctx, EQ_EINVAL, "bad smoother");
+  if (smoother != EQ_SMOOTH_JACOBI && (smooth_degree < 1 || smooth_degree > kMaxChebDegree))
     return fail(ctx, EQ_EINVAL, "Chebyshev degree must be 1..8");
   if (omega)
     for (int l = 0; l < nlev; l++)
@@ -956,6 +961,16 @@ extern "C" int eq_amg_upload(eq_ctx* ctx, int nlev, const eq_csr* A, const eq_cs
   std::string err;
   if (int rc = finish_hierarchy(&H, omega, &err)) return fail(ctx, rc, err);
   return upload_hierarchy(ctx, H, smoother, smooth_degree, 0);
+}
+
+extern "C" int eq_amg_l0_info(const eq_ctx* ctx, int* smoother, int* degree, double* lo, double* hi) {
+  if (!ctx || !smoother || !degree || !lo || !hi) return EQ_EINVAL;
+  if (!ctx->have_amg) return EQ_ENOTREADY;
+  *smoother = ctx->l0_sweep ? EQ_SMOOTH_CHEBYSHEV_TILES : ctx->l0_cheb ? EQ_SMOOTH_CHEBYSHEV : EQ_SMOOTH_JACOBI;
+  *degree = ctx->l0_degree;
+  *lo = ctx->l0_theta - ctx->l0_delta;
+  *hi = ctx->l0_theta + ctx->l0_delta;
+  return EQ_OK;
 }
 
 extern "C" int eq_amg_info(const eq_ctx* ctx, int* nlev, int64_t* n, int64_t* nnz, int cap) {
@@ -1114,6 +1129,72 @@ static void cheb_post(eq_ctx* ctx, const T* b, T* c, const T* coarse, double* c6
   cheb_launch<T>(ctx, st, k);
 }
 
+// Parameters of a whole-sweep launch: the step coefficients of cheb_pre and
+// cheb_post (the same recurrence), the first direction's scale 1 / theta.
+template <typename T>
+static ChebSweep<T> sweep_params(eq_ctx* ctx, int post) {
+  ChebSweep<T> op{};
+  const int k = ctx->l0_degree;
+  const double th = ctx->l0_theta, de = ctx->l0_delta, sg = th / de;
+  double rho_prev = 1.0 / sg;
+  for (int m = 1; m < k; m++) {
+    const double rho = 1.0 / (2.0 * sg - rho_prev);
+    op.c1[m] = (T)(rho * rho_prev);
+    op.c2[m] = (T)(2.0 * rho / de);
+    rho_prev = rho;
+  }
+  op.post = post;
+  op.degree = k;
+  op.s0 = (T)(1.0 / th);
+  op.nelx = ctx->g.nelx;
+  op.nely = ctx->g.nely;
+  for (int p = 0; p < 5; p++) op.gx[p] = (T)ctx->g.x[p];
+  op.mu = mu_of<T>(ctx);
+  op.dinv = dinv_of<T>(ctx);
+  op.part_rr = ctx->part_rr;
+  op.nrr = ctx->npart;
+  op.sc = ctx->sc;
+  op.part0 = ctx->part_rho;
+  return op;
+}
+
+// Tile-sweep pre-smoothing of b from x = 0 into chx (one launch, carrying the
+// cg! convergence test), then the residual b - K_II chx; returns its buffer.
+template <typename T>
+static const T* sweep_pre(eq_ctx* ctx, const T* b) {
+  CgWork<T>& w = work<T>(ctx);
+  ChebSweep<T> op = sweep_params<T>(ctx, 0);
+  op.b = b;
+  op.xout = w.chx;
+  op.check = 1;
+  launch_cheb_sweep<T>(ctx->g, op, ctx->s, lt_slot(ctx, sizeof(T) == 4 ? EQ_TK_SWEEP32 : EQ_TK_SWEEP64));
+  ChebStep<T> st;
+  st.src = w.chx;
+  st.rin = b;
+  st.rout = w.chr[0];
+  st.mode = 1;
+  launch_cheb<T>(ctx->g, st, mu_of<T>(ctx), dinv_of<T>(ctx), ctx->part_rr, ctx->npart, ctx->sc, ctx->part_rho,
+                 ctx->s, lt_slot(ctx, sizeof(T) == 4 ? EQ_TK_CHEB_STEP32 : EQ_TK_CHEB_STEP64));
+  return w.chr[0];
+}
+
+// Tile-sweep post-smoothing from chx (+ P0 coarse): output c and partial
+// <c, b> (with c64: c widened to FP64 into c64 and partial <c, r64>).
+template <typename T>
+static void sweep_post(eq_ctx* ctx, const T* b, T* c, const T* coarse, double* c64 = nullptr,
+                       const double* r64 = nullptr) {
+  CgWork<T>& w = work<T>(ctx);
+  ChebSweep<T> op = sweep_params<T>(ctx, 1);
+  op.b = b;
+  op.src = w.chx;
+  op.coarse = coarse;
+  op.xout = c;
+  op.rdot = b;
+  op.xout64 = c64;
+  op.rdot64 = r64;
+  launch_cheb_sweep<T>(ctx->g, op, ctx->s, lt_slot(ctx, sizeof(T) == 4 ? EQ_TK_SWEEP32 : EQ_TK_SWEEP64));
+}
+
 // c64/r64 (matrix-free Chebyshev level 0 only): the last smoothing step
 // writes c widened to FP64 into c64 and the partial <c, r64>.
 template <typename T>
@@ -1126,7 +1207,7 @@ static void vcycle(eq_ctx* ctx, const T* r, T* c, double* c64 = nullptr, const d
   const T om0 = (T)L0.omega;
   T* x0 = ctx->l0_cheb ? w.chx : w.x0;
   if (ctx->l0_cheb) {
-    const T* res = cheb_pre<T>(ctx, r);
+    const T* res = ctx->l0_sweep ? sweep_pre<T>(ctx, r) : cheb_pre<T>(ctx, r);
     launch_csr<T>(CSR_SPMV, L0.R, res + g.NX, nullptr, nullptr, T(0), lvecs<T>(ctx->lev[1]).b, nullptr, ctx->sc,
                   s);
   } else {
@@ -1151,7 +1232,9 @@ static void vcycle(eq_ctx* ctx, const T* r, T* c, double* c64 = nullptr, const d
   if (!fused_prolong)
     launch_csr<T>(CSR_PROLONG_ADD, L0.P, lvecs<T>(ctx->lev[1]).xp, nullptr, nullptr, T(0), x0 + g.NX, nullptr,
                   ctx->sc, s);
-  if (ctx->l0_cheb)
+  if (ctx->l0_sweep)
+    sweep_post<T>(ctx, r, c, fused_prolong ? lvecs<T>(ctx->lev[1]).xp : nullptr, c64, r64);
+  else if (ctx->l0_cheb)
     cheb_post<T>(ctx, r, c, fused_prolong ? lvecs<T>(ctx->lev[1]).xp : nullptr, c64, r64);
   else
     launch_l0_post<T>(g, w.x0, r, mu_of<T>(ctx), dinv_of<T>(ctx), om0, c, ctx->part_rho, ctx->sc, s);

@@ -160,6 +160,37 @@ void launch_cheb_pair(const Geom& g, const ChebStep<T>& a, const ChebStep<T>& b,
                       const double* part_rr, int nrr, CgScal* sc, double* part_rho, hipStream_t s,
                       LaunchEvents le = {});
 
+// A whole level-0 Chebyshev sweep of degree `degree` per tile in one launch,
+// on the tile and a ring of 2 nodes (restricted additive Schwarz; see
+// cheb_sweep_kernel).  Step m = 1 .. degree-1 uses c1[m], c2[m]; the first
+// direction is s0 D^-1 r.
+constexpr int kSweepMaxDeg = 8;
+template <typename T>
+struct ChebSweep {
+  int post = 0;                  // 0: from x = 0 (pre-smoothing); 1: from src (+ P0 coarse)
+  int degree = 1;
+  T s0 = 0;
+  T c1[kSweepMaxDeg] = {}, c2[kSweepMaxDeg] = {};
+  const T* b = nullptr;          // right-hand side of the smoothed system
+  const T* src = nullptr;        // post: x before the sweep
+  const T* coarse = nullptr;     // post: vertex correction added to src on the fly (or null)
+  int nelx = 0, nely = 0;
+  T gx[5] = {};                  // GLL nodes (bilinear prolongation weights)
+  const T* mu = nullptr;
+  const T* dinv = nullptr;
+  T* xout = nullptr;             // pre: x; post: c (unless xout64)
+  const T* rdot = nullptr;       // post: partial <c, rdot> per tile (or <c, rdot64>)
+  double* xout64 = nullptr;      // post of an FP32 sweep: c widened to FP64
+  const double* rdot64 = nullptr;
+  int check = 0;                 // carries the cg! convergence test (first kernel of a V-cycle)
+  const double* part_rr = nullptr;
+  int nrr = 0;
+  CgScal* sc = nullptr;
+  double* part0 = nullptr;       // post: one partial per tile
+};
+template <typename T>
+void launch_cheb_sweep(const Geom& g, const ChebSweep<T>& op, hipStream_t s, LaunchEvents le = {});
+
 struct DevCsr {
   int64_t n = 0, m = 0, nnz = 0;
   int32_t* rowptr = nullptr;

@@ -2075,6 +2075,218 @@ void launch_cheb_pair(const Geom& g, const ChebStep<T>& a, const ChebStep<T>& b,
   launch_timed(cheb_pair_kernel<T>, tile_grid(g), dim3(PBLOCK), s, le, g, op);
 }
 
+// --------------------------------------------------------------------------
+// A whole level-0 Chebyshev sweep per tile in one launch (restricted additive
+// Schwarz with a ring of SW_RING nodes)
+// --------------------------------------------------------------------------
+// The block of a TILE_I x TILE_J tile works on the window W: the tile and a
+// ring of SW_RING = 2 nodes around it (68 x 20 nodes).  The sweep's residual is
+// restricted to W, its k - 1 products with K_II are those of K_II[W, W]
+// (directions vanish outside W and on the fault row), and only the tile's own
+// nodes are written:
+//   x <- x + R_tile p(D_W^-1 K_W) D_W^-1 (b - K_II x)|_W
+// with p the degree-k Chebyshev polynomial of the global sweep.  The stencil of
+// a node of W reaches the elements that hold it, so the LDS arrays cover the
+// element-aligned box around W: columns I0-4 .. I0+TILE_I+4 and rows
+// J0-4 .. J0+TILE_J+4 (SW_I x SW_J); W starts at box index SW_OFF.
+//   pre  (post = 0): from x = 0: r = b, d0 = s0 D^-1 b, x = d0, then for
+//        m = 1 .. k-1: r -= K_W d, d = c1[m] d + c2[m] D^-1 r, x += d.
+//   post (post = 1): v = src (+ P0 coarse, formed in the box load) on the whole
+//        box; r = b - K_II v on W, d0 = s0 D^-1 r, x = v + d0, then the same
+//        k - 1 local steps; output c = x and the tile's partial <c, rdot>.
+// Each node of W keeps r, x, d and D^-1 in registers of the thread that owns
+// its column strip; d goes through LDS (two buffers) for the stencils.
+constexpr int SW_RING = 2;
+constexpr int SW_OFF = 4 - SW_RING;              // box index of W's first row / column
+constexpr int SW_WI = TILE_I + 2 * SW_RING;      // window columns
+constexpr int SW_WJ = TILE_J + 2 * SW_RING;      // window rows
+constexpr int SW_I = TILE_I + 9;                 // box columns
+constexpr int SW_J = TILE_J + 9;                 // box rows
+constexpr int SW_BOX = SW_I * SW_J;
+constexpr int SW_BLOCK = 512;
+constexpr int SW_XS = TILE_I / 4 + 2;            // x-strips (element segments) per window row
+constexpr int SW_YS = TILE_J / 4 + 2;            // y-strips per window column
+constexpr int SW_BOX_LOADS = (SW_BOX + SW_BLOCK - 1) / SW_BLOCK;
+static_assert(SW_WJ * SW_XS <= SW_BLOCK && SW_WI * SW_YS <= SW_BLOCK, "one strip per thread and pass");
+static_assert(TILE_I % 4 == 0 && TILE_J % 4 == 0, "element-aligned tiles");
+
+template <typename T>
+__global__ __launch_bounds__(SW_BLOCK) void cheb_sweep_kernel(Geom g, ChebSweep<T> op) {
+  __shared__ T sd[2][SW_BOX];       // directions (post: sd[0] first holds v)
+  __shared__ T sm[SW_BOX];          // mu
+  __shared__ T sx[SW_WJ * SW_WI];   // x-direction terms on W
+  if (op.sc->done) return;          // block-uniform, as OpCheb::prepare
+  if (op.check) {
+    const double rr = reduce_parts(op.part_rr, op.nrr);
+    const double resid = sqrt(rr);
+    const bool fin = (resid <= op.sc->reltol) || (op.sc->iter >= op.sc->maxiter);
+    if (threadIdx.x == 0 && blockIdx.x == 0 && blockIdx.y == 0) {
+      op.sc->resid = resid;
+      if (fin) op.sc->done = 1;
+    }
+    if (fin) return;
+  }
+  const int tile = xcd_tile(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+  const int I0 = (tile % gridDim.x) * TILE_I, J0 = (tile / gridDim.x) * TILE_J;
+  const int tid = threadIdx.x;
+  const int NX = g.NX, NY = g.NY;
+  // y-pass strip of this thread: window column yc, box rows 4 yk .. 4 yk + 3
+  const int yc = tid % SW_WI, yk = tid / SW_WI;
+  const int Iy = I0 - SW_RING + yc, bcy = yc + SW_OFF;
+  const bool ycol = tid < SW_WI * SW_YS && Iy >= 0 && Iy < NX;
+  T r[4], x[4], d[4], di[4];
+  double rd[4];
+  bool act[4], own[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int br = 4 * yk + j, J = J0 - 4 + br;
+    act[j] = ycol && br >= SW_OFF && br < SW_OFF + SW_WJ && J >= 1 && J < NY;
+    own[j] = ycol && br >= 4 && br < 4 + TILE_J && bcy >= 4 && bcy < 4 + TILE_I && J < NY;
+    r[j] = x[j] = d[j] = di[j] = T(0);
+    rd[j] = 0.0;
+    if (act[j]) {
+      const int64_t n = (int64_t)J * NX + Iy;
+      r[j] = op.b[n];
+      di[j] = op.dinv[n];
+      if (op.post && own[j]) rd[j] = op.rdot64 ? op.rdot64[n] : (double)op.rdot[n];
+    }
+  }
+  {
+    T ub[SW_BOX_LOADS], mb[SW_BOX_LOADS];
+#pragma unroll
+    for (int t = 0; t < SW_BOX_LOADS; t++) {
+      const int k = tid + t * SW_BLOCK;
+      const int br = k / SW_I, bc = k - br * SW_I;
+      const int I = I0 - 4 + bc, J = J0 - 4 + br;
+      ub[t] = mb[t] = T(0);
+      if (k < SW_BOX && I >= 0 && I < NX && J >= 0 && J < NY) {
+        const int64_t n = (int64_t)J * NX + I;
+        mb[t] = op.mu[n];
+        if (op.post)
+          ub[t] = cheb_source(op.src, op.b, op.dinv, T(0), op.coarse ? 2 : 0, op.coarse, op.nelx, op.nely, op.gx,
+                              n, I, J);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < SW_BOX_LOADS; t++) {
+      const int k = tid + t * SW_BLOCK;
+      if (k < SW_BOX) {
+        sm[k] = mb[t];
+        sd[0][k] = ub[t];
+        sd[1][k] = T(0);
+      }
+    }
+  }
+  __syncthreads();
+  // x-direction terms of K applied to su on W
+  auto xpass = [&](const T* su) {
+    if (tid < SW_WJ * SW_XS) {
+      const int wr = tid / SW_XS, k = tid - wr * SW_XS;
+      const int br = wr + SW_OFF, J = J0 - 4 + br;
+      if (J >= 1 && J < NY) {
+        const int sg = (I0 >> 2) - 1 + k;   // element segment along x: box columns 4k .. 4k+4
+        const T* up = su + br * SW_I + 4 * k;
+        const T* mp = sm + br * SW_I + 4 * k;
+        T o[5] = {0, 0, 0, 0, 0};
+        if (sg >= 0 && sg < g.nelx) seg_all<T, 1>(g, up, mp, o);
+        if (k > 0 && sg > 0 && sg - 1 < g.nelx) o[0] += seg_last<T, 1>(g, up - 4, mp - 4);
+        const T cx = (T)(g.c1 * wline(g, J, NY));
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const int bc = 4 * k + i;
+          if (bc >= SW_OFF && bc < SW_OFF + SW_WI) sx[wr * SW_WI + bc - SW_OFF] = cx * o[i];
+        }
+      }
+    }
+  };
+  // (K su) at this thread's active nodes
+  auto ypass = [&](const T* su, T ku[4]) {
+    if (!ycol) return;
+    const int tg = (J0 >> 2) - 1 + yk;     // element segment along y: box rows 4 yk .. 4 yk + 4
+    const T* up = su + 4 * yk * SW_I + bcy;
+    const T* mp = sm + 4 * yk * SW_I + bcy;
+    T o[5] = {0, 0, 0, 0, 0};
+    if (tg >= 0 && tg < g.nely) seg_all<T, SW_I>(g, up, mp, o);
+    if (yk > 0 && tg > 0 && tg - 1 < g.nely) o[0] += seg_last<T, SW_I>(g, up - 4 * SW_I, mp - 4 * SW_I);
+    const T cy = (T)(g.c2 * wline(g, Iy, NX));
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      if (act[j]) ku[j] = sx[(4 * yk + j - SW_OFF) * SW_WI + yc] + cy * o[j];
+  };
+  int cur = 0;
+  if (!op.post) {
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      if (act[j]) {
+        d[j] = op.s0 * di[j] * r[j];
+        x[j] = d[j];
+        sd[0][(4 * yk + j) * SW_I + bcy] = d[j];
+      }
+    __syncthreads();
+  } else {
+    xpass(sd[0]);
+    __syncthreads();
+    T ku[4];
+    ypass(sd[0], ku);
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      if (act[j]) {
+        const int k = (4 * yk + j) * SW_I + bcy;
+        r[j] = r[j] - ku[j];
+        d[j] = op.s0 * di[j] * r[j];
+        x[j] = sd[0][k] + d[j];
+        sd[1][k] = d[j];
+      }
+    __syncthreads();
+    cur = 1;
+  }
+#pragma unroll
+  for (int m = 1; m < kSweepMaxDeg; m++) {   // unrolled: c1[m], c2[m] at constant offsets
+    if (m >= op.degree) break;
+    T* nxt = sd[1 - cur];
+    xpass(sd[cur]);
+    if (op.post && m == 1)   // sd[0] held v on the whole box: clear it before it takes directions
+      for (int k = tid; k < SW_BOX; k += SW_BLOCK) nxt[k] = T(0);
+    __syncthreads();
+    T ku[4];
+    ypass(sd[cur], ku);
+    const T c1 = op.c1[m], c2 = op.c2[m];
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      if (act[j]) {
+        r[j] = r[j] - ku[j];
+        d[j] = c1 * d[j] + c2 * di[j] * r[j];
+        x[j] = x[j] + d[j];
+        nxt[(4 * yk + j) * SW_I + bcy] = d[j];
+      }
+    __syncthreads();
+    cur = 1 - cur;
+  }
+  double acc = 0.0;
+#pragma unroll
+  for (int j = 0; j < 4; j++)
+    if (own[j]) {
+      const int J = J0 - 4 + 4 * yk + j;
+      const int64_t n = (int64_t)J * NX + Iy;   // x = 0 on the fault row
+      if (!op.post) {
+        op.xout[n] = x[j];
+      } else {
+        if (op.xout64) op.xout64[n] = (double)x[j];
+        else op.xout[n] = x[j];
+        acc += (double)x[j] * rd[j];
+      }
+    }
+  if (op.post) {
+    const double s = block_sum(acc);
+    if (threadIdx.x == 0) op.part0[tile] = s;
+  }
+}
+
+template <typename T>
+void launch_cheb_sweep(const Geom& g, const ChebSweep<T>& op, hipStream_t s, LaunchEvents le) {
+  launch_timed(cheb_sweep_kernel<T>, tile_grid(g), dim3(SW_BLOCK), s, le, g, op);
+}
+
 template <typename T, int MODE>
 static void csr_dispatch(const DevCsr& M, const T* val, const T* x, const T* b, const T* dinv, T omega, T* y, T* xout,
                          const CgScal* sc, hipStream_t s) {
@@ -2310,6 +2522,7 @@ void launch_scale_mask(int64_t n, int nx, const double* x, double sign, double* 
                                     const double*, int, CgScal*, double*, hipStream_t, LaunchEvents);             \
   template void launch_cheb<T>(const Geom&, const ChebStep<T>&, const T*, const T*, const double*, int, CgScal*,   \
                                double*, hipStream_t, LaunchEvents);                                                \
+  template void launch_cheb_sweep<T>(const Geom&, const ChebSweep<T>&, hipStream_t, LaunchEvents);            \
   template void launch_fill<T>(int64_t, T*, T, hipStream_t);
 EQ_INST(double)
 EQ_INST(float)

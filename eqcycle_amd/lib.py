@@ -22,13 +22,13 @@ STATUS_NAMES = {0: "EQ_OK", 1: "EQ_EINVAL", 2: "EQ_HIP_ERR", 3: "EQ_NR_NOCONV", 
 EQ_MASK_ALL, EQ_MASK_INTERIOR, EQ_MASK_NEG, EQ_MASK_FAULT_ROWS = 0, 1, 2, 3
 EQ_PREC_FP64, EQ_PREC_FP32, EQ_PREC_MIXED, EQ_PREC_F32PC = 0, 1, 2, 3
 EQ_PC_NONE, EQ_PC_JACOBI, EQ_PC_AMG = 0, 1, 2
-EQ_SMOOTH_JACOBI, EQ_SMOOTH_CHEBYSHEV = 0, 1
+EQ_SMOOTH_JACOBI, EQ_SMOOTH_CHEBYSHEV, EQ_SMOOTH_CHEBYSHEV_TILES = 0, 1, 2
 EQ_GUESS_PREVIOUS, EQ_GUESS_LEARNED, EQ_GUESS_GALERKIN, EQ_GUESS_LINEAR = 0, 1, 2, 3
 
 # every symbol the header declares (checked by the CPU test-suite)
 EXPORTS = [
     "eq_create", "eq_destroy", "eq_last_error", "eq_sizes", "eq_set_material", "eq_set_params",
-    "eq_ku_apply", "eq_kni_apply", "eq_element_computation", "eq_amg_setup", "eq_amg_info",
+    "eq_ku_apply", "eq_kni_apply", "eq_element_computation", "eq_amg_setup", "eq_amg_info", "eq_amg_l0_info",
     "eq_amg_ldiv", "eq_cg", "eq_pcg_jacobi", "eq_ids", "eq_ids2", "eq_nrsearch", "eq_fbc",
     "eq_slr", "eq_dtevol", "eq_set_solver", "eq_state_init", "eq_state_get", "eq_state_set",
     "eq_step", "eq_run", "eq_time_kernel", "eq_learn_init", "eq_learn_record", "eq_learn_info",
@@ -41,7 +41,7 @@ EQ_TIME_KU, EQ_TIME_CG_DIR, EQ_TIME_VCYCLE, EQ_TIME_CG_ITER, EQ_TIME_NEWMARK, EQ
     EQ_TIME_CHEB2 = range(8)
 TK_NAMES = ["newmark_kernel", "cheb_pair_kernel<float>", "cheb_pair_kernel<double>",
             "tile_kernel<float, OpCheb>", "tile_kernel<double, OpCheb>", "tile_kernel<T, OpCgDir>",
-            "cg_update_kernel<T>"]
+            "cg_update_kernel<T>", "cheb_sweep_kernel<float>", "cheb_sweep_kernel<double>"]
 EQ_TK_COUNT = len(TK_NAMES)
 
 dp = C.POINTER(C.c_double)
@@ -127,6 +127,7 @@ def load():
         "eq_element_computation": ([vp, dp, dp], C.c_int),
         "eq_amg_setup": ([vp, C.c_double, C.c_int, C.c_int64, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
         "eq_amg_info": ([vp, C.POINTER(C.c_int), i64p, i64p, C.c_int], C.c_int),
+        "eq_amg_l0_info": ([vp, C.POINTER(C.c_int), C.POINTER(C.c_int), dp, dp], C.c_int),
         "eq_amg_ldiv": ([vp, dp, dp, C.c_int], C.c_int),
         "eq_cg": ([vp, dp, dp, C.c_double, C.c_int64, C.c_int, C.c_int, i64p, dp], C.c_int),
         "eq_pcg_jacobi": ([vp, dp, dp, C.c_double, C.c_int64, i64p], C.c_int),

@@ -37,7 +37,7 @@ enum {
 enum { EQ_MASK_ALL = 0, EQ_MASK_INTERIOR = 1, EQ_MASK_NEG = 2, EQ_MASK_FAULT_ROWS = 3 };
 enum { EQ_PREC_FP64 = 0, EQ_PREC_FP32 = 1, EQ_PREC_MIXED = 2, EQ_PREC_F32PC = 3 };
 enum { EQ_PC_NONE = 0, EQ_PC_JACOBI = 1, EQ_PC_AMG = 2 };
-enum { EQ_SMOOTH_JACOBI = 0, EQ_SMOOTH_CHEBYSHEV = 1 };
+enum { EQ_SMOOTH_JACOBI = 0, EQ_SMOOTH_CHEBYSHEV = 1, EQ_SMOOTH_CHEBYSHEV_TILES = 2 };
 enum { EQ_GUESS_PREVIOUS = 0, EQ_GUESS_LEARNED = 1, EQ_GUESS_GALERKIN = 2, EQ_GUESS_LINEAR = 3 };
 
 typedef struct eq_ctx eq_ctx;
@@ -107,7 +107,11 @@ int eq_element_computation(eq_ctx *ctx, const double *u, double *a_elem);
  * strength_abs = 1 is the |a_ij| strength test of AlgebraicMultigrid.jl's
  * Classical(theta); 0 uses -a_ij.  smoother: EQ_SMOOTH_JACOBI on every level,
  * or EQ_SMOOTH_CHEBYSHEV of smooth_degree on the matrix-free fine level
- * (Jacobi on the assembled coarse levels). */
+ * (Jacobi on the assembled coarse levels), one launch per one or two steps;
+ * EQ_SMOOTH_CHEBYSHEV_TILES applies the same polynomial per 64 x 16 tile on
+ * the tile and a ring of 2 nodes (restricted additive Schwarz: the sweep's
+ * products are those of K_II restricted to that window, only the tile's own
+ * nodes are kept), one launch per sweep. */
 int eq_amg_setup(eq_ctx *ctx, double theta, int max_levels, int64_t max_coarse, int strength_abs,
                  int smoother, int smooth_degree, int pcoarsen);
 /* A sparse matrix in 0-based CSR: rowptr[n + 1], col and val[rowptr[n]]. */
@@ -150,6 +154,10 @@ int eq_rs_split(const eq_rs *h, int l, int *is_c);
 void eq_rs_free(eq_rs *h);
 /* Per-level sizes: n[l], nnz[l] for l < *nlev (arrays of capacity cap). */
 int eq_amg_info(const eq_ctx *ctx, int *nlev, int64_t *n, int64_t *nnz, int cap);
+/* Fine-level smoother of the current hierarchy (no reference counterpart;
+ * for tests that restate the V-cycle): EQ_SMOOTH_*, the Chebyshev degree and
+ * the target interval [lo, hi] of D^-1 K_II the polynomial is built on. */
+int eq_amg_l0_info(const eq_ctx *ctx, int *smoother, int *degree, double *lo, double *hi);
 /* x = Pl \ b for kni (ldiv!(x, p, b)); x, b in FltNI order. */
 int eq_amg_ldiv(eq_ctx *ctx, const double *b, double *x, int prec);
 
@@ -365,7 +373,9 @@ enum { EQ_TK_NEWMARK = 0,     /* newmark_kernel: one dynamic step */
        EQ_TK_CHEB_STEP64 = 4, /* one fine-level Chebyshev step, FP64 */
        EQ_TK_CG_DIR = 5,      /* p = c + beta p, q = K_II p (any precision) */
        EQ_TK_CG_UPDATE = 6,   /* x += alpha p, r -= alpha q (any precision) */
-       EQ_TK_COUNT = 7 };
+       EQ_TK_SWEEP32 = 7,     /* cheb_sweep_kernel<float>: a whole fine-level smoothing sweep */
+       EQ_TK_SWEEP64 = 8,     /* cheb_sweep_kernel<double> */
+       EQ_TK_COUNT = 9 };
 /* on = 1: clear the counters and give every launch of the kinds above a pair
  * of HIP events on the context's stream until on = 0.  The launch goes
  * through hipExtLaunchKernelGGL, so the kernel's own dispatch stamps its start

@@ -34,7 +34,7 @@ const EQ_PCG_NOCONV = 5
 const EQ_MASK_ALL, EQ_MASK_INTERIOR, EQ_MASK_NEG, EQ_MASK_FAULT_ROWS = 0, 1, 2, 3
 const EQ_PREC_FP64, EQ_PREC_FP32, EQ_PREC_MIXED, EQ_PREC_F32PC = 0, 1, 2, 3
 const EQ_PC_NONE, EQ_PC_JACOBI, EQ_PC_AMG = 0, 1, 2
-const EQ_SMOOTH_JACOBI, EQ_SMOOTH_CHEBYSHEV = 0, 1
+const EQ_SMOOTH_JACOBI, EQ_SMOOTH_CHEBYSHEV, EQ_SMOOTH_CHEBYSHEV_TILES = 0, 1, 2
 
 struct MeshDesc
     ngll::Int64
@@ -106,7 +106,7 @@ function Context(P; W, iglob, NelX, NelY, dxe, dye, xgll, wgll, H, amg = true)
     set_params!(ctx, P)
     amg && check(ctx, ccall((:eq_amg_setup, LIB), Cint,
                            (Ptr{Cvoid}, Cdouble, Cint, Int64, Cint, Cint, Cint, Cint),
-                           ctx.h, 0.25, 10, 2000, 1, 1, 5, 1))
+                           ctx.h, 0.25, 10, 2000, 1, EQ_SMOOTH_CHEBYSHEV_TILES, 5, 1))
     return use_context!(ctx)
 end
 

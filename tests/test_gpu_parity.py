@@ -175,7 +175,7 @@ def test_cheb_pairs_equal_single_steps(degree):
     from eqcycle_amd.api import EqContext
     P = set_parameters(small(40, 13, material="trapezoid", lognormal_sigma=0.3))
     print(f"[gpu] cheb pairs degree {degree} nglob={P.nglob}", flush=True)
-    ctx = EqContext(P, amg={"hierarchy": "pmg", "degree": degree})
+    ctx = EqContext(P, amg={"hierarchy": "pmg", "degree": degree, "smoother": "chebyshev"})
     b = normal(41, P.nglob - P.FltNglob)
     out = {}
     for pairs in (True, False):
@@ -194,7 +194,7 @@ def test_cheb_pairs_run_equals_single_steps():
     launches (CG batches replayed as graphs): identical histories and state."""
     from eqcycle_amd.api import EqContext
     P = set_parameters(small(32, 8, material="trapezoid", total_years=3))
-    ctx = EqContext(P, amg={"hierarchy": "pmg", "degree": 5})
+    ctx = EqContext(P, amg={"hierarchy": "pmg", "degree": 5, "smoother": "chebyshev"})
     res = {}
     for pairs in (True, False):
         ctx.set_solver(tol=1e-8, pc="amg", prec="f32pc", cheb_pairs=pairs)
@@ -282,14 +282,24 @@ def test_amg_upload_matches_numpy_vcycle(mesh, oracle):
         ctx.amg_upload([1.01 * A[0]] + A[1:], Pm)
 
 
-def test_amg_ldiv_is_spd(mesh):
-    P, ctx, _, kni = mesh
+@pytest.mark.parametrize("smoother", ["chebyshev", "tiles"])
+def test_amg_ldiv_is_spd(smoother):
+    """kni is negative definite, so is its preconditioner.  With whole-mesh
+    Chebyshev sweeps the V-cycle is symmetric to rounding; with tile sweeps
+    (restricted additive Schwarz, the same sweep before and after the coarse
+    correction) only nearly so: it is pinned to 1e-2 here."""
+    from eqcycle_amd.api import EqContext
+    P = set_parameters(small(40, 40, material="trapezoid", lognormal_sigma=0.3))
+    ctx = EqContext(P, amg={"smoother": smoother})
     n = P.nglob - P.FltNglob
     x, y = normal(19, n), normal(23, n)
     Mx, My = ctx.amg_ldiv(x), ctx.amg_ldiv(y)
-    # kni is negative definite, so is its preconditioner; symmetric to rounding
-    assert abs(x @ My - y @ Mx) <= 1e-10 * abs(x @ My)
-    assert x @ Mx < 0
+    asym = abs(x @ My - y @ Mx) / abs(x @ My)
+    print(f"[gpu] {smoother} V-cycle asymmetry {asym:.2e}", flush=True)
+    assert asym <= (1e-10 if smoother == "chebyshev" else 1e-2)
+    for seed in (19, 23, 29):
+        z = normal(seed, n)
+        assert z @ ctx.amg_ldiv(z) < 0
 
 
 def test_pcg_jacobi(mesh, oracle):
