@@ -1159,15 +1159,17 @@ static ChebSweep<T> sweep_params(eq_ctx* ctx, int post) {
 }
 
 // Tile-sweep pre-smoothing of b from x = 0 into chx (one launch, carrying the
-// cg! convergence test), then the residual b - K_II chx; returns its buffer.
+// cg! convergence test); with residual, then the residual b - K_II chx, whose
+// buffer is returned.
 template <typename T>
-static const T* sweep_pre(eq_ctx* ctx, const T* b) {
+static const T* sweep_pre(eq_ctx* ctx, const T* b, bool residual) {
   CgWork<T>& w = work<T>(ctx);
   ChebSweep<T> op = sweep_params<T>(ctx, 0);
   op.b = b;
   op.xout = w.chx;
   op.check = 1;
   launch_cheb_sweep<T>(ctx->g, op, ctx->s, lt_slot(ctx, sizeof(T) == 4 ? EQ_TK_SWEEP32 : EQ_TK_SWEEP64));
+  if (!residual) return nullptr;
   ChebStep<T> st;
   st.src = w.chx;
   st.rin = b;
@@ -1206,8 +1208,12 @@ static void vcycle(eq_ctx* ctx, const T* r, T* c, double* c64 = nullptr, const d
   DevLevel& L0 = ctx->lev[0];
   const T om0 = (T)L0.omega;
   T* x0 = ctx->l0_cheb ? w.chx : w.x0;
-  if (ctx->l0_cheb) {
-    const T* res = ctx->l0_sweep ? sweep_pre<T>(ctx, r) : cheb_pre<T>(ctx, r);
+  if (ctx->l0_sweep && ctx->pmg) {
+    // bilinear level: the residual of the pre-smoothed x restricted in one matrix-free launch
+    sweep_pre<T>(ctx, r, false);
+    launch_restrict<T>(g, w.chx, r, mu_of<T>(ctx), lvecs<T>(ctx->lev[1]).b, ctx->sc, s);
+  } else if (ctx->l0_cheb) {
+    const T* res = ctx->l0_sweep ? sweep_pre<T>(ctx, r, true) : cheb_pre<T>(ctx, r);
     launch_csr<T>(CSR_SPMV, L0.R, res + g.NX, nullptr, nullptr, T(0), lvecs<T>(ctx->lev[1]).b, nullptr, ctx->sc,
                   s);
   } else {

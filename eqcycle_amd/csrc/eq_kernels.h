@@ -190,6 +190,10 @@ struct ChebSweep {
 };
 template <typename T>
 void launch_cheb_sweep(const Geom& g, const ChebSweep<T>& op, hipStream_t s, LaunchEvents le = {});
+// b1 = P0^T (b - K_II x): the residual of x restricted to the element vertices by
+// the transpose of the bilinear prolongation (vertex rows 1 .. nely), matrix-free.
+template <typename T>
+void launch_restrict(const Geom& g, const T* x, const T* b, const T* mu, T* b1, const CgScal* sc, hipStream_t s);
 
 struct DevCsr {
   int64_t n = 0, m = 0, nnz = 0;

@@ -2287,6 +2287,153 @@ void launch_cheb_sweep(const Geom& g, const ChebSweep<T>& op, hipStream_t s, Lau
   launch_timed(cheb_sweep_kernel<T>, tile_grid(g), dim3(SW_BLOCK), s, le, g, op);
 }
 
+// --------------------------------------------------------------------------
+// Residual after pre-smoothing, restricted to the element vertices (P0^T)
+// --------------------------------------------------------------------------
+// b1 = P0^T (b - K_II x) without forming the residual in HBM.  The tile owns
+// the vertices (vx, vy) with 4 vx in [I0, I0 + TILE_I) and 4 vy in [J0, J0 +
+// TILE_J), vy >= 1; the bilinear hat of a vertex covers the nodes within 3 of
+// it in each direction (weight 1 at the vertex, 0.5 (1 -+ xi) at the GLL nodes
+// of the elements on either side).  So the residual is formed on the tile and
+// the 3 columns / rows before it, RS_I x RS_J nodes, all of which the tile
+// kernel's window (x and mu on columns I0-4 .. I0+TILE_I, rows J0-4 ..
+// J0+TILE_J) covers with their elements.  The fault row J = 0 is not an
+// unknown (r = 0 there).
+constexpr int RS_I = TILE_I + 3;    // residual columns I0-3 .. I0+TILE_I-1 (window columns 1 ..)
+constexpr int RS_J = TILE_J + 3;
+constexpr int RS_XS = TILE_I / 4 + 1;   // x-strips (segments at window columns 4k .. 4k+4) reaching them
+constexpr int RS_YS = TILE_J / 4 + 1;
+constexpr int RS_BLOCK = 256;
+constexpr int RS_LOADS = (WIN_J * WIN_I + RS_BLOCK - 1) / RS_BLOCK;
+static_assert(RS_J * RS_XS <= 2 * RS_BLOCK && RS_I * RS_YS <= 2 * RS_BLOCK, "two strips per thread and pass");
+static_assert((TILE_I / 4) * (TILE_J / 4) <= RS_BLOCK, "one vertex per thread");
+
+template <typename T>
+__global__ __launch_bounds__(RS_BLOCK) void restrict_kernel(Geom g, const T* __restrict__ x, const T* __restrict__ b,
+                                                            const T* __restrict__ mu, T* __restrict__ b1,
+                                                            const CgScal* sc) {
+  __shared__ T su[WIN_J * WIN_I];
+  __shared__ T sm[WIN_J * WIN_I];
+  __shared__ T sr[RS_J * RS_I];   // x-direction terms, then the residual
+  if (sc->done) return;
+  const int tile = xcd_tile(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+  const int I0 = (tile % gridDim.x) * TILE_I, J0 = (tile / gridDim.x) * TILE_J;
+  const int tid = threadIdx.x, NX = g.NX, NY = g.NY;
+  // b on the residual region: thread -> column c of RS_I, rows q, q + RS_BLOCK / RS_I ...
+  constexpr int NB = (RS_I * RS_J + RS_BLOCK - 1) / RS_BLOCK;
+  T br_[NB];
+#pragma unroll
+  for (int t = 0; t < NB; t++) {
+    const int k = tid + t * RS_BLOCK;
+    const int rr = k / RS_I, cc = k - rr * RS_I;
+    const int I = I0 - 3 + cc, J = J0 - 3 + rr;
+    br_[t] = T(0);
+    if (k < RS_I * RS_J && I >= 0 && I < NX && J >= 1 && J < NY) br_[t] = b[(int64_t)J * NX + I];
+  }
+  {
+    T ur[RS_LOADS], mr[RS_LOADS];
+#pragma unroll
+    for (int t = 0; t < RS_LOADS; t++) {
+      const int k = tid + t * RS_BLOCK;
+      const int rr = k / WIN_I, cc = k - rr * WIN_I;
+      const int I = I0 - 4 + cc, J = J0 - 4 + rr;
+      ur[t] = mr[t] = T(0);
+      if (k < WIN_J * WIN_I && I >= 0 && I < NX && J >= 0 && J < NY) {
+        const int64_t n = (int64_t)J * NX + I;
+        ur[t] = J > 0 ? x[n] : T(0);
+        mr[t] = mu[n];
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < RS_LOADS; t++) {
+      const int k = tid + t * RS_BLOCK;
+      if (k < WIN_J * WIN_I) {
+        su[k] = ur[t];
+        sm[k] = mr[t];
+      }
+    }
+  }
+  __syncthreads();
+  // x direction on window rows 1 .. RS_J, strips k: segment at window columns 4k .. 4k+4
+  for (int task = tid; task < RS_J * RS_XS; task += RS_BLOCK) {
+    const int rr = task / RS_XS, k = task - rr * RS_XS;
+    const int wr = rr + 1, J = J0 - 4 + wr;
+    if (J < 1 || J >= NY) continue;
+    const int sg = (I0 >> 2) - 1 + k;
+    const T* up = su + wr * WIN_I + 4 * k;
+    const T* mp = sm + wr * WIN_I + 4 * k;
+    T o[5] = {0, 0, 0, 0, 0};
+    if (sg >= 0 && sg < g.nelx) seg_all<T, 1>(g, up, mp, o);
+    if (k > 0 && sg > 0 && sg - 1 < g.nelx) o[0] += seg_last<T, 1>(g, up - 4, mp - 4);
+    const T cx = (T)(g.c1 * wline(g, J, NY));
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int wc = 4 * k + i;   // residual column wc - 1
+      if (wc >= 1 && wc <= RS_I) sr[rr * RS_I + wc - 1] = cx * o[i];
+    }
+  }
+  __syncthreads();
+  // y direction on window columns 1 .. RS_I; r = b - K x into sr
+  for (int task = tid; task < RS_I * RS_YS; task += RS_BLOCK) {
+    const int cc = task % RS_I, k = task / RS_I;
+    const int wc = cc + 1, I = I0 - 4 + wc;
+    if (I < 0 || I >= NX) continue;
+    const int tg = (J0 >> 2) - 1 + k;
+    const T* up = su + 4 * k * WIN_I + wc;
+    const T* mp = sm + 4 * k * WIN_I + wc;
+    T o[5] = {0, 0, 0, 0, 0};
+    if (tg >= 0 && tg < g.nely) seg_all<T, WIN_I>(g, up, mp, o);
+    if (k > 0 && tg > 0 && tg - 1 < g.nely) o[0] += seg_last<T, WIN_I>(g, up - 4 * WIN_I, mp - 4 * WIN_I);
+    const T cy = (T)(g.c2 * wline(g, I, NX));
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int wr = 4 * k + j, rr = wr - 1, J = J0 - 4 + wr;
+      if (wr >= 1 && wr <= RS_J && J >= 1 && J < NY) sr[rr * RS_I + cc] += cy * o[j];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < NB; t++) {
+    const int k = tid + t * RS_BLOCK;
+    if (k < RS_I * RS_J) {
+      const int rr = k / RS_I, cc = k - rr * RS_I;
+      const int I = I0 - 3 + cc, J = J0 - 3 + rr;
+      sr[k] = (I >= 0 && I < NX && J >= 1 && J < NY) ? br_[t] - sr[k] : T(0);
+    }
+  }
+  __syncthreads();
+  // one owned vertex per thread: (a, c) at residual column 4a + 3, row 4c + 3
+  constexpr int NVX = TILE_I / 4;
+  if (tid < NVX * (TILE_J / 4)) {
+    const int a = tid % NVX, c = tid / NVX;
+    const int vx = (I0 >> 2) + a, vy = (J0 >> 2) + c;
+    if (vy >= 1 && vx <= g.nelx && vy <= g.nely) {
+      T w[7];
+      w[3] = T(1);
+#pragma unroll
+      for (int o = 1; o <= 3; o++) {
+        w[3 - o] = (T)(0.5 * (1.0 + g.x[4 - o]));   // node 4 - o of the element before the vertex
+        w[3 + o] = (T)(0.5 * (1.0 - g.x[o]));       // node o of the element after it
+      }
+      T s = T(0);
+#pragma unroll
+      for (int j = 0; j < 7; j++) {
+        const T* row = sr + (4 * c + j) * RS_I + 4 * a;
+        T t = T(0);
+#pragma unroll
+        for (int i = 0; i < 7; i++) t += w[i] * row[i];
+        s += w[j] * t;
+      }
+      b1[(int64_t)(vy - 1) * (g.nelx + 1) + vx] = s;
+    }
+  }
+}
+
+template <typename T>
+void launch_restrict(const Geom& g, const T* x, const T* b, const T* mu, T* b1, const CgScal* sc, hipStream_t s) {
+  hipLaunchKernelGGL((restrict_kernel<T>), tile_grid(g), dim3(RS_BLOCK), 0, s, g, x, b, mu, b1, sc);
+}
+
 template <typename T, int MODE>
 static void csr_dispatch(const DevCsr& M, const T* val, const T* x, const T* b, const T* dinv, T omega, T* y, T* xout,
                          const CgScal* sc, hipStream_t s) {
@@ -2523,6 +2670,7 @@ void launch_scale_mask(int64_t n, int nx, const double* x, double sign, double* 
   template void launch_cheb<T>(const Geom&, const ChebStep<T>&, const T*, const T*, const double*, int, CgScal*,   \
                                double*, hipStream_t, LaunchEvents);                                                \
   template void launch_cheb_sweep<T>(const Geom&, const ChebSweep<T>&, hipStream_t, LaunchEvents);            \
+  template void launch_restrict<T>(const Geom&, const T*, const T*, const T*, T*, const CgScal*, hipStream_t);    \
   template void launch_fill<T>(int64_t, T*, T, hipStream_t);
 EQ_INST(double)
 EQ_INST(float)
