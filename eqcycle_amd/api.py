@@ -507,7 +507,8 @@ class EqContext:
         """Average device ms per launch, HIP events on the library's stream."""
         w = {"ku": L.EQ_TIME_KU, "cg_dir": L.EQ_TIME_CG_DIR, "vcycle": L.EQ_TIME_VCYCLE,
              "cg_iter": L.EQ_TIME_CG_ITER, "newmark": L.EQ_TIME_NEWMARK, "cheb": L.EQ_TIME_CHEB,
-             "copy": L.EQ_TIME_COPY, "cheb2": L.EQ_TIME_CHEB2}[which]
+             "copy": L.EQ_TIME_COPY, "cheb2": L.EQ_TIME_CHEB2, "sweep": L.EQ_TIME_SWEEP,
+             "sweep_post": L.EQ_TIME_SWEEP_POST, "restrict": L.EQ_TIME_RESTRICT}[which]
         out = C.c_double()
         self._check(self.lib.eq_time_kernel(self.h, w, PREC[prec], reps, C.byref(out)))
         return out.value

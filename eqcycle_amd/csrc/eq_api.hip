@@ -2516,6 +2516,28 @@ static void time_body(eq_ctx* ctx, int which) {
                      ctx->s);
       break;
     }
+    case EQ_TIME_SWEEP: {
+      // one whole-tile sweep from x = 0 (pre-smoothing) at the hierarchy's degree
+      ChebSweep<T> op = sweep_params<T>(ctx, 0);
+      op.b = w.r;
+      op.xout = w.chx;
+      launch_cheb_sweep<T>(g, op, ctx->s);
+      break;
+    }
+    case EQ_TIME_SWEEP_POST: {
+      // one whole-tile sweep from chx + P0 coarse (post-smoothing), output c and <c, r>
+      ChebSweep<T> op = sweep_params<T>(ctx, 1);
+      op.b = w.r;
+      op.src = w.chx;
+      op.coarse = lvecs<T>(ctx->lev[1]).xp;
+      op.xout = w.c;
+      op.rdot = w.r;
+      launch_cheb_sweep<T>(g, op, ctx->s);
+      break;
+    }
+    case EQ_TIME_RESTRICT:
+      launch_restrict<T>(g, w.chx, w.r, mu_of<T>(ctx), lvecs<T>(ctx->lev[1]).b, ctx->sc, ctx->s);
+      break;
     case EQ_TIME_CHEB2: {
       // two middle Chebyshev steps in one launch: window d, reads r, x, mu, dinv; writes r'', d'', x
       ChebStep<T> st[2];
@@ -2545,6 +2567,9 @@ extern "C" int eq_time_kernel(eq_ctx* ctx, int which, int prec, int reps, double
   ctx->qs_r_valid = false;
   if (!ctx->have_material) return fail(ctx, EQ_ENOTREADY, "eq_set_material first");
   if (which == EQ_TIME_VCYCLE && !ctx->have_amg) return fail(ctx, EQ_ENOTREADY, "eq_amg_setup first");
+  if ((which == EQ_TIME_SWEEP || which == EQ_TIME_SWEEP_POST || which == EQ_TIME_RESTRICT) &&
+      (!ctx->have_amg || !ctx->pmg || !ctx->l0_cheb))
+    return fail(ctx, EQ_ENOTREADY, "a p-multigrid hierarchy with Chebyshev smoothing first");
   if (which == EQ_TIME_NEWMARK && (!ctx->have_state || !ctx->have_params))
     return fail(ctx, EQ_ENOTREADY, "eq_state_init first");
   hipStream_t s = ctx->s;

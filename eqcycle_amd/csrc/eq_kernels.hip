@@ -2103,19 +2103,57 @@ constexpr int SW_WJ = TILE_J + 2 * SW_RING;      // window rows
 constexpr int SW_I = TILE_I + 9;                 // box columns
 constexpr int SW_J = TILE_J + 9;                 // box rows
 constexpr int SW_BOX = SW_I * SW_J;
-constexpr int SW_BLOCK = 512;
+constexpr int SW_BLOCK = 448;
 constexpr int SW_XS = TILE_I / 4 + 2;            // x-strips (element segments) per window row
 constexpr int SW_YS = TILE_J / 4 + 2;            // y-strips per window column
 constexpr int SW_BOX_LOADS = (SW_BOX + SW_BLOCK - 1) / SW_BLOCK;
+constexpr int SW_SXS = SW_WI + 1;                // row stride of the x-term array (odd: no LDS bank conflicts)
+constexpr int SW_VI = TILE_I / 4 + 4;            // element vertices I0/4 - 1 .. I0/4 + TILE_I/4 + 2: those
+constexpr int SW_VJ = TILE_J / 4 + 4;            // of the box's elements
+static_assert(SW_VI * SW_VJ <= SW_BLOCK, "one vertex value per thread");
 static_assert(SW_WJ * SW_XS <= SW_BLOCK && SW_WI * SW_YS <= SW_BLOCK, "one strip per thread and pass");
 static_assert(TILE_I % 4 == 0 && TILE_J % 4 == 0, "element-aligned tiles");
 
+// Segment outputs o[0..4] of one 5-node segment (strided view, stride ST)
+// with the quadrature weight folded into wm (wm_p = w_p mu_p):
+// o_i = sum_p D[p][i] wm_p sum_k D[p][k] u_k.
+template <typename T, int ST>
+__device__ __forceinline__ void seg_all_wm(const Geom& g, const T* u, const T* wm, T o[5]) {
+  T t[5];
+#pragma unroll
+  for (int p = 0; p < 5; p++) {
+    T s = 0;
+#pragma unroll
+    for (int k = 0; k < 5; k++) s += (T)g.D[p * 5 + k] * u[k * ST];
+    t[p] = s * wm[p * ST];
+  }
+#pragma unroll
+  for (int i = 0; i < 5; i++) {
+    T r = 0;
+#pragma unroll
+    for (int p = 0; p < 5; p++) r += (T)g.D[p * 5 + i] * t[p];
+    o[i] = r;
+  }
+}
+
+// One product with K per step in two phases.  Phase A: every x-strip thread
+// (a window row, an element segment along x) and every y-strip thread (a
+// window column, a segment along y) evaluates its segment once: outputs 0..3
+// go to the segment's first four nodes (x: the LDS term array; y: registers),
+// output 4, the segment's end node, to a carry slot that the next strip's
+// node 0 adds in phase B.  Phase B: the owner of each node of W forms K d
+// there and updates r, d and x.
 template <typename T>
 __global__ __launch_bounds__(SW_BLOCK) void cheb_sweep_kernel(Geom g, ChebSweep<T> op) {
-  __shared__ T sd[2][SW_BOX];       // directions (post: sd[0] first holds v)
-  __shared__ T sm[SW_BOX];          // mu
-  __shared__ T sx[SW_WJ * SW_WI];   // x-direction terms on W
-  if (op.sc->done) return;          // block-uniform, as OpCheb::prepare
+  __shared__ T sd[2][SW_BOX];        // directions (post: sd[0] first holds v)
+  __shared__ T swx[SW_BOX];          // w_(I mod 4) mu: quadrature weight along x times mu
+  __shared__ T swy[SW_BOX];          // w_(J mod 4) mu
+  __shared__ T sx[SW_WJ * SW_SXS];   // x terms (times c1 wline(J)) on W
+  __shared__ T cxr[SW_XS * SW_WJ];   // x carry of strip k in row r (k * SW_WJ + r): its segment's output 4,
+                                     // times c1 wline(J)
+  __shared__ T cyr[SW_YS * SW_WI];   // y carry of strip k: its segment's output 4
+  __shared__ T scv[SW_VJ * SW_VI];   // post: the coarse (element-vertex) correction around the box
+  if (op.sc->done) return;           // block-uniform, as OpCheb::prepare
   if (op.check) {
     const double rr = reduce_parts(op.part_rr, op.nrr);
     const double resid = sqrt(rr);
@@ -2130,10 +2168,24 @@ __global__ __launch_bounds__(SW_BLOCK) void cheb_sweep_kernel(Geom g, ChebSweep<
   const int I0 = (tile % gridDim.x) * TILE_I, J0 = (tile / gridDim.x) * TILE_J;
   const int tid = threadIdx.x;
   const int NX = g.NX, NY = g.NY;
-  // y-pass strip of this thread: window column yc, box rows 4 yk .. 4 yk + 3
+  // y-strip of this thread (and owner of its nodes): window column yc, box rows 4 yk .. 4 yk + 3
   const int yc = tid % SW_WI, yk = tid / SW_WI;
   const int Iy = I0 - SW_RING + yc, bcy = yc + SW_OFF;
   const bool ycol = tid < SW_WI * SW_YS && Iy >= 0 && Iy < NX;
+  const int ytg = (J0 >> 2) - 1 + yk;   // element segment along y: box rows 4 yk .. 4 yk + 4
+  const bool yseg = ytg >= 0 && ytg < g.nely;
+  const T cy = ycol ? (T)(g.c2 * wline(g, Iy, NX)) : T(0);
+  const int xs_own = bcy >> 2;          // x-strip holding this thread's column
+  const bool xi0 = (bcy & 3) == 0;      // the column is a segment's node 0 (adds the previous strip's carry)
+  // x-strip of this thread: window row xwr, box columns 4 xk .. 4 xk + 4 (rows vary fastest:
+  // consecutive threads read LDS rows apart, an odd stride)
+  const bool xtask = tid < SW_WJ * SW_XS;
+  const int xk = tid / SW_WJ, xwr = tid - xk * SW_WJ;
+  const int xbr = xwr + SW_OFF, xJ = J0 - 4 + xbr;
+  const bool xrow = xtask && xJ >= 1 && xJ < NY;
+  const int xsg = (I0 >> 2) - 1 + xk;
+  const bool xseg = xsg >= 0 && xsg < g.nelx;
+  const T cx = xrow ? (T)(g.c1 * wline(g, xJ, NY)) : T(0);
   T r[4], x[4], d[4], di[4];
   double rd[4];
   bool act[4], own[4];
@@ -2152,6 +2204,17 @@ __global__ __launch_bounds__(SW_BLOCK) void cheb_sweep_kernel(Geom g, ChebSweep<
     }
   }
   {
+    // post with a coarse correction: the values at the vertices of the box's elements (vertex
+    // columns I0/4 - 1 .. I0/4 + TILE_I/4 + 2, rows likewise; 0 outside the mesh and on the
+    // fault row) go to LDS first
+    const bool pcoarse = op.post && op.coarse;
+    if (pcoarse && tid < SW_VI * SW_VJ) {
+      const int a = tid % SW_VI, c = tid / SW_VI;
+      const int vx = (I0 >> 2) - 1 + a, vy = (J0 >> 2) - 1 + c;
+      T v = T(0);
+      if (vx >= 0 && vx <= op.nelx && vy >= 1 && vy <= op.nely) v = op.coarse[(int64_t)(vy - 1) * (op.nelx + 1) + vx];
+      scv[tid] = v;
+    }
     T ub[SW_BOX_LOADS], mb[SW_BOX_LOADS];
 #pragma unroll
     for (int t = 0; t < SW_BOX_LOADS; t++) {
@@ -2162,56 +2225,69 @@ __global__ __launch_bounds__(SW_BLOCK) void cheb_sweep_kernel(Geom g, ChebSweep<
       if (k < SW_BOX && I >= 0 && I < NX && J >= 0 && J < NY) {
         const int64_t n = (int64_t)J * NX + I;
         mb[t] = op.mu[n];
-        if (op.post)
-          ub[t] = cheb_source(op.src, op.b, op.dinv, T(0), op.coarse ? 2 : 0, op.coarse, op.nelx, op.nely, op.gx,
-                              n, I, J);
+        if (op.post && J > 0) ub[t] = op.src[n];
       }
     }
+    if (pcoarse) __syncthreads();
 #pragma unroll
     for (int t = 0; t < SW_BOX_LOADS; t++) {
       const int k = tid + t * SW_BLOCK;
       if (k < SW_BOX) {
-        sm[k] = mb[t];
-        sd[0][k] = ub[t];
+        const int br = k / SW_I, bc = k - br * SW_I;   // I0, J0 are multiples of 4: local index = box index mod 4
+        const int I = I0 - 4 + bc, J = J0 - 4 + br;
+        T v = ub[t];
+        if (pcoarse && I >= 0 && I < NX && J > 0 && J < NY) {
+          // bilinear interpolation of the vertex values, as cheb_source forms it
+          const int ex = min(I >> 2, op.nelx - 1), i = I - 4 * ex;
+          const int ey = min(J >> 2, op.nely - 1), j = J - 4 * ey;
+          const T ax1 = T(0.5) * (T(1) + op.gx[i]), ax0 = T(1) - ax1;
+          const T ay1 = T(0.5) * (T(1) + op.gx[j]), ay0 = T(1) - ay1;
+          const T* c0 = scv + (ey - (J0 >> 2) + 1) * SW_VI + ex - (I0 >> 2) + 1;   // vertex (ex, ey)
+          T p = T(0);
+          if (ey > 0) p += ay0 * (ax0 * c0[0] + ax1 * c0[1]);
+          p += ay1 * (ax0 * c0[SW_VI] + ax1 * c0[SW_VI + 1]);
+          v = v + p;
+        }
+        swx[k] = (T)g.w[bc & 3] * mb[t];
+        swy[k] = (T)g.w[br & 3] * mb[t];
+        sd[0][k] = v;
         sd[1][k] = T(0);
       }
     }
   }
   __syncthreads();
-  // x-direction terms of K applied to su on W
-  auto xpass = [&](const T* su) {
-    if (tid < SW_WJ * SW_XS) {
-      const int wr = tid / SW_XS, k = tid - wr * SW_XS;
-      const int br = wr + SW_OFF, J = J0 - 4 + br;
-      if (J >= 1 && J < NY) {
-        const int sg = (I0 >> 2) - 1 + k;   // element segment along x: box columns 4k .. 4k+4
-        const T* up = su + br * SW_I + 4 * k;
-        const T* mp = sm + br * SW_I + 4 * k;
-        T o[5] = {0, 0, 0, 0, 0};
-        if (sg >= 0 && sg < g.nelx) seg_all<T, 1>(g, up, mp, o);
-        if (k > 0 && sg > 0 && sg - 1 < g.nelx) o[0] += seg_last<T, 1>(g, up - 4, mp - 4);
-        const T cx = (T)(g.c1 * wline(g, J, NY));
+  // phase A on direction su: x terms and carries to LDS, this thread's y outputs to oy
+  T oy[5];
+  auto phase_a = [&](const T* su) {
+    if (xrow) {
+      T o[5] = {0, 0, 0, 0, 0};
+      if (xseg) seg_all_wm<T, 1>(g, su + xbr * SW_I + 4 * xk, swx + xbr * SW_I + 4 * xk, o);
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-          const int bc = 4 * k + i;
-          if (bc >= SW_OFF && bc < SW_OFF + SW_WI) sx[wr * SW_WI + bc - SW_OFF] = cx * o[i];
-        }
+      for (int i = 0; i < 4; i++) {
+        const int bc = 4 * xk + i;
+        if (bc >= SW_OFF && bc < SW_OFF + SW_WI) sx[xwr * SW_SXS + bc - SW_OFF] = cx * o[i];
       }
+      cxr[xk * SW_WJ + xwr] = cx * o[4];
+    }
+#pragma unroll
+    for (int j = 0; j < 5; j++) oy[j] = T(0);
+    if (ycol) {
+      if (yseg) seg_all_wm<T, SW_I>(g, su + 4 * yk * SW_I + bcy, swy + 4 * yk * SW_I + bcy, oy);
+      cyr[yk * SW_WI + yc] = oy[4];
     }
   };
-  // (K su) at this thread's active nodes
-  auto ypass = [&](const T* su, T ku[4]) {
-    if (!ycol) return;
-    const int tg = (J0 >> 2) - 1 + yk;     // element segment along y: box rows 4 yk .. 4 yk + 4
-    const T* up = su + 4 * yk * SW_I + bcy;
-    const T* mp = sm + 4 * yk * SW_I + bcy;
-    T o[5] = {0, 0, 0, 0, 0};
-    if (tg >= 0 && tg < g.nely) seg_all<T, SW_I>(g, up, mp, o);
-    if (yk > 0 && tg > 0 && tg - 1 < g.nely) o[0] += seg_last<T, SW_I>(g, up - 4 * SW_I, mp - 4 * SW_I);
-    const T cy = (T)(g.c2 * wline(g, Iy, NX));
+  // phase B: (K su) at this thread's active nodes
+  auto phase_b = [&](T ku[4]) {
 #pragma unroll
     for (int j = 0; j < 4; j++)
-      if (act[j]) ku[j] = sx[(4 * yk + j - SW_OFF) * SW_WI + yc] + cy * o[j];
+      if (act[j]) {
+        const int wr = 4 * yk + j - SW_OFF;
+        T X = sx[wr * SW_SXS + yc];
+        if (xi0 && xs_own > 0) X += cxr[(xs_own - 1) * SW_WJ + wr];
+        T Y = oy[j];
+        if (j == 0 && yk > 0) Y += cyr[(yk - 1) * SW_WI + yc];
+        ku[j] = X + cy * Y;
+      }
   };
   int cur = 0;
   if (!op.post) {
@@ -2224,10 +2300,10 @@ __global__ __launch_bounds__(SW_BLOCK) void cheb_sweep_kernel(Geom g, ChebSweep<
       }
     __syncthreads();
   } else {
-    xpass(sd[0]);
+    phase_a(sd[0]);
     __syncthreads();
     T ku[4];
-    ypass(sd[0], ku);
+    phase_b(ku);
 #pragma unroll
     for (int j = 0; j < 4; j++)
       if (act[j]) {
@@ -2244,12 +2320,12 @@ __global__ __launch_bounds__(SW_BLOCK) void cheb_sweep_kernel(Geom g, ChebSweep<
   for (int m = 1; m < kSweepMaxDeg; m++) {   // unrolled: c1[m], c2[m] at constant offsets
     if (m >= op.degree) break;
     T* nxt = sd[1 - cur];
-    xpass(sd[cur]);
+    phase_a(sd[cur]);
     if (op.post && m == 1)   // sd[0] held v on the whole box: clear it before it takes directions
       for (int k = tid; k < SW_BOX; k += SW_BLOCK) nxt[k] = T(0);
     __syncthreads();
     T ku[4];
-    ypass(sd[cur], ku);
+    phase_b(ku);
     const T c1 = op.c1[m], c2 = op.c2[m];
 #pragma unroll
     for (int j = 0; j < 4; j++)

@@ -38,7 +38,7 @@ EXPORTS = [
     "eq_learn_predict_linear", "eq_learn_snapshots", "eq_rs_build", "eq_rs_info", "eq_rs_level", "eq_rs_split", "eq_rs_free",
 ]
 EQ_TIME_KU, EQ_TIME_CG_DIR, EQ_TIME_VCYCLE, EQ_TIME_CG_ITER, EQ_TIME_NEWMARK, EQ_TIME_CHEB, EQ_TIME_COPY, \
-    EQ_TIME_CHEB2 = range(8)
+    EQ_TIME_CHEB2, EQ_TIME_SWEEP, EQ_TIME_SWEEP_POST, EQ_TIME_RESTRICT = range(11)
 TK_NAMES = ["newmark_kernel", "cheb_pair_kernel<float>", "cheb_pair_kernel<double>",
             "tile_kernel<float, OpCheb>", "tile_kernel<double, OpCheb>", "tile_kernel<T, OpCgDir>",
             "cg_update_kernel<T>", "cheb_sweep_kernel<float>", "cheb_sweep_kernel<double>"]

@@ -3,7 +3,9 @@
     python -m eqcycle_amd.profile_kernels [--config cfg2] [--reps 20] [--kernels cheb,ku,...]
 
 Kernels: ku (K u), cheb (one fine-level Chebyshev step), cheb2 (two steps in
-one launch), cg_dir, vcycle, cg_iter, newmark.  Each kernel runs on the
+one launch), sweep / sweep_post (a whole tile sweep before / after the coarse
+correction), restrict (residual restricted to the element vertices), cg_dir,
+vcycle, cg_iter, newmark.  Each kernel runs on the
 library's stream through eq_time_kernel (one warm-up launch, then --reps timed
 launches on well-defined inputs), and the average
 launch time measured with HIP events is printed.  A device-to-device copy of
@@ -21,7 +23,7 @@ import sys
 
 sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 
-KERNELS = ("ku", "cheb", "cheb2", "cg_dir", "vcycle", "cg_iter", "newmark")
+KERNELS = ("ku", "cheb", "cheb2", "sweep", "sweep_post", "restrict", "cg_dir", "vcycle", "cg_iter", "newmark")
 
 
 def main():
@@ -31,6 +33,8 @@ def main():
     ap.add_argument("--kernels", default="cheb,ku,newmark")
     ap.add_argument("--prec", default="fp64")
     ap.add_argument("--copy", type=int, default=1, help="1: also time the 512 MiB calibration copy")
+    ap.add_argument("--smoother", default="tiles", help="fine-level smoother: tiles | chebyshev")
+    ap.add_argument("--degree", type=int, default=5, help="fine-level Chebyshev degree")
     ap.add_argument("--cheb-pairs", type=int, default=1,
                     help="V-cycle and CG iteration with two Chebyshev steps per launch (0: one per launch)")
     a = ap.parse_args()
@@ -40,10 +44,11 @@ def main():
 
     cfg = CONFIGS[a.config]
     P = set_parameters(cfg)
-    ctx = EqContext(P, amg={"hierarchy": "pmg", "degree": 5})
+    ctx = EqContext(P, amg={"hierarchy": "pmg", "degree": a.degree, "smoother": a.smoother})
     ctx.set_solver(tol=cfg.pcg_tol, pc="amg", prec="fp64", cheb_pairs=bool(a.cheb_pairs))
     ctx.state_init()
-    out = {"config": a.config, "nglob": P.nglob, "prec": a.prec, "reps": a.reps, "avg_us": {}}
+    out = {"config": a.config, "nglob": P.nglob, "prec": a.prec, "reps": a.reps, "smoother": a.smoother,
+           "degree": a.degree, "avg_us": {}}
     for k in a.kernels.split(","):
         if k not in KERNELS:
             raise SystemExit(f"unknown kernel {k!r}; choose from {KERNELS}")

@@ -353,14 +353,19 @@ int eq_learn_galerkin(eq_ctx *ctx, const double *x, const double *b, double *x_o
 
 /* ---- measurement hook (no reference counterpart) ------------------------ */
 enum { EQ_TIME_KU = 0, EQ_TIME_CG_DIR = 1, EQ_TIME_VCYCLE = 2, EQ_TIME_CG_ITER = 3, EQ_TIME_NEWMARK = 4,
-       EQ_TIME_CHEB = 5, EQ_TIME_COPY = 6, EQ_TIME_CHEB2 = 7 };
+       EQ_TIME_CHEB = 5, EQ_TIME_COPY = 6, EQ_TIME_CHEB2 = 7, EQ_TIME_SWEEP = 8, EQ_TIME_SWEEP_POST = 9,
+       EQ_TIME_RESTRICT = 10 };
 /* Average device time (ms) of one launch of the given kernel group over reps
  * back-to-back launches, bracketed by HIP events on the context's stream.
  * Runs on scratch vectors; EQ_TIME_NEWMARK advances a copy of the state and
  * must not be called in the middle of a run.  EQ_TIME_COPY copies a 512 MiB
  * FP64 buffer into another (allocated for the call, then freed): a kernel of
  * known traffic that calibrates hardware byte counters.  EQ_TIME_CHEB is one
- * level-0 Chebyshev step, EQ_TIME_CHEB2 two steps in one launch. */
+ * level-0 Chebyshev step, EQ_TIME_CHEB2 two steps in one launch, EQ_TIME_SWEEP
+ * and EQ_TIME_SWEEP_POST one whole tile sweep before / after the coarse
+ * correction at the hierarchy's degree, EQ_TIME_RESTRICT the residual
+ * restricted to the element vertices (the last three need a p-multigrid
+ * hierarchy with Chebyshev smoothing). */
 int eq_time_kernel(eq_ctx *ctx, int which, int prec, int reps, double *avg_ms);
 
 /* Kinds of launch that eq_launch_timing brackets: the fine-level kernels of
