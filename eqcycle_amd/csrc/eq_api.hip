@@ -114,7 +114,11 @@ struct eq_ctx {
   // solver
   eq_solver_opts opts{1e-6, 0, EQ_PC_AMG, EQ_PREC_FP64, EQ_GUESS_PREVIOUS, 1, 0, 0, 0, 0};
   // captured CG iteration batches: (precision, f32pc, pc, iterations, parity, x) -> executable graph
-  std::map<std::tuple<int, int, int, int64_t, int, const void*>, hipGraphExec_t> cg_graphs;
+  std::map<std::tuple<int, int, int, int64_t, int, const void*, double, int64_t>, hipGraphExec_t> cg_graphs;
+  // the next V-cycle's first kernel sets the solve's scalars up (see sweep_pre); tol, maxiter
+  bool cg_init_pending = false;
+  double cg_init_tol = 0;
+  int64_t cg_init_maxiter = 0;
   CgWork<double> w64;
   CgWork<float> w32;
   double *part_rr = nullptr, *part_bb = nullptr, *part_rho = nullptr, *part_uq = nullptr;
@@ -1168,6 +1172,13 @@ static const T* sweep_pre(eq_ctx* ctx, const T* b, bool residual) {
   op.b = b;
   op.xout = w.chx;
   op.check = 1;
+  if (ctx->cg_init_pending) {
+    op.init = 1;
+    op.part_bb = ctx->part_bb;
+    op.tol = ctx->cg_init_tol;
+    op.maxiter = (int)ctx->cg_init_maxiter;
+    ctx->cg_init_pending = false;
+  }
   launch_cheb_sweep<T>(ctx->g, op, ctx->s, lt_slot(ctx, sizeof(T) == 4 ? EQ_TK_SWEEP32 : EQ_TK_SWEEP64));
   if (!residual) return nullptr;
   ChebStep<T> st;
@@ -1293,7 +1304,9 @@ static int enqueue_iterations(eq_ctx* ctx, T* x, int pc, bool f32pc, int64_t n) 
     for (int64_t k = 0; k < n; k++) cg_iteration<T>(ctx, x, pc, f32pc);
     return EQ_OK;
   }
-  const auto key = std::make_tuple((int)sizeof(T), f32pc ? 1 : 0, pc, n, w.cur, (const void*)x);
+  const bool init = ctx->cg_init_pending;
+  const auto key = std::make_tuple((int)sizeof(T), f32pc ? 1 : 0, pc, n, w.cur, (const void*)x,
+                                   init ? ctx->cg_init_tol : -1.0, init ? ctx->cg_init_maxiter : (int64_t)-1);
   auto it = ctx->cg_graphs.find(key);
   if (it == ctx->cg_graphs.end()) {
     const int cur0 = w.cur;
@@ -1309,14 +1322,18 @@ static int enqueue_iterations(eq_ctx* ctx, T* x, int pc, bool f32pc, int64_t n) 
     it = ctx->cg_graphs.emplace(key, exec).first;
   }
   HIPCHK(ctx, hipGraphLaunch(it->second, ctx->s));
+  ctx->cg_init_pending = false;   // the replayed first kernel carries it
   if (n & 1) w.cur = 1 - w.cur;
   return EQ_OK;
 }
 
 // CG set-up: r = b - K_II x (unless r_ready), the device scalars, and the
-// FP32 copy of r for an FP32 V-cycle.
+// FP32 copy of r for an FP32 V-cycle.  When the first batch (first_batch
+// iterations) opens with a tile sweep, that kernel sets the scalars up
+// instead of a launch of its own.
 template <typename T>
-static int cg_begin(eq_ctx* ctx, T* x, const T* b, double tol, int64_t maxiter, int pc, bool f32pc, bool r_ready) {
+static int cg_begin(eq_ctx* ctx, T* x, const T* b, double tol, int64_t maxiter, int pc, bool f32pc, bool r_ready,
+                    int64_t first_batch) {
   CgWork<T>& w = work<T>(ctx);
   hipStream_t s = ctx->s;
   if (pc == EQ_PC_AMG && !ctx->have_amg) return fail(ctx, EQ_ENOTREADY, "eq_amg_setup first");
@@ -1326,7 +1343,13 @@ static int cg_begin(eq_ctx* ctx, T* x, const T* b, double tol, int64_t maxiter, 
   if (!r_ready)
     launch_cg_residual<T>(ctx->g, x, mu_of<T>(ctx), b, w.r, ctx->part_rr, ctx->part_bb, ctx->npart, s,
                           f32pc ? ctx->w32.r : nullptr);
-  launch_cg_init_scal(ctx->sc, ctx->part_rr, ctx->part_bb, ctx->npart, tol, (int)maxiter, s);
+  if (first_batch > 0 && pc == EQ_PC_AMG && ctx->l0_sweep) {
+    ctx->cg_init_pending = true;
+    ctx->cg_init_tol = tol;
+    ctx->cg_init_maxiter = maxiter;
+  } else {
+    launch_cg_init_scal(ctx->sc, ctx->part_rr, ctx->part_bb, ctx->npart, tol, (int)maxiter, s);
+  }
   if (f32pc && r_ready) launch_convert<T, float>(ctx->N, w.r, ctx->w32.r, s);
   return EQ_OK;
 }
@@ -1335,7 +1358,9 @@ static int cg_begin(eq_ctx* ctx, T* x, const T* b, double tol, int64_t maxiter, 
 // scalars in the pinned host copy; nothing waits.
 template <typename T>
 static int cg_batch(eq_ctx* ctx, T* x, int pc, bool f32pc, int64_t batch) {
-  if (int rc = enqueue_iterations<T>(ctx, x, pc, f32pc, batch)) return rc;
+  const int rc = enqueue_iterations<T>(ctx, x, pc, f32pc, batch);
+  ctx->cg_init_pending = false;   // consumed by the batch's first kernel (or nothing to set up after an error)
+  if (rc) return rc;
   launch_cg_check(ctx->sc, ctx->part_rr, ctx->npart, ctx->s, ctx->sc_host);
   return EQ_OK;
 }
@@ -1366,8 +1391,8 @@ static int cg_end(eq_ctx* ctx, int64_t* iters, double* relres) {
 template <typename T>
 static int cg_device(eq_ctx* ctx, T* x, const T* b, double tol, int64_t maxiter, int pc, int64_t* iters,
                      double* relres, bool f32pc = false, bool r_ready = false) {
-  if (int rc = cg_begin<T>(ctx, x, b, tol, maxiter, pc, f32pc, r_ready)) return rc;
   int64_t batch = ctx->pred_iters[ctx->cg_slot];
+  if (int rc = cg_begin<T>(ctx, x, b, tol, maxiter, pc, f32pc, r_ready, batch)) return rc;
   if (int rc = cg_batch<T>(ctx, x, pc, f32pc, batch)) return rc;
   if (int rc = spin_sync(ctx)) return rc;
   if (int rc = cg_rest<T>(ctx, x, pc, f32pc, batch)) return rc;
@@ -2048,7 +2073,8 @@ static int qs_step(eq_ctx* ctx, eq_step_info* info, bool r_valid) {
       // pass 2 ahead of its read-back (eq_step finishes it)
       const bool f32pc = prec == EQ_PREC_F32PC && ctx->opts.pc == EQ_PC_AMG;
       ctx->cg_slot = 1;
-      int rc = cg_begin<double>(ctx, ctx->d, ctx->bq, ctx->opts.tol, ctx->opts.maxiter, ctx->opts.pc, f32pc, false);
+      int rc = cg_begin<double>(ctx, ctx->d, ctx->bq, ctx->opts.tol, ctx->opts.maxiter, ctx->opts.pc, f32pc, false,
+                                ctx->pred_iters[1]);
       if (!rc) rc = cg_batch<double>(ctx, ctx->d, ctx->opts.pc, f32pc, ctx->pred_iters[1]);
       ctx->cg_slot = 2;
       if (rc) return rc;

@@ -183,6 +183,12 @@ struct ChebSweep {
   double* xout64 = nullptr;      // post of an FP32 sweep: c widened to FP64
   const double* rdot64 = nullptr;
   int check = 0;                 // carries the cg! convergence test (first kernel of a V-cycle)
+  // check with init: the first kernel of a solve also sets the solve's scalars up from the
+  // partials of ||r||^2 and ||b||^2 (what cg_init_scal_kernel does), tol and maxiter
+  int init = 0;
+  const double* part_bb = nullptr;
+  double tol = 0;
+  int maxiter = 0;
   const double* part_rr = nullptr;
   int nrr = 0;
   CgScal* sc = nullptr;

@@ -2153,16 +2153,37 @@ __global__ __launch_bounds__(SW_BLOCK) void cheb_sweep_kernel(Geom g, ChebSweep<
                                      // times c1 wline(J)
   __shared__ T cyr[SW_YS * SW_WI];   // y carry of strip k: its segment's output 4
   __shared__ T scv[SW_VJ * SW_VI];   // post: the coarse (element-vertex) correction around the box
-  if (op.sc->done) return;           // block-uniform, as OpCheb::prepare
-  if (op.check) {
+  if (op.init) {
+    // a solve's first kernel: every block forms the test from the partials; block 0 writes the
+    // scalars (nothing else of this launch reads them)
     const double rr = reduce_parts(op.part_rr, op.nrr);
-    const double resid = sqrt(rr);
-    const bool fin = (resid <= op.sc->reltol) || (op.sc->iter >= op.sc->maxiter);
+    const double bb = reduce_parts(op.part_bb, op.nrr);
+    const double resid = sqrt(rr), bnorm = sqrt(bb);
+    const bool fin = resid <= op.tol * bnorm || op.maxiter <= 0;
     if (threadIdx.x == 0 && blockIdx.x == 0 && blockIdx.y == 0) {
-      op.sc->resid = resid;
-      if (fin) op.sc->done = 1;
+      CgScal h{};
+      h.rho_prev = 1.0;
+      h.resid = resid;
+      h.bnorm = bnorm;
+      h.reltol = op.tol * bnorm;
+      h.iter = 0;
+      h.maxiter = op.maxiter;
+      h.done = fin ? 1 : 0;
+      *op.sc = h;
     }
     if (fin) return;
+  } else {
+    if (op.sc->done) return;         // block-uniform, as OpCheb::prepare
+    if (op.check) {
+      const double rr = reduce_parts(op.part_rr, op.nrr);
+      const double resid = sqrt(rr);
+      const bool fin = (resid <= op.sc->reltol) || (op.sc->iter >= op.sc->maxiter);
+      if (threadIdx.x == 0 && blockIdx.x == 0 && blockIdx.y == 0) {
+        op.sc->resid = resid;
+        if (fin) op.sc->done = 1;
+      }
+      if (fin) return;
+    }
   }
   const int tile = xcd_tile(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
   const int I0 = (tile % gridDim.x) * TILE_I, J0 = (tile / gridDim.x) * TILE_J;
