@@ -91,6 +91,13 @@ def test_julia_binding_matches_header():
                             ("eq_solver_opts", "SolverOpts"), ("eq_step_info", "StepInfo"),
                             ("eq_results", "Results"), ("eq_fault_params", "FaultParams")):
         assert _jl_struct_fields(jl, jl_name) == _c_struct_fields(hdr, c_name), c_name
+    # every positional construction of a struct mirror passes one value per field
+    for jl_name in ("SolverOpts",):
+        nf = len(_jl_struct_fields(jl, jl_name))
+        calls = re.findall(r"\b" + jl_name + r"\(([^()]*)\)", jl)
+        assert calls, jl_name
+        for args in calls:
+            assert len([a for a in args.split(",") if a.strip()]) == nf, (jl_name, args)
 
 
 # The reference routines' signatures (name, positional arguments, defaults):
