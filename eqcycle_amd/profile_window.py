@@ -28,7 +28,9 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--config", default="cfg4")
     ap.add_argument("--variant", default="galerkin", help="plain | network | linear | galerkin")
-    ap.add_argument("--degree", type=int, default=5)
+    ap.add_argument("--degree", type=int, default=None,
+                    help="fine-level Chebyshev degree of the measured pass (default as bench.py: 5 plain, 7 else)")
+    ap.add_argument("--degree-plain", type=int, default=5, help="degree of the checkpoint run")
     ap.add_argument("--graphs", type=int, default=1)
     a = ap.parse_args()
     import bench
@@ -38,9 +40,12 @@ def main():
 
     cfg = CONFIGS[a.config]
     P = set_parameters(cfg)
-    ctx = EqContext(P, amg={"hierarchy": "pmg", "degree": a.degree})
+    ctx = EqContext(P, amg={"hierarchy": "pmg", "degree": a.degree_plain})
     tol = cfg.pcg_tol
     ck, _ = bench.checkpoint(ctx, cfg, tol, cfg.learned, 300, a.graphs)
+    degree = a.degree if a.degree is not None else (a.degree_plain if a.variant == "plain" else 7)
+    if degree != a.degree_plain:
+        ctx.amg_setup(hierarchy="pmg", degree=degree)
     guess = {"plain": "previous", "network": "learned", "linear": "linear", "galerkin": "galerkin"}[a.variant]
     prec = "fp64" if a.variant == "plain" else cfg.precision
     ctx.set_solver(tol=tol, pc="amg", prec=prec, guess=guess, graphs=bool(a.graphs), basis_fp16=guess == "galerkin")
@@ -49,7 +54,8 @@ def main():
     print("[profile_window] checkpoint and warm-up pass done; the measured pass follows", flush=True)
     ctx.time_kernel("copy", "fp64", reps=1)   # trace marker: the pass starts after this copy
     wall, R = bench.one_pass(ctx, ck, t_end)
-    out = {"config": a.config, "variant": a.variant, "prec": prec, "graphs": bool(a.graphs), "wall_s": wall,
+    out = {"config": a.config, "variant": a.variant, "prec": prec, "degree": degree, "graphs": bool(a.graphs),
+           "wall_s": wall,
            "qs_steps": int(R["n_qs"]), "dyn_steps": int(R["n_dyn"]),
            "pcg_iters_per_solve": R["total_pcg_iters"] / max(1, 2 * R["n_qs"]),
            "wall_qs_s": R["wall_qs"], "wall_dyn_s": R["wall_dyn"], "events": int(len(R["tStart"]))}
