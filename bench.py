@@ -56,9 +56,13 @@ SEGMENT_YEARS = 100.0
 
 # HBM bytes per launch from two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE)
 # calibrated by a 512 MiB copy on the 410,881-node mesh, keyed by (kernel, nodes)
+# (the tile sweeps: the mean of the pre- and post-smoothing sweep, as they alternate; FP32 post in
+# the F32PC form)
 TRAFFIC = {("newmark_kernel", 410881): 27.19e6,
            ("cheb_pair_kernel<double>", 410881): 28.07e6,
-           ("cheb_pair_kernel<float>", 410881): 14.29e6}
+           ("cheb_pair_kernel<float>", 410881): 14.29e6,
+           ("cheb_sweep_kernel<float>", 410881): 10.80e6,
+           ("cheb_sweep_kernel<double>", 410881): 16.14e6}
 
 
 def alg_bytes(kernel, nglob, prec, smoother="tiles"):

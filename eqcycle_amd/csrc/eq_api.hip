@@ -2551,13 +2551,18 @@ static void time_body(eq_ctx* ctx, int which) {
       break;
     }
     case EQ_TIME_SWEEP_POST: {
-      // one whole-tile sweep from chx + P0 coarse (post-smoothing), output c and <c, r>
+      // one whole-tile sweep from chx + P0 coarse (post-smoothing), output c and <c, r>; in FP32 as
+      // the V-cycle of an F32PC solve runs it: c widened to FP64, the partial with the FP64 residual
       ChebSweep<T> op = sweep_params<T>(ctx, 1);
       op.b = w.r;
       op.src = w.chx;
       op.coarse = lvecs<T>(ctx->lev[1]).xp;
       op.xout = w.c;
       op.rdot = w.r;
+      if (sizeof(T) == 4) {
+        op.xout64 = ctx->w64.c;
+        op.rdot64 = ctx->w64.r;
+      }
       launch_cheb_sweep<T>(g, op, ctx->s);
       break;
     }
@@ -2630,6 +2635,7 @@ extern "C" int eq_time_kernel(eq_ctx* ctx, int which, int prec, int reps, double
   if (prec == EQ_PREC_FP32) {
     launch_fill<float>(ctx->N, ctx->w32.x, 1.0f, s); launch_fill<float>(ctx->N, ctx->w32.r, 1.0f, s);
     launch_fill<float>(ctx->N, ctx->w32.c, 1.0f, s); launch_fill<float>(ctx->N, ctx->w32.u[0], 0.0f, s);
+    launch_fill<double>(ctx->N, ctx->w64.r, 1.0, s);   // the FP64 residual of the F32PC post-sweep
   } else {
     launch_fill<double>(ctx->N, ctx->w64.x, 1.0, s); launch_fill<double>(ctx->N, ctx->w64.r, 1.0, s);
     launch_fill<double>(ctx->N, ctx->w64.c, 1.0, s); launch_fill<double>(ctx->N, ctx->w64.u[0], 0.0, s);
