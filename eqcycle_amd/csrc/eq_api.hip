@@ -117,6 +117,7 @@ struct eq_ctx {
   std::map<std::tuple<int, int, int, int64_t, int, const void*, double, int64_t>, hipGraphExec_t> cg_graphs;
   // the next V-cycle's first kernel sets the solve's scalars up (see sweep_pre); tol, maxiter
   bool cg_init_pending = false;
+  bool tail_done = false;   // this step's tail already ran (the quasi-static end kernel)
   double cg_init_tol = 0;
   int64_t cg_init_maxiter = 0;
   CgWork<double> w64;
@@ -2050,6 +2051,16 @@ static void galerkin_correction(eq_ctx* ctx, double* x, const double* b, const d
   ctx->n_learned_solves++;
 }
 
+// The end of a quasi-static step (see launch_qs_end): the second pass's fault
+// update, the step finish, the step tail; with spec also the second solve's
+// convergence test and the save that eq_step restores if its batch fell short.
+static void qs_end(eq_ctx* ctx, bool spec) {
+  launch_qs_end(ctx->g, ctx->fp, ctx->fs, ctx->mu, ctx->d, ctx->dPre, ctx->v, ctx->a, ctx->dt, ctx->dt0,
+                spec ? ctx->spec_save : nullptr, spec ? ctx->sc : nullptr, ctx->part_rr, ctx->npart,
+                spec ? ctx->sc_host : nullptr, ctx->st, ctx->flags, capture(ctx, 0, true), ctx->s);
+  ctx->tail_done = true;
+}
+
 // r_valid: the last solve's residual may be reused (eq_ctx::qs_r_valid).
 static int qs_step(eq_ctx* ctx, eq_step_info* info, bool r_valid) {
   const Geom& g = ctx->g;
@@ -2075,7 +2086,9 @@ static int qs_step(eq_ctx* ctx, eq_step_info* info, bool r_valid) {
       ctx->cg_slot = 1;
       int rc = cg_begin<double>(ctx, ctx->d, ctx->bq, ctx->opts.tol, ctx->opts.maxiter, ctx->opts.pc, f32pc, false,
                                 ctx->pred_iters[1]);
-      if (!rc) rc = cg_batch<double>(ctx, ctx->d, ctx->opts.pc, f32pc, ctx->pred_iters[1]);
+      // the batch's convergence test goes into the step's end kernel below
+      if (!rc) rc = enqueue_iterations<double>(ctx, ctx->d, ctx->opts.pc, f32pc, ctx->pred_iters[1]);
+      ctx->cg_init_pending = false;
       ctx->cg_slot = 2;
       if (rc) return rc;
       // the fault update saves psi, Vf and the fault row of d first (restored
@@ -2083,7 +2096,6 @@ static int qs_step(eq_ctx* ctx, eq_step_info* info, bool r_valid) {
       ctx->spec_pending = true;
       ctx->spec_f32pc = f32pc;
       ctx->spec_batch = ctx->pred_iters[1];
-      launch_qs_fault(g, ctx->fp, ctx->fs, ctx->mu, ctx->d, ctx->v, dt, s, ctx->spec_save);
       continue;
     }
     ctx->cg_slot = pass;
@@ -2094,9 +2106,11 @@ static int qs_step(eq_ctx* ctx, eq_step_info* info, bool r_valid) {
     else if (rc) return rc;
     info->pcg_iters[pass] = it;
     if (pass == 0 && ctx->learn_record_on) learn_record(&ctx->learn, ctx->d, ctx->dPre, s);
-    launch_qs_fault(g, ctx->fp, ctx->fs, ctx->mu, ctx->d, ctx->v, dt, s);
+    if (pass == 0) launch_qs_fault(g, ctx->fp, ctx->fs, ctx->mu, ctx->d, ctx->v, dt, s);
   }
-  launch_qs_finish(g, ctx->fp, ctx->fs, ctx->d, ctx->dPre, ctx->v, ctx->a, dt, s);
+  // second pass's fault update, the step finish and the step tail (with a pass
+  // sent ahead, also its convergence test and the save of what the update reads)
+  qs_end(ctx, ctx->spec_pending);
   return EQ_OK;
 }
 
@@ -2126,10 +2140,12 @@ extern "C" int eq_step(eq_ctx* ctx, eq_step_info* info) {
   ctx->flags_dirty = true;   // until the step tail has reset them
   const bool r_valid = ctx->qs_r_valid;
   ctx->qs_r_valid = false;
+  ctx->tail_done = false;
   int rc = (ctx->isolver == 1) ? qs_step(ctx, info, r_valid) : dyn_step(ctx);
   if (rc) return rc;
-  launch_step_tail(ctx->fp, ctx->d, ctx->v, ctx->a, ctx->fs.tau, ctx->dt, ctx->dt0, ctx->isolver, ctx->st,
-                   ctx->flags, capture(ctx, 0, true), ctx->s);
+  if (!ctx->tail_done)
+    launch_step_tail(ctx->fp, ctx->d, ctx->v, ctx->a, ctx->fs.tau, ctx->dt, ctx->dt0, ctx->isolver, ctx->st,
+                     ctx->flags, capture(ctx, 0, true), ctx->s);
   const bool spec = ctx->spec_pending;
   ctx->spec_pending = false;
   if (int rc2 = sync_status(ctx)) return rc2;
@@ -2144,11 +2160,8 @@ extern "C" int eq_step(eq_ctx* ctx, eq_step_info* info) {
       int rc3 = cg_rest<double>(ctx, ctx->d, ctx->opts.pc, ctx->spec_f32pc, ctx->spec_batch);
       ctx->cg_slot = 2;
       if (rc3) return rc3;
-      launch_qs_fault(ctx->g, ctx->fp, ctx->fs, ctx->mu, ctx->d, ctx->v, ctx->dt, s);
-      launch_qs_finish(ctx->g, ctx->fp, ctx->fs, ctx->d, ctx->dPre, ctx->v, ctx->a, ctx->dt, s);
       // (the first step tail left the flags cleared; nothing in this step sets them)
-      launch_step_tail(ctx->fp, ctx->d, ctx->v, ctx->a, ctx->fs.tau, ctx->dt, ctx->dt0, ctx->isolver, ctx->st,
-                       ctx->flags, capture(ctx, 0, true), s);
+      qs_end(ctx, false);
       if (int rc4 = sync_status(ctx)) return rc4;
     }
     ctx->cg_slot = 1;

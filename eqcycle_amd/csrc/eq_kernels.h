@@ -256,8 +256,15 @@ void launch_qs_begin(const Geom& g, const FaultDev& fp, FaultState fs, const dou
 // save: psi, Vf and the fault row of d before the update (3 nflt), or null
 void launch_qs_fault(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, const double* d,
                      double* v, double dt, hipStream_t s, double* save = nullptr);
-void launch_qs_finish(const Geom& g, const FaultDev& fp, FaultState fs, double* d, const double* dPre, double* v,
-                      double* a, double dt, hipStream_t s);
+// The end of a quasi-static step in two launches: the interior of the step
+// finish (v = (d - dPre) / dt off the fault, a = 0), then one block with, when
+// sc is set, the second solve's convergence test (scalars to sc_mirror), the
+// second pass's fault update (save: psi, Vf and d of the fault row first), the
+// fault row of the finish and the step tail (status, flags, cap): what
+// launch_cg_check, launch_qs_fault and launch_step_tail do, and the finish.
+void launch_qs_end(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, double* d, const double* dPre,
+                   double* v, double* a, double dt, double dtmin, double* save, CgScal* sc, const double* part_rr,
+                   int nparts, CgScal* sc_mirror, StepStatus* st, int* flags, const StepCapture& cap, hipStream_t s);
 // Vfmax, regime switch, dtevol!, Newton flags (then reset for the next step),
 // and the step record of cap.
 void launch_step_tail(const FaultDev& fp, const double* d, const double* v, const double* a, const double* tau,

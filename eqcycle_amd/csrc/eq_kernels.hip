@@ -1321,10 +1321,11 @@ __global__ __launch_bounds__(BLOCK) void qs_begin_kernel(Geom g, FaultDev fp, Fa
 // tau1 = -(K d)[fault]/FltB (0 on creeping nodes), slrFunc!, Vf and fault v update
 // save (or null): [psi | Vf | fault row of d], each nflt long, as they were
 // before this update (the state a redone second pass restores)
-__global__ __launch_bounds__(BLOCK) void qs_fault_kernel(Geom g, FaultDev fp, FaultState fs, const double* mu,
-                                                         const double* d, double* v, double dt, double* save) {
-  const int j = blockIdx.x * BLOCK + threadIdx.x;
-  if (j >= fp.nflt) return;
+// Fault update of a quasi-static pass at node j: tau1 = -(K d)[fault] / FltB, slrFunc!'s
+// psi1 and Vf1, the average Vf and v on the fault (src/main.jl:262-282); with save,
+// psi, Vf and d of the node first.
+__device__ void qs_fault_node(const Geom& g, const FaultDev& fp, const FaultState& fs, const double* mu,
+                              const double* d, double* v, double dt, double* save, int j) {
   if (save) {
     save[j] = fs.psi[j];
     save[fp.nflt + j] = fs.Vf[j];
@@ -1348,29 +1349,40 @@ __global__ __launch_bounds__(BLOCK) void qs_fault_kernel(Geom g, FaultDev fp, Fa
   fs.Vf1[j] = Vf1;
   const double Vf = (fs.Vf0[j] + Vf1) / 2;
   fs.Vf[j] = Vf;
-  v[j] = 0.5 * (Vf - fp.Vpl);
+  if (v) v[j] = 0.5 * (Vf - fp.Vpl);
 }
 
-__global__ __launch_bounds__(BLOCK) void qs_finish_kernel(Geom g, FaultDev fp, FaultState fs, double* d,
-                                                          const double* dPre, double* v, double* a, double dt) {
+__global__ __launch_bounds__(BLOCK) void qs_fault_kernel(Geom g, FaultDev fp, FaultState fs, const double* mu,
+                                                         const double* d, double* v, double dt, double* save) {
+  const int j = blockIdx.x * BLOCK + threadIdx.x;
+  if (j >= fp.nflt) return;
+  qs_fault_node(g, fp, fs, mu, d, v, dt, save, j);
+}
+
+// The fault row of the step finish at node j (src/main.jl:285-293): psi, tau, v on
+// the fault; creeping nodes slide at Vpl with d = v = 0 there.
+__device__ __forceinline__ void qs_finish_fault_node(const FaultDev& fp, const FaultState& fs, double* d, double* v,
+                                                     int j) {
+  const bool deep = j < fp.j0;
+  fs.psi[j] = fs.psi1[j];
+  fs.tau[j] = deep ? 0.0 : fs.tau1[j];
+  if (deep) fs.Vf1[j] = fp.Vpl;
+  double vv = 0.5 * (fs.Vf1[j] - fp.Vpl);
+  if (deep) {
+    vv = 0.0;
+    d[j] = 0.0;
+  }
+  v[j] = vv;
+}
+
+// The interior part of the step finish: v = (d - dPre) / dt off the fault, a = 0
+// everywhere (the fault row's v is qs_end_kernel's).
+__global__ __launch_bounds__(BLOCK) void qs_finish_interior_kernel(Geom g, const double* d, const double* dPre,
+                                                                   double* v, double* a, double dt) {
   const int64_t N = (int64_t)g.NX * g.NY;
   const int64_t n = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   if (n >= N) return;
-  if (n < g.NX) {
-    const int j = (int)n;
-    const bool deep = j < fp.j0;
-    fs.psi[j] = fs.psi1[j];
-    fs.tau[j] = deep ? 0.0 : fs.tau1[j];
-    if (deep) fs.Vf1[j] = fp.Vpl;
-    double vv = 0.5 * (fs.Vf1[j] - fp.Vpl);
-    if (deep) {
-      vv = 0.0;
-      d[n] = 0.0;
-    }
-    v[n] = vv;
-  } else {
-    v[n] = (d[n] - dPre[n]) / dt;
-  }
+  if (n >= g.NX) v[n] = (d[n] - dPre[n]) / dt;
   a[n] = 0.0;
 }
 
@@ -1401,10 +1413,11 @@ __device__ double dtevol_dev(const double* XiLf, const double* v, bool slip_rate
   return dtnx;
 }
 
-__global__ __launch_bounds__(BLOCK) void step_tail_kernel(FaultDev fp, const double* d, const double* v,
-                                                          const double* a, const double* tau, double dt,
-                                                          double dtmin, int isolver, StepStatus* st, int* flags,
-                                                          StepCapture cap) {
+// The step tail over the fault with a block of NT threads.
+template <int NT>
+__device__ void step_tail_body(const FaultDev& fp, const double* d, const double* v, const double* a,
+                               const double* tau, double dt, double dtmin, int isolver, StepStatus* st, int* flags,
+                               const StepCapture& cap) {
   // one pass over the fault: the maximum of v (first index on ties) and the
   // quasi-static dtevol! candidate min XiLf/|Vf| over the cells that qualify,
   // reduced across each wave by shuffles and across the four waves in LDS
@@ -1412,7 +1425,7 @@ __global__ __launch_bounds__(BLOCK) void step_tail_kernel(FaultDev fp, const dou
   const double dtmax = 50 * 24 * 60 * 60;
   double m = -INFINITY, dtnx = dtmax;
   int mi = INT_MAX;
-  for (int j = threadIdx.x; j < fp.nflt; j += BLOCK) {
+  for (int j = threadIdx.x; j < fp.nflt; j += NT) {
     const double vj = v[j];
     if (findmax_before(vj, j, m, mi)) { m = vj; mi = j; }
     if (j >= fp.j0) {
@@ -1430,8 +1443,8 @@ __global__ __launch_bounds__(BLOCK) void step_tail_kernel(FaultDev fp, const dou
     if (findmax_before(om, oi, m, mi)) { m = om; mi = oi; }
     dtnx = fmin(dtnx, __shfl_down(dtnx, off, 64));
   }
-  __shared__ double sm[BLOCK / 64], sdt[BLOCK / 64];
-  __shared__ int si[BLOCK / 64];
+  __shared__ double sm[NT / 64], sdt[NT / 64];
+  __shared__ int si[NT / 64];
   if ((threadIdx.x & 63) == 0) {
     sm[threadIdx.x >> 6] = m;
     si[threadIdx.x >> 6] = mi;
@@ -1441,7 +1454,7 @@ __global__ __launch_bounds__(BLOCK) void step_tail_kernel(FaultDev fp, const dou
   m = sm[0];
   mi = si[0];
   dtnx = sdt[0];
-  for (int w = 1; w < BLOCK / 64; w++) {
+  for (int w = 1; w < NT / 64; w++) {
     if (findmax_before(sm[w], si[w], m, mi)) { m = sm[w]; mi = si[w]; }
     dtnx = fmin(dtnx, sdt[w]);
   }
@@ -1455,14 +1468,14 @@ __global__ __launch_bounds__(BLOCK) void step_tail_kernel(FaultDev fp, const dou
     dtn = dtnx;
   }
   if (cap.fault) {
-    for (int j = threadIdx.x; j < fp.nflt; j += BLOCK) {
+    for (int j = threadIdx.x; j < fp.nflt; j += NT) {
       cap.fault[j] = d[j];
       cap.fault[fp.nflt + j] = v[j];
       cap.fault[2 * fp.nflt + j] = tau[j];
     }
   }
   if (cap.seis) {
-    for (int q = threadIdx.x; q < cap.nseis; q += BLOCK) {
+    for (int q = threadIdx.x; q < cap.nseis; q += NT) {
       const int64_t n = cap.seis_idx[q];
       const double dn = d[n], vn = v[n], an = a[n];
       cap.seis[3 * q] = dn;
@@ -1488,6 +1501,42 @@ __global__ __launch_bounds__(BLOCK) void step_tail_kernel(FaultDev fp, const dou
     flags[0] = INT_MAX;
     flags[1] = 0;
   }
+}
+
+__global__ __launch_bounds__(BLOCK) void step_tail_kernel(FaultDev fp, const double* d, const double* v,
+                                                          const double* a, const double* tau, double dt,
+                                                          double dtmin, int isolver, StepStatus* st, int* flags,
+                                                          StepCapture cap) {
+  step_tail_body<BLOCK>(fp, d, v, a, tau, dt, dtmin, isolver, st, flags, cap);
+}
+
+// The end of a quasi-static step in one block (QE_BLOCK threads, fault nodes strided):
+// with sc, the second solve's convergence test (cg_check_kernel's, mirror included);
+// the second pass's fault update (with save: psi, Vf and d of each node first); the
+// fault row of the step finish; the step tail, which reads every fault node.  The
+// interior part of the finish (qs_finish_interior_kernel) runs before it.
+constexpr int QE_BLOCK = 1024;
+__global__ __launch_bounds__(QE_BLOCK) void qs_end_kernel(Geom g, FaultDev fp, FaultState fs, const double* mu,
+                                                          double* d, double* v, const double* a, double dt,
+                                                          double dtmin, double* save, CgScal* sc,
+                                                          const double* part_rr, int nparts, CgScal* sc_mirror,
+                                                          StepStatus* st, int* flags, StepCapture cap) {
+  if (sc) {
+    if (!sc->done) {   // block-uniform
+      const double rr = reduce_parts(part_rr, nparts);
+      if (threadIdx.x == 0) {
+        const double resid = sqrt(rr);
+        sc->resid = resid;
+        if (resid <= sc->reltol || sc->iter >= sc->maxiter) sc->done = 1;
+      }
+    }
+    if (sc_mirror && threadIdx.x == 0) *sc_mirror = *sc;
+  }
+  for (int j = threadIdx.x; j < fp.nflt; j += QE_BLOCK) qs_fault_node(g, fp, fs, mu, d, nullptr, dt, save, j);
+  __syncthreads();   // every tau1 read d's fault row before creeping nodes reset it
+  for (int j = threadIdx.x; j < fp.nflt; j += QE_BLOCK) qs_finish_fault_node(fp, fs, d, v, j);
+  __syncthreads();
+  step_tail_body<QE_BLOCK>(fp, d, v, a, fs.tau, dt, dtmin, 1, st, flags, cap);
 }
 
 // --------------------------------------------------------------------------
@@ -2655,10 +2704,13 @@ void launch_qs_fault(const Geom& g, const FaultDev& fp, FaultState fs, const dou
   hipLaunchKernelGGL(qs_fault_kernel, dim3(nblk(fp.nflt)), dim3(BLOCK), 0, s, g, fp, fs, mu, d, v, dt, save);
 }
 
-void launch_qs_finish(const Geom& g, const FaultDev& fp, FaultState fs, double* d, const double* dPre, double* v,
-                      double* a, double dt, hipStream_t s) {
+void launch_qs_end(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, double* d, const double* dPre,
+                   double* v, double* a, double dt, double dtmin, double* save, CgScal* sc, const double* part_rr,
+                   int nparts, CgScal* sc_mirror, StepStatus* st, int* flags, const StepCapture& cap, hipStream_t s) {
   const int64_t N = (int64_t)g.NX * g.NY;
-  hipLaunchKernelGGL(qs_finish_kernel, dim3(nblk(N)), dim3(BLOCK), 0, s, g, fp, fs, d, dPre, v, a, dt);
+  hipLaunchKernelGGL(qs_finish_interior_kernel, dim3(nblk(N)), dim3(BLOCK), 0, s, g, d, dPre, v, a, dt);
+  hipLaunchKernelGGL(qs_end_kernel, dim3(1), dim3(QE_BLOCK), 0, s, g, fp, fs, mu, d, v, a, dt, dtmin, save, sc,
+                     part_rr, nparts, sc_mirror, st, flags, cap);
 }
 
 void launch_step_tail(const FaultDev& fp, const double* d, const double* v, const double* a, const double* tau,
