@@ -53,6 +53,8 @@ YR = 365 * 24 * 60 * 60
 HBM_PEAK_GBS = 8000.0
 CHECKPOINT_YEARS = 110.0
 SEGMENT_YEARS = 100.0
+# the Galerkin start's first pass from the last increment extrapolated (eq_solver_opts.extrapolate)
+EXTRAPOLATE = True
 
 # HBM bytes per launch from two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE)
 # calibrated by a 512 MiB copy on the 410,881-node mesh, keyed by (kernel, nodes)
@@ -109,6 +111,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--plain-only", action="store_true")
     ap.add_argument("--nbasis", type=int, default=64, help="POD basis vectors of the learned starts")
+    ap.add_argument("--extrapolate", type=int, default=1,
+                    help="Galerkin start from the last increment extrapolated (1) or from the last solution (0)")
     return ap.parse_args()
 
 
@@ -160,7 +164,8 @@ def one_pass(ctx, ck, t_end):
 def variant(ctx, ck, tol, prec, guess, warmup, steps, graphs):
     """warmup + steps passes over the cycle; statistics of the timed ones."""
     import numpy as np
-    ctx.set_solver(tol=tol, pc="amg", prec=prec, guess=guess, graphs=bool(graphs), basis_fp16=guess == "galerkin")
+    ctx.set_solver(tol=tol, pc="amg", prec=prec, guess=guess, graphs=bool(graphs), basis_fp16=guess == "galerkin",
+                   extrapolate=guess == "galerkin" and EXTRAPOLATE)
     t_end = ck["t"] + SEGMENT_YEARS * YR
     for _ in range(warmup):
         one_pass(ctx, ck, t_end)
@@ -196,7 +201,8 @@ def kernel_time(ctx, ck, tol, prec, guess, P, wall_pass, smoother):
     """One more pass of the variant with every fine-level launch bracketed by
     HIP events (CG batches launch eagerly: same kernels); device time per
     kernel, the dominant one, and its roofline."""
-    ctx.set_solver(tol=tol, pc="amg", prec=prec, guess=guess, graphs=False, basis_fp16=guess == "galerkin")
+    ctx.set_solver(tol=tol, pc="amg", prec=prec, guess=guess, graphs=False, basis_fp16=guess == "galerkin",
+                   extrapolate=guess == "galerkin" and EXTRAPOLATE)
     ctx.launch_timing(True)
     w, R = one_pass(ctx, ck, ck["t"] + SEGMENT_YEARS * YR)
     kt = ctx.launch_timing_get()
@@ -324,6 +330,8 @@ def cpu_baseline(P, tol, ck, dyn_state, mix, budget_s):
 
 def main():
     a = parse()
+    global EXTRAPOLATE
+    EXTRAPOLATE = bool(a.extrapolate)
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {a.gpus}): the library drives one device, and the "
               "decomposition over several GPUs is a design note, not a measured path", file=sys.stderr)

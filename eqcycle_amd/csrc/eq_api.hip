@@ -106,6 +106,9 @@ struct eq_ctx {
   // just the change of b
   bool qs_r_valid = false;
   double* gal_db = nullptr;           // [4 NX] change of b on rows 1..4
+  double* gal_bold = nullptr;         // [4 NX] rows 1..4 of b of the step before (extrapolated start)
+  bool qs_prev_r_valid = false;       // the last quasi-static step also followed one
+  double qs_prev_dt = 0;              // its dt
   bool spec_f32pc = false;
   int64_t spec_batch = 0;
   double t = 0, dt = 0;
@@ -1894,6 +1897,10 @@ static int alloc_state(eq_ctx* ctx) {
     ctx->spec_save = ctx->alloc<double>(3 * ctx->NX);
     NEED(ctx->spec_save);
   }
+  if (!ctx->gal_bold) {
+    ctx->gal_bold = ctx->alloc<double>(4 * ctx->NX);
+    NEED(ctx->gal_bold);
+  }
   if (!ctx->gal_db) {
     ctx->gal_db = ctx->alloc<double>(4 * ctx->NX);
     NEED(ctx->gal_db);
@@ -2066,13 +2073,22 @@ static int qs_step(eq_ctx* ctx, eq_step_info* info, bool r_valid) {
   const Geom& g = ctx->g;
   hipStream_t s = ctx->s;
   const double dt = ctx->dt;
+  bool bold_saved = false;
   for (int pass = 0; pass < 2; pass++) {
     const bool galerkin = ctx->opts.guess == EQ_GUESS_GALERKIN && ctx->learn.gal_ready;
     // the previous solve (this step's first, or the last step's second when
     // r_valid) left d solving K_II d = b_old; only b rows 1..4 change
     const bool upd = (pass == 1 || r_valid) && galerkin && !ctx->opts.galerkin_full;
+    // first pass after two Galerkin-started quasi-static steps in a row, each
+    // from the solve before it: start from the last increment extrapolated
+    // (d and dPre are then consecutive solutions, gal_bold the b of the older)
+    const double theta = (pass == 0 && galerkin && r_valid && ctx->opts.extrapolate && ctx->qs_prev_r_valid &&
+                          ctx->qs_prev_dt > 0)
+                             ? dt / ctx->qs_prev_dt
+                             : 0.0;
+    if (pass == 0) bold_saved = galerkin;
     launch_qs_begin(g, ctx->fp, ctx->fs, ctx->mu, ctx->d, ctx->dPre, ctx->v, ctx->bq, dt, pass == 0, s,
-                    upd ? ctx->gal_db : nullptr);
+                    upd ? ctx->gal_db : nullptr, theta, pass == 0 && galerkin ? ctx->gal_bold : nullptr);
     const bool learned = pass == 0 && ctx->opts.guess == EQ_GUESS_LEARNED && ctx->learn_on;
     const bool linear = pass == 0 && ctx->opts.guess == EQ_GUESS_LINEAR && ctx->learn.lin_ready;
     if (learned || linear) learned_correction(ctx, ctx->d, ctx->bq, linear);
@@ -2111,6 +2127,8 @@ static int qs_step(eq_ctx* ctx, eq_step_info* info, bool r_valid) {
   // second pass's fault update, the step finish and the step tail (with a pass
   // sent ahead, also its convergence test and the save of what the update reads)
   qs_end(ctx, ctx->spec_pending);
+  ctx->qs_prev_r_valid = r_valid && bold_saved;
+  ctx->qs_prev_dt = dt;
   return EQ_OK;
 }
 

@@ -250,9 +250,12 @@ void launch_permute(int64_t n, const int32_t* perm, const double* src, double* d
 // P = d + dt v + dt^2/2 a on n nodes
 void launch_newmark_predict(int64_t n, const double* d, const double* v, const double* a, double dt,
                             double half_dt_sq, double* P, hipStream_t s);
+// db: the change of b on rows 1..4 since the last solve; bold: those rows of b
+// before this pass (saved); theta != 0 (first pass): the start off the fault
+// becomes d + theta (d - dPre) and db the change less theta times bold's.
 void launch_qs_begin(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, double* d, double* dPre,
                      const double* v, double* b, double dt, int first_pass, hipStream_t s,
-                     double* db = nullptr);
+                     double* db = nullptr, double theta = 0.0, double* bold = nullptr);
 // save: psi, Vf and the fault row of d before the update (3 nflt), or null
 void launch_qs_fault(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, const double* d,
                      double* v, double dt, hipStream_t s, double* save = nullptr);

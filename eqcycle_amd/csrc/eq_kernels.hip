@@ -1288,12 +1288,16 @@ __global__ __launch_bounds__(BLOCK) void dense_kernel(int64_t n, const T* inv, c
 // b_new - b_old on rows 1..4 (row J at db[(J - 1) NX + I]).
 __global__ __launch_bounds__(BLOCK) void qs_begin_kernel(Geom g, FaultDev fp, FaultState fs, const double* mu,
                                                          double* d, double* dPre, const double* v, double* b,
-                                                         double dt, int first_pass, double* db) {
+                                                         double dt, int first_pass, double* db, double theta,
+                                                         double* bold) {
   const int64_t N = (int64_t)g.NX * g.NY;
   const int64_t n = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   if (first_pass) {
     if (n >= N) return;
-    dPre[n] = d[n];
+    const double dn = d[n];
+    // extrapolated start off the fault: d + theta (d - dPre), the last step's increment
+    if (theta != 0.0 && n >= g.NX) d[n] = dn + theta * (dn - dPre[n]);
+    dPre[n] = dn;
   } else if (n >= g.NX) {
     return;
   }
@@ -1312,8 +1316,13 @@ __global__ __launch_bounds__(BLOCK) void qs_begin_kernel(Geom g, FaultDev fp, Fa
     double s = 0.0;
     for (int q = 0; q < 5; q++) s += g.D[q * 5 + J] * (g.w[q] * mu[(int64_t)q * g.NX + I]) * g.D[q * 5 + 0];
     const int64_t m = (int64_t)J * g.NX + I;
+    const int64_t r4 = (int64_t)(J - 1) * g.NX + I;
     const double bn = -(cy * s) * F;
-    if (db) db[(int64_t)(J - 1) * g.NX + I] = bn - b[m];
+    // change of b since the solve the start comes from; with the extrapolation,
+    // less theta times the change the step before (the start then solves that
+    // extrapolated b off rows 1..4 already)
+    if (db) db[r4] = theta != 0.0 ? (bn - b[m]) - theta * (b[m] - bold[r4]) : bn - b[m];
+    if (bold) bold[r4] = b[m];
     b[m] = bn;
   }
 }
@@ -2693,10 +2702,11 @@ void launch_newmark(const Geom& g, const FaultDev& fp, FaultState fs, const doub
 }
 
 void launch_qs_begin(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, double* d, double* dPre,
-                     const double* v, double* b, double dt, int first_pass, hipStream_t s, double* db) {
+                     const double* v, double* b, double dt, int first_pass, hipStream_t s, double* db, double theta,
+                     double* bold) {
   const int64_t N = first_pass ? (int64_t)g.NX * g.NY : g.NX;
   hipLaunchKernelGGL(qs_begin_kernel, dim3(nblk(N)), dim3(BLOCK), 0, s, g, fp, fs, mu, d, dPre, v, b, dt,
-                     first_pass, db);
+                     first_pass, db, theta, bold);
 }
 
 void launch_qs_fault(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, const double* d, double* v,

@@ -232,6 +232,12 @@ typedef struct {
   int basis_fp16;          /* 1: the Galerkin start's expansion x += Phi c reads an FP16 copy of the
                               basis (half the bytes; the rounding, 5e-4 of the increment Phi c, is
                               left to CG, which runs to the same tolerance); 0: the FP32 basis */
+  int extrapolate;         /* 1: after two Galerkin-started quasi-static steps in a row, the first
+                              pass's start begins from d + (dt / dt_prev) (d - d_prev), the last
+                              step's increment extrapolated, and projects the change of b less the
+                              same multiple of the last step's (with galerkin_full, the full
+                              residual of that start); 0 (default): from d.  Off the Galerkin
+                              start it has no effect */
 } eq_solver_opts;
 int eq_set_solver(eq_ctx *ctx, const eq_solver_opts *opts);
 

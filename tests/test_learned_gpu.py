@@ -195,6 +195,42 @@ def test_fp16_basis_expansion(trained, tol):
     assert np.max(np.abs(d_a - d_b)) <= 20 * tol * np.max(np.abs(d_a))
 
 
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-10), ("f32pc", 1e-6)])
+def test_extrapolated_galerkin_start(trained, prec, tol):
+    """The first pass's Galerkin start from the last increment extrapolated,
+    d + (dt/dt_prev)(d - d_prev) (eq_solver_opts.extrapolate): the same 40
+    steps from the same state without it need more first-pass iterations,
+    and the displacements agree to the solver tolerance.  With it, projecting
+    the change of b less (dt/dt_prev) times the last step's (the start then
+    solves the extrapolated b off rows 1..4) gives the start that projecting
+    the full residual of the extrapolated displacements gives: iteration
+    counts within 1 % (and one iteration) and displacements to 10 tol.  A wrong
+    sign or factor in either the displacement or the b extrapolation breaks
+    the first or the second comparison."""
+    P, ctx, _ = trained
+    s0 = ctx.state_get()
+
+    def run(extrapolate, shortcut=True):
+        ctx.state_set(d=s0["d"], v=s0["v"], a=s0["a"], psi=s0["psi"], tau=s0["tau"], t=s0["t"], dt=s0["dt"],
+                      isolver=s0["isolver"])
+        ctx.set_solver(tol=tol, pc="amg", prec=prec, guess="galerkin", galerkin_shortcut=shortcut,
+                       extrapolate=extrapolate)
+        its = np.array([ctx.step().pcg_iters for _ in range(40)])
+        return its, ctx.state_get()["d"]
+
+    it_0, d_0 = run(False)
+    it_x, d_x = run(True)
+    it_f, d_f = run(True, shortcut=False)
+    print(f"[gpu] {prec}: first-pass iterations from d {it_0[:, 0].sum()}, extrapolated {it_x[:, 0].sum()}, "
+          f"extrapolated with the full residual {it_f[:, 0].sum()}", flush=True)
+    # the first two steps cannot extrapolate (no consecutive solves yet)
+    assert np.array_equal(it_x[:2], it_0[:2])
+    assert it_x[:, 0].sum() < it_0[:, 0].sum()
+    assert np.max(np.abs(d_x - d_0)) <= 20 * tol * np.max(np.abs(d_0))
+    assert abs(it_x.sum() - it_f.sum()) <= 0.01 * it_f.sum() + 1
+    assert np.max(np.abs(d_x - d_f)) <= 10 * tol * np.max(np.abs(d_f))
+
+
 def _bf16(x):
     """Round FP32 values to BF16 (nearest even) and back, as the kernels store weights."""
     u = np.asarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
