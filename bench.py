@@ -55,6 +55,8 @@ CHECKPOINT_YEARS = 110.0
 SEGMENT_YEARS = 100.0
 # the Galerkin start's first pass from the last increment extrapolated (eq_solver_opts.extrapolate)
 EXTRAPOLATE = True
+# its expansion and start residual in one launch (eq_solver_opts.galerkin_split = 0)
+GALERKIN_FUSED = True
 
 # HBM bytes per launch from two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE)
 # calibrated by a 512 MiB copy on the 410,881-node mesh, keyed by (kernel, nodes)
@@ -111,6 +113,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--plain-only", action="store_true")
     ap.add_argument("--nbasis", type=int, default=64, help="POD basis vectors of the learned starts")
+    ap.add_argument("--galerkin-fused", type=int, default=1,
+                    help="Galerkin expansion and start residual in one launch (1) or two (0)")
     ap.add_argument("--extrapolate", type=int, default=1,
                     help="Galerkin start from the last increment extrapolated (1) or from the last solution (0)")
     return ap.parse_args()
@@ -165,7 +169,8 @@ def variant(ctx, ck, tol, prec, guess, warmup, steps, graphs):
     """warmup + steps passes over the cycle; statistics of the timed ones."""
     import numpy as np
     ctx.set_solver(tol=tol, pc="amg", prec=prec, guess=guess, graphs=bool(graphs), basis_fp16=guess == "galerkin",
-                   extrapolate=guess == "galerkin" and EXTRAPOLATE)
+                   extrapolate=guess == "galerkin" and EXTRAPOLATE,
+                   galerkin_fused=GALERKIN_FUSED)
     t_end = ck["t"] + SEGMENT_YEARS * YR
     for _ in range(warmup):
         one_pass(ctx, ck, t_end)
@@ -202,7 +207,8 @@ def kernel_time(ctx, ck, tol, prec, guess, P, wall_pass, smoother):
     HIP events (CG batches launch eagerly: same kernels); device time per
     kernel, the dominant one, and its roofline."""
     ctx.set_solver(tol=tol, pc="amg", prec=prec, guess=guess, graphs=False, basis_fp16=guess == "galerkin",
-                   extrapolate=guess == "galerkin" and EXTRAPOLATE)
+                   extrapolate=guess == "galerkin" and EXTRAPOLATE,
+                   galerkin_fused=GALERKIN_FUSED)
     ctx.launch_timing(True)
     w, R = one_pass(ctx, ck, ck["t"] + SEGMENT_YEARS * YR)
     kt = ctx.launch_timing_get()
@@ -330,8 +336,9 @@ def cpu_baseline(P, tol, ck, dyn_state, mix, budget_s):
 
 def main():
     a = parse()
-    global EXTRAPOLATE
+    global EXTRAPOLATE, GALERKIN_FUSED
     EXTRAPOLATE = bool(a.extrapolate)
+    GALERKIN_FUSED = bool(a.galerkin_fused)
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {a.gpus}): the library drives one device, and the "
               "decomposition over several GPUs is a design note, not a measured path", file=sys.stderr)

@@ -297,7 +297,8 @@ class EqContext:
 
     # ---- time loop ------------------------------------------------------------
     def set_solver(self, tol=1e-6, maxiter=0, pc="amg", prec="fp64", guess="previous", graphs=True,
-                   cheb_pairs=True, early_tail=True, galerkin_shortcut=True, basis_fp16=False, extrapolate=False):
+                   cheb_pairs=True, early_tail=True, galerkin_shortcut=True, basis_fp16=False, extrapolate=False,
+                   galerkin_fused=True):
         """early_tail: the second quasi-static solve's fault update and the
         step's status read-back go out behind its first batch of iterations
         and are redone when that batch fell short (same results, one host
@@ -305,13 +306,15 @@ class EqContext:
         solve of the same x projects only the change of b (rows 1..4).
         basis_fp16: the Galerkin expansion reads the FP16 copy of the basis.
         extrapolate: the Galerkin start of the first pass begins from the last
-        step's increment extrapolated (after two quasi-static steps in a row)."""
+        step's increment extrapolated (after two quasi-static steps in a row).
+        galerkin_fused: the FP16 expansion and the start residual in one launch
+        (False: two, the same results bit for bit)."""
         o = L.SolverOpts(tol, maxiter, PC[pc], PREC[prec],
                          {"previous": L.EQ_GUESS_PREVIOUS, "learned": L.EQ_GUESS_LEARNED,
                           "galerkin": L.EQ_GUESS_GALERKIN, "linear": L.EQ_GUESS_LINEAR}[guess], 1,
                          1 if graphs else 0,
                          0 if cheb_pairs else 1, 0 if early_tail else 1, 0 if galerkin_shortcut else 1,
-                         1 if basis_fp16 else 0, 1 if extrapolate else 0)
+                         1 if basis_fp16 else 0, 1 if extrapolate else 0, 0 if galerkin_fused else 1)
         self._check(self.lib.eq_set_solver(self.h, C.byref(o)))
 
     def state_init(self, v_init=None, psi_init=None):

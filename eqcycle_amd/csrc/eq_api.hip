@@ -1337,7 +1337,7 @@ static int enqueue_iterations(eq_ctx* ctx, T* x, int pc, bool f32pc, int64_t n) 
 // instead of a launch of its own.
 template <typename T>
 static int cg_begin(eq_ctx* ctx, T* x, const T* b, double tol, int64_t maxiter, int pc, bool f32pc, bool r_ready,
-                    int64_t first_batch) {
+                    int64_t first_batch, bool r32_ready = false) {
   CgWork<T>& w = work<T>(ctx);
   hipStream_t s = ctx->s;
   if (pc == EQ_PC_AMG && !ctx->have_amg) return fail(ctx, EQ_ENOTREADY, "eq_amg_setup first");
@@ -1354,7 +1354,7 @@ static int cg_begin(eq_ctx* ctx, T* x, const T* b, double tol, int64_t maxiter, 
   } else {
     launch_cg_init_scal(ctx->sc, ctx->part_rr, ctx->part_bb, ctx->npart, tol, (int)maxiter, s);
   }
-  if (f32pc && r_ready) launch_convert<T, float>(ctx->N, w.r, ctx->w32.r, s);
+  if (f32pc && r_ready && !r32_ready) launch_convert<T, float>(ctx->N, w.r, ctx->w32.r, s);
   return EQ_OK;
 }
 
@@ -1391,12 +1391,13 @@ static int cg_end(eq_ctx* ctx, int64_t* iters, double* relres) {
 }
 
 // r_ready: w.r already holds b - A x with its ||r||^2 partials in part_rr and
-// the ||b||^2 partials in part_bb (the learned start leaves them so).
+// the ||b||^2 partials in part_bb (the learned and Galerkin starts leave them
+// so); r32_ready: and w32.r its FP32 copy.
 template <typename T>
 static int cg_device(eq_ctx* ctx, T* x, const T* b, double tol, int64_t maxiter, int pc, int64_t* iters,
-                     double* relres, bool f32pc = false, bool r_ready = false) {
+                     double* relres, bool f32pc = false, bool r_ready = false, bool r32_ready = false) {
   int64_t batch = ctx->pred_iters[ctx->cg_slot];
-  if (int rc = cg_begin<T>(ctx, x, b, tol, maxiter, pc, f32pc, r_ready, batch)) return rc;
+  if (int rc = cg_begin<T>(ctx, x, b, tol, maxiter, pc, f32pc, r_ready, batch, r32_ready)) return rc;
   if (int rc = cg_batch<T>(ctx, x, pc, f32pc, batch)) return rc;
   if (int rc = spin_sync(ctx)) return rc;
   if (int rc = cg_rest<T>(ctx, x, pc, f32pc, batch)) return rc;
@@ -1440,10 +1441,10 @@ static int cg_mixed(eq_ctx* ctx, double* x, const double* b, double tol, int64_t
 
 // r_ready: see cg_device (honoured by the FP64 and F32PC modes).
 static int cg_any(eq_ctx* ctx, double* x, const double* b, double tol, int64_t maxiter, int pc, int prec,
-                  int64_t* iters, double* relres, bool r_ready = false) {
+                  int64_t* iters, double* relres, bool r_ready = false, bool r32_ready = false) {
   if (prec == EQ_PREC_MIXED) return cg_mixed(ctx, x, b, tol, maxiter, pc, iters, relres);
   if (prec == EQ_PREC_F32PC)
-    return cg_device<double>(ctx, x, b, tol, maxiter, pc, iters, relres, pc == EQ_PC_AMG, r_ready);
+    return cg_device<double>(ctx, x, b, tol, maxiter, pc, iters, relres, pc == EQ_PC_AMG, r_ready, r32_ready);
   if (prec == EQ_PREC_FP32) {
     launch_convert<double, float>(ctx->N, x, ctx->w32.x, ctx->s);
     launch_convert<double, float>(ctx->N, b, ctx->w32.b, ctx->s);
@@ -1452,7 +1453,7 @@ static int cg_any(eq_ctx* ctx, double* x, const double* b, double tol, int64_t m
     int rc2 = sync_check(ctx);
     return rc ? rc : rc2;
   }
-  return cg_device<double>(ctx, x, b, tol, maxiter, pc, iters, relres, false, r_ready);
+  return cg_device<double>(ctx, x, b, tol, maxiter, pc, iters, relres, false, r_ready, r32_ready);
 }
 
 extern "C" int eq_cg(eq_ctx* ctx, double* x, const double* b, double tol, int64_t maxiter, int pc, int prec,
@@ -2058,6 +2059,27 @@ static void galerkin_correction(eq_ctx* ctx, double* x, const double* b, const d
   ctx->n_learned_solves++;
 }
 
+// The change-of-b Galerkin start from the FP16 basis with the start residual
+// in the same launch (launch_gal_residual): x = d + Phi c lands in d2, which
+// then becomes d (d2 is free between dynamic steps), and w64.r (with its FP32
+// copy for an F32PC V-cycle), part_rr and part_bb are what cg_begin's
+// residual would leave.  False when that form does not apply.
+static bool galerkin_start_fused(eq_ctx* ctx, const double* db) {
+  const eq_solver_opts& o = ctx->opts;
+  LearnState& ls = ctx->learn;
+  if (!db || o.galerkin_split || !o.basis_fp16 || !ls.Phi16 || ls.offset != ctx->NX ||
+      (o.prec != EQ_PREC_FP64 && o.prec != EQ_PREC_F32PC))
+    return false;
+  learn_project_rows(&ls, db, 4 * (int64_t)ctx->NX, ctx->s);
+  const bool r32 = o.prec == EQ_PREC_F32PC && o.pc == EQ_PC_AMG;
+  launch_gal_residual(ctx->g, ctx->d, (const half8*)ls.Phi16, ls.nI, ls.d.NB, ls.gal_c, ctx->mu, ctx->bq,
+                      ctx->w64.r, r32 ? ctx->w32.r : nullptr, ctx->d2, ctx->part_rr, ctx->part_bb, ctx->npart,
+                      ctx->s);
+  std::swap(ctx->d, ctx->d2);
+  ctx->n_learned_solves++;
+  return true;
+}
+
 // The end of a quasi-static step (see launch_qs_end): the second pass's fault
 // update, the step finish, the step tail; with spec also the second solve's
 // convergence test and the save that eq_step restores if its batch fell short.
@@ -2092,7 +2114,9 @@ static int qs_step(eq_ctx* ctx, eq_step_info* info, bool r_valid) {
     const bool learned = pass == 0 && ctx->opts.guess == EQ_GUESS_LEARNED && ctx->learn_on;
     const bool linear = pass == 0 && ctx->opts.guess == EQ_GUESS_LINEAR && ctx->learn.lin_ready;
     if (learned || linear) learned_correction(ctx, ctx->d, ctx->bq, linear);
-    if (galerkin) galerkin_correction(ctx, ctx->d, ctx->bq, upd ? ctx->gal_db : nullptr);
+    // r_fused: the Galerkin start left the residual (and its FP32 copy)
+    const bool r_fused = galerkin && galerkin_start_fused(ctx, upd ? ctx->gal_db : nullptr);
+    if (galerkin && !r_fused) galerkin_correction(ctx, ctx->d, ctx->bq, upd ? ctx->gal_db : nullptr);
     int64_t it = 0;
     double rr = 0;
     const int prec = ctx->opts.prec;
@@ -2100,8 +2124,8 @@ static int qs_step(eq_ctx* ctx, eq_step_info* info, bool r_valid) {
       // pass 2 ahead of its read-back (eq_step finishes it)
       const bool f32pc = prec == EQ_PREC_F32PC && ctx->opts.pc == EQ_PC_AMG;
       ctx->cg_slot = 1;
-      int rc = cg_begin<double>(ctx, ctx->d, ctx->bq, ctx->opts.tol, ctx->opts.maxiter, ctx->opts.pc, f32pc, false,
-                                ctx->pred_iters[1]);
+      int rc = cg_begin<double>(ctx, ctx->d, ctx->bq, ctx->opts.tol, ctx->opts.maxiter, ctx->opts.pc, f32pc, r_fused,
+                                ctx->pred_iters[1], r_fused);
       // the batch's convergence test goes into the step's end kernel below
       if (!rc) rc = enqueue_iterations<double>(ctx, ctx->d, ctx->opts.pc, f32pc, ctx->pred_iters[1]);
       ctx->cg_init_pending = false;
@@ -2116,7 +2140,7 @@ static int qs_step(eq_ctx* ctx, eq_step_info* info, bool r_valid) {
     }
     ctx->cg_slot = pass;
     int rc = cg_any(ctx, ctx->d, ctx->bq, ctx->opts.tol, ctx->opts.maxiter, ctx->opts.pc, ctx->opts.prec, &it, &rr,
-                    learned);
+                    learned || r_fused, r_fused);
     ctx->cg_slot = 2;
     if (rc == EQ_PCG_NOCONV) ctx->pcg_noconv++;
     else if (rc) return rc;

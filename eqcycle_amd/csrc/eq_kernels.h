@@ -12,6 +12,9 @@ struct LaunchEvents {
 };
 
 constexpr int BLOCK = 256;
+
+// 8 FP16 basis coefficients of one node (one 16-byte load)
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 #ifndef EQ_TILE_I
 #define EQ_TILE_I 64
 #endif
@@ -94,6 +97,12 @@ void launch_cg_residual(const Geom& g, const T* x, const T* mu, const T* b, T* r
                         double* part_bb, int npart, hipStream_t s, float* r32 = nullptr);
 void launch_cg_init_scal(CgScal* sc, const double* part_rr, const double* part_bb, int nparts, double tol,
                          int maxiter, hipStream_t s);
+// Galerkin start x = d + Phi c (interior; Phi the blocked FP16 basis
+// [ceil(nb / 8)][nI][8]) into x_out, and r = b - K_II x with the partials of
+// launch_cg_residual, in one launch.  x_out must not alias d.
+void launch_gal_residual(const Geom& g, const double* d, const half8* phi16, int64_t nI, int nb, const double* c,
+                         const double* mu, const double* b, double* r, float* r32, double* x_out, double* part_rr,
+                         double* part_bb, int npart, hipStream_t s);
 // mirror: pinned host copy of the scalars, written after the check (or null)
 void launch_cg_check(CgScal* sc, const double* part_rr, int nparts, hipStream_t s, CgScal* mirror = nullptr);
 template <typename T>

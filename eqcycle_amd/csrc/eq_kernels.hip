@@ -278,6 +278,13 @@ struct LatePrepare : std::false_type {};
 template <class Op>
 struct LatePrepare<Op, std::void_t<decltype(Op::kLatePrepare)>> : std::bool_constant<Op::kLatePrepare> {};
 
+// Op::kWindowAdd (when declared true): after the window loads, op.window_add
+// adds a term of its own to each loaded value (before the values reach LDS).
+template <class Op, class = void>
+struct WindowAdd : std::false_type {};
+template <class Op>
+struct WindowAdd<Op, std::void_t<decltype(Op::kWindowAdd)>> : std::bool_constant<Op::kWindowAdd> {};
+
 template <typename T, class Op>
 __device__ __forceinline__ void tile_body(const Geom& g, const Op& op, int bx, int by, int ntx, int nty) {
   constexpr int NPT = TILE_NPT;
@@ -310,6 +317,7 @@ __device__ __forceinline__ void tile_body(const Geom& g, const Op& op, int bx, i
         mr[t] = op.mu[n];
       }
     }
+    if constexpr (WindowAdd<Op>::value) op.window_add(g, I0, J0, ur);
     if (late && !op.prepare(pre)) return;   // block-uniform
 #pragma unroll
     for (int t = 0; t < WIN_LOADS; t++) {
@@ -432,6 +440,97 @@ struct OpResidual {
     if (r32) r32[n] = (float)rv;
     a0 += (double)rv * (double)rv;
     a1 += (double)bv * (double)bv;
+  }
+};
+
+// Galerkin start and its residual in one launch: the window loads x = d +
+// Phi c off the fault row (the FP32 sum of expand_add16_kernel, in the same
+// order, so x is that kernel's bit for bit), then r = b - K_II x with the partials of
+// OpResidual; x goes to x_out at every owned node (the fault row as d).  Phi:
+// the FP16 basis of the interior (node n = NX + i) in groups of 8 vectors,
+// [ceil(nb / 8)][nI][8].  x_out must not alias d: neighbouring tiles still
+// load d on their windows.
+constexpr int kGalGroups = 8;   // groups of 8 basis vectors: up to 64
+// kFull: nb = 64 (no basis index to test: the coefficients load in blocks and
+// the sums follow the loads as they arrive)
+template <bool kFull>
+struct OpGalResidual {
+  using Pre = PreNone;
+  static constexpr int kParts = 2;
+  static constexpr bool kWindowAdd = true;
+  __device__ void block_end(double, double) const {}
+  int npart;
+  __device__ void zero_tail(int nb_) const {
+    for (int k = nb_ + threadIdx.x; k < npart; k += blockDim.x) {
+      part0[k] = 0.0;
+      part1[k] = 0.0;
+    }
+  }
+  const double* d;
+  const half8* phi;
+  int64_t nI;
+  int nb;
+  const double* c;
+  const double* mu;
+  const double* b;
+  double* r;
+  float* r32;
+  double* x_out;
+  double* part0;
+  double* part1;
+  __device__ bool prepare(Pre&) const { return true; }
+  struct Ext { double b, d0; };
+  __device__ void fetch(const Pre&, int64_t n, int, int J, Ext& e) const {
+    e.b = J == 0 ? 0.0 : b[n];
+    e.d0 = J == 0 ? d[n] : 0.0;
+  }
+  __device__ double load(const Pre&, int64_t n, int, int J) const { return J == 0 ? 0.0 : d[n]; }
+  template <int NL>
+  __device__ void window_add(const Geom& g, int I0, int J0, double (&ur)[NL]) const {
+    int64_t idx[NL];
+    bool ok[NL];
+    float acc[NL];
+#pragma unroll
+    for (int t = 0; t < NL; t++) {
+      const int k = threadIdx.x + t * TBLOCK;
+      const int rr = k / WIN_I, cc = k - rr * WIN_I;
+      const int I = I0 - 4 + cc, J = J0 - 4 + rr;
+      ok[t] = k < WIN_J * WIN_I && I >= 0 && I < g.NX && J >= 1 && J < g.NY;
+      idx[t] = ok[t] ? (int64_t)(J - 1) * g.NX + I : 0;   // off the interior: node 0's row, sum discarded
+      acc[t] = 0.0f;
+    }
+    // every group's loads first (up to 8 x NL 16-byte loads in flight), then
+    // the sums in basis order; no branch per node, so the NL sums interleave
+    constexpr int NG = kGalGroups;
+    const int ng = kFull ? NG : (nb + 7) / 8;
+    half8 v[NG][NL];
+#pragma unroll
+    for (int gi = 0; gi < NG; gi++)
+#pragma unroll
+      for (int t = 0; t < NL; t++) v[gi][t] = gi < ng ? phi[(int64_t)gi * nI + idx[t]] : half8{};
+#pragma unroll
+    for (int gi = 0; gi < NG; gi++) {
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        if (kFull || 8 * gi + j < nb) {
+          const float ck = (float)c[8 * gi + j];
+#pragma unroll
+          for (int t = 0; t < NL; t++) acc[t] = __builtin_fmaf((float)v[gi][t][j], ck, acc[t]);
+        }
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < NL; t++) ur[t] += ok[t] ? (double)acc[t] : 0.0;
+  }
+  __device__ void epilogue(const Pre&, int64_t n, int, int J, double ku, double xn, const Ext& e, double& a0,
+                           double& a1) const {
+    x_out[n] = J == 0 ? e.d0 : xn;
+    const double bv = e.b;
+    const double rv = J == 0 ? 0.0 : bv - ku;
+    r[n] = rv;
+    if (r32) r32[n] = (float)rv;
+    a0 += rv * rv;
+    a1 += bv * bv;
   }
 };
 
@@ -1754,6 +1853,18 @@ void launch_cg_residual(const Geom& g, const T* x, const T* mu, const T* b, T* r
                         int npart, hipStream_t s, float* r32) {
   OpResidual<T> op{npart, x, mu, b, r, part_rr, part_bb, r32};
   hipLaunchKernelGGL((tile_kernel<T, OpResidual<T>>), tile_grid(g), dim3(TBLOCK), 0, s, g, op);
+}
+
+void launch_gal_residual(const Geom& g, const double* d, const half8* phi16, int64_t nI, int nb, const double* c,
+                         const double* mu, const double* b, double* r, float* r32, double* x_out, double* part_rr,
+                         double* part_bb, int npart, hipStream_t s) {
+  if (nb == 8 * kGalGroups) {
+    OpGalResidual<true> op{npart, d, phi16, nI, nb, c, mu, b, r, r32, x_out, part_rr, part_bb};
+    hipLaunchKernelGGL((tile_kernel<double, OpGalResidual<true>>), tile_grid(g), dim3(TBLOCK), 0, s, g, op);
+  } else {
+    OpGalResidual<false> op{npart, d, phi16, nI, nb, c, mu, b, r, r32, x_out, part_rr, part_bb};
+    hipLaunchKernelGGL((tile_kernel<double, OpGalResidual<false>>), tile_grid(g), dim3(TBLOCK), 0, s, g, op);
+  }
 }
 
 void launch_cg_init_scal(CgScal* sc, const double* part_rr, const double* part_bb, int nparts, double tol,

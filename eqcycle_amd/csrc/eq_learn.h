@@ -10,6 +10,11 @@
 
 namespace eq {
 
+// 8 FP16 basis coefficients of one node (one 16-byte load)
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+// groups of 8 basis vectors in the FP16 copy of the basis
+__host__ __device__ inline int phi16_groups(int NB) { return (NB + 7) / 8; }
+
 struct NetDims {
   int L0 = 0;        // input length (fault nodes)
   int C = 32;        // channels of every conv layer
@@ -63,7 +68,8 @@ struct LearnState {
   double* gal_c = nullptr;      // [NB] projections / coefficients
   double* gal_Ginv = nullptr;   // [NB x NB] row-major inverse of Phi^T K_II Phi
   float* Phi_rm = nullptr;      // [nI x NB] row-major copy of Phi (projection)
-  _Float16* Phi16 = nullptr;    // [nI x NB] FP16 copy of Phi for the expansion, or null
+  _Float16* Phi16 = nullptr;    // FP16 copy of Phi for the expansion, in groups of 8 vectors:
+                                // [phi16_groups(NB)][nI][8] (zero past NB)
   double* gal_T = nullptr;      // [NB x gal_trows] Ginv Phi^T on the first gal_trows interior rows
   int64_t gal_trows = 0;
   int gal_nblk = 0;

@@ -732,9 +732,15 @@ __global__ __launch_bounds__(LB) void gal_coef_kernel(int64_t nrows, const doubl
   if (threadIdx.x == 0) c[k] = a;
 }
 
-__global__ void to_f16_kernel(int64_t n, const float* x, _Float16* y) {
+// FP16 basis in groups of 8 vectors: y[(g nI + i) 8 + j] = Phi[8 g + j, i]
+// (zero past NB), so that one 16-byte load holds 8 coefficients of a node
+__global__ void to_f16_blocked_kernel(int64_t nI, int NB, const float* Phi, _Float16* y) {
   const int64_t e = (int64_t)blockIdx.x * LB + threadIdx.x;
-  if (e < n) y[e] = (_Float16)x[e];
+  if (e >= nI * 8 * phi16_groups(NB)) return;
+  const int64_t g = e / (nI * 8), rem = e - g * nI * 8;
+  const int64_t i = rem >> 3;
+  const int k = (int)(8 * g + (rem & 7));
+  y[e] = k < NB ? (_Float16)Phi[(int64_t)k * nI + i] : (_Float16)0.0f;
 }
 
 // Phi_rm[i, k] = Phi[k, i]
@@ -747,8 +753,7 @@ __global__ void phi_rowmajor_kernel(int64_t nI, int NB, const float* Phi, float*
 }
 
 // x[off + i] += sum_k Phi[k, i] c[k]
-template <typename PT>
-__global__ __launch_bounds__(LB) void expand_add_kernel(int64_t nI, int64_t off, int NB, const PT* Phi,
+__global__ __launch_bounds__(LB) void expand_add_kernel(int64_t nI, int64_t off, int NB, const float* Phi,
                                                         const double* c, double* x) {
   __shared__ double cs[NBMAX];
   if (threadIdx.x < NB) cs[threadIdx.x] = c[threadIdx.x];
@@ -759,6 +764,29 @@ __global__ __launch_bounds__(LB) void expand_add_kernel(int64_t nI, int64_t off,
 #pragma unroll 16
   for (int k = 0; k < NB; k++) acc += (double)Phi[(int64_t)k * nI + i] * cs[k];
   x[off + i] += acc;
+}
+
+// The same from the blocked FP16 basis: one 16-byte load per 8 vectors, the
+// sum in FP32 fused multiply-adds (Phi's FP16 rounding, 5e-4 of the
+// increment, dwarfs the FP32 sum's), k = 0, 1, ... in order; the residual
+// kernel that expands on its own windows (launch_gal_residual) sums alike
+__global__ __launch_bounds__(LB) void expand_add16_kernel(int64_t nI, int64_t off, int NB, const half8* Phi8,
+                                                          const double* c, double* x) {
+  __shared__ float cs[NBMAX];
+  if (threadIdx.x < NB) cs[threadIdx.x] = (float)c[threadIdx.x];
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * LB + threadIdx.x;
+  if (i >= nI) return;
+  float acc = 0.0f;
+  const int ng = phi16_groups(NB);
+#pragma unroll 2
+  for (int gi = 0; gi < ng; gi++) {
+    const half8 v = Phi8[(int64_t)gi * nI + i];
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+      if (8 * gi + j < NB) acc = __builtin_fmaf((float)v[j], cs[8 * gi + j], acc);
+  }
+  x[off + i] += (double)acc;
 }
 
 __global__ void basis_col_kernel(int64_t nI, int64_t off, const float* col, double* out) {
@@ -826,7 +854,7 @@ int learn_init(LearnState* ls, int64_t nI, int64_t nflt, int64_t offset, int cap
   ls->gal_part = dalloc<double>((int64_t)ls->gal_nblk * NB);
   ls->gal_c = dalloc<double>(2 * NBMAX);   // [0, NBMAX): coefficients, [NBMAX, 2 NBMAX): projections
   ls->Phi_rm = dalloc<float>(nI * (int64_t)NB);
-  ls->Phi16 = dalloc<_Float16>(nI * (int64_t)NB);
+  ls->Phi16 = dalloc<_Float16>(nI * (int64_t)phi16_groups(NB) * 8);
   ls->gal_Ginv = dalloc<double>((int64_t)NB * NB);
   if (!ls->gal_part || !ls->gal_c || !ls->gal_Ginv || !ls->Phi_rm || !ls->Phi16 || !ls->norms || !ls->S || !ls->F || !ls->Phi || !ls->X || !ls->Y || !ls->w32 || !ls->w16 || !ls->grad || !ls->adam_m ||
       !ls->adam_v || !ls->inf_act || !ls->coef) {
@@ -966,8 +994,8 @@ int learn_set_basis(LearnState* ls, const double* V, const double* sigma, const 
   }
   hipLaunchKernelGGL(phi_rowmajor_kernel, dim3(gblk(ls->nI * NB)), dim3(LB), 0, s, ls->nI, NB, ls->Phi, ls->Phi_rm);
   // the FP16 copy the Galerkin expansion may read (eq_solver_opts.basis_fp16)
-  hipLaunchKernelGGL(to_f16_kernel, dim3(gblk(ls->nI * NB)), dim3(LB), 0, s, ls->nI * (int64_t)NB, ls->Phi,
-                     ls->Phi16);
+  hipLaunchKernelGGL(to_f16_blocked_kernel, dim3(gblk(ls->nI * 8 * phi16_groups(NB))), dim3(LB), 0, s, ls->nI, NB,
+                     ls->Phi, ls->Phi16);
   // dataset: X_j = dF_j / |dF_j|, Y_j = Phi^T s_j / |dF_j| = Winv^T (sigma_k V_jk) / |dF_j|
   std::vector<float> F((int64_t)m * L0), X((int64_t)m * L0), Y((int64_t)m * NB);
   LCHK(hipMemcpyAsync(F.data(), ls->F, sizeof(float) * F.size(), hipMemcpyDeviceToHost, s));
@@ -1279,10 +1307,10 @@ void learn_project_rows(LearnState* ls, const double* v, int64_t nrows, hipStrea
 
 void learn_expand_add(LearnState* ls, double* x, hipStream_t s, bool fp16) {
   if (fp16 && ls->Phi16)
-    hipLaunchKernelGGL(expand_add_kernel<_Float16>, dim3(gblk(ls->nI)), dim3(LB), 0, s, ls->nI, ls->offset, ls->d.NB,
-                       ls->Phi16, ls->gal_c, x);
+    hipLaunchKernelGGL(expand_add16_kernel, dim3(gblk(ls->nI)), dim3(LB), 0, s, ls->nI, ls->offset, ls->d.NB,
+                       (const half8*)ls->Phi16, ls->gal_c, x);
   else
-    hipLaunchKernelGGL(expand_add_kernel<float>, dim3(gblk(ls->nI)), dim3(LB), 0, s, ls->nI, ls->offset, ls->d.NB,
+    hipLaunchKernelGGL(expand_add_kernel, dim3(gblk(ls->nI)), dim3(LB), 0, s, ls->nI, ls->offset, ls->d.NB,
                        ls->Phi, ls->gal_c, x);
 }
 

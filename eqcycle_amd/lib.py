@@ -74,7 +74,8 @@ class SolverOpts(C.Structure):
     _fields_ = [("tol", C.c_double), ("maxiter", C.c_int64), ("pc", C.c_int), ("prec", C.c_int),
                 ("guess", C.c_int), ("record_snapshots", C.c_int), ("graphs", C.c_int),
                 ("cheb_single", C.c_int), ("sync_pass2", C.c_int),
-                ("galerkin_full", C.c_int), ("basis_fp16", C.c_int), ("extrapolate", C.c_int)]
+                ("galerkin_full", C.c_int), ("basis_fp16", C.c_int), ("extrapolate", C.c_int),
+                ("galerkin_split", C.c_int)]
 
 
 class CsrDesc(C.Structure):

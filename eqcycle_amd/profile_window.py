@@ -32,6 +32,7 @@ def main():
                     help="fine-level Chebyshev degree of the measured pass (default as bench.py: 5 plain, 7 else)")
     ap.add_argument("--degree-plain", type=int, default=5, help="degree of the checkpoint run")
     ap.add_argument("--graphs", type=int, default=1)
+    ap.add_argument("--extrapolate", type=int, default=1, help="extrapolated Galerkin start, as bench.py")
     a = ap.parse_args()
     import bench
     from eqcycle_amd.api import EqContext
@@ -48,7 +49,8 @@ def main():
         ctx.amg_setup(hierarchy="pmg", degree=degree)
     guess = {"plain": "previous", "network": "learned", "linear": "linear", "galerkin": "galerkin"}[a.variant]
     prec = "fp64" if a.variant == "plain" else cfg.precision
-    ctx.set_solver(tol=tol, pc="amg", prec=prec, guess=guess, graphs=bool(a.graphs), basis_fp16=guess == "galerkin")
+    ctx.set_solver(tol=tol, pc="amg", prec=prec, guess=guess, graphs=bool(a.graphs), basis_fp16=guess == "galerkin",
+                   extrapolate=guess == "galerkin" and bool(a.extrapolate))
     t_end = ck["t"] + bench.SEGMENT_YEARS * bench.YR
     bench.one_pass(ctx, ck, t_end)
     print("[profile_window] checkpoint and warm-up pass done; the measured pass follows", flush=True)

@@ -238,6 +238,10 @@ typedef struct {
                               same multiple of the last step's (with galerkin_full, the full
                               residual of that start); 0 (default): from d.  Off the Galerkin
                               start it has no effect */
+  int galerkin_split;      /* 1: the Galerkin start's expansion x += Phi c and the start residual
+                              b - K_II x in two launches; 0 (default): in one (the same x and r bit
+                              for bit) wherever the change-of-b start applies with the FP16 basis
+                              and an FP64 or F32PC solve */
 } eq_solver_opts;
 int eq_set_solver(eq_ctx *ctx, const eq_solver_opts *opts);
 

@@ -25,6 +25,10 @@ def trained():
     print(f"[gpu] recorded {info['nsnap']} snapshots, {info['nparams']} parameters", flush=True)
     fit = learned.fit_basis(ctx, tol=1e-10)
     print(f"[gpu] POD energy captured {fit['energy_captured']:.6f}", flush=True)
+    # a quasi-static state for the tests that compare starts (the tests before
+    # them may leave the context inside an event, where no solve runs)
+    fit["qs_state"] = ctx.state_get()
+    assert fit["qs_state"]["isolver"] == 1
     return P, ctx, fit
 
 
@@ -207,8 +211,8 @@ def test_extrapolated_galerkin_start(trained, prec, tol):
     counts within 1 % (and one iteration) and displacements to 10 tol.  A wrong
     sign or factor in either the displacement or the b extrapolation breaks
     the first or the second comparison."""
-    P, ctx, _ = trained
-    s0 = ctx.state_get()
+    P, ctx, fit = trained
+    s0 = fit["qs_state"]
 
     def run(extrapolate, shortcut=True):
         ctx.state_set(d=s0["d"], v=s0["v"], a=s0["a"], psi=s0["psi"], tau=s0["tau"], t=s0["t"], dt=s0["dt"],
@@ -226,9 +230,40 @@ def test_extrapolated_galerkin_start(trained, prec, tol):
     # the first two steps cannot extrapolate (no consecutive solves yet)
     assert np.array_equal(it_x[:2], it_0[:2])
     assert it_x[:, 0].sum() < it_0[:, 0].sum()
+    assert it_0[:, 0].min() > 0   # every step solved
     assert np.max(np.abs(d_x - d_0)) <= 20 * tol * np.max(np.abs(d_0))
     assert abs(it_x.sum() - it_f.sum()) <= 0.01 * it_f.sum() + 1
     assert np.max(np.abs(d_x - d_f)) <= 10 * tol * np.max(np.abs(d_f))
+
+
+@pytest.mark.parametrize("prec", ["fp64", "f32pc"])
+def test_fused_galerkin_start_is_bit_identical(trained, prec):
+    """The change-of-b Galerkin start from the FP16 basis with the start
+    residual in the same launch (eq_solver_opts.galerkin_split = 0: the window
+    of the residual's tiles loads d + Phi c, summed in the order of the
+    separate expansion, and x lands in the second displacement buffer) against
+    the expansion and the residual in two launches: 40 steps from the same
+    state, with the extrapolated start, give the same iteration counts and the
+    same state bit for bit."""
+    P, ctx, fit = trained
+    s0 = fit["qs_state"]
+
+    def run(fused):
+        ctx.state_set(d=s0["d"], v=s0["v"], a=s0["a"], psi=s0["psi"], tau=s0["tau"], t=s0["t"], dt=s0["dt"],
+                      isolver=s0["isolver"])
+        ctx.set_solver(tol=1e-8, pc="amg", prec=prec, guess="galerkin", basis_fp16=True, extrapolate=True,
+                       galerkin_fused=fused)
+        its = np.array([ctx.step().pcg_iters for _ in range(40)])
+        return its, ctx.state_get()
+
+    it_s, st_s = run(False)
+    it_f, st_f = run(True)
+    print(f"[gpu] {prec}: iterations split {it_s.sum()}, fused {it_f.sum()}", flush=True)
+    assert it_s[:, 0].min() > 0 and it_s[:, 1].min() > 0   # every step solved both passes
+    assert np.array_equal(it_s, it_f)
+    for k in ("d", "v", "psi", "tau"):
+        assert np.array_equal(st_s[k], st_f[k]), k
+    assert st_s["t"] == st_f["t"] and st_s["dt"] == st_f["dt"]
 
 
 def _bf16(x):
@@ -282,7 +317,7 @@ def test_linear_predictor_against_network_and_previous_guess(trained):
     want = n * (A @ (dF / n).astype(np.float32).astype(np.float64) + b)
     got = ctx.learn_predict_linear(dF)
     assert np.linalg.norm(got - want) <= 1e-5 * np.linalg.norm(want)
-    s0 = ctx.state_get()
+    s0 = fit["qs_state"]
     its, dfin = {}, {}
     for guess in ("previous", "learned", "linear"):
         ctx.state_set(d=s0["d"], v=s0["v"], a=s0["a"], psi=s0["psi"], tau=s0["tau"], t=s0["t"], dt=s0["dt"],
@@ -291,6 +326,7 @@ def test_linear_predictor_against_network_and_previous_guess(trained):
         its[guess] = sum(ctx.step().pcg_iters[0] for _ in range(40))
         dfin[guess] = ctx.state_get()["d"]
     print(f"[gpu] pass-1 iterations over 40 steps: {its}", flush=True)
+    assert its["previous"] > 0
     assert its["linear"] <= its["previous"] and its["learned"] <= its["previous"]
     dref = np.max(np.abs(dfin["previous"]))
     assert np.max(np.abs(dfin["linear"] - dfin["previous"])) <= 1e-7 * dref
