@@ -2109,8 +2109,12 @@ static int qs_step(eq_ctx* ctx, eq_step_info* info, bool r_valid) {
                              ? dt / ctx->qs_prev_dt
                              : 0.0;
     if (pass == 0) bold_saved = galerkin;
-    launch_qs_begin(g, ctx->fp, ctx->fs, ctx->mu, ctx->d, ctx->dPre, ctx->v, ctx->bq, dt, pass == 0, s,
-                    upd ? ctx->gal_db : nullptr, theta, pass == 0 && galerkin ? ctx->gal_bold : nullptr);
+    if (pass == 0)
+      launch_qs_begin(g, ctx->fp, ctx->fs, ctx->mu, ctx->d, ctx->dPre, ctx->v, ctx->bq, dt, 1, s,
+                      upd ? ctx->gal_db : nullptr, theta, galerkin ? ctx->gal_bold : nullptr);
+    else   // the first pass's fault update, then this pass's begin
+      launch_qs_mid(g, ctx->fp, ctx->fs, ctx->mu, ctx->d, ctx->dPre, ctx->v, ctx->bq, dt,
+                    upd ? ctx->gal_db : nullptr, s);
     const bool learned = pass == 0 && ctx->opts.guess == EQ_GUESS_LEARNED && ctx->learn_on;
     const bool linear = pass == 0 && ctx->opts.guess == EQ_GUESS_LINEAR && ctx->learn.lin_ready;
     if (learned || linear) learned_correction(ctx, ctx->d, ctx->bq, linear);
@@ -2146,7 +2150,7 @@ static int qs_step(eq_ctx* ctx, eq_step_info* info, bool r_valid) {
     else if (rc) return rc;
     info->pcg_iters[pass] = it;
     if (pass == 0 && ctx->learn_record_on) learn_record(&ctx->learn, ctx->d, ctx->dPre, s);
-    if (pass == 0) launch_qs_fault(g, ctx->fp, ctx->fs, ctx->mu, ctx->d, ctx->v, dt, s);
+    // (the first pass's fault update opens the second pass: launch_qs_mid)
   }
   // second pass's fault update, the step finish and the step tail (with a pass
   // sent ahead, also its convergence test and the save of what the update reads)

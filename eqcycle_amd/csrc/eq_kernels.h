@@ -265,11 +265,15 @@ void launch_newmark_predict(int64_t n, const double* d, const double* v, const d
 void launch_qs_begin(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, double* d, double* dPre,
                      const double* v, double* b, double dt, int first_pass, hipStream_t s,
                      double* db = nullptr, double theta = 0.0, double* bold = nullptr);
+// launch_qs_fault (no save) then the fault-row part of launch_qs_begin for the
+// second pass (first_pass = 0, theta = 0, no bold), in one single-block launch
+void launch_qs_mid(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, double* d, const double* dPre,
+                   double* v, double* b, double dt, double* db, hipStream_t s);
 // save: psi, Vf and the fault row of d before the update (3 nflt), or null
 void launch_qs_fault(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, const double* d,
                      double* v, double dt, hipStream_t s, double* save = nullptr);
-// The end of a quasi-static step in two launches: the interior of the step
-// finish (v = (d - dPre) / dt off the fault, a = 0), then one block with, when
+// The end of a quasi-static step in one launch: the interior of the step
+// finish (v = (d - dPre) / dt off the fault, a = 0) beside one block with, when
 // sc is set, the second solve's convergence test (scalars to sc_mirror), the
 // second pass's fault update (save: psi, Vf and d of the fault row first), the
 // fault row of the finish and the step tail (status, flags, cap): what

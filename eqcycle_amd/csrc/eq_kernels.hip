@@ -1385,23 +1385,11 @@ __global__ __launch_bounds__(BLOCK) void dense_kernel(int64_t n, const T* inv, c
 // F = dPre + v dt on the fault, d[fault] = F, b rows 1..4 = -(K F) (only the
 // bottom element row couples to the fault).  db (optional, [4 NX]) receives
 // b_new - b_old on rows 1..4 (row J at db[(J - 1) NX + I]).
-__global__ __launch_bounds__(BLOCK) void qs_begin_kernel(Geom g, FaultDev fp, FaultState fs, const double* mu,
-                                                         double* d, double* dPre, const double* v, double* b,
-                                                         double dt, int first_pass, double* db, double theta,
-                                                         double* bold) {
-  const int64_t N = (int64_t)g.NX * g.NY;
-  const int64_t n = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-  if (first_pass) {
-    if (n >= N) return;
-    const double dn = d[n];
-    // extrapolated start off the fault: d + theta (d - dPre), the last step's increment
-    if (theta != 0.0 && n >= g.NX) d[n] = dn + theta * (dn - dPre[n]);
-    dPre[n] = dn;
-  } else if (n >= g.NX) {
-    return;
-  }
-  if (n >= g.NX) return;
-  const int I = (int)n;
+// The fault-row part at fault node I (first pass: after dPre[I] = d[I]).
+__device__ __forceinline__ void qs_begin_fault_node(const Geom& g, const FaultDev& fp, const FaultState& fs,
+                                                    const double* mu, double* d, const double* dPre,
+                                                    const double* v, double* b, double dt, int first_pass,
+                                                    double* db, double theta, double* bold, int I) {
   const double vf = v[I];
   if (first_pass) {
     const double V0 = 2 * vf + fp.Vpl;
@@ -1424,6 +1412,23 @@ __global__ __launch_bounds__(BLOCK) void qs_begin_kernel(Geom g, FaultDev fp, Fa
     if (bold) bold[r4] = b[m];
     b[m] = bn;
   }
+}
+
+__global__ __launch_bounds__(BLOCK) void qs_begin_kernel(Geom g, FaultDev fp, FaultState fs, const double* mu,
+                                                         double* d, double* dPre, const double* v, double* b,
+                                                         double dt, int first_pass, double* db, double theta,
+                                                         double* bold) {
+  const int64_t N = (int64_t)g.NX * g.NY;
+  const int64_t n = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (first_pass) {
+    if (n >= N) return;
+    const double dn = d[n];
+    // extrapolated start off the fault: d + theta (d - dPre), the last step's increment
+    if (theta != 0.0 && n >= g.NX) d[n] = dn + theta * (dn - dPre[n]);
+    dPre[n] = dn;
+  }
+  if (n >= g.NX) return;
+  qs_begin_fault_node(g, fp, fs, mu, d, dPre, v, b, dt, first_pass, db, theta, bold, (int)n);
 }
 
 // tau1 = -(K d)[fault]/FltB (0 on creeping nodes), slrFunc!, Vf and fault v update
@@ -1483,17 +1488,6 @@ __device__ __forceinline__ void qs_finish_fault_node(const FaultDev& fp, const F
   v[j] = vv;
 }
 
-// The interior part of the step finish: v = (d - dPre) / dt off the fault, a = 0
-// everywhere (the fault row's v is qs_end_kernel's).
-__global__ __launch_bounds__(BLOCK) void qs_finish_interior_kernel(Geom g, const double* d, const double* dPre,
-                                                                   double* v, double* a, double dt) {
-  const int64_t N = (int64_t)g.NX * g.NY;
-  const int64_t n = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-  if (n >= N) return;
-  if (n >= g.NX) v[n] = (d[n] - dPre[n]) / dt;
-  a[n] = 0.0;
-}
-
 // Vfmax (src/main.jl:348), regime switch (src/main.jl:439-443), dtevol! (src/dtevol.jl:5-44)
 __device__ double dtevol_dev(const double* XiLf, const double* v, bool slip_rate_from_v, double Vpl, int j0, int n,
                              double dt, double dtmin, int isolver) {
@@ -1522,10 +1516,13 @@ __device__ double dtevol_dev(const double* XiLf, const double* v, bool slip_rate
 }
 
 // The step tail over the fault with a block of NT threads.
+// qs_dPre (the quasi-static end, which runs beside the interior finish): the
+// receivers off the fault take v = (d - dPre) / dt and a = 0, the values the
+// finish writes there, and on the fault the block's own v and a = 0.
 template <int NT>
 __device__ void step_tail_body(const FaultDev& fp, const double* d, const double* v, const double* a,
                                const double* tau, double dt, double dtmin, int isolver, StepStatus* st, int* flags,
-                               const StepCapture& cap) {
+                               const StepCapture& cap, const double* qs_dPre = nullptr, int64_t NX = 0) {
   // one pass over the fault: the maximum of v (first index on ties) and the
   // quasi-static dtevol! candidate min XiLf/|Vf| over the cells that qualify,
   // reduced across each wave by shuffles and across the four waves in LDS
@@ -1585,7 +1582,9 @@ __device__ void step_tail_body(const FaultDev& fp, const double* d, const double
   if (cap.seis) {
     for (int q = threadIdx.x; q < cap.nseis; q += NT) {
       const int64_t n = cap.seis_idx[q];
-      const double dn = d[n], vn = v[n], an = a[n];
+      const double dn = d[n];
+      const double vn = (qs_dPre && n >= NX) ? (dn - qs_dPre[n]) / dt : v[n];
+      const double an = qs_dPre ? 0.0 : a[n];
       cap.seis[3 * q] = dn;
       cap.seis[3 * q + 1] = vn;
       cap.seis[3 * q + 2] = an;
@@ -1618,17 +1617,27 @@ __global__ __launch_bounds__(BLOCK) void step_tail_kernel(FaultDev fp, const dou
   step_tail_body<BLOCK>(fp, d, v, a, tau, dt, dtmin, isolver, st, flags, cap);
 }
 
-// The end of a quasi-static step in one block (QE_BLOCK threads, fault nodes strided):
-// with sc, the second solve's convergence test (cg_check_kernel's, mirror included);
-// the second pass's fault update (with save: psi, Vf and d of each node first); the
-// fault row of the step finish; the step tail, which reads every fault node.  The
-// interior part of the finish (qs_finish_interior_kernel) runs before it.
+// The end of a quasi-static step.  Block 0 (QE_BLOCK threads, fault nodes
+// strided): with sc, the second solve's convergence test (cg_check_kernel's,
+// mirror included); the second pass's fault update (with save: psi, Vf and d of
+// each node first); the fault row of the step finish; the step tail, which reads
+// every fault node.  Blocks 1.. : the interior part of the finish (v = (d -
+// dPre) / dt off the fault, a = 0), which block 0 neither reads nor writes (its
+// receiver capture forms those values itself).
 constexpr int QE_BLOCK = 1024;
 __global__ __launch_bounds__(QE_BLOCK) void qs_end_kernel(Geom g, FaultDev fp, FaultState fs, const double* mu,
-                                                          double* d, double* v, const double* a, double dt,
-                                                          double dtmin, double* save, CgScal* sc,
+                                                          double* d, const double* dPre, double* v, double* a,
+                                                          double dt, double dtmin, double* save, CgScal* sc,
                                                           const double* part_rr, int nparts, CgScal* sc_mirror,
                                                           StepStatus* st, int* flags, StepCapture cap) {
+  if (blockIdx.x > 0) {
+    const int64_t N = (int64_t)g.NX * g.NY;
+    const int64_t n = (int64_t)(blockIdx.x - 1) * QE_BLOCK + threadIdx.x;
+    if (n >= N) return;
+    if (n >= g.NX) v[n] = (d[n] - dPre[n]) / dt;
+    a[n] = 0.0;
+    return;
+  }
   if (sc) {
     if (!sc->done) {   // block-uniform
       const double rr = reduce_parts(part_rr, nparts);
@@ -1644,7 +1653,20 @@ __global__ __launch_bounds__(QE_BLOCK) void qs_end_kernel(Geom g, FaultDev fp, F
   __syncthreads();   // every tau1 read d's fault row before creeping nodes reset it
   for (int j = threadIdx.x; j < fp.nflt; j += QE_BLOCK) qs_finish_fault_node(fp, fs, d, v, j);
   __syncthreads();
-  step_tail_body<QE_BLOCK>(fp, d, v, a, fs.tau, dt, dtmin, 1, st, flags, cap);
+  step_tail_body<QE_BLOCK>(fp, d, v, a, fs.tau, dt, dtmin, 1, st, flags, cap, dPre, g.NX);
+}
+
+// Between the passes of a quasi-static step, in one block: the first pass's
+// fault update (qs_fault_node at every fault node), then, once every node's
+// K d has read the fault row, the second pass's begin on the fault row
+// (qs_begin_fault_node: F = dPre + v dt, b rows 1..4, db).
+__global__ __launch_bounds__(QE_BLOCK) void qs_mid_kernel(Geom g, FaultDev fp, FaultState fs, const double* mu,
+                                                          double* d, const double* dPre, double* v, double* b,
+                                                          double dt, double* db) {
+  for (int j = threadIdx.x; j < fp.nflt; j += QE_BLOCK) qs_fault_node(g, fp, fs, mu, d, v, dt, nullptr, j);
+  __syncthreads();
+  for (int I = threadIdx.x; I < g.NX; I += QE_BLOCK)
+    qs_begin_fault_node(g, fp, fs, mu, d, dPre, v, b, dt, 0, db, 0.0, nullptr, I);
 }
 
 // --------------------------------------------------------------------------
@@ -2829,9 +2851,14 @@ void launch_qs_end(const Geom& g, const FaultDev& fp, FaultState fs, const doubl
                    double* v, double* a, double dt, double dtmin, double* save, CgScal* sc, const double* part_rr,
                    int nparts, CgScal* sc_mirror, StepStatus* st, int* flags, const StepCapture& cap, hipStream_t s) {
   const int64_t N = (int64_t)g.NX * g.NY;
-  hipLaunchKernelGGL(qs_finish_interior_kernel, dim3(nblk(N)), dim3(BLOCK), 0, s, g, d, dPre, v, a, dt);
-  hipLaunchKernelGGL(qs_end_kernel, dim3(1), dim3(QE_BLOCK), 0, s, g, fp, fs, mu, d, v, a, dt, dtmin, save, sc,
-                     part_rr, nparts, sc_mirror, st, flags, cap);
+  const int nb = (int)(1 + (N + QE_BLOCK - 1) / QE_BLOCK);
+  hipLaunchKernelGGL(qs_end_kernel, dim3(nb), dim3(QE_BLOCK), 0, s, g, fp, fs, mu, d, dPre, v, a, dt, dtmin, save,
+                     sc, part_rr, nparts, sc_mirror, st, flags, cap);
+}
+
+void launch_qs_mid(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, double* d, const double* dPre,
+                   double* v, double* b, double dt, double* db, hipStream_t s) {
+  hipLaunchKernelGGL(qs_mid_kernel, dim3(1), dim3(QE_BLOCK), 0, s, g, fp, fs, mu, d, dPre, v, b, dt, db);
 }
 
 void launch_step_tail(const FaultDev& fp, const double* d, const double* v, const double* a, const double* tau,
