@@ -1157,6 +1157,11 @@ static ChebSweep<T> sweep_params(eq_ctx* ctx, int post) {
   op.nelx = ctx->g.nelx;
   op.nely = ctx->g.nely;
   for (int p = 0; p < 5; p++) op.gx[p] = (T)ctx->g.x[p];
+  for (int p = 0; p < 5; p++) {
+    int e = 0;
+    for (int i = 0; i < 5; i++)
+      for (int kk = i; kk < 5; kk++) op.dd[p][e++] = (float)(ctx->g.D[p * 5 + i] * ctx->g.D[p * 5 + kk]);
+  }
   op.mu = mu_of<T>(ctx);
   op.dinv = dinv_of<T>(ctx);
   op.part_rr = ctx->part_rr;

@@ -185,6 +185,9 @@ struct ChebSweep {
   const T* coarse = nullptr;     // post: vertex correction added to src on the fly (or null)
   int nelx = 0, nely = 0;
   T gx[5] = {};                  // GLL nodes (bilinear prolongation weights)
+  // D[p][i] D[p][k] for i <= k (15 per p, row-major in (i, k)): a segment's operator
+  // D^T diag(wm) D is then sum_p wm_p dd[p] (FP32 sweeps)
+  float dd[5][15] = {};
   const T* mu = nullptr;
   const T* dinv = nullptr;
   T* xout = nullptr;             // pre: x; post: c (unless xout64)

@@ -2327,6 +2327,39 @@ __device__ __forceinline__ void seg_all_wm(const Geom& g, const T* u, const T* w
   }
 }
 
+// The same segment as one symmetric 5 x 5 operator, o = M u with
+// M = s D^T diag(wm) D (s: a row factor folded in), stored as its 15 upper
+// entries (row-major, i <= k): M = sum_p (s wm_p) dd[p], dd[p] the products
+// D[p][i] D[p][k].
+template <int ST>
+__device__ __forceinline__ void seg_matrix(const float (&dd)[5][15], const float* wm, bool on, float s,
+                                           float m[15]) {
+  float w[5];
+#pragma unroll
+  for (int p = 0; p < 5; p++) w[p] = on ? s * wm[p * ST] : 0.0f;
+#pragma unroll
+  for (int e = 0; e < 15; e++) {
+    float a = dd[0][e] * w[0];
+#pragma unroll
+    for (int p = 1; p < 5; p++) a += dd[p][e] * w[p];
+    m[e] = a;
+  }
+}
+template <typename T, int ST>
+__device__ __forceinline__ void seg_apply(const T* u, const T m[15], T o[5]) {
+  T v[5];
+#pragma unroll
+  for (int k = 0; k < 5; k++) v[k] = u[k * ST];
+  constexpr int row[5] = {0, 5, 9, 12, 14};   // index of m(i, i)
+#pragma unroll
+  for (int i = 0; i < 5; i++) {
+    T r = 0;
+#pragma unroll
+    for (int k = 0; k < 5; k++) r += m[i <= k ? row[i] + k - i : row[k] + i - k] * v[k];
+    o[i] = r;
+  }
+}
+
 // One product with K per step in two phases.  Phase A: every x-strip thread
 // (a window row, an element segment along x) and every y-strip thread (a
 // window column, a segment along y) evaluates its segment once: outputs 0..3
@@ -2468,23 +2501,43 @@ __global__ __launch_bounds__(SW_BLOCK) void cheb_sweep_kernel(Geom g, ChebSweep<
     }
   }
   __syncthreads();
+  // FP32: the thread's x and y segments (the same in every step) as 5 x 5
+  // operators in registers, the x one times cx: a segment then costs 25
+  // multiply-adds instead of 55 and reads no weights from LDS
+  constexpr bool kSegM = std::is_same<T, float>::value;
+  T mx[15], my[15];
+  if constexpr (kSegM) {
+    const bool xon = xrow && xseg, yon = ycol && yseg;
+    seg_matrix<1>(op.dd, swx + (xon ? xbr * SW_I + 4 * xk : 0), xon, cx, mx);
+    seg_matrix<SW_I>(op.dd, swy + (yon ? 4 * yk * SW_I + bcy : 0), yon, 1.0f, my);
+  }
   // phase A on direction su: x terms and carries to LDS, this thread's y outputs to oy
   T oy[5];
   auto phase_a = [&](const T* su) {
     if (xrow) {
       T o[5] = {0, 0, 0, 0, 0};
-      if (xseg) seg_all_wm<T, 1>(g, su + xbr * SW_I + 4 * xk, swx + xbr * SW_I + 4 * xk, o);
+      if constexpr (kSegM) {
+        if (xseg) seg_apply<T, 1>(su + xbr * SW_I + 4 * xk, mx, o);
+      } else {
+        if (xseg) seg_all_wm<T, 1>(g, su + xbr * SW_I + 4 * xk, swx + xbr * SW_I + 4 * xk, o);
+#pragma unroll
+        for (int i = 0; i < 5; i++) o[i] = cx * o[i];
+      }
 #pragma unroll
       for (int i = 0; i < 4; i++) {
         const int bc = 4 * xk + i;
-        if (bc >= SW_OFF && bc < SW_OFF + SW_WI) sx[xwr * SW_SXS + bc - SW_OFF] = cx * o[i];
+        if (bc >= SW_OFF && bc < SW_OFF + SW_WI) sx[xwr * SW_SXS + bc - SW_OFF] = o[i];
       }
-      cxr[xk * SW_WJ + xwr] = cx * o[4];
+      cxr[xk * SW_WJ + xwr] = o[4];
     }
 #pragma unroll
     for (int j = 0; j < 5; j++) oy[j] = T(0);
     if (ycol) {
-      if (yseg) seg_all_wm<T, SW_I>(g, su + 4 * yk * SW_I + bcy, swy + 4 * yk * SW_I + bcy, oy);
+      if constexpr (kSegM) {
+        if (yseg) seg_apply<T, SW_I>(su + 4 * yk * SW_I + bcy, my, oy);
+      } else {
+        if (yseg) seg_all_wm<T, SW_I>(g, su + 4 * yk * SW_I + bcy, swy + 4 * yk * SW_I + bcy, oy);
+      }
       cyr[yk * SW_WI + yc] = oy[4];
     }
   };
