@@ -148,6 +148,11 @@ class EqContext:
         self.hierarchy = hierarchy
         return self.amg_info()
 
+    def amg_coarse_mode(self, collapsed=True):
+        """Apply the coarse levels collapsed (one product down, one up) or level
+        by level (eq_amg_coarse_mode)."""
+        self._check(self.lib.eq_amg_coarse_mode(self.h, 1 if collapsed else 0))
+
     def amg_l0_info(self):
         """(smoother name, Chebyshev degree, lo, hi): the fine-level smoother and its
         target interval of D^-1 K_II."""

@@ -211,6 +211,59 @@ void condense_level(const HostLevel& Lin, double omega, HostCsr* M, HostCsr* U, 
   csr_add(1.0, L.P, -omega, AP, V);
 }
 
+void collapse_coarse(const std::vector<HostCsr>& M, const std::vector<HostCsr>& U, const std::vector<HostCsr>& V,
+                     const std::vector<int64_t>& col_off, int64_t ncols, HostCsr* S, HostCsr* C) {
+  const size_t k = M.size();   // condensed levels 1 .. k
+  // down: T_1 = M_1, T_j = M_j T_(j-1), stacked
+  std::vector<HostCsr> T(k);
+  T[0] = M[0];
+  for (size_t j = 1; j < k; j++) csr_matmul(M[j], T[j - 1], &T[j]);
+  S->n = 0;
+  S->m = M[0].m;
+  S->rowptr.assign(1, 0);
+  S->col.clear();
+  S->val.clear();
+  for (const HostCsr& B : T) {
+    for (int64_t i = 0; i < B.n; i++) {
+      for (int64_t q = B.rowptr[i]; q < B.rowptr[i + 1]; q++) {
+        S->col.push_back(B.col[q]);
+        S->val.push_back(B.val[q]);
+      }
+      S->rowptr.push_back((int64_t)S->col.size());
+    }
+    S->n += B.n;
+  }
+  // up: column blocks Q_j U_(j+1) (j = 0 .. k-1) and Q_k, Q_0 = I, Q_j = Q_(j-1) V_j
+  std::vector<HostCsr> blk(k + 1);
+  HostCsr Q;
+  for (size_t j = 0; j < k; j++) {
+    if (j == 0) {
+      blk[0] = U[0];
+      Q = V[0];
+    } else {
+      csr_matmul(Q, U[j], &blk[j]);
+      HostCsr Qn;
+      csr_matmul(Q, V[j], &Qn);
+      Q = std::move(Qn);
+    }
+  }
+  blk[k] = std::move(Q);
+  const int64_t n = U[0].n;
+  C->n = n;
+  C->m = ncols;
+  C->rowptr.assign(n + 1, 0);
+  C->col.clear();
+  C->val.clear();
+  for (int64_t i = 0; i < n; i++) {
+    for (size_t j = 0; j <= k; j++)
+      for (int64_t q = blk[j].rowptr[i]; q < blk[j].rowptr[i + 1]; q++) {
+        C->col.push_back((int32_t)(col_off[j] + blk[j].col[q]));
+        C->val.push_back(blk[j].val[q]);
+      }
+    C->rowptr[i + 1] = (int64_t)C->col.size();
+  }
+}
+
 // Strong connections of row i: |a_ij| >= theta max_{k!=i} |a_ik| (abs) or
 // -a_ij >= theta max_{k!=i} (-a_ik) (signed).  The diagonal is kept, with A's
 // values stored at the strong positions.

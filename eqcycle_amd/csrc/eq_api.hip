@@ -139,6 +139,14 @@ struct eq_ctx {
   double* coarse_inv = nullptr;
   float* coarse_inv32 = nullptr;      // FP32 copy for the FP32 V-cycle
   int64_t ncoarse = 0;
+  // the condensed coarse levels collapsed to one product down and one up
+  // (collapse_coarse), on [b_1 | ... | b_(L-1) | e_(L-1)] in cvec64 / cvec32,
+  // where the levels' b (and the last level's xp) then live
+  DevCsr coarse_down, coarse_up;
+  bool coarse_have = false;           // the collapsed operators exist (L >= 4, fill bounded)
+  bool coarse_collapsed = false;      // the V-cycle applies them (eq_amg_coarse_mode)
+  double* cvec64 = nullptr;
+  float* cvec32 = nullptr;
   std::vector<int64_t> amg_n, amg_nnz;
   int pmg = 0;                        // level-0 prolongation is the bilinear vertex interpolation
   int l0_cheb = 0;                    // level-0 smoother: 0 Jacobi, 1 Chebyshev
@@ -787,6 +795,8 @@ static int upload_hierarchy(eq_ctx* ctx, const HostHierarchy& H, int smoother, i
   ctx->l0_cheb = smoother == EQ_SMOOTH_CHEBYSHEV || smoother == EQ_SMOOTH_CHEBYSHEV_TILES;
   ctx->l0_sweep = smoother == EQ_SMOOTH_CHEBYSHEV_TILES;
   ctx->l0_degree = ctx->l0_cheb ? smooth_degree : 1;
+  ctx->coarse_have = ctx->coarse_collapsed = false;
+  std::vector<HostCsr> cM, cU, cV;   // the condensed sweeps of levels 1 .. L-2
   {
     // Chebyshev target interval [lambda_max / EQ_CHEB_LOWER, 1.1 lambda_max] of D^-1 K_II
     const double lmax = 1.1 * H.lev[0].lambda_max, lmin = H.lev[0].lambda_max / EQ_CHEB_LOWER;
@@ -822,6 +832,9 @@ static int upload_hierarchy(eq_ctx* ctx, const HostHierarchy& H, int smoother, i
       if (int rc = upload_csr(ctx, M, &D.M)) return rc;
       if (int rc = upload_csr(ctx, U, &D.U)) return rc;
       if (int rc = upload_csr(ctx, V, &D.V)) return rc;
+      cM.push_back(std::move(M));
+      cU.push_back(std::move(U));
+      cV.push_back(std::move(V));
     }
     if (l > 0) {
       if (int rc = alloc_level_vecs(ctx, D.v64, D.n)) return rc;
@@ -836,9 +849,56 @@ static int upload_hierarchy(eq_ctx* ctx, const HostHierarchy& H, int smoother, i
   ctx->coarse_inv32 = ctx->alloc<float>(H.ncoarse * H.ncoarse);
   NEED(ctx->coarse_inv32);
   launch_convert<double, float>(H.ncoarse * H.ncoarse, ctx->coarse_inv, ctx->coarse_inv32, ctx->s);
+  // two or more condensed levels: collapse them (one launch down, one up for 2 k
+  // launches), unless the fill costs more HBM time than the launches saved: at
+  // most 8 MB more to stream (8 bytes per entry: FP32 value and column) per
+  // launch saved, about 2 us at the rate these short products reach
+  const int nl = (int)H.lev.size();
+  if (nl >= 4) {
+    std::vector<int64_t> off(nl + 1, 0);   // off[l]: b_l (l = 1 .. nl-1); off[nl]: e_(nl-1)
+    for (int l = 1; l < nl; l++) off[l + 1] = off[l] + ctx->lev[l].n;
+    const int64_t total = off[nl] + ctx->lev[nl - 1].n;
+    std::vector<int64_t> col_off;
+    for (int l = 1; l <= nl - 2; l++) col_off.push_back(off[l]);
+    col_off.push_back(off[nl]);
+    HostCsr S, Cu;
+    collapse_coarse(cM, cU, cV, col_off, total, &S, &Cu);
+    int64_t per_level = 0;
+    for (size_t j = 0; j < cM.size(); j++) per_level += cM[j].nnz() + cU[j].nnz() + cV[j].nnz();
+    const int64_t extra_bytes = 8 * (S.nnz() + Cu.nnz() - per_level);
+    const int64_t saved = 2 * (int64_t)cM.size() - 2;
+    if (extra_bytes <= saved * (8ll << 20)) {
+      if (int rc = upload_csr(ctx, S, &ctx->coarse_down)) return rc;
+      if (int rc = upload_csr(ctx, Cu, &ctx->coarse_up)) return rc;
+      ctx->cvec64 = ctx->alloc<double>(total);
+      ctx->cvec32 = ctx->alloc<float>(total);
+      NEED(ctx->cvec64); NEED(ctx->cvec32);
+      for (int l = 1; l < nl; l++) {
+        ctx->lev[l].v64.b = ctx->cvec64 + off[l];
+        ctx->lev[l].v32.b = ctx->cvec32 + off[l];
+      }
+      ctx->lev[nl - 1].v64.xp = ctx->cvec64 + off[nl];
+      ctx->lev[nl - 1].v32.xp = ctx->cvec32 + off[nl];
+      ctx->coarse_have = ctx->coarse_collapsed = true;
+    }
+  }
   ctx->pmg = pcoarsen ? 1 : 0;
   ctx->have_amg = true;
   return sync_check(ctx);
+}
+
+// Which form of the coarse levels the V-cycle applies (no reference
+// counterpart; for tests that compare them): 1 collapsed, 0 level by level.
+extern "C" int eq_amg_coarse_mode(eq_ctx* ctx, int collapsed) {
+  if (!ctx) return EQ_EINVAL;
+  if (!ctx->have_amg) return fail(ctx, EQ_ENOTREADY, "eq_amg_setup first");
+  if (collapsed && !ctx->coarse_have) return fail(ctx, EQ_EINVAL, "no collapsed coarse levels in this hierarchy");
+  if ((collapsed != 0) != ctx->coarse_collapsed) {
+    for (auto& kv : ctx->cg_graphs) (void)hipGraphExecDestroy(kv.second);   // they hold the other form
+    ctx->cg_graphs.clear();
+    ctx->coarse_collapsed = collapsed != 0;
+  }
+  return EQ_OK;
 }
 
 extern "C" int eq_amg_setup(eq_ctx* ctx, double theta, int max_levels, int64_t max_coarse, int strength_abs,
@@ -1021,6 +1081,12 @@ template <>
 const double* coarse_inv_of<double>(eq_ctx* c) { return c->coarse_inv; }
 template <>
 const float* coarse_inv_of<float>(eq_ctx* c) { return c->coarse_inv32; }
+template <typename T>
+static T* cvec_of(eq_ctx* c);
+template <>
+double* cvec_of<double>(eq_ctx* c) { return c->cvec64; }
+template <>
+float* cvec_of<float>(eq_ctx* c) { return c->cvec32; }
 template <typename T>
 static const T* ones_of(eq_ctx* c);
 template <>
@@ -1241,18 +1307,27 @@ static void vcycle(eq_ctx* ctx, const T* r, T* c, double* c64 = nullptr, const d
     launch_csr<T>(CSR_SPMV, L0.R, w.res0 + g.NX, nullptr, nullptr, T(0), lvecs<T>(ctx->lev[1]).b, nullptr, ctx->sc,
                   s);
   }
-  // coarse levels: one kernel down (b_next = M b) and one up (xp = U b + V xp_next) per level
-  for (int l = 1; l <= L - 2; l++) {
-    DevLevel& D = ctx->lev[l];
-    launch_csr<T>(CSR_SPMV, D.M, lvecs<T>(D).b, nullptr, nullptr, T(0), lvecs<T>(ctx->lev[l + 1]).b, nullptr,
-                  ctx->sc, s);
-  }
   DevLevelVecs<T>& vc = lvecs<T>(ctx->lev[L - 1]);
-  launch_dense<T>(ctx->ncoarse, coarse_inv_of<T>(ctx), vc.b, vc.xp, ctx->sc, s);
-  for (int l = L - 2; l >= 1; l--) {
-    DevLevel& D = ctx->lev[l];
-    DevLevelVecs<T>& v = lvecs<T>(D);
-    launch_csr_up<T>(D.U, D.V, v.b, lvecs<T>(ctx->lev[l + 1]).xp, v.xp, ctx->sc, s);
+  if (ctx->coarse_collapsed) {
+    // collapsed: every coarse b from b_1 in one product, the dense level, e_1 in one product
+    T* cv = cvec_of<T>(ctx);
+    launch_csr<T>(CSR_SPMV, ctx->coarse_down, cv, nullptr, nullptr, T(0), cv + ctx->lev[1].n, nullptr, ctx->sc, s);
+    launch_dense<T>(ctx->ncoarse, coarse_inv_of<T>(ctx), vc.b, vc.xp, ctx->sc, s);
+    launch_csr<T>(CSR_SPMV, ctx->coarse_up, cv, nullptr, nullptr, T(0), lvecs<T>(ctx->lev[1]).xp, nullptr, ctx->sc,
+                  s);
+  } else {
+    // coarse levels: one kernel down (b_next = M b) and one up (xp = U b + V xp_next) per level
+    for (int l = 1; l <= L - 2; l++) {
+      DevLevel& D = ctx->lev[l];
+      launch_csr<T>(CSR_SPMV, D.M, lvecs<T>(D).b, nullptr, nullptr, T(0), lvecs<T>(ctx->lev[l + 1]).b, nullptr,
+                    ctx->sc, s);
+    }
+    launch_dense<T>(ctx->ncoarse, coarse_inv_of<T>(ctx), vc.b, vc.xp, ctx->sc, s);
+    for (int l = L - 2; l >= 1; l--) {
+      DevLevel& D = ctx->lev[l];
+      DevLevelVecs<T>& v = lvecs<T>(D);
+      launch_csr_up<T>(D.U, D.V, v.b, lvecs<T>(ctx->lev[l + 1]).xp, v.xp, ctx->sc, s);
+    }
   }
   const bool fused_prolong = ctx->l0_cheb && ctx->pmg;
   if (!fused_prolong)

@@ -75,6 +75,14 @@ void assemble_kii(const Geom& g, const double* mu_struct, HostCsr* K);
 // the same linear operators as smoothing, residual, restriction,
 // prolongation and smoothing applied one after the other.
 void condense_level(const HostLevel& L, double omega, HostCsr* M, HostCsr* U, HostCsr* V);
+// The condensed levels 1 .. k (M[j - 1], U[j - 1], V[j - 1]) above a dense level
+// k + 1, collapsed to one product down and one up:
+//   [b_2; ...; b_(k+1)] = S b_1,                 S = [M_1; M_2 M_1; ...; M_k ... M_1]
+//   e_1 = C [b_1, ..., b_k, e_(k+1)],            C = [U_1, V_1 U_2, ..., V_1 ... V_(k-1) U_k, V_1 ... V_k]
+// (e_(k+1) the dense level's correction); C's column block j starts at
+// col_off[j] of a vector of ncols entries (block k: e_(k+1)).
+void collapse_coarse(const std::vector<HostCsr>& M, const std::vector<HostCsr>& U, const std::vector<HostCsr>& V,
+                     const std::vector<int64_t>& col_off, int64_t ncols, HostCsr* S, HostCsr* C);
 // p-multigrid: level 0 = K_II with the Q4 -> Q1 vertex prolongation, then RS on
 // the Galerkin Q1 operator.
 void build_pmg_p0(const Geom& g, HostCsr* P);

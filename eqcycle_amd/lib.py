@@ -28,7 +28,7 @@ EQ_GUESS_PREVIOUS, EQ_GUESS_LEARNED, EQ_GUESS_GALERKIN, EQ_GUESS_LINEAR = 0, 1, 
 # every symbol the header declares (checked by the CPU test-suite)
 EXPORTS = [
     "eq_create", "eq_destroy", "eq_last_error", "eq_sizes", "eq_set_material", "eq_set_params",
-    "eq_ku_apply", "eq_kni_apply", "eq_element_computation", "eq_amg_setup", "eq_amg_info", "eq_amg_l0_info",
+    "eq_ku_apply", "eq_kni_apply", "eq_element_computation", "eq_amg_setup", "eq_amg_info", "eq_amg_l0_info", "eq_amg_coarse_mode",
     "eq_amg_ldiv", "eq_cg", "eq_pcg_jacobi", "eq_ids", "eq_ids2", "eq_nrsearch", "eq_fbc",
     "eq_slr", "eq_dtevol", "eq_set_solver", "eq_state_init", "eq_state_get", "eq_state_set",
     "eq_step", "eq_run", "eq_time_kernel", "eq_learn_init", "eq_learn_record", "eq_learn_info",
@@ -129,6 +129,7 @@ def load():
         "eq_amg_setup": ([vp, C.c_double, C.c_int, C.c_int64, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
         "eq_amg_info": ([vp, C.POINTER(C.c_int), i64p, i64p, C.c_int], C.c_int),
         "eq_amg_l0_info": ([vp, C.POINTER(C.c_int), C.POINTER(C.c_int), dp, dp], C.c_int),
+        "eq_amg_coarse_mode": ([vp, C.c_int], C.c_int),
         "eq_amg_ldiv": ([vp, dp, dp, C.c_int], C.c_int),
         "eq_cg": ([vp, dp, dp, C.c_double, C.c_int64, C.c_int, C.c_int, i64p, dp], C.c_int),
         "eq_pcg_jacobi": ([vp, dp, dp, C.c_double, C.c_int64, i64p], C.c_int),

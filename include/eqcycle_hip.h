@@ -158,6 +158,16 @@ int eq_amg_info(const eq_ctx *ctx, int *nlev, int64_t *n, int64_t *nnz, int cap)
  * for tests that restate the V-cycle): EQ_SMOOTH_*, the Chebyshev degree and
  * the target interval [lo, hi] of D^-1 K_II the polynomial is built on. */
 int eq_amg_l0_info(const eq_ctx *ctx, int *smoother, int *degree, double *lo, double *hi);
+/* How the V-cycle applies the coarse levels (no reference counterpart; for
+ * tests that compare the two forms).  With two or more condensed coarse levels
+ * (one Jacobi step before and after each coarse correction, as linear maps),
+ * the set-up also composes them into one sparse product down (every coarse
+ * right-hand side from the first) and one up (the first level's correction
+ * from every right-hand side and the dense level's correction), three launches
+ * instead of 2 L - 3, unless their fill would stream more than 8 MB more per
+ * launch saved.  collapsed = 1 (the default when built): those; 0: level by
+ * level.  EQ_EINVAL when the hierarchy has no collapsed form. */
+int eq_amg_coarse_mode(eq_ctx *ctx, int collapsed);
 /* x = Pl \ b for kni (ldiv!(x, p, b)); x, b in FltNI order. */
 int eq_amg_ldiv(eq_ctx *ctx, const double *b, double *x, int prec);
 
