@@ -113,6 +113,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--plain-only", action="store_true")
     ap.add_argument("--nbasis", type=int, default=64, help="POD basis vectors of the learned starts")
+    ap.add_argument("--max-coarse", type=int, default=None,
+                    help="largest coarsest level of the hierarchy (default: the library's, 2000 for pmg)")
     ap.add_argument("--galerkin-fused", type=int, default=1,
                     help="Galerkin expansion and start residual in one launch (1) or two (0)")
     ap.add_argument("--extrapolate", type=int, default=1,
@@ -352,7 +354,8 @@ def main():
     cfg = CONFIGS[a.config]
     tol = a.tol or cfg.pcg_tol
     P = set_parameters(cfg)
-    amg_plain = {"hierarchy": a.hierarchy, "degree": a.degree_plain, "smoother": a.smoother}
+    amg_plain = {"hierarchy": a.hierarchy, "degree": a.degree_plain, "smoother": a.smoother,
+                 "max_coarse": a.max_coarse}
     ctx = EqContext(P, amg=amg_plain)
     learned = cfg.learned and not a.plain_only
 
@@ -365,7 +368,7 @@ def main():
     if learned:
         extra["training"] = training
         # each variant with its best fine-level degree (set-up outside the timed passes)
-        ctx.amg_setup(hierarchy=a.hierarchy, degree=a.degree, smoother=a.smoother)
+        ctx.amg_setup(hierarchy=a.hierarchy, degree=a.degree, smoother=a.smoother, max_coarse=a.max_coarse)
         lrn, s_lrn, R_lrn = variant(ctx, ck, tol, cfg.precision, "learned", a.warmup, a.steps, a.graphs)
         print(f"network: {lrn['sim_years_per_hour']:.0f}", file=sys.stderr, flush=True)
         lin, s_lin, R_lin = variant(ctx, ck, tol, cfg.precision, "linear", a.warmup, a.steps, a.graphs)

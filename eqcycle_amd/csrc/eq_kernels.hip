@@ -64,6 +64,16 @@ __device__ double reduce_parts(const double* p, int n) {
   return block_sum(s);
 }
 
+// reduce_parts in two halves (the same sum, bit for bit), so that a kernel
+// can issue its own loads between the partial loads and the block sum: the
+// barriers of block_sum wait for LDS only, so those loads stay in flight.
+__device__ __forceinline__ double parts_load(const double* p, int n) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s += p[i];
+  return s;
+}
+__device__ __forceinline__ double parts_total(double s) { return block_sum(s); }
+
 // --------------------------------------------------------------------------
 // geometry helpers
 // --------------------------------------------------------------------------
@@ -284,6 +294,12 @@ template <class Op, class = void>
 struct WindowAdd : std::false_type {};
 template <class Op>
 struct WindowAdd<Op, std::void_t<decltype(Op::kWindowAdd)>> : std::bool_constant<Op::kWindowAdd> {};
+// Op::kWindowPost (when declared true): after a late op.prepare, op.window_post
+// (pre, ...) may still change the loaded window values (before LDS).
+template <class Op, class = void>
+struct WindowPost : std::false_type {};
+template <class Op>
+struct WindowPost<Op, std::void_t<decltype(Op::kWindowPost)>> : std::bool_constant<Op::kWindowPost> {};
 
 template <typename T, class Op>
 __device__ __forceinline__ void tile_body(const Geom& g, const Op& op, int bx, int by, int ntx, int nty) {
@@ -319,6 +335,7 @@ __device__ __forceinline__ void tile_body(const Geom& g, const Op& op, int bx, i
     }
     if constexpr (WindowAdd<Op>::value) op.window_add(g, I0, J0, ur);
     if (late && !op.prepare(pre)) return;   // block-uniform
+    if constexpr (WindowPost<Op>::value) op.window_post(pre, g, I0, J0, ur);
 #pragma unroll
     for (int t = 0; t < WIN_LOADS; t++) {
       const int k = threadIdx.x + t * TBLOCK;
@@ -540,6 +557,10 @@ template <typename T>
 struct OpCgDir {
   struct Pre { T beta; bool first; };
   static constexpr int kParts = 1;
+  // the window loads c before beta is known; after the first iteration
+  // window_post adds beta u_old (c + beta u_old, the same rounding)
+  static constexpr bool kLatePrepare = true;
+  static constexpr bool kWindowPost = true;
   __device__ void zero_tail(int) const {}
   __device__ void block_end(double, double) const {}
   const T* c;
@@ -561,8 +582,18 @@ struct OpCgDir {
   }
   using Ext = ExtNone;
   __device__ void fetch(const Pre&, int64_t, int, int, Ext&) const {}
-  __device__ T load(const Pre& p, int64_t n, int, int J) const {
-    return J == 0 ? T(0) : (p.first ? c[n] : c[n] + p.beta * u_old[n]);
+  __device__ T load(const Pre&, int64_t n, int, int J) const { return J == 0 ? T(0) : c[n]; }
+  template <int NL>
+  __device__ void window_post(const Pre& p, const Geom& g, int I0, int J0, T (&ur)[NL]) const {
+    if (p.first) return;   // block-uniform
+#pragma unroll
+    for (int t = 0; t < NL; t++) {
+      const int k = threadIdx.x + t * TBLOCK;
+      const int rr = k / WIN_I, cc = k - rr * WIN_I;
+      const int I = I0 - 4 + cc, J = J0 - 4 + rr;
+      if (k < WIN_J * WIN_I && I >= 0 && I < g.NX && J >= 1 && J < g.NY)
+        ur[t] = ur[t] + p.beta * u_old[(int64_t)J * g.NX + I];
+    }
   }
   __device__ void epilogue(const Pre&, int64_t n, int, int J, T ku, T uc, const Ext&, double& a0, double&) const {
     const T qv = J == 0 ? T(0) : ku;
@@ -1205,24 +1236,41 @@ __global__ __launch_bounds__(BLOCK) void cg_update_kernel(int64_t n, T* x, T* r,
                                                           const double* part_rho, int nrho, const double* part_uq,
                                                           int nuq, CgScal* sc, double* part_rr, int npart,
                                                           float* r32) {
-  if (sc->done) return;
+  // the partials of rho and u.q, then this thread's nodes, all in flight before
+  // alpha is formed
+  const double prho = parts_load(part_rho, nrho);
+  const double puq = parts_load(part_uq, nuq);
+  const int done = sc->done;
+  const int64_t base = (int64_t)blockIdx.x * BLOCK * 4;
+  T xv[4], uv[4], rv[4], qv[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int64_t i = base + k * BLOCK + threadIdx.x;
+    xv[k] = uv[k] = rv[k] = qv[k] = T(0);
+    if (i < n) {
+      xv[k] = x[i];
+      uv[k] = u[i];
+      rv[k] = r[i];
+      qv[k] = q[i];
+    }
+  }
+  if (done) return;
   // the residual partials are consumed over npart entries: clear the tail
   if (blockIdx.x == 0)
     for (int i = gridDim.x + threadIdx.x; i < npart; i += BLOCK) part_rr[i] = 0.0;
-  const double rho = reduce_parts(part_rho, nrho);
-  const double uq = reduce_parts(part_uq, nuq);
+  const double rho = parts_total(prho);
+  const double uq = parts_total(puq);
   const T alpha = (T)(rho / uq);
   double acc = 0.0;
-  const int64_t base = (int64_t)blockIdx.x * BLOCK * 4;
 #pragma unroll
   for (int k = 0; k < 4; k++) {
     const int64_t i = base + k * BLOCK + threadIdx.x;
     if (i < n) {
-      x[i] += alpha * u[i];
-      const T rv = r[i] - alpha * q[i];
-      r[i] = rv;
-      if (r32) r32[i] = (float)rv;
-      acc += (double)rv * (double)rv;
+      x[i] = xv[k] + alpha * uv[k];
+      const T rn = rv[k] - alpha * qv[k];
+      r[i] = rn;
+      if (r32) r32[i] = (float)rn;
+      acc += (double)rn * (double)rn;
     }
   }
   const double s = block_sum(acc);
@@ -2377,38 +2425,12 @@ __global__ __launch_bounds__(SW_BLOCK) void cheb_sweep_kernel(Geom g, ChebSweep<
                                      // times c1 wline(J)
   __shared__ T cyr[SW_YS * SW_WI];   // y carry of strip k: its segment's output 4
   __shared__ T scv[SW_VJ * SW_VI];   // post: the coarse (element-vertex) correction around the box
-  if (op.init) {
-    // a solve's first kernel: every block forms the test from the partials; block 0 writes the
-    // scalars (nothing else of this launch reads them)
-    const double rr = reduce_parts(op.part_rr, op.nrr);
-    const double bb = reduce_parts(op.part_bb, op.nrr);
-    const double resid = sqrt(rr), bnorm = sqrt(bb);
-    const bool fin = resid <= op.tol * bnorm || op.maxiter <= 0;
-    if (threadIdx.x == 0 && blockIdx.x == 0 && blockIdx.y == 0) {
-      CgScal h{};
-      h.rho_prev = 1.0;
-      h.resid = resid;
-      h.bnorm = bnorm;
-      h.reltol = op.tol * bnorm;
-      h.iter = 0;
-      h.maxiter = op.maxiter;
-      h.done = fin ? 1 : 0;
-      *op.sc = h;
-    }
-    if (fin) return;
-  } else {
-    if (op.sc->done) return;         // block-uniform, as OpCheb::prepare
-    if (op.check) {
-      const double rr = reduce_parts(op.part_rr, op.nrr);
-      const double resid = sqrt(rr);
-      const bool fin = (resid <= op.sc->reltol) || (op.sc->iter >= op.sc->maxiter);
-      if (threadIdx.x == 0 && blockIdx.x == 0 && blockIdx.y == 0) {
-        op.sc->resid = resid;
-        if (fin) op.sc->done = 1;
-      }
-      if (fin) return;
-    }
-  }
+  // the convergence test's partials (and with init ||b||^2's) load first and are
+  // summed below, while the tile's own loads are in flight
+  const bool test = op.init || op.check;
+  const double prr = test ? parts_load(op.part_rr, op.nrr) : 0.0;
+  const double pbb = op.init ? parts_load(op.part_bb, op.nrr) : 0.0;
+  const int done_in = op.init ? 0 : op.sc->done;
   const int tile = xcd_tile(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
   const int I0 = (tile % gridDim.x) * TILE_I, J0 = (tile / gridDim.x) * TILE_J;
   const int tid = threadIdx.x;
@@ -2472,6 +2494,33 @@ __global__ __launch_bounds__(SW_BLOCK) void cheb_sweep_kernel(Geom g, ChebSweep<
         mb[t] = op.mu[n];
         if (op.post && J > 0) ub[t] = op.src[n];
       }
+    }
+    // the test (block-uniform)
+    if (done_in) return;               // as OpCheb::prepare
+    if (op.init) {
+      // a solve's first kernel: block 0 writes the scalars (nothing else of this launch reads them)
+      const double resid = sqrt(parts_total(prr)), bnorm = sqrt(parts_total(pbb));
+      const bool fin = resid <= op.tol * bnorm || op.maxiter <= 0;
+      if (threadIdx.x == 0 && blockIdx.x == 0 && blockIdx.y == 0) {
+        CgScal h{};
+        h.rho_prev = 1.0;
+        h.resid = resid;
+        h.bnorm = bnorm;
+        h.reltol = op.tol * bnorm;
+        h.iter = 0;
+        h.maxiter = op.maxiter;
+        h.done = fin ? 1 : 0;
+        *op.sc = h;
+      }
+      if (fin) return;
+    } else if (op.check) {
+      const double resid = sqrt(parts_total(prr));
+      const bool fin = (resid <= op.sc->reltol) || (op.sc->iter >= op.sc->maxiter);
+      if (threadIdx.x == 0 && blockIdx.x == 0 && blockIdx.y == 0) {
+        op.sc->resid = resid;
+        if (fin) op.sc->done = 1;
+      }
+      if (fin) return;
     }
     if (pcoarse) __syncthreads();
 #pragma unroll
