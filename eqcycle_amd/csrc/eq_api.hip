@@ -289,10 +289,11 @@ static int sync_check(eq_ctx* ctx) {
 
 // Wait for the stream by polling an event: the time loop reads a few scalars
 // back several times per step, and a busy poll answers sooner than a
-// blocking synchronisation.
-static int spin_sync(eq_ctx* ctx) {
+// blocking synchronisation.  spin_wait polls ev_spin as recorded earlier on
+// the stream (work queued after it keeps the device busy meanwhile);
+// spin_sync records it at the stream's end first.
+static int spin_wait(eq_ctx* ctx) {
   HIPCHK(ctx, hipGetLastError());
-  HIPCHK(ctx, hipEventRecord(ctx->ev_spin, ctx->s));
   for (;;) {
     const hipError_t e = hipEventQuery(ctx->ev_spin);
     if (e == hipSuccess) return EQ_OK;
@@ -301,6 +302,10 @@ static int spin_sync(eq_ctx* ctx) {
       return EQ_HIP_ERR;
     }
   }
+}
+static int spin_sync(eq_ctx* ctx) {
+  HIPCHK(ctx, hipEventRecord(ctx->ev_spin, ctx->s));
+  return spin_wait(ctx);
 }
 
 template <typename T>
@@ -2143,8 +2148,10 @@ static void galerkin_correction(eq_ctx* ctx, double* x, const double* b, const d
 // in the same launch (launch_gal_residual): x = d + Phi c lands in d2, which
 // then becomes d (d2 is free between dynamic steps), and w64.r (with its FP32
 // copy for an F32PC V-cycle), part_rr and part_bb are what cg_begin's
-// residual would leave.  False when that form does not apply.
-static bool galerkin_start_fused(eq_ctx* ctx, const double* db) {
+// residual would leave.  False when that form does not apply.  gate (or null):
+// the residual launch does nothing unless gate->done, and d2 is then left as
+// it was (the caller swaps d and d2 back).
+static bool galerkin_start_fused(eq_ctx* ctx, const double* db, const CgScal* gate = nullptr) {
   const eq_solver_opts& o = ctx->opts;
   LearnState& ls = ctx->learn;
   if (!db || o.galerkin_split || !o.basis_fp16 || !ls.Phi16 || ls.offset != ctx->NX ||
@@ -2154,7 +2161,7 @@ static bool galerkin_start_fused(eq_ctx* ctx, const double* db) {
   const bool r32 = o.prec == EQ_PREC_F32PC && o.pc == EQ_PC_AMG;
   launch_gal_residual(ctx->g, ctx->d, (const half8*)ls.Phi16, ls.nI, ls.d.NB, ls.gal_c, ctx->mu, ctx->bq,
                       ctx->w64.r, r32 ? ctx->w32.r : nullptr, ctx->d2, ctx->part_rr, ctx->part_bb, ctx->npart,
-                      ctx->s);
+                      ctx->s, gate);
   std::swap(ctx->d, ctx->d2);
   ctx->n_learned_solves++;
   return true;
@@ -2176,6 +2183,12 @@ static int qs_step(eq_ctx* ctx, eq_step_info* info, bool r_valid) {
   hipStream_t s = ctx->s;
   const double dt = ctx->dt;
   bool bold_saved = false;
+  bool mid_sent = false;   // the second pass's begin and start were enqueued with the first solve
+  bool r_fused_mid = false;
+  const int prec = ctx->opts.prec;
+  // the second solve may run ahead of its read-back (eq_step finishes it)
+  const bool ahead = !ctx->opts.sync_pass2 && (prec == EQ_PREC_FP64 || prec == EQ_PREC_F32PC) && ctx->spec_save;
+  const bool f32pc = prec == EQ_PREC_F32PC && ctx->opts.pc == EQ_PC_AMG;
   for (int pass = 0; pass < 2; pass++) {
     const bool galerkin = ctx->opts.guess == EQ_GUESS_GALERKIN && ctx->learn.gal_ready;
     // the previous solve (this step's first, or the last step's second when
@@ -2192,21 +2205,56 @@ static int qs_step(eq_ctx* ctx, eq_step_info* info, bool r_valid) {
     if (pass == 0)
       launch_qs_begin(g, ctx->fp, ctx->fs, ctx->mu, ctx->d, ctx->dPre, ctx->v, ctx->bq, dt, 1, s,
                       upd ? ctx->gal_db : nullptr, theta, galerkin ? ctx->gal_bold : nullptr);
-    else   // the first pass's fault update, then this pass's begin
+    else if (!mid_sent)   // the first pass's fault update, then this pass's begin
       launch_qs_mid(g, ctx->fp, ctx->fs, ctx->mu, ctx->d, ctx->dPre, ctx->v, ctx->bq, dt,
                     upd ? ctx->gal_db : nullptr, s);
     const bool learned = pass == 0 && ctx->opts.guess == EQ_GUESS_LEARNED && ctx->learn_on;
     const bool linear = pass == 0 && ctx->opts.guess == EQ_GUESS_LINEAR && ctx->learn.lin_ready;
     if (learned || linear) learned_correction(ctx, ctx->d, ctx->bq, linear);
     // r_fused: the Galerkin start left the residual (and its FP32 copy)
-    const bool r_fused = galerkin && galerkin_start_fused(ctx, upd ? ctx->gal_db : nullptr);
+    const bool r_fused = mid_sent ? r_fused_mid : galerkin && galerkin_start_fused(ctx, upd ? ctx->gal_db : nullptr);
     if (galerkin && !r_fused) galerkin_correction(ctx, ctx->d, ctx->bq, upd ? ctx->gal_db : nullptr);
     int64_t it = 0;
     double rr = 0;
-    const int prec = ctx->opts.prec;
-    if (pass == 1 && !ctx->opts.sync_pass2 && (prec == EQ_PREC_FP64 || prec == EQ_PREC_F32PC) && ctx->spec_save) {
+    if (pass == 0 && ahead && r_fused && !ctx->learn_record_on) {
+      // The first solve's batch; then, before the host waits, the fault update
+      // with the batch's convergence test (qs_mid_kernel) and the second
+      // pass's start, which both change nothing unless the test passed.  The
+      // host reads the test after the fault update while the start runs.
+      // (The fused start applied to this pass, so it applies to the next.)
+      const int64_t batch = ctx->pred_iters[0];
+      ctx->cg_slot = 0;
+      int rc = cg_begin<double>(ctx, ctx->d, ctx->bq, ctx->opts.tol, ctx->opts.maxiter, ctx->opts.pc, f32pc, true,
+                                batch, true);
+      if (!rc) rc = enqueue_iterations<double>(ctx, ctx->d, ctx->opts.pc, f32pc, batch);
+      ctx->cg_init_pending = false;
+      if (rc) return rc;
+      launch_qs_mid(g, ctx->fp, ctx->fs, ctx->mu, ctx->d, ctx->dPre, ctx->v, ctx->bq, dt, ctx->gal_db, s, ctx->sc,
+                    ctx->part_rr, ctx->npart, ctx->sc_host);
+      HIPCHK(ctx, hipEventRecord(ctx->ev_spin, s));
+      r_fused_mid = galerkin_start_fused(ctx, ctx->gal_db, ctx->sc);
+      if (int rc2 = spin_wait(ctx)) return rc2;
+      if (!ctx->sc_host->done) {
+        // the batch fell short: the gated launches left d, d2 and the fault
+        // as they were; finish the solve, then enqueue them again
+        if (r_fused_mid) {
+          std::swap(ctx->d, ctx->d2);
+          ctx->n_learned_solves--;
+        }
+        if (int rc3 = cg_rest<double>(ctx, ctx->d, ctx->opts.pc, f32pc, batch)) return rc3;
+        launch_qs_mid(g, ctx->fp, ctx->fs, ctx->mu, ctx->d, ctx->dPre, ctx->v, ctx->bq, dt, ctx->gal_db, s);
+        r_fused_mid = galerkin_start_fused(ctx, ctx->gal_db);
+      }
+      rc = cg_end(ctx, &it, &rr);
+      ctx->cg_slot = 2;
+      if (rc == EQ_PCG_NOCONV) ctx->pcg_noconv++;
+      else if (rc) return rc;
+      info->pcg_iters[0] = it;
+      mid_sent = true;
+      continue;
+    }
+    if (pass == 1 && ahead) {
       // pass 2 ahead of its read-back (eq_step finishes it)
-      const bool f32pc = prec == EQ_PREC_F32PC && ctx->opts.pc == EQ_PC_AMG;
       ctx->cg_slot = 1;
       int rc = cg_begin<double>(ctx, ctx->d, ctx->bq, ctx->opts.tol, ctx->opts.maxiter, ctx->opts.pc, f32pc, r_fused,
                                 ctx->pred_iters[1], r_fused);

@@ -99,10 +99,11 @@ void launch_cg_init_scal(CgScal* sc, const double* part_rr, const double* part_b
                          int maxiter, hipStream_t s);
 // Galerkin start x = d + Phi c (interior; Phi the blocked FP16 basis
 // [ceil(nb / 8)][nI][8]) into x_out, and r = b - K_II x with the partials of
-// launch_cg_residual, in one launch.  x_out must not alias d.
+// launch_cg_residual, in one launch.  x_out must not alias d.  gate (or
+// null): the launch does nothing unless gate->done.
 void launch_gal_residual(const Geom& g, const double* d, const half8* phi16, int64_t nI, int nb, const double* c,
                          const double* mu, const double* b, double* r, float* r32, double* x_out, double* part_rr,
-                         double* part_bb, int npart, hipStream_t s);
+                         double* part_bb, int npart, hipStream_t s, const CgScal* gate = nullptr);
 // mirror: pinned host copy of the scalars, written after the check (or null)
 void launch_cg_check(CgScal* sc, const double* part_rr, int nparts, hipStream_t s, CgScal* mirror = nullptr);
 template <typename T>
@@ -269,9 +270,12 @@ void launch_qs_begin(const Geom& g, const FaultDev& fp, FaultState fs, const dou
                      const double* v, double* b, double dt, int first_pass, hipStream_t s,
                      double* db = nullptr, double theta = 0.0, double* bold = nullptr);
 // launch_qs_fault (no save) then the fault-row part of launch_qs_begin for the
-// second pass (first_pass = 0, theta = 0, no bold), in one single-block launch
+// second pass (first_pass = 0, theta = 0, no bold), in one single-block launch;
+// sc (or null): the first solve's convergence test first (cg_check's, with
+// sc_mirror), and nothing more while that solve has not converged
 void launch_qs_mid(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, double* d, const double* dPre,
-                   double* v, double* b, double dt, double* db, hipStream_t s);
+                   double* v, double* b, double dt, double* db, hipStream_t s, CgScal* sc = nullptr,
+                   const double* part_rr = nullptr, int nparts = 0, CgScal* sc_mirror = nullptr);
 // save: psi, Vf and the fault row of d before the update (3 nflt), or null
 void launch_qs_fault(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, const double* d,
                      double* v, double dt, hipStream_t s, double* save = nullptr);

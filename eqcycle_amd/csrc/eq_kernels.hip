@@ -495,7 +495,8 @@ struct OpGalResidual {
   double* x_out;
   double* part0;
   double* part1;
-  __device__ bool prepare(Pre&) const { return true; }
+  const CgScal* gate;   // or null; with it the launch does nothing until gate->done
+  __device__ bool prepare(Pre&) const { return !gate || gate->done; }
   struct Ext { double b, d0; };
   __device__ void fetch(const Pre&, int64_t n, int, int J, Ext& e) const {
     e.b = J == 0 ? 0.0 : b[n];
@@ -1707,10 +1708,30 @@ __global__ __launch_bounds__(QE_BLOCK) void qs_end_kernel(Geom g, FaultDev fp, F
 // Between the passes of a quasi-static step, in one block: the first pass's
 // fault update (qs_fault_node at every fault node), then, once every node's
 // K d has read the fault row, the second pass's begin on the fault row
-// (qs_begin_fault_node: F = dPre + v dt, b rows 1..4, db).
+// (qs_begin_fault_node: F = dPre + v dt, b rows 1..4, db).  With sc, first
+// the first solve's convergence test (cg_check_kernel's, mirror included);
+// while that solve has not converged the kernel changes nothing else.
 __global__ __launch_bounds__(QE_BLOCK) void qs_mid_kernel(Geom g, FaultDev fp, FaultState fs, const double* mu,
                                                           double* d, const double* dPre, double* v, double* b,
-                                                          double dt, double* db) {
+                                                          double dt, double* db, CgScal* sc, const double* part_rr,
+                                                          int nparts, CgScal* sc_mirror) {
+  if (sc) {
+    __shared__ int done;
+    if (!sc->done) {   // block-uniform
+      const double rr = reduce_parts(part_rr, nparts);
+      if (threadIdx.x == 0) {
+        const double resid = sqrt(rr);
+        sc->resid = resid;
+        if (resid <= sc->reltol || sc->iter >= sc->maxiter) sc->done = 1;
+      }
+    }
+    if (threadIdx.x == 0) {
+      done = sc->done;
+      if (sc_mirror) *sc_mirror = *sc;
+    }
+    __syncthreads();
+    if (!done) return;
+  }
   for (int j = threadIdx.x; j < fp.nflt; j += QE_BLOCK) qs_fault_node(g, fp, fs, mu, d, v, dt, nullptr, j);
   __syncthreads();
   for (int I = threadIdx.x; I < g.NX; I += QE_BLOCK)
@@ -1927,12 +1948,12 @@ void launch_cg_residual(const Geom& g, const T* x, const T* mu, const T* b, T* r
 
 void launch_gal_residual(const Geom& g, const double* d, const half8* phi16, int64_t nI, int nb, const double* c,
                          const double* mu, const double* b, double* r, float* r32, double* x_out, double* part_rr,
-                         double* part_bb, int npart, hipStream_t s) {
+                         double* part_bb, int npart, hipStream_t s, const CgScal* gate) {
   if (nb == 8 * kGalGroups) {
-    OpGalResidual<true> op{npart, d, phi16, nI, nb, c, mu, b, r, r32, x_out, part_rr, part_bb};
+    OpGalResidual<true> op{npart, d, phi16, nI, nb, c, mu, b, r, r32, x_out, part_rr, part_bb, gate};
     hipLaunchKernelGGL((tile_kernel<double, OpGalResidual<true>>), tile_grid(g), dim3(TBLOCK), 0, s, g, op);
   } else {
-    OpGalResidual<false> op{npart, d, phi16, nI, nb, c, mu, b, r, r32, x_out, part_rr, part_bb};
+    OpGalResidual<false> op{npart, d, phi16, nI, nb, c, mu, b, r, r32, x_out, part_rr, part_bb, gate};
     hipLaunchKernelGGL((tile_kernel<double, OpGalResidual<false>>), tile_grid(g), dim3(TBLOCK), 0, s, g, op);
   }
 }
@@ -2959,8 +2980,10 @@ void launch_qs_end(const Geom& g, const FaultDev& fp, FaultState fs, const doubl
 }
 
 void launch_qs_mid(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, double* d, const double* dPre,
-                   double* v, double* b, double dt, double* db, hipStream_t s) {
-  hipLaunchKernelGGL(qs_mid_kernel, dim3(1), dim3(QE_BLOCK), 0, s, g, fp, fs, mu, d, dPre, v, b, dt, db);
+                   double* v, double* b, double dt, double* db, hipStream_t s, CgScal* sc, const double* part_rr,
+                   int nparts, CgScal* sc_mirror) {
+  hipLaunchKernelGGL(qs_mid_kernel, dim3(1), dim3(QE_BLOCK), 0, s, g, fp, fs, mu, d, dPre, v, b, dt, db, sc, part_rr,
+                     nparts, sc_mirror);
 }
 
 void launch_step_tail(const FaultDev& fp, const double* d, const double* v, const double* a, const double* tau,
