@@ -307,7 +307,10 @@ class EqContext:
         """early_tail: the second quasi-static solve's fault update and the
         step's status read-back go out behind its first batch of iterations
         and are redone when that batch fell short (same results, one host
-        wait fewer per step).  galerkin_shortcut: a Galerkin start after a
+        wait fewer per step); with the fused Galerkin start also the first
+        solve's fault update and the second pass's start, and in run() the
+        next step's begin and start, go out before the host has read the test
+        each waits on, and do nothing unless it passed (same results).  galerkin_shortcut: a Galerkin start after a
         solve of the same x projects only the change of b (rows 1..4).
         basis_fp16: the Galerkin expansion reads the FP16 copy of the basis.
         extrapolate: the Galerkin start of the first pass begins from the last

@@ -111,6 +111,13 @@ struct eq_ctx {
   double qs_prev_dt = 0;              // its dt
   bool spec_f32pc = false;
   int64_t spec_batch = 0;
+  // inside eq_run: the run goes on after the quasi-static step being taken,
+  // so its end may send the next step's begin and Galerkin start ahead of
+  // the read-back, gated on the step tail's status (StepStatus::go)
+  bool run_more = false;
+  bool begin_sent = false;            // sent, and the status said go
+  bool begin_r_fused = false;
+  bool spin_recorded = false;         // ev_spin marks the step's end (work queued after it)
   double t = 0, dt = 0;
   int64_t it = 0;
   int isolver = 1;
@@ -2096,7 +2103,13 @@ static StepCapture capture(eq_ctx* ctx, int slot, bool mirror = false) {
 
 // The status of a step whose tail wrote the pinned copies (capture with
 // mirror): only the synchronisation.
-static int sync_status(eq_ctx* ctx) { return spin_sync(ctx); }
+static int sync_status(eq_ctx* ctx) {
+  if (ctx->spin_recorded) {
+    ctx->spin_recorded = false;
+    return spin_wait(ctx);
+  }
+  return spin_sync(ctx);
+}
 
 static void clean_flags(eq_ctx* ctx) {
   if (!ctx->flags_dirty) return;
@@ -2149,14 +2162,18 @@ static void galerkin_correction(eq_ctx* ctx, double* x, const double* b, const d
 // then becomes d (d2 is free between dynamic steps), and w64.r (with its FP32
 // copy for an F32PC V-cycle), part_rr and part_bb are what cg_begin's
 // residual would leave.  False when that form does not apply.  gate (or null):
-// the residual launch does nothing unless gate->done, and d2 is then left as
-// it was (the caller swaps d and d2 back).
-static bool galerkin_start_fused(eq_ctx* ctx, const double* db, const CgScal* gate = nullptr) {
+// the residual launch does nothing unless *gate, and d2 is then left as it
+// was (the caller swaps d and d2 back).
+static bool fused_start_applies(const eq_ctx* ctx) {
+  const eq_solver_opts& o = ctx->opts;
+  const LearnState& ls = ctx->learn;
+  return !o.galerkin_split && o.basis_fp16 && ls.Phi16 && ls.offset == ctx->NX &&
+         (o.prec == EQ_PREC_FP64 || o.prec == EQ_PREC_F32PC);
+}
+static bool galerkin_start_fused(eq_ctx* ctx, const double* db, const int* gate = nullptr) {
   const eq_solver_opts& o = ctx->opts;
   LearnState& ls = ctx->learn;
-  if (!db || o.galerkin_split || !o.basis_fp16 || !ls.Phi16 || ls.offset != ctx->NX ||
-      (o.prec != EQ_PREC_FP64 && o.prec != EQ_PREC_F32PC))
-    return false;
+  if (!db || !fused_start_applies(ctx)) return false;
   learn_project_rows(&ls, db, 4 * (int64_t)ctx->NX, ctx->s);
   const bool r32 = o.prec == EQ_PREC_F32PC && o.pc == EQ_PC_AMG;
   launch_gal_residual(ctx->g, ctx->d, (const half8*)ls.Phi16, ls.nI, ls.d.NB, ls.gal_c, ctx->mu, ctx->bq,
@@ -2178,6 +2195,40 @@ static void qs_end(eq_ctx* ctx, bool spec) {
 }
 
 // r_valid: the last solve's residual may be reused (eq_ctx::qs_r_valid).
+// At the end of a quasi-static step inside eq_run: the next step's begin
+// (dt from the step tail's status, the extrapolation factor from it and this
+// step's dt) and its change-of-b Galerkin start, sent before the host reads
+// the status, so the device has them while the host turns round.  Both do
+// nothing unless the status says go; eq_step then takes them back
+// (qs_take_back) and the next step starts as usual.  Its conditions are the
+// next step's: it follows a quasi-static solve (r_valid) and extrapolates
+// when this step did follow one.
+static void qs_send_next(eq_ctx* ctx) {
+  const eq_solver_opts& o = ctx->opts;
+  if (o.sync_pass2 || o.guess != EQ_GUESS_GALERKIN || !ctx->learn.gal_ready || o.galerkin_full ||
+      !fused_start_applies(ctx) || ctx->learn_record_on)
+    return;
+  hipStream_t s = ctx->s;
+  if (hipEventRecord(ctx->ev_spin, s) != hipSuccess) return;   // (the step's own synchronisation reports it)
+  ctx->spin_recorded = true;
+  const bool theta_on = o.extrapolate && ctx->qs_prev_r_valid;
+  launch_qs_begin(ctx->g, ctx->fp, ctx->fs, ctx->mu, ctx->d, ctx->dPre, ctx->v, ctx->bq, 0.0, 1, s, ctx->gal_db,
+                  theta_on ? 1.0 : 0.0, ctx->gal_bold, ctx->st, ctx->dt);
+  ctx->begin_r_fused = galerkin_start_fused(ctx, ctx->gal_db, &ctx->st->go);
+  ctx->begin_sent = true;
+}
+
+// After the step's status is on the host: a begin sent ahead that the status
+// stopped did nothing, so d and d2 go back.
+static void qs_take_back(eq_ctx* ctx) {
+  if (!ctx->begin_sent || ctx->st_host->go) return;
+  if (ctx->begin_r_fused) {
+    std::swap(ctx->d, ctx->d2);
+    ctx->n_learned_solves--;
+  }
+  ctx->begin_sent = false;
+}
+
 static int qs_step(eq_ctx* ctx, eq_step_info* info, bool r_valid) {
   const Geom& g = ctx->g;
   hipStream_t s = ctx->s;
@@ -2189,6 +2240,9 @@ static int qs_step(eq_ctx* ctx, eq_step_info* info, bool r_valid) {
   // the second solve may run ahead of its read-back (eq_step finishes it)
   const bool ahead = !ctx->opts.sync_pass2 && (prec == EQ_PREC_FP64 || prec == EQ_PREC_F32PC) && ctx->spec_save;
   const bool f32pc = prec == EQ_PREC_F32PC && ctx->opts.pc == EQ_PC_AMG;
+  // the first pass's begin and start went out with the step before (qs_send_next)
+  const bool sent = ctx->begin_sent;
+  ctx->begin_sent = false;
   for (int pass = 0; pass < 2; pass++) {
     const bool galerkin = ctx->opts.guess == EQ_GUESS_GALERKIN && ctx->learn.gal_ready;
     // the previous solve (this step's first, or the last step's second when
@@ -2202,17 +2256,20 @@ static int qs_step(eq_ctx* ctx, eq_step_info* info, bool r_valid) {
                              ? dt / ctx->qs_prev_dt
                              : 0.0;
     if (pass == 0) bold_saved = galerkin;
-    if (pass == 0)
-      launch_qs_begin(g, ctx->fp, ctx->fs, ctx->mu, ctx->d, ctx->dPre, ctx->v, ctx->bq, dt, 1, s,
-                      upd ? ctx->gal_db : nullptr, theta, galerkin ? ctx->gal_bold : nullptr);
-    else if (!mid_sent)   // the first pass's fault update, then this pass's begin
+    if (pass == 0) {
+      if (!sent)
+        launch_qs_begin(g, ctx->fp, ctx->fs, ctx->mu, ctx->d, ctx->dPre, ctx->v, ctx->bq, dt, 1, s,
+                        upd ? ctx->gal_db : nullptr, theta, galerkin ? ctx->gal_bold : nullptr);
+    } else if (!mid_sent)   // the first pass's fault update, then this pass's begin
       launch_qs_mid(g, ctx->fp, ctx->fs, ctx->mu, ctx->d, ctx->dPre, ctx->v, ctx->bq, dt,
                     upd ? ctx->gal_db : nullptr, s);
     const bool learned = pass == 0 && ctx->opts.guess == EQ_GUESS_LEARNED && ctx->learn_on;
     const bool linear = pass == 0 && ctx->opts.guess == EQ_GUESS_LINEAR && ctx->learn.lin_ready;
     if (learned || linear) learned_correction(ctx, ctx->d, ctx->bq, linear);
     // r_fused: the Galerkin start left the residual (and its FP32 copy)
-    const bool r_fused = mid_sent ? r_fused_mid : galerkin && galerkin_start_fused(ctx, upd ? ctx->gal_db : nullptr);
+    const bool r_fused = mid_sent                ? r_fused_mid
+                         : (pass == 0 && sent) ? ctx->begin_r_fused
+                                               : galerkin && galerkin_start_fused(ctx, upd ? ctx->gal_db : nullptr);
     if (galerkin && !r_fused) galerkin_correction(ctx, ctx->d, ctx->bq, upd ? ctx->gal_db : nullptr);
     int64_t it = 0;
     double rr = 0;
@@ -2232,7 +2289,7 @@ static int qs_step(eq_ctx* ctx, eq_step_info* info, bool r_valid) {
       launch_qs_mid(g, ctx->fp, ctx->fs, ctx->mu, ctx->d, ctx->dPre, ctx->v, ctx->bq, dt, ctx->gal_db, s, ctx->sc,
                     ctx->part_rr, ctx->npart, ctx->sc_host);
       HIPCHK(ctx, hipEventRecord(ctx->ev_spin, s));
-      r_fused_mid = galerkin_start_fused(ctx, ctx->gal_db, ctx->sc);
+      r_fused_mid = galerkin_start_fused(ctx, ctx->gal_db, &ctx->sc->done);
       if (int rc2 = spin_wait(ctx)) return rc2;
       if (!ctx->sc_host->done) {
         // the batch fell short: the gated launches left d, d2 and the fault
@@ -2285,6 +2342,7 @@ static int qs_step(eq_ctx* ctx, eq_step_info* info, bool r_valid) {
   qs_end(ctx, ctx->spec_pending);
   ctx->qs_prev_r_valid = r_valid && bold_saved;
   ctx->qs_prev_dt = dt;
+  if (ctx->run_more) qs_send_next(ctx);
   return EQ_OK;
 }
 
@@ -2323,6 +2381,7 @@ extern "C" int eq_step(eq_ctx* ctx, eq_step_info* info) {
   const bool spec = ctx->spec_pending;
   ctx->spec_pending = false;
   if (int rc2 = sync_status(ctx)) return rc2;
+  qs_take_back(ctx);
   if (spec) {
     // pass 2's scalars came back with the status
     if (!ctx->sc_host->done) {
@@ -2685,7 +2744,10 @@ extern "C" int eq_run(eq_ctx* ctx, double total_time, int64_t max_steps, eq_resu
       }
     } else {
       eq_step_info info;
+      // (eq_step advances t by this dt, so this is the loop's test after it)
+      ctx->run_more = ctx->t + ctx->dt < total_time && (max_steps <= 0 || R->n_steps + 1 < max_steps);
       int rc = eq_step(ctx, &info);
+      ctx->run_more = false;
       const double el = std::chrono::duration<double>(clk::now() - t0).count();
       if (info.isolver_used == 1) { R->n_qs++; R->wall_qs += el; } else { R->n_dyn++; R->wall_dyn += el; }
       if (rc) return rc;

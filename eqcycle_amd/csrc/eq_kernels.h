@@ -61,6 +61,8 @@ struct StepStatus {   // written by the step tail kernel
   int hypo;          // argmax of the slip rate (0-based fault index)
   int nr_flag;       // 1 + first failing node of the FBC! guard, or 0
   int nr_noconv;     // Newton searches that took the failure exit
+  int go;            // 1: the next step is quasi-static and this one ended well (NR searches,
+                     // and the second solve's convergence test where the step end ran it)
 };
 
 // Per-step record written by the step tail kernel besides its StepStatus:
@@ -100,10 +102,10 @@ void launch_cg_init_scal(CgScal* sc, const double* part_rr, const double* part_b
 // Galerkin start x = d + Phi c (interior; Phi the blocked FP16 basis
 // [ceil(nb / 8)][nI][8]) into x_out, and r = b - K_II x with the partials of
 // launch_cg_residual, in one launch.  x_out must not alias d.  gate (or
-// null): the launch does nothing unless gate->done.
+// null): the launch does nothing unless *gate.
 void launch_gal_residual(const Geom& g, const double* d, const half8* phi16, int64_t nI, int nb, const double* c,
                          const double* mu, const double* b, double* r, float* r32, double* x_out, double* part_rr,
-                         double* part_bb, int npart, hipStream_t s, const CgScal* gate = nullptr);
+                         double* part_bb, int npart, hipStream_t s, const int* gate = nullptr);
 // mirror: pinned host copy of the scalars, written after the check (or null)
 void launch_cg_check(CgScal* sc, const double* part_rr, int nparts, hipStream_t s, CgScal* mirror = nullptr);
 template <typename T>
@@ -266,9 +268,14 @@ void launch_newmark_predict(int64_t n, const double* d, const double* v, const d
 // db: the change of b on rows 1..4 since the last solve; bold: those rows of b
 // before this pass (saved); theta != 0 (first pass): the start off the fault
 // becomes d + theta (d - dPre) and db the change less theta times bold's.
+// st_next (or null; first pass only): the begin of the step after the one
+// whose step tail wrote *st_next, enqueued before the host has read it: the
+// launch does nothing unless st_next->go, dt is st_next->dt_next and a
+// nonzero theta becomes dt / dt_prev.
 void launch_qs_begin(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, double* d, double* dPre,
                      const double* v, double* b, double dt, int first_pass, hipStream_t s,
-                     double* db = nullptr, double theta = 0.0, double* bold = nullptr);
+                     double* db = nullptr, double theta = 0.0, double* bold = nullptr,
+                     const StepStatus* st_next = nullptr, double dt_prev = 0.0);
 // launch_qs_fault (no save) then the fault-row part of launch_qs_begin for the
 // second pass (first_pass = 0, theta = 0, no bold), in one single-block launch;
 // sc (or null): the first solve's convergence test first (cg_check's, with

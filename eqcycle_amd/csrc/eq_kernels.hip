@@ -495,8 +495,8 @@ struct OpGalResidual {
   double* x_out;
   double* part0;
   double* part1;
-  const CgScal* gate;   // or null; with it the launch does nothing until gate->done
-  __device__ bool prepare(Pre&) const { return !gate || gate->done; }
+  const int* gate;   // or null; with it the launch does nothing unless *gate
+  __device__ bool prepare(Pre&) const { return !gate || *gate; }
   struct Ext { double b, d0; };
   __device__ void fetch(const Pre&, int64_t n, int, int J, Ext& e) const {
     e.b = J == 0 ? 0.0 : b[n];
@@ -1466,7 +1466,12 @@ __device__ __forceinline__ void qs_begin_fault_node(const Geom& g, const FaultDe
 __global__ __launch_bounds__(BLOCK) void qs_begin_kernel(Geom g, FaultDev fp, FaultState fs, const double* mu,
                                                          double* d, double* dPre, const double* v, double* b,
                                                          double dt, int first_pass, double* db, double theta,
-                                                         double* bold) {
+                                                         double* bold, const StepStatus* st_next, double dt_prev) {
+  if (st_next) {   // block-uniform
+    if (!st_next->go) return;
+    dt = st_next->dt_next;
+    if (theta != 0.0) theta = dt / dt_prev;
+  }
   const int64_t N = (int64_t)g.NX * g.NY;
   const int64_t n = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   if (first_pass) {
@@ -1571,7 +1576,8 @@ __device__ double dtevol_dev(const double* XiLf, const double* v, bool slip_rate
 template <int NT>
 __device__ void step_tail_body(const FaultDev& fp, const double* d, const double* v, const double* a,
                                const double* tau, double dt, double dtmin, int isolver, StepStatus* st, int* flags,
-                               const StepCapture& cap, const double* qs_dPre = nullptr, int64_t NX = 0) {
+                               const StepCapture& cap, const double* qs_dPre = nullptr, int64_t NX = 0,
+                               const CgScal* sc_go = nullptr) {
   // one pass over the fault: the maximum of v (first index on ties) and the
   // quasi-static dtevol! candidate min XiLf/|Vf| over the cells that qualify,
   // reduced across each wave by shuffles and across the four waves in LDS
@@ -1652,6 +1658,7 @@ __device__ void step_tail_body(const FaultDev& fp, const double* d, const double
     o.hypo = mi;
     o.nr_flag = (flags[0] == INT_MAX) ? 0 : flags[0];
     o.nr_noconv = flags[1];
+    o.go = (next == 1 && o.nr_flag == 0 && (!sc_go || sc_go->done)) ? 1 : 0;
     *st = o;
     if (cap.st_mirror) *cap.st_mirror = o;
     flags[0] = INT_MAX;
@@ -1702,7 +1709,7 @@ __global__ __launch_bounds__(QE_BLOCK) void qs_end_kernel(Geom g, FaultDev fp, F
   __syncthreads();   // every tau1 read d's fault row before creeping nodes reset it
   for (int j = threadIdx.x; j < fp.nflt; j += QE_BLOCK) qs_finish_fault_node(fp, fs, d, v, j);
   __syncthreads();
-  step_tail_body<QE_BLOCK>(fp, d, v, a, fs.tau, dt, dtmin, 1, st, flags, cap, dPre, g.NX);
+  step_tail_body<QE_BLOCK>(fp, d, v, a, fs.tau, dt, dtmin, 1, st, flags, cap, dPre, g.NX, sc);
 }
 
 // Between the passes of a quasi-static step, in one block: the first pass's
@@ -1948,7 +1955,7 @@ void launch_cg_residual(const Geom& g, const T* x, const T* mu, const T* b, T* r
 
 void launch_gal_residual(const Geom& g, const double* d, const half8* phi16, int64_t nI, int nb, const double* c,
                          const double* mu, const double* b, double* r, float* r32, double* x_out, double* part_rr,
-                         double* part_bb, int npart, hipStream_t s, const CgScal* gate) {
+                         double* part_bb, int npart, hipStream_t s, const int* gate) {
   if (nb == 8 * kGalGroups) {
     OpGalResidual<true> op{npart, d, phi16, nI, nb, c, mu, b, r, r32, x_out, part_rr, part_bb, gate};
     hipLaunchKernelGGL((tile_kernel<double, OpGalResidual<true>>), tile_grid(g), dim3(TBLOCK), 0, s, g, op);
@@ -2959,10 +2966,10 @@ void launch_newmark(const Geom& g, const FaultDev& fp, FaultState fs, const doub
 
 void launch_qs_begin(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, double* d, double* dPre,
                      const double* v, double* b, double dt, int first_pass, hipStream_t s, double* db, double theta,
-                     double* bold) {
+                     double* bold, const StepStatus* st_next, double dt_prev) {
   const int64_t N = first_pass ? (int64_t)g.NX * g.NY : g.NX;
   hipLaunchKernelGGL(qs_begin_kernel, dim3(nblk(N)), dim3(BLOCK), 0, s, g, fp, fs, mu, d, dPre, v, b, dt,
-                     first_pass, db, theta, bold);
+                     first_pass, db, theta, bold, st_next, dt_prev);
 }
 
 void launch_qs_fault(const Geom& g, const FaultDev& fp, FaultState fs, const double* mu, const double* d, double* v,

@@ -234,7 +234,11 @@ typedef struct {
   int sync_pass2;          /* 1: the host waits for the second quasi-static solve to converge before
                               the fault update; 0 (default, FP64 and F32PC): the fault update, the
                               step finish and the status read-back follow the solve's first batch of
-                              iterations and are redone when that batch fell short (same results) */
+                              iterations and are redone when that batch fell short (same results).
+                              With 0 and the fused Galerkin start also: the first solve's fault
+                              update and the second pass's start follow its first batch, and inside
+                              eq_run the next step's begin and start follow the step's end, each
+                              doing nothing unless the test it waits on passed (same results) */
   int galerkin_full;       /* 1: every Galerkin start projects the full residual b - K_II x; 0
                               (default): when the previous solve left x solving the old b and only
                               b's rows 1..4 changed, just that change is projected (the old

@@ -275,6 +275,54 @@ def test_fused_galerkin_start_is_bit_identical(trained, prec):
     assert st_s["t"] == st_f["t"] and st_s["dt"] == st_f["dt"]
 
 
+@pytest.mark.parametrize("prec", ["fp64", "f32pc"])
+def test_run_sends_next_begin_ahead_bit_identical(trained, prec):
+    """Inside run(), the end of a quasi-static step sends the next step's
+    begin and Galerkin start before the host has read the step's status, and
+    they do nothing unless the status lets the next step go (quasi-static, the
+    searches sound, the second solve converged in its first batch).  40 steps
+    of run() against 40 calls of step(), which send nothing ahead; then 400
+    steps of run() through an event (the last quasi-static step before it
+    sends a begin that its status stops) against run() with early_tail=False
+    (every read-back waited for): the same iteration counts and the same
+    state bit for bit."""
+    P, ctx, fit = trained
+    s0 = fit["qs_state"]
+
+    def reset(early=True):
+        ctx.state_set(d=s0["d"], v=s0["v"], a=s0["a"], psi=s0["psi"], tau=s0["tau"], t=s0["t"], dt=s0["dt"],
+                      isolver=s0["isolver"])
+        ctx.set_solver(tol=1e-8, pc="amg", prec=prec, guess="galerkin", basis_fp16=True, extrapolate=True,
+                       early_tail=early)
+
+    def same(a, b):
+        for k in ("d", "v", "a", "psi", "tau"):
+            assert np.array_equal(a[k], b[k]), k
+        assert a["t"] == b["t"] and a["dt"] == b["dt"] and a["isolver"] == b["isolver"]
+
+    reset()
+    its = np.array([ctx.step().pcg_iters for _ in range(40)])
+    st_step = ctx.state_get()
+    reset()
+    out = ctx.run(max_steps=40)
+    print(f"[gpu] {prec}: 40 steps, iterations step() {its.sum()}, run() {out['pcg_iters'].sum()}", flush=True)
+    assert out["n_qs"] == 40 and its[:, 1].min() > 0
+    assert np.array_equal(out["pcg_iters"], its)
+    same(ctx.state_get(), st_step)
+
+    reset(early=True)
+    a = ctx.run(max_steps=400)
+    sa = ctx.state_get()
+    reset(early=False)
+    b = ctx.run(max_steps=400)
+    print(f"[gpu] {prec}: 400 steps, {a['n_qs']} quasi-static, {a['n_dyn']} dynamic, iterations "
+          f"{a['total_pcg_iters']} / {b['total_pcg_iters']}", flush=True)
+    assert a["n_dyn"] > 0 and a["n_qs"] > 40
+    assert np.array_equal(a["pcg_iters"], b["pcg_iters"])
+    assert np.array_equal(a["time_"], b["time_"])
+    same(sa, ctx.state_get())
+
+
 def _bf16(x):
     """Round FP32 values to BF16 (nearest even) and back, as the kernels store weights."""
     u = np.asarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
