@@ -108,7 +108,8 @@ def parse():
     ap.add_argument("--smoother", default="tiles",
                     help="fine-level smoother: tiles (one launch per Chebyshev sweep) | chebyshev | jacobi")
     ap.add_argument("--epochs", type=int, default=300)
-    ap.add_argument("--graphs", type=int, default=1, help="replay CG batches as HIP graphs (1) or launch eagerly")
+    ap.add_argument("--graphs", type=int, default=0,
+                    help="replay CG batches as HIP graphs (1) or launch them eagerly (0, faster here)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU budget per thread setting")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--plain-only", action="store_true")

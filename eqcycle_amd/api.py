@@ -301,7 +301,7 @@ class EqContext:
         return out.value
 
     # ---- time loop ------------------------------------------------------------
-    def set_solver(self, tol=1e-6, maxiter=0, pc="amg", prec="fp64", guess="previous", graphs=True,
+    def set_solver(self, tol=1e-6, maxiter=0, pc="amg", prec="fp64", guess="previous", graphs=False,
                    cheb_pairs=True, early_tail=True, galerkin_shortcut=True, basis_fp16=False, extrapolate=False,
                    galerkin_fused=True):
         """early_tail: the second quasi-static solve's fault update and the

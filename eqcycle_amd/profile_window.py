@@ -1,6 +1,6 @@
 """One pass of a solver variant over the bench's earthquake cycle, for profilers.
 
-    python -m eqcycle_amd.profile_window [--config cfg4] [--variant galerkin] [--graphs 0]
+    python -m eqcycle_amd.profile_window [--config cfg4] [--variant galerkin] [--graphs 1]
 
 Builds the bench's checkpoint (the plain solver from the seeded state to year
 110, recording the training snapshots on the way; POD basis, network and
@@ -8,9 +8,9 @@ linear predictor fitted as bench.py does), runs one untimed pass of the
 variant, then the 512 MiB calibration copy once (`convert_kernel<double,
 double>`, through eq_time_kernel) and one more pass: in a `rocprofv3
 --kernel-trace` of this program the pass is every dispatch after that copy
-(eqcycle_amd.trace_window).  Under kernel tracing pass --graphs 0: the
-tracer's interception of HIP stream capture has crashed the host process on
-these machines, and eager launches run the same kernels.
+(eqcycle_amd.trace_window).  CG batches launch eagerly unless --graphs 1;
+keep them eager under kernel tracing: the tracer's interception of HIP stream
+capture has crashed the host process on these machines.
 """
 from __future__ import annotations
 
@@ -31,7 +31,7 @@ def main():
     ap.add_argument("--degree", type=int, default=None,
                     help="fine-level Chebyshev degree of the measured pass (default as bench.py: 5 plain, 7 else)")
     ap.add_argument("--degree-plain", type=int, default=5, help="degree of the checkpoint run")
-    ap.add_argument("--graphs", type=int, default=1)
+    ap.add_argument("--graphs", type=int, default=0)
     ap.add_argument("--extrapolate", type=int, default=1, help="extrapolated Galerkin start, as bench.py")
     a = ap.parse_args()
     import bench

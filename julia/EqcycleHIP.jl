@@ -319,7 +319,7 @@ end
 """set_solver!(ctx; tol = 1e-6, ...): the cg! settings of the loop's quasi-static
 solves (src/main.jl:256: tol = 1e-6, previous-step start)."""
 function set_solver!(ctx::Context; tol = 1e-6, maxiter = 0, pc = EQ_PC_AMG, prec = EQ_PREC_FP64,
-                     guess = EQ_GUESS_PREVIOUS, record_snapshots = true, graphs = true)
+                     guess = EQ_GUESS_PREVIOUS, record_snapshots = true, graphs = false)
     # cheb_single, sync_pass2, galerkin_full, basis_fp16, extrapolate, galerkin_split at the library's defaults
     o = SolverOpts(tol, maxiter, pc, prec, guess, record_snapshots, graphs, 0, 0, 0, 0, 0, 0)
     check(ctx, ccall((:eq_set_solver, LIB), Cint, (Ptr{Cvoid}, Ref{SolverOpts}), ctx.h, o))
