@@ -1446,19 +1446,31 @@ __device__ __forceinline__ void qs_begin_fault_node(const Geom& g, const FaultDe
     fs.Vf[I] = V0;
   }
   const double F = (first_pass ? d[I] : dPre[I]) + vf * dt;
+  // every load before the first store (the stores could alias them)
+  double muq[5], bm[4], bo[4];
+#pragma unroll
+  for (int q = 0; q < 5; q++) muq[q] = mu[(int64_t)q * g.NX + I];
+#pragma unroll
+  for (int J = 1; J <= 4; J++) {
+    bm[J - 1] = b[(int64_t)J * g.NX + I];
+    bo[J - 1] = (db && theta != 0.0) ? bold[(int64_t)(J - 1) * g.NX + I] : 0.0;
+  }
   d[I] = F;
   const double cy = g.c2 * wline(g, I, g.NX);
+#pragma unroll
   for (int J = 1; J <= 4; J++) {
     double s = 0.0;
-    for (int q = 0; q < 5; q++) s += g.D[q * 5 + J] * (g.w[q] * mu[(int64_t)q * g.NX + I]) * g.D[q * 5 + 0];
+#pragma unroll
+    for (int q = 0; q < 5; q++) s += g.D[q * 5 + J] * (g.w[q] * muq[q]) * g.D[q * 5 + 0];
     const int64_t m = (int64_t)J * g.NX + I;
     const int64_t r4 = (int64_t)(J - 1) * g.NX + I;
     const double bn = -(cy * s) * F;
+    const double bv = bm[J - 1];
     // change of b since the solve the start comes from; with the extrapolation,
     // less theta times the change the step before (the start then solves that
     // extrapolated b off rows 1..4 already)
-    if (db) db[r4] = theta != 0.0 ? (bn - b[m]) - theta * (b[m] - bold[r4]) : bn - b[m];
-    if (bold) bold[r4] = b[m];
+    if (db) db[r4] = theta != 0.0 ? (bn - bv) - theta * (bv - bo[J - 1]) : bn - bv;
+    if (bold) bold[r4] = bv;
     b[m] = bn;
   }
 }

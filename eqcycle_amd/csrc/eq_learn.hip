@@ -723,11 +723,26 @@ __global__ __launch_bounds__(LB) void gal_rows_op_kernel(int64_t nrows, int NB, 
   T[(int64_t)k * nrows + i] = a;
 }
 
-// c[k] = sum_i T[k, i] v[i], one block per coefficient, fixed order
+// c[k] = sum_i T[k, i] v[i], one block per coefficient, fixed order: each
+// thread sums i = tid, tid + LB, ... in turn, its loads of a 16 LB chunk
+// issued first (a few thousand rows are one chunk)
 __global__ __launch_bounds__(LB) void gal_coef_kernel(int64_t nrows, const double* T, const double* v, double* c) {
   const int k = blockIdx.x;
+  const double* Tk = T + (int64_t)k * nrows;
+  constexpr int U = 16;
   double a = 0.0;
-  for (int64_t i = threadIdx.x; i < nrows; i += LB) a += T[(int64_t)k * nrows + i] * v[i];
+  for (int64_t i0 = threadIdx.x; i0 < nrows; i0 += (int64_t)U * LB) {
+    double tv[U], vv[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int64_t i = i0 + (int64_t)u * LB;
+      tv[u] = i < nrows ? Tk[i] : 0.0;
+      vv[u] = i < nrows ? v[i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (i0 + (int64_t)u * LB < nrows) a += tv[u] * vv[u];
+  }
   a = lblock_sum(a);
   if (threadIdx.x == 0) c[k] = a;
 }
