@@ -1355,7 +1355,23 @@ __global__ __launch_bounds__(BLOCK) void csr_kernel(int64_t n, const int32_t* __
   double acc = 0.0;
   if (row < n) {
     const int q1 = rp[row + 1];
-    for (int q = rp[row] + lane; q < q1; q += LPR) {
+    int q = rp[row] + lane;
+    // four entries at a time: their column and value loads, then the
+    // gathers, then the sums in entry order (the gathers wait on the columns)
+    for (; q + 3 * LPR < q1; q += 4 * LPR) {
+      int c[4];
+      T v[4], xv[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        c[u] = col[q + u * LPR];
+        v[u] = val[q + u * LPR];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) xv[u] = (MODE == CSR_DOWN) ? omega * dinv[c[u]] * b[c[u]] : x[c[u]];
+#pragma unroll
+      for (int u = 0; u < 4; u++) acc += (double)v[u] * (double)xv[u];
+    }
+    for (; q < q1; q += LPR) {
       const int c = col[q];
       const T xv = (MODE == CSR_DOWN) ? omega * dinv[c] * b[c] : x[c];
       acc += (double)val[q] * (double)xv;
