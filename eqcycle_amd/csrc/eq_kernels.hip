@@ -468,9 +468,10 @@ struct OpResidual {
 // [ceil(nb / 8)][nI][8].  x_out must not alias d: neighbouring tiles still
 // load d on their windows.
 constexpr int kGalGroups = 8;   // groups of 8 basis vectors: up to 64
-// kFull: nb = 64 (no basis index to test: the coefficients load in blocks and
-// the sums follow the loads as they arrive)
-template <bool kFull>
+// kFull: nb = 8 kNG (no basis index to test: the coefficients load in blocks
+// and the sums follow the loads as they arrive; kNG = 4 holds half the
+// registers of 8)
+template <bool kFull, int kNG = kGalGroups>
 struct OpGalResidual {
   using Pre = PreNone;
   static constexpr int kParts = 2;
@@ -519,7 +520,7 @@ struct OpGalResidual {
     }
     // every group's loads first (up to 8 x NL 16-byte loads in flight), then
     // the sums in basis order; no branch per node, so the NL sums interleave
-    constexpr int NG = kGalGroups;
+    constexpr int NG = kNG;
     const int ng = kFull ? NG : (nb + 7) / 8;
     half8 v[NG][NL];
 #pragma unroll
@@ -1987,6 +1988,9 @@ void launch_gal_residual(const Geom& g, const double* d, const half8* phi16, int
   if (nb == 8 * kGalGroups) {
     OpGalResidual<true> op{npart, d, phi16, nI, nb, c, mu, b, r, r32, x_out, part_rr, part_bb, gate};
     hipLaunchKernelGGL((tile_kernel<double, OpGalResidual<true>>), tile_grid(g), dim3(TBLOCK), 0, s, g, op);
+  } else if (nb == 32) {
+    OpGalResidual<true, 4> op{npart, d, phi16, nI, nb, c, mu, b, r, r32, x_out, part_rr, part_bb, gate};
+    hipLaunchKernelGGL((tile_kernel<double, OpGalResidual<true, 4>>), tile_grid(g), dim3(TBLOCK), 0, s, g, op);
   } else {
     OpGalResidual<false> op{npart, d, phi16, nI, nb, c, mu, b, r, r32, x_out, part_rr, part_bb, gate};
     hipLaunchKernelGGL((tile_kernel<double, OpGalResidual<false>>), tile_grid(g), dim3(TBLOCK), 0, s, g, op);

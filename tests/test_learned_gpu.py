@@ -388,3 +388,37 @@ def test_linear_predictor_against_network_and_previous_guess(trained):
     dref = np.max(np.abs(dfin["previous"]))
     assert np.max(np.abs(dfin["linear"] - dfin["previous"])) <= 1e-7 * dref
 
+
+
+@pytest.mark.parametrize("nb", [32, 64])
+def test_full_group_galerkin_start_is_bit_identical(nb):
+    """The fused start's variants without a basis-index test (32 and 64
+    vectors: four and eight 8-vector groups) against the split expansion and
+    residual: 20 steps from the same state give the same iteration counts and
+    the same state bit for bit."""
+    from eqcycle_amd import learned
+    from eqcycle_amd.api import EqContext
+    P = set_parameters(small(32, 16, material="trapezoid", total_years=50))
+    ctx = EqContext(P, amg=True)
+    ctx.set_solver(tol=1e-10, pc="amg")
+    ctx.state_init()
+    learned.record(ctx, max_steps=120, capacity=160, nbasis=nb)
+    learned.fit_basis(ctx, tol=1e-10)
+    s0 = ctx.state_get()
+    assert s0["isolver"] == 1
+
+    def run(fused):
+        ctx.state_set(d=s0["d"], v=s0["v"], a=s0["a"], psi=s0["psi"], tau=s0["tau"], t=s0["t"], dt=s0["dt"],
+                      isolver=s0["isolver"])
+        ctx.set_solver(tol=1e-8, pc="amg", prec="f32pc", guess="galerkin", basis_fp16=True, extrapolate=True,
+                       galerkin_fused=fused)
+        its = np.array([ctx.step().pcg_iters for _ in range(20)])
+        return its, ctx.state_get()
+
+    it_s, st_s = run(False)
+    it_f, st_f = run(True)
+    print(f"[gpu] nb {nb}: iterations split {it_s.sum()}, fused {it_f.sum()}", flush=True)
+    assert it_s[:, 0].min() > 0 and it_s[:, 1].min() > 0
+    assert np.array_equal(it_s, it_f)
+    for k in ("d", "v", "psi", "tau"):
+        assert np.array_equal(st_s[k], st_f[k]), k
