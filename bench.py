@@ -113,7 +113,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU budget per thread setting")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--plain-only", action="store_true")
-    ap.add_argument("--nbasis", type=int, default=32, help="POD basis vectors of the learned starts")
+    ap.add_argument("--nbasis", type=int, default=16, help="POD basis vectors of the learned starts")
     ap.add_argument("--max-coarse", type=int, default=None,
                     help="largest coarsest level of the hierarchy (default: the library's, 2000 for pmg)")
     ap.add_argument("--galerkin-fused", type=int, default=1,
@@ -126,7 +126,7 @@ def parse():
 # ---------------------------------------------------------------------------
 # GPU side
 # ---------------------------------------------------------------------------
-def checkpoint(ctx, cfg, tol, learned, epochs, graphs, nbasis=32):
+def checkpoint(ctx, cfg, tol, learned, epochs, graphs, nbasis=16):
     """The plain solver from the seeded state to CHECKPOINT_YEARS; with the
     learned start, its first cfg.n_snapshots pass-1 solves are the training
     snapshots (POD basis and network fitted afterwards)."""

@@ -469,8 +469,8 @@ struct OpResidual {
 // load d on their windows.
 constexpr int kGalGroups = 8;   // groups of 8 basis vectors: up to 64
 // kFull: nb = 8 kNG (no basis index to test: the coefficients load in blocks
-// and the sums follow the loads as they arrive; kNG = 4 holds half the
-// registers of 8)
+// and the sums follow the loads as they arrive; fewer groups hold fewer
+// registers)
 template <bool kFull, int kNG = kGalGroups>
 struct OpGalResidual {
   using Pre = PreNone;
@@ -1985,13 +1985,20 @@ void launch_cg_residual(const Geom& g, const T* x, const T* mu, const T* b, T* r
 void launch_gal_residual(const Geom& g, const double* d, const half8* phi16, int64_t nI, int nb, const double* c,
                          const double* mu, const double* b, double* r, float* r32, double* x_out, double* part_rr,
                          double* part_bb, int npart, hipStream_t s, const int* gate) {
-  if (nb == 8 * kGalGroups) {
-    OpGalResidual<true> op{npart, d, phi16, nI, nb, c, mu, b, r, r32, x_out, part_rr, part_bb, gate};
-    hipLaunchKernelGGL((tile_kernel<double, OpGalResidual<true>>), tile_grid(g), dim3(TBLOCK), 0, s, g, op);
-  } else if (nb == 32) {
-    OpGalResidual<true, 4> op{npart, d, phi16, nI, nb, c, mu, b, r, r32, x_out, part_rr, part_bb, gate};
-    hipLaunchKernelGGL((tile_kernel<double, OpGalResidual<true, 4>>), tile_grid(g), dim3(TBLOCK), 0, s, g, op);
-  } else {
+#define EQ_GAL_FULL(NG)                                                                                    \
+  {                                                                                                        \
+    OpGalResidual<true, NG> op{npart, d, phi16, nI, nb, c, mu, b, r, r32, x_out, part_rr, part_bb, gate};  \
+    hipLaunchKernelGGL((tile_kernel<double, OpGalResidual<true, NG>>), tile_grid(g), dim3(TBLOCK), 0, s, g, op); \
+  }
+  if (nb == 64) EQ_GAL_FULL(8)
+  else if (nb == 48) EQ_GAL_FULL(6)
+  else if (nb == 40) EQ_GAL_FULL(5)
+  else if (nb == 32) EQ_GAL_FULL(4)
+  else if (nb == 24) EQ_GAL_FULL(3)
+  else if (nb == 16) EQ_GAL_FULL(2)
+  else if (nb == 8) EQ_GAL_FULL(1)
+#undef EQ_GAL_FULL
+  else {
     OpGalResidual<false> op{npart, d, phi16, nI, nb, c, mu, b, r, r32, x_out, part_rr, part_bb, gate};
     hipLaunchKernelGGL((tile_kernel<double, OpGalResidual<false>>), tile_grid(g), dim3(TBLOCK), 0, s, g, op);
   }

@@ -390,10 +390,10 @@ def test_linear_predictor_against_network_and_previous_guess(trained):
 
 
 
-@pytest.mark.parametrize("nb", [32, 64])
+@pytest.mark.parametrize("nb", [16, 32, 64])
 def test_full_group_galerkin_start_is_bit_identical(nb):
-    """The fused start's variants without a basis-index test (32 and 64
-    vectors: four and eight 8-vector groups) against the split expansion and
+    """The fused start's variants without a basis-index test (16, 32 and 64
+    vectors: two, four and eight 8-vector groups) against the split expansion and
     residual: 20 steps from the same state give the same iteration counts and
     the same state bit for bit."""
     from eqcycle_amd import learned
